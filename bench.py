@@ -1,0 +1,156 @@
+"""Benchmark: exact k-NN queries/s on one MI355X.
+
+Workload (one "step"): exact brute-force k-NN of 16,384 queries against
+1,048,576 reference points, dim 128, k = 32, float32 uniform [0,1), dim x n
+row-major, all inputs resident in HBM (generated on the device by the
+library's seeded generator).  The step is the full path behind the drop-in
+API: squared norms, fused MFMA scan + top-k, exact FP32 re-rank with
+certificate, fallback launch, outputs in k x query_nb layout.
+
+Prints one JSON line.  `roofline` is measured live for the dominant kernel
+(the scan) from HIP events the library records on the stream it launches on;
+`traffic` comes from the committed rocprofv3 counter summary of that kernel
+when present.  `cpu_baseline` times the oracle's single-threaded restatement
+of the reference ground truth on a bounded query sample of the same data.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "knn-cuda_amd"))
+
+METRIC = "exact-kNN queries/s on 1 MI355X; BF16+re-rank+router speedup vs plain FP32"
+FP32_PEAK_TFLOPS = 157.3
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "scan_f32_1Mx16k_d128_k32_traffic.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ref-nb", type=int, default=1 << 20)
+    ap.add_argument("--query-nb", type=int, default=16384)
+    ap.add_argument("--dim", type=int, default=128)
+    ap.add_argument("--k", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the cpu_baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the reference-set "
+              "shard across GPUs is described in DESIGN.md and is not run here.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import knnhip
+
+    R, Q, D, k = args.ref_nb, args.query_nb, args.dim, args.k
+    dev = torch.device("cuda:0")
+    seed_r, seed_q = 0x6B4E4E02, 0x6B4E4E12
+    ref = knnhip.generate_uniform(R, D, seed=seed_r, device=dev)
+    qry = knnhip.generate_uniform(Q, D, seed=seed_q, device=dev)
+    dist = torch.empty((k, Q), dtype=torch.float32, device=dev)
+    idx = torch.empty((k, Q), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        knnhip.search_device(ref, qry, k, dist, idx, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+
+    scan_ms = []
+    stage_ms = [[] for _ in range(4)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        knnhip.search_device(ref, qry, k, dist, idx, timing=True, stream=stream.cuda_stream)
+        for s in range(4):
+            stage_ms[s].append(knnhip.last_stage_ms(s))
+        scan_ms.append(knnhip.last_stage_ms(1))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    fallbacks = knnhip.last_fallback_count()
+
+    ms_per_step = 1000.0 * elapsed / args.steps
+    qps = Q / (elapsed / args.steps)
+    flops = 2.0 * R * Q * D
+    scan_avg = sum(scan_ms) / len(scan_ms)
+    achieved = flops / (scan_avg * 1e-3) / 1e12
+
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("workload") == [R, Q, D, k]:
+            traffic = t.get("hbm_bytes_per_launch")
+
+    # Correctness gate + CPU baseline on the same data.
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    ref_h = ref.cpu().numpy()
+    qry_h = qry.cpu().numpy()
+    d_h = dist.cpu().numpy()
+    i_h = idx.cpu().numpy()
+    cpu = None
+    gate_q = 0
+    if not args.no_cpu_baseline:
+        t1 = time.perf_counter()
+        gd, gi = oracle.knn_c_queries(ref_h, qry_h, k, 0, 1)
+        one = time.perf_counter() - t1
+        n = int(max(2, min(64, args.cpu_seconds / max(one, 1e-6))))
+        t1 = time.perf_counter()
+        gd, gi = oracle.knn_c_queries(ref_h, qry_h, k, 0, n)
+        cpu_s = time.perf_counter() - t1
+        gate_q = n
+        ok = np.array_equal(gi[:, :n], i_h[:, :n]) and np.array_equal(
+            gd[:, :n].view(np.uint32), d_h[:, :n].view(np.uint32))
+        if not ok:
+            print(json.dumps({"error": "parity gate failed", "queries_checked": n}), flush=True)
+            sys.exit(1)
+        cpu = {"value": n / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
+               "sample": f"first {n} of {Q} queries against all {R} refs (dim {D}, k {k}), "
+                         f"single-threaded oracle restatement of knn_c, {cpu_s:.1f} s"}
+
+    out = {
+        "metric": METRIC,
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform [0,1), generated on device)",
+        "config": {"workload": "plain FP32 exact kNN at scale", "ref_nb": R, "query_nb": Q,
+                   "dim": D, "k": k, "layout": "dim x n row-major", "parallelism": "1 GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "scan_f32_kernel", "kernel_ms": scan_avg,
+                     "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+        "stages_ms": {name: sum(v) / len(v) for name, v in
+                      zip(("norms", "scan", "select_rerank", "fallback"), stage_ms)},
+        "fallback_queries": fallbacks,
+        "parity_gate_queries": gate_q,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,104 @@
+/*
+ * knncuda.h — drop-in C/C++ interface of the MI355X exact k-NN library
+ * (libknnhip.so, built from knn-cuda_amd/csrc).
+ *
+ * Point sets are dim x n matrices stored row-major: coordinate d of point i is
+ * at ptr[d*n + i].  Results are k x query_nb row-major: neighbour rank j of
+ * query i is at knn_dist[j*query_nb + i] / knn_index[j*query_nb + i], ranks in
+ * ascending distance, indexes 0-based, equal distances ordered by the lower
+ * reference index.  Every entry point returns the ground-truth result of the
+ * reference (code/test.cpp:129-175): distances are sqrtf of the FP32 sum of
+ * squared differences accumulated in ascending d, bit for bit.
+ */
+#ifndef KNNHIP_KNNCUDA_H
+#define KNNHIP_KNNCUDA_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+/*
+ * The three reference entry points, with C++ linkage so that code compiled
+ * against the reference header links against this library unchanged.
+ * Host pointers in and out; the call returns once the results are in host
+ * memory.  On failure: a line starting "ERROR:" on stdout and false.
+ *
+ * Replaces code/knncuda.h:14-21 (knn_cuda_global).
+ */
+bool knn_cuda_global(const float *ref, int ref_nb, const float *query, int query_nb, int dim,
+                     int k, float *knn_dist, int *knn_index);
+/* Replaces code/knncuda.h:37-44 (knn_cuda_texture).  No texture objects and no
+ * width limit: reference tiles are staged through LDS. */
+bool knn_cuda_texture(const float *ref, int ref_nb, const float *query, int query_nb, int dim,
+                      int k, float *knn_dist, int *knn_index);
+/* Replaces code/knncuda.h:61-68 (knn_cublas).  The ||r||^2 - 2 q.r contraction
+ * runs on the library's own MFMA kernel (no vendor BLAS); distances are never
+ * NaN. */
+bool knn_cublas(const float *ref, int ref_nb, const float *query, int query_nb, int dim, int k,
+                float *knn_dist, int *knn_index);
+extern "C" {
+#endif
+
+/* C-linkage aliases of the three entry points above, for FFI callers
+ * (ctypes, cgo, JNI).  Return 1 on success, 0 on failure. */
+int knnhip_global(const float *ref, int ref_nb, const float *query, int query_nb, int dim, int k,
+                  float *knn_dist, int *knn_index);
+int knnhip_texture(const float *ref, int ref_nb, const float *query, int query_nb, int dim, int k,
+                   float *knn_dist, int *knn_index);
+int knnhip_cublas(const float *ref, int ref_nb, const float *query, int query_nb, int dim, int k,
+                  float *knn_dist, int *knn_index);
+
+/* Options of the extended entry point.  Zero-initialise, then set fields;
+ * zero means "library default" for every field. */
+typedef struct knnhip_options {
+    int variant;        /* 0 global (default), 1 texture (LDS-staged), 2 cublas form */
+    int margin;         /* extra candidates kept per query beyond k (default 16) */
+    int slices;         /* reference slices scanned in parallel (default: chosen) */
+    int force_fallback; /* 1: every query takes the exact full-scan path (testing) */
+    int timing;         /* 1: record per-stage device times (knnhip_last_stage_ms) */
+    int reserved[8];
+} knnhip_options;
+
+/*
+ * Extended entry point: same problem as knn_cuda_global, with every pointer a
+ * device pointer (ref is dim rows of ref_ld floats, query dim rows of query_ld
+ * floats; outputs k x query_nb, leading dimension query_nb).  Work is queued
+ * on `stream` (a hipStream_t, NULL for the null stream) and the call returns
+ * without synchronising, except when the exact fallback is needed.  Returns 1
+ * on success, 0 on failure (knnhip_last_error).  No counterpart in the
+ * reference: the reference only takes host pointers (code/knncuda.h:14-21).
+ */
+int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const float *query,
+                         int query_nb, long long query_ld, int dim, int k, float *knn_dist,
+                         int *knn_index, const knnhip_options *opt, void *stream);
+
+/* Number of queries of the last call that failed the candidate certificate
+ * and were answered by the exact full scan.  Synchronises the last stream. */
+int knnhip_last_fallback_count(void);
+
+/* Device time in milliseconds of stage `stage` in the last call made with
+ * options.timing = 1 (0 norms, 1 scan, 2 select/re-rank, 3 fallback); -1 if
+ * not recorded. */
+float knnhip_last_stage_ms(int stage);
+
+/* Text of the last error reported by this library (empty if none). */
+const char *knnhip_last_error(void);
+
+/* Seeded synthetic point sets written directly to device memory, dim x n
+ * row-major.  Uniform: scale * U[0,1) from a counter-based hash (bit-identical
+ * to the CPU generator in the oracle).  Mixture: ncomp isotropic Gaussians of
+ * standard deviation sigma, centres uniform in [-1,1]^dim drawn from
+ * centre_seed. */
+int knnhip_generate_uniform(float *out, int n, int dim, unsigned long long seed, float scale,
+                            void *stream);
+int knnhip_generate_mixture(float *out, int n, int dim, int ncomp, float sigma,
+                            unsigned long long centre_seed, unsigned long long point_seed,
+                            void *stream);
+
+/* Library version string. */
+const char *knnhip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KNNHIP_KNNCUDA_H */

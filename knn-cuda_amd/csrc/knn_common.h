@@ -1,0 +1,143 @@
+// Shared device helpers and launch descriptors of the exact k-NN engine.
+//
+// Layout conventions (all device buffers):
+//   point sets   dim-major: coordinate d of point i at p[d*ld + i]
+//   results      k x Q: rank j of query i at out[j*Q + i]
+//   partials     [slice][query][kp] candidate heaps written by the scan
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace knnhip {
+
+constexpr int kWave = 64;            // CDNA wavefront width
+constexpr int kQueriesPerWave = 32;  // one MFMA 32-column block of queries per wave
+constexpr int kRefsPerTile = 32;     // one MFMA 32-row block of references
+constexpr int kMTiles = 4;           // reference tiles per scan step (128 refs)
+constexpr int kRefsPerStep = kRefsPerTile * kMTiles;
+constexpr int kMaxUnion = 2048;      // max slices*kp entries merged per query
+constexpr int kMaxCand = 1024;       // max candidates re-ranked exactly per query
+
+// Arguments of the FP32 MFMA scan (one launch covers all queries and slices).
+struct ScanArgs {
+    const float *ref;    // dpad rows of ldr floats (rows >= dim are zero)
+    long long ldr;
+    int R;
+    const float *query;  // dpad rows of ldq floats (rows >= dim are zero)
+    long long ldq;
+    int Q;
+    const float *rnorm;  // R squared norms (FP32)
+    int slice_len;       // references per slice, multiple of kRefsPerStep
+    int nslices;
+    int kp;              // heap capacity per (query, slice)
+    int nqb;             // query blocks (waves_per_block * 32 queries each)
+    int nchunks;         // dpad / (2*KS) dimension chunks
+    int xcd_map;         // 1: keep each slice on one XCD (nslices % 8 == 0)
+    float *part_val;     // [nslices][Q][kp]
+    int *part_idx;
+};
+
+// Arguments of the select / exact re-rank / certificate stage.
+struct FinalArgs {
+    const float *ref;
+    long long ldr;
+    int R;
+    const float *query;
+    long long ldq;
+    int Q;
+    int dim;
+    int k;
+    int kp;
+    int nslices;
+    int slice_len;            // references per slice (<= kp: no heap ever rejects)
+    int maxc;                 // candidate capacity per query (<= kMaxCand)
+    const float *part_val;
+    const int *part_idx;
+    const float *qnorm;       // Q squared norms (FP32)
+    const unsigned *rmax;     // bit pattern of max_r ||r||^2 (non-negative float)
+    int force_fallback;
+    float *out_dist;          // k x Q
+    int *out_idx;
+    int *fb_count;            // queries that failed the certificate
+    int *fb_list;
+};
+
+// Arguments of the exact per-query full scan (fallback).
+struct ExactArgs {
+    const float *ref;
+    long long ldr;
+    int R;
+    const float *query;
+    long long ldq;
+    int Q;
+    int dim;
+    int k;
+    const int *list;     // query ids to answer (nullptr: all queries 0..Q-1)
+    const int *count;    // number of ids in list (nullptr: Q)
+    float *scratch_key;  // [gridDim.x][64][k]
+    int *scratch_idx;
+    float *out_dist;
+    int *out_idx;
+};
+
+// Order-preserving map of a float onto uint32 (total order, -0 < +0).
+__device__ __forceinline__ unsigned float_to_ordered(float x) {
+    unsigned b = __float_as_uint(x);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ordered_to_float(unsigned u) {
+    unsigned b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    return __uint_as_float(b);
+}
+
+// Compiler-level ordering point for LDS traffic shared between lanes of one
+// wave (no hardware barrier: waves of a block never wait for each other).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Squared distance exactly as the reference ground truth forms it
+// (code/test.cpp:61-66): ascending d, FP32 difference, FP32 square, FP32 sum,
+// no contraction into FMA.
+__device__ __forceinline__ float canonical_sq(const float *__restrict__ ref, long long ldr,
+                                              int r, const float *__restrict__ query,
+                                              long long ldq, int q, int dim) {
+#pragma clang fp contract(off)
+    float acc = 0.0f;
+    const float *rp = ref + r;
+    const float *qp = query + q;
+    int d = 0;
+    for (; d + 8 <= dim; d += 8) {
+        float rv[8], qv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            rv[j] = rp[(long long)(d + j) * ldr];
+            qv[j] = qp[(long long)(d + j) * ldq];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            float t = rv[j] - qv[j];
+            float s = t * t;
+            acc = acc + s;
+        }
+    }
+    for (; d < dim; ++d) {
+        float t = rp[(long long)d * ldr] - qp[(long long)d * ldq];
+        float s = t * t;
+        acc = acc + s;
+    }
+    return acc;
+}
+
+// Correctly rounded FP32 square root: the double square root of a float
+// argument rounded once to float equals the correctly rounded float result.
+__device__ __forceinline__ float sqrt_rn(float x) { return (float)__builtin_sqrt((double)x); }
+
+// Lexicographic (key, index) order of the reference top-k.
+__device__ __forceinline__ bool lex_less(float ka, int ia, float kb, int ib) {
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+}  // namespace knnhip

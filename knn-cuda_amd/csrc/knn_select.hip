@@ -1,0 +1,340 @@
+// Stages around the scan: squared norms, candidate selection with exact FP32
+// re-rank and certificate, and the exact per-query full scan used when a
+// certificate fails.
+//
+// Exactness argument (the result equals the reference ground truth
+// code/test.cpp:129-175 bit for bit):
+//   * the scan leaves, per query, a candidate set C and a threshold tau such
+//     that every reference outside C has approximate key >= tau;
+//   * the approximate key a(r) = ||r||^2 - 2 r.q (FP32 MFMA) differs from the
+//     true ||r-q||^2 - ||q||^2 by at most g*(||q||^2 + 3.1*max||r||^2), with
+//     g = 2*(dim+4)*2^-24 (twice the standard gamma bound of a dim+2 term
+//     FP32 dot product and of the FP32 norm);
+//   * the ground-truth FP32 sum differs from ||r-q||^2 by at most the
+//     relative factor g;
+//   * so every reference outside C has ground-truth sum >= L, computed in
+//     double below.  If L > (k-th key * (1 + 2^-22))^2, every outside
+//     reference has sqrtf key strictly above the k-th candidate key, and the
+//     k best candidates under the reference's own (sqrtf key, index) order
+//     are the reference's answer.  Otherwise the query is answered by the
+//     exact full scan.
+#include <limits.h>
+
+#include "knn_common.h"
+
+namespace knnhip {
+
+// ---------------------------------------------------------------------------
+// Squared norms of the columns of a dim-major point set (replaces
+// code/knncuda.cu:217-227).  Optionally the maximum over all points, as the
+// bit pattern of a non-negative float (integer order = float order).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void sq_norms_kernel(const float *__restrict__ p, long long ld,
+                                                        int n, int dim, float *__restrict__ out,
+                                                        unsigned *maxbits) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float s = 0.0f;
+    if (i < n) {
+        const float *col = p + i;
+        for (int d = 0; d < dim; ++d) {
+            const float v = col[(long long)d * ld];
+            s = fmaf(v, v, s);
+        }
+        out[i] = s;
+    }
+    if (maxbits) {
+        unsigned m = __float_as_uint(s);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+        if ((threadIdx.x & 63) == 0) atomicMax(maxbits, m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Select the kp smallest approximate keys of the union of the slice heaps,
+// re-rank them with the ground-truth FP32 distance, write the k best and
+// check the certificate.  One wave per query.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <int NPL>
+__global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int waves = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * waves + wave;
+    if (q >= a.Q) return;
+
+    const int maxc = a.maxc;
+    int *cidx = reinterpret_cast<int *>(smem) + (size_t)wave * maxc;
+    float *ckey = reinterpret_cast<float *>(smem + (size_t)waves * maxc * sizeof(int)) +
+                  (size_t)wave * maxc;
+    float *kth_slot = reinterpret_cast<float *>(smem + (size_t)waves * maxc * 2 * sizeof(int)) + wave;
+
+    // 1. Gather the union of the slice heaps (key order-mapped; placeholders
+    //    of unfilled heaps map to the largest code and are never counted).
+    const int kp = a.kp;
+    const int U = a.nslices * kp;
+    unsigned code[NPL];
+    int ids[NPL];
+    int nreal = 0;
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) {
+        const int e = j * kWave + lane;
+        code[j] = 0xFFFFFFFFu;
+        ids[j] = -1;
+        if (e < U) {
+            const int s = e / kp, t = e - s * kp;
+            const long long off = ((long long)s * a.Q + q) * kp + t;
+            const int id = a.part_idx[off];
+            if (id >= 0) {
+                ids[j] = id;
+                code[j] = float_to_ordered(a.part_val[off]);
+                ++nreal;
+            }
+        }
+    }
+    const int N = wave_sum(nreal);
+
+    // 2. tau = kp-th smallest key (bisection on the order-mapped code).  With
+    //    fewer than kp real entries, or slices no longer than kp, no heap ever
+    //    rejected a reference and the union is the whole reference set.
+    unsigned cut = 0xFFFFFFFEu;
+    const bool all_in = N < kp || a.slice_len <= kp;
+    if (!all_in) {
+        unsigned lo = 0, hi = 0xFFFFFFFEu;
+        while (lo < hi) {
+            const unsigned mid = lo + ((hi - lo) >> 1);
+            int c = 0;
+#pragma unroll
+            for (int j = 0; j < NPL; ++j) c += code[j] <= mid;
+            if (wave_sum(c) >= kp)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        cut = lo;
+    }
+
+    // 3. Compact the candidates (code <= cut) in a fixed order.
+    int nc = 0;
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) {
+        const bool pred = code[j] <= cut;
+        const unsigned long long m = __ballot(pred);
+        const int pos = nc + __popcll(m & ((1ull << lane) - 1ull));
+        if (pred && pos < maxc) cidx[pos] = ids[j];
+        nc += __popcll(m);
+    }
+    const bool overflow = nc > maxc;
+    wave_sync();
+
+    // 4. Ground-truth FP32 distance of every candidate, then rank by the
+    //    reference order (sqrtf key, index) and write the k best.
+    if (!overflow) {
+        for (int c = lane; c < nc; c += kWave) {
+            const int r = cidx[c];
+            ckey[c] = sqrt_rn(canonical_sq(a.ref, a.ldr, r, a.query, a.ldq, q, a.dim));
+        }
+        wave_sync();
+        for (int c = lane; c < nc; c += kWave) {
+            const float kc = ckey[c];
+            const int ic = cidx[c];
+            int rank = 0;
+            for (int j = 0; j < nc; ++j) rank += lex_less(ckey[j], cidx[j], kc, ic);
+            if (rank < a.k) {
+                a.out_dist[(long long)rank * a.Q + q] = kc;
+                a.out_idx[(long long)rank * a.Q + q] = ic;
+            }
+            if (rank == a.k - 1) *kth_slot = kc;
+        }
+        wave_sync();
+    }
+
+    // 5. Certificate.
+    if (lane == 0) {
+        bool ok = !overflow && !a.force_fallback;
+        if (ok && !all_in) {
+            const double tau = (double)ordered_to_float(cut);
+            const double g = 2.0 * (double)(a.dim + 4) * 5.9604644775390625e-8;
+            const double nq = (double)a.qnorm[q];
+            const double nr = (double)__uint_as_float(*a.rmax);
+            const double lower =
+                (tau + nq * (1.0 - g) - g * (nq + 3.1 * nr) - (double)a.dim * 1e-36) * (1.0 - g);
+            const double kth = (double)(*kth_slot) * (1.0 + 2.384185791015625e-7);
+            ok = lower > kth * kth;
+        }
+        if (!ok) {
+            const int slot = atomicAdd(a.fb_count, 1);
+            a.fb_list[slot] = q;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact per-query scan: the reference ground truth computed on the device.
+// One wave per query; lane l takes references l, l+64, ... in increasing
+// order and keeps its own top-k with the reference's insertion semantics
+// (code/test.cpp:85-114); the 64 sorted lists are then merged in
+// (key, index) order.  Used for queries whose certificate failed, and for
+// every query when k is too large for the scan's heaps.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void exact_scan_kernel(ExactArgs a) {
+    const int lane = threadIdx.x;
+    const int n = a.count ? *a.count : a.Q;
+    const int k = a.k;
+    float *mykey = a.scratch_key + ((long long)blockIdx.x * kWave + lane) * k;
+    int *myidx = a.scratch_idx + ((long long)blockIdx.x * kWave + lane) * k;
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const int q = a.list ? a.list[t] : t;
+        int filled = 0;
+        for (int r = lane; r < a.R; r += kWave) {
+            const float key = sqrt_rn(canonical_sq(a.ref, a.ldr, r, a.query, a.ldq, q, a.dim));
+            if (filled == k && !(key < mykey[k - 1])) continue;
+            int pos = filled < k ? filled : k - 1;
+            while (pos > 0 && mykey[pos - 1] > key) {
+                mykey[pos] = mykey[pos - 1];
+                myidx[pos] = myidx[pos - 1];
+                --pos;
+            }
+            mykey[pos] = key;
+            myidx[pos] = r;
+            if (filled < k) ++filled;
+        }
+        int head = 0;
+        for (int j = 0; j < k; ++j) {
+            const float hk = head < filled ? mykey[head] : INFINITY;
+            const int hid = head < filled ? myidx[head] : INT_MAX;
+            float bk = hk;
+            int bi = hid;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const float ok = __shfl_xor(bk, off);
+                const int oi = __shfl_xor(bi, off);
+                if (lex_less(ok, oi, bk, bi)) {
+                    bk = ok;
+                    bi = oi;
+                }
+            }
+            if (head < filled && hid == bi) ++head;
+            if (lane == 0) {
+                a.out_dist[(long long)j * a.Q + q] = bk;
+                a.out_idx[(long long)j * a.Q + q] = bi;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_norms(const float *p, long long ld, int n, int dim, float *out,
+                        unsigned *maxbits, hipStream_t s) {
+    hipLaunchKernelGGL(sq_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, ld, n, dim,
+                       out, maxbits);
+    return hipGetLastError();
+}
+
+size_t select_lds_bytes(int maxc, int waves) {
+    return (size_t)waves * (maxc * 2 * sizeof(int) + sizeof(float)) + 16;
+}
+
+template <int NPL>
+static hipError_t launch_select_t(const FinalArgs &a, hipStream_t s) {
+    const int waves = 4;
+    const size_t lds = select_lds_bytes(a.maxc, waves);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)select_rerank_kernel<NPL>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(select_rerank_kernel<NPL>, dim3((a.Q + waves - 1) / waves),
+                       dim3(waves * kWave), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(const FinalArgs &a, hipStream_t s) {
+    const int U = a.nslices * a.kp;
+    if (U <= 4 * kWave) return launch_select_t<4>(a, s);
+    if (U <= 8 * kWave) return launch_select_t<8>(a, s);
+    if (U <= 16 * kWave) return launch_select_t<16>(a, s);
+    if (U <= 32 * kWave) return launch_select_t<32>(a, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_exact(const ExactArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(exact_scan_kernel, dim3(grid), dim3(kWave), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Seeded generators (same hash as the oracle's CPU generator).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t rng_u64(uint64_t seed, uint64_t stream, uint64_t counter) {
+    return mix64(seed * 0x9E3779B97F4A7C15ull + stream * 0xD1B54A32D192ED03ull +
+                 counter * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+}
+__device__ __forceinline__ float unit24(uint64_t u) {
+    return (float)(u >> 40) * (1.0f / 16777216.0f);
+}
+
+__global__ void gen_uniform_kernel(float *out, int n, int dim, uint64_t seed, float scale) {
+#pragma clang fp contract(off)
+    const long long total = (long long)n * dim;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long d = e / n, i = e - d * n;
+        out[e] = scale * unit24(rng_u64(seed, 0, (uint64_t)i * dim + d));
+    }
+}
+
+__global__ void gen_mixture_kernel(float *out, int n, int dim, int ncomp, float sigma,
+                                   uint64_t centre_seed, uint64_t point_seed) {
+#pragma clang fp contract(off)
+    const long long total = (long long)n * dim;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long d = e / n, i = e - d * n;
+        const uint64_t c = ((rng_u64(point_seed, 2, (uint64_t)i) >> 32) * (uint64_t)ncomp) >> 32;
+        const float centre =
+            2.0f * unit24(rng_u64(centre_seed, 1, c * (uint64_t)dim + d)) - 1.0f;
+        const uint64_t cnt = (uint64_t)i * dim + d;
+        const double u1 = ((double)(rng_u64(point_seed, 3, cnt) >> 11) + 0.5) *
+                          (1.0 / 9007199254740992.0);
+        const double u2 =
+            (double)(rng_u64(point_seed, 4, cnt) >> 11) * (1.0 / 9007199254740992.0);
+        const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        out[e] = centre + sigma * (float)z;
+    }
+}
+
+static int gen_grid(long long total) {
+    long long g = (total + 255) / 256;
+    return (int)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
+}
+
+hipError_t launch_gen_uniform(float *out, int n, int dim, uint64_t seed, float scale,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(gen_uniform_kernel, dim3(gen_grid((long long)n * dim)), dim3(256), 0, s,
+                       out, n, dim, seed, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_mixture(float *out, int n, int dim, int ncomp, float sigma,
+                              uint64_t centre_seed, uint64_t point_seed, hipStream_t s) {
+    hipLaunchKernelGGL(gen_mixture_kernel, dim3(gen_grid((long long)n * dim)), dim3(256), 0, s,
+                       out, n, dim, ncomp, sigma, centre_seed, point_seed);
+    return hipGetLastError();
+}
+
+}  // namespace knnhip

@@ -1,0 +1,214 @@
+"""Python binding of libknnhip.so (the MI355X exact k-NN drop-in library).
+
+Mirrors the reference interface (code/knncuda.h:14-21, :37-44, :61-68):
+``knn_cuda_global``, ``knn_cuda_texture`` and ``knn_cublas`` take dim x n
+row-major point sets and return k x query_nb distances and indexes, with the
+reference's failure behaviour (an "ERROR:" line from the library and a false
+result, raised here as ``KnnError``).
+
+``search_device`` is the device-resident form used by the benchmark: torch
+tensors already in GPU memory, queued on the current torch stream.
+
+The library is loaded from this directory only; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libknnhip.so")
+
+# Every symbol the public header declares (include/knncuda.h).
+CXX_SYMBOLS = (
+    "_Z15knn_cuda_globalPKfiS0_iiiPfPi",
+    "_Z16knn_cuda_texturePKfiS0_iiiPfPi",
+    "_Z10knn_cublasPKfiS0_iiiPfPi",
+)
+C_SYMBOLS = (
+    "knnhip_global",
+    "knnhip_texture",
+    "knnhip_cublas",
+    "knnhip_search_device",
+    "knnhip_last_fallback_count",
+    "knnhip_last_stage_ms",
+    "knnhip_last_error",
+    "knnhip_generate_uniform",
+    "knnhip_generate_mixture",
+    "knnhip_version",
+)
+
+VARIANTS = {"global": 0, "texture": 1, "cublas": 2}
+
+
+class KnnError(RuntimeError):
+    pass
+
+
+class Options(ctypes.Structure):
+    """knnhip_options (include/knncuda.h)."""
+
+    _fields_ = [
+        ("variant", ctypes.c_int),
+        ("margin", ctypes.c_int),
+        ("slices", ctypes.c_int),
+        ("force_fallback", ctypes.c_int),
+        ("timing", ctypes.c_int),
+        ("reserved", ctypes.c_int * 8),
+    ]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libknnhip.so from this directory (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KnnError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(ctypes.c_int)
+    for name in ("knnhip_global", "knnhip_texture", "knnhip_cublas"):
+        fn = getattr(lib, name)
+        fn.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, ip]
+        fn.restype = ctypes.c_int
+    lib.knnhip_search_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.POINTER(Options), ctypes.c_void_p,
+    ]
+    lib.knnhip_search_device.restype = ctypes.c_int
+    lib.knnhip_last_fallback_count.restype = ctypes.c_int
+    lib.knnhip_last_stage_ms.argtypes = [ctypes.c_int]
+    lib.knnhip_last_stage_ms.restype = ctypes.c_float
+    lib.knnhip_last_error.restype = ctypes.c_char_p
+    lib.knnhip_version.restype = ctypes.c_char_p
+    lib.knnhip_generate_uniform.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_float,
+        ctypes.c_void_p,
+    ]
+    lib.knnhip_generate_uniform.restype = ctypes.c_int
+    lib.knnhip_generate_mixture.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+        ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_void_p,
+    ]
+    lib.knnhip_generate_mixture.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().knnhip_last_error().decode()
+
+
+def _host_call(name: str, ref: np.ndarray, query: np.ndarray, k: int) -> Tuple[np.ndarray, np.ndarray]:
+    lib = load()
+    ref = np.ascontiguousarray(ref, dtype=np.float32)
+    query = np.ascontiguousarray(query, dtype=np.float32)
+    if ref.ndim != 2 or query.ndim != 2 or ref.shape[0] != query.shape[0]:
+        raise KnnError("ref and query must be dim x n arrays with the same dim")
+    dim, ref_nb = ref.shape
+    query_nb = query.shape[1]
+    dist = np.empty((k, query_nb), dtype=np.float32)
+    idx = np.empty((k, query_nb), dtype=np.int32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(ctypes.c_int)
+    ok = getattr(lib, name)(
+        ref.ctypes.data_as(fp), ref_nb, query.ctypes.data_as(fp), query_nb, dim, k,
+        dist.ctypes.data_as(fp), idx.ctypes.data_as(ip),
+    )
+    if not ok:
+        raise KnnError(last_error())
+    return dist, idx
+
+
+def knn_cuda_global(ref: np.ndarray, query: np.ndarray, k: int):
+    """Drop-in for code/knncuda.h:14-21 on host arrays; returns (dist, index), each k x query_nb."""
+    return _host_call("knnhip_global", ref, query, k)
+
+
+def knn_cuda_texture(ref: np.ndarray, query: np.ndarray, k: int):
+    """Drop-in for code/knncuda.h:37-44."""
+    return _host_call("knnhip_texture", ref, query, k)
+
+
+def knn_cublas(ref: np.ndarray, query: np.ndarray, k: int):
+    """Drop-in for code/knncuda.h:61-68."""
+    return _host_call("knnhip_cublas", ref, query, k)
+
+
+def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "global",
+                  margin: int = 0, slices: int = 0, force_fallback: bool = False,
+                  timing: bool = False, stream: Optional[int] = None):
+    """Exact k-NN on device-resident torch tensors (dim x n, float32, row stride = n or more).
+
+    Returns (dist, index) torch tensors of shape (k, query_nb).
+    """
+    import torch
+
+    lib = load()
+    if ref.dtype != torch.float32 or query.dtype != torch.float32:
+        raise KnnError("ref and query must be float32")
+    if not (ref.is_cuda and query.is_cuda):
+        raise KnnError("ref and query must be device tensors")
+    if ref.stride(1) != 1 or query.stride(1) != 1:
+        raise KnnError("rows must be contiguous")
+    dim, ref_nb = ref.shape
+    query_nb = query.shape[1]
+    if dist is None:
+        dist = torch.empty((k, query_nb), dtype=torch.float32, device=query.device)
+    if index is None:
+        index = torch.empty((k, query_nb), dtype=torch.int32, device=query.device)
+    opt = Options()
+    opt.variant = VARIANTS[variant]
+    opt.margin = margin
+    opt.slices = slices
+    opt.force_fallback = 1 if force_fallback else 0
+    opt.timing = 1 if timing else 0
+    if stream is None:
+        stream = torch.cuda.current_stream(query.device).cuda_stream
+    ok = lib.knnhip_search_device(
+        ref.data_ptr(), ref_nb, ref.stride(0), query.data_ptr(), query_nb, query.stride(0), dim, k,
+        dist.data_ptr(), index.data_ptr(), ctypes.byref(opt), ctypes.c_void_p(stream),
+    )
+    if not ok:
+        raise KnnError(last_error())
+    return dist, index
+
+
+def last_fallback_count() -> int:
+    return load().knnhip_last_fallback_count()
+
+
+def last_stage_ms(stage: int) -> float:
+    return float(load().knnhip_last_stage_ms(stage))
+
+
+def generate_uniform(n: int, dim: int, seed: int, scale: float = 1.0, device: str = "cuda"):
+    """Seeded uniform point set generated on the device (dim x n float32 tensor)."""
+    import torch
+
+    out = torch.empty((dim, n), dtype=torch.float32, device=device)
+    stream = torch.cuda.current_stream(out.device).cuda_stream
+    if not load().knnhip_generate_uniform(out.data_ptr(), n, dim, seed, scale, ctypes.c_void_p(stream)):
+        raise KnnError(last_error())
+    return out
+
+
+def generate_mixture(n: int, dim: int, ncomp: int, sigma: float, centre_seed: int, point_seed: int,
+                     device: str = "cuda"):
+    """Seeded Gaussian-mixture point set generated on the device (dim x n float32 tensor)."""
+    import torch
+
+    out = torch.empty((dim, n), dtype=torch.float32, device=device)
+    stream = torch.cuda.current_stream(out.device).cuda_stream
+    if not load().knnhip_generate_mixture(out.data_ptr(), n, dim, ncomp, sigma, centre_seed,
+                                          point_seed, ctypes.c_void_p(stream)):
+        raise KnnError(last_error())
+    return out

@@ -1,0 +1,195 @@
+"""GPU parity tests: the HIP path through the C-ABI against the oracle (bit-exact).
+
+Bar: indexes equal and distances bit-identical to the reference ground truth
+(code/test.cpp:129-175), on the golden vectors, on seeded random shapes that
+hit every tile tail, on degenerate ties, on the repo-default configuration in
+full, and through size-independent properties at the 1M-reference scale.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import knnhip
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from test_oracle import ARRAY_CASES, GEN_CASES, load_case  # noqa: E402
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def assert_exact(d, i, gd, gi, what=""):
+    d = d.cpu().numpy() if hasattr(d, "cpu") else d
+    i = i.cpu().numpy() if hasattr(i, "cpu") else i
+    bad_i = np.argwhere(i != gi)
+    assert bad_i.size == 0, f"{what}: {len(bad_i)} index mismatches, first at {bad_i[:3].tolist()}"
+    bad_d = np.argwhere(bits(d) != bits(gd))
+    assert bad_d.size == 0, f"{what}: {len(bad_d)} distance mismatches, first at {bad_d[:3].tolist()}"
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("entry", ["knn_cuda_global", "knn_cuda_texture", "knn_cublas"])
+@pytest.mark.parametrize("name", ARRAY_CASES + GEN_CASES)
+def test_golden_through_host_entry_points(entry, name):
+    say(f"golden {name} via {entry}")
+    ref, qry, k, gd, gi = load_case(name)
+    d, i = getattr(knnhip, entry)(ref, qry, k)
+    assert_exact(d, i, gd, gi, f"{name}/{entry}")
+    # the reference's own acceptance check (code/test.cpp:256-272)
+    assert oracle.check(d, i, gd, gi) == (1.0, 1.0)
+
+
+SHAPES = [
+    # (R, Q, D, k)
+    (1, 1, 1, 1),
+    (5, 3, 2, 5),
+    (129, 33, 3, 7),
+    (1000, 65, 16, 16),
+    (1031, 97, 17, 32),
+    (4097, 31, 31, 1),
+    (3000, 130, 64, 64),
+    (2500, 40, 96, 16),
+    (2049, 70, 100, 8),
+    (5000, 50, 128, 128),
+    (1500, 20, 200, 24),
+    (2000, 45, 256, 64),
+    (700, 35, 300, 10),
+    (900, 17, 512, 5),
+    (3333, 9, 128, 200),
+    (600, 12, 24, 600),
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "R%d_Q%d_D%d_k%d" % s)
+def test_random_shapes_device_api(shape):
+    R, Q, D, k = shape
+    say(f"shape R={R} Q={Q} D={D} k={k}")
+    rng = np.random.default_rng(R * 131 + Q * 7 + D)
+    ref = rng.random((D, R), dtype=np.float32)
+    qry = rng.random((D, Q), dtype=np.float32)
+    gd, gi = oracle.knn_c(ref, qry, k)
+    d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k)
+    torch.cuda.synchronize()
+    assert_exact(d, i, gd, gi, f"shape {shape}")
+
+
+def test_strided_device_inputs():
+    R, Q, D, k = 1000, 40, 32, 12
+    rng = np.random.default_rng(9)
+    big_r = torch.from_numpy(rng.random((D, R + 77), dtype=np.float32)).cuda()
+    big_q = torch.from_numpy(rng.random((D, Q + 5), dtype=np.float32)).cuda()
+    ref, qry = big_r[:, :R], big_q[:, :Q]
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    d, i = knnhip.search_device(ref, qry, k)
+    assert_exact(d, i, gd, gi, "strided")
+
+
+@pytest.mark.parametrize("variant", ["global", "texture", "cublas"])
+def test_forced_fallback_equals_oracle(variant):
+    R, Q, D, k = 3000, 50, 40, 20
+    rng = np.random.default_rng(21)
+    ref = rng.random((D, R), dtype=np.float32)
+    qry = rng.random((D, Q), dtype=np.float32)
+    gd, gi = oracle.knn_c(ref, qry, k)
+    d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k,
+                                variant=variant, force_fallback=True)
+    assert knnhip.last_fallback_count() == Q
+    assert_exact(d, i, gd, gi, "forced fallback")
+
+
+def test_massive_ties_take_the_exact_path():
+    # every reference point identical: every key ties, the answer is indexes 0..k-1
+    R, Q, D, k = 2000, 10, 8, 16
+    ref = np.full((D, R), 0.5, dtype=np.float32)
+    qry = np.random.default_rng(2).random((D, Q), dtype=np.float32)
+    gd, gi = oracle.knn_c(ref, qry, k)
+    assert np.array_equal(gi, np.tile(np.arange(k)[:, None], (1, Q)))
+    d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k)
+    assert_exact(d, i, gd, gi, "all-equal references")
+
+
+def test_tiny_margin_on_clustered_data_stays_exact():
+    # margin 1 on near-tie data forces certificate failures; the result must not change
+    R, Q, D, k = 6000, 64, 64, 16
+    ref = oracle.gen_mixture(R, D, 8, 0.01, 5, 6)
+    qry = oracle.gen_mixture(Q, D, 8, 0.01, 5, 7)
+    gd, gi = oracle.knn_c(ref, qry, k)
+    d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k, margin=1)
+    nfb = knnhip.last_fallback_count()
+    say(f"margin=1 fallbacks: {nfb}/{Q}")
+    assert_exact(d, i, gd, gi, "margin 1")
+
+
+@pytest.mark.parametrize("slices", [1, 3, 8, 16])
+def test_slice_counts_agree(slices):
+    R, Q, D, k = 20000, 96, 48, 24
+    rng = np.random.default_rng(slices)
+    ref = rng.random((D, R), dtype=np.float32)
+    qry = rng.random((D, Q), dtype=np.float32)
+    gd, gi = oracle.knn_c(ref, qry, k)
+    d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k, slices=slices)
+    assert_exact(d, i, gd, gi, f"slices={slices}")
+
+
+def test_device_generator_matches_cpu_generator():
+    a = knnhip.generate_uniform(5000, 13, seed=77, scale=10.0).cpu().numpy()
+    b = oracle.gen_uniform(5000, 13, 77, 10.0)
+    assert np.array_equal(bits(a), bits(b))
+    m = knnhip.generate_mixture(3000, 12, 5, 0.1, 3, 4).cpu().numpy()
+    mc = oracle.gen_mixture(3000, 12, 5, 0.1, 3, 4)
+    assert np.max(np.abs(m - mc)) < 1e-5
+
+
+def test_repo_default_config_full():
+    # code/test.cpp:295-298: 16384 refs, 4096 queries, dim 128, k 16, data 10*U[0,1)
+    R, Q, D, k = 16384, 4096, 128, 16
+    say("repo default config 16384x4096x128 k=16")
+    ref = knnhip.generate_uniform(R, D, seed=0x6B4E4E00, scale=10.0)
+    qry = knnhip.generate_uniform(Q, D, seed=0x6B4E4E10, scale=10.0)
+    for variant in ("global", "texture", "cublas"):
+        d, i = knnhip.search_device(ref, qry, k, variant=variant)
+        torch.cuda.synchronize()
+        if variant == "global":
+            gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+        assert_exact(d, i, gd, gi, f"default config / {variant}")
+    say(f"fallbacks: {knnhip.last_fallback_count()}")
+    # stated error against FP64 truth on a query subset
+    sub = qry[:, :256].cpu().numpy()
+    d64, _ = oracle.knn_f64(ref.cpu().numpy(), sub, k)
+    err = np.abs(d[:, :256].cpu().numpy().astype(np.float64) - d64) / d64
+    say(f"max relative distance error vs FP64: {err.max():.3e}")
+    assert err.max() < 1e-5
+
+
+def test_one_million_refs_properties():
+    # 1,048,576 refs x 2048 queries, dim 128, k 32: size-independent properties on all queries,
+    # exact equality with the oracle on a query sample.
+    R, Q, D, k = 1 << 20, 2048, 128, 32
+    say("1M refs properties")
+    ref = knnhip.generate_uniform(R, D, seed=0x6B4E4E02)
+    qry = knnhip.generate_uniform(Q, D, seed=0x6B4E4E12)
+    d, i = knnhip.search_device(ref, qry, k)
+    torch.cuda.synchronize()
+    dh, ih = d.cpu().numpy(), i.cpu().numpy()
+    assert np.all(np.diff(dh, axis=0) >= 0)
+    assert np.all((ih >= 0) & (ih < R))
+    srt = np.sort(ih, axis=0)
+    assert np.all(np.diff(srt, axis=0) > 0)  # distinct per query
+    sample = [0, 1, 777, 2047]
+    ref_h = ref.cpu().numpy()
+    qs = np.ascontiguousarray(qry.cpu().numpy()[:, sample])
+    gd, gi = oracle.knn_c(ref_h, qs, k)
+    assert_exact(dh[:, sample], ih[:, sample], gd, gi, "1M sample")
+    say(f"fallbacks: {knnhip.last_fallback_count()}")
