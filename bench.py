@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device copy of this many MiB before the timed region (a "
+                         "known-size kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -65,6 +68,14 @@ def main():
     idx = torch.empty((k, Q), dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream(dev)
     torch.cuda.synchronize()
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         knnhip.search_device(ref, qry, k, dist, idx, stream=stream.cuda_stream)
@@ -147,6 +158,7 @@ def main():
         "stages_ms": {name: sum(v) / len(v) for name, v in
                       zip(("norms", "scan", "select_rerank", "fallback"), stage_ms)},
         "fallback_queries": fallbacks,
+        "plan": knnhip.last_plan(),
         "parity_gate_queries": gate_q,
     }
     print(json.dumps(out), flush=True)

@@ -83,6 +83,10 @@ float knnhip_last_stage_ms(int stage);
 /* Text of the last error reported by this library (empty if none). */
 const char *knnhip_last_error(void);
 
+/* Launch plan of the last search as a JSON object (slices, heap size, blocks
+ * per CU, ...), for benchmarks and diagnostics. */
+const char *knnhip_last_plan(void);
+
 /* Seeded synthetic point sets written directly to device memory, dim x n
  * row-major.  Uniform: scale * U[0,1) from a counter-based hash (bit-identical
  * to the CPU generator in the oracle).  Mixture: ncomp isotropic Gaussians of

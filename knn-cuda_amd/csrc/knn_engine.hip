@@ -23,6 +23,8 @@ namespace knnhip {
 
 // Launchers defined in knn_scan.hip / knn_select.hip.
 int scan_pick_ks(int dim);
+size_t scan_lds_bytes(int kp, int waves);
+int scan_blocks_per_cu(int ks, bool multi, int waves, int kp);
 hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s);
 hipError_t launch_norms(const float *p, long long ld, int n, int dim, float *out,
                         unsigned *maxbits, hipStream_t s);
@@ -43,6 +45,8 @@ std::mutex g_mu;
 std::string g_last_error;
 float g_stage_ms[4] = {-1.f, -1.f, -1.f, -1.f};
 hipStream_t g_last_stream = nullptr;
+std::string g_last_plan;
+int g_cu_count = 0;
 
 void set_error(const char *fmt, ...) {
     char buf[512];
@@ -99,38 +103,76 @@ struct Plan {
     int nqb = 0;
     int xcd = 0;
     int maxc = 0;
+    int blocks_per_cu = 0;
+    int cus = 0;
 };
 
-Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
+// Cost model of one slice count: the scan runs in `rounds` waves of
+// resident blocks, each block streaming ceil(steps / S) reference steps plus
+// a warm-up of about two steps while its heaps fill.
+double slice_cost(int S, int steps, int nqb, int resident) {
+    const long long units = (long long)nqb * S;
+    const long long rounds = (units + resident - 1) / resident;
+    return (double)rounds * ((double)((steps + S - 1) / S) + 2.0);
+}
+
+// Bytes one dimension chunk of the reference matrix spans (the scan
+// addresses a chunk through one buffer resource with 32-bit offsets).
+size_t chunk_span(int ks, long long ldr) { return (size_t)2 * ks * ldr * sizeof(float); }
+constexpr size_t kMaxSpan = 0xFFFFFFFFull;
+
+Plan make_plan(int R, int Q, int dim, int k, long long ldr, const knnhip_options &o) {
     Plan p;
     const int margin = o.margin > 0 ? o.margin : kDefaultMargin;
     p.kp = std::min(k + margin, R);
     if (p.kp < k) p.kp = k;
-    // Heaps: waves * 32 queries * kp * 8 bytes of LDS per block.
-    const size_t per_wave = (size_t)kQueriesPerWave * p.kp * 8;
+    // Heaps: waves * 32 queries * (kp | 1) * 8 bytes of LDS per block.
+    const size_t per_wave = scan_lds_bytes(p.kp, 1);
     if (per_wave > kScanLdsMax) return p;  // k too large for the scan's heaps
     p.waves = (4 * per_wave <= 64 * 1024) ? 4 : (2 * per_wave <= 80 * 1024) ? 2 : 1;
     p.ks = scan_pick_ks(dim);
-    if (p.ks) {
+    if (p.ks && chunk_span(p.ks, ldr) < kMaxSpan) {
         p.dpad = 2 * p.ks;
         p.nchunks = 1;
         p.multi = false;
     } else {
-        p.ks = 64;
         p.multi = true;
-        p.nchunks = (dim + 127) / 128;
-        p.dpad = 128 * p.nchunks;
+        p.ks = chunk_span(64, ldr) < kMaxSpan ? 64 : 16;
+        if (chunk_span(p.ks, ldr) >= kMaxSpan) return p;  // row pitch too wide for the scan
+        p.nchunks = (dim + 2 * p.ks - 1) / (2 * p.ks);
+        p.dpad = 2 * p.ks * p.nchunks;
     }
     const int qpb = p.waves * kQueriesPerWave;
     p.nqb = (Q + qpb - 1) / qpb;
-    // Slices: enough blocks to fill 256 CUs several times over, bounded by the
-    // merge capacity of the select stage and by one step per slice.
+    // Slices: the count whose blocks fill whole rounds of resident blocks
+    // (cost model above), bounded by the merge capacity of the select stage
+    // and by one step per slice.
     const int steps = (R + kRefsPerStep - 1) / kRefsPerStep;
-    int s = o.slices > 0 ? o.slices : std::max(1, (2048 + p.nqb - 1) / p.nqb);
-    s = std::min(s, kMaxUnion / p.kp);
-    s = std::min(s, steps);
-    s = std::max(s, 1);
-    if (s >= 8 && o.slices <= 0) s -= s % 8;
+    const int smax = std::max(1, std::min(kMaxUnion / p.kp, steps));
+    if (g_cu_count <= 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            g_cu_count = n;
+        else
+            (void)hipGetLastError();
+    }
+    p.cus = g_cu_count > 0 ? g_cu_count : 256;
+    p.blocks_per_cu = std::max(1, scan_blocks_per_cu(p.ks, p.multi, p.waves, p.kp));
+    const int resident = p.cus * p.blocks_per_cu;
+    int s = 1;
+    if (o.slices > 0) {
+        s = std::min(o.slices, smax);
+    } else {
+        double best = slice_cost(1, steps, p.nqb, resident);
+        for (int c = 2; c <= smax; ++c) {
+            const double cost = slice_cost(c, steps, p.nqb, resident);
+            if (cost < best) {
+                best = cost;
+                s = c;
+            }
+        }
+    }
     const int steps_per_slice = (steps + s - 1) / s;
     p.slice_len = steps_per_slice * kRefsPerStep;
     p.nslices = s;
@@ -138,6 +180,18 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     p.maxc = std::min(kMaxCand, std::max(2 * p.kp, 256));
     p.use_scan = true;
     return p;
+}
+
+std::string plan_json(const Plan &p, int R, int Q, int dim, int k) {
+    char buf[512];
+    snprintf(buf, sizeof(buf),
+             "{\"ref_nb\": %d, \"query_nb\": %d, \"dim\": %d, \"k\": %d, \"scan\": %s, "
+             "\"ks\": %d, \"chunks\": %d, \"dpad\": %d, \"waves_per_block\": %d, \"kp\": %d, "
+             "\"slices\": %d, \"slice_len\": %d, \"query_blocks\": %d, \"xcd_map\": %d, "
+             "\"blocks_per_cu\": %d, \"cus\": %d, \"max_candidates\": %d}",
+             R, Q, dim, k, p.use_scan ? "true" : "false", p.ks, p.nchunks, p.dpad, p.waves, p.kp,
+             p.nslices, p.slice_len, p.nqb, p.xcd, p.blocks_per_cu, p.cus, p.maxc);
+    return buf;
 }
 
 bool check_args(const void *ref, int R, const void *query, int Q, int dim, int k, const void *dist,
@@ -320,7 +374,8 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
     if (!check_args(ref, R, query, Q, dim, k, knn_dist, knn_index)) return false;
     if (!ensure_device()) return false;
     const knnhip_options o = default_options(variant);
-    const Plan p = make_plan(R, Q, dim, k, o);
+    const Plan p = make_plan(R, Q, dim, k, R, o);
+    g_last_plan = plan_json(p, R, Q, dim, k);
     const int rows = p.use_scan ? p.dpad : dim;
     float *ref_d = (float *)ws_get(kRefPad, (size_t)rows * R * sizeof(float));
     float *qry_d = (float *)ws_get(kQryPad, (size_t)rows * Q * sizeof(float));
@@ -413,7 +468,9 @@ int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const f
     if (!ensure_device()) return 0;
     knnhip_options o = opt ? *opt : default_options(0);
     hipStream_t s = (hipStream_t)stream;
-    const Plan p = make_plan(ref_nb, query_nb, dim, k, o);
+    Plan p = make_plan(ref_nb, query_nb, dim, k, ref_ld, o);
+    if (p.use_scan && p.dpad != dim) p = make_plan(ref_nb, query_nb, dim, k, ref_nb, o);
+    g_last_plan = plan_json(p, ref_nb, query_nb, dim, k);
     const float *ref_use = ref, *qry_use = query;
     long long ldr = ref_ld, ldq = query_ld;
     if (p.use_scan && p.dpad != dim) {
@@ -465,6 +522,8 @@ float knnhip_last_stage_ms(int stage) {
 }
 
 const char *knnhip_last_error(void) { return g_last_error.c_str(); }
+
+const char *knnhip_last_plan(void) { return g_last_plan.c_str(); }
 
 int knnhip_generate_uniform(float *out, int n, int dim, unsigned long long seed, float scale,
                             void *stream) {

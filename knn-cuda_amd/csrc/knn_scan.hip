@@ -59,6 +59,8 @@ __device__ __forceinline__ float heap_replace_top(float *hv, int *hi, int kp, fl
     return hv[0];
 }
 
+__host__ __device__ constexpr int heap_stride(int kp) { return kp | 1; }
+
 template <int KS, bool MULTI>
 __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -88,17 +90,20 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     if (qbase >= a.Q) return;  // whole wave is past the last query
     const int q = min(qbase + col, a.Q - 1);
 
+    // Heaps in LDS with an odd row stride, so the 32 lanes of a half walking
+    // their own heaps in step hit distinct banks.
     const int kp = a.kp;
-    float *hv = reinterpret_cast<float *>(smem) + (size_t)wave * kQueriesPerWave * kp;
-    int *hi = reinterpret_cast<int *>(smem + (size_t)waves * kQueriesPerWave * kp * sizeof(float)) +
-              (size_t)wave * kQueriesPerWave * kp;
-    for (int e = lane; e < kQueriesPerWave * kp; e += kWave) {
+    const int kps = heap_stride(kp);
+    float *hv = reinterpret_cast<float *>(smem) + (size_t)wave * kQueriesPerWave * kps;
+    int *hi = reinterpret_cast<int *>(smem + (size_t)waves * kQueriesPerWave * kps * sizeof(float)) +
+              (size_t)wave * kQueriesPerWave * kps;
+    for (int e = lane; e < kQueriesPerWave * kps; e += kWave) {
         hv[e] = INFINITY;
         hi[e] = -1;
     }
     wave_sync();
-    float *myv = hv + col * kp;
-    int *myi = hi + col * kp;
+    float *myv = hv + col * kps;
+    int *myi = hi + col * kps;
     float theta = INFINITY;
 
     const long long ldr = a.ldr, ldq = a.ldq;
@@ -112,47 +117,71 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
 
     const int r_begin = slice * a.slice_len;
     const int r_end = min(a.R, r_begin + a.slice_len);
-    for (int r0 = r_begin; r0 < r_end; r0 += kRefsPerStep) {
-        f32x16 acc[kMTiles];
-        unsigned voff[kMTiles];
+
+    // Reference operands come through a raw buffer resource per dimension
+    // chunk: the lane's byte offset (point + lane-half row) stays in a VGPR,
+    // the k-step row offset is a scalar, so a load is one instruction.
+    const unsigned kstep_bytes = (unsigned)(2 * ldr * sizeof(float));
+    const unsigned span_bytes = (unsigned)min((long long)2 * KS * ldr * (long long)sizeof(float),
+                                              (long long)0xFFFFFFFFll);
+    auto lane_offsets = [&](int rs, unsigned *vo) {
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt) {
-            const int r = r0 + kRefsPerTile * mt + col;
-            const bool ok = r < a.R;
-            const int rc = ok ? r : a.R - 1;
-            voff[mt] = (unsigned)((long long)h * ldr + rc);
-            const float nv = ok ? a.rnorm[rc] : INFINITY;
-            const float av = h ? 0.0f : nv;
-            const float bv = h ? 0.0f : 1.0f;
-            f32x16 z = {};
-            acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, z, 0, 0, 0);
+            const int r = min(rs + kRefsPerTile * mt + col, a.R - 1);
+            vo[mt] = (unsigned)(((long long)h * ldr + r) * (long long)sizeof(float));
         }
+    };
+    auto chunk_rsrc = [&](int c) {
+        return __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(a.ref + (long long)c * 2 * KS * ldr), (short)0, (int)span_bytes,
+            0x00020000);
+    };
+    float abuf[2][CH][kMTiles];
+    auto issue = [&](float (&buf)[CH][kMTiles], __amdgpu_buffer_rsrc_t rs, int g,
+                     const unsigned *vo) {
+#pragma unroll
+        for (int kk = 0; kk < CH; ++kk) {
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+                buf[kk][mt] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                    rs, (int)vo[mt], (int)((g * CH + kk) * kstep_bytes), 0));
+        }
+    };
+    // With an even number of prefetch groups the next step's first group can
+    // be fetched while the current step's last group computes.
+    constexpr bool kCrossStep = !MULTI && (NG % 2 == 0);
+
+    unsigned voff[kMTiles];
+    lane_offsets(r_begin, voff);
+    if (kCrossStep && r_begin < r_end) issue(abuf[0], chunk_rsrc(0), 0, voff);
+
+    for (int r0 = r_begin; r0 < r_end; r0 += kRefsPerStep) {
+        f32x16 acc[kMTiles];
+        float nrm[kMTiles];
+#pragma unroll
+        for (int mt = 0; mt < kMTiles; ++mt) {
+            acc[mt] = f32x16{};
+            const int r = r0 + kRefsPerTile * mt + col;
+            nrm[mt] = r < a.R ? a.rnorm[r] : INFINITY;
+        }
+        const int r_next = r0 + kRefsPerStep;
+        unsigned voff_next[kMTiles];
+        lane_offsets(r_next, voff_next);
         for (int c = 0; c < nchunks; ++c) {
+            const __amdgpu_buffer_rsrc_t rs = chunk_rsrc(c);
             if (MULTI) {
 #pragma unroll
                 for (int kk = 0; kk < KS; ++kk)
                     bq[kk] = -2.0f * qcol[(long long)(c * 2 * KS + 2 * kk) * ldq];
             }
-            const float *__restrict__ cbase = a.ref + (long long)c * 2 * KS * ldr;
-            float abuf[2][CH][kMTiles];
-#pragma unroll
-            for (int kk = 0; kk < CH; ++kk) {
-                const float *__restrict__ row = cbase + (long long)(2 * kk) * ldr;
-#pragma unroll
-                for (int mt = 0; mt < kMTiles; ++mt) abuf[0][kk][mt] = row[voff[mt]];
-            }
+            if (!kCrossStep) issue(abuf[0], rs, 0, voff);
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                if (g + 1 < NG) {
-#pragma unroll
-                    for (int kk = 0; kk < CH; ++kk) {
-                        const float *__restrict__ row =
-                            cbase + (long long)(2 * ((g + 1) * CH + kk)) * ldr;
-#pragma unroll
-                        for (int mt = 0; mt < kMTiles; ++mt)
-                            abuf[(g + 1) & 1][kk][mt] = row[voff[mt]];
-                    }
-                }
+                if (g + 1 < NG)
+                    issue(abuf[(g + 1) & 1], rs, g + 1, voff);
+                else if (kCrossStep && r_next < r_end)
+                    issue(abuf[0], rs, 0, voff_next);
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int kk = 0; kk < CH; ++kk) {
 #pragma unroll
@@ -162,6 +191,13 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
                 }
             }
         }
+        // ||r||^2 enters as one more k-step: A = norm (lane half 0), B = 1.
+#pragma unroll
+        for (int mt = 0; mt < kMTiles; ++mt)
+            acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(h ? 0.0f : nrm[mt], h ? 0.0f : 1.0f,
+                                                           acc[mt], 0, 0, 0);
+#pragma unroll
+        for (int mt = 0; mt < kMTiles; ++mt) voff[mt] = voff_next[mt];
 
         // Epilogue: one compare per key; the slow path runs only when some
         // lane of the wave holds a key below its query's threshold.
@@ -173,9 +209,13 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         if (__ballot(cand)) {
 #pragma unroll
             for (int hh = 0; hh < 2; ++hh) {
-                if (h == hh) {
 #pragma unroll
-                    for (int mt = 0; mt < kMTiles; ++mt) {
+                for (int mt = 0; mt < kMTiles; ++mt) {
+                    bool cm = false;
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) cm |= acc[mt][i] < theta;
+                    if (__ballot(cm && h == hh) == 0) continue;
+                    if (h == hh) {
 #pragma unroll
                         for (int i = 0; i < 16; ++i) {
                             const float v = acc[mt][i];
@@ -192,14 +232,15 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         }
     }
 
-    // Heaps of the wave's 32 consecutive queries are one contiguous block in
-    // LDS and in the [slice][query][kp] output.
+    // The wave's 32 consecutive queries are one contiguous block of the
+    // [slice][query][kp] output.
     wave_sync();
     const long long obase = ((long long)slice * a.Q + qbase) * kp;
     const int nvalid = min(kQueriesPerWave, a.Q - qbase) * kp;
     for (int e = lane; e < nvalid; e += kWave) {
-        a.part_val[obase + e] = hv[e];
-        a.part_idx[obase + e] = hi[e];
+        const int row = e / kp, j = e - row * kp;
+        a.part_val[obase + e] = hv[row * kps + j];
+        a.part_idx[obase + e] = hi[row * kps + j];
     }
 }
 
@@ -225,11 +266,51 @@ static hipError_t launch_scan_t(const ScanArgs &a, int waves, size_t lds, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s) {
-    const size_t lds = (size_t)waves * kQueriesPerWave * a.kp * (sizeof(float) + sizeof(int));
+size_t scan_lds_bytes(int kp, int waves) {
+    return (size_t)waves * kQueriesPerWave * heap_stride(kp) * (sizeof(float) + sizeof(int));
+}
+
+static const void *scan_fn(int ks, bool multi) {
     if (multi) {
-        if (ks != 64) return hipErrorInvalidValue;
-        return launch_scan_t<64, true>(a, waves, lds, s);
+        if (ks == 64) return (const void *)scan_f32_kernel<64, true>;
+        if (ks == 16) return (const void *)scan_f32_kernel<16, true>;
+        return nullptr;
+    }
+    switch (ks) {
+        case 8: return (const void *)scan_f32_kernel<8, false>;
+        case 16: return (const void *)scan_f32_kernel<16, false>;
+        case 24: return (const void *)scan_f32_kernel<24, false>;
+        case 32: return (const void *)scan_f32_kernel<32, false>;
+        case 48: return (const void *)scan_f32_kernel<48, false>;
+        case 64: return (const void *)scan_f32_kernel<64, false>;
+        case 96: return (const void *)scan_f32_kernel<96, false>;
+        case 128: return (const void *)scan_f32_kernel<128, false>;
+        default: return nullptr;
+    }
+}
+
+// Resident blocks per CU of one scan instance (VGPR and LDS limits).
+int scan_blocks_per_cu(int ks, bool multi, int waves, int kp) {
+    const void *fn = scan_fn(ks, multi);
+    if (!fn) return 0;
+    const size_t lds = scan_lds_bytes(kp, waves);
+    if (lds > 65536 &&
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return 0;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, waves * kWave, lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s) {
+    const size_t lds = scan_lds_bytes(a.kp, waves);
+    if (multi) {
+        if (ks == 64) return launch_scan_t<64, true>(a, waves, lds, s);
+        if (ks == 16) return launch_scan_t<16, true>(a, waves, lds, s);
+        return hipErrorInvalidValue;
     }
     switch (ks) {
         case 8: return launch_scan_t<8, false>(a, waves, lds, s);

@@ -36,7 +36,15 @@ __global__ __launch_bounds__(256) void sq_norms_kernel(const float *__restrict__
     float s = 0.0f;
     if (i < n) {
         const float *col = p + i;
-        for (int d = 0; d < dim; ++d) {
+        int d = 0;
+        for (; d + 8 <= dim; d += 8) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = col[(long long)(d + j) * ld];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s = fmaf(v[j], v[j], s);
+        }
+        for (; d < dim; ++d) {
             const float v = col[(long long)d * ld];
             s = fmaf(v, v, s);
         }

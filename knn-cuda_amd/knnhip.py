@@ -36,6 +36,7 @@ C_SYMBOLS = (
     "knnhip_last_fallback_count",
     "knnhip_last_stage_ms",
     "knnhip_last_error",
+    "knnhip_last_plan",
     "knnhip_generate_uniform",
     "knnhip_generate_mixture",
     "knnhip_version",
@@ -88,6 +89,7 @@ def load() -> ctypes.CDLL:
     lib.knnhip_last_stage_ms.argtypes = [ctypes.c_int]
     lib.knnhip_last_stage_ms.restype = ctypes.c_float
     lib.knnhip_last_error.restype = ctypes.c_char_p
+    lib.knnhip_last_plan.restype = ctypes.c_char_p
     lib.knnhip_version.restype = ctypes.c_char_p
     lib.knnhip_generate_uniform.argtypes = [
         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_float,
@@ -180,6 +182,14 @@ def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "
     if not ok:
         raise KnnError(last_error())
     return dist, index
+
+
+def last_plan() -> dict:
+    """Launch plan of the last search (slices, heap size, blocks per CU, ...)."""
+    import json
+
+    text = load().knnhip_last_plan().decode()
+    return json.loads(text) if text else {}
 
 
 def last_fallback_count() -> int:
