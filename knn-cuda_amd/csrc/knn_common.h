@@ -54,6 +54,9 @@ struct FinalArgs {
     int maxc;                 // candidate capacity per query (<= kMaxCand)
     const float *part_val;
     const int *part_idx;
+    const float *ref_pm;      // point-major copies: point i at [i*pm_ld], dim floats
+    const float *query_pm;
+    int pm_ld;                // multiple of 4 (16-byte rows)
     const float *qnorm;       // Q squared norms (FP32)
     const unsigned *rmax;     // bit pattern of max_r ||r||^2 (non-negative float)
     int force_fallback;
@@ -126,6 +129,36 @@ __device__ __forceinline__ float canonical_sq(const float *__restrict__ ref, lon
     for (; d < dim; ++d) {
         float t = rp[(long long)d * ldr] - qp[(long long)d * ldq];
         float s = t * t;
+        acc = acc + s;
+    }
+    return acc;
+}
+
+// The same ground-truth sum over two contiguous point-major rows.
+__device__ __forceinline__ float canonical_sq_rows(const float *__restrict__ r,
+                                                   const float *__restrict__ q, int dim) {
+#pragma clang fp contract(off)
+    float acc = 0.0f;
+    int d = 0;
+    for (; d + 4 <= dim; d += 4) {
+        const float4 rv = *reinterpret_cast<const float4 *>(r + d);
+        const float4 qv = *reinterpret_cast<const float4 *>(q + d);
+        float t = rv.x - qv.x;
+        float s = t * t;
+        acc = acc + s;
+        t = rv.y - qv.y;
+        s = t * t;
+        acc = acc + s;
+        t = rv.z - qv.z;
+        s = t * t;
+        acc = acc + s;
+        t = rv.w - qv.w;
+        s = t * t;
+        acc = acc + s;
+    }
+    for (; d < dim; ++d) {
+        const float t = r[d] - q[d];
+        const float s = t * t;
         acc = acc + s;
     }
     return acc;

@@ -27,7 +27,7 @@ size_t scan_lds_bytes(int kp, int waves);
 int scan_blocks_per_cu(int ks, bool multi, int waves, int kp);
 hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s);
 hipError_t launch_norms(const float *p, long long ld, int n, int dim, float *out,
-                        unsigned *maxbits, hipStream_t s);
+                        unsigned *maxbits, float *pm, int pm_ld, hipStream_t s);
 hipError_t launch_select(const FinalArgs &a, hipStream_t s);
 hipError_t launch_exact(const ExactArgs &a, int grid, hipStream_t s);
 hipError_t launch_gen_uniform(float *out, int n, int dim, uint64_t seed, float scale,
@@ -60,7 +60,7 @@ void set_error(const char *fmt, ...) {
 // Grow-only device buffers, one per role.
 enum Slot {
     kRefPad, kQryPad, kRNorm, kQNorm, kRMax, kPartVal, kPartIdx, kFbCount, kFbList,
-    kFbKey, kFbIdx, kOutDist, kOutIdx, kNumSlots
+    kFbKey, kFbIdx, kOutDist, kOutIdx, kRefPm, kQryPm, kNumSlots
 };
 struct Workspace {
     void *ptr[kNumSlots] = {};
@@ -284,14 +284,18 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         float *pval = (float *)ws_get(kPartVal, npart * sizeof(float));
         int *pidx = (int *)ws_get(kPartIdx, npart * sizeof(int));
         int *fb_list = (int *)ws_get(kFbList, (size_t)Q * sizeof(int));
-        if (!rnorm || !qnorm || !rmax || !pval || !pidx || !fb_list) {
+        const int pm_ld = (dim + 3) & ~3;
+        float *ref_pm = (float *)ws_get(kRefPm, (size_t)R * pm_ld * sizeof(float));
+        float *qry_pm = (float *)ws_get(kQryPm, (size_t)Q * pm_ld * sizeof(float));
+        if (!rnorm || !qnorm || !rmax || !pval || !pidx || !fb_list || !ref_pm || !qry_pm) {
             set_error("Memory allocation error");
             return false;
         }
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[0], s), "event");
         KNN_TRY(hipMemsetAsync(rmax, 0, sizeof(unsigned), s), "memset");
-        KNN_TRY(launch_norms(ref, ldr, R, dim, rnorm, rmax, s), "norm kernel launch");
-        KNN_TRY(launch_norms(query, ldq, Q, dim, qnorm, nullptr, s), "norm kernel launch");
+        KNN_TRY(launch_norms(ref, ldr, R, dim, rnorm, rmax, ref_pm, pm_ld, s), "norm kernel launch");
+        KNN_TRY(launch_norms(query, ldq, Q, dim, qnorm, nullptr, qry_pm, pm_ld, s),
+                "norm kernel launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[1], s), "event");
 
         ScanArgs sa{};
@@ -328,6 +332,9 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         fa.maxc = p.maxc;
         fa.part_val = pval;
         fa.part_idx = pidx;
+        fa.ref_pm = ref_pm;
+        fa.query_pm = qry_pm;
+        fa.pm_ld = pm_ld;
         fa.qnorm = qnorm;
         fa.rmax = rmax;
         fa.force_fallback = o.force_fallback;

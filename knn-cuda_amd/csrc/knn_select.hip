@@ -29,13 +29,17 @@ namespace knnhip {
 // code/knncuda.cu:217-227).  Optionally the maximum over all points, as the
 // bit pattern of a non-negative float (integer order = float order).
 // ---------------------------------------------------------------------------
+// Also writes the point-major copy used by the exact re-rank's gathers
+// (point i at pm[i*pm_ld], pm_ld a multiple of 4).
 __global__ __launch_bounds__(256) void sq_norms_kernel(const float *__restrict__ p, long long ld,
                                                         int n, int dim, float *__restrict__ out,
-                                                        unsigned *maxbits) {
+                                                        unsigned *maxbits, float *__restrict__ pm,
+                                                        int pm_ld) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float s = 0.0f;
     if (i < n) {
         const float *col = p + i;
+        float *row = pm + (long long)i * pm_ld;
         int d = 0;
         for (; d + 8 <= dim; d += 8) {
             float v[8];
@@ -43,10 +47,13 @@ __global__ __launch_bounds__(256) void sq_norms_kernel(const float *__restrict__
             for (int j = 0; j < 8; ++j) v[j] = col[(long long)(d + j) * ld];
 #pragma unroll
             for (int j = 0; j < 8; ++j) s = fmaf(v[j], v[j], s);
+            *reinterpret_cast<float4 *>(row + d) = make_float4(v[0], v[1], v[2], v[3]);
+            *reinterpret_cast<float4 *>(row + d + 4) = make_float4(v[4], v[5], v[6], v[7]);
         }
         for (; d < dim; ++d) {
             const float v = col[(long long)d * ld];
             s = fmaf(v, v, s);
+            row[d] = v;
         }
         out[i] = s;
     }
@@ -147,7 +154,8 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
     if (!overflow) {
         for (int c = lane; c < nc; c += kWave) {
             const int r = cidx[c];
-            ckey[c] = sqrt_rn(canonical_sq(a.ref, a.ldr, r, a.query, a.ldq, q, a.dim));
+            ckey[c] = sqrt_rn(canonical_sq_rows(a.ref_pm + (long long)r * a.pm_ld,
+                                                a.query_pm + (long long)q * a.pm_ld, a.dim));
         }
         wave_sync();
         for (int c = lane; c < nc; c += kWave) {
@@ -242,9 +250,9 @@ __global__ __launch_bounds__(64) void exact_scan_kernel(ExactArgs a) {
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_norms(const float *p, long long ld, int n, int dim, float *out,
-                        unsigned *maxbits, hipStream_t s) {
+                        unsigned *maxbits, float *pm, int pm_ld, hipStream_t s) {
     hipLaunchKernelGGL(sq_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, ld, n, dim,
-                       out, maxbits);
+                       out, maxbits, pm, pm_ld);
     return hipGetLastError();
 }
 
