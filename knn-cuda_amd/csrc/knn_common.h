@@ -21,21 +21,40 @@ constexpr int kMaxCand = 1024;       // max candidates re-ranked exactly per que
 
 // Arguments of the FP32 MFMA scan (one launch covers all queries and slices).
 struct ScanArgs {
-    const float *ref;    // dpad rows of ldr floats (rows >= dim are zero)
-    long long ldr;
+    const float *packed;  // packed reference tiles (see knn_scan.hip)
+    int oct;              // dimension octets per point in the packed layout
     int R;
-    const float *query;  // dpad rows of ldq floats (rows >= dim are zero)
+    int dim;
+    const float *query;   // dim rows of ldq floats
     long long ldq;
     int Q;
-    const float *rnorm;  // R squared norms (FP32)
-    int slice_len;       // references per slice, multiple of kRefsPerStep
+    const float *rnorm;   // R squared norms (FP32)
+    unsigned *theta;      // Q shared thresholds (order-mapped float codes)
+    int slice_len;        // references per slice, multiple of kRefsPerStep
+    long long slice_span; // packed bytes of one slice (< 4 GiB)
+    int r_pad;            // references covered by the packed layout
     int nslices;
-    int kp;              // heap capacity per (query, slice)
-    int nqb;             // query blocks (waves_per_block * 32 queries each)
-    int nchunks;         // dpad / (2*KS) dimension chunks
-    int xcd_map;         // 1: keep each slice on one XCD (nslices % 8 == 0)
-    float *part_val;     // [nslices][Q][kp]
+    int kp;               // heap capacity per (query, slice)
+    int nqb;              // query blocks (waves_per_block * 32 queries each)
+    int nchunks;          // dimension chunks of 2*KS dims
+    int xcd_map;          // 1: keep each slice on one XCD (nslices % 8 == 0)
+    float *part_val;      // [nslices][Q][kp]
     int *part_idx;
+};
+
+// Arguments of the norms / layout pass.
+struct NormArgs {
+    const float *p;      // dim rows of ld floats
+    long long ld;
+    int n;
+    int n_pad;           // points covered (packed tail tiles are zeroed)
+    int dim;
+    int oct;             // dimension octets written
+    float *out;          // n squared norms
+    unsigned *maxbits;   // max norm bits, or nullptr
+    float *pm;           // point-major copy (row stride pm_ld), or nullptr
+    int pm_ld;
+    float *packed;       // packed tile layout, or nullptr
 };
 
 // Arguments of the select / exact re-rank / certificate stage.
@@ -50,7 +69,6 @@ struct FinalArgs {
     int k;
     int kp;
     int nslices;
-    int slice_len;            // references per slice (<= kp: no heap ever rejects)
     int maxc;                 // candidate capacity per query (<= kMaxCand)
     const float *part_val;
     const int *part_idx;
@@ -59,6 +77,7 @@ struct FinalArgs {
     int pm_ld;                // multiple of 4 (16-byte rows)
     const float *qnorm;       // Q squared norms (FP32)
     const unsigned *rmax;     // bit pattern of max_r ||r||^2 (non-negative float)
+    const unsigned *theta;    // final shared thresholds of the scan (order-mapped)
     int force_fallback;
     float *out_dist;          // k x Q
     int *out_idx;

@@ -26,8 +26,7 @@ int scan_pick_ks(int dim);
 size_t scan_lds_bytes(int kp, int waves);
 int scan_blocks_per_cu(int ks, bool multi, int waves, int kp);
 hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s);
-hipError_t launch_norms(const float *p, long long ld, int n, int dim, float *out,
-                        unsigned *maxbits, float *pm, int pm_ld, hipStream_t s);
+hipError_t launch_norms(const NormArgs &a, hipStream_t s);
 hipError_t launch_select(const FinalArgs &a, hipStream_t s);
 hipError_t launch_exact(const ExactArgs &a, int grid, hipStream_t s);
 hipError_t launch_gen_uniform(float *out, int n, int dim, uint64_t seed, float scale,
@@ -60,7 +59,7 @@ void set_error(const char *fmt, ...) {
 // Grow-only device buffers, one per role.
 enum Slot {
     kRefPad, kQryPad, kRNorm, kQNorm, kRMax, kPartVal, kPartIdx, kFbCount, kFbList,
-    kFbKey, kFbIdx, kOutDist, kOutIdx, kRefPm, kQryPm, kNumSlots
+    kFbKey, kFbIdx, kOutDist, kOutIdx, kRefPm, kQryPm, kPacked, kTheta, kNumSlots
 };
 struct Workspace {
     void *ptr[kNumSlots] = {};
@@ -105,6 +104,9 @@ struct Plan {
     int maxc = 0;
     int blocks_per_cu = 0;
     int cus = 0;
+    int r_pad = 0;            // references rounded up to whole scan steps
+    int oct = 0;              // dimension octets of the packed layout
+    long long slice_span = 0; // packed bytes per slice
 };
 
 // Cost model of one slice count: the scan runs in `rounds` waves of
@@ -116,12 +118,7 @@ double slice_cost(int S, int steps, int nqb, int resident) {
     return (double)rounds * ((double)((steps + S - 1) / S) + 2.0);
 }
 
-// Bytes one dimension chunk of the reference matrix spans (the scan
-// addresses a chunk through one buffer resource with 32-bit offsets).
-size_t chunk_span(int ks, long long ldr) { return (size_t)2 * ks * ldr * sizeof(float); }
-constexpr size_t kMaxSpan = 0xFFFFFFFFull;
-
-Plan make_plan(int R, int Q, int dim, int k, long long ldr, const knnhip_options &o) {
+Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     Plan p;
     const int margin = o.margin > 0 ? o.margin : kDefaultMargin;
     p.kp = std::min(k + margin, R);
@@ -131,24 +128,29 @@ Plan make_plan(int R, int Q, int dim, int k, long long ldr, const knnhip_options
     if (per_wave > kScanLdsMax) return p;  // k too large for the scan's heaps
     p.waves = (4 * per_wave <= 64 * 1024) ? 4 : (2 * per_wave <= 80 * 1024) ? 2 : 1;
     p.ks = scan_pick_ks(dim);
-    if (p.ks && chunk_span(p.ks, ldr) < kMaxSpan) {
-        p.dpad = 2 * p.ks;
+    if (p.ks) {
         p.nchunks = 1;
         p.multi = false;
     } else {
+        p.ks = 64;
         p.multi = true;
-        p.ks = chunk_span(64, ldr) < kMaxSpan ? 64 : 16;
-        if (chunk_span(p.ks, ldr) >= kMaxSpan) return p;  // row pitch too wide for the scan
         p.nchunks = (dim + 2 * p.ks - 1) / (2 * p.ks);
-        p.dpad = 2 * p.ks * p.nchunks;
     }
+    p.dpad = 2 * p.ks * p.nchunks;
+    p.oct = p.dpad / 8;
     const int qpb = p.waves * kQueriesPerWave;
     p.nqb = (Q + qpb - 1) / qpb;
     // Slices: the count whose blocks fill whole rounds of resident blocks
     // (cost model above), bounded by the merge capacity of the select stage
     // and by one step per slice.
     const int steps = (R + kRefsPerStep - 1) / kRefsPerStep;
+    p.r_pad = steps * kRefsPerStep;
+    // A slice's packed tiles are addressed with 32-bit offsets.
+    const long long step_bytes = (long long)kRefsPerStep * p.dpad * sizeof(float);
+    const long long max_steps_per_slice = 0xFFFFFFFFll / step_bytes;
+    const int smin = (int)((steps + max_steps_per_slice - 1) / max_steps_per_slice);
     const int smax = std::max(1, std::min(kMaxUnion / p.kp, steps));
+    if (smin > smax) return p;  // too large for one scan launch
     if (g_cu_count <= 0) {
         int dev = 0, n = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -160,12 +162,12 @@ Plan make_plan(int R, int Q, int dim, int k, long long ldr, const knnhip_options
     p.cus = g_cu_count > 0 ? g_cu_count : 256;
     p.blocks_per_cu = std::max(1, scan_blocks_per_cu(p.ks, p.multi, p.waves, p.kp));
     const int resident = p.cus * p.blocks_per_cu;
-    int s = 1;
+    int s = smin;
     if (o.slices > 0) {
-        s = std::min(o.slices, smax);
+        s = std::max(smin, std::min(o.slices, smax));
     } else {
-        double best = slice_cost(1, steps, p.nqb, resident);
-        for (int c = 2; c <= smax; ++c) {
+        double best = slice_cost(smin, steps, p.nqb, resident);
+        for (int c = smin + 1; c <= smax; ++c) {
             const double cost = slice_cost(c, steps, p.nqb, resident);
             if (cost < best) {
                 best = cost;
@@ -175,6 +177,7 @@ Plan make_plan(int R, int Q, int dim, int k, long long ldr, const knnhip_options
     }
     const int steps_per_slice = (steps + s - 1) / s;
     p.slice_len = steps_per_slice * kRefsPerStep;
+    p.slice_span = (long long)steps_per_slice * step_bytes;
     p.nslices = s;
     p.xcd = (s % 8 == 0) ? 1 : 0;
     p.maxc = std::min(kMaxCand, std::max(2 * p.kp, 256));
@@ -235,8 +238,7 @@ bool ensure_device() {
     return true;
 }
 
-// Core: device pointers; ref/query rows >= dim up to plan.dpad must be zero
-// when the scan is used (the caller of run_device guarantees it).
+// Core: device pointers, dim rows of ldr / ldq floats.
 bool run_device(const float *ref, long long ldr, int R, const float *query, long long ldq, int Q,
                 int dim, int k, float *out_dist, int *out_idx, const Plan &p,
                 const knnhip_options &o, hipStream_t s) {
@@ -287,21 +289,51 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         const int pm_ld = (dim + 3) & ~3;
         float *ref_pm = (float *)ws_get(kRefPm, (size_t)R * pm_ld * sizeof(float));
         float *qry_pm = (float *)ws_get(kQryPm, (size_t)Q * pm_ld * sizeof(float));
-        if (!rnorm || !qnorm || !rmax || !pval || !pidx || !fb_list || !ref_pm || !qry_pm) {
+        float *packed = (float *)ws_get(kPacked, (size_t)p.r_pad * p.dpad * sizeof(float));
+        unsigned *theta = (unsigned *)ws_get(kTheta, (size_t)Q * sizeof(unsigned));
+        if (!rnorm || !qnorm || !rmax || !pval || !pidx || !fb_list || !ref_pm || !qry_pm ||
+            !packed || !theta) {
             set_error("Memory allocation error");
             return false;
         }
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[0], s), "event");
         KNN_TRY(hipMemsetAsync(rmax, 0, sizeof(unsigned), s), "memset");
-        KNN_TRY(launch_norms(ref, ldr, R, dim, rnorm, rmax, ref_pm, pm_ld, s), "norm kernel launch");
-        KNN_TRY(launch_norms(query, ldq, Q, dim, qnorm, nullptr, qry_pm, pm_ld, s),
-                "norm kernel launch");
+        KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)theta, 0xFF800000u /* +inf, order-mapped */, Q, s),
+                "memset");
+        NormArgs na{};
+        na.p = ref;
+        na.ld = ldr;
+        na.n = R;
+        na.n_pad = p.r_pad;
+        na.dim = dim;
+        na.oct = p.oct;
+        na.out = rnorm;
+        na.maxbits = rmax;
+        na.pm = ref_pm;
+        na.pm_ld = pm_ld;
+        na.packed = packed;
+        KNN_TRY(launch_norms(na, s), "norm kernel launch");
+        NormArgs nq{};
+        nq.p = query;
+        nq.ld = ldq;
+        nq.n = Q;
+        nq.n_pad = Q;
+        nq.dim = dim;
+        nq.oct = (dim + 7) / 8;
+        nq.out = qnorm;
+        nq.pm = qry_pm;
+        nq.pm_ld = pm_ld;
+        KNN_TRY(launch_norms(nq, s), "norm kernel launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[1], s), "event");
 
         ScanArgs sa{};
-        sa.ref = ref;
-        sa.ldr = ldr;
+        sa.packed = packed;
+        sa.oct = p.oct;
         sa.R = R;
+        sa.dim = dim;
+        sa.theta = theta;
+        sa.slice_span = p.slice_span;
+        sa.r_pad = p.r_pad;
         sa.query = query;
         sa.ldq = ldq;
         sa.Q = Q;
@@ -328,7 +360,6 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         fa.k = k;
         fa.kp = p.kp;
         fa.nslices = p.nslices;
-        fa.slice_len = p.slice_len;
         fa.maxc = p.maxc;
         fa.part_val = pval;
         fa.part_idx = pidx;
@@ -337,6 +368,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         fa.pm_ld = pm_ld;
         fa.qnorm = qnorm;
         fa.rmax = rmax;
+        fa.theta = theta;
         fa.force_fallback = o.force_fallback;
         fa.out_dist = out_dist;
         fa.out_idx = out_idx;
@@ -381,11 +413,10 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
     if (!check_args(ref, R, query, Q, dim, k, knn_dist, knn_index)) return false;
     if (!ensure_device()) return false;
     const knnhip_options o = default_options(variant);
-    const Plan p = make_plan(R, Q, dim, k, R, o);
+    const Plan p = make_plan(R, Q, dim, k, o);
     g_last_plan = plan_json(p, R, Q, dim, k);
-    const int rows = p.use_scan ? p.dpad : dim;
-    float *ref_d = (float *)ws_get(kRefPad, (size_t)rows * R * sizeof(float));
-    float *qry_d = (float *)ws_get(kQryPad, (size_t)rows * Q * sizeof(float));
+    float *ref_d = (float *)ws_get(kRefPad, (size_t)dim * R * sizeof(float));
+    float *qry_d = (float *)ws_get(kQryPad, (size_t)dim * Q * sizeof(float));
     float *dist_d = (float *)ws_get(kOutDist, (size_t)k * Q * sizeof(float));
     int *idx_d = (int *)ws_get(kOutIdx, (size_t)k * Q * sizeof(int));
     if (!ref_d || !qry_d || !dist_d || !idx_d) {
@@ -398,14 +429,6 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
     KNN_TRY(hipMemcpyAsync(qry_d, query, (size_t)dim * Q * sizeof(float), hipMemcpyHostToDevice,
                            s),
             "Unable to copy data from host to device");
-    if (rows > dim) {
-        KNN_TRY(hipMemsetAsync(ref_d + (size_t)dim * R, 0, (size_t)(rows - dim) * R * sizeof(float),
-                               s),
-                "memset");
-        KNN_TRY(hipMemsetAsync(qry_d + (size_t)dim * Q, 0, (size_t)(rows - dim) * Q * sizeof(float),
-                               s),
-                "memset");
-    }
     if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s)) return false;
     KNN_TRY(hipMemcpyAsync(knn_dist, dist_d, (size_t)k * Q * sizeof(float), hipMemcpyDeviceToHost,
                            s),
@@ -475,36 +498,9 @@ int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const f
     if (!ensure_device()) return 0;
     knnhip_options o = opt ? *opt : default_options(0);
     hipStream_t s = (hipStream_t)stream;
-    Plan p = make_plan(ref_nb, query_nb, dim, k, ref_ld, o);
-    if (p.use_scan && p.dpad != dim) p = make_plan(ref_nb, query_nb, dim, k, ref_nb, o);
+    const Plan p = make_plan(ref_nb, query_nb, dim, k, o);
     g_last_plan = plan_json(p, ref_nb, query_nb, dim, k);
-    const float *ref_use = ref, *qry_use = query;
-    long long ldr = ref_ld, ldq = query_ld;
-    if (p.use_scan && p.dpad != dim) {
-        // The scan reads whole k-steps: stage zero-padded copies.
-        float *rp = (float *)ws_get(kRefPad, (size_t)p.dpad * ref_nb * sizeof(float));
-        float *qp = (float *)ws_get(kQryPad, (size_t)p.dpad * query_nb * sizeof(float));
-        if (!rp || !qp) {
-            set_error("Memory allocation error");
-            return 0;
-        }
-        if (hipMemsetAsync(rp, 0, (size_t)p.dpad * ref_nb * sizeof(float), s) != hipSuccess ||
-            hipMemsetAsync(qp, 0, (size_t)p.dpad * query_nb * sizeof(float), s) != hipSuccess ||
-            hipMemcpy2DAsync(rp, (size_t)ref_nb * sizeof(float), ref, (size_t)ref_ld * sizeof(float),
-                             (size_t)ref_nb * sizeof(float), dim, hipMemcpyDeviceToDevice,
-                             s) != hipSuccess ||
-            hipMemcpy2DAsync(qp, (size_t)query_nb * sizeof(float), query,
-                             (size_t)query_ld * sizeof(float), (size_t)query_nb * sizeof(float),
-                             dim, hipMemcpyDeviceToDevice, s) != hipSuccess) {
-            set_error("padding copy failed: %s", hipGetErrorString(hipGetLastError()));
-            return 0;
-        }
-        ref_use = rp;
-        qry_use = qp;
-        ldr = ref_nb;
-        ldq = query_nb;
-    }
-    return run_device(ref_use, ldr, ref_nb, qry_use, ldq, query_nb, dim, k, knn_dist, knn_index,
+    return run_device(ref, ref_ld, ref_nb, query, query_ld, query_nb, dim, k, knn_dist, knn_index,
                       p, o, s)
                ? 1
                : 0;

@@ -8,25 +8,32 @@
 //
 // Contraction: for reference r and query q the wave computes on MFMA
 //     key(r, q) = ||r||^2 - 2 r.q          (= ||r - q||^2 - ||q||^2)
-// with v_mfma_f32_32x32x2_f32: A = references (32 rows per tile, read
-// straight from the dim-major input, which is already the A-operand layout:
-// lanes 0-31 hold dimension 2k, lanes 32-63 dimension 2k+1 of 32 consecutive
-// points), B = -2 q resident in VGPRs for the whole scan, and one extra
+// with v_mfma_f32_32x32x2_f32.  A = references from the packed tile layout
+// (below), B = -2 q resident in VGPRs for the whole scan, and one extra
 // k-step that adds ||r||^2 (A = norm, B = 1).  The per-query constant
 // ||q||^2 does not change the order and is added back by the re-rank stage.
 //
+// Packed reference layout (written by the norms pass): for tile t of 32
+// points and dimension octet j, 64 lanes x 4 floats, lane m + 32h holding
+// point 32t+m, dimensions 8j+4h .. 8j+4h+3 (zero past dim and past R).  One
+// 16-byte load per lane feeds four consecutive k-steps: k-step 4j+u reads
+// dimension 8j + 4h + u on lane half h, and the B operand uses the same map.
+//
 // Selection: the accumulator holds 16 references of one query per lane.  A
-// key below the query's current threshold theta (the heap root) is a
-// candidate; the common case is one compare per key and a wave ballot.
-// Candidates replace the heap root and sift down.  The two lane halves hold
-// the same 32 queries, so they take turns and hand theta across with a
-// shuffle.  Invariant used by the certificate: every reference not left in a
-// heap has key >= that heap's final root.
+// key below the query's threshold theta is a candidate; the common case is
+// one compare per key and a wave ballot.  Candidates replace the root of the
+// query's max-heap (kp entries, LDS) and sift down.  theta is the smaller of
+// the heap root and the query's shared threshold, the minimum root published
+// by every slice scanning that query (an agent-scope atomic min), so slices
+// prune with each other's progress.  Invariant used by the certificate:
+// every reference left out of the heaps has key >= the final shared
+// threshold, or >= the kp-th smallest key kept.
 #include "knn_common.h"
 
 namespace knnhip {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Row within a 32-row tile of accumulator element i on lane half h of the
 // 32x32 MFMA result (4 consecutive rows per group, groups 8 rows apart).
@@ -35,7 +42,7 @@ __device__ __forceinline__ constexpr int acc_row(int i, int h) {
 }
 
 // Max-heap of kp (key, index) pairs: replace the root by a smaller key and
-// restore the heap.  Returns the new root (the query's threshold).
+// restore the heap.  Returns the new root.
 __device__ __forceinline__ float heap_replace_top(float *hv, int *hi, int kp, float v, int id) {
     int i = 0;
     while (true) {
@@ -61,12 +68,15 @@ __device__ __forceinline__ float heap_replace_top(float *hv, int *hi, int kp, fl
 
 __host__ __device__ constexpr int heap_stride(int kp) { return kp | 1; }
 
-template <int KS, bool MULTI>
+// KS: k-steps per dimension chunk (B operands held in VGPRs), a multiple of 4.
+// NBUF: ring of one-octet (4 k-step) load buffers; loads run NBUF-1 octets
+// ahead of the MFMAs that consume them.
+template <int KS, int NBUF, bool MULTI>
 __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int CH = (KS > 64) ? 4 : 8;  // k-steps per prefetch group
-    static_assert(KS % CH == 0, "k-steps must be a multiple of the prefetch group");
-    constexpr int NG = KS / CH;
+    constexpr int OCT = KS / 4;  // octets per chunk
+    static_assert(KS % 4 == 0, "k-steps come in octets of 4");
+    static_assert(OCT % NBUF == 0, "ring position must repeat every chunk");
 
     const int waves = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
@@ -104,60 +114,63 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     wave_sync();
     float *myv = hv + col * kps;
     int *myi = hi + col * kps;
-    float theta = INFINITY;
+    float theta = INFINITY;      // threshold in use (<= own heap root)
+    float published = INFINITY;  // last value this wave pushed to the shared threshold
+    unsigned *theta_q = a.theta + q;
 
-    const long long ldr = a.ldr, ldq = a.ldq;
-    const float *__restrict__ qcol = a.query + (long long)h * ldq + q;
+    const long long ldq = a.ldq;
+    const float *__restrict__ qcol = a.query + q;
     float bq[KS];
-    if (!MULTI) {
+    auto load_b = [&](int c) {
 #pragma unroll
-        for (int kk = 0; kk < KS; ++kk) bq[kk] = -2.0f * qcol[(long long)(2 * kk) * ldq];
-    }
+        for (int kk = 0; kk < KS; ++kk) {
+            // Branch-free: clamp the row, then zero the padding dimensions.
+            const int d = c * 2 * KS + 8 * (kk >> 2) + 4 * h + (kk & 3);
+            const float v = qcol[(long long)min(d, a.dim - 1) * ldq];
+            bq[kk] = d < a.dim ? -2.0f * v : 0.0f;
+        }
+    };
+    if (!MULTI) load_b(0);
     const int nchunks = MULTI ? a.nchunks : 1;
 
     const int r_begin = slice * a.slice_len;
     const int r_end = min(a.R, r_begin + a.slice_len);
 
-    // Reference operands come through a raw buffer resource per dimension
-    // chunk: the lane's byte offset (point + lane-half row) stays in a VGPR,
-    // the k-step row offset is a scalar, so a load is one instruction.
-    const unsigned kstep_bytes = (unsigned)(2 * ldr * sizeof(float));
-    const unsigned span_bytes = (unsigned)min((long long)2 * KS * ldr * (long long)sizeof(float),
-                                              (long long)0xFFFFFFFFll);
-    auto lane_offsets = [&](int rs, unsigned *vo) {
+    // Packed operands through one buffer resource per slice; the lane's
+    // offset is a VGPR, the octet offset a scalar.  The resource covers only
+    // the slice's own tiles, so the prefetch of the step after the slice's
+    // last one (issued unconditionally, to keep the load counter static)
+    // falls outside it and returns zeros without touching memory.
+    const long long tile_bytes = (long long)a.oct * kWave * 16;
+    const long long valid_tiles =
+        max(0, min(a.slice_len, a.r_pad - r_begin)) / kRefsPerTile;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(a.packed) + (long long)(r_begin / kRefsPerTile) * a.oct * kWave * 4,
+        (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
+    auto lane_offsets = [&](int r_start, unsigned *vo) {
+        const int t0 = (r_start - r_begin) / kRefsPerTile;
 #pragma unroll
-        for (int mt = 0; mt < kMTiles; ++mt) {
-            const int r = min(rs + kRefsPerTile * mt + col, a.R - 1);
-            vo[mt] = (unsigned)(((long long)h * ldr + r) * (long long)sizeof(float));
-        }
+        for (int mt = 0; mt < kMTiles; ++mt)
+            vo[mt] = (unsigned)((t0 + mt) * tile_bytes + lane * 16);
     };
-    auto chunk_rsrc = [&](int c) {
-        return __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<float *>(a.ref + (long long)c * 2 * KS * ldr), (short)0, (int)span_bytes,
-            0x00020000);
-    };
-    float abuf[2][CH][kMTiles];
-    auto issue = [&](float (&buf)[CH][kMTiles], __amdgpu_buffer_rsrc_t rs, int g,
-                     const unsigned *vo) {
+    u32x4 abuf[NBUF][kMTiles];
+    auto issue = [&](u32x4(&buf)[kMTiles], int oct, const unsigned *vo) {
 #pragma unroll
-        for (int kk = 0; kk < CH; ++kk) {
-#pragma unroll
-            for (int mt = 0; mt < kMTiles; ++mt)
-                buf[kk][mt] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                    rs, (int)vo[mt], (int)((g * CH + kk) * kstep_bytes), 0));
-        }
+        for (int mt = 0; mt < kMTiles; ++mt)
+            buf[mt] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)vo[mt], oct * kWave * 16, 0);
     };
-    // With an even number of prefetch groups the next step's first group can
-    // be fetched while the current step's last group computes.
-    constexpr bool kCrossStep = !MULTI && (NG % 2 == 0);
 
     unsigned voff[kMTiles];
     lane_offsets(r_begin, voff);
-    if (kCrossStep && r_begin < r_end) issue(abuf[0], chunk_rsrc(0), 0, voff);
+    if (!MULTI && r_begin < r_end) {
+#pragma unroll
+        for (int o = 0; o < NBUF - 1; ++o) issue(abuf[o], o, voff);
+    }
 
     for (int r0 = r_begin; r0 < r_end; r0 += kRefsPerStep) {
         f32x16 acc[kMTiles];
         float nrm[kMTiles];
+        const unsigned tq = __hip_atomic_load(theta_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt) {
             acc[mt] = f32x16{};
@@ -168,26 +181,27 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         unsigned voff_next[kMTiles];
         lane_offsets(r_next, voff_next);
         for (int c = 0; c < nchunks; ++c) {
-            const __amdgpu_buffer_rsrc_t rs = chunk_rsrc(c);
             if (MULTI) {
+                load_b(c);
 #pragma unroll
-                for (int kk = 0; kk < KS; ++kk)
-                    bq[kk] = -2.0f * qcol[(long long)(c * 2 * KS + 2 * kk) * ldq];
+                for (int o = 0; o < NBUF - 1; ++o) issue(abuf[o], c * OCT + o, voff);
             }
-            if (!kCrossStep) issue(abuf[0], rs, 0, voff);
 #pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                if (g + 1 < NG)
-                    issue(abuf[(g + 1) & 1], rs, g + 1, voff);
-                else if (kCrossStep && r_next < r_end)
-                    issue(abuf[0], rs, 0, voff_next);
+            for (int o = 0; o < OCT; ++o) {
+                constexpr int ahead = NBUF - 1;
+                const int oi = o + ahead;
+                if (oi < OCT)
+                    issue(abuf[oi % NBUF], c * OCT + oi, voff);
+                else if (!MULTI)
+                    issue(abuf[oi % NBUF], oi - OCT, voff_next);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int kk = 0; kk < CH; ++kk) {
+                for (int u = 0; u < 4; ++u) {
 #pragma unroll
                     for (int mt = 0; mt < kMTiles; ++mt)
                         acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(
-                            abuf[g & 1][kk][mt], bq[g * CH + kk], acc[mt], 0, 0, 0);
+                            __uint_as_float(abuf[o % NBUF][mt][u]), bq[4 * o + u], acc[mt], 0, 0,
+                            0);
                 }
             }
         }
@@ -201,6 +215,7 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
 
         // Epilogue: one compare per key; the slow path runs only when some
         // lane of the wave holds a key below its query's threshold.
+        theta = fminf(theta, ordered_to_float(tq));
         bool cand = false;
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt)
@@ -221,13 +236,19 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
                             const float v = acc[mt][i];
                             if (v < theta) {
                                 const int r = r0 + kRefsPerTile * mt + acc_row(i, hh);
-                                if (r < r_end) theta = heap_replace_top(myv, myi, kp, v, r);
+                                if (r < r_end)
+                                    theta = fminf(theta, heap_replace_top(myv, myi, kp, v, r));
                             }
                         }
                     }
                 }
                 wave_sync();
                 theta = __shfl(theta, col + 32 * hh);
+            }
+            if (h == 0 && theta < published) {
+                published = theta;
+                __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
@@ -244,8 +265,8 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     }
 }
 
-// Dimension k-steps with an instantiated scan (padded dimension = 2*KS).
-static const int kScanKS[] = {8, 16, 24, 32, 48, 64, 96, 128};
+// Dimension k-steps with an instantiated single-chunk scan (padded dim = 2*KS).
+static const int kScanKS[] = {8, 16, 24, 32, 48, 64, 96};
 
 int scan_pick_ks(int dim) {
     for (int ks : kScanKS)
@@ -253,40 +274,27 @@ int scan_pick_ks(int dim) {
     return 0;  // dimension needs the chunked scan
 }
 
-template <int KS, bool MULTI>
-static hipError_t launch_scan_t(const ScanArgs &a, int waves, size_t lds, hipStream_t s) {
-    auto fn = scan_f32_kernel<KS, MULTI>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)fn,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    const int blocks = a.nqb * a.nslices;
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(waves * kWave), lds, s, a);
-    return hipGetLastError();
-}
-
 size_t scan_lds_bytes(int kp, int waves) {
     return (size_t)waves * kQueriesPerWave * heap_stride(kp) * (sizeof(float) + sizeof(int));
 }
 
+// Ring depth per instance: 4 octets when the chunk's octet count allows it.
+#define KNN_SCAN_INSTANCES(X) \
+    X(8, 2, false)            \
+    X(16, 4, false)           \
+    X(24, 2, false)           \
+    X(32, 4, false)           \
+    X(48, 4, false)           \
+    X(64, 4, false)           \
+    X(96, 2, false)           \
+    X(64, 4, true)
+
 static const void *scan_fn(int ks, bool multi) {
-    if (multi) {
-        if (ks == 64) return (const void *)scan_f32_kernel<64, true>;
-        if (ks == 16) return (const void *)scan_f32_kernel<16, true>;
-        return nullptr;
-    }
-    switch (ks) {
-        case 8: return (const void *)scan_f32_kernel<8, false>;
-        case 16: return (const void *)scan_f32_kernel<16, false>;
-        case 24: return (const void *)scan_f32_kernel<24, false>;
-        case 32: return (const void *)scan_f32_kernel<32, false>;
-        case 48: return (const void *)scan_f32_kernel<48, false>;
-        case 64: return (const void *)scan_f32_kernel<64, false>;
-        case 96: return (const void *)scan_f32_kernel<96, false>;
-        case 128: return (const void *)scan_f32_kernel<128, false>;
-        default: return nullptr;
-    }
+#define X(K, N, M) \
+    if (ks == K && multi == M) return (const void *)scan_f32_kernel<K, N, M>;
+    KNN_SCAN_INSTANCES(X)
+#undef X
+    return nullptr;
 }
 
 // Resident blocks per CU of one scan instance (VGPR and LDS limits).
@@ -306,23 +314,75 @@ int scan_blocks_per_cu(int ks, bool multi, int waves, int kp) {
 }
 
 hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s) {
+    const void *fn = scan_fn(ks, multi);
+    if (!fn) return hipErrorInvalidValue;
     const size_t lds = scan_lds_bytes(a.kp, waves);
-    if (multi) {
-        if (ks == 64) return launch_scan_t<64, true>(a, waves, lds, s);
-        if (ks == 16) return launch_scan_t<16, true>(a, waves, lds, s);
-        return hipErrorInvalidValue;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
     }
-    switch (ks) {
-        case 8: return launch_scan_t<8, false>(a, waves, lds, s);
-        case 16: return launch_scan_t<16, false>(a, waves, lds, s);
-        case 24: return launch_scan_t<24, false>(a, waves, lds, s);
-        case 32: return launch_scan_t<32, false>(a, waves, lds, s);
-        case 48: return launch_scan_t<48, false>(a, waves, lds, s);
-        case 64: return launch_scan_t<64, false>(a, waves, lds, s);
-        case 96: return launch_scan_t<96, false>(a, waves, lds, s);
-        case 128: return launch_scan_t<128, false>(a, waves, lds, s);
-        default: return hipErrorInvalidValue;
+    const int blocks = a.nqb * a.nslices;
+#define X(K, N, M)                                                                             \
+    if (ks == K && multi == M) {                                                               \
+        hipLaunchKernelGGL((scan_f32_kernel<K, N, M>), dim3(blocks), dim3(waves * kWave), lds, \
+                           s, a);                                                              \
+        return hipGetLastError();                                                              \
     }
+    KNN_SCAN_INSTANCES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Norms pass: squared norms, the maximum reference norm, the point-major copy
+// used by the exact re-rank, and (references only) the packed tile layout
+// read by the scan.  One thread per point; threads past n zero the packed
+// tail tiles.  Replaces code/knncuda.cu:217-227.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void norms_pack_kernel(NormArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in_pad = i < a.n_pad;  // no early return: the wave reduces below
+    const bool real = i < a.n;
+    float s = 0.0f;
+    const float *col = a.p + (real ? i : 0);
+    float *row = (a.pm && real) ? a.pm + (long long)i * a.pm_ld : nullptr;
+    const int tile = i / kRefsPerTile, m = i % kRefsPerTile;
+    for (int j = 0; in_pad && j < a.oct; ++j) {
+        float v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int d = 8 * j + t;
+            const float x = col[(long long)min(d, a.dim - 1) * a.ld];
+            v[t] = (real && d < a.dim) ? x : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) s = fmaf(v[t], v[t], s);
+        if (row) {
+            // pm_ld is a multiple of 4: whole 16-byte pieces
+            if (8 * j + 4 <= a.pm_ld)
+                *reinterpret_cast<float4 *>(row + 8 * j) = make_float4(v[0], v[1], v[2], v[3]);
+            if (8 * j + 8 <= a.pm_ld)
+                *reinterpret_cast<float4 *>(row + 8 * j + 4) = make_float4(v[4], v[5], v[6], v[7]);
+        }
+        if (a.packed) {
+            float *dst = a.packed + (((long long)tile * a.oct + j) * kWave + m) * 4;
+            *reinterpret_cast<float4 *>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+            *reinterpret_cast<float4 *>(dst + 32 * 4) = make_float4(v[4], v[5], v[6], v[7]);
+        }
+    }
+    if (real) a.out[i] = s;
+    if (a.maxbits) {
+        unsigned mbits = real ? __float_as_uint(s) : 0u;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1)
+            mbits = max(mbits, (unsigned)__shfl_xor((int)mbits, off));
+        if ((threadIdx.x & 63) == 0) atomicMax(a.maxbits, mbits);
+    }
+}
+
+hipError_t launch_norms(const NormArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(norms_pack_kernel, dim3((a.n_pad + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 }  // namespace knnhip

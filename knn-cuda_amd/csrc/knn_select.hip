@@ -25,47 +25,6 @@
 namespace knnhip {
 
 // ---------------------------------------------------------------------------
-// Squared norms of the columns of a dim-major point set (replaces
-// code/knncuda.cu:217-227).  Optionally the maximum over all points, as the
-// bit pattern of a non-negative float (integer order = float order).
-// ---------------------------------------------------------------------------
-// Also writes the point-major copy used by the exact re-rank's gathers
-// (point i at pm[i*pm_ld], pm_ld a multiple of 4).
-__global__ __launch_bounds__(256) void sq_norms_kernel(const float *__restrict__ p, long long ld,
-                                                        int n, int dim, float *__restrict__ out,
-                                                        unsigned *maxbits, float *__restrict__ pm,
-                                                        int pm_ld) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    float s = 0.0f;
-    if (i < n) {
-        const float *col = p + i;
-        float *row = pm + (long long)i * pm_ld;
-        int d = 0;
-        for (; d + 8 <= dim; d += 8) {
-            float v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = col[(long long)(d + j) * ld];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) s = fmaf(v[j], v[j], s);
-            *reinterpret_cast<float4 *>(row + d) = make_float4(v[0], v[1], v[2], v[3]);
-            *reinterpret_cast<float4 *>(row + d + 4) = make_float4(v[4], v[5], v[6], v[7]);
-        }
-        for (; d < dim; ++d) {
-            const float v = col[(long long)d * ld];
-            s = fmaf(v, v, s);
-            row[d] = v;
-        }
-        out[i] = s;
-    }
-    if (maxbits) {
-        unsigned m = __float_as_uint(s);
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
-        if ((threadIdx.x & 63) == 0) atomicMax(maxbits, m);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Select the kp smallest approximate keys of the union of the slice heaps,
 // re-rank them with the ground-truth FP32 distance, write the k best and
 // check the certificate.  One wave per query.
@@ -116,11 +75,10 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
     }
     const int N = wave_sum(nreal);
 
-    // 2. tau = kp-th smallest key (bisection on the order-mapped code).  With
-    //    fewer than kp real entries, or slices no longer than kp, no heap ever
-    //    rejected a reference and the union is the whole reference set.
+    // 2. cut = kp-th smallest key (bisection on the order-mapped code); with
+    //    fewer than kp real entries every real entry is a candidate.
     unsigned cut = 0xFFFFFFFEu;
-    const bool all_in = N < kp || a.slice_len <= kp;
+    const bool all_in = N < kp;
     if (!all_in) {
         unsigned lo = 0, hi = 0xFFFFFFFEu;
         while (lo < hi) {
@@ -175,8 +133,13 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
     // 5. Certificate.
     if (lane == 0) {
         bool ok = !overflow && !a.force_fallback;
-        if (ok && !all_in) {
-            const double tau = (double)ordered_to_float(cut);
+        // Every reference outside the candidates has key >= tau: entries of
+        // the union past the cut, and references the scan rejected against
+        // a threshold no smaller than the final shared one.
+        float tau_f = ordered_to_float(a.theta[q]);
+        if (!all_in) tau_f = fminf(tau_f, ordered_to_float(cut));
+        if (ok && tau_f < INFINITY) {
+            const double tau = (double)tau_f;
             const double g = 2.0 * (double)(a.dim + 4) * 5.9604644775390625e-8;
             const double nq = (double)a.qnorm[q];
             const double nr = (double)__uint_as_float(*a.rmax);
@@ -249,13 +212,6 @@ __global__ __launch_bounds__(64) void exact_scan_kernel(ExactArgs a) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_norms(const float *p, long long ld, int n, int dim, float *out,
-                        unsigned *maxbits, float *pm, int pm_ld, hipStream_t s) {
-    hipLaunchKernelGGL(sq_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, ld, n, dim,
-                       out, maxbits, pm, pm_ld);
-    return hipGetLastError();
-}
-
 size_t select_lds_bytes(int maxc, int waves) {
     return (size_t)waves * (maxc * 2 * sizeof(int) + sizeof(float)) + 16;
 }
