@@ -110,12 +110,12 @@ struct Plan {
 };
 
 // Cost model of one slice count: the scan runs in `rounds` waves of
-// resident blocks, each block streaming ceil(steps / S) reference steps plus
-// a warm-up of about two steps while its heaps fill.
+// resident blocks, each block's four waves sharing ceil(steps / S) reference
+// steps, plus a warm-up of about two steps while the heaps fill.
 double slice_cost(int S, int steps, int nqb, int resident) {
     const long long units = (long long)nqb * S;
     const long long rounds = (units + resident - 1) / resident;
-    return (double)rounds * ((double)((steps + S - 1) / S) + 2.0);
+    return (double)rounds * ((double)((steps + S - 1) / S) / 4.0 + 2.0);
 }
 
 Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
@@ -123,10 +123,10 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     const int margin = o.margin > 0 ? o.margin : kDefaultMargin;
     p.kp = std::min(k + margin, R);
     if (p.kp < k) p.kp = k;
-    // Heaps: waves * 32 queries * (kp | 1) * 8 bytes of LDS per block.
-    const size_t per_wave = scan_lds_bytes(p.kp, 1);
-    if (per_wave > kScanLdsMax) return p;  // k too large for the scan's heaps
-    p.waves = (4 * per_wave <= 64 * 1024) ? 4 : (2 * per_wave <= 80 * 1024) ? 2 : 1;
+    // Heaps: 32 queries * (kp | 1) * 8 bytes of LDS per block, shared by its
+    // waves.
+    if (scan_lds_bytes(p.kp, 4) > kScanLdsMax) return p;  // k too large for the scan's heaps
+    p.waves = 4;
     p.ks = scan_pick_ks(dim);
     if (p.ks) {
         p.nchunks = 1;
@@ -138,8 +138,7 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     }
     p.dpad = 2 * p.ks * p.nchunks;
     p.oct = p.dpad / 8;
-    const int qpb = p.waves * kQueriesPerWave;
-    p.nqb = (Q + qpb - 1) / qpb;
+    p.nqb = (Q + kQueriesPerWave - 1) / kQueriesPerWave;
     // Slices: the count whose blocks fill whole rounds of resident blocks
     // (cost model above), bounded by the merge capacity of the select stage
     // and by one step per slice.

@@ -68,9 +68,38 @@ __device__ __forceinline__ float heap_replace_top(float *hv, int *hi, int kp, fl
 
 __host__ __device__ constexpr int heap_stride(int kp) { return kp | 1; }
 
+// Heap root read that other waves of the block may be updating (an atomic
+// load, so the compiler re-reads it on every use).
+__device__ __forceinline__ float shared_root(float *hv) {
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<unsigned *>(hv), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+
+// Per-query LDS lock.  LDS instructions of one wave execute in order, so a
+// release store issued after the heap writes publishes them; the fences are
+// compiler-ordering points only (no hardware wait is needed at wavefront
+// scope).
+__device__ __forceinline__ bool try_lock(int *l) {
+    int expected = 0;
+    const bool ok = __hip_atomic_compare_exchange_strong(l, &expected, 1, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return ok;
+}
+__device__ __forceinline__ void unlock(int *l) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __hip_atomic_store(l, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // KS: k-steps per dimension chunk (B operands held in VGPRs), a multiple of 4.
 // NBUF: ring of one-octet (4 k-step) load buffers; loads run NBUF-1 octets
 // ahead of the MFMAs that consume them.
+//
+// Block = W waves on the same 32 queries; wave w takes the slice's reference
+// steps w, w+W, w+2W, ...  All W waves feed one heap per query in LDS,
+// guarded by a per-query LDS lock, so every query's threshold is the kp-th
+// smallest key over everything the block has scanned so far.
 template <int KS, int NBUF, bool MULTI>
 __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -96,26 +125,28 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         slice = b / a.nqb;
         qb = b % a.nqb;
     }
-    const int qbase = (qb * waves + wave) * kQueriesPerWave;
-    if (qbase >= a.Q) return;  // whole wave is past the last query
+    const int qbase = qb * kQueriesPerWave;
+    if (qbase >= a.Q) return;  // whole block is past the last query
     const int q = min(qbase + col, a.Q - 1);
 
-    // Heaps in LDS with an odd row stride, so the 32 lanes of a half walking
-    // their own heaps in step hit distinct banks.
+    // Shared heaps (odd row stride: the lanes of a half walk 32 heaps in
+    // step on distinct banks) and one lock word per query.
     const int kp = a.kp;
     const int kps = heap_stride(kp);
-    float *hv = reinterpret_cast<float *>(smem) + (size_t)wave * kQueriesPerWave * kps;
-    int *hi = reinterpret_cast<int *>(smem + (size_t)waves * kQueriesPerWave * kps * sizeof(float)) +
-              (size_t)wave * kQueriesPerWave * kps;
-    for (int e = lane; e < kQueriesPerWave * kps; e += kWave) {
+    float *hv = reinterpret_cast<float *>(smem);
+    int *hi = reinterpret_cast<int *>(smem + (size_t)kQueriesPerWave * kps * sizeof(float));
+    int *lock = hi + (size_t)kQueriesPerWave * kps;
+    for (int e = threadIdx.x; e < kQueriesPerWave * kps; e += blockDim.x) {
         hv[e] = INFINITY;
         hi[e] = -1;
     }
-    wave_sync();
+    if (threadIdx.x < kQueriesPerWave) lock[threadIdx.x] = 0;
+    __syncthreads();
     float *myv = hv + col * kps;
     int *myi = hi + col * kps;
-    float theta = INFINITY;      // threshold in use (<= own heap root)
-    float published = INFINITY;  // last value this wave pushed to the shared threshold
+    int *mylock = lock + col;
+    float theta = INFINITY;      // threshold in use (<= the shared heap root)
+    float published = INFINITY;  // last value pushed to the cross-slice threshold
     unsigned *theta_q = a.theta + q;
 
     const long long ldq = a.ldq;
@@ -135,23 +166,23 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
 
     const int r_begin = slice * a.slice_len;
     const int r_end = min(a.R, r_begin + a.slice_len);
+    const int wstride = waves * kRefsPerStep;  // a wave's distance between steps
 
     // Packed operands through one buffer resource per slice; the lane's
     // offset is a VGPR, the octet offset a scalar.  The resource covers only
-    // the slice's own tiles, so the prefetch of the step after the slice's
-    // last one (issued unconditionally, to keep the load counter static)
-    // falls outside it and returns zeros without touching memory.
+    // the slice's own tiles, so the prefetch past a wave's last step (issued
+    // unconditionally, to keep the load counter static) falls outside it and
+    // returns zeros without touching memory.
     const long long tile_bytes = (long long)a.oct * kWave * 16;
-    const long long valid_tiles =
-        max(0, min(a.slice_len, a.r_pad - r_begin)) / kRefsPerTile;
+    const long long valid_tiles = max(0, min(a.slice_len, a.r_pad - r_begin)) / kRefsPerTile;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float *>(a.packed) + (long long)(r_begin / kRefsPerTile) * a.oct * kWave * 4,
         (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
     auto lane_offsets = [&](int r_start, unsigned *vo) {
-        const int t0 = (r_start - r_begin) / kRefsPerTile;
+        const long long t0 = (r_start - r_begin) / kRefsPerTile;
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt)
-            vo[mt] = (unsigned)((t0 + mt) * tile_bytes + lane * 16);
+            vo[mt] = (unsigned)min((t0 + mt) * tile_bytes + lane * 16, 0xFFFFFFF0ll);
     };
     u32x4 abuf[NBUF][kMTiles];
     auto issue = [&](u32x4(&buf)[kMTiles], int oct, const unsigned *vo) {
@@ -160,14 +191,15 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
             buf[mt] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)vo[mt], oct * kWave * 16, 0);
     };
 
+    const int r_first = r_begin + wave * kRefsPerStep;
     unsigned voff[kMTiles];
-    lane_offsets(r_begin, voff);
-    if (!MULTI && r_begin < r_end) {
+    lane_offsets(r_first, voff);
+    if (!MULTI && r_first < r_end) {
 #pragma unroll
         for (int o = 0; o < NBUF - 1; ++o) issue(abuf[o], o, voff);
     }
 
-    for (int r0 = r_begin; r0 < r_end; r0 += kRefsPerStep) {
+    for (int r0 = r_first; r0 < r_end; r0 += wstride) {
         f32x16 acc[kMTiles];
         float nrm[kMTiles];
         const unsigned tq = __hip_atomic_load(theta_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -177,7 +209,7 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
             const int r = r0 + kRefsPerTile * mt + col;
             nrm[mt] = r < a.R ? a.rnorm[r] : INFINITY;
         }
-        const int r_next = r0 + kRefsPerStep;
+        const int r_next = r0 + wstride;
         unsigned voff_next[kMTiles];
         lane_offsets(r_next, voff_next);
         for (int c = 0; c < nchunks; ++c) {
@@ -213,9 +245,10 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt) voff[mt] = voff_next[mt];
 
-        // Epilogue: one compare per key; the slow path runs only when some
-        // lane of the wave holds a key below its query's threshold.
-        theta = fminf(theta, ordered_to_float(tq));
+        // Epilogue: one compare per key against the freshest threshold (the
+        // shared heap root and the cross-slice value); the slow path runs
+        // only when some lane holds a key below it.
+        theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
         bool cand = false;
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt)
@@ -223,28 +256,35 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
             for (int i = 0; i < 16; ++i) cand |= acc[mt][i] < theta;
         if (__ballot(cand)) {
 #pragma unroll
-            for (int hh = 0; hh < 2; ++hh) {
+            for (int mt = 0; mt < kMTiles; ++mt) {
+                bool cm = false;
 #pragma unroll
-                for (int mt = 0; mt < kMTiles; ++mt) {
-                    bool cm = false;
+                for (int i = 0; i < 16; ++i) cm |= acc[mt][i] < theta;
+                if (__ballot(cm) == 0) continue;
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) cm |= acc[mt][i] < theta;
-                    if (__ballot(cm && h == hh) == 0) continue;
-                    if (h == hh) {
-#pragma unroll
-                        for (int i = 0; i < 16; ++i) {
-                            const float v = acc[mt][i];
-                            if (v < theta) {
-                                const int r = r0 + kRefsPerTile * mt + acc_row(i, hh);
-                                if (r < r_end)
-                                    theta = fminf(theta, heap_replace_top(myv, myi, kp, v, r));
+                for (int i = 0; i < 16; ++i) {
+                    const float v = acc[mt][i];
+                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                    bool want = v < theta && r < r_end;
+                    // Lanes with a candidate take their query's lock in turn;
+                    // the loop ends when every lane has inserted or seen the
+                    // root drop below its key.  A lane releases its lock in
+                    // the iteration that took it.
+                    while (__ballot(want)) {
+                        if (want) {
+                            if (!(v < shared_root(myv))) {
+                                want = false;
+                            } else if (try_lock(mylock)) {
+                                if (v < myv[0]) heap_replace_top(myv, myi, kp, v, r);
+                                theta = fminf(theta, myv[0]);
+                                unlock(mylock);
+                                want = false;
                             }
                         }
                     }
                 }
-                wave_sync();
-                theta = __shfl(theta, col + 32 * hh);
             }
+            theta = fminf(theta, shared_root(myv));
             if (h == 0 && theta < published) {
                 published = theta;
                 __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
@@ -253,12 +293,12 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         }
     }
 
-    // The wave's 32 consecutive queries are one contiguous block of the
-    // [slice][query][kp] output.
-    wave_sync();
+    // All waves done: the block's 32 consecutive queries are one contiguous
+    // block of the [slice][query][kp] output.
+    __syncthreads();
     const long long obase = ((long long)slice * a.Q + qbase) * kp;
     const int nvalid = min(kQueriesPerWave, a.Q - qbase) * kp;
-    for (int e = lane; e < nvalid; e += kWave) {
+    for (int e = threadIdx.x; e < nvalid; e += blockDim.x) {
         const int row = e / kp, j = e - row * kp;
         a.part_val[obase + e] = hv[row * kps + j];
         a.part_idx[obase + e] = hi[row * kps + j];
@@ -275,7 +315,9 @@ int scan_pick_ks(int dim) {
 }
 
 size_t scan_lds_bytes(int kp, int waves) {
-    return (size_t)waves * kQueriesPerWave * heap_stride(kp) * (sizeof(float) + sizeof(int));
+    (void)waves;  // one heap per query, shared by the block's waves
+    return (size_t)kQueriesPerWave * heap_stride(kp) * (sizeof(float) + sizeof(int)) +
+           kQueriesPerWave * sizeof(int);
 }
 
 // Ring depth per instance: 4 octets when the chunk's octet count allows it.
