@@ -68,6 +68,15 @@ __device__ __forceinline__ float heap_replace_top(float *hv, int *hi, int kp, fl
 
 __host__ __device__ constexpr int heap_stride(int kp) { return kp | 1; }
 
+// LDS of one scan block: the shared heaps and locks, then one candidate
+// queue per wave.
+constexpr int kQueueCap = 768;  // >= 512 (one half tile of candidates) + slack
+constexpr size_t kQueueBytes = (size_t)kQueueCap * 3 * sizeof(float);
+__host__ __device__ constexpr size_t scan_heap_bytes(int kp) {
+    return (((size_t)kQueriesPerWave * heap_stride(kp) * 2 + kQueriesPerWave) * sizeof(float) + 15) &
+           ~(size_t)15;
+}
+
 // Heap root read that other waves of the block may be updating (an atomic
 // load, so the compiler re-reads it on every use).
 __device__ __forceinline__ float shared_root(float *hv) {
@@ -143,11 +152,50 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     if (threadIdx.x < kQueriesPerWave) lock[threadIdx.x] = 0;
     __syncthreads();
     float *myv = hv + col * kps;
-    int *myi = hi + col * kps;
-    int *mylock = lock + col;
     float theta = INFINITY;      // threshold in use (<= the shared heap root)
     float published = INFINITY;  // last value pushed to the cross-slice threshold
     unsigned *theta_q = a.theta + q;
+
+    // Per-wave candidate queue (key, reference, query column) in LDS, after
+    // the heaps and locks.
+    unsigned char *qbase_ptr = smem + scan_heap_bytes(kp) + (size_t)wave * kQueueBytes;
+    float *qv = reinterpret_cast<float *>(qbase_ptr);
+    int *qr = reinterpret_cast<int *>(qbase_ptr + kQueueCap * sizeof(float));
+    int *qc = reinterpret_cast<int *>(qbase_ptr + kQueueCap * 2 * sizeof(float));
+    int qcount = 0;  // wave-uniform
+    const unsigned long long lanes_below = (1ull << lane) - 1ull;
+    // Insert every queued candidate still below its query's heap root.  The
+    // wave takes 64 entries at a time, one per lane; entries of one query are
+    // serialised by the query's lock.  The retry loop is wave-uniform (its
+    // condition is a ballot) and each lane makes one attempt per iteration,
+    // releasing a lock in the iteration that took it: a per-lane spin loop
+    // would let the compiler move a winner's critical section past the loop
+    // while a losing lane of the same wave spins on that lock.
+    auto drain = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (int base = 0; base < qcount; base += kWave) {
+            const int j = base + lane;
+            bool pending = j < qcount;
+            const float v = pending ? qv[j] : 0.0f;
+            const int r = pending ? qr[j] : 0;
+            const int c = pending ? qc[j] : 0;
+            float *cv = hv + c * kps;
+            int *ci = hi + c * kps;
+            while (__ballot(pending)) {
+                if (pending) {
+                    if (!(v < shared_root(cv))) {
+                        pending = false;
+                    } else if (try_lock(lock + c)) {
+                        if (v < cv[0]) heap_replace_top(cv, ci, kp, v, r);
+                        unlock(lock + c);
+                        pending = false;
+                    }
+                }
+            }
+        }
+        qcount = 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
 
     const long long ldq = a.ldq;
     const float *__restrict__ qcol = a.query + q;
@@ -227,6 +275,8 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
                 else if (!MULTI)
                     issue(abuf[oi % NBUF], oi - OCT, voff_next);
                 __builtin_amdgcn_sched_barrier(0);
+                // One k-step across the four tiles per group: consecutive
+                // MFMAs always target different accumulators.
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
 #pragma unroll
@@ -234,6 +284,7 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
                         acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(
                             __uint_as_float(abuf[o % NBUF][mt][u]), bq[4 * o + u], acc[mt], 0, 0,
                             0);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
         }
@@ -246,8 +297,9 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         for (int mt = 0; mt < kMTiles; ++mt) voff[mt] = voff_next[mt];
 
         // Epilogue: one compare per key against the freshest threshold (the
-        // shared heap root and the cross-slice value); the slow path runs
-        // only when some lane holds a key below it.
+        // shared heap root and the cross-slice value).  Keys below it are
+        // appended to the wave's candidate queue; the queue is drained into
+        // the heaps by all lanes at once when it fills.
         theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
         bool cand = false;
 #pragma unroll
@@ -263,35 +315,41 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
                 if (__ballot(cm) == 0) continue;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    const float v = acc[mt][i];
-                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
-                    bool want = v < theta && r < r_end;
-                    // Lanes with a candidate take their query's lock in turn;
-                    // the loop ends when every lane has inserted or seen the
-                    // root drop below its key.  A lane releases its lock in
-                    // the iteration that took it.
-                    while (__ballot(want)) {
-                        if (want) {
-                            if (!(v < shared_root(myv))) {
-                                want = false;
-                            } else if (try_lock(mylock)) {
-                                if (v < myv[0]) heap_replace_top(myv, myi, kp, v, r);
-                                theta = fminf(theta, myv[0]);
-                                unlock(mylock);
-                                want = false;
-                            }
-                        }
+                    if ((i & 7) == 0 && qcount > kQueueCap - 8 * kWave) {
+                        // room for a half tile of candidates (8 keys x 64 lanes)
+                        drain();
+                        theta = fminf(theta, shared_root(myv));
                     }
+                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                    const bool take = acc[mt][i] < theta && r < r_end;
+                    const unsigned long long m = __ballot(take);
+                    if (m == 0) continue;
+                    const int n = __popcll(m);
+                    if (take) {
+                        const int pos = qcount + __popcll(m & lanes_below);
+                        qv[pos] = acc[mt][i];
+                        qr[pos] = r;
+                        qc[pos] = col;
+                    }
+                    qcount += n;
                 }
             }
-            theta = fminf(theta, shared_root(myv));
-            if (h == 0 && theta < published) {
-                published = theta;
-                __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (qcount >= kQueueCap / 2) {
+                drain();
+                theta = fminf(theta, shared_root(myv));
+                if (h == 0 && theta < published) {
+                    published = theta;
+                    __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
         }
     }
+    drain();
+    theta = fminf(theta, shared_root(myv));
+    if (h == 0 && theta < published)
+        __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
 
     // All waves done: the block's 32 consecutive queries are one contiguous
     // block of the [slice][query][kp] output.
@@ -315,9 +373,7 @@ int scan_pick_ks(int dim) {
 }
 
 size_t scan_lds_bytes(int kp, int waves) {
-    (void)waves;  // one heap per query, shared by the block's waves
-    return (size_t)kQueriesPerWave * heap_stride(kp) * (sizeof(float) + sizeof(int)) +
-           kQueriesPerWave * sizeof(int);
+    return scan_heap_bytes(kp) + (size_t)waves * kQueueBytes;
 }
 
 // Ring depth per instance: 4 octets when the chunk's octet count allows it.
