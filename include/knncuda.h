@@ -98,6 +98,27 @@ int knnhip_generate_mixture(float *out, int n, int dim, int ncomp, float sigma,
                             unsigned long long centre_seed, unsigned long long point_seed,
                             void *stream);
 
+/*
+ * Block index (router form).  No counterpart in the reference, which only
+ * has the brute-force scan (code/knncuda.h).  The index partitions the
+ * reference set into `nblocks` k-means blocks (`iters` Lloyd iterations) and
+ * keeps a block-contiguous copy; a search scans each query's `probes`
+ * highest-ranked blocks and skips the rest only where the triangle
+ * inequality proves they hold no neighbour, so results equal
+ * knn_cuda_global's exactly.  `scores` (query_nb x nblocks, row-major, device)
+ * ranks blocks per query (e.g. router outputs); NULL ranks by the blocks'
+ * distance lower bounds.  The index keeps a pointer to `ref` (the exact
+ * fallback reads it): the caller keeps that buffer alive and unchanged.
+ */
+typedef struct knnhip_index knnhip_index;
+int knnhip_index_build(const float *ref, int ref_nb, long long ref_ld, int dim, int nblocks,
+                       int iters, knnhip_index **out, void *stream);
+int knnhip_index_search(knnhip_index *index, const float *query, int query_nb, long long query_ld,
+                        int k, int probes, const float *scores, float *knn_dist, int *knn_index,
+                        const knnhip_options *opt, void *stream);
+void knnhip_index_free(knnhip_index *index);
+int knnhip_index_info(const knnhip_index *index, int *nblocks, int *n_pos, float *build_ms);
+
 /* Library version string. */
 const char *knnhip_version(void);
 

@@ -19,6 +19,15 @@ constexpr int kRefsPerStep = kRefsPerTile * kMTiles;
 constexpr int kMaxUnion = 2048;      // max slices*kp entries merged per query
 constexpr int kMaxCand = 1024;       // max candidates re-ranked exactly per query
 
+// One scan task: references [ref_begin, ref_end) (ref_begin a multiple of
+// 32 in the packed layout) against up to 32 queries qlist[qoff .. qoff+qcount).
+struct ScanTask {
+    int ref_begin;
+    int ref_end;
+    int qoff;
+    int qcount;
+};
+
 // Arguments of the FP32 MFMA scan (one launch covers all queries and slices).
 struct ScanArgs {
     const float *packed;  // packed reference tiles (see knn_scan.hip)
@@ -38,8 +47,11 @@ struct ScanArgs {
     int nqb;              // query blocks (waves_per_block * 32 queries each)
     int nchunks;          // dimension chunks of 2*KS dims
     int xcd_map;          // 1: keep each slice on one XCD (nslices % 8 == 0)
-    float *part_val;      // [nslices][Q][kp]
+    float *part_val;      // [row][kp]; plain searches: row = slice * nqb * 32 + query
     int *part_idx;
+    const ScanTask *tasks;  // explicit tasks (nullptr: the plain slice x query-block grid)
+    int ntasks;           // one block per task
+    const int *qlist;     // query ids referenced by the tasks
 };
 
 // Arguments of the norms / layout pass.
@@ -68,7 +80,10 @@ struct FinalArgs {
     int dim;
     int k;
     int kp;
-    int nslices;
+    int nrows;                // heap rows per query
+    long long row_stride;     // row of (query q, j) = j * row_stride + q when rows == nullptr
+    const int *rows;          // else rows[q * nrows + j] (negative: unused)
+    const int *perm;          // reference position -> caller's index (nullptr: identity)
     int maxc;                 // candidate capacity per query (<= kMaxCand)
     const float *part_val;
     const int *part_idx;
@@ -78,6 +93,8 @@ struct FinalArgs {
     const float *qnorm;       // Q squared norms (FP32)
     const unsigned *rmax;     // bit pattern of max_r ||r||^2 (non-negative float)
     const unsigned *theta;    // final shared thresholds of the scan (order-mapped)
+    const float *lb_rest;     // per query, a lower bound on the distance (not squared) of
+                              // every reference the scan skipped (nullptr: none skipped)
     int force_fallback;
     float *out_dist;          // k x Q
     int *out_idx;

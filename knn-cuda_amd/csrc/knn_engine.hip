@@ -18,6 +18,7 @@
 
 #include "../../include/knncuda.h"
 #include "knn_common.h"
+#include "knn_index.h"
 
 namespace knnhip {
 
@@ -44,6 +45,7 @@ std::mutex g_mu;
 std::string g_last_error;
 float g_stage_ms[4] = {-1.f, -1.f, -1.f, -1.f};
 hipStream_t g_last_stream = nullptr;
+const int *g_last_fb = nullptr;  // device counter of the last search's exact-scan queries
 std::string g_last_plan;
 int g_cu_count = 0;
 
@@ -281,7 +283,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         float *rnorm = (float *)ws_get(kRNorm, (size_t)R * sizeof(float));
         float *qnorm = (float *)ws_get(kQNorm, (size_t)Q * sizeof(float));
         unsigned *rmax = (unsigned *)ws_get(kRMax, sizeof(unsigned));
-        const size_t npart = (size_t)p.nslices * Q * p.kp;
+        const size_t npart = (size_t)p.nslices * p.nqb * kQueriesPerWave * p.kp;
         float *pval = (float *)ws_get(kPartVal, npart * sizeof(float));
         int *pidx = (int *)ws_get(kPartIdx, npart * sizeof(int));
         int *fb_list = (int *)ws_get(kFbList, (size_t)Q * sizeof(int));
@@ -358,7 +360,8 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         fa.dim = dim;
         fa.k = k;
         fa.kp = p.kp;
-        fa.nslices = p.nslices;
+        fa.nrows = p.nslices;
+        fa.row_stride = (long long)p.nqb * kQueriesPerWave;
         fa.maxc = p.maxc;
         fa.part_val = pval;
         fa.part_idx = pidx;
@@ -384,6 +387,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[4], s), "event");
     }
     g_last_stream = s;
+    g_last_fb = fb_count;
     if (timing) {
         KNN_TRY(hipEventSynchronize(g_ws.ev[4]), "event sync");
         if (p.use_scan) {
@@ -447,6 +451,351 @@ bool report(bool ok) {
     return ok;
 }
 
+
+// ---------------------------------------------------------------------------
+// Block index (router form)
+// ---------------------------------------------------------------------------
+
+}  // namespace
+
+// Owned device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    bool alloc(size_t bytes) {
+        release();
+        if (bytes == 0) bytes = 256;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return false;
+        }
+        n = bytes;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+}  // namespace knnhip
+
+// The opaque index handle of the C API.
+struct knnhip_index {
+    int R = 0, dim = 0, nb = 0, pm_ld = 0, oct = 0, dpad = 0, ks = 0, nchunks = 1;
+    bool multi = false;
+    int n_pos = 0, r_pad = 0;
+    const float *ref = nullptr;  // caller's dim-major references (exact fallback)
+    long long ldr = 0;
+    float build_ms = 0.f;
+    knnhip::DevBuf packed, pm, norms, perm, block_start, block_count, cent, radius, rmax;
+    // last search statistics
+    int last_probes = 0;
+    long long last_tasks = 0;
+    float last_ms[6] = {-1, -1, -1, -1, -1, -1};
+};
+
+namespace knnhip {
+namespace {
+
+bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int dim, int nb,
+                 int iters, hipStream_t s) {
+    hipEvent_t e0, e1;
+    KNN_TRY(hipEventCreate(&e0), "event");
+    KNN_TRY(hipEventCreate(&e1), "event");
+    KNN_TRY(hipEventRecord(e0, s), "event");
+    ix->R = R;
+    ix->dim = dim;
+    ix->nb = nb;
+    ix->ref = ref;
+    ix->ldr = ldr;
+    ix->pm_ld = (dim + 3) & ~3;
+    // scan shape (same choice as the plain path)
+    knnhip_options o;
+    memset(&o, 0, sizeof(o));
+    {
+        Plan p = make_plan(R, 32, dim, 1, o);
+        if (!p.use_scan) {
+            set_error("dimension too large for the block index");
+            return false;
+        }
+        ix->ks = p.ks;
+        ix->multi = p.multi;
+        ix->nchunks = p.nchunks;
+        ix->dpad = p.dpad;
+        ix->oct = p.oct;
+    }
+    const int pm_ld = ix->pm_ld;
+    DevBuf pm0, norms0, cent_dm, assign, adist, counts, start, cursor, perm0;
+    if (!pm0.alloc((size_t)R * pm_ld * 4) || !norms0.alloc((size_t)R * 4) ||
+        !ix->rmax.alloc(4) || !ix->cent.alloc((size_t)nb * pm_ld * 4) ||
+        !cent_dm.alloc((size_t)nb * dim * 4) || !assign.alloc((size_t)R * 4) ||
+        !adist.alloc((size_t)R * 4) || !counts.alloc((size_t)nb * 4) ||
+        !start.alloc((size_t)(nb + 1) * 4) || !cursor.alloc((size_t)(nb + 1) * 4) ||
+        !perm0.alloc((size_t)R * 4) || !ix->block_start.alloc((size_t)(nb + 1) * 4) ||
+        !ix->block_count.alloc((size_t)nb * 4) || !ix->radius.alloc((size_t)nb * 8)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    // point-major copy in the caller's order, norms, max norm
+    KNN_TRY(hipMemsetAsync(ix->rmax.p, 0, 4, s), "memset");
+    NormArgs na{};
+    na.p = ref;
+    na.ld = ldr;
+    na.n = R;
+    na.n_pad = R;
+    na.dim = dim;
+    na.oct = (dim + 7) / 8;
+    na.out = norms0.as<float>();
+    na.maxbits = ix->rmax.as<unsigned>();
+    na.pm = pm0.as<float>();
+    na.pm_ld = pm_ld;
+    KNN_TRY(launch_norms(na, s), "norm kernel launch");
+    KNN_TRY(idx_init_centroids(pm0.as<float>(), pm_ld, R, dim, nb, ix->cent.as<float>(), pm_ld, s),
+            "k-means init");
+    knnhip_options ao;
+    memset(&ao, 0, sizeof(ao));
+    ao.margin = 8;
+    const Plan ap = make_plan(nb, R, dim, 1, ao);
+    for (int it = 0; it <= iters; ++it) {
+        // assignment: nearest centroid of every reference (the engine itself,
+        // centroids as the reference set, k = 1)
+        KNN_TRY(idx_transpose(ix->cent.as<float>(), pm_ld, nb, dim, cent_dm.as<float>(), s),
+                "transpose");
+        if (!run_device(cent_dm.as<float>(), nb, nb, ref, ldr, R, dim, 1, adist.as<float>(),
+                        assign.as<int>(), ap, ao, s))
+            return false;
+        KNN_TRY(hipMemsetAsync(counts.p, 0, (size_t)nb * 4, s), "memset");
+        KNN_TRY(idx_histogram(assign.as<int>(), R, counts.as<int>(), s), "histogram");
+        if (it == iters) break;
+        // update: members grouped by block, centroid = mean (empty blocks keep theirs)
+        KNN_TRY(idx_prefix(counts.as<int>(), nb, 1, start.as<int>(), nullptr, s), "prefix");
+        KNN_TRY(hipMemcpyAsync(cursor.p, start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+                "copy");
+        KNN_TRY(idx_scatter(assign.as<int>(), R, cursor.as<int>(), perm0.as<int>(), s), "scatter");
+        KNN_TRY(idx_centroids(pm0.as<float>(), pm_ld, dim, perm0.as<int>(), start.as<int>(),
+                              counts.as<int>(), nb, ix->cent.as<float>(), pm_ld, s),
+                "centroids");
+    }
+    // final layout: blocks contiguous, each starting at a multiple of 32
+    KNN_TRY(hipMemcpyAsync(ix->block_count.p, counts.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+            "copy");
+    KNN_TRY(idx_prefix(counts.as<int>(), nb, kRefsPerTile, ix->block_start.as<int>(), nullptr, s),
+            "prefix");
+    int n_pos = 0;
+    KNN_TRY(hipMemcpyAsync(&n_pos, ix->block_start.as<int>() + nb, 4, hipMemcpyDeviceToHost, s),
+            "copy");
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    ix->n_pos = n_pos;
+    ix->r_pad = (n_pos + kRefsPerStep - 1) / kRefsPerStep * kRefsPerStep;
+    if (!ix->perm.alloc((size_t)ix->r_pad * 4) ||
+        !ix->packed.alloc((size_t)ix->r_pad * ix->dpad * 4) ||
+        !ix->pm.alloc((size_t)ix->r_pad * pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)ix->perm.p, -1, ix->r_pad, s), "memset");
+    KNN_TRY(hipMemsetAsync(ix->pm.p, 0, (size_t)ix->r_pad * pm_ld * 4, s), "memset");
+    KNN_TRY(hipMemcpyAsync(cursor.p, ix->block_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+            "copy");
+    KNN_TRY(idx_scatter(assign.as<int>(), R, cursor.as<int>(), ix->perm.as<int>(), s), "scatter");
+    KNN_TRY(idx_gather_pack(pm0.as<float>(), pm_ld, dim, ix->oct, ix->perm.as<int>(), n_pos,
+                            ix->r_pad, ix->packed.as<float>(), ix->pm.as<float>(),
+                            ix->norms.as<float>(), s),
+            "gather");
+    KNN_TRY(idx_radius(ix->pm.as<float>(), pm_ld, dim, ix->block_start.as<int>(),
+                       ix->block_count.as<int>(), nb, ix->cent.as<float>(), pm_ld,
+                       ix->radius.as<double>(), s),
+            "radius");
+    KNN_TRY(hipEventRecord(e1, s), "event");
+    KNN_TRY(hipEventSynchronize(e1), "event");
+    KNN_TRY(hipEventElapsedTime(&ix->build_ms, e0, e1), "event");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return true;
+}
+
+// Workspace of index searches (grown on demand, kept between calls).
+struct IndexWork {
+    DevBuf qnorm, qpm, lb, score, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks, pval,
+        pidx, theta, fb_count, fb_list, fb_key, fb_idx;
+    bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
+};
+IndexWork g_iw;
+
+bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, int k, int P,
+                  const float *router_scores, float *out_dist, int *out_idx,
+                  const knnhip_options &o, hipStream_t s) {
+    const int nb = ix->nb, dim = ix->dim, pm_ld = ix->pm_ld;
+    const int margin = o.margin > 0 ? o.margin : kDefaultMargin;
+    const int kp = std::max(k, std::min(k + margin, ix->R));
+    if (scan_lds_bytes(kp, 4) > kScanLdsMax || (long long)P * kp > kMaxUnion) {
+        set_error("k or probe count too large for the block index (k=%d probes=%d)", k, P);
+        return false;
+    }
+    P = std::max(1, std::min(P, nb));
+    const long long npairs = (long long)Q * P;
+    const int max_tasks = (int)((npairs + kQueriesPerWave - 1) / kQueriesPerWave) + nb;
+    const int egrid = std::min(kExactGrid, Q);
+    IndexWork &w = g_iw;
+    if (!w.need(w.qnorm, (size_t)Q * 4) || !w.need(w.qpm, (size_t)Q * pm_ld * 4) ||
+        !w.need(w.lb, (size_t)Q * nb * 4) || !w.need(w.score, (size_t)Q * nb * 4) ||
+        !w.need(w.chosen, (size_t)npairs * 4) || !w.need(w.lb_rest, (size_t)Q * 4) ||
+        !w.need(w.pcount, (size_t)nb * 4) || !w.need(w.qstart, (size_t)(nb + 1) * 4) ||
+        !w.need(w.cursor, (size_t)(nb + 1) * 4) ||
+        !w.need(w.qlist, (size_t)max_tasks * kQueriesPerWave * 4) ||
+        !w.need(w.rows, (size_t)npairs * 4) || !w.need(w.tasks, (size_t)max_tasks * sizeof(ScanTask)) ||
+        !w.need(w.pval, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
+        !w.need(w.pidx, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
+        !w.need(w.theta, (size_t)Q * 4) || !w.need(w.fb_count, 4) || !w.need(w.fb_list, (size_t)Q * 4) ||
+        !w.need(w.fb_key, (size_t)egrid * kWave * k * 4) ||
+        !w.need(w.fb_idx, (size_t)egrid * kWave * k * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    hipEvent_t ev[6];
+    const bool timing = o.timing != 0;
+    if (timing)
+        for (auto &e : ev) KNN_TRY(hipEventCreate(&e), "event");
+    if (timing) KNN_TRY(hipEventRecord(ev[0], s), "event");
+    // queries: norms + point-major copy
+    NormArgs nq{};
+    nq.p = query;
+    nq.ld = ldq;
+    nq.n = Q;
+    nq.n_pad = Q;
+    nq.dim = dim;
+    nq.oct = (dim + 7) / 8;
+    nq.out = w.qnorm.as<float>();
+    nq.pm = w.qpm.as<float>();
+    nq.pm_ld = pm_ld;
+    KNN_TRY(launch_norms(nq, s), "norm kernel launch");
+    // block lower bounds and ranking
+    KNN_TRY(idx_lower_bounds(w.qpm.as<float>(), pm_ld, Q, dim, ix->cent.as<float>(), pm_ld, nb,
+                             ix->radius.as<double>(), w.lb.as<float>(), s),
+            "lower bounds");
+    const float *score = router_scores;
+    if (!score) {
+        KNN_TRY(idx_negate(w.lb.as<float>(), (long long)Q * nb, w.score.as<float>(), s), "negate");
+        score = w.score.as<float>();
+    }
+    KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
+                            w.chosen.as<int>(), w.lb_rest.as<float>(), s),
+            "pick blocks");
+    if (timing) KNN_TRY(hipEventRecord(ev[1], s), "event");
+    // (query, block) pairs -> per-block query lists -> scan tasks
+    KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nb * 4, s), "memset");
+    KNN_TRY(idx_pair_count(w.chosen.as<int>(), npairs, w.pcount.as<int>(), s), "pairs");
+    KNN_TRY(idx_prefix(w.pcount.as<int>(), nb, kQueriesPerWave, w.qstart.as<int>(), nullptr, s),
+            "prefix");
+    KNN_TRY(hipMemcpyAsync(w.cursor.p, w.qstart.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+            "copy");
+    KNN_TRY(idx_pair_scatter(w.chosen.as<int>(), npairs, P, w.cursor.as<int>(), w.qlist.as<int>(),
+                             w.rows.as<int>(), s),
+            "pairs");
+    KNN_TRY(idx_make_tasks(w.qstart.as<int>(), w.pcount.as<int>(), nb, ix->block_start.as<int>(),
+                           ix->block_count.as<int>(), w.tasks.as<ScanTask>(), max_tasks, s),
+            "tasks");
+    if (timing) KNN_TRY(hipEventRecord(ev[2], s), "event");
+    // scan the chosen blocks
+    KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Q, s), "memset");
+    KNN_TRY(hipMemsetAsync(w.fb_count.p, 0, 4, s), "memset");
+    ScanArgs sa{};
+    sa.packed = ix->packed.as<float>();
+    sa.oct = ix->oct;
+    sa.R = ix->n_pos;
+    sa.dim = dim;
+    sa.query = query;
+    sa.ldq = ldq;
+    sa.Q = Q;
+    sa.rnorm = ix->norms.as<float>();
+    sa.theta = w.theta.as<unsigned>();
+    sa.slice_len = ix->r_pad;
+    sa.slice_span = (long long)ix->r_pad * ix->dpad * 4;
+    sa.r_pad = ix->r_pad;
+    sa.nslices = 1;
+    sa.kp = kp;
+    sa.nqb = 1;
+    sa.nchunks = ix->nchunks;
+    sa.xcd_map = 0;
+    sa.part_val = w.pval.as<float>();
+    sa.part_idx = w.pidx.as<int>();
+    sa.tasks = w.tasks.as<ScanTask>();
+    sa.ntasks = max_tasks;
+    sa.qlist = w.qlist.as<int>();
+    KNN_TRY(launch_scan(sa, ix->ks, ix->multi, 4, s), "scan kernel launch");
+    if (timing) KNN_TRY(hipEventRecord(ev[3], s), "event");
+    // exact re-rank + certificate (skipped blocks bounded by lb_rest)
+    FinalArgs fa{};
+    fa.ref = ix->ref;
+    fa.ldr = ix->ldr;
+    fa.R = ix->R;
+    fa.query = query;
+    fa.ldq = ldq;
+    fa.Q = Q;
+    fa.dim = dim;
+    fa.k = k;
+    fa.kp = kp;
+    fa.nrows = P;
+    fa.rows = w.rows.as<int>();
+    fa.perm = ix->perm.as<int>();
+    fa.maxc = std::min(kMaxCand, std::max(2 * kp, 256));
+    fa.part_val = w.pval.as<float>();
+    fa.part_idx = w.pidx.as<int>();
+    fa.ref_pm = ix->pm.as<float>();
+    fa.query_pm = w.qpm.as<float>();
+    fa.pm_ld = pm_ld;
+    fa.qnorm = w.qnorm.as<float>();
+    fa.rmax = ix->rmax.as<unsigned>();
+    fa.theta = w.theta.as<unsigned>();
+    fa.lb_rest = w.lb_rest.as<float>();
+    fa.force_fallback = o.force_fallback;
+    fa.out_dist = out_dist;
+    fa.out_idx = out_idx;
+    fa.fb_count = w.fb_count.as<int>();
+    fa.fb_list = w.fb_list.as<int>();
+    KNN_TRY(launch_select(fa, s), "select kernel launch");
+    if (timing) KNN_TRY(hipEventRecord(ev[4], s), "event");
+    // queries whose certificate failed: exact full scan of the caller's references
+    ExactArgs ea{};
+    ea.ref = ix->ref;
+    ea.ldr = ix->ldr;
+    ea.R = ix->R;
+    ea.query = query;
+    ea.ldq = ldq;
+    ea.Q = Q;
+    ea.dim = dim;
+    ea.k = k;
+    ea.list = w.fb_list.as<int>();
+    ea.count = w.fb_count.as<int>();
+    ea.scratch_key = w.fb_key.as<float>();
+    ea.scratch_idx = w.fb_idx.as<int>();
+    ea.out_dist = out_dist;
+    ea.out_idx = out_idx;
+    KNN_TRY(launch_exact(ea, egrid, s), "exact scan launch");
+    ix->last_probes = P;
+    ix->last_tasks = max_tasks;
+    g_last_stream = s;
+    for (float &m : g_stage_ms) m = -1.f;
+    if (timing) {
+        KNN_TRY(hipEventRecord(ev[5], s), "event");
+        KNN_TRY(hipEventSynchronize(ev[5]), "event");
+        for (int i = 0; i < 5; ++i) KNN_TRY(hipEventElapsedTime(&ix->last_ms[i], ev[i], ev[i + 1]), "event");
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        // stage slots: 0 ranking (norms, bounds, pick + tasks), 1 scan, 2 select, 3 fallback
+        g_stage_ms[0] = ix->last_ms[0] + ix->last_ms[1];
+        g_stage_ms[1] = ix->last_ms[2];
+        g_stage_ms[2] = ix->last_ms[3];
+        g_stage_ms[3] = ix->last_ms[4];
+    }
+    g_last_fb = w.fb_count.as<int>();
+    return true;
+}
+
 }  // namespace
 }  // namespace knnhip
 
@@ -507,10 +856,10 @@ int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const f
 
 int knnhip_last_fallback_count(void) {
     std::lock_guard<std::mutex> lock(g_mu);
-    if (!g_ws.ptr[kFbCount]) return 0;
+    if (!g_last_fb) return 0;
     int n = -1;
     if (hipStreamSynchronize(g_last_stream) != hipSuccess ||
-        hipMemcpy(&n, g_ws.ptr[kFbCount], sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+        hipMemcpy(&n, g_last_fb, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("cannot read the fallback count");
         return -1;
     }
@@ -561,6 +910,66 @@ int knnhip_generate_mixture(float *out, int n, int dim, int ncomp, float sigma,
     return 1;
 }
 
-const char *knnhip_version(void) { return "knnhip 0.1 (gfx950)"; }
+int knnhip_index_build(const float *ref, int ref_nb, long long ref_ld, int dim, int nblocks,
+                       int iters, knnhip_index **out, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!out || !ref || ref_nb < 1 || dim < 1 || ref_ld < ref_nb || nblocks < 1 ||
+        nblocks > ref_nb || nblocks > 65536 || iters < 0) {
+        set_error("bad index arguments");
+        return 0;
+    }
+    *out = nullptr;
+    if (!ensure_device()) return 0;
+    knnhip_index *ix = new knnhip_index();
+    if (!index_build(ix, ref, ref_nb, ref_ld, dim, nblocks, iters, (hipStream_t)stream)) {
+        delete ix;
+        return 0;
+    }
+    *out = ix;
+    return 1;
+}
+
+int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long long query_ld,
+                        int k, int probes, const float *scores, float *knn_dist, int *knn_index,
+                        const knnhip_options *opt, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ix) {
+        set_error("null index");
+        return 0;
+    }
+    if (!check_args(ix->ref, ix->R, query, query_nb, ix->dim, k, knn_dist, knn_index)) return 0;
+    if (query_ld < query_nb || probes < 1) {
+        set_error("bad search arguments");
+        return 0;
+    }
+    if (!ensure_device()) return 0;
+    knnhip_options o = opt ? *opt : default_options(0);
+    char buf[256];
+    snprintf(buf, sizeof(buf),
+             "{\"index_blocks\": %d, \"probes\": %d, \"router\": %s, \"query_nb\": %d, \"k\": %d}",
+             ix->nb, std::min(probes, ix->nb), scores ? "true" : "false", query_nb, k);
+    g_last_plan = buf;
+    return index_search(ix, query, query_nb, query_ld, k, probes, scores, knn_dist, knn_index, o,
+                        (hipStream_t)stream)
+               ? 1
+               : 0;
+}
+
+void knnhip_index_free(knnhip_index *ix) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    delete ix;
+}
+
+int knnhip_index_info(const knnhip_index *ix, int *nblocks, int *n_pos, float *build_ms) {
+    if (!ix) return 0;
+    if (nblocks) *nblocks = ix->nb;
+    if (n_pos) *n_pos = ix->n_pos;
+    if (build_ms) *build_ms = ix->build_ms;
+    return 1;
+}
+
+const char *knnhip_version(void) { return "knnhip 0.2 (gfx950)"; }
 
 }  // extern "C"

@@ -122,21 +122,40 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     const int col = lane & 31;
     const int h = lane >> 5;
 
-    int slice, qb;
-    const int b = blockIdx.x;
-    if (a.xcd_map) {
-        // Consecutive block ids land on different XCDs; give every block of
-        // one slice the same XCD so the slice is read from HBM once per L2.
-        const int xcd = b & 7, j = b >> 3;
-        qb = j % a.nqb;
-        slice = (j / a.nqb) * 8 + xcd;
+    // One task per block: a reference range and up to 32 queries; its heaps
+    // leave as output rows task_row .. task_row+31.
+    const int t = blockIdx.x;
+    int r_begin, r_end, nq, task_row;
+    const int *qids = nullptr;  // query ids of the task (nullptr: consecutive from qfirst)
+    int qfirst = 0;
+    if (a.tasks) {
+        const ScanTask tk = a.tasks[t];
+        r_begin = tk.ref_begin;
+        r_end = tk.ref_end;
+        nq = tk.qcount;
+        qids = a.qlist + tk.qoff;
+        task_row = t * kQueriesPerWave;
     } else {
-        slice = b / a.nqb;
-        qb = b % a.nqb;
+        int slice, qb;
+        if (a.xcd_map) {
+            // Consecutive block ids land on different XCDs; give every block of
+            // one slice the same XCD so the slice is read from HBM once per L2.
+            const int xcd = t & 7, j = t >> 3;
+            qb = j % a.nqb;
+            slice = (j / a.nqb) * 8 + xcd;
+        } else {
+            slice = t / a.nqb;
+            qb = t % a.nqb;
+        }
+        qfirst = qb * kQueriesPerWave;
+        nq = min(kQueriesPerWave, a.Q - qfirst);
+        r_begin = slice * a.slice_len;
+        r_end = min(a.R, r_begin + a.slice_len);
+        task_row = (slice * a.nqb + qb) * kQueriesPerWave;
     }
-    const int qbase = qb * kQueriesPerWave;
-    if (qbase >= a.Q) return;  // whole block is past the last query
-    const int q = min(qbase + col, a.Q - 1);
+    if (nq <= 0) return;  // whole block is past the last query
+    const int qsel = col < nq ? col : 0;
+    const int q = qids ? qids[qsel] : qfirst + qsel;
 
     // Shared heaps (odd row stride: the lanes of a half walk 32 heaps in
     // step on distinct banks) and one lock word per query.
@@ -212,8 +231,6 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     if (!MULTI) load_b(0);
     const int nchunks = MULTI ? a.nchunks : 1;
 
-    const int r_begin = slice * a.slice_len;
-    const int r_end = min(a.R, r_begin + a.slice_len);
     const int wstride = waves * kRefsPerStep;  // a wave's distance between steps
 
     // Packed operands through one buffer resource per slice; the lane's
@@ -222,7 +239,8 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     // unconditionally, to keep the load counter static) falls outside it and
     // returns zeros without touching memory.
     const long long tile_bytes = (long long)a.oct * kWave * 16;
-    const long long valid_tiles = max(0, min(a.slice_len, a.r_pad - r_begin)) / kRefsPerTile;
+    const int span_refs = (r_end - r_begin + kRefsPerStep - 1) / kRefsPerStep * kRefsPerStep;
+    const long long valid_tiles = max(0, min(span_refs, a.r_pad - r_begin)) / kRefsPerTile;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float *>(a.packed) + (long long)(r_begin / kRefsPerTile) * a.oct * kWave * 4,
         (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
@@ -351,11 +369,11 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
 
-    // All waves done: the block's 32 consecutive queries are one contiguous
-    // block of the [slice][query][kp] output.
+    // All waves done: the task's heaps are rows task_row .. task_row+nq-1
+    // of the [row][kp] output.
     __syncthreads();
-    const long long obase = ((long long)slice * a.Q + qbase) * kp;
-    const int nvalid = min(kQueriesPerWave, a.Q - qbase) * kp;
+    const long long obase = (long long)task_row * kp;
+    const int nvalid = nq * kp;
     for (int e = threadIdx.x; e < nvalid; e += blockDim.x) {
         const int row = e / kp, j = e - row * kp;
         a.part_val[obase + e] = hv[row * kps + j];
@@ -385,7 +403,7 @@ size_t scan_lds_bytes(int kp, int waves) {
     X(48, 4, false)           \
     X(64, 4, false)           \
     X(96, 2, false)           \
-    X(64, 4, true)
+    X(64, 2, true)
 
 static const void *scan_fn(int ks, bool multi) {
 #define X(K, N, M) \
@@ -419,7 +437,7 @@ hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStre
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    const int blocks = a.nqb * a.nslices;
+    const int blocks = a.tasks ? a.ntasks : a.nqb * a.nslices;
 #define X(K, N, M)                                                                             \
     if (ks == K && multi == M) {                                                               \
         hipLaunchKernelGGL((scan_f32_kernel<K, N, M>), dim3(blocks), dim3(waves * kWave), lds, \

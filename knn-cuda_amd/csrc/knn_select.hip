@@ -45,15 +45,16 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
     if (q >= a.Q) return;
 
     const int maxc = a.maxc;
-    int *cidx = reinterpret_cast<int *>(smem) + (size_t)wave * maxc;
-    float *ckey = reinterpret_cast<float *>(smem + (size_t)waves * maxc * sizeof(int)) +
+    int *cidx = reinterpret_cast<int *>(smem) + (size_t)wave * maxc;      // reference position
+    int *corg = reinterpret_cast<int *>(smem) + (size_t)(waves + wave) * maxc;  // caller's index
+    float *ckey = reinterpret_cast<float *>(smem + (size_t)waves * maxc * 2 * sizeof(int)) +
                   (size_t)wave * maxc;
-    float *kth_slot = reinterpret_cast<float *>(smem + (size_t)waves * maxc * 2 * sizeof(int)) + wave;
+    float *kth_slot = reinterpret_cast<float *>(smem + (size_t)waves * maxc * 3 * sizeof(int)) + wave;
 
-    // 1. Gather the union of the slice heaps (key order-mapped; placeholders
+    // 1. Gather the union of the query's heaps (key order-mapped; placeholders
     //    of unfilled heaps map to the largest code and are never counted).
     const int kp = a.kp;
-    const int U = a.nslices * kp;
+    const int U = a.nrows * kp;
     unsigned code[NPL];
     int ids[NPL];
     int nreal = 0;
@@ -64,8 +65,10 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
         ids[j] = -1;
         if (e < U) {
             const int s = e / kp, t = e - s * kp;
-            const long long off = ((long long)s * a.Q + q) * kp + t;
-            const int id = a.part_idx[off];
+            const long long row = a.rows ? (long long)a.rows[(long long)q * a.nrows + s]
+                                         : s * a.row_stride + q;
+            const int id = row >= 0 ? a.part_idx[row * kp + t] : -1;
+            const long long off = row * kp + t;
             if (id >= 0) {
                 ids[j] = id;
                 code[j] = float_to_ordered(a.part_val[off]);
@@ -114,13 +117,14 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
             const int r = cidx[c];
             ckey[c] = sqrt_rn(canonical_sq_rows(a.ref_pm + (long long)r * a.pm_ld,
                                                 a.query_pm + (long long)q * a.pm_ld, a.dim));
+            corg[c] = a.perm ? a.perm[r] : r;
         }
         wave_sync();
         for (int c = lane; c < nc; c += kWave) {
             const float kc = ckey[c];
-            const int ic = cidx[c];
+            const int ic = corg[c];
             int rank = 0;
-            for (int j = 0; j < nc; ++j) rank += lex_less(ckey[j], cidx[j], kc, ic);
+            for (int j = 0; j < nc; ++j) rank += lex_less(ckey[j], corg[j], kc, ic);
             if (rank < a.k) {
                 a.out_dist[(long long)rank * a.Q + q] = kc;
                 a.out_idx[(long long)rank * a.Q + q] = ic;
@@ -132,7 +136,7 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
 
     // 5. Certificate.
     if (lane == 0) {
-        bool ok = !overflow && !a.force_fallback;
+        bool ok = !overflow && !a.force_fallback && nc >= a.k;
         // Every reference outside the candidates has key >= tau: entries of
         // the union past the cut, and references the scan rejected against
         // a threshold no smaller than the final shared one.
@@ -147,6 +151,15 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
                 (tau + nq * (1.0 - g) - g * (nq + 3.1 * nr) - (double)a.dim * 1e-36) * (1.0 - g);
             const double kth = (double)(*kth_slot) * (1.0 + 2.384185791015625e-7);
             ok = lower > kth * kth;
+        }
+        if (ok && a.lb_rest) {
+            // References of skipped blocks are at least lb_rest away (triangle
+            // inequality); their ground-truth sum is within the relative factor
+            // g of the true squared distance.
+            const double lb = (double)a.lb_rest[q];
+            const double g = 2.0 * (double)(a.dim + 4) * 5.9604644775390625e-8;
+            const double kth = (double)(*kth_slot) * (1.0 + 2.384185791015625e-7);
+            ok = lb * lb * (1.0 - g) > kth * kth;
         }
         if (!ok) {
             const int slot = atomicAdd(a.fb_count, 1);
@@ -213,7 +226,7 @@ __global__ __launch_bounds__(64) void exact_scan_kernel(ExactArgs a) {
 // Launchers
 // ---------------------------------------------------------------------------
 size_t select_lds_bytes(int maxc, int waves) {
-    return (size_t)waves * (maxc * 2 * sizeof(int) + sizeof(float)) + 16;
+    return (size_t)waves * (maxc * 3 * sizeof(int) + sizeof(float)) + 16;
 }
 
 template <int NPL>
@@ -231,7 +244,7 @@ static hipError_t launch_select_t(const FinalArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_select(const FinalArgs &a, hipStream_t s) {
-    const int U = a.nslices * a.kp;
+    const int U = a.nrows * a.kp;
     if (U <= 4 * kWave) return launch_select_t<4>(a, s);
     if (U <= 8 * kWave) return launch_select_t<8>(a, s);
     if (U <= 16 * kWave) return launch_select_t<16>(a, s);

@@ -39,6 +39,10 @@ C_SYMBOLS = (
     "knnhip_last_plan",
     "knnhip_generate_uniform",
     "knnhip_generate_mixture",
+    "knnhip_index_build",
+    "knnhip_index_search",
+    "knnhip_index_free",
+    "knnhip_index_info",
     "knnhip_version",
 )
 
@@ -101,6 +105,24 @@ def load() -> ctypes.CDLL:
         ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_void_p,
     ]
     lib.knnhip_generate_mixture.restype = ctypes.c_int
+    lib.knnhip_index_build.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+    ]
+    lib.knnhip_index_build.restype = ctypes.c_int
+    lib.knnhip_index_search.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
+        ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Options),
+        ctypes.c_void_p,
+    ]
+    lib.knnhip_index_search.restype = ctypes.c_int
+    lib.knnhip_index_free.argtypes = [ctypes.c_void_p]
+    lib.knnhip_index_free.restype = None
+    lib.knnhip_index_info.argtypes = [
+        ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+        ctypes.POINTER(ctypes.c_float),
+    ]
+    lib.knnhip_index_info.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -222,3 +244,81 @@ def generate_mixture(n: int, dim: int, ncomp: int, sigma: float, centre_seed: in
                                           point_seed, ctypes.c_void_p(stream)):
         raise KnnError(last_error())
     return out
+
+
+class BlockIndex:
+    """k-means block index over a device-resident reference set (router form).
+
+    Searches return exactly what ``search_device`` returns on the same data: blocks are skipped
+    only where the triangle inequality proves they hold no neighbour.  The index reads ``ref``
+    during exact fallbacks, so ``ref`` is kept alive here and must not be modified.
+    """
+
+    def __init__(self, ref, nblocks: int, iters: int = 8, stream: Optional[int] = None):
+        import torch
+
+        lib = load()
+        if ref.dtype != torch.float32 or not ref.is_cuda or ref.stride(1) != 1:
+            raise KnnError("ref must be a float32 device tensor with contiguous rows")
+        self.ref = ref
+        self.dim, self.ref_nb = ref.shape
+        self.nblocks = nblocks
+        handle = ctypes.c_void_p()
+        if stream is None:
+            stream = torch.cuda.current_stream(ref.device).cuda_stream
+        if not lib.knnhip_index_build(ref.data_ptr(), self.ref_nb, ref.stride(0), self.dim, nblocks,
+                                      iters, ctypes.byref(handle), ctypes.c_void_p(stream)):
+            raise KnnError(last_error())
+        self._h = handle
+
+    def info(self) -> dict:
+        nb = ctypes.c_int()
+        npos = ctypes.c_int()
+        ms = ctypes.c_float()
+        load().knnhip_index_info(self._h, ctypes.byref(nb), ctypes.byref(npos), ctypes.byref(ms))
+        return {"nblocks": nb.value, "positions": npos.value, "build_ms": ms.value}
+
+    def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
+               margin: int = 0, force_fallback: bool = False, timing: bool = False,
+               stream: Optional[int] = None):
+        import torch
+
+        lib = load()
+        if query.dtype != torch.float32 or not query.is_cuda or query.stride(1) != 1:
+            raise KnnError("query must be a float32 device tensor with contiguous rows")
+        dim, query_nb = query.shape
+        if dim != self.dim:
+            raise KnnError("query dim differs from the index")
+        if dist is None:
+            dist = torch.empty((k, query_nb), dtype=torch.float32, device=query.device)
+        if index is None:
+            index = torch.empty((k, query_nb), dtype=torch.int32, device=query.device)
+        sp = None
+        if scores is not None:
+            if scores.shape != (query_nb, self.nblocks) or scores.dtype != torch.float32:
+                raise KnnError("scores must be float32 (query_nb, nblocks)")
+            scores = scores.contiguous()
+            sp = scores.data_ptr()
+        opt = Options()
+        opt.margin = margin
+        opt.force_fallback = 1 if force_fallback else 0
+        opt.timing = 1 if timing else 0
+        if stream is None:
+            stream = torch.cuda.current_stream(query.device).cuda_stream
+        ok = lib.knnhip_index_search(self._h, query.data_ptr(), query_nb, query.stride(0), k, probes,
+                                     sp, dist.data_ptr(), index.data_ptr(), ctypes.byref(opt),
+                                     ctypes.c_void_p(stream))
+        if not ok:
+            raise KnnError(last_error())
+        return dist, index
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().knnhip_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,102 @@
+"""GPU parity tests of the block index (router form): every search equals the oracle bit for bit,
+whatever the probe count, ranking or data (clustered, uniform, ties)."""
+import numpy as np
+import pytest
+
+import knnhip
+import oracle
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def check(d, i, gd, gi, what):
+    d, i = d.cpu().numpy(), i.cpu().numpy()
+    assert np.array_equal(i, gi), f"{what}: {int((i != gi).sum())} index mismatches"
+    assert np.array_equal(bits(d), bits(gd)), f"{what}: distance mismatches"
+
+
+def say(m):
+    print(m, flush=True)
+
+
+@pytest.fixture(scope="module")
+def clustered():
+    R, Q, D, k = 20000, 600, 32, 8
+    ref = knnhip.generate_mixture(R, D, 64, 0.05, 31, 32)
+    qry = knnhip.generate_mixture(Q, D, 64, 0.05, 31, 33)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    return ref, qry, k, gd, gi
+
+
+@pytest.mark.parametrize("probes", [1, 2, 4, 16])
+def test_clustered_probes(clustered, probes):
+    ref, qry, k, gd, gi = clustered
+    say(f"index clustered probes={probes}")
+    ix = knnhip.BlockIndex(ref, nblocks=64, iters=6)
+    d, i = ix.search(qry, k, probes=probes)
+    torch.cuda.synchronize()
+    say(f"fallbacks {knnhip.last_fallback_count()} / {qry.shape[1]}")
+    check(d, i, gd, gi, f"probes={probes}")
+    ix.close()
+
+
+def test_clustered_pruning_is_effective(clustered):
+    ref, qry, k, gd, gi = clustered
+    ix = knnhip.BlockIndex(ref, nblocks=64, iters=6)
+    d, i = ix.search(qry, k, probes=4)
+    nfb = knnhip.last_fallback_count()
+    check(d, i, gd, gi, "probes=4")
+    assert nfb <= qry.shape[1] // 10, f"too many exact fallbacks: {nfb}"
+
+
+def test_uniform_data_stays_exact():
+    R, Q, D, k = 8000, 200, 16, 10
+    ref = knnhip.generate_uniform(R, D, seed=41)
+    qry = knnhip.generate_uniform(Q, D, seed=42)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    ix = knnhip.BlockIndex(ref, nblocks=32, iters=4)
+    d, i = ix.search(qry, k, probes=3)
+    check(d, i, gd, gi, "uniform")
+
+
+def test_arbitrary_scores_stay_exact(clustered):
+    ref, qry, k, gd, gi = clustered
+    ix = knnhip.BlockIndex(ref, nblocks=64, iters=6)
+    scores = torch.rand((qry.shape[1], 64), device="cuda")
+    d, i = ix.search(qry, k, probes=3, scores=scores)
+    check(d, i, gd, gi, "random scores")
+
+
+def test_duplicate_points_and_forced_fallback():
+    rng = np.random.default_rng(5)
+    base = np.round(3 * rng.random((8, 300))).astype(np.float32)
+    ref = torch.from_numpy(np.concatenate([base, base, base], axis=1)).cuda()
+    qry = torch.from_numpy(np.round(3 * rng.random((8, 40))).astype(np.float32)).cuda()
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), 12)
+    ix = knnhip.BlockIndex(ref, nblocks=16, iters=3)
+    check(*ix.search(qry, 12, probes=2), gd, gi, "duplicates")
+    check(*ix.search(qry, 12, probes=2, force_fallback=True), gd, gi, "forced fallback")
+    assert knnhip.last_fallback_count() == 40
+
+
+def test_index_matches_plain_path_at_scale():
+    # 262,144 refs (512 clusters, dim 96), 1024 queries, k 16: index == plain search
+    R, Q, D, k = 1 << 18, 1024, 96, 16
+    say("index vs plain at 262144 x 1024 x 96")
+    ref = knnhip.generate_mixture(R, D, 512, 0.1, 51, 52)
+    qry = knnhip.generate_mixture(Q, D, 512, 0.1, 51, 53)
+    pd, pi = knnhip.search_device(ref, qry, k)
+    ix = knnhip.BlockIndex(ref, nblocks=256, iters=8)
+    d, i = ix.search(qry, k, probes=4)
+    torch.cuda.synchronize()
+    nfb = knnhip.last_fallback_count()
+    say(f"index info {ix.info()}, fallbacks {nfb}")
+    assert torch.equal(i, pi) and torch.equal(d, pd)
+    sample = [0, 5, 500, 1023]
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), np.ascontiguousarray(qry.cpu().numpy()[:, sample]), k)
+    check(d[:, sample], i[:, sample], gd, gi, "oracle sample")
