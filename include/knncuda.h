@@ -118,6 +118,10 @@ int knnhip_index_search(knnhip_index *index, const float *query, int query_nb, l
                         const knnhip_options *opt, void *stream);
 void knnhip_index_free(knnhip_index *index);
 int knnhip_index_info(const knnhip_index *index, int *nblocks, int *n_pos, float *build_ms);
+/* Device times (ms) of the last search made with options.timing = 1: query
+ * norms, sub-ball bounds, block bounds, block choice, task build, scan,
+ * select/re-rank, exact fallback (n <= 8 values written). */
+int knnhip_index_stage_ms(const knnhip_index *index, float *ms, int n);
 
 /* Library version string. */
 const char *knnhip_version(void);

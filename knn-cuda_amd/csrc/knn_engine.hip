@@ -492,29 +492,76 @@ struct knnhip_index {
     const float *ref = nullptr;  // caller's dim-major references (exact fallback)
     long long ldr = 0;
     float build_ms = 0.f;
-    knnhip::DevBuf packed, pm, norms, perm, block_start, block_count, cent, radius, rmax;
+    int nf = 0;  // fine clusters (sub-balls): each block is a union of them
+    knnhip::DevBuf packed, pm, norms, perm, block_start, block_count, rmax;
+    knnhip::DevBuf fcent, fnorm, frad, fine_by_block, fb_start;
     // last search statistics
     int last_probes = 0;
+    int last_fail = 0;
     long long last_tasks = 0;
     float last_ms[6] = {-1, -1, -1, -1, -1, -1};
+    float rank_ms[4] = {-1, -1, -1, -1};  // norms, fine bounds, block bounds, pick
 };
 
 namespace knnhip {
 namespace {
 
+// Lloyd iterations on the point-major rows `pm` (n points) into `k`
+// centroids `cent` (point-major, ld pm_ld).  The assignment step is the
+// engine itself (centroids as the reference set, k = 1); the final
+// assignment is left in `assign`, its histogram in `counts`.
+bool kmeans(const float *pm, const float *dm, long long ldd, int n, int dim, int pm_ld, int k,
+            int iters, float *cent, int *assign, int *counts, hipStream_t s) {
+    DevBuf cent_dm, adist, start, cursor, order;
+    if (!cent_dm.alloc((size_t)k * dim * 4) || !adist.alloc((size_t)n * 4) ||
+        !start.alloc((size_t)(k + 1) * 4) || !cursor.alloc((size_t)(k + 1) * 4) ||
+        !order.alloc((size_t)n * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(idx_init_centroids(pm, pm_ld, n, dim, k, cent, pm_ld, s), "k-means init");
+    knnhip_options ao;
+    memset(&ao, 0, sizeof(ao));
+    ao.margin = 8;
+    const Plan ap = make_plan(k, n, dim, 1, ao);
+    for (int it = 0; it <= iters; ++it) {
+        KNN_TRY(idx_transpose(cent, pm_ld, k, dim, cent_dm.as<float>(), s), "transpose");
+        if (!run_device(cent_dm.as<float>(), k, k, dm, ldd, n, dim, 1, adist.as<float>(), assign,
+                        ap, ao, s))
+            return false;
+        KNN_TRY(hipMemsetAsync(counts, 0, (size_t)k * 4, s), "memset");
+        KNN_TRY(idx_histogram(assign, n, counts, s), "histogram");
+        if (it == iters) break;
+        KNN_TRY(idx_prefix(counts, k, 1, start.as<int>(), nullptr, s), "prefix");
+        KNN_TRY(hipMemcpyAsync(cursor.p, start.p, (size_t)k * 4, hipMemcpyDeviceToDevice, s), "copy");
+        KNN_TRY(idx_scatter(assign, n, cursor.as<int>(), order.as<int>(), s), "scatter");
+        KNN_TRY(idx_centroids(pm, pm_ld, dim, order.as<int>(), start.as<int>(), counts, k, cent,
+                              pm_ld, s),
+                "centroids");
+    }
+    return true;
+}
+
+// Two-level index: `nf` fine clusters (k-means over the references) grouped
+// into `nb` blocks (k-means over the fine centroids, weighted by size).  The
+// references are stored block by block (each block starting at a multiple
+// of 32), each fine cluster contiguous inside its block; a block's lower
+// bound for a query is the smallest over its fine clusters' balls.
 bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int dim, int nb,
                  int iters, hipStream_t s) {
     hipEvent_t e0, e1;
     KNN_TRY(hipEventCreate(&e0), "event");
     KNN_TRY(hipEventCreate(&e1), "event");
     KNN_TRY(hipEventRecord(e0, s), "event");
+    constexpr int kSubBalls = 4;
+    const int nf = std::min(R, nb * kSubBalls);
     ix->R = R;
     ix->dim = dim;
     ix->nb = nb;
+    ix->nf = nf;
     ix->ref = ref;
     ix->ldr = ldr;
     ix->pm_ld = (dim + 3) & ~3;
-    // scan shape (same choice as the plain path)
     knnhip_options o;
     memset(&o, 0, sizeof(o));
     {
@@ -530,14 +577,16 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
         ix->oct = p.oct;
     }
     const int pm_ld = ix->pm_ld;
-    DevBuf pm0, norms0, cent_dm, assign, adist, counts, start, cursor, perm0;
-    if (!pm0.alloc((size_t)R * pm_ld * 4) || !norms0.alloc((size_t)R * 4) ||
-        !ix->rmax.alloc(4) || !ix->cent.alloc((size_t)nb * pm_ld * 4) ||
-        !cent_dm.alloc((size_t)nb * dim * 4) || !assign.alloc((size_t)R * 4) ||
-        !adist.alloc((size_t)R * 4) || !counts.alloc((size_t)nb * 4) ||
-        !start.alloc((size_t)(nb + 1) * 4) || !cursor.alloc((size_t)(nb + 1) * 4) ||
-        !perm0.alloc((size_t)R * 4) || !ix->block_start.alloc((size_t)(nb + 1) * 4) ||
-        !ix->block_count.alloc((size_t)nb * 4) || !ix->radius.alloc((size_t)nb * 8)) {
+    DevBuf pm0, norms0, fassign, fcount, cent, fcent_dm, fblock, bcnt, cursor, fstart, tmp;
+    if (!pm0.alloc((size_t)R * pm_ld * 4) || !norms0.alloc((size_t)R * 4) || !ix->rmax.alloc(4) ||
+        !fassign.alloc((size_t)R * 4) || !fcount.alloc((size_t)nf * 4) ||
+        !ix->fcent.alloc((size_t)nf * pm_ld * 4) || !cent.alloc((size_t)nb * pm_ld * 4) ||
+        !fcent_dm.alloc((size_t)nf * dim * 4) || !fblock.alloc((size_t)nf * 4) ||
+        !bcnt.alloc((size_t)nb * 4) || !cursor.alloc((size_t)(std::max(nb, nf) + 1) * 4) ||
+        !fstart.alloc((size_t)nf * 4) || !tmp.alloc((size_t)(nb + 1) * 4) ||
+        !ix->block_start.alloc((size_t)(nb + 1) * 4) || !ix->block_count.alloc((size_t)nb * 4) ||
+        !ix->frad.alloc((size_t)nf * 8) || !ix->fnorm.alloc((size_t)nf * 4) ||
+        !ix->fine_by_block.alloc((size_t)nf * 4) || !ix->fb_start.alloc((size_t)(nb + 1) * 4)) {
         set_error("Memory allocation error");
         return false;
     }
@@ -555,36 +604,56 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
     na.pm = pm0.as<float>();
     na.pm_ld = pm_ld;
     KNN_TRY(launch_norms(na, s), "norm kernel launch");
-    KNN_TRY(idx_init_centroids(pm0.as<float>(), pm_ld, R, dim, nb, ix->cent.as<float>(), pm_ld, s),
-            "k-means init");
-    knnhip_options ao;
-    memset(&ao, 0, sizeof(ao));
-    ao.margin = 8;
-    const Plan ap = make_plan(nb, R, dim, 1, ao);
-    for (int it = 0; it <= iters; ++it) {
-        // assignment: nearest centroid of every reference (the engine itself,
-        // centroids as the reference set, k = 1)
-        KNN_TRY(idx_transpose(ix->cent.as<float>(), pm_ld, nb, dim, cent_dm.as<float>(), s),
-                "transpose");
-        if (!run_device(cent_dm.as<float>(), nb, nb, ref, ldr, R, dim, 1, adist.as<float>(),
-                        assign.as<int>(), ap, ao, s))
+    // fine level: k-means over the references
+    if (!kmeans(pm0.as<float>(), ref, ldr, R, dim, pm_ld, nf, iters, ix->fcent.as<float>(),
+                fassign.as<int>(), fcount.as<int>(), s))
+        return false;
+    // coarse level: k-means over the fine centroids (weighted by size)
+    KNN_TRY(idx_transpose(ix->fcent.as<float>(), pm_ld, nf, dim, fcent_dm.as<float>(), s),
+            "transpose");
+    {
+        DevBuf cent_dm, adist;
+        if (!cent_dm.alloc((size_t)nb * dim * 4) || !adist.alloc((size_t)nf * 4)) {
+            set_error("Memory allocation error");
             return false;
-        KNN_TRY(hipMemsetAsync(counts.p, 0, (size_t)nb * 4, s), "memset");
-        KNN_TRY(idx_histogram(assign.as<int>(), R, counts.as<int>(), s), "histogram");
-        if (it == iters) break;
-        // update: members grouped by block, centroid = mean (empty blocks keep theirs)
-        KNN_TRY(idx_prefix(counts.as<int>(), nb, 1, start.as<int>(), nullptr, s), "prefix");
-        KNN_TRY(hipMemcpyAsync(cursor.p, start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
-                "copy");
-        KNN_TRY(idx_scatter(assign.as<int>(), R, cursor.as<int>(), perm0.as<int>(), s), "scatter");
-        KNN_TRY(idx_centroids(pm0.as<float>(), pm_ld, dim, perm0.as<int>(), start.as<int>(),
-                              counts.as<int>(), nb, ix->cent.as<float>(), pm_ld, s),
-                "centroids");
+        }
+        KNN_TRY(idx_init_centroids(ix->fcent.as<float>(), pm_ld, nf, dim, nb, cent.as<float>(),
+                                   pm_ld, s),
+                "k-means init");
+        knnhip_options ao;
+        memset(&ao, 0, sizeof(ao));
+        ao.margin = 8;
+        const Plan ap = make_plan(nb, nf, dim, 1, ao);
+        for (int it = 0; it <= iters; ++it) {
+            KNN_TRY(idx_transpose(cent.as<float>(), pm_ld, nb, dim, cent_dm.as<float>(), s),
+                    "transpose");
+            if (!run_device(cent_dm.as<float>(), nb, nb, fcent_dm.as<float>(), nf, nf, dim, 1,
+                            adist.as<float>(), fblock.as<int>(), ap, ao, s))
+                return false;
+            // fine clusters grouped by block
+            KNN_TRY(hipMemsetAsync(bcnt.p, 0, (size_t)nb * 4, s), "memset");
+            KNN_TRY(idx_histogram(fblock.as<int>(), nf, bcnt.as<int>(), s), "histogram");
+            KNN_TRY(idx_prefix(bcnt.as<int>(), nb, 1, ix->fb_start.as<int>(), nullptr, s), "prefix");
+            KNN_TRY(hipMemcpyAsync(cursor.p, ix->fb_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice,
+                                   s),
+                    "copy");
+            KNN_TRY(idx_scatter(fblock.as<int>(), nf, cursor.as<int>(),
+                                ix->fine_by_block.as<int>(), s),
+                    "scatter");
+            if (it == iters) break;
+            KNN_TRY(idx_coarse_centroids(ix->fcent.as<float>(), pm_ld, dim,
+                                         ix->fine_by_block.as<int>(), ix->fb_start.as<int>(),
+                                         fcount.as<int>(), nb, cent.as<float>(), s),
+                    "coarse centroids");
+        }
     }
-    // final layout: blocks contiguous, each starting at a multiple of 32
-    KNN_TRY(hipMemcpyAsync(ix->block_count.p, counts.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
-            "copy");
-    KNN_TRY(idx_prefix(counts.as<int>(), nb, kRefsPerTile, ix->block_start.as<int>(), nullptr, s),
+    // layout: blocks contiguous (starts aligned to 32), fine clusters
+    // contiguous inside their block
+    KNN_TRY(hipMemsetAsync(ix->block_count.p, 0, (size_t)nb * 4, s), "memset");
+    KNN_TRY(idx_block_count(fblock.as<int>(), fcount.as<int>(), nf, ix->block_count.as<int>(), s),
+            "block count");
+    KNN_TRY(idx_prefix(ix->block_count.as<int>(), nb, kRefsPerTile, ix->block_start.as<int>(),
+                       nullptr, s),
             "prefix");
     int n_pos = 0;
     KNN_TRY(hipMemcpyAsync(&n_pos, ix->block_start.as<int>() + nb, 4, hipMemcpyDeviceToHost, s),
@@ -598,19 +667,23 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
         set_error("Memory allocation error");
         return false;
     }
+    KNN_TRY(hipMemcpyAsync(tmp.p, ix->block_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+            "copy");
+    KNN_TRY(idx_fine_start(fblock.as<int>(), fcount.as<int>(), nf, tmp.as<int>(), fstart.as<int>(), s),
+            "fine start");
+    KNN_TRY(hipMemcpyAsync(cursor.p, fstart.p, (size_t)nf * 4, hipMemcpyDeviceToDevice, s), "copy");
     KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)ix->perm.p, -1, ix->r_pad, s), "memset");
     KNN_TRY(hipMemsetAsync(ix->pm.p, 0, (size_t)ix->r_pad * pm_ld * 4, s), "memset");
-    KNN_TRY(hipMemcpyAsync(cursor.p, ix->block_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
-            "copy");
-    KNN_TRY(idx_scatter(assign.as<int>(), R, cursor.as<int>(), ix->perm.as<int>(), s), "scatter");
+    KNN_TRY(idx_scatter(fassign.as<int>(), R, cursor.as<int>(), ix->perm.as<int>(), s), "scatter");
     KNN_TRY(idx_gather_pack(pm0.as<float>(), pm_ld, dim, ix->oct, ix->perm.as<int>(), n_pos,
                             ix->r_pad, ix->packed.as<float>(), ix->pm.as<float>(),
                             ix->norms.as<float>(), s),
             "gather");
-    KNN_TRY(idx_radius(ix->pm.as<float>(), pm_ld, dim, ix->block_start.as<int>(),
-                       ix->block_count.as<int>(), nb, ix->cent.as<float>(), pm_ld,
-                       ix->radius.as<double>(), s),
+    // sub-ball radii and centroid norms
+    KNN_TRY(idx_radius(ix->pm.as<float>(), pm_ld, dim, fstart.as<int>(), fcount.as<int>(), nf,
+                       ix->fcent.as<float>(), pm_ld, ix->frad.as<double>(), s),
             "radius");
+    KNN_TRY(idx_row_norms(ix->fcent.as<float>(), pm_ld, nf, dim, ix->fnorm.as<float>(), s), "norms");
     KNN_TRY(hipEventRecord(e1, s), "event");
     KNN_TRY(hipEventSynchronize(e1), "event");
     KNN_TRY(hipEventElapsedTime(&ix->build_ms, e0, e1), "event");
@@ -621,8 +694,8 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
 
 // Workspace of index searches (grown on demand, kept between calls).
 struct IndexWork {
-    DevBuf qnorm, qpm, lb, score, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks, pval,
-        pidx, theta, fb_count, fb_list, fb_key, fb_idx;
+    DevBuf qnorm, qpm, lbf, lb, score, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks,
+        pval, pidx, theta, fb_count, fb_list, fb_key, fb_idx;
     bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
 IndexWork g_iw;
@@ -639,10 +712,18 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
     }
     P = std::max(1, std::min(P, nb));
     const long long npairs = (long long)Q * P;
-    const int max_tasks = (int)((npairs + kQueriesPerWave - 1) / kQueriesPerWave) + nb;
-    const int egrid = std::min(kExactGrid, Q);
+    // Two scan phases: each query's first-ranked block, then its other
+    // probes, which start from the thresholds the first phase left.
+    const int nphase = P > 1 ? 2 : 1;
+    const int pr[3] = {0, 1, P};
+    int max_t[2] = {0, 0};
+    for (int ph = 0; ph < nphase; ++ph)
+        max_t[ph] = (int)(((long long)Q * (pr[ph + 1] - pr[ph]) + kQueriesPerWave - 1) /
+                          kQueriesPerWave) + nb;
+    const int max_tasks = max_t[0] + max_t[1];
     IndexWork &w = g_iw;
     if (!w.need(w.qnorm, (size_t)Q * 4) || !w.need(w.qpm, (size_t)Q * pm_ld * 4) ||
+        !w.need(w.lbf, (size_t)Q * ix->nf * 4) ||
         !w.need(w.lb, (size_t)Q * nb * 4) || !w.need(w.score, (size_t)Q * nb * 4) ||
         !w.need(w.chosen, (size_t)npairs * 4) || !w.need(w.lb_rest, (size_t)Q * 4) ||
         !w.need(w.pcount, (size_t)nb * 4) || !w.need(w.qstart, (size_t)(nb + 1) * 4) ||
@@ -651,14 +732,14 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
         !w.need(w.rows, (size_t)npairs * 4) || !w.need(w.tasks, (size_t)max_tasks * sizeof(ScanTask)) ||
         !w.need(w.pval, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
         !w.need(w.pidx, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
-        !w.need(w.theta, (size_t)Q * 4) || !w.need(w.fb_count, 4) || !w.need(w.fb_list, (size_t)Q * 4) ||
-        !w.need(w.fb_key, (size_t)egrid * kWave * k * 4) ||
-        !w.need(w.fb_idx, (size_t)egrid * kWave * k * 4)) {
+        !w.need(w.theta, (size_t)Q * 4) || !w.need(w.fb_count, 4) || !w.need(w.fb_list, (size_t)Q * 4)) {
         set_error("Memory allocation error");
         return false;
     }
-    hipEvent_t ev[6];
+    hipEvent_t ev[6], evr[4];
     const bool timing = o.timing != 0;
+    if (timing)
+        for (auto &e : evr) KNN_TRY(hipEventCreate(&e), "event");
     if (timing)
         for (auto &e : ev) KNN_TRY(hipEventCreate(&e), "event");
     if (timing) KNN_TRY(hipEventRecord(ev[0], s), "event");
@@ -674,60 +755,70 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
     nq.pm = w.qpm.as<float>();
     nq.pm_ld = pm_ld;
     KNN_TRY(launch_norms(nq, s), "norm kernel launch");
-    // block lower bounds and ranking
-    KNN_TRY(idx_lower_bounds(w.qpm.as<float>(), pm_ld, Q, dim, ix->cent.as<float>(), pm_ld, nb,
-                             ix->radius.as<double>(), w.lb.as<float>(), s),
+    if (timing) KNN_TRY(hipEventRecord(evr[0], s), "event");
+    // lower bounds: every fine ball, then each block's smallest; the
+    // router-free ranking is the negated block bound (nearest block first)
+    KNN_TRY(idx_fine_lb(w.qpm.as<float>(), pm_ld, Q, dim, w.qnorm.as<float>(), ix->fcent.as<float>(),
+                        ix->nf, ix->fnorm.as<float>(), ix->frad.as<double>(), w.lbf.as<float>(), s),
             "lower bounds");
-    const float *score = router_scores;
-    if (!score) {
-        KNN_TRY(idx_negate(w.lb.as<float>(), (long long)Q * nb, w.score.as<float>(), s), "negate");
-        score = w.score.as<float>();
-    }
+    if (timing) KNN_TRY(hipEventRecord(evr[1], s), "event");
+    KNN_TRY(idx_block_lb(w.lbf.as<float>(), Q, ix->nf, ix->fine_by_block.as<int>(),
+                         ix->fb_start.as<int>(), nb, w.lb.as<float>(),
+                         router_scores ? nullptr : w.score.as<float>(), s),
+            "block bounds");
+    const float *score = router_scores ? router_scores : w.score.as<float>();
+    if (timing) KNN_TRY(hipEventRecord(evr[2], s), "event");
     KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
                             w.chosen.as<int>(), w.lb_rest.as<float>(), s),
             "pick blocks");
     if (timing) KNN_TRY(hipEventRecord(ev[1], s), "event");
-    // (query, block) pairs -> per-block query lists -> scan tasks
-    KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nb * 4, s), "memset");
-    KNN_TRY(idx_pair_count(w.chosen.as<int>(), npairs, w.pcount.as<int>(), s), "pairs");
-    KNN_TRY(idx_prefix(w.pcount.as<int>(), nb, kQueriesPerWave, w.qstart.as<int>(), nullptr, s),
-            "prefix");
-    KNN_TRY(hipMemcpyAsync(w.cursor.p, w.qstart.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
-            "copy");
-    KNN_TRY(idx_pair_scatter(w.chosen.as<int>(), npairs, P, w.cursor.as<int>(), w.qlist.as<int>(),
-                             w.rows.as<int>(), s),
-            "pairs");
-    KNN_TRY(idx_make_tasks(w.qstart.as<int>(), w.pcount.as<int>(), nb, ix->block_start.as<int>(),
-                           ix->block_count.as<int>(), w.tasks.as<ScanTask>(), max_tasks, s),
-            "tasks");
     if (timing) KNN_TRY(hipEventRecord(ev[2], s), "event");
-    // scan the chosen blocks
     KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Q, s), "memset");
     KNN_TRY(hipMemsetAsync(w.fb_count.p, 0, 4, s), "memset");
-    ScanArgs sa{};
-    sa.packed = ix->packed.as<float>();
-    sa.oct = ix->oct;
-    sa.R = ix->n_pos;
-    sa.dim = dim;
-    sa.query = query;
-    sa.ldq = ldq;
-    sa.Q = Q;
-    sa.rnorm = ix->norms.as<float>();
-    sa.theta = w.theta.as<unsigned>();
-    sa.slice_len = ix->r_pad;
-    sa.slice_span = (long long)ix->r_pad * ix->dpad * 4;
-    sa.r_pad = ix->r_pad;
-    sa.nslices = 1;
-    sa.kp = kp;
-    sa.nqb = 1;
-    sa.nchunks = ix->nchunks;
-    sa.xcd_map = 0;
-    sa.part_val = w.pval.as<float>();
-    sa.part_idx = w.pidx.as<int>();
-    sa.tasks = w.tasks.as<ScanTask>();
-    sa.ntasks = max_tasks;
-    sa.qlist = w.qlist.as<int>();
-    KNN_TRY(launch_scan(sa, ix->ks, ix->multi, 4, s), "scan kernel launch");
+    int tbase = 0;
+    for (int ph = 0; ph < nphase; ++ph) {
+        // (query, block) pairs of this phase -> per-block query lists -> tasks
+        const int base = tbase * kQueriesPerWave;
+        KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nb * 4, s), "memset");
+        KNN_TRY(idx_pair_count(w.chosen.as<int>(), Q, P, pr[ph], pr[ph + 1], w.pcount.as<int>(), s),
+                "pairs");
+        KNN_TRY(idx_prefix(w.pcount.as<int>(), nb, kQueriesPerWave, w.qstart.as<int>(), nullptr, s),
+                "prefix");
+        KNN_TRY(hipMemcpyAsync(w.cursor.p, w.qstart.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+                "copy");
+        KNN_TRY(idx_pair_scatter(w.chosen.as<int>(), Q, P, pr[ph], pr[ph + 1], w.cursor.as<int>(),
+                                 w.qlist.as<int>() + base, w.rows.as<int>(), base, s),
+                "pairs");
+        KNN_TRY(idx_make_tasks(w.qstart.as<int>(), w.pcount.as<int>(), nb, ix->block_start.as<int>(),
+                               ix->block_count.as<int>(), w.tasks.as<ScanTask>() + tbase, max_t[ph],
+                               s),
+                "tasks");
+        ScanArgs sa{};
+        sa.packed = ix->packed.as<float>();
+        sa.oct = ix->oct;
+        sa.R = ix->n_pos;
+        sa.dim = dim;
+        sa.query = query;
+        sa.ldq = ldq;
+        sa.Q = Q;
+        sa.rnorm = ix->norms.as<float>();
+        sa.theta = w.theta.as<unsigned>();
+        sa.slice_len = ix->r_pad;
+        sa.slice_span = (long long)ix->r_pad * ix->dpad * 4;
+        sa.r_pad = ix->r_pad;
+        sa.nslices = 1;
+        sa.kp = kp;
+        sa.nqb = 1;
+        sa.nchunks = ix->nchunks;
+        sa.xcd_map = 0;
+        sa.part_val = w.pval.as<float>() + (size_t)base * kp;
+        sa.part_idx = w.pidx.as<int>() + (size_t)base * kp;
+        sa.tasks = w.tasks.as<ScanTask>() + tbase;
+        sa.ntasks = max_t[ph];
+        sa.qlist = w.qlist.as<int>() + base;
+        KNN_TRY(launch_scan(sa, ix->ks, ix->multi, 4, s), "scan kernel launch");
+        tbase += max_t[ph];
+    }
     if (timing) KNN_TRY(hipEventRecord(ev[3], s), "event");
     // exact re-rank + certificate (skipped blocks bounded by lb_rest)
     FinalArgs fa{};
@@ -760,23 +851,31 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
     fa.fb_list = w.fb_list.as<int>();
     KNN_TRY(launch_select(fa, s), "select kernel launch");
     if (timing) KNN_TRY(hipEventRecord(ev[4], s), "event");
-    // queries whose certificate failed: exact full scan of the caller's references
-    ExactArgs ea{};
-    ea.ref = ix->ref;
-    ea.ldr = ix->ldr;
-    ea.R = ix->R;
-    ea.query = query;
-    ea.ldq = ldq;
-    ea.Q = Q;
-    ea.dim = dim;
-    ea.k = k;
-    ea.list = w.fb_list.as<int>();
-    ea.count = w.fb_count.as<int>();
-    ea.scratch_key = w.fb_key.as<float>();
-    ea.scratch_idx = w.fb_idx.as<int>();
-    ea.out_dist = out_dist;
-    ea.out_idx = out_idx;
-    KNN_TRY(launch_exact(ea, egrid, s), "exact scan launch");
+    // queries whose certificate failed: the plain exact search on just those
+    int nfail = 0;
+    KNN_TRY(hipMemcpyAsync(&nfail, w.fb_count.p, 4, hipMemcpyDeviceToHost, s), "copy");
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    if (nfail > 0) {
+        DevBuf qsub, dsub, isub;
+        if (!qsub.alloc((size_t)nfail * dim * 4) || !dsub.alloc((size_t)nfail * k * 4) ||
+            !isub.alloc((size_t)nfail * k * 4)) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        KNN_TRY(idx_gather_columns(query, ldq, dim, w.fb_list.as<int>(), nfail, qsub.as<float>(), s),
+                "gather");
+        knnhip_options po = o;
+        po.force_fallback = 0;
+        po.timing = 0;
+        const Plan pp = make_plan(ix->R, nfail, dim, k, po);
+        if (!run_device(ix->ref, ix->ldr, ix->R, qsub.as<float>(), nfail, nfail, dim, k,
+                        dsub.as<float>(), isub.as<int>(), pp, po, s))
+            return false;
+        KNN_TRY(idx_scatter_results(dsub.as<float>(), isub.as<int>(), k, nfail, w.fb_list.as<int>(),
+                                    Q, out_dist, out_idx, s),
+                "scatter");
+        KNN_TRY(hipStreamSynchronize(s), "sync");
+    }
     ix->last_probes = P;
     ix->last_tasks = max_tasks;
     g_last_stream = s;
@@ -784,8 +883,14 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
     if (timing) {
         KNN_TRY(hipEventRecord(ev[5], s), "event");
         KNN_TRY(hipEventSynchronize(ev[5]), "event");
+        ix->last_fail = nfail;
         for (int i = 0; i < 5; ++i) KNN_TRY(hipEventElapsedTime(&ix->last_ms[i], ev[i], ev[i + 1]), "event");
+        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[0], ev[0], evr[0]), "event");
+        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[1], evr[0], evr[1]), "event");
+        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[2], evr[1], evr[2]), "event");
+        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[3], evr[2], ev[1]), "event");
         for (auto &e : ev) (void)hipEventDestroy(e);
+        for (auto &e : evr) (void)hipEventDestroy(e);
         // stage slots: 0 ranking (norms, bounds, pick + tasks), 1 scan, 2 select, 3 fallback
         g_stage_ms[0] = ix->last_ms[0] + ix->last_ms[1];
         g_stage_ms[1] = ix->last_ms[2];
@@ -960,6 +1065,14 @@ int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long
 void knnhip_index_free(knnhip_index *ix) {
     std::lock_guard<std::mutex> lock(g_mu);
     delete ix;
+}
+
+int knnhip_index_stage_ms(const knnhip_index *ix, float *ms, int n) {
+    if (!ix || !ms) return 0;
+    const float all[9] = {ix->rank_ms[0], ix->rank_ms[1], ix->rank_ms[2], ix->rank_ms[3],
+                          ix->last_ms[1], ix->last_ms[2], ix->last_ms[3], ix->last_ms[4], -1.f};
+    for (int i = 0; i < n && i < 9; ++i) ms[i] = all[i];
+    return 1;
 }
 
 int knnhip_index_info(const knnhip_index *ix, int *nblocks, int *n_pos, float *build_ms) {

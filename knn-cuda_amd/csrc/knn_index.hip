@@ -266,33 +266,63 @@ __global__ __launch_bounds__(256) void pick_blocks_kernel(const float *__restric
     if (lane == 0) lb_rest[q] = m;
 }
 
-// Invert the (query -> P blocks) lists into per-block query lists and scan
-// tasks of up to 32 queries.  Step 1: count pairs per block.
-__global__ void pair_count_kernel(const int *__restrict__ chosen, long long npairs,
+// Invert the (query -> blocks) lists, probes [p0, p1) of each query's P, into
+// per-block query lists and scan tasks of up to 32 queries.  Step 1: count
+// pairs per block.
+__global__ void pair_count_kernel(const int *__restrict__ chosen, int Q, int P, int p0, int p1,
                                   int *__restrict__ counts) {
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < npairs;
+    const int w = p1 - p0;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)Q * w;
          e += (long long)gridDim.x * blockDim.x) {
-        const int b = chosen[e];
+        const long long q = e / w;
+        const int b = chosen[q * P + p0 + (e - q * w)];
         if (b >= 0) atomicAdd(&counts[b], 1);
     }
 }
 
 // Step 2: place each pair in its block's list (qlist, grouped by block with
 // each block's group starting at a multiple of 32) and record the heap row
-// it will produce: the pair's slot in the list is exactly its output row.
-__global__ void pair_scatter_kernel(const int *__restrict__ chosen, long long npairs, int P,
+// it will produce: the pair's slot in the list, plus row_base, is its
+// output row.
+__global__ void pair_scatter_kernel(const int *__restrict__ chosen, int Q, int P, int p0, int p1,
                                     int *__restrict__ cursor, int *__restrict__ qlist,
-                                    int *__restrict__ rows) {
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < npairs;
+                                    int *__restrict__ rows, int row_base) {
+    const int w = p1 - p0;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)Q * w;
          e += (long long)gridDim.x * blockDim.x) {
-        const int b = chosen[e];
+        const long long q = e / w;
+        const long long pi = q * P + p0 + (e - q * w);
+        const int b = chosen[pi];
         if (b < 0) {
-            rows[e] = -1;
+            rows[pi] = -1;
             continue;
         }
         const int slot = atomicAdd(&cursor[b], 1);
-        qlist[slot] = (int)(e / P);
-        rows[e] = slot;
+        qlist[slot] = (int)q;
+        rows[pi] = row_base + slot;
+    }
+}
+
+// Gather query columns list[0..n) of a dim-major set into a compact one,
+// and scatter k x n results back into the k x Q outputs.
+__global__ void gather_columns_kernel(const float *__restrict__ src, long long ld, int dim,
+                                      const int *__restrict__ list, int n, float *__restrict__ dst) {
+    const long long total = (long long)n * dim;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long d = e / n, j = e - d * n;
+        dst[e] = src[d * ld + list[j]];
+    }
+}
+__global__ void scatter_results_kernel(const float *__restrict__ dsub, const int *__restrict__ isub,
+                                       int k, int n, const int *__restrict__ list, int Q,
+                                       float *__restrict__ dist, int *__restrict__ idx) {
+    const long long total = (long long)n * k;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long r = e / n, j = e - r * n;
+        dist[r * Q + list[j]] = dsub[e];
+        idx[r * Q + list[j]] = isub[e];
     }
 }
 
@@ -337,6 +367,153 @@ __global__ void negate_kernel(const float *__restrict__ x, long long n, float *_
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
          e += (long long)gridDim.x * blockDim.x)
         y[e] = -x[e];
+}
+
+// ---------------------------------------------------------------------------
+// Two-level layout: fine clusters (sub-balls) grouped into blocks
+// ---------------------------------------------------------------------------
+
+// bcount[fblock[f]] += fcount[f]
+__global__ void block_count_kernel(const int *__restrict__ fblock, const int *__restrict__ fcount,
+                                   int nf, int *__restrict__ bcount) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x)
+        atomicAdd(&bcount[fblock[f]], fcount[f]);
+}
+
+// Each fine cluster takes a contiguous range inside its block.
+__global__ void fine_start_kernel(const int *__restrict__ fblock, const int *__restrict__ fcount,
+                                  int nf, int *__restrict__ bcursor, int *__restrict__ fstart) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x)
+        fstart[f] = atomicAdd(&bcursor[fblock[f]], fcount[f]);
+}
+
+// Coarse centroid = mean of its fine centroids weighted by their sizes (a
+// block with no members keeps its centroid).  One block per coarse cluster;
+// members listed by fine_by_block[fb_start[b] .. fb_start[b+1]).
+__global__ void coarse_centroid_kernel(const float *__restrict__ fcent, int ld, int dim,
+                                       const int *__restrict__ fine_by_block,
+                                       const int *__restrict__ fb_start,
+                                       const int *__restrict__ fcount, float *__restrict__ cent) {
+    const int b = blockIdx.x;
+    const int f0 = fb_start[b], f1 = fb_start[b + 1];
+    long long w = 0;
+    for (int j = f0; j < f1; ++j) w += fcount[fine_by_block[j]];
+    if (w == 0) return;
+    for (int d = threadIdx.x; d < dim; d += blockDim.x) {
+        double acc = 0.0;
+        for (int j = f0; j < f1; ++j) {
+            const int f = fine_by_block[j];
+            acc += (double)fcent[(long long)f * ld + d] * (double)fcount[f];
+        }
+        cent[(long long)b * ld + d] = (float)(acc / (double)w);
+    }
+}
+
+// Lower bound of every reference of fine cluster f for query q (distance,
+// not squared, rounded down): max(0, |q - c_f| - rho_f).  |q - c_f|^2 comes
+// from ||q||^2 + ||c||^2 - 2 q.c on FP32 MFMA, lowered by the error bound
+// g*(||q||^2 + ||c||^2 + 2|q||c|), g = 2*(dim+4)*2^-24.  One wave computes a
+// 32 (clusters) x 32 (queries) tile; k-step kk reads dimension kk + h*KH on
+// lane half h (KH = half the padded dimension), 32 k-steps per chunk.
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void fine_lb_kernel(const float *__restrict__ qpm, int pm_ld, int Q,
+                                                      int dim, const float *__restrict__ qnorm,
+                                                      const float *__restrict__ fcent, int nf,
+                                                      const float *__restrict__ fnorm,
+                                                      const double *__restrict__ frad,
+                                                      float *__restrict__ lbf) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 31, h = lane >> 5;
+    const int ntf = (nf + 31) / 32;
+    const long long tile = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+    const int tq = (int)(tile / ntf), tf = (int)(tile - (long long)tq * ntf);
+    const int q0 = tq * 32, f0 = tf * 32;
+    if (q0 >= Q) return;
+    const int qa = min(q0 + col, Q - 1);  // B operand: query col
+    const int fa = min(f0 + col, nf - 1); // A operand: cluster col
+    const float *qr = qpm + (long long)qa * pm_ld;
+    const float *cr = fcent + (long long)fa * pm_ld;
+    const int KH = (dim + 1) / 2;
+    f32x16_t acc = {};
+    if ((KH & 3) == 0 && (pm_ld & 3) == 0) {
+        // each lane reads its own row 16 bytes at a time (dimensions kk..kk+3 of its half)
+        const float *ca = cr + h * KH, *qa4 = qr + h * KH;
+        for (int k0 = 0; k0 < KH; k0 += 16) {
+            float4 av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = k0 + 4 * u < KH;
+                av[u] = ok ? *reinterpret_cast<const float4 *>(ca + k0 + 4 * u) : make_float4(0, 0, 0, 0);
+                bv[u] = ok ? *reinterpret_cast<const float4 *>(qa4 + k0 + 4 * u) : make_float4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].x, bv[u].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].y, bv[u].y, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].z, bv[u].z, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].w, bv[u].w, acc, 0, 0, 0);
+            }
+        }
+    } else {
+        for (int k0 = 0; k0 < KH; k0 += 32) {
+            float av[32], bv[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int kk = k0 + u;
+                const int d = kk + h * KH;
+                const bool ok = kk < KH && d < dim;
+                av[u] = ok ? cr[d] : 0.0f;
+                bv[u] = ok ? qr[d] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 32; ++u)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bv[u], acc, 0, 0, 0);
+        }
+    }
+    const int q = q0 + col;
+    if (q >= Q) return;
+    const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
+    const double nq = (double)qnorm[q];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int f = f0 + ((i >> 2) << 3) + (h << 2) + (i & 3);
+        if (f >= nf) continue;
+        const double nc = (double)fnorm[f];
+        const double dot = (double)acc[i];
+        double d2 = nq + nc - 2.0 * dot - g * (nq + nc + 2.0 * sqrt(nq * nc)) * 1.5;
+        d2 = d2 > 0.0 ? d2 : 0.0;
+        const double v = sqrt(d2) * (1.0 - 1e-12) - frad[f];
+        float fv = (float)v;
+        if ((double)fv > v) fv = nextafterf(fv, -INFINITY);
+        lbf[(long long)q * nf + f] = fv > 0.0f ? fv : 0.0f;
+    }
+}
+
+// Squared norms of point-major rows (FP32, sequential).
+__global__ void row_norms_kernel(const float *__restrict__ pm, int ld, int n, int dim,
+                                 float *__restrict__ out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int d = 0; d < dim; ++d) s = fmaf(pm[(long long)i * ld + d], pm[(long long)i * ld + d], s);
+        out[i] = s;
+    }
+}
+
+// Block lower bound = min over the block's fine clusters; negated as the
+// router-free ranking score.  lbb and score are [q][nb].
+__global__ void block_lb_kernel(const float *__restrict__ lbf, int Q, int nf,
+                                const int *__restrict__ fine_by_block,
+                                const int *__restrict__ fb_start, int nb, float *__restrict__ lbb,
+                                float *__restrict__ score) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)Q * nb) return;
+    const int q = (int)(e / nb), b = (int)(e - (long long)q * nb);
+    float m = INFINITY;
+    for (int j = fb_start[b]; j < fb_start[b + 1]; ++j)
+        m = fminf(m, lbf[(long long)q * nf + fine_by_block[j]]);
+    lbb[e] = m;
+    if (score) score[e] = -m;
 }
 
 // ---------------------------------------------------------------------------
@@ -402,15 +579,28 @@ hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float
                        counts, chosen, lb_rest);
     return hipGetLastError();
 }
-hipError_t idx_pair_count(const int *chosen, long long npairs, int *counts, hipStream_t s) {
-    hipLaunchKernelGGL(pair_count_kernel, dim3(grid_for(npairs)), dim3(256), 0, s, chosen, npairs,
-                       counts);
+hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int *counts,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(pair_count_kernel, dim3(grid_for((long long)Q * (p1 - p0))), dim3(256), 0, s,
+                       chosen, Q, P, p0, p1, counts);
     return hipGetLastError();
 }
-hipError_t idx_pair_scatter(const int *chosen, long long npairs, int P, int *cursor, int *qlist,
-                            int *rows, hipStream_t s) {
-    hipLaunchKernelGGL(pair_scatter_kernel, dim3(grid_for(npairs)), dim3(256), 0, s, chosen,
-                       npairs, P, cursor, qlist, rows);
+hipError_t idx_pair_scatter(const int *chosen, int Q, int P, int p0, int p1, int *cursor,
+                            int *qlist, int *rows, int row_base, hipStream_t s) {
+    hipLaunchKernelGGL(pair_scatter_kernel, dim3(grid_for((long long)Q * (p1 - p0))), dim3(256), 0,
+                       s, chosen, Q, P, p0, p1, cursor, qlist, rows, row_base);
+    return hipGetLastError();
+}
+hipError_t idx_gather_columns(const float *src, long long ld, int dim, const int *list, int n,
+                              float *dst, hipStream_t s) {
+    hipLaunchKernelGGL(gather_columns_kernel, dim3(grid_for((long long)n * dim)), dim3(256), 0, s,
+                       src, ld, dim, list, n, dst);
+    return hipGetLastError();
+}
+hipError_t idx_scatter_results(const float *dsub, const int *isub, int k, int n, const int *list,
+                               int Q, float *dist, int *idx, hipStream_t s) {
+    hipLaunchKernelGGL(scatter_results_kernel, dim3(grid_for((long long)n * k)), dim3(256), 0, s,
+                       dsub, isub, k, n, list, Q, dist, idx);
     return hipGetLastError();
 }
 hipError_t idx_make_tasks(const int *qstart, const int *counts, int nb, const int *block_start,
@@ -423,6 +613,45 @@ hipError_t idx_make_tasks(const int *qstart, const int *counts, int nb, const in
 }
 hipError_t idx_negate(const float *x, long long n, float *y, hipStream_t s) {
     hipLaunchKernelGGL(negate_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, n, y);
+    return hipGetLastError();
+}
+
+hipError_t idx_block_count(const int *fblock, const int *fcount, int nf, int *bcount,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(block_count_kernel, dim3(grid_for(nf)), dim3(256), 0, s, fblock, fcount, nf,
+                       bcount);
+    return hipGetLastError();
+}
+hipError_t idx_fine_start(const int *fblock, const int *fcount, int nf, int *bcursor, int *fstart,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(fine_start_kernel, dim3(grid_for(nf)), dim3(256), 0, s, fblock, fcount, nf,
+                       bcursor, fstart);
+    return hipGetLastError();
+}
+hipError_t idx_coarse_centroids(const float *fcent, int ld, int dim, const int *fine_by_block,
+                                const int *fb_start, const int *fcount, int nb, float *cent,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(coarse_centroid_kernel, dim3(nb), dim3(128), 0, s, fcent, ld, dim,
+                       fine_by_block, fb_start, fcount, cent);
+    return hipGetLastError();
+}
+hipError_t idx_fine_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
+                       const float *fcent, int nf, const float *fnorm, const double *frad,
+                       float *lbf, hipStream_t s) {
+    const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);
+    hipLaunchKernelGGL(fine_lb_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, qpm, pm_ld,
+                       Q, dim, qnorm, fcent, nf, fnorm, frad, lbf);
+    return hipGetLastError();
+}
+hipError_t idx_row_norms(const float *pm, int ld, int n, int dim, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(row_norms_kernel, dim3(grid_for(n)), dim3(256), 0, s, pm, ld, n, dim, out);
+    return hipGetLastError();
+}
+hipError_t idx_block_lb(const float *lbf, int Q, int nf, const int *fine_by_block,
+                        const int *fb_start, int nb, float *lbb, float *score, hipStream_t s) {
+    const long long n = (long long)Q * nb;
+    hipLaunchKernelGGL(block_lb_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lbf, Q,
+                       nf, fine_by_block, fb_start, nb, lbb, score);
     return hipGetLastError();
 }
 

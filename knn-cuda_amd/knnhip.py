@@ -43,6 +43,7 @@ C_SYMBOLS = (
     "knnhip_index_search",
     "knnhip_index_free",
     "knnhip_index_info",
+    "knnhip_index_stage_ms",
     "knnhip_version",
 )
 
@@ -123,6 +124,8 @@ def load() -> ctypes.CDLL:
         ctypes.POINTER(ctypes.c_float),
     ]
     lib.knnhip_index_info.restype = ctypes.c_int
+    lib.knnhip_index_stage_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+    lib.knnhip_index_stage_ms.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -277,6 +280,13 @@ class BlockIndex:
         ms = ctypes.c_float()
         load().knnhip_index_info(self._h, ctypes.byref(nb), ctypes.byref(npos), ctypes.byref(ms))
         return {"nblocks": nb.value, "positions": npos.value, "build_ms": ms.value}
+
+    STAGES = ("norms", "sub_ball_bounds", "block_bounds", "pick", "tasks", "scan", "select", "fallback")
+
+    def stage_ms(self) -> dict:
+        buf = (ctypes.c_float * 8)()
+        load().knnhip_index_stage_ms(self._h, buf, 8)
+        return dict(zip(self.STAGES, list(buf)))
 
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
                margin: int = 0, force_fallback: bool = False, timing: bool = False,
