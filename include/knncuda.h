@@ -100,15 +100,17 @@ int knnhip_generate_mixture(float *out, int n, int dim, int ncomp, float sigma,
 
 /*
  * Block index (router form).  No counterpart in the reference, which only
- * has the brute-force scan (code/knncuda.h).  The index partitions the
- * reference set into `nblocks` k-means blocks (`iters` Lloyd iterations) and
- * keeps a block-contiguous copy; a search scans each query's `probes`
- * highest-ranked blocks and skips the rest only where the triangle
- * inequality proves they hold no neighbour, so results equal
- * knn_cuda_global's exactly.  `scores` (query_nb x nblocks, row-major, device)
- * ranks blocks per query (e.g. router outputs); NULL ranks by the blocks'
- * distance lower bounds.  The index keeps a pointer to `ref` (the exact
- * fallback reads it): the caller keeps that buffer alive and unchanged.
+ * has the brute-force scan (code/knncuda.h).  The index covers the reference
+ * set with 4 * nblocks k-means balls (`iters` Lloyd iterations) grouped into
+ * `nblocks` blocks of balanced size, and keeps a block-contiguous copy; a
+ * search scans each query's `probes` highest-ranked blocks and skips the
+ * rest only where the triangle inequality (over the blocks' balls) proves
+ * they hold no neighbour, so results equal knn_cuda_global's exactly.
+ * `scores` (nblocks x query_nb, row-major, device: one column per query, like
+ * the point layout) ranks blocks per query, e.g. router outputs; NULL ranks
+ * by the blocks' distance lower bounds.  The index keeps a pointer to `ref`
+ * (the full-scan fallback reads it): the caller keeps that buffer alive and
+ * unchanged.  nblocks <= min(ref_nb, 32768).
  */
 typedef struct knnhip_index knnhip_index;
 int knnhip_index_build(const float *ref, int ref_nb, long long ref_ld, int dim, int nblocks,
@@ -119,9 +121,15 @@ int knnhip_index_search(knnhip_index *index, const float *query, int query_nb, l
 void knnhip_index_free(knnhip_index *index);
 int knnhip_index_info(const knnhip_index *index, int *nblocks, int *n_pos, float *build_ms);
 /* Device times (ms) of the last search made with options.timing = 1: query
- * norms, sub-ball bounds, block bounds, block choice, task build, scan,
- * select/re-rank, exact fallback (n <= 8 values written). */
+ * norms, block bounds, block choice, task build, scan, select/re-rank,
+ * full-scan fallback (n <= 7 values written). */
 int knnhip_index_stage_ms(const knnhip_index *index, float *ms, int n);
+/* Block layout of the index: block_start / block_count (host, nblocks ints
+ * each: first position and size of each block in the block-contiguous copy)
+ * and ref_block (device, ref_nb ints: the block holding each reference point,
+ * written on `stream`).  Any of the three may be NULL. */
+int knnhip_index_layout(const knnhip_index *index, int *block_start, int *block_count,
+                        int *ref_block, void *stream);
 
 /* Library version string. */
 const char *knnhip_version(void);

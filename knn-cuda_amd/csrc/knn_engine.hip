@@ -492,23 +492,26 @@ struct knnhip_index {
     const float *ref = nullptr;  // caller's dim-major references (exact fallback)
     long long ldr = 0;
     float build_ms = 0.f;
-    int nf = 0;  // fine clusters (sub-balls): each block is a union of them
     knnhip::DevBuf packed, pm, norms, perm, block_start, block_count, rmax;
-    knnhip::DevBuf fcent, fnorm, frad, fine_by_block, fb_start;
+    // fine balls in block-sorted order: centre, squared norm of the centre,
+    // radius (< 0: empty ball), block; block b's balls are fb_start[b] ..
+    // fb_start[b+1]
+    int nf = 0;
+    knnhip::DevBuf fcent, fnorm, frad, fblock, fb_start;
     // last search statistics
     int last_probes = 0;
     int last_fail = 0;
     long long last_tasks = 0;
     float last_ms[6] = {-1, -1, -1, -1, -1, -1};
-    float rank_ms[4] = {-1, -1, -1, -1};  // norms, fine bounds, block bounds, pick
+    float rank_ms[3] = {-1, -1, -1};  // query norms, block bounds, block choice
 };
 
 namespace knnhip {
 namespace {
 
-// Lloyd iterations on the point-major rows `pm` (n points) into `k`
-// centroids `cent` (point-major, ld pm_ld).  The assignment step is the
-// engine itself (centroids as the reference set, k = 1); the final
+// Lloyd iterations on the point-major rows `pm` (n points) from the `k`
+// centroids already in `cent` (point-major, ld pm_ld).  The assignment step
+// is the engine itself (centroids as the reference set, k = 1); the final
 // assignment is left in `assign`, its histogram in `counts`.
 bool kmeans(const float *pm, const float *dm, long long ldd, int n, int dim, int pm_ld, int k,
             int iters, float *cent, int *assign, int *counts, hipStream_t s) {
@@ -519,7 +522,6 @@ bool kmeans(const float *pm, const float *dm, long long ldd, int n, int dim, int
         set_error("Memory allocation error");
         return false;
     }
-    KNN_TRY(idx_init_centroids(pm, pm_ld, n, dim, k, cent, pm_ld, s), "k-means init");
     knnhip_options ao;
     memset(&ao, 0, sizeof(ao));
     ao.margin = 8;
@@ -542,26 +544,39 @@ bool kmeans(const float *pm, const float *dm, long long ldd, int n, int dim, int
     return true;
 }
 
-// Two-level index: `nf` fine clusters (k-means over the references) grouped
-// into `nb` blocks (k-means over the fine centroids, weighted by size).  The
-// references are stored block by block (each block starting at a multiple
-// of 32), each fine cluster contiguous inside its block; a block's lower
-// bound for a query is the smallest over its fine clusters' balls.
+// Block index build.
+//  1. Fine balls: k-means over the references with kBallsPerBlock * nb
+//     centres (k-means++ seeding on an evenly strided sample of kSeedPerCentre
+//     points per centre, then `iters` Lloyd steps with the engine as the
+//     assignment step).  Each ball's radius bounds its members.
+//  2. Blocks: nb seeds drawn by k-means++ over the ball centres; each ball
+//     joins the nearest of its kGroupCandidates nearest seeds that stays
+//     within kBlockSlack times the mean block size.
+//  3. Layout: references block by block (each block starting at a multiple
+//     of 32), each ball contiguous inside its block; balls sorted by block.
+// A block's lower bound for a query is the smallest over its balls.
+constexpr int kSeedPerCentre = 16;
+constexpr int kGroupCandidates = 32;
+constexpr double kBlockSlack = 1.25;
+constexpr unsigned long long kBallSeed = 0x6B4E4E5EEDBA11Full;
+constexpr unsigned long long kBlockSeed = 0x6B4E4E5EEDB10C5ull;
+
 bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int dim, int nb,
                  int iters, hipStream_t s) {
     hipEvent_t e0, e1;
     KNN_TRY(hipEventCreate(&e0), "event");
     KNN_TRY(hipEventCreate(&e1), "event");
     KNN_TRY(hipEventRecord(e0, s), "event");
-    constexpr int kSubBalls = 4;
-    const int nf = std::min(R, nb * kSubBalls);
     ix->R = R;
     ix->dim = dim;
     ix->nb = nb;
-    ix->nf = nf;
     ix->ref = ref;
     ix->ldr = ldr;
     ix->pm_ld = (dim + 3) & ~3;
+    if (ix->pm_ld > kIndexMaxLd) {
+        set_error("dimension too large for the block index (dim=%d, at most %d)", dim, kIndexMaxLd);
+        return false;
+    }
     knnhip_options o;
     memset(&o, 0, sizeof(o));
     {
@@ -577,16 +592,27 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
         ix->oct = p.oct;
     }
     const int pm_ld = ix->pm_ld;
-    DevBuf pm0, norms0, fassign, fcount, cent, fcent_dm, fblock, bcnt, cursor, fstart, tmp;
+    const int nf = (int)std::min<long long>(R, (long long)kBallsPerBlock * nb);
+    ix->nf = nf;
+    const int m = (int)std::min<long long>(R, (long long)kSeedPerCentre * nf);
+    const int kc = std::min(kGroupCandidates, nb);
+    const int cap = (int)std::ceil(kBlockSlack * (double)R / (double)nb);
+    const int mmax = std::max(m, nf);
+    DevBuf pm0, norms0, fassign, fcount, fcent, bcent, fcent_dm, bcent_dm, cand_d, cand_i, fblock,
+        bfine, cursor, fstart, fsorted, mind, cidx, wsum, ticket;
     if (!pm0.alloc((size_t)R * pm_ld * 4) || !norms0.alloc((size_t)R * 4) || !ix->rmax.alloc(4) ||
         !fassign.alloc((size_t)R * 4) || !fcount.alloc((size_t)nf * 4) ||
-        !ix->fcent.alloc((size_t)nf * pm_ld * 4) || !cent.alloc((size_t)nb * pm_ld * 4) ||
-        !fcent_dm.alloc((size_t)nf * dim * 4) || !fblock.alloc((size_t)nf * 4) ||
-        !bcnt.alloc((size_t)nb * 4) || !cursor.alloc((size_t)(std::max(nb, nf) + 1) * 4) ||
-        !fstart.alloc((size_t)nf * 4) || !tmp.alloc((size_t)(nb + 1) * 4) ||
-        !ix->block_start.alloc((size_t)(nb + 1) * 4) || !ix->block_count.alloc((size_t)nb * 4) ||
+        !fcent.alloc((size_t)nf * pm_ld * 4) || !bcent.alloc((size_t)nb * pm_ld * 4) ||
+        !fcent_dm.alloc((size_t)nf * dim * 4) || !bcent_dm.alloc((size_t)nb * dim * 4) ||
+        !cand_d.alloc((size_t)kc * nf * 4) || !cand_i.alloc((size_t)kc * nf * 4) ||
+        !fblock.alloc((size_t)nf * 4) || !bfine.alloc((size_t)nb * 4) ||
+        !cursor.alloc((size_t)(nb + 1) * 4) || !fstart.alloc((size_t)nf * 4) ||
+        !fsorted.alloc((size_t)nf * 4) || !mind.alloc((size_t)mmax * 4) ||
+        !cidx.alloc((size_t)std::max(nf, nb) * 4) || !wsum.alloc((size_t)kKppGroups * 8) ||
+        !ticket.alloc(4) || !ix->block_start.alloc((size_t)(nb + 1) * 4) ||
+        !ix->block_count.alloc((size_t)nb * 4) || !ix->fcent.alloc((size_t)nf * pm_ld * 4) ||
         !ix->frad.alloc((size_t)nf * 8) || !ix->fnorm.alloc((size_t)nf * 4) ||
-        !ix->fine_by_block.alloc((size_t)nf * 4) || !ix->fb_start.alloc((size_t)(nb + 1) * 4)) {
+        !ix->fblock.alloc((size_t)nf * 4) || !ix->fb_start.alloc((size_t)(nb + 1) * 4)) {
         set_error("Memory allocation error");
         return false;
     }
@@ -604,85 +630,77 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
     na.pm = pm0.as<float>();
     na.pm_ld = pm_ld;
     KNN_TRY(launch_norms(na, s), "norm kernel launch");
-    // fine level: k-means over the references
-    if (!kmeans(pm0.as<float>(), ref, ldr, R, dim, pm_ld, nf, iters, ix->fcent.as<float>(),
+    // 1. fine balls
+    KNN_TRY(idx_kpp_seed(pm0.as<float>(), pm_ld, dim, R, m, nf, kBallSeed, mind.as<float>(),
+                         cidx.as<int>(), wsum.as<double>(), ticket.as<unsigned>(),
+                         fcent.as<float>(), s),
+            "k-means++ seeding");
+    if (!kmeans(pm0.as<float>(), ref, ldr, R, dim, pm_ld, nf, iters, fcent.as<float>(),
                 fassign.as<int>(), fcount.as<int>(), s))
         return false;
-    // coarse level: k-means over the fine centroids (weighted by size)
-    KNN_TRY(idx_transpose(ix->fcent.as<float>(), pm_ld, nf, dim, fcent_dm.as<float>(), s),
-            "transpose");
+    // 2. blocks: seeds over the ball centres, then balanced grouping
+    KNN_TRY(idx_kpp_seed(fcent.as<float>(), pm_ld, dim, nf, nf, nb, kBlockSeed, mind.as<float>(),
+                         cidx.as<int>(), wsum.as<double>(), ticket.as<unsigned>(),
+                         bcent.as<float>(), s),
+            "k-means++ seeding");
+    KNN_TRY(idx_transpose(fcent.as<float>(), pm_ld, nf, dim, fcent_dm.as<float>(), s), "transpose");
+    KNN_TRY(idx_transpose(bcent.as<float>(), pm_ld, nb, dim, bcent_dm.as<float>(), s), "transpose");
     {
-        DevBuf cent_dm, adist;
-        if (!cent_dm.alloc((size_t)nb * dim * 4) || !adist.alloc((size_t)nf * 4)) {
-            set_error("Memory allocation error");
-            return false;
-        }
-        KNN_TRY(idx_init_centroids(ix->fcent.as<float>(), pm_ld, nf, dim, nb, cent.as<float>(),
-                                   pm_ld, s),
-                "k-means init");
         knnhip_options ao;
         memset(&ao, 0, sizeof(ao));
-        ao.margin = 8;
-        const Plan ap = make_plan(nb, nf, dim, 1, ao);
-        for (int it = 0; it <= iters; ++it) {
-            KNN_TRY(idx_transpose(cent.as<float>(), pm_ld, nb, dim, cent_dm.as<float>(), s),
-                    "transpose");
-            if (!run_device(cent_dm.as<float>(), nb, nb, fcent_dm.as<float>(), nf, nf, dim, 1,
-                            adist.as<float>(), fblock.as<int>(), ap, ao, s))
-                return false;
-            // fine clusters grouped by block
-            KNN_TRY(hipMemsetAsync(bcnt.p, 0, (size_t)nb * 4, s), "memset");
-            KNN_TRY(idx_histogram(fblock.as<int>(), nf, bcnt.as<int>(), s), "histogram");
-            KNN_TRY(idx_prefix(bcnt.as<int>(), nb, 1, ix->fb_start.as<int>(), nullptr, s), "prefix");
-            KNN_TRY(hipMemcpyAsync(cursor.p, ix->fb_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice,
-                                   s),
-                    "copy");
-            KNN_TRY(idx_scatter(fblock.as<int>(), nf, cursor.as<int>(),
-                                ix->fine_by_block.as<int>(), s),
-                    "scatter");
-            if (it == iters) break;
-            KNN_TRY(idx_coarse_centroids(ix->fcent.as<float>(), pm_ld, dim,
-                                         ix->fine_by_block.as<int>(), ix->fb_start.as<int>(),
-                                         fcount.as<int>(), nb, cent.as<float>(), s),
-                    "coarse centroids");
-        }
+        const Plan ap = make_plan(nb, nf, dim, kc, ao);
+        if (!run_device(bcent_dm.as<float>(), nb, nb, fcent_dm.as<float>(), nf, nf, dim, kc,
+                        cand_d.as<float>(), cand_i.as<int>(), ap, ao, s))
+            return false;
     }
-    // layout: blocks contiguous (starts aligned to 32), fine clusters
-    // contiguous inside their block
+    KNN_TRY(idx_group_fine(cand_i.as<int>(), kc, nf, fcount.as<int>(), nb, cap, fblock.as<int>(), s),
+            "grouping");
+    // 3. layout
     KNN_TRY(hipMemsetAsync(ix->block_count.p, 0, (size_t)nb * 4, s), "memset");
-    KNN_TRY(idx_block_count(fblock.as<int>(), fcount.as<int>(), nf, ix->block_count.as<int>(), s),
+    KNN_TRY(hipMemsetAsync(bfine.p, 0, (size_t)nb * 4, s), "memset");
+    KNN_TRY(idx_block_count(fblock.as<int>(), fcount.as<int>(), nf, ix->block_count.as<int>(),
+                            bfine.as<int>(), s),
             "block count");
     KNN_TRY(idx_prefix(ix->block_count.as<int>(), nb, kRefsPerTile, ix->block_start.as<int>(),
                        nullptr, s),
             "prefix");
+    KNN_TRY(idx_prefix(bfine.as<int>(), nb, 1, ix->fb_start.as<int>(), nullptr, s), "prefix");
     int n_pos = 0;
     KNN_TRY(hipMemcpyAsync(&n_pos, ix->block_start.as<int>() + nb, 4, hipMemcpyDeviceToHost, s),
             "copy");
     KNN_TRY(hipStreamSynchronize(s), "sync");
     ix->n_pos = n_pos;
     ix->r_pad = (n_pos + kRefsPerStep - 1) / kRefsPerStep * kRefsPerStep;
+    DevBuf slot_cursor;
     if (!ix->perm.alloc((size_t)ix->r_pad * 4) ||
         !ix->packed.alloc((size_t)ix->r_pad * ix->dpad * 4) ||
-        !ix->pm.alloc((size_t)ix->r_pad * pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4)) {
+        !ix->pm.alloc((size_t)ix->r_pad * pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4) ||
+        !slot_cursor.alloc((size_t)(nb + 1) * 4)) {
         set_error("Memory allocation error");
         return false;
     }
-    KNN_TRY(hipMemcpyAsync(tmp.p, ix->block_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+    KNN_TRY(hipMemcpyAsync(cursor.p, ix->block_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
             "copy");
-    KNN_TRY(idx_fine_start(fblock.as<int>(), fcount.as<int>(), nf, tmp.as<int>(), fstart.as<int>(), s),
-            "fine start");
-    KNN_TRY(hipMemcpyAsync(cursor.p, fstart.p, (size_t)nf * 4, hipMemcpyDeviceToDevice, s), "copy");
+    KNN_TRY(hipMemcpyAsync(slot_cursor.p, ix->fb_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+            "copy");
+    KNN_TRY(idx_fine_start(fblock.as<int>(), fcount.as<int>(), nf, cursor.as<int>(),
+                           fstart.as<int>(), slot_cursor.as<int>(), fsorted.as<int>(), s),
+            "ball ranges");
+    // positions: ball f's members go to fstart[f] .. fstart[f] + fcount[f]
+    KNN_TRY(hipMemcpyAsync(fcent_dm.p, fstart.p, (size_t)nf * 4, hipMemcpyDeviceToDevice, s),
+            "copy");  // fcent_dm is free now: scatter cursor
     KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)ix->perm.p, -1, ix->r_pad, s), "memset");
     KNN_TRY(hipMemsetAsync(ix->pm.p, 0, (size_t)ix->r_pad * pm_ld * 4, s), "memset");
-    KNN_TRY(idx_scatter(fassign.as<int>(), R, cursor.as<int>(), ix->perm.as<int>(), s), "scatter");
+    KNN_TRY(idx_scatter(fassign.as<int>(), R, fcent_dm.as<int>(), ix->perm.as<int>(), s), "scatter");
     KNN_TRY(idx_gather_pack(pm0.as<float>(), pm_ld, dim, ix->oct, ix->perm.as<int>(), n_pos,
                             ix->r_pad, ix->packed.as<float>(), ix->pm.as<float>(),
                             ix->norms.as<float>(), s),
             "gather");
-    // sub-ball radii and centroid norms
-    KNN_TRY(idx_radius(ix->pm.as<float>(), pm_ld, dim, fstart.as<int>(), fcount.as<int>(), nf,
-                       ix->fcent.as<float>(), pm_ld, ix->frad.as<double>(), s),
-            "radius");
+    // balls in block order: centres, radii over their member positions, norms
+    KNN_TRY(idx_finalize_fine(ix->pm.as<float>(), pm_ld, dim, fcent.as<float>(), fsorted.as<int>(),
+                              fstart.as<int>(), fcount.as<int>(), fblock.as<int>(), nf,
+                              ix->fcent.as<float>(), ix->frad.as<double>(), ix->fblock.as<int>(), s),
+            "balls");
     KNN_TRY(idx_row_norms(ix->fcent.as<float>(), pm_ld, nf, dim, ix->fnorm.as<float>(), s), "norms");
     KNN_TRY(hipEventRecord(e1, s), "event");
     KNN_TRY(hipEventSynchronize(e1), "event");
@@ -694,15 +712,20 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
 
 // Workspace of index searches (grown on demand, kept between calls).
 struct IndexWork {
-    DevBuf qnorm, qpm, lbf, lb, score, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks,
-        pval, pidx, theta, fb_count, fb_list, fb_key, fb_idx;
+    DevBuf qnorm, qpm, lb, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks,
+        pval, pidx, theta, fb_count, fb_list, qsub, dsub, isub;
     bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
-IndexWork g_iw;
+IndexWork g_iw, g_iw_retry;
 
-bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, int k, int P,
-                  const float *router_scores, float *out_dist, int *out_idx,
-                  const knnhip_options &o, hipStream_t s) {
+// Probes of the retry that queries failing their certificate get before the
+// full scan (ranked by the lower bounds alone).
+constexpr int kRetryProbes = 32;
+
+// depth 0: a caller's search; depth 1: the retry of its failed queries.
+bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query, int Q,
+                  long long ldq, int k, int P, const float *router_scores, float *out_dist,
+                  int *out_idx, const knnhip_options &o, hipStream_t s) {
     const int nb = ix->nb, dim = ix->dim, pm_ld = ix->pm_ld;
     const int margin = o.margin > 0 ? o.margin : kDefaultMargin;
     const int kp = std::max(k, std::min(k + margin, ix->R));
@@ -721,10 +744,8 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
         max_t[ph] = (int)(((long long)Q * (pr[ph + 1] - pr[ph]) + kQueriesPerWave - 1) /
                           kQueriesPerWave) + nb;
     const int max_tasks = max_t[0] + max_t[1];
-    IndexWork &w = g_iw;
     if (!w.need(w.qnorm, (size_t)Q * 4) || !w.need(w.qpm, (size_t)Q * pm_ld * 4) ||
-        !w.need(w.lbf, (size_t)Q * ix->nf * 4) ||
-        !w.need(w.lb, (size_t)Q * nb * 4) || !w.need(w.score, (size_t)Q * nb * 4) ||
+        !w.need(w.lb, (size_t)Q * nb * 4) ||
         !w.need(w.chosen, (size_t)npairs * 4) || !w.need(w.lb_rest, (size_t)Q * 4) ||
         !w.need(w.pcount, (size_t)nb * 4) || !w.need(w.qstart, (size_t)(nb + 1) * 4) ||
         !w.need(w.cursor, (size_t)(nb + 1) * 4) ||
@@ -736,7 +757,7 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
         set_error("Memory allocation error");
         return false;
     }
-    hipEvent_t ev[6], evr[4];
+    hipEvent_t ev[6], evr[2];
     const bool timing = o.timing != 0;
     if (timing)
         for (auto &e : evr) KNN_TRY(hipEventCreate(&e), "event");
@@ -756,18 +777,14 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
     nq.pm_ld = pm_ld;
     KNN_TRY(launch_norms(nq, s), "norm kernel launch");
     if (timing) KNN_TRY(hipEventRecord(evr[0], s), "event");
-    // lower bounds: every fine ball, then each block's smallest; the
-    // router-free ranking is the negated block bound (nearest block first)
-    KNN_TRY(idx_fine_lb(w.qpm.as<float>(), pm_ld, Q, dim, w.qnorm.as<float>(), ix->fcent.as<float>(),
-                        ix->nf, ix->fnorm.as<float>(), ix->frad.as<double>(), w.lbf.as<float>(), s),
-            "lower bounds");
-    if (timing) KNN_TRY(hipEventRecord(evr[1], s), "event");
-    KNN_TRY(idx_block_lb(w.lbf.as<float>(), Q, ix->nf, ix->fine_by_block.as<int>(),
-                         ix->fb_start.as<int>(), nb, w.lb.as<float>(),
-                         router_scores ? nullptr : w.score.as<float>(), s),
+    // block lower bounds (smallest over each block's balls), block-major;
+    // without a router the ranking is by the negated bound (nearest first)
+    KNN_TRY(idx_ball_lb(w.qpm.as<float>(), pm_ld, Q, dim, w.qnorm.as<float>(), ix->fcent.as<float>(),
+                        ix->nf, ix->fnorm.as<float>(), ix->frad.as<double>(), ix->fblock.as<int>(),
+                        ix->fb_start.as<int>(), nb, w.lb.as<float>(), s),
             "block bounds");
-    const float *score = router_scores ? router_scores : w.score.as<float>();
-    if (timing) KNN_TRY(hipEventRecord(evr[2], s), "event");
+    const float *score = router_scores;
+    if (timing) KNN_TRY(hipEventRecord(evr[1], s), "event");
     KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
                             w.chosen.as<int>(), w.lb_rest.as<float>(), s),
             "pick blocks");
@@ -851,14 +868,15 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
     fa.fb_list = w.fb_list.as<int>();
     KNN_TRY(launch_select(fa, s), "select kernel launch");
     if (timing) KNN_TRY(hipEventRecord(ev[4], s), "event");
-    // queries whose certificate failed: the plain exact search on just those
+    // queries whose certificate failed: once more through the index with
+    // more probes, then (or at once, on a retry) the plain full scan
     int nfail = 0;
     KNN_TRY(hipMemcpyAsync(&nfail, w.fb_count.p, 4, hipMemcpyDeviceToHost, s), "copy");
     KNN_TRY(hipStreamSynchronize(s), "sync");
     if (nfail > 0) {
-        DevBuf qsub, dsub, isub;
-        if (!qsub.alloc((size_t)nfail * dim * 4) || !dsub.alloc((size_t)nfail * k * 4) ||
-            !isub.alloc((size_t)nfail * k * 4)) {
+        DevBuf &qsub = w.qsub, &dsub = w.dsub, &isub = w.isub;
+        if (!w.need(qsub, (size_t)nfail * dim * 4) || !w.need(dsub, (size_t)nfail * k * 4) ||
+            !w.need(isub, (size_t)nfail * k * 4)) {
             set_error("Memory allocation error");
             return false;
         }
@@ -867,10 +885,17 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
         knnhip_options po = o;
         po.force_fallback = 0;
         po.timing = 0;
-        const Plan pp = make_plan(ix->R, nfail, dim, k, po);
-        if (!run_device(ix->ref, ix->ldr, ix->R, qsub.as<float>(), nfail, nfail, dim, k,
-                        dsub.as<float>(), isub.as<int>(), pp, po, s))
-            return false;
+        const int P2 = std::min({nb, kMaxUnion / kp, std::max(4 * P, kRetryProbes)});
+        if (depth == 0 && P2 > P) {
+            if (!index_search(ix, g_iw_retry, 1, qsub.as<float>(), nfail, nfail, k, P2, nullptr,
+                              dsub.as<float>(), isub.as<int>(), po, s))
+                return false;
+        } else {
+            const Plan pp = make_plan(ix->R, nfail, dim, k, po);
+            if (!run_device(ix->ref, ix->ldr, ix->R, qsub.as<float>(), nfail, nfail, dim, k,
+                            dsub.as<float>(), isub.as<int>(), pp, po, s))
+                return false;
+        }
         KNN_TRY(idx_scatter_results(dsub.as<float>(), isub.as<int>(), k, nfail, w.fb_list.as<int>(),
                                     Q, out_dist, out_idx, s),
                 "scatter");
@@ -887,8 +912,7 @@ bool index_search(knnhip_index *ix, const float *query, int Q, long long ldq, in
         for (int i = 0; i < 5; ++i) KNN_TRY(hipEventElapsedTime(&ix->last_ms[i], ev[i], ev[i + 1]), "event");
         KNN_TRY(hipEventElapsedTime(&ix->rank_ms[0], ev[0], evr[0]), "event");
         KNN_TRY(hipEventElapsedTime(&ix->rank_ms[1], evr[0], evr[1]), "event");
-        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[2], evr[1], evr[2]), "event");
-        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[3], evr[2], ev[1]), "event");
+        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[2], evr[1], ev[1]), "event");
         for (auto &e : ev) (void)hipEventDestroy(e);
         for (auto &e : evr) (void)hipEventDestroy(e);
         // stage slots: 0 ranking (norms, bounds, pick + tasks), 1 scan, 2 select, 3 fallback
@@ -1020,7 +1044,7 @@ int knnhip_index_build(const float *ref, int ref_nb, long long ref_ld, int dim, 
     std::lock_guard<std::mutex> lock(g_mu);
     g_last_error.clear();
     if (!out || !ref || ref_nb < 1 || dim < 1 || ref_ld < ref_nb || nblocks < 1 ||
-        nblocks > ref_nb || nblocks > 65536 || iters < 0) {
+        nblocks > ref_nb || nblocks > knnhip::kIndexMaxBlocks || iters < 0) {
         set_error("bad index arguments");
         return 0;
     }
@@ -1056,8 +1080,8 @@ int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long
              "{\"index_blocks\": %d, \"probes\": %d, \"router\": %s, \"query_nb\": %d, \"k\": %d}",
              ix->nb, std::min(probes, ix->nb), scores ? "true" : "false", query_nb, k);
     g_last_plan = buf;
-    return index_search(ix, query, query_nb, query_ld, k, probes, scores, knn_dist, knn_index, o,
-                        (hipStream_t)stream)
+    return index_search(ix, g_iw, 0, query, query_nb, query_ld, k, probes, scores, knn_dist,
+                        knn_index, o, (hipStream_t)stream)
                ? 1
                : 0;
 }
@@ -1069,9 +1093,9 @@ void knnhip_index_free(knnhip_index *ix) {
 
 int knnhip_index_stage_ms(const knnhip_index *ix, float *ms, int n) {
     if (!ix || !ms) return 0;
-    const float all[9] = {ix->rank_ms[0], ix->rank_ms[1], ix->rank_ms[2], ix->rank_ms[3],
-                          ix->last_ms[1], ix->last_ms[2], ix->last_ms[3], ix->last_ms[4], -1.f};
-    for (int i = 0; i < n && i < 9; ++i) ms[i] = all[i];
+    const float all[7] = {ix->rank_ms[0], ix->rank_ms[1], ix->rank_ms[2], ix->last_ms[1],
+                          ix->last_ms[2], ix->last_ms[3], ix->last_ms[4]};
+    for (int i = 0; i < n && i < 7; ++i) ms[i] = all[i];
     return 1;
 }
 
@@ -1080,6 +1104,32 @@ int knnhip_index_info(const knnhip_index *ix, int *nblocks, int *n_pos, float *b
     if (nblocks) *nblocks = ix->nb;
     if (n_pos) *n_pos = ix->n_pos;
     if (build_ms) *build_ms = ix->build_ms;
+    return 1;
+}
+
+int knnhip_index_layout(const knnhip_index *ix, int *block_start, int *block_count, int *ref_block,
+                        void *stream) {
+    if (!ix) {
+        set_error("null index");
+        return 0;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ensure_device()) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (ref_block)
+        KNN_TRY(idx_ref_block(ix->block_start.as<int>(), ix->block_count.as<int>(), ix->nb,
+                                ix->perm.as<int>(), ref_block, s),
+                  "block of each reference");
+    if (block_start)
+        KNN_TRY(hipMemcpyAsync(block_start, ix->block_start.p, (size_t)ix->nb * 4,
+                                 hipMemcpyDeviceToHost, s),
+                  "copy");
+    if (block_count)
+        KNN_TRY(hipMemcpyAsync(block_count, ix->block_count.p, (size_t)ix->nb * 4,
+                                 hipMemcpyDeviceToHost, s),
+                  "copy");
+    if (block_start || block_count) KNN_TRY(hipStreamSynchronize(s), "sync");
     return 1;
 }
 

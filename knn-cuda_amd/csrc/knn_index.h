@@ -5,21 +5,26 @@
 
 namespace knnhip {
 
+// Fine balls per block on average (the fine k-means has this many centres
+// per block).
+constexpr int kBallsPerBlock = 4;
+// Workgroups of a k-means++ seeding step.
+constexpr int kKppGroups = 256;
+// Largest padded dimension the block index supports (LDS of the seeding
+// kernel).
+constexpr int kIndexMaxLd = 4096;
+// Largest block count (LDS of the grouping kernel).
+constexpr int kIndexMaxBlocks = 32768;
+
 hipError_t idx_histogram(const int *assign, int n, int *counts, hipStream_t s);
 hipError_t idx_prefix(const int *counts, int nb, int align, int *start, int *plain, hipStream_t s);
 hipError_t idx_scatter(const int *assign, int n, int *cursor, int *perm, hipStream_t s);
 hipError_t idx_centroids(const float *pm, int pm_ld, int dim, const int *perm, const int *start,
                          const int *counts, int nb, float *cent, int cent_ld, hipStream_t s);
 hipError_t idx_transpose(const float *src, int ld, int nrows, int dim, float *dst, hipStream_t s);
-hipError_t idx_init_centroids(const float *pm, int pm_ld, int n, int dim, int nb, float *cent,
-                              int cent_ld, hipStream_t s);
 hipError_t idx_gather_pack(const float *pm_src, int pm_ld, int dim, int oct, const int *perm,
                            int n_pos, int n_pad, float *packed, float *pm_dst, float *norms,
                            hipStream_t s);
-hipError_t idx_radius(const float *pm, int pm_ld, int dim, const int *start, const int *counts,
-                      int nb, const float *cent, int cent_ld, double *radius, hipStream_t s);
-hipError_t idx_lower_bounds(const float *qpm, int pm_ld, int Q, int dim, const float *cent,
-                            int cent_ld, int nb, const double *radius, float *lb, hipStream_t s);
 hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float *lb,
                            const int *counts, int *chosen, float *lb_rest, hipStream_t s);
 hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int *counts,
@@ -32,19 +37,25 @@ hipError_t idx_scatter_results(const float *dsub, const int *isub, int k, int n,
                                int Q, float *dist, int *idx, hipStream_t s);
 hipError_t idx_make_tasks(const int *qstart, const int *counts, int nb, const int *block_start,
                           const int *block_count, ScanTask *tasks, int max_tasks, hipStream_t s);
-hipError_t idx_negate(const float *x, long long n, float *y, hipStream_t s);
-hipError_t idx_block_count(const int *fblock, const int *fcount, int nf, int *bcount,
-                           hipStream_t s);
-hipError_t idx_fine_start(const int *fblock, const int *fcount, int nf, int *bcursor, int *fstart,
-                          hipStream_t s);
-hipError_t idx_coarse_centroids(const float *fcent, int ld, int dim, const int *fine_by_block,
-                                const int *fb_start, const int *fcount, int nb, float *cent,
-                                hipStream_t s);
-hipError_t idx_fine_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
-                       const float *fcent, int nf, const float *fnorm, const double *frad,
-                       float *lbf, hipStream_t s);
+hipError_t idx_ref_block(const int *block_start, const int *block_count, int nb, const int *perm,
+                         int *out, hipStream_t s);
 hipError_t idx_row_norms(const float *pm, int ld, int n, int dim, float *out, hipStream_t s);
-hipError_t idx_block_lb(const float *lbf, int Q, int nf, const int *fine_by_block,
-                        const int *fb_start, int nb, float *lbb, float *score, hipStream_t s);
+
+hipError_t idx_kpp_seed(const float *pm, int ld, int dim, long long n, int m, int nb,
+                        unsigned long long seed, float *mind, int *cidx, double *wsum,
+                        unsigned *ticket, float *cent, hipStream_t s);
+hipError_t idx_group_fine(const int *cand, int kc, int nf, const int *fcount, int nb, int cap,
+                          int *fblock, hipStream_t s);
+hipError_t idx_block_count(const int *fblock, const int *fcount, int nf, int *bcount, int *bfine,
+                           hipStream_t s);
+hipError_t idx_fine_start(const int *fblock, const int *fcount, int nf, int *pos_cursor,
+                          int *fstart, int *slot_cursor, int *fine_sorted, hipStream_t s);
+hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcent,
+                             const int *fine_sorted, const int *fstart, const int *fcount,
+                             const int *fblock, int nf, float *cent_s, double *rad_s, int *block_s,
+                             hipStream_t s);
+hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
+                       const float *fcent, int nf, const float *fnorm, const double *frad,
+                       const int *block_s, const int *fb_start, int nb, float *lb, hipStream_t s);
 
 }  // namespace knnhip

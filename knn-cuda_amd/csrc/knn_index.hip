@@ -6,8 +6,9 @@
 // The result stays the reference ground truth (code/test.cpp:129-175) bit
 // for bit: the scanned candidates go through the same exact re-rank and
 // certificate as the plain path, and the certificate also requires every
-// unscanned block's lower bound max(0, |q - c_B| - rho_B)^2 to clear the k-th
-// key; a query that fails is answered by the exact full scan.
+// unscanned block's lower bound (the smallest max(0, |q - c_f| - rho_f)^2
+// over the sub-balls f covering the block) to clear the k-th key; a query
+// that fails is answered by the plain full scan.
 #include <float.h>
 
 #include "knn_common.h"
@@ -103,18 +104,6 @@ __global__ void transpose_kernel(const float *__restrict__ src, int ld, int nrow
     }
 }
 
-// Initial centroids: references at an even stride.
-__global__ void init_centroids_kernel(const float *__restrict__ pm, int pm_ld, int n, int dim,
-                                      int nb, float *__restrict__ cent, int cent_ld) {
-    const long long total = (long long)nb * dim;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const long long b = e / dim, d = e - b * dim;
-        const long long src = (b * (long long)n) / nb;
-        cent[b * cent_ld + d] = pm[src * pm_ld + d];
-    }
-}
-
 // Reordered layouts: position i of the block-contiguous order holds
 // reference perm[i] (or nothing: perm[i] < 0 pads a block to 32).  Writes the
 // packed tiles (as the norms pass does), the point-major copy and the norms.
@@ -150,91 +139,115 @@ __global__ void gather_pack_kernel(const float *__restrict__ pm_src, int pm_ld, 
     if (i < n_pos) norms[i] = s;
 }
 
-// Block radius rho_B = max over members of |r - c_B|, in double, then
-// inflated by a relative 1e-12 so that it bounds the real distance.
-__global__ void radius_kernel(const float *__restrict__ pm, int pm_ld, int dim,
-                              const int *__restrict__ start, const int *__restrict__ counts,
-                              const float *__restrict__ cent, int cent_ld,
-                              double *__restrict__ radius) {
-    const int b = blockIdx.x;
-    const int n = counts[b];
-    const int s0 = start[b];
-    const float *c = cent + (long long)b * cent_ld;
-    double best = 0.0;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const float *r = pm + (long long)(s0 + j) * pm_ld;
-        double acc = 0.0;
-        for (int d = 0; d < dim; ++d) {
-            const double t = (double)r[d] - (double)c[d];
-            acc += t * t;
-        }
-        best = fmax(best, acc);
-    }
-    __shared__ double red[256];
-    red[threadIdx.x] = best;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) radius[b] = sqrt(red[0]) * (1.0 + 1e-12);
-}
-
 // ---------------------------------------------------------------------------
 // Search support
 // ---------------------------------------------------------------------------
 
-// Lower bound of every reference of block B for query q, rounded down to
-// float: max(0, |q - c_B| - rho_B), |q - c_B| in double.  One thread per
-// (query, block); lb is [q][nb].
-__global__ void lower_bound_kernel(const float *__restrict__ qpm, int pm_ld, int Q, int dim,
-                                   const float *__restrict__ cent, int cent_ld, int nb,
-                                   const double *__restrict__ radius, float *__restrict__ lb) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)Q * nb) return;
-    const int q = (int)(e / nb), b = (int)(e - (long long)q * nb);
-    const float *qr = qpm + (long long)q * pm_ld;
-    const float *c = cent + (long long)b * cent_ld;
-    double acc = 0.0;
-    for (int d = 0; d < dim; ++d) {
-        const double t = (double)qr[d] - (double)c[d];
-        acc += t * t;
-    }
-    const double v = sqrt(acc) * (1.0 - 1e-12) - radius[b];
-    float f = (float)v;
-    if ((double)f > v) f = nextafterf(f, -INFINITY);
-    lb[e] = f > 0.0f ? f : 0.0f;
+// Per query: the P blocks of largest score (ties: lower block), and the
+// smallest lower bound over the blocks not taken.  score and lb are
+// block-major ([b][q], query_nb per row); score == nullptr ranks by -lb
+// (nearest block first); a NaN score ranks last.  A workgroup serves 64
+// queries (one per lane, so reads are whole rows) with kPickWaves waves,
+// each over its share of the blocks; every round each wave offers its best
+// block strictly after the previous pick and wave 0 keeps the best offer.
+// Blocks without members are never taken (they hold no references).
+constexpr int kPickWaves = 16;
+// At most this many queries, one workgroup per query instead.
+constexpr int kPickSingleMaxQ = 1024;
+
+__device__ inline float rank_score(const float *__restrict__ score, const float *__restrict__ lb,
+                                   long long e) {
+    const float s = score ? score[e] : -lb[e];
+    return s == s ? s : -INFINITY;
 }
 
-// Per query: the P blocks of largest score (ties: lower block), and the
-// smallest lower bound over the blocks not taken.  One wave per query.
-// Blocks without members are never taken (they hold no references).
-__global__ __launch_bounds__(256) void pick_blocks_kernel(const float *__restrict__ score, int Q,
-                                                          int nb, int P,
-                                                          const float *__restrict__ lb,
-                                                          const int *__restrict__ counts,
-                                                          int *__restrict__ chosen,
-                                                          float *__restrict__ lb_rest) {
+__global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_kernel(
+    const float *__restrict__ score, int Q, int nb, int P, const float *__restrict__ lb,
+    const int *__restrict__ counts, int *__restrict__ chosen, float *__restrict__ lb_rest) {
+    __shared__ float offer_s[kPickWaves][64];
+    __shared__ int offer_b[kPickWaves][64];
+    __shared__ float keep_s[64];
+    __shared__ int keep_b[64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int q = blockIdx.x * (blockDim.x >> 6) + wave;
-    if (q >= Q) return;
-    const float *sc = score + (long long)q * nb;
-    const float *l = lb + (long long)q * nb;
-    unsigned long long taken_mask_dummy = 0;
-    (void)taken_mask_dummy;
-    // Per round: wave argmax over blocks not yet taken.
+    const int q = blockIdx.x * 64 + lane;
+    const bool valid = q < Q;
+    const int per = (nb + kPickWaves - 1) / kPickWaves;
+    const int b0 = min(nb, wave * per), b1 = min(nb, b0 + per);
+    // (last_s, last_b): the previous pick; "after" = lower score, or equal
+    // score and higher block
     float last_s = INFINITY;
     int last_b = -1;
     for (int p = 0; p < P; ++p) {
         float bs = -INFINITY;
-        int bb = 0x7FFFFFFF;
-        for (int b = lane; b < nb; b += kWave) {
+        int bb = -1;
+        if (valid)
+            for (int b = b0; b < b1; ++b) {
+                if (counts[b] == 0) continue;
+                const float s = rank_score(score, lb, (long long)b * Q + q);
+                const bool after = (s < last_s) || (s == last_s && b > last_b);
+                if (after && (bb < 0 || s > bs)) {
+                    bs = s;
+                    bb = b;
+                }
+            }
+        offer_s[wave][lane] = bs;
+        offer_b[wave][lane] = bb;
+        __syncthreads();
+        if (wave == 0) {
+            float ks = -INFINITY;
+            int kb = -1;
+            for (int w = 0; w < kPickWaves; ++w) {
+                const int ob = offer_b[w][lane];
+                const float os = offer_s[w][lane];
+                if (ob >= 0 && (kb < 0 || os > ks || (os == ks && ob < kb))) {
+                    ks = os;
+                    kb = ob;
+                }
+            }
+            if (valid) chosen[(long long)q * P + p] = kb;
+            // nothing left: no block is after (-inf, INT_MAX)
+            keep_s[lane] = kb >= 0 ? ks : -INFINITY;
+            keep_b[lane] = kb >= 0 ? kb : 0x7FFFFFFF;
+        }
+        __syncthreads();
+        last_s = keep_s[lane];
+        last_b = keep_b[lane];
+    }
+    // smallest lower bound over the blocks after the last pick
+    float m = INFINITY;
+    if (valid)
+        for (int b = b0; b < b1; ++b) {
             if (counts[b] == 0) continue;
-            const float s = sc[b];
-            // strictly after the previous pick in (score desc, block asc) order
+            const float s = rank_score(score, lb, (long long)b * Q + q);
             const bool after = (s < last_s) || (s == last_s && b > last_b);
-            if (!after) continue;
-            if (s > bs || (s == bs && b < bb)) {
+            if (after) m = fminf(m, lb[(long long)b * Q + q]);
+        }
+    offer_s[wave][lane] = m;
+    __syncthreads();
+    if (wave == 0 && valid) {
+        for (int w = 1; w < kPickWaves; ++w) m = fminf(m, offer_s[w][lane]);
+        lb_rest[q] = m;
+    }
+}
+
+// The same selection for few queries: one workgroup per query, its threads
+// spread over the blocks, so each round is a workgroup argmax.
+__global__ __launch_bounds__(256) void pick_blocks_single_kernel(
+    const float *__restrict__ score, int Q, int nb, int P, const float *__restrict__ lb,
+    const int *__restrict__ counts, int *__restrict__ chosen, float *__restrict__ lb_rest) {
+    __shared__ float red_s[4];
+    __shared__ int red_b[4];
+    const int q = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    float last_s = INFINITY;
+    int last_b = -1;
+    for (int p = 0; p < P; ++p) {
+        float bs = -INFINITY;
+        int bb = -1;
+        for (int b = tid; b < nb; b += blockDim.x) {
+            if (counts[b] == 0) continue;
+            const float s = rank_score(score, lb, (long long)b * Q + q);
+            const bool after = (s < last_s) || (s == last_s && b > last_b);
+            if (after && (bb < 0 || s > bs)) {
                 bs = s;
                 bb = b;
             }
@@ -243,27 +256,46 @@ __global__ __launch_bounds__(256) void pick_blocks_kernel(const float *__restric
         for (int off = 32; off >= 1; off >>= 1) {
             const float os = __shfl_xor(bs, off);
             const int ob = __shfl_xor(bb, off);
-            if (os > bs || (os == bs && ob < bb)) {
+            if (ob >= 0 && (bb < 0 || os > bs || (os == bs && ob < bb))) {
                 bs = os;
                 bb = ob;
             }
         }
-        if (lane == 0) chosen[(long long)q * P + p] = bb == 0x7FFFFFFF ? -1 : bb;
-        last_s = bs;
-        last_b = bb;
+        if (lane == 0) {
+            red_s[wave] = bs;
+            red_b[wave] = bb;
+        }
+        __syncthreads();
+        float ks = -INFINITY;
+        int kb = -1;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            const int ob = red_b[w];
+            const float os = red_s[w];
+            if (ob >= 0 && (kb < 0 || os > ks || (os == ks && ob < kb))) {
+                ks = os;
+                kb = ob;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) chosen[(long long)q * P + p] = kb;
+        last_s = kb >= 0 ? ks : -INFINITY;
+        last_b = kb >= 0 ? kb : 0x7FFFFFFF;
     }
-    // smallest lower bound over the blocks not taken (those after the last
-    // pick in the same order), empty blocks excluded
     float m = INFINITY;
-    for (int b = lane; b < nb; b += kWave) {
+    for (int b = tid; b < nb; b += blockDim.x) {
         if (counts[b] == 0) continue;
-        const float s = sc[b];
+        const float s = rank_score(score, lb, (long long)b * Q + q);
         const bool after = (s < last_s) || (s == last_s && b > last_b);
-        if (after) m = fminf(m, l[b]);
+        if (after) m = fminf(m, lb[(long long)b * Q + q]);
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) m = fminf(m, __shfl_xor(m, off));
-    if (lane == 0) lb_rest[q] = m;
+    if (lane == 0) red_s[wave] = m;
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fminf(m, red_s[w]);
+        lb_rest[q] = m;
+    }
 }
 
 // Invert the (query -> blocks) lists, probes [p0, p1) of each query's P, into
@@ -361,132 +393,16 @@ __global__ void clear_tasks_kernel(ScanTask *__restrict__ tasks, int n) {
     }
 }
 
-// Negated lower bound as the ranking score of the router-free search
-// (nearest block first).
-__global__ void negate_kernel(const float *__restrict__ x, long long n, float *__restrict__ y) {
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (long long)gridDim.x * blockDim.x)
-        y[e] = -x[e];
-}
-
-// ---------------------------------------------------------------------------
-// Two-level layout: fine clusters (sub-balls) grouped into blocks
-// ---------------------------------------------------------------------------
-
-// bcount[fblock[f]] += fcount[f]
-__global__ void block_count_kernel(const int *__restrict__ fblock, const int *__restrict__ fcount,
-                                   int nf, int *__restrict__ bcount) {
-    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x)
-        atomicAdd(&bcount[fblock[f]], fcount[f]);
-}
-
-// Each fine cluster takes a contiguous range inside its block.
-__global__ void fine_start_kernel(const int *__restrict__ fblock, const int *__restrict__ fcount,
-                                  int nf, int *__restrict__ bcursor, int *__restrict__ fstart) {
-    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x)
-        fstart[f] = atomicAdd(&bcursor[fblock[f]], fcount[f]);
-}
-
-// Coarse centroid = mean of its fine centroids weighted by their sizes (a
-// block with no members keeps its centroid).  One block per coarse cluster;
-// members listed by fine_by_block[fb_start[b] .. fb_start[b+1]).
-__global__ void coarse_centroid_kernel(const float *__restrict__ fcent, int ld, int dim,
-                                       const int *__restrict__ fine_by_block,
-                                       const int *__restrict__ fb_start,
-                                       const int *__restrict__ fcount, float *__restrict__ cent) {
+// Block of every reference point (original order): one workgroup per block
+// walks the block's positions.  Padding positions (perm < 0) are skipped.
+__global__ void ref_block_kernel(const int *__restrict__ block_start,
+                                 const int *__restrict__ block_count, const int *__restrict__ perm,
+                                 int *__restrict__ out) {
     const int b = blockIdx.x;
-    const int f0 = fb_start[b], f1 = fb_start[b + 1];
-    long long w = 0;
-    for (int j = f0; j < f1; ++j) w += fcount[fine_by_block[j]];
-    if (w == 0) return;
-    for (int d = threadIdx.x; d < dim; d += blockDim.x) {
-        double acc = 0.0;
-        for (int j = f0; j < f1; ++j) {
-            const int f = fine_by_block[j];
-            acc += (double)fcent[(long long)f * ld + d] * (double)fcount[f];
-        }
-        cent[(long long)b * ld + d] = (float)(acc / (double)w);
-    }
-}
-
-// Lower bound of every reference of fine cluster f for query q (distance,
-// not squared, rounded down): max(0, |q - c_f| - rho_f).  |q - c_f|^2 comes
-// from ||q||^2 + ||c||^2 - 2 q.c on FP32 MFMA, lowered by the error bound
-// g*(||q||^2 + ||c||^2 + 2|q||c|), g = 2*(dim+4)*2^-24.  One wave computes a
-// 32 (clusters) x 32 (queries) tile; k-step kk reads dimension kk + h*KH on
-// lane half h (KH = half the padded dimension), 32 k-steps per chunk.
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
-
-__global__ __launch_bounds__(256) void fine_lb_kernel(const float *__restrict__ qpm, int pm_ld, int Q,
-                                                      int dim, const float *__restrict__ qnorm,
-                                                      const float *__restrict__ fcent, int nf,
-                                                      const float *__restrict__ fnorm,
-                                                      const double *__restrict__ frad,
-                                                      float *__restrict__ lbf) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int col = lane & 31, h = lane >> 5;
-    const int ntf = (nf + 31) / 32;
-    const long long tile = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
-    const int tq = (int)(tile / ntf), tf = (int)(tile - (long long)tq * ntf);
-    const int q0 = tq * 32, f0 = tf * 32;
-    if (q0 >= Q) return;
-    const int qa = min(q0 + col, Q - 1);  // B operand: query col
-    const int fa = min(f0 + col, nf - 1); // A operand: cluster col
-    const float *qr = qpm + (long long)qa * pm_ld;
-    const float *cr = fcent + (long long)fa * pm_ld;
-    const int KH = (dim + 1) / 2;
-    f32x16_t acc = {};
-    if ((KH & 3) == 0 && (pm_ld & 3) == 0) {
-        // each lane reads its own row 16 bytes at a time (dimensions kk..kk+3 of its half)
-        const float *ca = cr + h * KH, *qa4 = qr + h * KH;
-        for (int k0 = 0; k0 < KH; k0 += 16) {
-            float4 av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const bool ok = k0 + 4 * u < KH;
-                av[u] = ok ? *reinterpret_cast<const float4 *>(ca + k0 + 4 * u) : make_float4(0, 0, 0, 0);
-                bv[u] = ok ? *reinterpret_cast<const float4 *>(qa4 + k0 + 4 * u) : make_float4(0, 0, 0, 0);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].x, bv[u].x, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].y, bv[u].y, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].z, bv[u].z, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].w, bv[u].w, acc, 0, 0, 0);
-            }
-        }
-    } else {
-        for (int k0 = 0; k0 < KH; k0 += 32) {
-            float av[32], bv[32];
-#pragma unroll
-            for (int u = 0; u < 32; ++u) {
-                const int kk = k0 + u;
-                const int d = kk + h * KH;
-                const bool ok = kk < KH && d < dim;
-                av[u] = ok ? cr[d] : 0.0f;
-                bv[u] = ok ? qr[d] : 0.0f;
-            }
-#pragma unroll
-            for (int u = 0; u < 32; ++u)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bv[u], acc, 0, 0, 0);
-        }
-    }
-    const int q = q0 + col;
-    if (q >= Q) return;
-    const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
-    const double nq = (double)qnorm[q];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int f = f0 + ((i >> 2) << 3) + (h << 2) + (i & 3);
-        if (f >= nf) continue;
-        const double nc = (double)fnorm[f];
-        const double dot = (double)acc[i];
-        double d2 = nq + nc - 2.0 * dot - g * (nq + nc + 2.0 * sqrt(nq * nc)) * 1.5;
-        d2 = d2 > 0.0 ? d2 : 0.0;
-        const double v = sqrt(d2) * (1.0 - 1e-12) - frad[f];
-        float fv = (float)v;
-        if ((double)fv > v) fv = nextafterf(fv, -INFINITY);
-        lbf[(long long)q * nf + f] = fv > 0.0f ? fv : 0.0f;
+    const int s0 = block_start[b], n = block_count[b];
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const int r = perm[s0 + p];
+        if (r >= 0) out[r] = b;
     }
 }
 
@@ -500,20 +416,427 @@ __global__ void row_norms_kernel(const float *__restrict__ pm, int ld, int n, in
     }
 }
 
-// Block lower bound = min over the block's fine clusters; negated as the
-// router-free ranking score.  lbb and score are [q][nb].
-__global__ void block_lb_kernel(const float *__restrict__ lbf, int Q, int nf,
-                                const int *__restrict__ fine_by_block,
-                                const int *__restrict__ fb_start, int nb, float *__restrict__ lbb,
-                                float *__restrict__ score) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)Q * nb) return;
-    const int q = (int)(e / nb), b = (int)(e - (long long)q * nb);
+// ---------------------------------------------------------------------------
+// k-means++ seeding (D^2 sampling, Arthur & Vassilvitskii, SODA 2007)
+// ---------------------------------------------------------------------------
+
+// splitmix64 finaliser of x, as a double uniform in [0, 1).
+__device__ inline double hash_unit(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (double)(x >> 11) * 0x1.0p-53;
+}
+
+// Squared distance of two point-major rows over `dim` coordinates (FP32;
+// rows are 16-byte aligned, pad coordinates are not read).
+__device__ inline float row_sqdist(const float *__restrict__ a, const float *__restrict__ b, int dim) {
+    float s = 0.0f;
+    int d = 0;
+    for (; d + 4 <= dim; d += 4) {
+        const float4 x = *reinterpret_cast<const float4 *>(a + d);
+        const float4 y = *reinterpret_cast<const float4 *>(b + d);
+        const float t0 = x.x - y.x, t1 = x.y - y.y, t2 = x.z - y.z, t3 = x.w - y.w;
+        s = fmaf(t0, t0, s);
+        s = fmaf(t1, t1, s);
+        s = fmaf(t2, t2, s);
+        s = fmaf(t3, t3, s);
+    }
+    for (; d < dim; ++d) {
+        const float t = a[d] - b[d];
+        s = fmaf(t, t, s);
+    }
+    return s;
+}
+
+// One wave: weighted pick over v[0..n) (v >= 0).  Lane l sums the
+// contiguous range [l*per, (l+1)*per); the pick is the first index whose
+// running sum exceeds the target (target = u * total if `relative`, else
+// u itself), or the last positive entry if rounding leaves none; -1 when
+// every entry is 0.  *before receives the running sum ahead of the pick.
+// *target_out (if given) receives the target.  All lanes return the same
+// values.
+template <class T>
+__device__ int wave_pick(const T *__restrict__ v, int n, double u, bool relative, double *before,
+                         double *target_out) {
+    const int lane = threadIdx.x & 63;
+    const int per = (n + 63) / 64;
+    const int i0 = min(n, lane * per), i1 = min(n, i0 + per);
+    double s = 0.0;
+    int lastpos = -1;
+    for (int i = i0; i < i1; ++i) {
+        const double x = (double)v[i];
+        s += x;
+        if (x > 0.0) lastpos = i;
+    }
+    double incl = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    double excl = __shfl_up(incl, 1);
+    if (lane == 0) excl = 0.0;
+    const double total = __shfl(incl, 63);
+    const double target = relative ? u * total : u;
+    if (target_out) *target_out = target;
+    const unsigned long long cross = __ballot(i1 > i0 && incl > target);
+    int pick = -1;
+    double pb = 0.0;
+    if (cross) {
+        const int L = __ffsll((long long)cross) - 1;
+        int p = -1;
+        double b = 0.0;
+        if (lane == L) {
+            double run = excl;
+            int lp = -1;
+            double lpb = 0.0;
+            for (int i = i0; i < i1; ++i) {
+                const double x = (double)v[i];
+                if (x > 0.0) {
+                    lp = i;
+                    lpb = run;
+                }
+                if (run + x > target && x > 0.0) {
+                    p = i;
+                    b = run;
+                    break;
+                }
+                run += x;
+            }
+            if (p < 0) {
+                p = lp;
+                b = lpb;
+            }
+        }
+        pick = __shfl(p, L);
+        pb = __shfl(b, L);
+    }
+    if (pick < 0) {
+        const unsigned long long has = __ballot(lastpos >= 0);
+        if (has) {
+            const int L = 63 - __clzll((long long)has);
+            pick = __shfl(lastpos, L);
+            pb = __shfl(excl, L);
+        }
+    }
+    if (before) *before = pb;
+    return pick;
+}
+
+// Sample i of the seeding set is point floor(i * n / m) (an even stride).
+__device__ inline long long sample_point(int i, long long n, int m) { return (long long)i * n / m; }
+
+__global__ void kpp_init_kernel(float *__restrict__ mind, int m, int *__restrict__ cidx,
+                                unsigned *__restrict__ ticket, unsigned long long seed) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        mind[i] = INFINITY;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cidx[0] = min(m - 1, (int)(hash_unit(seed) * m));
+        *ticket = 0u;
+    }
+}
+
+// Step t: fold centre t-1 into every sample's squared distance to its
+// nearest centre, then (in the workgroup that finishes last) draw centre t
+// with probability proportional to that distance.  Each workgroup owns a
+// contiguous sample range and publishes its sum; the draw picks a range by
+// those sums and a sample inside it.
+__global__ __launch_bounds__(256) void kpp_step_kernel(const float *__restrict__ pm, int ld, int dim,
+                                                       long long n, int m, float *__restrict__ mind,
+                                                       int *__restrict__ cidx, int t,
+                                                       unsigned long long seed,
+                                                       double *__restrict__ wsum,
+                                                       unsigned *__restrict__ ticket) {
+    extern __shared__ float csh[];  // the new centre, ld floats
+    __shared__ double red[256];
+    __shared__ int is_last;
+    const int tid = threadIdx.x;
+    const float *c = pm + sample_point(cidx[t - 1], n, m) * ld;
+    for (int d = tid; d < ld; d += blockDim.x) csh[d] = c[d];
+    __syncthreads();
+    const int per = (m + gridDim.x - 1) / gridDim.x;
+    const int i0 = min(m, (int)blockIdx.x * per), i1 = min(m, i0 + per);
+    double acc = 0.0;
+    for (int i = i0 + tid; i < i1; i += blockDim.x) {
+        const float d2 = row_sqdist(pm + sample_point(i, n, m) * ld, csh, dim);
+        const float v = fminf(mind[i], d2);
+        mind[i] = v;
+        acc += (double)v;
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) wsum[blockIdx.x] = red[0];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) is_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    if (tid < 64) {
+        const double u = hash_unit(seed ^ ((unsigned long long)t * 0xD1B54A32D192ED03ull));
+        double before = 0.0, target = 0.0;
+        const int w = wave_pick(wsum, gridDim.x, u, true, &before, &target);
+        int pick = t % m;  // every sample already a centre: any choice
+        if (w >= 0) {
+            const int j0 = min(m, w * per), j1 = min(m, j0 + per);
+            const int r = wave_pick(mind + j0, j1 - j0, target - before, false, nullptr, nullptr);
+            pick = r >= 0 ? j0 + r : j0;
+        }
+        if (tid == 0) {
+            cidx[t] = pick;
+            *ticket = 0u;
+        }
+    }
+}
+
+// Centres from the drawn samples: cent[b] = row of sample cidx[b].
+__global__ void kpp_gather_kernel(const float *__restrict__ pm, int ld, long long n, int m,
+                                  const int *__restrict__ cidx, int nb, float *__restrict__ cent) {
+    const long long total = (long long)nb * ld;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long b = e / ld, d = e - b * ld;
+        cent[e] = pm[sample_point(cidx[b], n, m) * ld + d];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blocks as balanced groups of fine balls
+// ---------------------------------------------------------------------------
+
+// Fine ball f (in index order) joins the nearest of its candidate blocks
+// cand[j * nf + f], j < kc (ascending distance, as the engine returns them)
+// whose load stays within `cap` points with it; when none has room, the
+// least-loaded candidate (ties: nearer).  Empty balls join their nearest
+// candidate.  One wave, sequential over the balls; loads live in LDS.
+__global__ __launch_bounds__(64) void group_fine_kernel(const int *__restrict__ cand, int kc, int nf,
+                                                        const int *__restrict__ fcount, int nb,
+                                                        int cap, int *__restrict__ fblock) {
+    extern __shared__ int load[];
+    const int lane = threadIdx.x;
+    for (int b = lane; b < nb; b += 64) load[b] = 0;
+    __syncthreads();
+    for (int f = 0; f < nf; ++f) {
+        const int c = fcount[f];
+        const int b = lane < kc ? cand[(long long)lane * nf + f] : -1;
+        const int l = b >= 0 ? load[b] : 0x7FFFFFFF;
+        const unsigned long long fits = __ballot(b >= 0 && (c == 0 || l <= cap - c));
+        int pick;
+        if (fits) {
+            pick = __shfl(b, __ffsll((long long)fits) - 1);
+        } else {
+            int bl = l, bj = lane;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const int ol = __shfl_xor(bl, off), oj = __shfl_xor(bj, off);
+                if (ol < bl || (ol == bl && oj < bj)) {
+                    bl = ol;
+                    bj = oj;
+                }
+            }
+            pick = __shfl(b, bj);
+        }
+        if (lane == 0) {
+            fblock[f] = pick;
+            load[pick] += c;
+        }
+        __syncthreads();
+    }
+}
+
+// Per block: point count and fine-ball count.
+__global__ void block_count_kernel(const int *__restrict__ fblock, const int *__restrict__ fcount,
+                                   int nf, int *__restrict__ bcount, int *__restrict__ bfine) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        atomicAdd(&bcount[fblock[f]], fcount[f]);
+        atomicAdd(&bfine[fblock[f]], 1);
+    }
+}
+
+// Each fine ball takes a contiguous range of positions inside its block
+// (fstart) and a slot in the block-sorted ball order (fine_sorted).
+__global__ void fine_start_kernel(const int *__restrict__ fblock, const int *__restrict__ fcount,
+                                  int nf, int *__restrict__ pos_cursor, int *__restrict__ fstart,
+                                  int *__restrict__ slot_cursor, int *__restrict__ fine_sorted) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        fstart[f] = atomicAdd(&pos_cursor[fblock[f]], fcount[f]);
+        fine_sorted[atomicAdd(&slot_cursor[fblock[f]], 1)] = f;
+    }
+}
+
+// Ball s of the block-sorted order (fine ball f = fine_sorted[s]): its
+// centre row, block, and radius = max member distance (double, widened by
+// a relative 1e-12 so that it bounds the exact distance; -1 when empty).
+// One workgroup per ball; members are positions fstart[f] .. + fcount[f].
+__global__ __launch_bounds__(256) void finalize_fine_kernel(const float *__restrict__ pm, int ld,
+                                                            int dim, const float *__restrict__ fcent,
+                                                            const int *__restrict__ fine_sorted,
+                                                            const int *__restrict__ fstart,
+                                                            const int *__restrict__ fcount,
+                                                            const int *__restrict__ fblock,
+                                                            float *__restrict__ cent_s,
+                                                            double *__restrict__ rad_s,
+                                                            int *__restrict__ block_s) {
+    __shared__ double red[256];
+    const int sidx = blockIdx.x, tid = threadIdx.x;
+    const int f = fine_sorted[sidx];
+    const int n = fcount[f];
+    const float *c = fcent + (long long)f * ld;
+    for (int d = tid; d < ld; d += blockDim.x) cent_s[(long long)sidx * ld + d] = d < dim ? c[d] : 0.0f;
+    double best = 0.0;
+    const long long s0 = fstart[f];
+    for (int j = tid; j < n; j += blockDim.x) {
+        const float *r = pm + (s0 + j) * ld;
+        double acc = 0.0;
+        for (int d = 0; d < dim; ++d) {
+            const double t = (double)r[d] - (double)c[d];
+            acc += t * t;
+        }
+        best = fmax(best, acc);
+    }
+    red[tid] = best;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        rad_s[sidx] = n > 0 ? sqrt(red[0]) * (1.0 + 1e-12) : -1.0;
+        block_s[sidx] = fblock[f];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Search: block lower bounds
+// ---------------------------------------------------------------------------
+
+// Lower bound of every reference of block b for query q (distance, not
+// squared, rounded down to float): the smallest over the block's fine balls
+// f of max(0, |q - c_f| - rho_f), empty balls excluded; lb[b][q] (block-
+// major, query_nb per row) must hold +inf on entry and keeps +inf for a
+// block without balls.  |q - c_f|^2 comes from ||q||^2 + ||c||^2 - 2 q.c on
+// FP32 MFMA, lowered by the error bound 1.5 g (||q||^2 + ||c||^2 + 2|q||c|),
+// g = 2 (dim + 4) 2^-24.  Balls are in block-sorted order (block_s
+// ascending, block b's balls at fb_start[b] .. fb_start[b+1]).  Each wave
+// computes a 32 (balls) x 32 (queries) tile (k-step kk reads coordinate
+// kk + h*KH on lane half h, KH = half the padded dimension), stages it in
+// LDS, and each lane then folds 16 consecutive balls of one query into
+// per-block minima: a block wholly inside those 16 balls is stored, one
+// that extends past them is merged with an atomic min on the float bits
+// (non-negative floats order like their bit patterns).
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ qpm, int pm_ld, int Q,
+                                                      int dim, const float *__restrict__ qnorm,
+                                                      const float *__restrict__ fcent, int nf,
+                                                      const float *__restrict__ fnorm,
+                                                      const double *__restrict__ frad,
+                                                      const int *__restrict__ block_s,
+                                                      const int *__restrict__ fb_start,
+                                                      float *__restrict__ lb) {
+    __shared__ float patch[4][32][33];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 31, h = lane >> 5;
+    const int ntf = (nf + 31) / 32;
+    const long long tile = (long long)blockIdx.x * 4 + wave;
+    const int tq = (int)(tile / ntf), tf = (int)(tile - (long long)tq * ntf);
+    const int q0 = tq * 32, f0 = tf * 32;
+    const bool active = q0 < Q;
+    if (active) {
+        const int qa = min(q0 + col, Q - 1);  // B operand: query col
+        const int fa = min(f0 + col, nf - 1); // A operand: ball col
+        const float *qr = qpm + (long long)qa * pm_ld;
+        const float *cr = fcent + (long long)fa * pm_ld;
+        const int KH = (dim + 1) / 2;
+        f32x16_t acc = {};
+        if ((KH & 3) == 0 && (pm_ld & 3) == 0) {
+            const float *ca = cr + h * KH, *qb = qr + h * KH;
+            for (int k0 = 0; k0 < KH; k0 += 16) {
+                float4 av[4], bv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool ok = k0 + 4 * u < KH;
+                    av[u] = ok ? *reinterpret_cast<const float4 *>(ca + k0 + 4 * u) : make_float4(0, 0, 0, 0);
+                    bv[u] = ok ? *reinterpret_cast<const float4 *>(qb + k0 + 4 * u) : make_float4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].x, bv[u].x, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].y, bv[u].y, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].z, bv[u].z, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u].w, bv[u].w, acc, 0, 0, 0);
+                }
+            }
+        } else {
+            for (int k0 = 0; k0 < KH; k0 += 32) {
+                float av[32], bv[32];
+#pragma unroll
+                for (int u = 0; u < 32; ++u) {
+                    const int kk = k0 + u;
+                    const int d = kk + h * KH;
+                    const bool ok = kk < KH && d < dim;
+                    av[u] = ok ? cr[d] : 0.0f;
+                    bv[u] = ok ? qr[d] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 32; ++u)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bv[u], acc, 0, 0, 0);
+            }
+        }
+        // acc[i] is ball f0 + 8(i/4) + 4h + i%4 for query q0 + col
+        const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
+        const double nq = (double)qnorm[qa];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = ((i >> 2) << 3) + (h << 2) + (i & 3);
+            const int f = f0 + r;
+            float out = INFINITY;
+            const double rad = f < nf ? frad[f] : -1.0;
+            if (rad >= 0.0) {
+                const double nc = (double)fnorm[f];
+                const double dot = (double)acc[i];
+                double d2 = nq + nc - 2.0 * dot - g * (nq + nc + 2.0 * sqrt(nq * nc)) * 1.5;
+                d2 = d2 > 0.0 ? d2 : 0.0;
+                const double v = sqrt(d2) * (1.0 - 1e-12) - rad;
+                float fv = (float)v;
+                if ((double)fv > v) fv = nextafterf(fv, -INFINITY);
+                out = fv > 0.0f ? fv : 0.0f;
+            }
+            patch[wave][r][col] = out;
+        }
+    }
+    __syncthreads();
+    const int q = q0 + col;
+    if (!active || q >= Q) return;
+    const int lo = f0 + 16 * h, hi = min(nf, lo + 16);
+    int cur = -1;
     float m = INFINITY;
-    for (int j = fb_start[b]; j < fb_start[b + 1]; ++j)
-        m = fminf(m, lbf[(long long)q * nf + fine_by_block[j]]);
-    lbb[e] = m;
-    if (score) score[e] = -m;
+    auto flush = [&](int b, float v) {
+        if (b < 0) return;
+        float *dst = lb + (long long)b * Q + q;
+        if (fb_start[b] >= lo && fb_start[b + 1] <= hi)
+            *dst = v;
+        else
+            atomicMin(reinterpret_cast<unsigned *>(dst), __float_as_uint(v));
+    };
+    for (int f = lo; f < hi; ++f) {
+        const int b = block_s[f];
+        const float v = patch[wave][f - f0][col];
+        if (b != cur) {
+            flush(cur, m);
+            cur = b;
+            m = v;
+        } else {
+            m = fminf(m, v);
+        }
+    }
+    flush(cur, m);
 }
 
 // ---------------------------------------------------------------------------
@@ -547,12 +870,6 @@ hipError_t idx_transpose(const float *src, int ld, int nrows, int dim, float *ds
                        src, ld, nrows, dim, dst);
     return hipGetLastError();
 }
-hipError_t idx_init_centroids(const float *pm, int pm_ld, int n, int dim, int nb, float *cent,
-                              int cent_ld, hipStream_t s) {
-    hipLaunchKernelGGL(init_centroids_kernel, dim3(grid_for((long long)nb * dim)), dim3(256), 0, s,
-                       pm, pm_ld, n, dim, nb, cent, cent_ld);
-    return hipGetLastError();
-}
 hipError_t idx_gather_pack(const float *pm_src, int pm_ld, int dim, int oct, const int *perm,
                            int n_pos, int n_pad, float *packed, float *pm_dst, float *norms,
                            hipStream_t s) {
@@ -560,23 +877,14 @@ hipError_t idx_gather_pack(const float *pm_src, int pm_ld, int dim, int oct, con
                        pm_ld, dim, oct, perm, n_pos, n_pad, packed, pm_dst, norms);
     return hipGetLastError();
 }
-hipError_t idx_radius(const float *pm, int pm_ld, int dim, const int *start, const int *counts,
-                      int nb, const float *cent, int cent_ld, double *radius, hipStream_t s) {
-    hipLaunchKernelGGL(radius_kernel, dim3(nb), dim3(256), 0, s, pm, pm_ld, dim, start, counts,
-                       cent, cent_ld, radius);
-    return hipGetLastError();
-}
-hipError_t idx_lower_bounds(const float *qpm, int pm_ld, int Q, int dim, const float *cent,
-                            int cent_ld, int nb, const double *radius, float *lb, hipStream_t s) {
-    const long long n = (long long)Q * nb;
-    hipLaunchKernelGGL(lower_bound_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, qpm,
-                       pm_ld, Q, dim, cent, cent_ld, nb, radius, lb);
-    return hipGetLastError();
-}
 hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float *lb,
                            const int *counts, int *chosen, float *lb_rest, hipStream_t s) {
-    hipLaunchKernelGGL(pick_blocks_kernel, dim3((Q + 3) / 4), dim3(256), 0, s, score, Q, nb, P, lb,
-                       counts, chosen, lb_rest);
+    if (Q <= kPickSingleMaxQ)
+        hipLaunchKernelGGL(pick_blocks_single_kernel, dim3(Q), dim3(256), 0, s, score, Q, nb, P, lb,
+                           counts, chosen, lb_rest);
+    else
+        hipLaunchKernelGGL(pick_blocks_kernel, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
+                           score, Q, nb, P, lb, counts, chosen, lb_rest);
     return hipGetLastError();
 }
 hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int *counts,
@@ -611,47 +919,71 @@ hipError_t idx_make_tasks(const int *qstart, const int *counts, int nb, const in
                        block_start, block_count, tasks, max_tasks);
     return hipGetLastError();
 }
-hipError_t idx_negate(const float *x, long long n, float *y, hipStream_t s) {
-    hipLaunchKernelGGL(negate_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, n, y);
+hipError_t idx_ref_block(const int *block_start, const int *block_count, int nb, const int *perm,
+                         int *out, hipStream_t s) {
+    hipLaunchKernelGGL(ref_block_kernel, dim3(nb), dim3(256), 0, s, block_start, block_count, perm,
+                       out);
     return hipGetLastError();
 }
 
-hipError_t idx_block_count(const int *fblock, const int *fcount, int nf, int *bcount,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(block_count_kernel, dim3(grid_for(nf)), dim3(256), 0, s, fblock, fcount, nf,
-                       bcount);
-    return hipGetLastError();
-}
-hipError_t idx_fine_start(const int *fblock, const int *fcount, int nf, int *bcursor, int *fstart,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(fine_start_kernel, dim3(grid_for(nf)), dim3(256), 0, s, fblock, fcount, nf,
-                       bcursor, fstart);
-    return hipGetLastError();
-}
-hipError_t idx_coarse_centroids(const float *fcent, int ld, int dim, const int *fine_by_block,
-                                const int *fb_start, const int *fcount, int nb, float *cent,
-                                hipStream_t s) {
-    hipLaunchKernelGGL(coarse_centroid_kernel, dim3(nb), dim3(128), 0, s, fcent, ld, dim,
-                       fine_by_block, fb_start, fcount, cent);
-    return hipGetLastError();
-}
-hipError_t idx_fine_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
-                       const float *fcent, int nf, const float *fnorm, const double *frad,
-                       float *lbf, hipStream_t s) {
-    const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);
-    hipLaunchKernelGGL(fine_lb_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, qpm, pm_ld,
-                       Q, dim, qnorm, fcent, nf, fnorm, frad, lbf);
-    return hipGetLastError();
-}
 hipError_t idx_row_norms(const float *pm, int ld, int n, int dim, float *out, hipStream_t s) {
     hipLaunchKernelGGL(row_norms_kernel, dim3(grid_for(n)), dim3(256), 0, s, pm, ld, n, dim, out);
     return hipGetLastError();
 }
-hipError_t idx_block_lb(const float *lbf, int Q, int nf, const int *fine_by_block,
-                        const int *fb_start, int nb, float *lbb, float *score, hipStream_t s) {
-    const long long n = (long long)Q * nb;
-    hipLaunchKernelGGL(block_lb_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lbf, Q,
-                       nf, fine_by_block, fb_start, nb, lbb, score);
+
+hipError_t idx_kpp_seed(const float *pm, int ld, int dim, long long n, int m, int nb,
+                        unsigned long long seed, float *mind, int *cidx, double *wsum,
+                        unsigned *ticket, float *cent, hipStream_t s) {
+    const int nwg = std::max(1, std::min(kKppGroups, (m + 255) / 256));
+    hipLaunchKernelGGL(kpp_init_kernel, dim3(grid_for(m)), dim3(256), 0, s, mind, m, cidx, ticket,
+                       seed);
+    for (int t = 1; t < nb; ++t)
+        hipLaunchKernelGGL(kpp_step_kernel, dim3(nwg), dim3(256), (size_t)ld * 4, s, pm, ld, dim, n,
+                           m, mind, cidx, t, seed, wsum, ticket);
+    hipLaunchKernelGGL(kpp_gather_kernel, dim3(grid_for((long long)nb * ld)), dim3(256), 0, s, pm,
+                       ld, n, m, cidx, nb, cent);
+    return hipGetLastError();
+}
+hipError_t idx_group_fine(const int *cand, int kc, int nf, const int *fcount, int nb, int cap,
+                          int *fblock, hipStream_t s) {
+    const size_t lds = (size_t)nb * 4;
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void *)group_fine_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(group_fine_kernel, dim3(1), dim3(64), lds, s, cand, kc, nf, fcount, nb, cap,
+                       fblock);
+    return hipGetLastError();
+}
+hipError_t idx_block_count(const int *fblock, const int *fcount, int nf, int *bcount, int *bfine,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(block_count_kernel, dim3(grid_for(nf)), dim3(256), 0, s, fblock, fcount, nf,
+                       bcount, bfine);
+    return hipGetLastError();
+}
+hipError_t idx_fine_start(const int *fblock, const int *fcount, int nf, int *pos_cursor,
+                          int *fstart, int *slot_cursor, int *fine_sorted, hipStream_t s) {
+    hipLaunchKernelGGL(fine_start_kernel, dim3(grid_for(nf)), dim3(256), 0, s, fblock, fcount, nf,
+                       pos_cursor, fstart, slot_cursor, fine_sorted);
+    return hipGetLastError();
+}
+hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcent,
+                             const int *fine_sorted, const int *fstart, const int *fcount,
+                             const int *fblock, int nf, float *cent_s, double *rad_s, int *block_s,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(finalize_fine_kernel, dim3(nf), dim3(256), 0, s, pm, ld, dim, fcent,
+                       fine_sorted, fstart, fcount, fblock, cent_s, rad_s, block_s);
+    return hipGetLastError();
+}
+hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
+                       const float *fcent, int nf, const float *fnorm, const double *frad,
+                       const int *block_s, const int *fb_start, int nb, float *lb, hipStream_t s) {
+    hipError_t e = hipMemsetD32Async((hipDeviceptr_t)lb, 0x7F800000, (size_t)nb * Q, s);
+    if (e != hipSuccess) return e;
+    const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);
+    hipLaunchKernelGGL(ball_lb_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, qpm,
+                       pm_ld, Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb);
     return hipGetLastError();
 }
 

@@ -44,6 +44,7 @@ C_SYMBOLS = (
     "knnhip_index_free",
     "knnhip_index_info",
     "knnhip_index_stage_ms",
+    "knnhip_index_layout",
     "knnhip_version",
 )
 
@@ -126,6 +127,9 @@ def load() -> ctypes.CDLL:
     lib.knnhip_index_info.restype = ctypes.c_int
     lib.knnhip_index_stage_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     lib.knnhip_index_stage_ms.restype = ctypes.c_int
+    lib.knnhip_index_layout.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+    lib.knnhip_index_layout.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -281,12 +285,30 @@ class BlockIndex:
         load().knnhip_index_info(self._h, ctypes.byref(nb), ctypes.byref(npos), ctypes.byref(ms))
         return {"nblocks": nb.value, "positions": npos.value, "build_ms": ms.value}
 
-    STAGES = ("norms", "sub_ball_bounds", "block_bounds", "pick", "tasks", "scan", "select", "fallback")
+    STAGES = ("norms", "bounds", "pick", "tasks", "scan", "select", "fallback")
 
     def stage_ms(self) -> dict:
-        buf = (ctypes.c_float * 8)()
-        load().knnhip_index_stage_ms(self._h, buf, 8)
+        buf = (ctypes.c_float * 7)()
+        load().knnhip_index_stage_ms(self._h, buf, 7)
         return dict(zip(self.STAGES, list(buf)))
+
+    def layout(self, stream: Optional[int] = None):
+        """(block_start, block_count) as host int32 arrays and the block of every reference
+        point as a device int32 tensor of ref_nb entries."""
+        import numpy as np
+        import torch
+
+        nb = self.info()["nblocks"]
+        start = np.zeros(nb, dtype=np.int32)
+        count = np.zeros(nb, dtype=np.int32)
+        ref_block = torch.empty(self.ref.shape[1], dtype=torch.int32, device=self.ref.device)
+        st = stream if stream is not None else torch.cuda.current_stream(self.ref.device).cuda_stream
+        ok = load().knnhip_index_layout(self._h, start.ctypes.data, count.ctypes.data,
+                                        ref_block.data_ptr(), st)
+        if not ok:
+            raise KnnError(last_error())
+        torch.cuda.synchronize(self.ref.device)
+        return start, count, ref_block
 
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
                margin: int = 0, force_fallback: bool = False, timing: bool = False,
@@ -305,8 +327,8 @@ class BlockIndex:
             index = torch.empty((k, query_nb), dtype=torch.int32, device=query.device)
         sp = None
         if scores is not None:
-            if scores.shape != (query_nb, self.nblocks) or scores.dtype != torch.float32:
-                raise KnnError("scores must be float32 (query_nb, nblocks)")
+            if scores.shape != (self.nblocks, query_nb) or scores.dtype != torch.float32:
+                raise KnnError("scores must be float32 (nblocks, query_nb): one column per query")
             scores = scores.contiguous()
             sp = scores.data_ptr()
         opt = Options()

@@ -67,7 +67,7 @@ def test_uniform_data_stays_exact():
 def test_arbitrary_scores_stay_exact(clustered):
     ref, qry, k, gd, gi = clustered
     ix = knnhip.BlockIndex(ref, nblocks=64, iters=6)
-    scores = torch.rand((qry.shape[1], 64), device="cuda")
+    scores = torch.rand((64, qry.shape[1]), device="cuda")
     d, i = ix.search(qry, k, probes=3, scores=scores)
     check(d, i, gd, gi, "random scores")
 
@@ -100,3 +100,56 @@ def test_index_matches_plain_path_at_scale():
     sample = [0, 5, 500, 1023]
     gd, gi = oracle.knn_c(ref.cpu().numpy(), np.ascontiguousarray(qry.cpu().numpy()[:, sample]), k)
     check(d[:, sample], i[:, sample], gd, gi, "oracle sample")
+
+
+def test_layout_is_a_partition(clustered):
+    # every reference sits in exactly one block; blocks start on 32-point tiles and do not overlap
+    ref, qry, k, gd, gi = clustered
+    R = ref.shape[1]
+    ix = knnhip.BlockIndex(ref, nblocks=64, iters=6)
+    start, count, rb = ix.layout()
+    rb = rb.cpu().numpy()
+    assert int(count.sum()) == R
+    assert np.all(start % 32 == 0)
+    assert np.all(start[1:] >= start[:-1] + count[:-1])
+    assert rb.min() >= 0 and rb.max() < 64
+    assert np.array_equal(np.bincount(rb, minlength=64), count)
+    ix.close()
+
+
+def test_blocks_balanced_on_separated_clusters():
+    # 256 well separated clusters of equal weight into 64 blocks: k-means++ seeding covers the
+    # clusters, so no block swallows many of them (a block of 4 clusters holds about R/64 points)
+    R, Q, D, k = 1 << 16, 256, 48, 8
+    ref = knnhip.generate_mixture(R, D, 256, 0.05, 61, 62)
+    qry = knnhip.generate_mixture(Q, D, 256, 0.05, 61, 63)
+    ix = knnhip.BlockIndex(ref, nblocks=64, iters=8)
+    start, count, rb = ix.layout()
+    say(f"block sizes: min {count.min()} max {count.max()} mean {count.mean():.0f}")
+    assert count.max() <= 4 * R // 64
+    pd, pi = knnhip.search_device(ref, qry, k)
+    d, i = ix.search(qry, k, probes=1)
+    torch.cuda.synchronize()
+    nfb = knnhip.last_fallback_count()
+    say(f"fallbacks at one probe: {nfb}")
+    assert torch.equal(i, pi) and torch.equal(d, pd)
+    # neighbours lie in the query's own cluster, whose sub-balls are tight: one probe certifies most
+    assert nfb <= Q // 10
+    ix.close()
+
+
+@pytest.mark.parametrize("ranking", ["bounds", "random"])
+def test_many_queries_both_rankings(ranking):
+    # 2000 queries: the block choice runs 64 queries per workgroup (the few-query form runs one
+    # query per workgroup and is covered above); random scores force many certificate retries
+    R, Q, D, k, nb = 20000, 2000, 32, 8, 64
+    ref = knnhip.generate_mixture(R, D, 64, 0.05, 71, 72)
+    qry = knnhip.generate_mixture(Q, D, 64, 0.05, 71, 73)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    ix = knnhip.BlockIndex(ref, nblocks=nb, iters=6)
+    scores = None if ranking == "bounds" else torch.rand((nb, Q), device="cuda")
+    d, i = ix.search(qry, k, probes=2, scores=scores)
+    torch.cuda.synchronize()
+    say(f"{ranking}: first-pass certificate failures {knnhip.last_fallback_count()} / {Q}")
+    check(d, i, gd, gi, ranking)
+    ix.close()
