@@ -131,6 +131,46 @@ int knnhip_index_stage_ms(const knnhip_index *index, float *ms, int n);
 int knnhip_index_layout(const knnhip_index *index, int *block_start, int *block_count,
                         int *ref_block, void *stream);
 
+/*
+ * Learned router for the block index.  No counterpart in the reference.  An
+ * MLP dim -> hidden -> hidden -> nblocks (ReLU after both hidden layers,
+ * bfloat16 products with FP32 accumulation) scores the blocks of each query;
+ * its scores go to knnhip_index_search as `scores`.  Training runs on the
+ * GPU: exact neighbours of the training queries (dim-major, like the point
+ * sets) come from the plain search, the target of a query is the share of
+ * its k neighbours in each block, the loss is softmax cross-entropy and the
+ * optimiser Adam (beta1 0.9, beta2 0.999, eps 1e-8).  Parameters are one FP32
+ * vector: W1 [hidden][dim], b1, W2 [hidden][hidden], b2, W3 [nblocks][hidden],
+ * b3 (weights row-major, out x in).  Device pointers unless said otherwise.
+ */
+typedef struct knnhip_router knnhip_router;
+int knnhip_router_create(int dim, int hidden, int nblocks, unsigned long long seed,
+                         knnhip_router **out);
+void knnhip_router_free(knnhip_router *router);
+long long knnhip_router_num_params(const knnhip_router *router);
+/* Copy the parameter vector to (set = 0) or from (set = 1) host memory; a set
+ * also resets the optimiser state. */
+int knnhip_router_params(knnhip_router *router, float *host_params, long long n, int set);
+/* One forward + backward pass over n samples x (dim x n, leading dimension
+ * ld) with targets tgt (k x n, leading dimension ldt: the block of each
+ * neighbour): gradient of the mean loss into grad (num_params floats, may be
+ * NULL) and the mean loss into *loss (host, may be NULL). */
+int knnhip_router_loss_grad(knnhip_router *router, const float *x, int n, long long ld,
+                            const int *tgt, long long ldt, int k, float *grad, float *loss,
+                            void *stream);
+/* The same pass followed by one Adam update with learning rate lr. */
+int knnhip_router_step(knnhip_router *router, const float *x, int n, long long ld, const int *tgt,
+                       long long ldt, int k, float lr, void *stream);
+/* Train on n queries (dim x n, ld) against the index's blocks: k exact
+ * neighbours per query, `epochs` shuffled passes in batches of `batch`.
+ * *final_loss (host, may be NULL) gets the mean loss of the last epoch. */
+int knnhip_router_train(knnhip_router *router, const knnhip_index *index, const float *train_query,
+                        int n, long long ld, int k, int epochs, int batch, float lr,
+                        unsigned long long seed, float *final_loss, void *stream);
+/* Block scores of query_nb queries into scores (nblocks x query_nb). */
+int knnhip_router_scores(knnhip_router *router, const float *query, int query_nb,
+                         long long query_ld, float *scores, void *stream);
+
 /* Library version string. */
 const char *knnhip_version(void);
 

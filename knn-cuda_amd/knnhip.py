@@ -45,6 +45,14 @@ C_SYMBOLS = (
     "knnhip_index_info",
     "knnhip_index_stage_ms",
     "knnhip_index_layout",
+    "knnhip_router_create",
+    "knnhip_router_free",
+    "knnhip_router_num_params",
+    "knnhip_router_params",
+    "knnhip_router_loss_grad",
+    "knnhip_router_step",
+    "knnhip_router_train",
+    "knnhip_router_scores",
     "knnhip_version",
 )
 
@@ -130,6 +138,25 @@ def load() -> ctypes.CDLL:
     lib.knnhip_index_layout.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]
     lib.knnhip_index_layout.restype = ctypes.c_int
+    vp, i32, i64, f32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
+    lib.knnhip_router_create.argtypes = [i32, i32, i32, ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    lib.knnhip_router_create.restype = i32
+    lib.knnhip_router_free.argtypes = [vp]
+    lib.knnhip_router_free.restype = None
+    lib.knnhip_router_num_params.argtypes = [vp]
+    lib.knnhip_router_num_params.restype = i64
+    lib.knnhip_router_params.argtypes = [vp, vp, i64, i32]
+    lib.knnhip_router_params.restype = i32
+    lib.knnhip_router_loss_grad.argtypes = [vp, vp, i32, i64, vp, i64, i32, vp,
+                                            ctypes.POINTER(f32), vp]
+    lib.knnhip_router_loss_grad.restype = i32
+    lib.knnhip_router_step.argtypes = [vp, vp, i32, i64, vp, i64, i32, f32, vp]
+    lib.knnhip_router_step.restype = i32
+    lib.knnhip_router_train.argtypes = [vp, vp, vp, i32, i64, i32, i32, i32, f32,
+                                        ctypes.c_ulonglong, ctypes.POINTER(f32), vp]
+    lib.knnhip_router_train.restype = i32
+    lib.knnhip_router_scores.argtypes = [vp, vp, i32, i64, vp, vp]
+    lib.knnhip_router_scores.restype = i32
     _lib = lib
     return lib
 
@@ -347,6 +374,112 @@ class BlockIndex:
     def close(self):
         if getattr(self, "_h", None):
             load().knnhip_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream_of(t, stream):
+    import torch
+
+    return stream if stream is not None else torch.cuda.current_stream(t.device).cuda_stream
+
+
+class Router:
+    """Learned block router: MLP dim -> hidden -> hidden -> nblocks, trained on the GPU.
+
+    Inputs are dim-major device tensors (dim x n), like the point sets; targets are k x n int32
+    block ids (the block of each exact neighbour).  ``scores`` returns an (nblocks, n) tensor for
+    ``BlockIndex.search(scores=...)``.
+    """
+
+    def __init__(self, dim: int, hidden: int, nblocks: int, seed: int = 0x6B4E4E07):
+        h = ctypes.c_void_p()
+        if not load().knnhip_router_create(dim, hidden, nblocks, seed, ctypes.byref(h)):
+            raise KnnError(last_error())
+        self._h = h
+        self.dim, self.hidden, self.nblocks = dim, hidden, nblocks
+
+    @property
+    def num_params(self) -> int:
+        return int(load().knnhip_router_num_params(self._h))
+
+    def get_params(self):
+        import numpy as np
+
+        out = np.empty(self.num_params, dtype=np.float32)
+        if not load().knnhip_router_params(self._h, out.ctypes.data, out.size, 0):
+            raise KnnError(last_error())
+        return out
+
+    def set_params(self, params):
+        import numpy as np
+
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if not load().knnhip_router_params(self._h, p.ctypes.data, p.size, 1):
+            raise KnnError(last_error())
+
+    @staticmethod
+    def _check(x, tgt):
+        import torch
+
+        if x.dtype != torch.float32 or not x.is_cuda or x.stride(1) != 1:
+            raise KnnError("x must be a float32 device tensor with contiguous rows (dim x n)")
+        if tgt.dtype != torch.int32 or not tgt.is_cuda or tgt.stride(1) != 1:
+            raise KnnError("tgt must be an int32 device tensor with contiguous rows (k x n)")
+
+    def loss_grad(self, x, tgt, stream: Optional[int] = None):
+        import torch
+
+        self._check(x, tgt)
+        grad = torch.empty(self.num_params, dtype=torch.float32, device=x.device)
+        loss = ctypes.c_float()
+        ok = load().knnhip_router_loss_grad(self._h, x.data_ptr(), x.shape[1], x.stride(0),
+                                            tgt.data_ptr(), tgt.stride(0), tgt.shape[0],
+                                            grad.data_ptr(), ctypes.byref(loss),
+                                            _stream_of(x, stream))
+        if not ok:
+            raise KnnError(last_error())
+        return grad, loss.value
+
+    def step(self, x, tgt, lr: float, stream: Optional[int] = None):
+        self._check(x, tgt)
+        ok = load().knnhip_router_step(self._h, x.data_ptr(), x.shape[1], x.stride(0),
+                                       tgt.data_ptr(), tgt.stride(0), tgt.shape[0], lr,
+                                       _stream_of(x, stream))
+        if not ok:
+            raise KnnError(last_error())
+
+    def train(self, index: "BlockIndex", train_query, k: int, epochs: int = 10, batch: int = 1024,
+              lr: float = 1e-3, seed: int = 0x6B4E4E17, stream: Optional[int] = None) -> float:
+        loss = ctypes.c_float()
+        ok = load().knnhip_router_train(self._h, index._h, train_query.data_ptr(),
+                                        train_query.shape[1], train_query.stride(0), k, epochs,
+                                        batch, lr, seed, ctypes.byref(loss),
+                                        _stream_of(train_query, stream))
+        if not ok:
+            raise KnnError(last_error())
+        return loss.value
+
+    def scores(self, query, out=None, stream: Optional[int] = None):
+        import torch
+
+        if out is None:
+            out = torch.empty((self.nblocks, query.shape[1]), dtype=torch.float32,
+                              device=query.device)
+        ok = load().knnhip_router_scores(self._h, query.data_ptr(), query.shape[1],
+                                         query.stride(0), out.data_ptr(), _stream_of(query, stream))
+        if not ok:
+            raise KnnError(last_error())
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().knnhip_router_free(self._h)
             self._h = None
 
     def __del__(self):
