@@ -12,6 +12,11 @@ Prints one JSON line.  `roofline` is measured live for the dominant kernel
 `traffic` comes from the committed rocprofv3 counter summary of that kernel
 when present.  `cpu_baseline` times the oracle's single-threaded restatement
 of the reference ground truth on a bounded query sample of the same data.
+
+`router_form` measures the second half of the metric on the clustered
+workload (4,194,304 refs, 16,384 queries, dim 96, k 16, 2,048 blocks): the
+plain FP32 search against the block-index search on the same data, both exact
+(the outputs are compared for equality), and the learned MLP router.
 """
 from __future__ import annotations
 
@@ -41,10 +46,73 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-router-form", action="store_true",
+                    help="skip the router-form measurement (4M clustered refs, block index)")
+    ap.add_argument("--router-reps", type=int, default=5)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this many MiB before the timed region (a "
                          "known-size kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def router_form(args, knnhip, torch):
+    """Router form on the clustered workload: 4,194,304 refs, 16,384 queries, dim 96, k 16,
+    mixture of 4,096 Gaussians (sigma 0.1, centres U[-1,1]^96); 2,048 blocks.  Times the plain FP32
+    search and the block-index search on the same device-resident data (both exact; the results
+    are compared for equality), plus the learned MLP router trained on 65,536 mixture queries."""
+    import time
+
+    R, Q, D, k, NB = 4 << 20, 16384, 96, 16, 2048
+    seed_c, seed_r, seed_q, seed_t = 0x6B4E4E05, 0x6B4E4E06, 0x6B4E4E16, 0x6B4E4E26
+    ref = knnhip.generate_mixture(R, D, 4096, 0.1, seed_c, seed_r)
+    qry = knnhip.generate_mixture(Q, D, 4096, 0.1, seed_c, seed_q)
+    torch.cuda.synchronize()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            res = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps * 1e3, res
+
+    plain_ms, (pd, pi) = timed(lambda: knnhip.search_device(ref, qry, k), max(2, args.router_reps // 2))
+    t = time.perf_counter()
+    ix = knnhip.BlockIndex(ref, nblocks=NB, iters=8)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t
+    rows = []
+    for P in (2, 4, 8):
+        ms, (d, i) = timed(lambda: ix.search(qry, k, probes=P), args.router_reps)
+        equal = bool(torch.equal(i, pi) and torch.equal(d, pd))
+        ix.search(qry, k, probes=P, timing=True)
+        rows.append({"probes": P, "ms": ms, "queries_per_s": Q / ms * 1e3, "speedup_vs_plain": plain_ms / ms,
+                     "certificate_failures": knnhip.last_fallback_count(), "equal_to_plain": equal,
+                     "stages_ms": {a: round(b, 4) for a, b in ix.stage_ms().items()}})
+    best = min(rows, key=lambda r: r["ms"])
+    # learned router: trained on the GPU, then used as the block ranking
+    trq = knnhip.generate_mixture(65536, D, 4096, 0.1, seed_c, seed_t)
+    mlp = knnhip.Router(D, 256, NB, seed=0x6B4E4E07)
+    t = time.perf_counter()
+    loss = mlp.train(ix, trq, k, epochs=8, batch=1024, lr=1e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t
+    P = best["probes"]
+    ms_m, (d, i) = timed(lambda: ix.search(qry, k, probes=P, scores=mlp.scores(qry)), args.router_reps)
+    mlp_row = {"probes": P, "ms": ms_m, "speedup_vs_plain": plain_ms / ms_m,
+               "certificate_failures": knnhip.last_fallback_count(),
+               "equal_to_plain": bool(torch.equal(i, pi) and torch.equal(d, pd)),
+               "train_s": train_s, "train_queries": 65536, "epochs": 8, "final_loss": loss,
+               "layers": [D, 256, 256, NB], "dtype": "bf16 products, fp32 accumulation"}
+    mlp.close()
+    ix.close()
+    return {"workload": "4,194,304 refs x 16,384 queries, dim 96, k 16, mixture of 4,096 Gaussians "
+                        "(sigma 0.1), 2,048 blocks",
+            "plain_fp32_ms": plain_ms, "plain_queries_per_s": Q / plain_ms * 1e3,
+            "index_build_s": build_s, "ranking": "certificate lower bounds",
+            "best": {"probes": best["probes"], "ms": best["ms"], "speedup_vs_plain": best["speedup_vs_plain"]},
+            "probes": rows, "mlp_router": mlp_row}
 
 
 def main():
@@ -135,6 +203,9 @@ def main():
                "sample": f"first {n} of {Q} queries against all {R} refs (dim {D}, k {k}), "
                          f"single-threaded oracle restatement of knn_c, {cpu_s:.1f} s"}
 
+    plan = knnhip.last_plan()
+    router = None if args.no_router_form else router_form(args, knnhip, torch)
+
     out = {
         "metric": METRIC,
         "value": qps,
@@ -158,8 +229,9 @@ def main():
         "stages_ms": {name: sum(v) / len(v) for name, v in
                       zip(("norms", "scan", "select_rerank", "fallback"), stage_ms)},
         "fallback_queries": fallbacks,
-        "plan": knnhip.last_plan(),
+        "plan": plan,
         "parity_gate_queries": gate_q,
+        "router_form": router,
     }
     print(json.dumps(out), flush=True)
 

@@ -168,14 +168,27 @@ __global__ __launch_bounds__(256) void softmax_ce_kernel(const float *__restrict
     }
 }
 
-// out[j] = sum_i X[i*ld + j] (FP32, rows in order): bias gradients.
-__global__ void colsum_kernel(const float *__restrict__ X, int n, int m, long long ld,
-                              float *__restrict__ out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m) return;
+// out[j] = sum_i X[i*ld + j] (FP32): bias gradients.  A workgroup takes 64
+// columns; wave w sums rows w, w + kColsumWaves, ... in order and the
+// partial sums are added in wave order, so the result is deterministic.
+constexpr int kColsumWaves = 16;
+
+__global__ __launch_bounds__(64 * kColsumWaves) void colsum_kernel(const float *__restrict__ X, int n,
+                                                                   int m, long long ld,
+                                                                   float *__restrict__ out) {
+    __shared__ float part[kColsumWaves][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 64 + lane;
     float s = 0.0f;
-    for (int i = 0; i < n; ++i) s += X[(long long)i * ld + j];
-    out[j] = s;
+    if (j < m)
+        for (int i = wave; i < n; i += kColsumWaves) s += X[(long long)i * ld + j];
+    part[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && j < m) {
+        float t = part[0][lane];
+        for (int w = 1; w < kColsumWaves; ++w) t += part[w][lane];
+        out[j] = t;
+    }
 }
 
 // Adam (Kingma & Ba 2015), the usual bias-corrected form:
@@ -259,7 +272,8 @@ hipError_t launch_softmax_ce(const float *Z, int n, int nb, const int *tgt, long
     return hipGetLastError();
 }
 hipError_t launch_colsum(const float *X, int n, int m, long long ld, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(colsum_kernel, dim3((m + 255) / 256), dim3(256), 0, s, X, n, m, ld, out);
+    hipLaunchKernelGGL(colsum_kernel, dim3((m + 63) / 64), dim3(64 * kColsumWaves), 0, s, X, n, m, ld,
+                       out);
     return hipGetLastError();
 }
 hipError_t launch_adam(float *w, const float *g, float *m, float *v, long long n, float lr, float b1,
