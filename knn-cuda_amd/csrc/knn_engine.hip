@@ -24,6 +24,30 @@
 
 namespace knnhip {
 
+// Owned device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    bool alloc(size_t bytes) {
+        release();
+        if (bytes == 0) bytes = 256;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return false;
+        }
+        n = bytes;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
 // Launchers defined in knn_scan.hip / knn_select.hip.
 int scan_pick_ks(int dim);
 size_t scan_lds_bytes(int kp, int waves);
@@ -39,7 +63,17 @@ hipError_t launch_gen_mixture(float *out, int n, int dim, int ncomp, float sigma
 
 namespace {
 
+// Candidates kept beyond k (kp = k + margin): the certificate needs the
+// kp-th approximate key to clear the k-th exact distance by the FP32 error
+// bound.  Large k goes with dense neighbourhoods, so it starts wider.
 constexpr int kDefaultMargin = 16;
+constexpr int kDefaultMarginLargeK = 64;
+constexpr int kLargeK = 64;
+// Certificate failures are retried through the scan with wider margins
+// (kEscalationMargins[level - 1]) before the exact per-query scan.
+constexpr int kEscalationMargins[] = {64, 256};
+constexpr int kMaxEscalation = 2;
+int default_margin(int k) { return k >= kLargeK ? kDefaultMarginLargeK : kDefaultMargin; }
 constexpr int kExactGrid = 512;
 constexpr size_t kScanLdsMax = 160 * 1024;
 
@@ -124,7 +158,7 @@ double slice_cost(int S, int steps, int nqb, int resident) {
 
 Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     Plan p;
-    const int margin = o.margin > 0 ? o.margin : kDefaultMargin;
+    const int margin = o.margin > 0 ? o.margin : default_margin(k);
     p.kp = std::min(k + margin, R);
     if (p.kp < k) p.kp = k;
     // Heaps: 32 queries * (kp | 1) * 8 bytes of LDS per block, shared by its
@@ -242,9 +276,16 @@ bool ensure_device() {
 }
 
 // Core: device pointers, dim rows of ldr / ldq floats.
+// Buffers of one escalation level: the failed queries' ids, their columns
+// and their results.
+struct Escalation {
+    DevBuf list, qsub, dsub, isub;
+};
+Escalation g_esc[kMaxEscalation];
+
 bool run_device(const float *ref, long long ldr, int R, const float *query, long long ldq, int Q,
                 int dim, int k, float *out_dist, int *out_idx, const Plan &p,
-                const knnhip_options &o, hipStream_t s) {
+                const knnhip_options &o, hipStream_t s, int level = 0) {
     const bool timing = o.timing != 0;
     if (timing && !g_ws.have_events) {
         for (auto &e : g_ws.ev) KNN_TRY(hipEventCreate(&e), "event create");
@@ -381,11 +422,47 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         KNN_TRY(launch_select(fa, s), "select kernel launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[3], s), "event");
 
-        // Queries that failed the certificate: the grid reads the count on
-        // the device, so no host round trip is needed.
-        ea.list = fb_list;
-        ea.count = fb_count;
-        KNN_TRY(launch_exact(ea, egrid, s), "exact scan launch");
+        // Queries that failed the certificate: once more through the scan
+        // with a wider margin while one is left, else the exact per-query
+        // scan (whose grid reads the count on the device).  A forced
+        // fallback goes to the exact scan directly.
+        const int cur_margin = p.kp - k;
+        int next_margin = 0;
+        for (int l = level; l < kMaxEscalation && !next_margin && !o.force_fallback; ++l)
+            if (kEscalationMargins[l] > cur_margin && k + cur_margin < R) next_margin = kEscalationMargins[l];
+        int nfail = 0;
+        if (next_margin) {
+            KNN_TRY(hipMemcpyAsync(&nfail, fb_count, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+            KNN_TRY(hipStreamSynchronize(s), "sync");
+        }
+        if (next_margin && nfail > 0) {
+            Escalation &e = g_esc[level];
+            auto need = [](DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); };
+            if (!need(e.list, (size_t)nfail * 4) || !need(e.qsub, (size_t)nfail * dim * 4) ||
+                !need(e.dsub, (size_t)nfail * k * 4) || !need(e.isub, (size_t)nfail * k * 4)) {
+                set_error("Memory allocation error");
+                return false;
+            }
+            KNN_TRY(hipMemcpyAsync(e.list.p, fb_list, (size_t)nfail * 4, hipMemcpyDeviceToDevice, s),
+                    "copy");
+            KNN_TRY(idx_gather_columns(query, ldq, dim, e.list.as<int>(), nfail, e.qsub.as<float>(), s),
+                    "gather");
+            knnhip_options o2 = o;
+            o2.margin = next_margin;
+            o2.force_fallback = 0;
+            o2.timing = 0;
+            const Plan p2 = make_plan(R, nfail, dim, k, o2);
+            if (!run_device(ref, ldr, R, e.qsub.as<float>(), nfail, nfail, dim, k, e.dsub.as<float>(),
+                            e.isub.as<int>(), p2, o2, s, level + 1))
+                return false;
+            KNN_TRY(idx_scatter_results(e.dsub.as<float>(), e.isub.as<int>(), k, nfail,
+                                        e.list.as<int>(), Q, out_dist, out_idx, s),
+                    "scatter");
+        } else if (!next_margin) {
+            ea.list = fb_list;
+            ea.count = fb_count;
+            KNN_TRY(launch_exact(ea, egrid, s), "exact scan launch");
+        }
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[4], s), "event");
     }
     g_last_stream = s;
@@ -460,29 +537,6 @@ bool report(bool ok) {
 
 }  // namespace
 
-// Owned device buffer.
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-    bool alloc(size_t bytes) {
-        release();
-        if (bytes == 0) bytes = 256;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            return false;
-        }
-        n = bytes;
-        return true;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { release(); }
-    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
-};
 
 }  // namespace knnhip
 
@@ -729,7 +783,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                   long long ldq, int k, int P, const float *router_scores, float *out_dist,
                   int *out_idx, const knnhip_options &o, hipStream_t s) {
     const int nb = ix->nb, dim = ix->dim, pm_ld = ix->pm_ld;
-    const int margin = o.margin > 0 ? o.margin : kDefaultMargin;
+    const int margin = o.margin > 0 ? o.margin : default_margin(k);
     const int kp = std::max(k, std::min(k + margin, ix->R));
     if (scan_lds_bytes(kp, 4) > kScanLdsMax || (long long)P * kp > kMaxUnion) {
         set_error("k or probe count too large for the block index (k=%d probes=%d)", k, P);

@@ -193,3 +193,19 @@ def test_one_million_refs_properties():
     gd, gi = oracle.knn_c(ref_h, qs, k)
     assert_exact(dh[:, sample], ih[:, sample], gd, gi, "1M sample")
     say(f"fallbacks: {knnhip.last_fallback_count()}")
+
+
+def test_dense_clusters_escalate_margin_not_exact_scan():
+    # C3's geometry at 1/8 the size: 128 Gaussians (sigma 0.05, centres in [-1,1]^256) of about 1024
+    # points each, k = 64.  With margin 16 the 64th and 80th keys are closer than the FP32 error
+    # bound, so level 0 fails for most queries; the wider-margin rescan certifies them and the
+    # per-query exact scan takes none.
+    R, Q, D, k = 1 << 17, 64, 256, 64
+    ref = knnhip.generate_mixture(R, D, 128, 0.05, 0x6B4E4E33, 0x6B4E4E34)
+    qry = knnhip.generate_mixture(Q, D, 128, 0.05, 0x6B4E4E33, 0x6B4E4E35)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    for margin in (16, 0):
+        d, i = knnhip.search_device(ref, qry, k, margin=margin)
+        torch.cuda.synchronize()
+        assert knnhip.last_fallback_count() == 0, f"margin {margin}: exact-scan queries"
+        assert_exact(d, i, gd, gi, f"margin {margin}")
