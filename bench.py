@@ -17,6 +17,8 @@ of the reference ground truth on a bounded query sample of the same data.
 workload (4,194,304 refs, 16,384 queries, dim 96, k 16, 2,048 blocks): the
 plain FP32 search against the block-index search on the same data, both exact
 (the outputs are compared for equality), and the learned MLP router.
+`batched_form` times the 512-problem batched workload in one call against
+the same problems searched one call at a time (outputs compared for equality).
 """
 from __future__ import annotations
 
@@ -49,6 +51,8 @@ def parse():
     ap.add_argument("--no-router-form", action="store_true",
                     help="skip the router-form measurement (4M clustered refs, block index)")
     ap.add_argument("--router-reps", type=int, default=5)
+    ap.add_argument("--no-batched-form", action="store_true",
+                    help="skip the batched-form measurement (512 problems, tests/golden/c4_problems.json)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this many MiB before the timed region (a "
                          "known-size kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -113,6 +117,49 @@ def router_form(args, knnhip, torch):
             "index_build_s": build_s, "ranking": "certificate lower bounds",
             "best": {"probes": best["probes"], "ms": best["ms"], "speedup_vs_plain": best["speedup_vs_plain"]},
             "probes": rows, "mlp_router": mlp_row}
+
+
+def batched_form(knnhip, torch):
+    """Batched form: the 512 problems of tests/golden/c4_problems.json (R log-uniform in [1K, 256K],
+    Q in [16, 4096], dim in {32..512}, k in {1..128}, uniform data from the listed seeds), all
+    device-resident.  Times one call of the batched search (untuned, then after the autotuner has
+    picked each class's tile shape) against the same problems searched one call at a time; all
+    outputs are compared for equality."""
+    import time
+
+    with open(os.path.join(ROOT, "tests", "golden", "c4_problems.json")) as f:
+        P = json.load(f)["problems"]
+    items = [(knnhip.generate_uniform(p["ref_nb"], p["dim"], seed=p["ref_seed"]),
+              knnhip.generate_uniform(p["query_nb"], p["dim"], seed=p["query_seed"]), p["k"]) for p in P]
+    torch.cuda.synchronize()
+    flops = sum(2.0 * r.shape[1] * q.shape[1] * r.shape[0] for r, q, _ in items)
+    nq = sum(q.shape[1] for _, q, _ in items)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3, out
+
+    seq_ms, seq = timed(lambda: [knnhip.search_device(r, q, k) for r, q, k in items])
+    for c in knnhip.batched_table():
+        knnhip.batched_set_slice(c["dim"], c["k"], 0)
+    unt_ms, out = timed(lambda: knnhip.search_batched(items))
+    equal = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]) for a, b in zip(out, seq))
+    t = time.perf_counter()
+    knnhip.batched_autotune(items)
+    torch.cuda.synchronize()
+    tune_s = time.perf_counter() - t
+    tun_ms, out = timed(lambda: knnhip.search_batched(items))
+    equal = equal and all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]) for a, b in zip(out, seq))
+    return {"workload": "512 problems (tests/golden/c4_problems.json), uniform [0,1)",
+            "flops": flops, "queries": nq, "sequential_ms": seq_ms, "batched_untuned_ms": unt_ms,
+            "batched_tuned_ms": tun_ms, "autotune_s": tune_s,
+            "queries_per_s": nq / tun_ms * 1e3, "tflops": flops / tun_ms / 1e9,
+            "speedup_vs_sequential": seq_ms / tun_ms, "certificate_failures": knnhip.batched_last_failures(),
+            "equal_to_single_searches": equal, "classes": len(knnhip.batched_table())}
 
 
 def main():
@@ -205,6 +252,7 @@ def main():
 
     plan = knnhip.last_plan()
     router = None if args.no_router_form else router_form(args, knnhip, torch)
+    batched = None if args.no_batched_form else batched_form(knnhip, torch)
 
     out = {
         "metric": METRIC,
@@ -232,6 +280,7 @@ def main():
         "plan": plan,
         "parity_gate_queries": gate_q,
         "router_form": router,
+        "batched_form": batched,
     }
     print(json.dumps(out), flush=True)
 

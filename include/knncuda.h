@@ -171,6 +171,39 @@ int knnhip_router_train(knnhip_router *router, const knnhip_index *index, const 
 int knnhip_router_scores(knnhip_router *router, const float *query, int query_nb,
                          long long query_ld, float *scores, void *stream);
 
+/*
+ * Batched form.  No counterpart in the reference.  Runs many independent
+ * searches, each exactly as knn_cuda_global would (device pointers: dim-major
+ * point sets with leading dimensions, k x query_nb outputs).  Problems
+ * sharing (dim, k) form one class and run as one scan launch; the references
+ * per scan task (the class's tile shape) come from the tuned table, or from
+ * a cost model when the class is untuned.  knnhip_batched_autotune times the
+ * candidate shapes on the given problems and records the fastest per class;
+ * knnhip_batched_table writes the table as JSON (returns the length needed,
+ * terminating zero included); knnhip_batched_set_slice sets or (0) clears
+ * an entry.  The search synchronises the stream before it returns.
+ */
+typedef struct knnhip_problem {
+    const float *ref;
+    int ref_nb;
+    long long ref_ld;
+    const float *query;
+    int query_nb;
+    long long query_ld;
+    int dim;
+    int k;
+    float *knn_dist;
+    int *knn_index;
+} knnhip_problem;
+int knnhip_search_batched(const knnhip_problem *problems, int count, const knnhip_options *opt,
+                          void *stream);
+/* Certificate failures of the last batched search (answered by the plain
+ * search of their problem, so still exact). */
+long long knnhip_batched_last_failures(void);
+int knnhip_batched_autotune(const knnhip_problem *problems, int count, int reps, void *stream);
+int knnhip_batched_table(char *buf, int len);
+int knnhip_batched_set_slice(int dim, int k, int slice_len);
+
 /* Library version string. */
 const char *knnhip_version(void);
 

@@ -57,5 +57,7 @@ hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcen
 hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
                        const float *fcent, int nf, const float *fnorm, const double *frad,
                        const int *block_s, const int *fb_start, int nb, float *lb, hipStream_t s);
+hipError_t idx_batch_perm(const int *starts, const int *counts, int nsets, int n_pos, int *perm,
+                          hipStream_t s);
 
 }  // namespace knnhip

@@ -416,6 +416,26 @@ __global__ void row_norms_kernel(const float *__restrict__ pm, int ld, int n, in
     }
 }
 
+// Batched search: position i of a concatenation of point sets (set p at
+// starts[p] .. starts[p] + counts[p], each start a multiple of 32) maps to
+// the set's own index i - starts[p]; positions between sets get -1.  One
+// thread per position, the set found by binary search over starts.
+__global__ void batch_perm_kernel(const int *__restrict__ starts, const int *__restrict__ counts,
+                                  int nsets, int n_pos, int *__restrict__ perm) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_pos; i += gridDim.x * blockDim.x) {
+        int lo = 0, hi = nsets - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (starts[mid] <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int local = i - starts[lo];
+        perm[i] = local < counts[lo] ? local : -1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k-means++ seeding (D^2 sampling, Arthur & Vassilvitskii, SODA 2007)
 // ---------------------------------------------------------------------------
@@ -984,6 +1004,12 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
     const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);
     hipLaunchKernelGGL(ball_lb_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, qpm,
                        pm_ld, Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb);
+    return hipGetLastError();
+}
+hipError_t idx_batch_perm(const int *starts, const int *counts, int nsets, int n_pos, int *perm,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(batch_perm_kernel, dim3(grid_for(n_pos)), dim3(256), 0, s, starts, counts,
+                       nsets, n_pos, perm);
     return hipGetLastError();
 }
 

@@ -53,6 +53,11 @@ C_SYMBOLS = (
     "knnhip_router_step",
     "knnhip_router_train",
     "knnhip_router_scores",
+    "knnhip_search_batched",
+    "knnhip_batched_last_failures",
+    "knnhip_batched_autotune",
+    "knnhip_batched_table",
+    "knnhip_batched_set_slice",
     "knnhip_version",
 )
 
@@ -157,6 +162,16 @@ def load() -> ctypes.CDLL:
     lib.knnhip_router_train.restype = i32
     lib.knnhip_router_scores.argtypes = [vp, vp, i32, i64, vp, vp]
     lib.knnhip_router_scores.restype = i32
+    lib.knnhip_search_batched.argtypes = [vp, i32, ctypes.POINTER(Options), vp]
+    lib.knnhip_search_batched.restype = i32
+    lib.knnhip_batched_last_failures.argtypes = []
+    lib.knnhip_batched_last_failures.restype = i64
+    lib.knnhip_batched_autotune.argtypes = [vp, i32, i32, vp]
+    lib.knnhip_batched_autotune.restype = i32
+    lib.knnhip_batched_table.argtypes = [ctypes.c_char_p, i32]
+    lib.knnhip_batched_table.restype = i32
+    lib.knnhip_batched_set_slice.argtypes = [i32, i32, i32]
+    lib.knnhip_batched_set_slice.restype = i32
     _lib = lib
     return lib
 
@@ -487,3 +502,80 @@ class Router:
             self.close()
         except Exception:
             pass
+
+
+class Problem(ctypes.Structure):
+    """One search of a batch (device pointers; the layout of ``knnhip_problem``)."""
+
+    _fields_ = [
+        ("ref", ctypes.c_void_p), ("ref_nb", ctypes.c_int), ("ref_ld", ctypes.c_longlong),
+        ("query", ctypes.c_void_p), ("query_nb", ctypes.c_int), ("query_ld", ctypes.c_longlong),
+        ("dim", ctypes.c_int), ("k", ctypes.c_int),
+        ("knn_dist", ctypes.c_void_p), ("knn_index", ctypes.c_void_p),
+    ]
+
+
+def _problems(items):
+    """items: (ref, query, k) device tensors (dim x n).  Returns (ctypes array, outputs)."""
+    import torch
+
+    arr = (Problem * len(items))()
+    outs = []
+    for j, (ref, query, k) in enumerate(items):
+        if ref.dtype != torch.float32 or query.dtype != torch.float32 or not ref.is_cuda or \
+                not query.is_cuda or ref.stride(1) != 1 or query.stride(1) != 1:
+            raise KnnError(f"problem {j}: float32 device tensors with contiguous rows expected")
+        d = torch.empty((k, query.shape[1]), dtype=torch.float32, device=query.device)
+        i = torch.empty((k, query.shape[1]), dtype=torch.int32, device=query.device)
+        arr[j] = Problem(ref.data_ptr(), ref.shape[1], ref.stride(0), query.data_ptr(),
+                         query.shape[1], query.stride(0), ref.shape[0], k, d.data_ptr(),
+                         i.data_ptr())
+        outs.append((d, i))
+    return arr, outs
+
+
+def search_batched(items, *, margin: int = 0, stream: Optional[int] = None):
+    """Exact k-NN of many independent problems in one call.
+
+    items: sequence of (ref, query, k) with dim x n float32 device tensors.  Returns a list of
+    (dist, index) pairs, each exactly what ``search_device`` returns for that problem.
+    """
+    import torch
+
+    arr, outs = _problems(items)
+    opt = Options()
+    opt.margin = margin
+    st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    if not load().knnhip_search_batched(arr, len(items), ctypes.byref(opt), st):
+        raise KnnError(last_error())
+    return outs
+
+
+def batched_autotune(items, reps: int = 1, stream: Optional[int] = None) -> list:
+    """Time the tile shapes of every (dim, k) class of these problems; keep the fastest."""
+    import json
+    import torch
+
+    arr, _ = _problems(items)
+    st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    if not load().knnhip_batched_autotune(arr, len(items), reps, st):
+        raise KnnError(last_error())
+    return batched_table()
+
+
+def batched_table() -> list:
+    import json
+
+    n = load().knnhip_batched_table(None, 0)
+    buf = ctypes.create_string_buffer(n)
+    load().knnhip_batched_table(buf, n)
+    return json.loads(buf.value.decode())
+
+
+def batched_set_slice(dim: int, k: int, slice_len: int):
+    if not load().knnhip_batched_set_slice(dim, k, slice_len):
+        raise KnnError(last_error())
+
+
+def batched_last_failures() -> int:
+    return int(load().knnhip_batched_last_failures())
