@@ -55,7 +55,9 @@ typedef struct knnhip_options {
     int slices;         /* reference slices scanned in parallel (default: chosen) */
     int force_fallback; /* 1: every query takes the exact full-scan path (testing) */
     int timing;         /* 1: record per-stage device times (knnhip_last_stage_ms) */
-    int reserved[8];
+    int precision;      /* 0: FP32 products; 1: bfloat16 three-term products (same exact
+                           result; dim <= 256 and k + margin <= 107, else FP32) */
+    int reserved[7];
 } knnhip_options;
 
 /*

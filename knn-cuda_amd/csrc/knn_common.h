@@ -52,6 +52,7 @@ struct ScanArgs {
     const ScanTask *tasks;  // explicit tasks (nullptr: the plain slice x query-block grid)
     int ntasks;           // one block per task
     const int *qlist;     // query ids referenced by the tasks
+    const unsigned short *packed_bf;  // bfloat16 hi/lo tiles (precision scan), else unused
 };
 
 // Arguments of the norms / layout pass.
@@ -67,6 +68,8 @@ struct NormArgs {
     float *pm;           // point-major copy (row stride pm_ld), or nullptr
     int pm_ld;
     float *packed;       // packed tile layout, or nullptr
+    unsigned short *packed_bf;  // bfloat16 hi/lo tile layout of the precision scan, or nullptr
+    int nch_bf;                 // its 16-dimension chunks per tile (>= ceil(dim / 16))
 };
 
 // Arguments of the select / exact re-rank / certificate stage.
@@ -100,6 +103,9 @@ struct FinalArgs {
     int *out_idx;
     int *fb_count;            // queries that failed the certificate
     int *fb_list;
+    // Key error bound of the scan: |key - (||r||^2 - 2 r.q)| <= err_q * ||q||^2 +
+    // err_r * max ||r||^2.  Zero: the FP32 scan's bound (g and 3.1 g).
+    double err_q, err_r;
 };
 
 // Arguments of the exact per-query full scan (fallback).

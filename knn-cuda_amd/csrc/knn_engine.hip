@@ -55,6 +55,9 @@ int scan_pick_ks(int dim);
 size_t scan_lds_bytes(int kp, int waves);
 int scan_blocks_per_cu(int ks, bool multi, int waves, int kp);
 hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s);
+int scan_bf3_nch(int dim);
+size_t scan_bf3_lds_bytes(int kp);
+hipError_t launch_scan_bf3(const ScanArgs &a, int nch, hipStream_t s);
 hipError_t launch_norms(const NormArgs &a, hipStream_t s);
 hipError_t launch_select(const FinalArgs &a, hipStream_t s);
 hipError_t launch_exact(const ExactArgs &a, int grid, hipStream_t s);
@@ -100,7 +103,7 @@ void set_error(const char *fmt, ...) {
 // Grow-only device buffers, one per role.
 enum Slot {
     kRefPad, kQryPad, kRNorm, kQNorm, kRMax, kPartVal, kPartIdx, kFbCount, kFbList,
-    kFbKey, kFbIdx, kOutDist, kOutIdx, kRefPm, kQryPm, kPacked, kTheta, kNumSlots
+    kFbKey, kFbIdx, kOutDist, kOutIdx, kRefPm, kQryPm, kPacked, kTheta, kPackedBf, kNumSlots
 };
 struct Workspace {
     void *ptr[kNumSlots] = {};
@@ -148,7 +151,12 @@ struct Plan {
     int r_pad = 0;            // references rounded up to whole scan steps
     int oct = 0;              // dimension octets of the packed layout
     long long slice_span = 0; // packed bytes per slice
+    bool bf3 = false;         // precision scan (bfloat16 three-term products)
+    int nch = 0;              // its 16-dimension chunks
 };
+
+// Queries per scan block: 32 for the FP32 scan, 128 for the precision scan.
+int block_queries(const Plan &p) { return p.bf3 ? 128 : kQueriesPerWave; }
 
 // Cost model of one slice count: the scan runs in `rounds` waves of
 // resident blocks, each block's four waves sharing ceil(steps / S) reference
@@ -222,6 +230,40 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     p.xcd = (s % 8 == 0) ? 1 : 0;
     p.maxc = std::min(kMaxCand, std::max(2 * p.kp, 256));
     p.use_scan = true;
+    if (o.precision == 1) {
+        // Precision scan: 128 queries per block, one block per CU; slices
+        // fill whole rounds of CUs (the same cost model, resident = CUs).
+        const int nch = scan_bf3_nch(dim);
+        if (nch && scan_bf3_lds_bytes(p.kp) <= kScanLdsMax) {
+            p.bf3 = true;
+            p.nch = nch;
+            p.waves = 8;
+            p.dpad = 16 * nch;
+            p.oct = (dim + 7) / 8;
+            p.nqb = (Q + 127) / 128;
+            p.blocks_per_cu = 1;
+            const long long bstep = (long long)kRefsPerStep * p.dpad * sizeof(float);
+            const int bmin = (int)((steps + 0x7FFFFFFFll / bstep - 1) / (0x7FFFFFFFll / bstep));
+            int bs = std::max(1, bmin);
+            if (o.slices > 0) {
+                bs = std::max(bmin, std::min(o.slices, smax));
+            } else {
+                double best = slice_cost(bs, steps, p.nqb, p.cus) ;
+                for (int c = bs + 1; c <= smax; ++c) {
+                    const double cost = slice_cost(c, steps, p.nqb, p.cus);
+                    if (cost < best) {
+                        best = cost;
+                        bs = c;
+                    }
+                }
+            }
+            const int sps = (steps + bs - 1) / bs;
+            p.slice_len = sps * kRefsPerStep;
+            p.slice_span = (long long)sps * bstep;
+            p.nslices = (steps + sps - 1) / sps;
+            p.xcd = 0;
+        }
+    }
     return p;
 }
 
@@ -231,9 +273,10 @@ std::string plan_json(const Plan &p, int R, int Q, int dim, int k) {
              "{\"ref_nb\": %d, \"query_nb\": %d, \"dim\": %d, \"k\": %d, \"scan\": %s, "
              "\"ks\": %d, \"chunks\": %d, \"dpad\": %d, \"waves_per_block\": %d, \"kp\": %d, "
              "\"slices\": %d, \"slice_len\": %d, \"query_blocks\": %d, \"xcd_map\": %d, "
-             "\"blocks_per_cu\": %d, \"cus\": %d, \"max_candidates\": %d}",
+             "\"blocks_per_cu\": %d, \"cus\": %d, \"max_candidates\": %d, \"precision\": \"%s\"}",
              R, Q, dim, k, p.use_scan ? "true" : "false", p.ks, p.nchunks, p.dpad, p.waves, p.kp,
-             p.nslices, p.slice_len, p.nqb, p.xcd, p.blocks_per_cu, p.cus, p.maxc);
+             p.nslices, p.slice_len, p.nqb, p.xcd, p.blocks_per_cu, p.cus, p.maxc,
+             p.bf3 ? "bf16x3" : "fp32");
     return buf;
 }
 
@@ -279,6 +322,20 @@ bool ensure_device() {
 }
 
 // Core: device pointers, dim rows of ldr / ldq floats.
+// Key error bound of the precision scan (see knn_scan_bf16.hip): with
+// u = 2^-8, n = 3 * 16 * nch + 1 FP32 additions (unit 2^-23, allowing for any
+// rounding inside the MFMA) and the FP32 norm over dim terms,
+//   err_q = 3.01 u^2 + 1.02 g_n,  err_r = g_dim + 2.03 g_n + 3.01 u^2,
+// widened by 5 %.
+void bf3_error_bound(int dim, int nch, double *err_q, double *err_r) {
+    const double u2 = 1.0 / 65536.0;
+    const double n = 3.0 * 16.0 * nch + 1.0;
+    const double gn = n * 1.1920928955078125e-7 / (1.0 - n * 1.1920928955078125e-7);
+    const double gd = dim * 5.9604644775390625e-8 / (1.0 - dim * 5.9604644775390625e-8);
+    *err_q = 1.05 * (3.01 * u2 + 1.02 * gn);
+    *err_r = 1.05 * (gd + 2.03 * gn + 3.01 * u2);
+}
+
 // Buffers of one escalation level: the failed queries' ids, their columns
 // and their results.
 struct Escalation {
@@ -326,17 +383,18 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         KNN_TRY(launch_exact(ea, egrid, s), "exact scan launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[4], s), "event");
     } else {
-        float *rnorm = (float *)ws_get(kRNorm, (size_t)R * sizeof(float));
+        float *rnorm = (float *)ws_get(kRNorm, (size_t)p.r_pad * sizeof(float));
         float *qnorm = (float *)ws_get(kQNorm, (size_t)Q * sizeof(float));
         unsigned *rmax = (unsigned *)ws_get(kRMax, sizeof(unsigned));
-        const size_t npart = (size_t)p.nslices * p.nqb * kQueriesPerWave * p.kp;
+        const size_t npart = (size_t)p.nslices * p.nqb * block_queries(p) * p.kp;
         float *pval = (float *)ws_get(kPartVal, npart * sizeof(float));
         int *pidx = (int *)ws_get(kPartIdx, npart * sizeof(int));
         int *fb_list = (int *)ws_get(kFbList, (size_t)Q * sizeof(int));
         const int pm_ld = (dim + 3) & ~3;
         float *ref_pm = (float *)ws_get(kRefPm, (size_t)R * pm_ld * sizeof(float));
         float *qry_pm = (float *)ws_get(kQryPm, (size_t)Q * pm_ld * sizeof(float));
-        float *packed = (float *)ws_get(kPacked, (size_t)p.r_pad * p.dpad * sizeof(float));
+        // FP32 tiles, or bfloat16 hi/lo tiles (the same bytes) for the precision scan
+        void *packed = ws_get(p.bf3 ? kPackedBf : kPacked, (size_t)p.r_pad * p.dpad * sizeof(float));
         unsigned *theta = (unsigned *)ws_get(kTheta, (size_t)Q * sizeof(unsigned));
         if (!rnorm || !qnorm || !rmax || !pval || !pidx || !fb_list || !ref_pm || !qry_pm ||
             !packed || !theta) {
@@ -345,6 +403,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         }
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[0], s), "event");
         KNN_TRY(hipMemsetAsync(rmax, 0, sizeof(unsigned), s), "memset");
+        if (p.bf3) KNN_TRY(hipMemsetAsync(rnorm, 0, (size_t)p.r_pad * sizeof(float), s), "memset");
         KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)theta, 0xFF800000u /* +inf, order-mapped */, Q, s),
                 "memset");
         NormArgs na{};
@@ -358,7 +417,11 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         na.maxbits = rmax;
         na.pm = ref_pm;
         na.pm_ld = pm_ld;
-        na.packed = packed;
+        if (p.bf3) {
+            na.packed_bf = (unsigned short *)packed;
+            na.nch_bf = p.nch;
+        } else
+            na.packed = (float *)packed;
         KNN_TRY(launch_norms(na, s), "norm kernel launch");
         NormArgs nq{};
         nq.p = query;
@@ -374,7 +437,8 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[1], s), "event");
 
         ScanArgs sa{};
-        sa.packed = packed;
+        sa.packed = p.bf3 ? nullptr : (const float *)packed;
+        sa.packed_bf = p.bf3 ? (const unsigned short *)packed : nullptr;
         sa.oct = p.oct;
         sa.R = R;
         sa.dim = dim;
@@ -393,7 +457,10 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         sa.xcd_map = p.xcd;
         sa.part_val = pval;
         sa.part_idx = pidx;
-        KNN_TRY(launch_scan(sa, p.ks, p.multi, p.waves, s), "scan kernel launch");
+        if (p.bf3)
+            KNN_TRY(launch_scan_bf3(sa, p.nch, s), "precision scan launch");
+        else
+            KNN_TRY(launch_scan(sa, p.ks, p.multi, p.waves, s), "scan kernel launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[2], s), "event");
 
         FinalArgs fa{};
@@ -407,7 +474,8 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         fa.k = k;
         fa.kp = p.kp;
         fa.nrows = p.nslices;
-        fa.row_stride = (long long)p.nqb * kQueriesPerWave;
+        fa.row_stride = (long long)p.nqb * block_queries(p);
+        if (p.bf3) bf3_error_bound(dim, p.nch, &fa.err_q, &fa.err_r);
         fa.maxc = p.maxc;
         fa.part_val = pval;
         fa.part_idx = pidx;
@@ -454,6 +522,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
             o2.margin = next_margin;
             o2.force_fallback = 0;
             o2.timing = 0;
+            o2.precision = 0;  // the rescans use FP32 products
             const Plan p2 = make_plan(R, nfail, dim, k, o2);
             if (!run_device(ref, ldr, R, e.qsub.as<float>(), nfail, nfail, dim, k, e.dsub.as<float>(),
                             e.isub.as<int>(), p2, o2, s, level + 1))

@@ -29,44 +29,12 @@
 // every reference left out of the heaps has key >= the final shared
 // threshold, or >= the kp-th smallest key kept.
 #include "knn_common.h"
+#include "knn_heap.h"
 
 namespace knnhip {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-
-// Row within a 32-row tile of accumulator element i on lane half h of the
-// 32x32 MFMA result (4 consecutive rows per group, groups 8 rows apart).
-__device__ __forceinline__ constexpr int acc_row(int i, int h) {
-    return ((i >> 2) << 3) + (h << 2) + (i & 3);
-}
-
-// Max-heap of kp (key, index) pairs: replace the root by a smaller key and
-// restore the heap.  Returns the new root.
-__device__ __forceinline__ float heap_replace_top(float *hv, int *hi, int kp, float v, int id) {
-    int i = 0;
-    while (true) {
-        int c = 2 * i + 1;
-        if (c >= kp) break;
-        float cv = hv[c];
-        if (c + 1 < kp) {
-            float c2 = hv[c + 1];
-            if (c2 > cv) {
-                cv = c2;
-                ++c;
-            }
-        }
-        if (!(cv > v)) break;
-        hv[i] = cv;
-        hi[i] = hi[c];
-        i = c;
-    }
-    hv[i] = v;
-    hi[i] = id;
-    return hv[0];
-}
-
-__host__ __device__ constexpr int heap_stride(int kp) { return kp | 1; }
 
 // LDS of one scan block: the shared heaps and locks, then one candidate
 // queue per wave.
@@ -75,30 +43,6 @@ constexpr size_t kQueueBytes = (size_t)kQueueCap * 3 * sizeof(float);
 __host__ __device__ constexpr size_t scan_heap_bytes(int kp) {
     return (((size_t)kQueriesPerWave * heap_stride(kp) * 2 + kQueriesPerWave) * sizeof(float) + 15) &
            ~(size_t)15;
-}
-
-// Heap root read that other waves of the block may be updating (an atomic
-// load, so the compiler re-reads it on every use).
-__device__ __forceinline__ float shared_root(float *hv) {
-    return __uint_as_float(__hip_atomic_load(reinterpret_cast<unsigned *>(hv), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP));
-}
-
-// Per-query LDS lock.  LDS instructions of one wave execute in order, so a
-// release store issued after the heap writes publishes them; the fences are
-// compiler-ordering points only (no hardware wait is needed at wavefront
-// scope).
-__device__ __forceinline__ bool try_lock(int *l) {
-    int expected = 0;
-    const bool ok = __hip_atomic_compare_exchange_strong(l, &expected, 1, __ATOMIC_RELAXED,
-                                                         __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    return ok;
-}
-__device__ __forceinline__ void unlock(int *l) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __hip_atomic_store(l, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // KS: k-steps per dimension chunk (B operands held in VGPRs), a multiple of 4.
@@ -449,6 +393,38 @@ hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStre
     return hipErrorInvalidValue;
 }
 
+// bfloat16 hi/lo layout of the precision scan: for tile t of 32 points and
+// 16-dimension chunk c, [hi | lo] x 64 lanes x 8 bfloat16; lane m + 32h holds
+// point 32t+m, dimensions 16c+8h .. +7.  hi = bf16(x), lo = bf16(x - hi),
+// both rounded to nearest even.  Octet j of a point is chunk j/2, half j%2.
+__device__ inline unsigned short bf16_bits(float x) {
+    const unsigned u = __float_as_uint(x);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (unsigned short)((u >> 16) | 0x40u);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline void write_bf3(unsigned short *base, int tile, int nch, int j, int m,
+                                 const float *v) {
+    unsigned short hi[8], lo[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        hi[e] = bf16_bits(v[e]);
+        lo[e] = bf16_bits(v[e] - __uint_as_float((unsigned)hi[e] << 16));
+    }
+    const int c = j >> 1, h = j & 1;
+    unsigned short *dst = base + ((((long long)tile * nch + c) * 2) * kWave + m + 32 * h) * 8;
+    uint4 ph, pl;
+    ph.x = hi[0] | ((unsigned)hi[1] << 16);
+    ph.y = hi[2] | ((unsigned)hi[3] << 16);
+    ph.z = hi[4] | ((unsigned)hi[5] << 16);
+    ph.w = hi[6] | ((unsigned)hi[7] << 16);
+    pl.x = lo[0] | ((unsigned)lo[1] << 16);
+    pl.y = lo[2] | ((unsigned)lo[3] << 16);
+    pl.z = lo[4] | ((unsigned)lo[5] << 16);
+    pl.w = lo[6] | ((unsigned)lo[7] << 16);
+    *reinterpret_cast<uint4 *>(dst) = ph;
+    *reinterpret_cast<uint4 *>(dst + kWave * 8) = pl;
+}
+
 // ---------------------------------------------------------------------------
 // Norms pass: squared norms, the maximum reference norm, the point-major copy
 // used by the exact re-rank, and (references only) the packed tile layout
@@ -485,6 +461,12 @@ __global__ __launch_bounds__(256) void norms_pack_kernel(NormArgs a) {
             *reinterpret_cast<float4 *>(dst) = make_float4(v[0], v[1], v[2], v[3]);
             *reinterpret_cast<float4 *>(dst + 32 * 4) = make_float4(v[4], v[5], v[6], v[7]);
         }
+        if (a.packed_bf) write_bf3(a.packed_bf, tile, a.nch_bf, j, m, v);
+    }
+    if (a.packed_bf && in_pad) {
+        // octets past the dimension, up to the kernel's chunk count, are zero
+        const float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = a.oct; j < 2 * a.nch_bf; ++j) write_bf3(a.packed_bf, tile, a.nch_bf, j, m, z);
     }
     if (real) a.out[i] = s;
     if (a.maxbits) {

@@ -1,0 +1,307 @@
+// Precision-form distance scan: the FP32 scan's key ||r||^2 - 2 r.q computed
+// with bfloat16 MFMA products (v_mfma_f32_32x32x16_bf16, FP32 accumulation)
+// on a three-term split, fused with the same streaming top-kp selection.
+//
+// Split: every operand x (reference coordinate, or -2q) is x_hi + x_lo with
+// x_hi = bf16(x) and x_lo = bf16(x - x_hi); the product r.q' is taken as
+// r_hi.q_hi + r_hi.q_lo + r_lo.q_hi (three MFMAs per 16 dimensions).  With
+// u = 2^-8 each dropped or rounded part is below 3.01 u^2 |r_i q'_i|, so the
+// key stays within err_q ||q||^2 + err_r max||r||^2 of the exact one (the
+// coefficients are set by the host and checked by the re-rank certificate,
+// which re-ranks every candidate with the reference's FP32 formula).  The
+// result is therefore the same as the FP32 path's, bit for bit.
+//
+// Mapping: a block holds 128 queries (four sets of 32) and 8 waves; waves w
+// and w+4 share query set w%4 (its heaps, under the per-query LDS locks)
+// and take tiles {0,1} / {2,3} of every 128-reference step.  Each 16-dim
+// chunk of a step (4 tiles x hi/lo x 64 lanes x 16 bytes = 8 KiB) is loaded
+// once per block, one 16-byte load per thread, into a two-slot LDS ring, so
+// reference traffic is a quarter of the FP32 scan's (32 queries per block).
+// Loads run PF chunks ahead in registers; one barrier per chunk hands a
+// slot from the loading threads to the computing waves.
+#include "knn_common.h"
+#include "knn_heap.h"
+
+namespace knnhip {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kBfWaves = 8;
+constexpr int kBfQueries = 128;
+constexpr int kBfQueueCap = 384;  // >= 4 keys x 64 lanes between drain checks, plus slack
+constexpr size_t kBfQueueBytes = (size_t)kBfQueueCap * 3 * sizeof(float);
+constexpr int kBfChunkBytes = 4 * 2 * kWave * 16;  // one 16-dim chunk of a 128-reference step
+
+__host__ __device__ constexpr size_t bf_heap_bytes(int kp) {
+    return (((size_t)kBfQueries * heap_stride(kp) * 2 + kBfQueries) * sizeof(float) + 15) &
+           ~(size_t)15;
+}
+
+size_t scan_bf3_lds_bytes(int kp) {
+    return bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes + 2 * (size_t)kBfChunkBytes;
+}
+
+__device__ inline bf16x8_t split_hi_lo(const float *x, bf16x8_t &lo) {
+    bf16x8_t hi;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = (__bf16)x[e];
+        hi[e] = h;
+        lo[e] = (__bf16)(x[e] - (float)h);
+    }
+    return hi;
+}
+
+// NCH: 16-dimension chunks (padded dim = 16 NCH); PF: chunks loaded ahead
+// (divides NCH, so the register ring index is static).
+template <int NCH, int PF>
+__global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    static_assert(NCH % PF == 0, "ring position must repeat every step");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 31, h = lane >> 5;
+    const int qset = wave & 3, half = wave >> 2;
+
+    const int t = blockIdx.x;
+    const int slice = t / a.nqb, qb = t - slice * a.nqb;
+    const int qfirst = qb * kBfQueries;
+    const int nq = min(kBfQueries, a.Q - qfirst);
+    if (nq <= 0) return;  // uniform over the block
+    const int r_begin = slice * a.slice_len;
+    const int r_end = min(a.R, r_begin + a.slice_len);
+    const int qlocal = qset * kQueriesPerWave + col;
+    const bool qvalid = qlocal < nq;
+    const int q = qfirst + (qvalid ? qlocal : 0);
+
+    // LDS: heaps + locks (128 queries), 8 candidate queues, 2 ring slots
+    const int kp = a.kp, kps = heap_stride(kp);
+    float *hv = reinterpret_cast<float *>(smem);
+    int *hi = reinterpret_cast<int *>(smem + (size_t)kBfQueries * kps * sizeof(float));
+    int *lock = hi + (size_t)kBfQueries * kps;
+    unsigned char *qbase = smem + bf_heap_bytes(kp) + (size_t)wave * kBfQueueBytes;
+    float *qv = reinterpret_cast<float *>(qbase);
+    int *qr = reinterpret_cast<int *>(qbase + kBfQueueCap * sizeof(float));
+    int *qc = reinterpret_cast<int *>(qbase + kBfQueueCap * 2 * sizeof(float));
+    unsigned char *ring = smem + bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes;
+    for (int e = threadIdx.x; e < kBfQueries * kps; e += blockDim.x) {
+        hv[e] = INFINITY;
+        hi[e] = -1;
+    }
+    if (threadIdx.x < kBfQueries) lock[threadIdx.x] = 0;
+
+    // B operands: -2q split, per chunk (lane: query col, dims 16c + 8h ..)
+    bf16x8_t bh[NCH], bl[NCH];
+    {
+        const float *qcol = a.query + q;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float x[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int d = 16 * c + 8 * h + e;
+                const float v = qcol[(long long)min(d, a.dim - 1) * a.ldq];
+                x[e] = d < a.dim ? -2.0f * v : 0.0f;
+            }
+            bh[c] = split_hi_lo(x, bl[c]);
+        }
+    }
+
+    float *myv = hv + qlocal * kps;
+    float theta = INFINITY;
+    float published = INFINITY;
+    unsigned *theta_q = a.theta + q;
+    int qcount = 0;  // wave-uniform
+    const unsigned long long lanes_below = (1ull << lane) - 1ull;
+    auto drain = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (int base = 0; base < qcount; base += kWave) {
+            const int j = base + lane;
+            bool pending = j < qcount;
+            const float v = pending ? qv[j] : 0.0f;
+            const int r = pending ? qr[j] : 0;
+            const int c = pending ? qc[j] : 0;
+            float *cv = hv + c * kps;
+            int *ci = hi + c * kps;
+            // wave-uniform retry loop (see the FP32 scan): one attempt per lane
+            // per iteration, a lock released in the iteration that took it
+            while (__ballot(pending)) {
+                if (pending) {
+                    if (!(v < shared_root(cv))) {
+                        pending = false;
+                    } else if (try_lock(lock + c)) {
+                        if (v < cv[0]) heap_replace_top(cv, ci, kp, v, r);
+                        unlock(lock + c);
+                        pending = false;
+                    }
+                }
+            }
+        }
+        qcount = 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    // Reference chunks: thread t loads 16 bytes of (tile t/128, hi/lo, lane)
+    // of each chunk; the buffer resource covers the slice's tiles only, so
+    // loads past the slice return zeros.
+    constexpr long long tile_bytes = (long long)NCH * 2 * kWave * 16;
+    const int nsteps = (r_end - r_begin + kRefsPerStep - 1) / kRefsPerStep;
+    const long long valid_tiles =
+        max(0, min(nsteps * kRefsPerStep, a.r_pad - r_begin)) / kRefsPerTile;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<unsigned short *>(a.packed_bf) + (long long)(r_begin / kRefsPerTile) * (tile_bytes / 2),
+        (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
+    const int ltile = threadIdx.x >> 7, lhl = (threadIdx.x >> 6) & 1, lln = threadIdx.x & 63;
+    const unsigned my_off = (unsigned)(ltile * tile_bytes + lhl * kWave * 16 + lln * 16);
+    // chunk g = step * NCH + c lives at step*4 tiles + c * 2 KiB
+    auto chunk_off = [&](int g) -> unsigned {
+        const long long st = g / NCH, c = g - st * NCH;
+        const long long off = st * 4 * tile_bytes + c * 2 * kWave * 16 + my_off;
+        return (unsigned)min(off, 0x7FFFFFF0ll);
+    };
+    u32x4_t pre[PF];
+#pragma unroll
+    for (int j = 0; j < PF; ++j)
+        pre[j] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(j), 0, 0);
+    // chunk 0 into slot 0, chunk PF into its register
+    *reinterpret_cast<u32x4_t *>(ring + threadIdx.x * 16) = pre[0];
+    pre[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(PF), 0, 0);
+
+    const int my_t0 = 2 * half;  // my tiles: my_t0, my_t0 + 1
+    for (int st = 0; st < nsteps; ++st) {
+        const int r0 = r_begin + st * kRefsPerStep;
+        f32x16_t acc[2];
+        // start from ||r||^2 (rows 8(i/4) + 4h + i%4 of each tile)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int g4 = 0; g4 < 4; ++g4) {
+                const float4 nr = *reinterpret_cast<const float4 *>(
+                    a.rnorm + r0 + kRefsPerTile * (my_t0 + mt) + 8 * g4 + 4 * h);
+                acc[mt][4 * g4 + 0] = nr.x;
+                acc[mt][4 * g4 + 1] = nr.y;
+                acc[mt][4 * g4 + 2] = nr.z;
+                acc[mt][4 * g4 + 3] = nr.w;
+            }
+        const unsigned tq = __hip_atomic_load(theta_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int g = st * NCH + c;
+            __syncthreads();  // chunk g is in slot g%2; slot (g+1)%2 is free
+            {
+                // chunk g+1 (register (c+1)%PF) into the other slot, then
+                // reuse that register for chunk g+1+PF
+                const int rslot = (c + 1) % PF;
+                *reinterpret_cast<u32x4_t *>(ring + ((g + 1) & 1) * kBfChunkBytes + threadIdx.x * 16) =
+                    pre[rslot];
+                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 1 + PF), 0, 0);
+            }
+            const unsigned char *slot = ring + (g & 1) * kBfChunkBytes;
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int tt = my_t0 + mt;
+                const u32x4_t ah = *reinterpret_cast<const u32x4_t *>(slot + ((tt * 2 + 0) * kWave + lane) * 16);
+                const u32x4_t al = *reinterpret_cast<const u32x4_t *>(slot + ((tt * 2 + 1) * kWave + lane) * 16);
+                const bf16x8_t a_hi = __builtin_bit_cast(bf16x8_t, ah);
+                const bf16x8_t a_lo = __builtin_bit_cast(bf16x8_t, al);
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, bh[c], acc[mt], 0, 0, 0);
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, bl[c], acc[mt], 0, 0, 0);
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_lo, bh[c], acc[mt], 0, 0, 0);
+            }
+        }
+
+        // Epilogue (as the FP32 scan): keys below the query's threshold go
+        // to the wave's queue, drained into the shared heaps.
+        theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
+        bool cand = false;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) cand |= acc[mt][i] < theta;
+        cand = cand && qvalid;
+        if (__ballot(cand)) {
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    if ((i & 3) == 0 && qcount > kBfQueueCap - 4 * kWave) {
+                        drain();
+                        theta = fminf(theta, shared_root(myv));
+                    }
+                    const int r = r0 + kRefsPerTile * (my_t0 + mt) + acc_row(i, h);
+                    const bool take = qvalid && acc[mt][i] < theta && r < r_end;
+                    const unsigned long long m = __ballot(take);
+                    if (m == 0) continue;
+                    if (take) {
+                        const int pos = qcount + __popcll(m & lanes_below);
+                        qv[pos] = acc[mt][i];
+                        qr[pos] = r;
+                        qc[pos] = qlocal;
+                    }
+                    qcount += __popcll(m);
+                }
+            }
+            if (qcount >= kBfQueueCap / 2) {
+                drain();
+                theta = fminf(theta, shared_root(myv));
+                if (h == 0 && qvalid && theta < published) {
+                    published = theta;
+                    __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+    }
+    drain();
+    theta = fminf(theta, shared_root(myv));
+    if (h == 0 && qvalid && theta < published)
+        __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+
+    // heaps of the block's queries: rows slice * (nqb * 128) + query
+    __syncthreads();
+    const long long obase = ((long long)slice * a.nqb * kBfQueries + qfirst) * kp;
+    for (int e = threadIdx.x; e < nq * kp; e += blockDim.x) {
+        const int row = e / kp, j = e - row * kp;
+        a.part_val[obase + e] = hv[row * kps + j];
+        a.part_idx[obase + e] = hi[row * kps + j];
+    }
+}
+
+#define KNN_BF3_INSTANCES(X) \
+    X(2, 2)                  \
+    X(4, 4)                  \
+    X(6, 3)                  \
+    X(8, 4)                  \
+    X(12, 4)                 \
+    X(16, 4)
+
+int scan_bf3_nch(int dim) {
+    const int n = (dim + 15) / 16;
+    for (int c : {2, 4, 6, 8, 12, 16})
+        if (c >= n) return c;
+    return 0;  // wider dimensions use the FP32 scan
+}
+
+hipError_t launch_scan_bf3(const ScanArgs &a, int nch, hipStream_t s) {
+    const size_t lds = scan_bf3_lds_bytes(a.kp);
+    const int blocks = a.nqb * a.nslices;
+#define X(N, P)                                                                                   \
+    if (nch == N) {                                                                               \
+        const void *fn = (const void *)scan_bf3_kernel<N, P>;                                     \
+        if (lds > 65536) {                                                                        \
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                               (int)lds);                                         \
+            if (e != hipSuccess) return e;                                                        \
+        }                                                                                         \
+        hipLaunchKernelGGL((scan_bf3_kernel<N, P>), dim3(blocks), dim3(kBfWaves * kWave), lds, s, \
+                           a);                                                                    \
+        return hipGetLastError();                                                                 \
+    }
+    KNN_BF3_INSTANCES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+}  // namespace knnhip

@@ -147,8 +147,9 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
             const double g = 2.0 * (double)(a.dim + 4) * 5.9604644775390625e-8;
             const double nq = (double)a.qnorm[q];
             const double nr = (double)__uint_as_float(*a.rmax);
+            const double eq = a.err_q > 0.0 ? a.err_q : g, er = a.err_r > 0.0 ? a.err_r : 3.1 * g;
             const double lower =
-                (tau + nq * (1.0 - g) - g * (nq + 3.1 * nr) - (double)a.dim * 1e-36) * (1.0 - g);
+                (tau + nq * (1.0 - g) - (eq * nq + er * nr) - (double)a.dim * 1e-36) * (1.0 - g);
             const double kth = (double)(*kth_slot) * (1.0 + 2.384185791015625e-7);
             ok = lower > kth * kth;
         }

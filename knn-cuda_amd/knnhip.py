@@ -77,7 +77,8 @@ class Options(ctypes.Structure):
         ("slices", ctypes.c_int),
         ("force_fallback", ctypes.c_int),
         ("timing", ctypes.c_int),
-        ("reserved", ctypes.c_int * 8),
+        ("precision", ctypes.c_int),
+        ("reserved", ctypes.c_int * 7),
     ]
 
 
@@ -218,6 +219,7 @@ def knn_cublas(ref: np.ndarray, query: np.ndarray, k: int):
 
 def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "global",
                   margin: int = 0, slices: int = 0, force_fallback: bool = False,
+                  precision: str = "fp32",
                   timing: bool = False, stream: Optional[int] = None):
     """Exact k-NN on device-resident torch tensors (dim x n, float32, row stride = n or more).
 
@@ -244,6 +246,9 @@ def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "
     opt.slices = slices
     opt.force_fallback = 1 if force_fallback else 0
     opt.timing = 1 if timing else 0
+    if precision not in ("fp32", "bf16x3"):
+        raise KnnError("precision must be 'fp32' or 'bf16x3'")
+    opt.precision = 1 if precision == "bf16x3" else 0
     if stream is None:
         stream = torch.cuda.current_stream(query.device).cuda_stream
     ok = lib.knnhip_search_device(

@@ -1,0 +1,107 @@
+"""GPU tests of the precision form (bfloat16 three-term products): its results equal the oracle's
+bit for bit, on every kernel instance and ragged shape, and fall back to FP32 rescans where its
+error bound is too wide for the data."""
+import numpy as np
+import pytest
+
+import knnhip
+import oracle
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def say(m):
+    print(m, flush=True)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def check(d, i, gd, gi, what):
+    d, i = d.cpu().numpy(), i.cpu().numpy()
+    assert np.array_equal(i, gi), f"{what}: {int((i != gi).sum())} index mismatches"
+    assert np.array_equal(bits(d), bits(gd)), f"{what}: distance mismatches"
+
+
+# (R, Q, dim, k): every chunk-count instance (2, 4, 6, 8, 12, 16 chunks of 16 dims), dims that
+# are not multiples of 16, query counts around the 128-query block, one query, R not a
+# multiple of the 128-reference step
+SHAPES = [
+    (5000, 300, 128, 16),
+    (3000, 130, 40, 8),
+    (4097, 129, 100, 32),
+    (2000, 64, 256, 40),
+    (1500, 1, 24, 1),
+    (6000, 257, 200, 48),
+    (2500, 127, 32, 20),
+    (3333, 200, 96, 16),
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_precision_equals_oracle(shape):
+    R, Q, D, k = shape
+    say(f"precision {shape}")
+    rng = np.random.default_rng(sum(shape))
+    ref = rng.random((D, R), dtype=np.float32)
+    qry = rng.random((D, Q), dtype=np.float32)
+    gd, gi = oracle.knn_c(ref, qry, k)
+    d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k,
+                                precision="bf16x3")
+    torch.cuda.synchronize()
+    assert knnhip.last_plan()["precision"] == "bf16x3"
+    check(d, i, gd, gi, f"{shape}")
+
+
+def test_clustered_data_rescans_in_fp32_and_stays_exact():
+    # tight clusters: the bfloat16 bound is wider than the gap after the k-th neighbour, so most
+    # queries fail the first certificate and are rescanned with FP32 products
+    R, Q, D, k = 20000, 200, 64, 10
+    ref = knnhip.generate_mixture(R, D, 40, 0.01, 91, 92)
+    qry = knnhip.generate_mixture(Q, D, 40, 0.01, 91, 93)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    d, i = knnhip.search_device(ref, qry, k, precision="bf16x3", margin=4)
+    torch.cuda.synchronize()
+    check(d, i, gd, gi, "clustered")
+
+
+def test_ties_and_duplicates():
+    R, Q, D, k = 3000, 150, 48, 12
+    rng = np.random.default_rng(7)
+    ref = np.floor(rng.random((D, R), dtype=np.float32) * 4) / 4  # coarse lattice: many ties
+    ref[:, 1000:1100] = ref[:, :100]  # exact duplicates
+    qry = np.floor(rng.random((D, Q), dtype=np.float32) * 4) / 4
+    gd, gi = oracle.knn_c(ref, qry, k)
+    d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k,
+                                precision="bf16x3")
+    torch.cuda.synchronize()
+    check(d, i, gd, gi, "ties")
+
+
+def test_plan_falls_back_to_fp32_where_not_eligible():
+    ref = knnhip.generate_uniform(4000, 300, seed=5)
+    qry = knnhip.generate_uniform(50, 300, seed=6)
+    knnhip.search_device(ref, qry, 8, precision="bf16x3")
+    assert knnhip.last_plan()["precision"] == "fp32"  # dim 300 > 256
+    ref = knnhip.generate_uniform(4000, 64, seed=5)
+    qry = knnhip.generate_uniform(50, 64, seed=6)
+    knnhip.search_device(ref, qry, 64, precision="bf16x3")
+    assert knnhip.last_plan()["precision"] == "fp32"  # k + margin = 128 > 107
+
+
+def test_precision_equals_fp32_at_1m_refs():
+    # 1,048,576 uniform refs, 2,048 queries, dim 128, k 32 (the plain workload's data): the
+    # precision result equals the FP32 result bit for bit, and both match the oracle on a sample
+    R, Q, D, k = 1 << 20, 2048, 128, 32
+    ref = knnhip.generate_uniform(R, D, seed=0x6B4E4E02)
+    qry = knnhip.generate_uniform(Q, D, seed=0x6B4E4E12)
+    d32, i32 = knnhip.search_device(ref, qry, k)
+    d16, i16 = knnhip.search_device(ref, qry, k, precision="bf16x3")
+    torch.cuda.synchronize()
+    say(f"plan {knnhip.last_plan()}, exact-scan queries {knnhip.last_fallback_count()}")
+    assert torch.equal(i16, i32) and torch.equal(d16, d32)
+    sample = [0, 777, 2047]
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), np.ascontiguousarray(qry.cpu().numpy()[:, sample]), k)
+    check(d16[:, sample], i16[:, sample], gd, gi, "oracle sample")
