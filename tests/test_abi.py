@@ -96,4 +96,7 @@ def test_options_struct_matches_header():
     fields = re.search(r"typedef struct knnhip_options \{(.*?)\} knnhip_options;", text, re.S).group(1)
     names = re.findall(r"int\s+(\w+)(?:\[\d+\])?;", fields)
     assert names == [f[0] for f in knnhip.Options._fields_]
-    assert ctypes.sizeof(knnhip.Options) == 4 * (len(names) - 1 + 8)
+    arrays = [int(n) for n in re.findall(r"int\s+\w+\[(\d+)\];", fields)]
+    ints = len(names) - len(arrays) + sum(arrays)
+    assert ctypes.sizeof(knnhip.Options) == 4 * ints
+    assert ints == 13  # the layout callers compiled against: 13 ints
