@@ -56,7 +56,7 @@ typedef struct knnhip_options {
     int force_fallback; /* 1: every query takes the exact full-scan path (testing) */
     int timing;         /* 1: record per-stage device times (knnhip_last_stage_ms) */
     int precision;      /* 0: FP32 products; 1: bfloat16 three-term products (same exact
-                           result; dim <= 256 and k + margin <= 107, else FP32) */
+                           result; dim <= 192 and k + margin <= 107, else FP32) */
     int reserved[7];
 } knnhip_options;
 

@@ -15,10 +15,11 @@
 // and w+4 share query set w%4 (its heaps, under the per-query LDS locks)
 // and take tiles {0,1} / {2,3} of every 128-reference step.  Each 16-dim
 // chunk of a step (4 tiles x hi/lo x 64 lanes x 16 bytes = 8 KiB) is loaded
-// once per block, one 16-byte load per thread, into a two-slot LDS ring, so
-// reference traffic is a quarter of the FP32 scan's (32 queries per block).
-// Loads run PF chunks ahead in registers; one barrier per chunk hands a
-// slot from the loading threads to the computing waves.
+// once per block, one 16-byte load per thread, into a three-slot LDS ring,
+// so reference traffic is a quarter of the FP32 scan's (32 queries per
+// block).  Global loads run PF chunks ahead in registers; each wave reads a
+// chunk's operands from LDS one chunk ahead of its MFMAs; one barrier per
+// chunk hands slots between the loading threads and the computing waves.
 #include "knn_common.h"
 #include "knn_heap.h"
 
@@ -33,6 +34,7 @@ constexpr int kBfQueries = 128;
 constexpr int kBfQueueCap = 384;  // >= 4 keys x 64 lanes between drain checks, plus slack
 constexpr size_t kBfQueueBytes = (size_t)kBfQueueCap * 3 * sizeof(float);
 constexpr int kBfChunkBytes = 4 * 2 * kWave * 16;  // one 16-dim chunk of a 128-reference step
+constexpr int kBfRing = 3;                         // LDS slots
 
 __host__ __device__ constexpr size_t bf_heap_bytes(int kp) {
     return (((size_t)kBfQueries * heap_stride(kp) * 2 + kBfQueries) * sizeof(float) + 15) &
@@ -40,7 +42,7 @@ __host__ __device__ constexpr size_t bf_heap_bytes(int kp) {
 }
 
 size_t scan_bf3_lds_bytes(int kp) {
-    return bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes + 2 * (size_t)kBfChunkBytes;
+    return bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes + kBfRing * (size_t)kBfChunkBytes;
 }
 
 __device__ inline bf16x8_t split_hi_lo(const float *x, bf16x8_t &lo) {
@@ -164,11 +166,28 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs 
 #pragma unroll
     for (int j = 0; j < PF; ++j)
         pre[j] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(j), 0, 0);
-    // chunk 0 into slot 0, chunk PF into its register
+    // Ring invariant at the top of chunk g: slot g%3 and slot (g+1)%3 hold
+    // chunks g and g+1 (stored in iterations g-2 and g-1), pre[x % PF] holds
+    // chunk x for g+2 <= x < g+2+PF, and this wave's operands of chunk g are
+    // in cur[].  Prologue: chunks 0 and 1 into slots 0 and 1.
     *reinterpret_cast<u32x4_t *>(ring + threadIdx.x * 16) = pre[0];
+    *reinterpret_cast<u32x4_t *>(ring + kBfChunkBytes + threadIdx.x * 16) = pre[1 % PF];
     pre[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(PF), 0, 0);
-
+    if (PF > 1) pre[1 % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(PF + 1), 0, 0);
+    __syncthreads();
     const int my_t0 = 2 * half;  // my tiles: my_t0, my_t0 + 1
+    auto read_ops = [&](int slotidx, u32x4_t *ops) {
+        const unsigned char *slot = ring + slotidx * kBfChunkBytes;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int hl = 0; hl < 2; ++hl)
+                ops[2 * mt + hl] =
+                    *reinterpret_cast<const u32x4_t *>(slot + (((my_t0 + mt) * 2 + hl) * kWave + lane) * 16);
+    };
+    u32x4_t cur[4];
+    read_ops(0, cur);
+
     for (int st = 0; st < nsteps; ++st) {
         const int r0 = r_begin + st * kRefsPerStep;
         f32x16_t acc[2];
@@ -188,27 +207,30 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs 
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const int g = st * NCH + c;
-            __syncthreads();  // chunk g is in slot g%2; slot (g+1)%2 is free
+            // next chunk's operands (slot (g+1)%3, complete since the last barrier)
+            u32x4_t nxt[4];
+            read_ops((g + 1) % kBfRing, nxt);
+            // chunk g+2 into slot (g+2)%3 (it held chunk g-1, which every wave
+            // finished reading before the last barrier); refill its register
             {
-                // chunk g+1 (register (c+1)%PF) into the other slot, then
-                // reuse that register for chunk g+1+PF
-                const int rslot = (c + 1) % PF;
-                *reinterpret_cast<u32x4_t *>(ring + ((g + 1) & 1) * kBfChunkBytes + threadIdx.x * 16) =
+                const int rslot = (c + 2) % PF;
+                *reinterpret_cast<u32x4_t *>(ring + ((g + 2) % kBfRing) * kBfChunkBytes + threadIdx.x * 16) =
                     pre[rslot];
-                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 1 + PF), 0, 0);
+                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 2 + PF), 0, 0);
             }
-            const unsigned char *slot = ring + (g & 1) * kBfChunkBytes;
+            const bf16x8_t a0h = __builtin_bit_cast(bf16x8_t, cur[0]);
+            const bf16x8_t a0l = __builtin_bit_cast(bf16x8_t, cur[1]);
+            const bf16x8_t a1h = __builtin_bit_cast(bf16x8_t, cur[2]);
+            const bf16x8_t a1l = __builtin_bit_cast(bf16x8_t, cur[3]);
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bh[c], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bh[c], acc[1], 0, 0, 0);
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bl[c], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bl[c], acc[1], 0, 0, 0);
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, bh[c], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, bh[c], acc[1], 0, 0, 0);
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                const int tt = my_t0 + mt;
-                const u32x4_t ah = *reinterpret_cast<const u32x4_t *>(slot + ((tt * 2 + 0) * kWave + lane) * 16);
-                const u32x4_t al = *reinterpret_cast<const u32x4_t *>(slot + ((tt * 2 + 1) * kWave + lane) * 16);
-                const bf16x8_t a_hi = __builtin_bit_cast(bf16x8_t, ah);
-                const bf16x8_t a_lo = __builtin_bit_cast(bf16x8_t, al);
-                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, bh[c], acc[mt], 0, 0, 0);
-                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_hi, bl[c], acc[mt], 0, 0, 0);
-                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_lo, bh[c], acc[mt], 0, 0, 0);
-            }
+            for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
+            __syncthreads();  // chunk g+2 stored; everyone is past chunk g's reads
         }
 
         // Epilogue (as the FP32 scan): keys below the query's threshold go
@@ -274,14 +296,13 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs 
     X(4, 4)                  \
     X(6, 3)                  \
     X(8, 4)                  \
-    X(12, 4)                 \
-    X(16, 4)
+    X(12, 4)
 
 int scan_bf3_nch(int dim) {
     const int n = (dim + 15) / 16;
-    for (int c : {2, 4, 6, 8, 12, 16})
+    for (int c : {2, 4, 6, 8, 12})
         if (c >= n) return c;
-    return 0;  // wider dimensions use the FP32 scan
+    return 0;  // wider dimensions use the FP32 scan (the B operands would not fit in VGPRs)
 }
 
 hipError_t launch_scan_bf3(const ScanArgs &a, int nch, hipStream_t s) {

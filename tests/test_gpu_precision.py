@@ -25,16 +25,16 @@ def check(d, i, gd, gi, what):
     assert np.array_equal(bits(d), bits(gd)), f"{what}: distance mismatches"
 
 
-# (R, Q, dim, k): every chunk-count instance (2, 4, 6, 8, 12, 16 chunks of 16 dims), dims that
+# (R, Q, dim, k): every chunk-count instance (2, 4, 6, 8, 12 chunks of 16 dims), dims that
 # are not multiples of 16, query counts around the 128-query block, one query, R not a
 # multiple of the 128-reference step
 SHAPES = [
     (5000, 300, 128, 16),
     (3000, 130, 40, 8),
     (4097, 129, 100, 32),
-    (2000, 64, 256, 40),
+    (2000, 64, 192, 40),
     (1500, 1, 24, 1),
-    (6000, 257, 200, 48),
+    (6000, 257, 180, 48),
     (2500, 127, 32, 20),
     (3333, 200, 96, 16),
 ]
@@ -84,7 +84,7 @@ def test_plan_falls_back_to_fp32_where_not_eligible():
     ref = knnhip.generate_uniform(4000, 300, seed=5)
     qry = knnhip.generate_uniform(50, 300, seed=6)
     knnhip.search_device(ref, qry, 8, precision="bf16x3")
-    assert knnhip.last_plan()["precision"] == "fp32"  # dim 300 > 256
+    assert knnhip.last_plan()["precision"] == "fp32"  # dim 300 > 192
     ref = knnhip.generate_uniform(4000, 64, seed=5)
     qry = knnhip.generate_uniform(50, 64, seed=6)
     knnhip.search_device(ref, qry, 64, precision="bf16x3")
