@@ -13,6 +13,8 @@ Prints one JSON line.  `roofline` is measured live for the dominant kernel
 when present.  `cpu_baseline` times the oracle's single-threaded restatement
 of the reference ground truth on a bounded query sample of the same data.
 
+`precision_form` runs the bfloat16 three-term scan on the same data (equality
+with the FP32 result checked) and the dense-cluster workload (dim 256, k 64).
 `router_form` measures the second half of the metric on the clustered
 workload (4,194,304 refs, 16,384 queries, dim 96, k 16, 2,048 blocks): the
 plain FP32 search against the block-index search on the same data, both exact
@@ -48,6 +50,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision-form", action="store_true",
+                    help="skip the precision-form measurement (bfloat16 three-term scan)")
     ap.add_argument("--no-router-form", action="store_true",
                     help="skip the router-form measurement (4M clustered refs, block index)")
     ap.add_argument("--router-reps", type=int, default=5)
@@ -57,6 +61,64 @@ def parse():
                     help="run one device copy of this many MiB before the timed region (a "
                          "known-size kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def precision_form(args, knnhip, torch, ref, qry, k, dist, idx, fp32_ms, fp32_scan_ms):
+    """Precision form (bfloat16 three-term products, exact re-rank) on the plain workload's own
+    data, against the FP32 result (compared for equality); and the dense-cluster workload
+    (1,048,576 x 8,192, dim 256, k 64, mixture of 1,024 Gaussians, sigma 0.05), where the
+    three-term bound is too wide for the neighbour gaps and the search runs FP32 products with
+    the k+64 candidate list."""
+    import time
+
+    R, D = ref.shape[1], ref.shape[0]
+    Q = qry.shape[1]
+    d16 = torch.empty_like(dist)
+    i16 = torch.empty_like(idx)
+    stream = torch.cuda.current_stream()
+    knnhip.search_device(ref, qry, k, d16, i16, precision="bf16x3", stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    scan = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        knnhip.search_device(ref, qry, k, d16, i16, precision="bf16x3", timing=True,
+                             stream=stream.cuda_stream)
+        scan.append(knnhip.last_stage_ms(1))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / args.steps
+    plan = knnhip.last_plan()
+    equal = bool(torch.equal(d16, dist) and torch.equal(i16, idx))
+    scan_ms = sum(scan) / len(scan)
+    flops = 2.0 * R * Q * D
+    out = {"workload": f"{R} x {Q}, dim {D}, k {k}, uniform [0,1) (the plain workload's data)",
+           "ms_per_search": ms, "queries_per_s": Q / ms * 1e3, "speedup_vs_fp32": fp32_ms / ms,
+           "scan_ms": scan_ms, "scan_speedup_vs_fp32": fp32_scan_ms / scan_ms,
+           "bf16_tflops_executed": 3.0 * flops / scan_ms / 1e9,
+           "bf16_peak_frac": 3.0 * flops / scan_ms / 1e9 / 2500.0,
+           "fp32_equivalent_tflops": flops / scan_ms / 1e9,
+           "exact_scan_queries": knnhip.last_fallback_count(), "equal_to_fp32": equal,
+           "precision": plan.get("precision")}
+    # dense clusters (k 64)
+    R3, Q3, D3, k3 = 1 << 20, 8192, 256, 64
+    ref3 = knnhip.generate_mixture(R3, D3, 1024, 0.05, 0x6B4E4E03, 0x6B4E4E13)
+    qry3 = knnhip.generate_mixture(Q3, D3, 1024, 0.05, 0x6B4E4E03, 0x6B4E4E23)
+    knnhip.search_device(ref3, qry3, k3, precision="bf16x3")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        knnhip.search_device(ref3, qry3, k3, precision="bf16x3")
+    torch.cuda.synchronize()
+    ms3 = (time.perf_counter() - t) * 1e3 / reps
+    p3 = knnhip.last_plan()
+    out["dense_clusters"] = {"workload": f"{R3} x {Q3}, dim {D3}, k {k3}, mixture of 1,024 Gaussians "
+                                         "(sigma 0.05)",
+                             "ms_per_search": ms3, "queries_per_s": Q3 / ms3 * 1e3,
+                             "precision": p3.get("precision"), "kp": p3.get("kp"),
+                             "exact_scan_queries": knnhip.last_fallback_count(),
+                             "fp32_scan_tflops": 2.0 * R3 * Q3 * D3 / ms3 / 1e9}
+    del ref3, qry3
+    return out
 
 
 def router_form(args, knnhip, torch):
@@ -251,6 +313,8 @@ def main():
                          f"single-threaded oracle restatement of knn_c, {cpu_s:.1f} s"}
 
     plan = knnhip.last_plan()
+    precision = None if args.no_precision_form else precision_form(args, knnhip, torch, ref, qry, k,
+                                                                      dist, idx, ms_per_step, scan_avg)
     router = None if args.no_router_form else router_form(args, knnhip, torch)
     batched = None if args.no_batched_form else batched_form(knnhip, torch)
 
@@ -279,6 +343,7 @@ def main():
         "fallback_queries": fallbacks,
         "plan": plan,
         "parity_gate_queries": gate_q,
+        "precision_form": precision,
         "router_form": router,
         "batched_form": batched,
     }
