@@ -46,6 +46,13 @@ int knnhip_texture(const float *ref, int ref_nb, const float *query, int query_n
                    float *knn_dist, int *knn_index);
 int knnhip_cublas(const float *ref, int ref_nb, const float *query, int query_nb, int dim, int k,
                   float *knn_dist, int *knn_index);
+/* Distance to the k-th nearest neighbour of every query (query_nb floats,
+ * host pointers), as the reference's legacy programs without indexes return
+ * it (legacy/KNN.CUDA.without.indexes/knn_cuda_without_indexes.cu:278,
+ * knn(ref, ref_width, query, query_width, height, k, dist)): equal to row k-1
+ * of knn_cuda_global's distances. */
+int knnhip_kth_distance(const float *ref, int ref_nb, const float *query, int query_nb, int dim,
+                        int k, float *kth_dist);
 
 /* Options of the extended entry point.  Zero-initialise, then set fields;
  * zero means "library default" for every field. */
@@ -122,6 +129,12 @@ int knnhip_index_search(knnhip_index *index, const float *query, int query_nb, l
                         const knnhip_options *opt, void *stream);
 void knnhip_index_free(knnhip_index *index);
 int knnhip_index_info(const knnhip_index *index, int *nblocks, int *n_pos, float *build_ms);
+/* Versioned file of an index's build (block layout, permutation, balls; see
+ * knn_engine.hip for the byte layout).  Loading rebuilds the tiles from the
+ * same reference set (ref_nb and dim must match the file). */
+int knnhip_index_save(const knnhip_index *index, const char *path);
+int knnhip_index_load(const char *path, const float *ref, int ref_nb, long long ref_ld, int dim,
+                      knnhip_index **out, void *stream);
 /* Device times (ms) of the last search made with options.timing = 1: query
  * norms, block bounds, block choice, task build, scan, select/re-rank,
  * full-scan fallback (n <= 7 values written). */
@@ -169,6 +182,9 @@ int knnhip_router_step(knnhip_router *router, const float *x, int n, long long l
 int knnhip_router_train(knnhip_router *router, const knnhip_index *index, const float *train_query,
                         int n, long long ld, int k, int epochs, int batch, float lr,
                         unsigned long long seed, float *final_loss, void *stream);
+/* Versioned file of a router's parameters (dimensions + FP32 vector). */
+int knnhip_router_save(const knnhip_router *router, const char *path);
+int knnhip_router_load(const char *path, knnhip_router **out);
 /* Block scores of query_nb queries into scores (nblocks x query_nb). */
 int knnhip_router_scores(knnhip_router *router, const float *query, int query_nb,
                          long long query_ld, float *scores, void *stream);

@@ -560,11 +560,14 @@ knnhip_options default_options(int variant) {
 }
 
 // Host-pointer form shared by the three drop-in entry points.
+// kth_only: knn_dist receives query_nb floats, the distance to each query's
+// k-th neighbour (row k-1 of the full result), and knn_index is not used.
 bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k,
-              float *knn_dist, int *knn_index, int variant) {
+              float *knn_dist, int *knn_index, int variant, bool kth_only = false) {
     std::lock_guard<std::mutex> lock(g_mu);
     g_last_error.clear();
-    if (!check_args(ref, R, query, Q, dim, k, knn_dist, knn_index)) return false;
+    if (!check_args(ref, R, query, Q, dim, k, knn_dist, kth_only ? (const void *)knn_dist : knn_index))
+        return false;
     if (!ensure_device()) return false;
     const knnhip_options o = default_options(variant);
     const Plan p = make_plan(R, Q, dim, k, o);
@@ -584,6 +587,13 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
                            s),
             "Unable to copy data from host to device");
     if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s)) return false;
+    if (kth_only) {
+        KNN_TRY(hipMemcpyAsync(knn_dist, dist_d + (size_t)(k - 1) * Q, (size_t)Q * sizeof(float),
+                               hipMemcpyDeviceToHost, s),
+                "Unable to copy data from device to host");
+        KNN_TRY(hipStreamSynchronize(s), "Unable to execute kernel");
+        return true;
+    }
     KNN_TRY(hipMemcpyAsync(knn_dist, dist_d, (size_t)k * Q * sizeof(float), hipMemcpyDeviceToHost,
                            s),
             "Unable to copy data from device to host");
@@ -835,6 +845,151 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
     KNN_TRY(hipEventElapsedTime(&ix->build_ms, e0, e1), "event");
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return true;
+}
+
+// On-disk format of a block index (little-endian): magic "KNNHIPIX", uint32
+// version (1), int32 R, dim, nb, nf, n_pos, r_pad; then block_start [nb+1],
+// block_count [nb], perm [r_pad] (int32), ball centres [nf][pm_ld], centre
+// norms [nf] (float32), radii [nf] (float64), ball blocks [nf] and ball
+// starts [nb+1] (int32).  The packed tiles, point-major copy and norms are
+// rebuilt from the caller's reference set when the file is loaded.
+constexpr char kIndexMagic[8] = {'K', 'N', 'N', 'H', 'I', 'P', 'I', 'X'};
+constexpr unsigned kIndexVersion = 1;
+
+struct FileOut {
+    FILE *f;
+    bool ok = true;
+    void put(const void *p, size_t n) { ok = ok && fwrite(p, 1, n, f) == n; }
+    void put_dev(const DevBuf &b, size_t n) {
+        std::vector<char> h(n);
+        ok = ok && hipMemcpy(h.data(), b.p, n, hipMemcpyDeviceToHost) == hipSuccess;
+        put(h.data(), n);
+    }
+};
+struct FileIn {
+    FILE *f;
+    bool ok = true;
+    void get(void *p, size_t n) { ok = ok && fread(p, 1, n, f) == n; }
+    void get_dev(DevBuf &b, size_t n) {
+        std::vector<char> h(n);
+        get(h.data(), n);
+        ok = ok && b.alloc(n) && hipMemcpy(b.p, h.data(), n, hipMemcpyHostToDevice) == hipSuccess;
+    }
+};
+
+bool index_save(const knnhip_index *ix, const char *path) {
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_error("cannot open %s for writing", path);
+        return false;
+    }
+    FileOut o{f};
+    const int hdr[6] = {ix->R, ix->dim, ix->nb, ix->nf, ix->n_pos, ix->r_pad};
+    o.put(kIndexMagic, 8);
+    o.put(&kIndexVersion, 4);
+    o.put(hdr, sizeof(hdr));
+    o.put_dev(ix->block_start, (size_t)(ix->nb + 1) * 4);
+    o.put_dev(ix->block_count, (size_t)ix->nb * 4);
+    o.put_dev(ix->perm, (size_t)ix->r_pad * 4);
+    o.put_dev(ix->fcent, (size_t)ix->nf * ix->pm_ld * 4);
+    o.put_dev(ix->fnorm, (size_t)ix->nf * 4);
+    o.put_dev(ix->frad, (size_t)ix->nf * 8);
+    o.put_dev(ix->fblock, (size_t)ix->nf * 4);
+    o.put_dev(ix->fb_start, (size_t)(ix->nb + 1) * 4);
+    const bool ok = o.ok && fclose(f) == 0;
+    if (!ok) set_error("cannot write %s", path);
+    return ok;
+}
+
+bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, long long ldr, int dim,
+                hipStream_t s) {
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        set_error("cannot open %s", path);
+        return false;
+    }
+    FileIn in{f};
+    char magic[8] = {0};
+    unsigned version = 0;
+    int hdr[6] = {0};
+    in.get(magic, 8);
+    in.get(&version, 4);
+    in.get(hdr, sizeof(hdr));
+    if (!in.ok || memcmp(magic, kIndexMagic, 8) != 0 || version != kIndexVersion) {
+        fclose(f);
+        set_error("%s is not a block index file of version %u", path, kIndexVersion);
+        return false;
+    }
+    if (hdr[0] != R || hdr[1] != dim || hdr[2] < 1 || hdr[3] < 1 || hdr[4] < R || hdr[5] < hdr[4]) {
+        fclose(f);
+        set_error("%s indexes %d points of dim %d, not %d of dim %d", path, hdr[0], hdr[1], R, dim);
+        return false;
+    }
+    ix->R = R;
+    ix->dim = dim;
+    ix->nb = hdr[2];
+    ix->nf = hdr[3];
+    ix->n_pos = hdr[4];
+    ix->r_pad = hdr[5];
+    ix->ref = ref;
+    ix->ldr = ldr;
+    ix->pm_ld = (dim + 3) & ~3;
+    knnhip_options o;
+    memset(&o, 0, sizeof(o));
+    const Plan p = make_plan(R, 32, dim, 1, o);
+    if (!p.use_scan || ix->pm_ld > kIndexMaxLd) {
+        fclose(f);
+        set_error("dimension too large for the block index");
+        return false;
+    }
+    ix->ks = p.ks;
+    ix->multi = p.multi;
+    ix->nchunks = p.nchunks;
+    ix->dpad = p.dpad;
+    ix->oct = p.oct;
+    in.get_dev(ix->block_start, (size_t)(ix->nb + 1) * 4);
+    in.get_dev(ix->block_count, (size_t)ix->nb * 4);
+    in.get_dev(ix->perm, (size_t)ix->r_pad * 4);
+    in.get_dev(ix->fcent, (size_t)ix->nf * ix->pm_ld * 4);
+    in.get_dev(ix->fnorm, (size_t)ix->nf * 4);
+    in.get_dev(ix->frad, (size_t)ix->nf * 8);
+    in.get_dev(ix->fblock, (size_t)ix->nf * 4);
+    in.get_dev(ix->fb_start, (size_t)(ix->nb + 1) * 4);
+    char extra;
+    const bool tail = fread(&extra, 1, 1, f) == 0;
+    fclose(f);
+    if (!in.ok || !tail) {
+        set_error("%s is truncated or has trailing data", path);
+        return false;
+    }
+    // rebuild the layouts from the references
+    DevBuf pm0, norms0;
+    if (!pm0.alloc((size_t)R * ix->pm_ld * 4) || !norms0.alloc((size_t)R * 4) || !ix->rmax.alloc(4) ||
+        !ix->packed.alloc((size_t)ix->r_pad * ix->dpad * 4) ||
+        !ix->pm.alloc((size_t)ix->r_pad * ix->pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(hipMemsetAsync(ix->rmax.p, 0, 4, s), "memset");
+    NormArgs na{};
+    na.p = ref;
+    na.ld = ldr;
+    na.n = R;
+    na.n_pad = R;
+    na.dim = dim;
+    na.oct = (dim + 7) / 8;
+    na.out = norms0.as<float>();
+    na.maxbits = ix->rmax.as<unsigned>();
+    na.pm = pm0.as<float>();
+    na.pm_ld = ix->pm_ld;
+    KNN_TRY(launch_norms(na, s), "norm kernel launch");
+    KNN_TRY(hipMemsetAsync(ix->pm.p, 0, (size_t)ix->r_pad * ix->pm_ld * 4, s), "memset");
+    KNN_TRY(idx_gather_pack(pm0.as<float>(), ix->pm_ld, dim, ix->oct, ix->perm.as<int>(), ix->n_pos,
+                            ix->r_pad, ix->packed.as<float>(), ix->pm.as<float>(), ix->norms.as<float>(),
+                            s),
+            "gather");
+    KNN_TRY(hipStreamSynchronize(s), "sync");
     return true;
 }
 
@@ -1653,6 +1808,15 @@ int knnhip_cublas(const float *ref, int ref_nb, const float *query, int query_nb
     return knn_cublas(ref, ref_nb, query, query_nb, dim, k, knn_dist, knn_index) ? 1 : 0;
 }
 
+// The k-th-neighbour distance of every query, as the reference's legacy
+// programs without indexes return it (legacy/KNN.CUDA.without.indexes/
+// knn_cuda_without_indexes.cu:278-398): kth_dist[q] = row k-1 of
+// knn_cuda_global's distances, bit for bit.
+int knnhip_kth_distance(const float *ref, int ref_nb, const float *query, int query_nb, int dim,
+                        int k, float *kth_dist) {
+    return report(run_host(ref, ref_nb, query, query_nb, dim, k, kth_dist, nullptr, 0, true)) ? 1 : 0;
+}
+
 int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const float *query,
                          int query_nb, long long query_ld, int dim, int k, float *knn_dist,
                          int *knn_index, const knnhip_options *opt, void *stream) {
@@ -1782,6 +1946,35 @@ void knnhip_index_free(knnhip_index *ix) {
     delete ix;
 }
 
+int knnhip_index_save(const knnhip_index *ix, const char *path) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ix || !path) {
+        set_error("bad index save arguments");
+        return 0;
+    }
+    return index_save(ix, path) ? 1 : 0;
+}
+
+int knnhip_index_load(const char *path, const float *ref, int ref_nb, long long ref_ld, int dim,
+                      knnhip_index **out, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!path || !ref || !out || ref_nb < 1 || dim < 1 || ref_ld < ref_nb) {
+        set_error("bad index load arguments");
+        return 0;
+    }
+    *out = nullptr;
+    if (!ensure_device()) return 0;
+    knnhip_index *ix = new knnhip_index();
+    if (!index_load(ix, path, ref, ref_nb, ref_ld, dim, (hipStream_t)stream)) {
+        delete ix;
+        return 0;
+    }
+    *out = ix;
+    return 1;
+}
+
 int knnhip_index_stage_ms(const knnhip_index *ix, float *ms, int n) {
     if (!ix || !ms) return 0;
     const float all[7] = {ix->rank_ms[0], ix->rank_ms[1], ix->rank_ms[2], ix->last_ms[1],
@@ -1857,6 +2050,83 @@ int knnhip_router_create(int dim, int hidden, int nblocks, unsigned long long se
 void knnhip_router_free(knnhip_router *r) {
     std::lock_guard<std::mutex> lock(g_mu);
     delete r;
+}
+
+// On-disk format of a router: magic "KNNHIPRT", uint32 version (1), int32 dim,
+// hidden, nblocks, int64 parameter count, then the FP32 parameter vector
+// (the layout of knnhip_router_params).  Optimiser state is not stored.
+int knnhip_router_save(const knnhip_router *r, const char *path) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!r || !path) {
+        set_error("bad router save arguments");
+        return 0;
+    }
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_error("cannot open %s for writing", path);
+        return 0;
+    }
+    static const char magic[8] = {'K', 'N', 'N', 'H', 'I', 'P', 'R', 'T'};
+    const unsigned version = 1;
+    const int hdr[3] = {r->dim, r->H, r->nb};
+    const long long n = r->lay.total;
+    FileOut o{f};
+    o.put(magic, 8);
+    o.put(&version, 4);
+    o.put(hdr, sizeof(hdr));
+    o.put(&n, 8);
+    o.put_dev(r->params, (size_t)n * 4);
+    const bool ok = o.ok && fclose(f) == 0;
+    if (!ok) set_error("cannot write %s", path);
+    return ok ? 1 : 0;
+}
+
+int knnhip_router_load(const char *path, knnhip_router **out) {
+    if (!path || !out) {
+        std::lock_guard<std::mutex> lock(g_mu);
+        set_error("bad router load arguments");
+        return 0;
+    }
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        std::lock_guard<std::mutex> lock(g_mu);
+        set_error("cannot open %s", path);
+        return 0;
+    }
+    char magic[8] = {0};
+    unsigned version = 0;
+    int hdr[3] = {0};
+    long long n = 0;
+    FileIn in{f};
+    in.get(magic, 8);
+    in.get(&version, 4);
+    in.get(hdr, sizeof(hdr));
+    in.get(&n, 8);
+    if (!in.ok || memcmp(magic, "KNNHIPRT", 8) != 0 || version != 1 || n != router_layout(hdr[0], hdr[1], hdr[2]).total) {
+        fclose(f);
+        std::lock_guard<std::mutex> lock(g_mu);
+        set_error("%s is not a router file of version 1", path);
+        return 0;
+    }
+    std::vector<float> params((size_t)n);
+    in.get(params.data(), (size_t)n * 4);
+    char extra;
+    const bool tail = fread(&extra, 1, 1, f) == 0;
+    fclose(f);
+    if (!in.ok || !tail) {
+        std::lock_guard<std::mutex> lock(g_mu);
+        set_error("%s is truncated or has trailing data", path);
+        return 0;
+    }
+    knnhip_router *r = nullptr;
+    if (!knnhip_router_create(hdr[0], hdr[1], hdr[2], 0, &r)) return 0;
+    if (!knnhip_router_params(r, params.data(), n, 1)) {
+        knnhip_router_free(r);
+        return 0;
+    }
+    *out = r;
+    return 1;
 }
 
 long long knnhip_router_num_params(const knnhip_router *r) { return r ? r->lay.total : 0; }

@@ -54,6 +54,11 @@ C_SYMBOLS = (
     "knnhip_router_train",
     "knnhip_router_scores",
     "knnhip_search_batched",
+    "knnhip_kth_distance",
+    "knnhip_index_save",
+    "knnhip_index_load",
+    "knnhip_router_save",
+    "knnhip_router_load",
     "knnhip_batched_last_failures",
     "knnhip_batched_autotune",
     "knnhip_batched_table",
@@ -163,6 +168,16 @@ def load() -> ctypes.CDLL:
     lib.knnhip_router_train.restype = i32
     lib.knnhip_router_scores.argtypes = [vp, vp, i32, i64, vp, vp]
     lib.knnhip_router_scores.restype = i32
+    lib.knnhip_kth_distance.argtypes = [vp, i32, vp, i32, i32, i32, vp]
+    lib.knnhip_kth_distance.restype = i32
+    lib.knnhip_index_save.argtypes = [vp, ctypes.c_char_p]
+    lib.knnhip_index_save.restype = i32
+    lib.knnhip_index_load.argtypes = [ctypes.c_char_p, vp, i32, i64, i32, ctypes.POINTER(vp), vp]
+    lib.knnhip_index_load.restype = i32
+    lib.knnhip_router_save.argtypes = [vp, ctypes.c_char_p]
+    lib.knnhip_router_save.restype = i32
+    lib.knnhip_router_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    lib.knnhip_router_load.restype = i32
     lib.knnhip_search_batched.argtypes = [vp, i32, ctypes.POINTER(Options), vp]
     lib.knnhip_search_batched.restype = i32
     lib.knnhip_batched_last_failures.argtypes = []
@@ -215,6 +230,21 @@ def knn_cuda_texture(ref: np.ndarray, query: np.ndarray, k: int):
 def knn_cublas(ref: np.ndarray, query: np.ndarray, k: int):
     """Drop-in for code/knncuda.h:61-68."""
     return _host_call("knnhip_cublas", ref, query, k)
+
+
+def kth_distance(ref, query, k: int):
+    """Distance to each query's k-th nearest neighbour (host numpy arrays, dim x n), as the
+    reference's legacy programs without indexes return it: row k-1 of knn_cuda_global."""
+    import numpy as np
+
+    ref = np.ascontiguousarray(ref, dtype=np.float32)
+    query = np.ascontiguousarray(query, dtype=np.float32)
+    dim, ref_nb = ref.shape
+    out = np.empty(query.shape[1], dtype=np.float32)
+    if not load().knnhip_kth_distance(ref.ctypes.data, ref_nb, query.ctypes.data, query.shape[1],
+                                      dim, k, out.ctypes.data):
+        raise KnnError(last_error())
+    return out
 
 
 def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "global",
@@ -339,6 +369,29 @@ class BlockIndex:
         load().knnhip_index_stage_ms(self._h, buf, 7)
         return dict(zip(self.STAGES, list(buf)))
 
+    def save(self, path: str):
+        if not load().knnhip_index_save(self._h, path.encode()):
+            raise KnnError(last_error())
+
+    @classmethod
+    def load(cls, path: str, ref, stream: Optional[int] = None) -> "BlockIndex":
+        """Index saved by ``save``, rebuilt over the same (device) reference set."""
+        import torch
+
+        if ref.dtype != torch.float32 or not ref.is_cuda or ref.stride(1) != 1:
+            raise KnnError("ref must be a float32 device tensor with contiguous rows")
+        h = ctypes.c_void_p()
+        st = stream if stream is not None else torch.cuda.current_stream(ref.device).cuda_stream
+        if not load().knnhip_index_load(path.encode(), ref.data_ptr(), ref.shape[1], ref.stride(0),
+                                        ref.shape[0], ctypes.byref(h), st):
+            raise KnnError(last_error())
+        self = cls.__new__(cls)
+        self.ref = ref
+        self.dim, self.ref_nb = ref.shape
+        self._h = h
+        self.nblocks = self.info()["nblocks"]
+        return self
+
     def layout(self, stream: Optional[int] = None):
         """(block_start, block_count) as host int32 arrays and the block of every reference
         point as a device int32 tensor of ref_nb entries."""
@@ -423,6 +476,27 @@ class Router:
             raise KnnError(last_error())
         self._h = h
         self.dim, self.hidden, self.nblocks = dim, hidden, nblocks
+
+    def save(self, path: str):
+        if not load().knnhip_router_save(self._h, path.encode()):
+            raise KnnError(last_error())
+
+    @classmethod
+    def load(cls, path: str) -> "Router":
+        h = ctypes.c_void_p()
+        if not load().knnhip_router_load(path.encode(), ctypes.byref(h)):
+            raise KnnError(last_error())
+        self = cls.__new__(cls)
+        self._h = h
+        n = self.num_params
+        # dims from the file header (dim, hidden, nblocks follow the magic and version)
+        with open(path, "rb") as f:
+            head = f.read(24)
+        import struct
+
+        self.dim, self.hidden, self.nblocks = struct.unpack_from("<3i", head, 12)
+        assert n > 0
+        return self
 
     @property
     def num_params(self) -> int:
