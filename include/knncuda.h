@@ -64,7 +64,9 @@ typedef struct knnhip_options {
     int timing;         /* 1: record per-stage device times (knnhip_last_stage_ms) */
     int precision;      /* 0: FP32 products; 1: bfloat16 three-term products (same exact
                            result; dim <= 192 and k + margin <= 107, else FP32) */
-    int reserved[7];
+    int block_queries;  /* FP32 scan shape: 0 chosen, 32 (reference tiles per wave), 128
+                           (reference tiles shared through LDS; dim <= 192, k + margin <= 107) */
+    int reserved[6];
 } knnhip_options;
 
 /*

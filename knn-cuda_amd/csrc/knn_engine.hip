@@ -57,7 +57,7 @@ int scan_blocks_per_cu(int ks, bool multi, int waves, int kp);
 hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s);
 int scan_bf3_nch(int dim);
 size_t scan_bf3_lds_bytes(int kp);
-hipError_t launch_scan_bf3(const ScanArgs &a, int nch, hipStream_t s);
+hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s);
 hipError_t launch_norms(const NormArgs &a, hipStream_t s);
 hipError_t launch_select(const FinalArgs &a, hipStream_t s);
 hipError_t launch_exact(const ExactArgs &a, int grid, hipStream_t s);
@@ -151,12 +151,14 @@ struct Plan {
     int r_pad = 0;            // references rounded up to whole scan steps
     int oct = 0;              // dimension octets of the packed layout
     long long slice_span = 0; // packed bytes per slice
-    bool bf3 = false;         // precision scan (bfloat16 three-term products)
+    bool lds = false;         // 128-query scan with LDS-shared reference chunks
+    bool bf3 = false;         // ... with bfloat16 three-term products
     int nch = 0;              // its 16-dimension chunks
 };
 
-// Queries per scan block: 32 for the FP32 scan, 128 for the precision scan.
-int block_queries(const Plan &p) { return p.bf3 ? 128 : kQueriesPerWave; }
+// Queries per scan block: 32 for the per-wave-tile FP32 scan, 128 for the
+// LDS-shared scan.
+int block_queries(const Plan &p) { return p.lds ? 128 : kQueriesPerWave; }
 
 // Cost model of one slice count: the scan runs in `rounds` waves of
 // resident blocks, each block's four waves sharing ceil(steps / S) reference
@@ -230,16 +232,19 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     p.xcd = (s % 8 == 0) ? 1 : 0;
     p.maxc = std::min(kMaxCand, std::max(2 * p.kp, 256));
     p.use_scan = true;
-    if (o.precision == 1) {
-        // Precision scan: 128 queries per block, one block per CU; slices
+    if (o.precision == 1 || o.block_queries == 128) {
+        // LDS-shared scan: 128 queries per block, one block per CU; slices
         // fill whole rounds of CUs (the same cost model, resident = CUs).
         const int nch = scan_bf3_nch(dim);
         if (nch && scan_bf3_lds_bytes(p.kp) <= kScanLdsMax) {
-            p.bf3 = true;
+            p.lds = true;
+            p.bf3 = o.precision == 1;
             p.nch = nch;
             p.waves = 8;
             p.dpad = 16 * nch;
-            p.oct = (dim + 7) / 8;
+            // octets written by the norms pass: the dimension's (bfloat16
+            // tiles zero-fill the rest), or every octet of the FP32 tiles
+            p.oct = p.bf3 ? (dim + 7) / 8 : 2 * nch;
             p.nqb = (Q + 127) / 128;
             p.blocks_per_cu = 1;
             const long long bstep = (long long)kRefsPerStep * p.dpad * sizeof(float);
@@ -273,10 +278,10 @@ std::string plan_json(const Plan &p, int R, int Q, int dim, int k) {
              "{\"ref_nb\": %d, \"query_nb\": %d, \"dim\": %d, \"k\": %d, \"scan\": %s, "
              "\"ks\": %d, \"chunks\": %d, \"dpad\": %d, \"waves_per_block\": %d, \"kp\": %d, "
              "\"slices\": %d, \"slice_len\": %d, \"query_blocks\": %d, \"xcd_map\": %d, "
-             "\"blocks_per_cu\": %d, \"cus\": %d, \"max_candidates\": %d, \"precision\": \"%s\"}",
+             "\"blocks_per_cu\": %d, \"cus\": %d, \"max_candidates\": %d, \"precision\": \"%s\", \"block_queries\": %d}",
              R, Q, dim, k, p.use_scan ? "true" : "false", p.ks, p.nchunks, p.dpad, p.waves, p.kp,
              p.nslices, p.slice_len, p.nqb, p.xcd, p.blocks_per_cu, p.cus, p.maxc,
-             p.bf3 ? "bf16x3" : "fp32");
+             p.bf3 ? "bf16x3" : "fp32", p.lds ? 128 : 32);
     return buf;
 }
 
@@ -403,7 +408,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         }
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[0], s), "event");
         KNN_TRY(hipMemsetAsync(rmax, 0, sizeof(unsigned), s), "memset");
-        if (p.bf3) KNN_TRY(hipMemsetAsync(rnorm, 0, (size_t)p.r_pad * sizeof(float), s), "memset");
+        if (p.lds) KNN_TRY(hipMemsetAsync(rnorm, 0, (size_t)p.r_pad * sizeof(float), s), "memset");
         KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)theta, 0xFF800000u /* +inf, order-mapped */, Q, s),
                 "memset");
         NormArgs na{};
@@ -457,8 +462,8 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         sa.xcd_map = p.xcd;
         sa.part_val = pval;
         sa.part_idx = pidx;
-        if (p.bf3)
-            KNN_TRY(launch_scan_bf3(sa, p.nch, s), "precision scan launch");
+        if (p.lds)
+            KNN_TRY(launch_scan_lds(sa, p.nch, p.bf3, s), "scan kernel launch");
         else
             KNN_TRY(launch_scan(sa, p.ks, p.multi, p.waves, s), "scan kernel launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[2], s), "event");

@@ -45,6 +45,14 @@ size_t scan_bf3_lds_bytes(int kp) {
     return bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes + kBfRing * (size_t)kBfChunkBytes;
 }
 
+// Workgroup barrier for LDS hand-offs: waits for this wave's LDS operations
+// only, so the global loads already in flight (the prefetch) keep running;
+// __syncthreads() would also wait for every outstanding global load.  The
+// memory clobber keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 __device__ inline bf16x8_t split_hi_lo(const float *x, bf16x8_t &lo) {
     bf16x8_t hi;
 #pragma unroll
@@ -57,9 +65,12 @@ __device__ inline bf16x8_t split_hi_lo(const float *x, bf16x8_t &lo) {
 }
 
 // NCH: 16-dimension chunks (padded dim = 16 NCH); PF: chunks loaded ahead
-// (divides NCH, so the register ring index is static).
-template <int NCH, int PF>
-__global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs a) {
+// (divides NCH, so the register ring index is static).  BF3: bfloat16
+// three-term products on the hi/lo tiles; else FP32 products
+// (v_mfma_f32_32x32x2_f32) on the FP32 packed tiles, whose two octets per
+// 16-dim chunk sit where the hi and lo halves do.
+template <int NCH, int PF, bool BF3>
+__global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     static_assert(NCH % PF == 0, "ring position must repeat every step");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -93,8 +104,10 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs 
     }
     if (threadIdx.x < kBfQueries) lock[threadIdx.x] = 0;
 
-    // B operands: -2q split, per chunk (lane: query col, dims 16c + 8h ..)
-    bf16x8_t bh[NCH], bl[NCH];
+    // B operands, per chunk: bf16 hi/lo of -2q (lane: query col, dims
+    // 16c + 8h ..), or FP32 -2q (dims 16c + 8o + 4h + u, octet o, k-step u)
+    bf16x8_t bh[BF3 ? NCH : 1], bl[BF3 ? NCH : 1];
+    float bq[BF3 ? 1 : NCH][8];
     {
         const float *qcol = a.query + q;
 #pragma unroll
@@ -102,11 +115,16 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs 
             float x[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int d = 16 * c + 8 * h + e;
+                const int d = BF3 ? 16 * c + 8 * h + e : 16 * c + 8 * (e >> 2) + 4 * h + (e & 3);
                 const float v = qcol[(long long)min(d, a.dim - 1) * a.ldq];
                 x[e] = d < a.dim ? -2.0f * v : 0.0f;
             }
-            bh[c] = split_hi_lo(x, bl[c]);
+            if constexpr (BF3) {
+                bh[c] = split_hi_lo(x, bl[c]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bq[c][e] = x[e];
+            }
         }
     }
 
@@ -151,8 +169,10 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs 
     const int nsteps = (r_end - r_begin + kRefsPerStep - 1) / kRefsPerStep;
     const long long valid_tiles =
         max(0, min(nsteps * kRefsPerStep, a.r_pad - r_begin)) / kRefsPerTile;
+    const unsigned char *pbase = BF3 ? reinterpret_cast<const unsigned char *>(a.packed_bf)
+                                     : reinterpret_cast<const unsigned char *>(a.packed);
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<unsigned short *>(a.packed_bf) + (long long)(r_begin / kRefsPerTile) * (tile_bytes / 2),
+        const_cast<unsigned char *>(pbase) + (long long)(r_begin / kRefsPerTile) * tile_bytes,
         (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
     const int ltile = threadIdx.x >> 7, lhl = (threadIdx.x >> 6) & 1, lln = threadIdx.x & 63;
     const unsigned my_off = (unsigned)(ltile * tile_bytes + lhl * kWave * 16 + lln * 16);
@@ -218,19 +238,33 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs 
                     pre[rslot];
                 pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 2 + PF), 0, 0);
             }
-            const bf16x8_t a0h = __builtin_bit_cast(bf16x8_t, cur[0]);
-            const bf16x8_t a0l = __builtin_bit_cast(bf16x8_t, cur[1]);
-            const bf16x8_t a1h = __builtin_bit_cast(bf16x8_t, cur[2]);
-            const bf16x8_t a1l = __builtin_bit_cast(bf16x8_t, cur[3]);
-            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bh[c], acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bh[c], acc[1], 0, 0, 0);
-            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bl[c], acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bl[c], acc[1], 0, 0, 0);
-            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, bh[c], acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, bh[c], acc[1], 0, 0, 0);
+            if constexpr (BF3) {
+                const bf16x8_t a0h = __builtin_bit_cast(bf16x8_t, cur[0]);
+                const bf16x8_t a0l = __builtin_bit_cast(bf16x8_t, cur[1]);
+                const bf16x8_t a1h = __builtin_bit_cast(bf16x8_t, cur[2]);
+                const bf16x8_t a1l = __builtin_bit_cast(bf16x8_t, cur[3]);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bh[c], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bh[c], acc[1], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bl[c], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bl[c], acc[1], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, bh[c], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, bh[c], acc[1], 0, 0, 0);
+            } else {
+                // octet o of tile mt is cur[2 mt + o]; k-step u of an octet reads
+                // dimension 8o + 4h + u on both operands
+#pragma unroll
+                for (int o = 0; o < 2; ++o)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(cur[o][u]),
+                                                                      bq[c][4 * o + u], acc[0], 0, 0, 0);
+                        acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(cur[2 + o][u]),
+                                                                      bq[c][4 * o + u], acc[1], 0, 0, 0);
+                    }
+            }
 #pragma unroll
             for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
-            __syncthreads();  // chunk g+2 stored; everyone is past chunk g's reads
+            lds_barrier();  // chunk g+2 stored; everyone is past chunk g's reads
         }
 
         // Epilogue (as the FP32 scan): keys below the query's threshold go
@@ -291,7 +325,7 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_bf3_kernel(ScanArgs 
     }
 }
 
-#define KNN_BF3_INSTANCES(X) \
+#define KNN_LDS_INSTANCES(X) \
     X(2, 2)                  \
     X(4, 4)                  \
     X(6, 3)                  \
@@ -302,25 +336,31 @@ int scan_bf3_nch(int dim) {
     const int n = (dim + 15) / 16;
     for (int c : {2, 4, 6, 8, 12})
         if (c >= n) return c;
-    return 0;  // wider dimensions use the FP32 scan (the B operands would not fit in VGPRs)
+    return 0;  // wider dimensions use the 32-query scan (the B operands would not fit in VGPRs)
 }
 
-hipError_t launch_scan_bf3(const ScanArgs &a, int nch, hipStream_t s) {
+// bf3: bfloat16 three-term products (packed_bf), else FP32 (packed).
+hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s) {
     const size_t lds = scan_bf3_lds_bytes(a.kp);
     const int blocks = a.nqb * a.nslices;
-#define X(N, P)                                                                                   \
-    if (nch == N) {                                                                               \
-        const void *fn = (const void *)scan_bf3_kernel<N, P>;                                     \
-        if (lds > 65536) {                                                                        \
-            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,     \
-                                               (int)lds);                                         \
-            if (e != hipSuccess) return e;                                                        \
-        }                                                                                         \
-        hipLaunchKernelGGL((scan_bf3_kernel<N, P>), dim3(blocks), dim3(kBfWaves * kWave), lds, s, \
-                           a);                                                                    \
-        return hipGetLastError();                                                                 \
+#define X(N, P)                                                                                 \
+    if (nch == N) {                                                                             \
+        const void *fn = bf3 ? (const void *)scan_lds_kernel<N, P, true>                         \
+                             : (const void *)scan_lds_kernel<N, P, false>;                       \
+        if (lds > 65536) {                                                                      \
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                               (int)lds);                                       \
+            if (e != hipSuccess) return e;                                                      \
+        }                                                                                       \
+        if (bf3)                                                                                \
+            hipLaunchKernelGGL((scan_lds_kernel<N, P, true>), dim3(blocks), dim3(kBfWaves * kWave), \
+                               lds, s, a);                                                      \
+        else                                                                                    \
+            hipLaunchKernelGGL((scan_lds_kernel<N, P, false>), dim3(blocks), dim3(kBfWaves * kWave), \
+                               lds, s, a);                                                      \
+        return hipGetLastError();                                                               \
     }
-    KNN_BF3_INSTANCES(X)
+    KNN_LDS_INSTANCES(X)
 #undef X
     return hipErrorInvalidValue;
 }

@@ -83,7 +83,8 @@ class Options(ctypes.Structure):
         ("force_fallback", ctypes.c_int),
         ("timing", ctypes.c_int),
         ("precision", ctypes.c_int),
-        ("reserved", ctypes.c_int * 7),
+        ("block_queries", ctypes.c_int),
+        ("reserved", ctypes.c_int * 6),
     ]
 
 
@@ -249,7 +250,7 @@ def kth_distance(ref, query, k: int):
 
 def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "global",
                   margin: int = 0, slices: int = 0, force_fallback: bool = False,
-                  precision: str = "fp32",
+                  precision: str = "fp32", block_queries: int = 0,
                   timing: bool = False, stream: Optional[int] = None):
     """Exact k-NN on device-resident torch tensors (dim x n, float32, row stride = n or more).
 
@@ -279,6 +280,7 @@ def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "
     if precision not in ("fp32", "bf16x3"):
         raise KnnError("precision must be 'fp32' or 'bf16x3'")
     opt.precision = 1 if precision == "bf16x3" else 0
+    opt.block_queries = block_queries
     if stream is None:
         stream = torch.cuda.current_stream(query.device).cuda_stream
     ok = lib.knnhip_search_device(

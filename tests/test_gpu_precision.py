@@ -40,18 +40,22 @@ SHAPES = [
 ]
 
 
+@pytest.mark.parametrize("form", ["bf16x3", "fp32-128"])
 @pytest.mark.parametrize("shape", SHAPES)
-def test_precision_equals_oracle(shape):
+def test_lds_scan_equals_oracle(shape, form):
+    # both products of the 128-query scan with LDS-shared reference chunks
     R, Q, D, k = shape
-    say(f"precision {shape}")
+    say(f"{form} {shape}")
     rng = np.random.default_rng(sum(shape))
     ref = rng.random((D, R), dtype=np.float32)
     qry = rng.random((D, Q), dtype=np.float32)
     gd, gi = oracle.knn_c(ref, qry, k)
+    prec = "bf16x3" if form == "bf16x3" else "fp32"
     d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k,
-                                precision="bf16x3")
+                                precision=prec, block_queries=128)
     torch.cuda.synchronize()
-    assert knnhip.last_plan()["precision"] == "bf16x3"
+    plan = knnhip.last_plan()
+    assert plan["precision"] == prec and plan["block_queries"] == 128
     check(d, i, gd, gi, f"{shape}")
 
 
