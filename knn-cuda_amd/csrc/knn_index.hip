@@ -859,6 +859,130 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
     flush(cur, m);
 }
 
+// The same bounds with the query operands in registers: a workgroup takes
+// 128 queries (32 per wave) and the kBallTilesPerWg consecutive ball tiles
+// [F0, F1); every wave walks those tiles (so their loads are shared through
+// L1) against its own 32 queries, whose half-rows (KH floats per lane,
+// KH % 4 == 0, KH <= KHM) stay in VGPRs.  The per-block fold runs on lanes
+// 0..31 (one query each) over the balls in order and carries the open block
+// from tile to tile, so a block is merged with an atomic min only when it
+// extends past [F0, F1); every other block is stored once.
+constexpr int kBallTilesPerWg = 8;
+
+template <int KHM>
+__global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restrict__ qpm, int pm_ld,
+                                                          int Q, int dim,
+                                                          const float *__restrict__ qnorm,
+                                                          const float *__restrict__ fcent, int nf,
+                                                          const float *__restrict__ fnorm,
+                                                          const double *__restrict__ frad,
+                                                          const int *__restrict__ block_s,
+                                                          const int *__restrict__ fb_start,
+                                                          float *__restrict__ lb) {
+    __shared__ float patch[4][32][33];
+    // per wave and tile: each ball's block, and that block's first ball and
+    // one past its last (loaded once per tile by lanes 0..31)
+    __shared__ int binfo[4][3][32];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 31, h = lane >> 5;
+    const int ntf = (nf + 31) / 32;
+    const int ngrp = (ntf + kBallTilesPerWg - 1) / kBallTilesPerWg;
+    const int qg = blockIdx.x / ngrp, grp = blockIdx.x - qg * ngrp;
+    const int q0 = qg * 128 + wave * 32;
+    if (q0 >= Q) return;  // whole wave past the last query (no workgroup barrier below)
+    const int F0 = grp * kBallTilesPerWg * 32;
+    const int F1 = min(nf, F0 + kBallTilesPerWg * 32);
+    const int KH = (dim + 1) / 2;
+    const int qa = min(q0 + col, Q - 1);
+    // this lane's query half-row: dims h*KH .. h*KH + KH - 1
+    float4 bq[KHM / 4];
+    {
+        const float *qr = qpm + (long long)qa * pm_ld + h * KH;
+#pragma unroll
+        for (int u = 0; u < KHM / 4; ++u)
+            bq[u] = 4 * u < KH ? *reinterpret_cast<const float4 *>(qr + 4 * u) : make_float4(0, 0, 0, 0);
+    }
+    const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
+    const double nq = (double)qnorm[qa];
+    const int q = q0 + col;
+    const bool folder = h == 0 && q < Q;
+    int cur = -1, cur_s = 0, cur_e = 0;
+    float m = INFINITY;
+    auto flush = [&]() {
+        if (cur < 0) return;
+        float *dst = lb + (long long)cur * Q + q;
+        if (cur_s >= F0 && cur_e <= F1)
+            *dst = m;
+        else
+            atomicMin(reinterpret_cast<unsigned *>(dst), __float_as_uint(m));
+    };
+    for (int tt = 0; tt < kBallTilesPerWg; ++tt) {
+        const int f0 = F0 + tt * 32;
+        if (f0 >= F1) break;  // uniform over the wave
+        const int fa = min(f0 + col, nf - 1);
+        const float *ca = fcent + (long long)fa * pm_ld + h * KH;
+        f32x16_t acc = {};
+#pragma unroll
+        for (int u = 0; u < KHM / 4; ++u) {
+            if (4 * u < KH) {
+                const float4 av = *reinterpret_cast<const float4 *>(ca + 4 * u);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[u].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[u].y, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[u].z, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[u].w, acc, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = ((i >> 2) << 3) + (h << 2) + (i & 3);
+            const int f = f0 + r;
+            float out = INFINITY;
+            const double rad = f < nf ? frad[f] : -1.0;
+            if (rad >= 0.0) {
+                const double nc = (double)fnorm[f];
+                const double dot = (double)acc[i];
+                double d2 = nq + nc - 2.0 * dot - g * (nq + nc + 2.0 * sqrt(nq * nc)) * 1.5;
+                d2 = d2 > 0.0 ? d2 : 0.0;
+                const double v = sqrt(d2) * (1.0 - 1e-12) - rad;
+                float fv = (float)v;
+                if ((double)fv > v) fv = nextafterf(fv, -INFINITY);
+                out = fv > 0.0f ? fv : 0.0f;
+            }
+            patch[wave][r][col] = out;
+        }
+        if (h == 0) {
+            const int f = f0 + col;
+            const int b = f < nf ? block_s[f] : -1;
+            binfo[wave][0][col] = b;
+            binfo[wave][1][col] = b >= 0 ? fb_start[b] : 0;
+            binfo[wave][2][col] = b >= 0 ? fb_start[b + 1] : 0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (folder) {
+            const int n = min(32, F1 - f0);
+            for (int r = 0; r < n; ++r) {
+                const int b = binfo[wave][0][r];
+                const float v = patch[wave][r][col];
+                if (b != cur) {
+                    flush();
+                    cur = b;
+                    cur_s = binfo[wave][1][r];
+                    cur_e = binfo[wave][2][r];
+                    m = v;
+                } else {
+                    m = fminf(m, v);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (folder) flush();
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -1001,6 +1125,18 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
                        const int *block_s, const int *fb_start, int nb, float *lb, hipStream_t s) {
     hipError_t e = hipMemsetD32Async((hipDeviceptr_t)lb, 0x7F800000, (size_t)nb * Q, s);
     if (e != hipSuccess) return e;
+    const int KH = (dim + 1) / 2;
+    if (KH % 4 == 0 && pm_ld % 4 == 0 && KH <= 64) {
+        const long long wgs = (long long)((Q + 127) / 128) *
+                              (((nf + 31) / 32 + kBallTilesPerWg - 1) / kBallTilesPerWg);
+        if (KH <= 32)
+            hipLaunchKernelGGL(ball_lb_reg_kernel<32>, dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
+                               Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb);
+        else
+            hipLaunchKernelGGL(ball_lb_reg_kernel<64>, dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
+                               Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb);
+        return hipGetLastError();
+    }
     const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);
     hipLaunchKernelGGL(ball_lb_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, qpm,
                        pm_ld, Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb);
