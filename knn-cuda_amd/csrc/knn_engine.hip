@@ -72,6 +72,7 @@ namespace {
 // kp-th approximate key to clear the k-th exact distance by the FP32 error
 // bound.  Large k goes with dense neighbourhoods, so it starts wider.
 constexpr int kDefaultMargin = 16;
+constexpr int kIndexMargin = 8;  // block-index searches: fewer heap insertions per block
 constexpr int kDefaultMarginLargeK = 64;
 constexpr int kLargeK = 64;
 // Certificate failures are retried through the scan with wider margins
@@ -1015,7 +1016,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                   long long ldq, int k, int P, const float *router_scores, float *out_dist,
                   int *out_idx, const knnhip_options &o, hipStream_t s) {
     const int nb = ix->nb, dim = ix->dim, pm_ld = ix->pm_ld;
-    const int margin = o.margin > 0 ? o.margin : default_margin(k);
+    const int margin = o.margin > 0 ? o.margin : (k >= kLargeK ? kDefaultMarginLargeK : kIndexMargin);
     const int kp = std::max(k, std::min(k + margin, ix->R));
     if (scan_lds_bytes(kp, 4) > kScanLdsMax || (long long)P * kp > kMaxUnion) {
         set_error("k or probe count too large for the block index (k=%d probes=%d)", k, P);
@@ -1173,7 +1174,10 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         knnhip_options po = o;
         po.force_fallback = 0;
         po.timing = 0;
-        const int P2 = std::min({nb, kMaxUnion / kp, std::max(4 * P, kRetryProbes)});
+        // the retry widens both the probe count and the candidate margin
+        po.margin = std::max(2 * margin, kDefaultMargin);
+        const int kp2 = std::max(k, std::min(k + po.margin, ix->R));
+        const int P2 = std::min({nb, kMaxUnion / kp2, std::max(4 * P, kRetryProbes)});
         if (depth == 0 && P2 > P) {
             if (!index_search(ix, g_iw_retry, 1, qsub.as<float>(), nfail, nfail, k, P2, nullptr,
                               dsub.as<float>(), isub.as<int>(), po, s))
