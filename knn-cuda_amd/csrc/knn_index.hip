@@ -230,6 +230,116 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_kernel(
     }
 }
 
+// Single-pass form for P <= PMAX: every thread keeps the best P blocks of its
+// share (sorted, in registers) and the smallest bound over the rest of its
+// share; P merge rounds then take the best head among the 16 waves' lists.
+// lb_rest = min(bounds a thread did not keep, kept entries not chosen).
+// Same result as pick_blocks_kernel, reading the bounds once.
+template <int PMAX>
+__global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
+    const float *__restrict__ score, int Q, int nb, int P, const float *__restrict__ lb,
+    const int *__restrict__ counts, int *__restrict__ chosen, float *__restrict__ lb_rest) {
+    __shared__ float offer_s[kPickWaves][64];
+    __shared__ int offer_b[kPickWaves][64];
+    __shared__ int winner[64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = blockIdx.x * 64 + lane;
+    const bool valid = q < Q;
+    const int per = (nb + kPickWaves - 1) / kPickWaves;
+    const int b0 = min(nb, wave * per), b1 = min(nb, b0 + per);
+    float ls[PMAX], ll[PMAX];
+    int lbk[PMAX];
+    int n = 0;
+    float rest = INFINITY;
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        ls[j] = -INFINITY;
+        ll[j] = INFINITY;
+        lbk[j] = 0x7FFFFFFF;
+    }
+    if (valid)
+        for (int b = b0; b < b1; ++b) {
+            if (counts[b] == 0) continue;
+            const long long e = (long long)b * Q + q;
+            const float s = rank_score(score, lb, e);
+            const float l = lb[e];
+            // blocks come in increasing order, so an equal score ranks after
+            // the entries already kept
+            if (n < P || s > ls[P - 1]) {
+                const int old_n = n;
+                if (n == P)
+                    rest = fminf(rest, ll[P - 1]);  // the entry that drops out
+                else
+                    ++n;
+                // bubble insert: from the new entry's place on, every entry
+                // moves one slot down
+                float cs = s, cl = l;
+                int cb = b;
+                bool moving = false;
+#pragma unroll
+                for (int j = 0; j < PMAX; ++j) {
+                    if (j < n && (moving || j >= old_n || cs > ls[j])) {
+                        const float ts = ls[j], tl = ll[j];
+                        const int tb = lbk[j];
+                        ls[j] = cs;
+                        ll[j] = cl;
+                        lbk[j] = cb;
+                        cs = ts;
+                        cl = tl;
+                        cb = tb;
+                        moving = true;
+                    }
+                }
+            } else {
+                rest = fminf(rest, l);
+            }
+        }
+    // merge rounds
+    int head = 0;
+    for (int p = 0; p < P; ++p) {
+        float hs = -INFINITY;
+        int hb = -1;
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j)
+            if (j == head && j < n) {
+                hs = ls[j];
+                hb = lbk[j];
+            }
+        offer_s[wave][lane] = hs;
+        offer_b[wave][lane] = hb;
+        __syncthreads();
+        if (wave == 0) {
+            float ks = -INFINITY;
+            int kb = -1, kw = -1;
+            for (int w = 0; w < kPickWaves; ++w) {
+                const int ob = offer_b[w][lane];
+                const float os = offer_s[w][lane];
+                if (ob >= 0 && (kb < 0 || os > ks || (os == ks && ob < kb))) {
+                    ks = os;
+                    kb = ob;
+                    kw = w;
+                }
+            }
+            if (valid) chosen[(long long)q * P + p] = kb;
+            winner[lane] = kw;
+        }
+        __syncthreads();
+        if (winner[lane] == wave) ++head;
+        __syncthreads();
+    }
+    // entries of this thread's list that were not chosen
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j)
+        if (j >= head && j < n) rest = fminf(rest, ll[j]);
+    offer_s[wave][lane] = rest;
+    __syncthreads();
+    if (wave == 0 && valid) {
+        float m = offer_s[0][lane];
+        for (int w = 1; w < kPickWaves; ++w) m = fminf(m, offer_s[w][lane]);
+        lb_rest[q] = m;
+    }
+}
+
 // The same selection for few queries: one workgroup per query, its threads
 // spread over the blocks, so each round is a workgroup argmax.
 __global__ __launch_bounds__(256) void pick_blocks_single_kernel(
@@ -1026,6 +1136,12 @@ hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float
     if (Q <= kPickSingleMaxQ)
         hipLaunchKernelGGL(pick_blocks_single_kernel, dim3(Q), dim3(256), 0, s, score, Q, nb, P, lb,
                            counts, chosen, lb_rest);
+    else if (P <= 8)
+        hipLaunchKernelGGL(pick_blocks_list_kernel<8>, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
+                           score, Q, nb, P, lb, counts, chosen, lb_rest);
+    else if (P <= 16)
+        hipLaunchKernelGGL(pick_blocks_list_kernel<16>, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0,
+                           s, score, Q, nb, P, lb, counts, chosen, lb_rest);
     else
         hipLaunchKernelGGL(pick_blocks_kernel, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
                            score, Q, nb, P, lb, counts, chosen, lb_rest);
