@@ -53,6 +53,12 @@ struct ScanArgs {
     int ntasks;           // one block per task
     const int *qlist;     // query ids referenced by the tasks
     const unsigned short *packed_bf;  // bfloat16 hi/lo tiles (precision scan), else unused
+    // Point-major query rows (row stride query_pm_ld, a multiple of 4, pads
+    // zero), or nullptr.  Task-mode searches pass them: a lane then reads its
+    // query's operands as 16-byte pieces of one row instead of scattered
+    // columns of the dim-major matrix.
+    const float *query_pm;
+    int query_pm_ld;
 };
 
 // Arguments of the norms / layout pass.

@@ -1122,6 +1122,8 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         sa.tasks = w.tasks.as<ScanTask>() + tbase;
         sa.ntasks = max_t[ph];
         sa.qlist = w.qlist.as<int>() + base;
+        sa.query_pm = w.qpm.as<float>();
+        sa.query_pm_ld = pm_ld;
         KNN_TRY(launch_scan(sa, ix->ks, ix->multi, 4, s), "scan kernel launch");
         tbase += max_t[ph];
     }
@@ -1657,6 +1659,8 @@ bool batch_class_run(const BatchClass &c, int L, const knnhip_options &o, hipStr
     sa.tasks = w.tasks.as<ScanTask>();
     sa.ntasks = ntasks;
     sa.qlist = w.qlist.as<int>();
+    sa.query_pm = w.qpm.as<float>();
+    sa.query_pm_ld = pm_ld;
     KNN_TRY(launch_scan(sa, pl.ks, pl.multi, 4, s), "scan kernel launch");
     if (scan_ms) KNN_TRY(hipEventRecord(e1, s), "event");
     FinalArgs fa{};

@@ -164,6 +164,23 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     const float *__restrict__ qcol = a.query + q;
     float bq[KS];
     auto load_b = [&](int c) {
+        if (!MULTI && a.query_pm) {  // (the chunked instance has no VGPRs to spare)
+            // k-steps 4j .. 4j+3 read dims 8j + 4h .. +3 of the query: one
+            // 16-byte piece of its row (whole pieces: the row stride is a
+            // multiple of 4 and the pads are zero)
+            const float *row = a.query_pm + (long long)q * a.query_pm_ld;
+#pragma unroll
+            for (int j = 0; j < KS / 4; ++j) {
+                const int d4 = c * 2 * KS + 8 * j + 4 * h;
+                const float4 v = d4 < a.query_pm_ld ? *reinterpret_cast<const float4 *>(row + d4)
+                                                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                bq[4 * j + 0] = -2.0f * v.x;
+                bq[4 * j + 1] = -2.0f * v.y;
+                bq[4 * j + 2] = -2.0f * v.z;
+                bq[4 * j + 3] = -2.0f * v.w;
+            }
+            return;
+        }
 #pragma unroll
         for (int kk = 0; kk < KS; ++kk) {
             // Branch-free: clamp the row, then zero the padding dimensions.
