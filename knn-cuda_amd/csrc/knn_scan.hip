@@ -445,58 +445,104 @@ __device__ inline void write_bf3(unsigned short *base, int tile, int nch, int j,
 // ---------------------------------------------------------------------------
 // Norms pass: squared norms, the maximum reference norm, the point-major copy
 // used by the exact re-rank, and (references only) the packed tile layout
-// read by the scan.  One thread per point; threads past n zero the packed
-// tail tiles.  Replaces code/knncuda.cu:217-227.
+// read by the scan.  Points in [n, n_pad) get zero tiles.  Each point's norm
+// is an FP32 fused multiply-add sum in dimension order.  Replaces
+// code/knncuda.cu:217-227.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void norms_pack_kernel(NormArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in_pad = i < a.n_pad;  // no early return: the wave reduces below
-    const bool real = i < a.n;
-    float s = 0.0f;
-    const float *col = a.p + (real ? i : 0);
-    float *row = (a.pm && real) ? a.pm + (long long)i * a.pm_ld : nullptr;
-    const int tile = i / kRefsPerTile, m = i % kRefsPerTile;
-    for (int j = 0; in_pad && j < a.oct; ++j) {
-        float v[8];
+    // 64 points per workgroup; 32 dimensions (4 octets) at a time go through
+    // LDS so that the column reads, the packed-tile writes and the
+    // point-major row writes are all contiguous.
+    constexpr int kPts = 64, kDims = 32;
+    __shared__ float tile[kDims][kPts + 1];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * kPts;
+    float s = 0.0f;  // thread t < 64: squared norm of point i0 + t, summed in dimension order
+    const int ngrp = (a.oct + 3) / 4;
+    for (int g = 0; g < ngrp; ++g) {
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int d = 8 * j + t;
-            const float x = col[(long long)min(d, a.dim - 1) * a.ld];
-            v[t] = (real && d < a.dim) ? x : 0.0f;
+        for (int e = 0; e < (kDims * kPts) / 256; ++e) {
+            const int idx = t + 256 * e;
+            const int dl = idx / kPts, p = idx % kPts;
+            const int d = kDims * g + dl, i = i0 + p;
+            tile[dl][p] = (i < a.n && d < a.dim) ? a.p[(long long)d * a.ld + i] : 0.0f;
         }
+        __syncthreads();
+        if (t < kPts) {
 #pragma unroll
-        for (int t = 0; t < 8; ++t) s = fmaf(v[t], v[t], s);
-        if (row) {
-            // pm_ld is a multiple of 4: whole 16-byte pieces
-            if (8 * j + 4 <= a.pm_ld)
-                *reinterpret_cast<float4 *>(row + 8 * j) = make_float4(v[0], v[1], v[2], v[3]);
-            if (8 * j + 8 <= a.pm_ld)
-                *reinterpret_cast<float4 *>(row + 8 * j + 4) = make_float4(v[4], v[5], v[6], v[7]);
+            for (int dl = 0; dl < kDims; ++dl) {
+                const float v = tile[dl][t];
+                if (8 * (4 * g) + dl < 8 * a.oct) s = fmaf(v, v, s);
+            }
         }
+        // packed FP32 tiles: (octet j, tile, lane m + 32h) <- dims 8j + 4h .. +3 of point m
         if (a.packed) {
-            float *dst = a.packed + (((long long)tile * a.oct + j) * kWave + m) * 4;
-            *reinterpret_cast<float4 *>(dst) = make_float4(v[0], v[1], v[2], v[3]);
-            *reinterpret_cast<float4 *>(dst + 32 * 4) = make_float4(v[4], v[5], v[6], v[7]);
-        }
-        if (a.packed_bf) write_bf3(a.packed_bf, tile, a.nch_bf, j, m, v);
-    }
-    if (a.packed_bf && in_pad) {
-        // octets past the dimension, up to the kernel's chunk count, are zero
-        const float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int j = a.oct; j < 2 * a.nch_bf; ++j) write_bf3(a.packed_bf, tile, a.nch_bf, j, m, z);
-    }
-    if (real) a.out[i] = s;
-    if (a.maxbits) {
-        unsigned mbits = real ? __float_as_uint(s) : 0u;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1)
-            mbits = max(mbits, (unsigned)__shfl_xor((int)mbits, off));
-        if ((threadIdx.x & 63) == 0) atomicMax(a.maxbits, mbits);
+            for (int r = 0; r < 2; ++r) {
+                const int idx = t + 256 * r;
+                const int jj = idx / 128, tl = (idx / 64) % 2, lane = idx % 64;
+                const int m = lane % 32, h = lane / 32;
+                const int j = 4 * g + jj, i = i0 + 32 * tl;
+                if (j < a.oct && i < a.n_pad) {
+                    const float4 v = make_float4(tile[8 * jj + 4 * h + 0][32 * tl + m],
+                                                 tile[8 * jj + 4 * h + 1][32 * tl + m],
+                                                 tile[8 * jj + 4 * h + 2][32 * tl + m],
+                                                 tile[8 * jj + 4 * h + 3][32 * tl + m]);
+                    *reinterpret_cast<float4 *>(
+                        a.packed + (((long long)(i / kRefsPerTile) * a.oct + j) * kWave + lane) * 4) = v;
+                }
+            }
+        }
+        // bfloat16 hi/lo tiles: octet j of point m
+        if (a.packed_bf) {
+            const int jj = t / 64, tl = (t / 32) % 2, m = t % 32;
+            const int j = 4 * g + jj, i = i0 + 32 * tl;
+            if (j < a.oct && i < a.n_pad) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = tile[8 * jj + u][32 * tl + m];
+                write_bf3(a.packed_bf, i / kRefsPerTile, a.nch_bf, j, m, v);
+            }
+        }
+        // point-major rows: 8 threads write a point's 32 dimensions (128 bytes)
+        if (a.pm) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int idx = t + 256 * r;
+                const int p = idx / 8, q4 = idx % 8;
+                const int d = kDims * g + 4 * q4, i = i0 + p;
+                if (i < a.n && d < a.pm_ld && 4 * q4 < 8 * (a.oct - 4 * g))
+                    *reinterpret_cast<float4 *>(a.pm + (long long)i * a.pm_ld + d) =
+                        make_float4(tile[4 * q4 + 0][p], tile[4 * q4 + 1][p], tile[4 * q4 + 2][p],
+                                    tile[4 * q4 + 3][p]);
+            }
+        }
+        __syncthreads();
+    }
+    // octets past the dimension, up to the precision scan's chunk count, are zero
+    if (a.packed_bf && t < kPts) {
+        const int i = i0 + t;
+        if (i < a.n_pad) {
+            const float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int j = a.oct; j < 2 * a.nch_bf; ++j)
+                write_bf3(a.packed_bf, i / kRefsPerTile, a.nch_bf, j, i % kRefsPerTile, z);
+        }
+    }
+    if (t < kPts) {
+        const int i = i0 + t;
+        if (i < a.n) a.out[i] = s;
+        if (a.maxbits) {
+            unsigned mbits = i < a.n ? __float_as_uint(s) : 0u;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1)
+                mbits = max(mbits, (unsigned)__shfl_xor((int)mbits, off));
+            if (t == 0) atomicMax(a.maxbits, mbits);
+        }
     }
 }
 
 hipError_t launch_norms(const NormArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(norms_pack_kernel, dim3((a.n_pad + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(norms_pack_kernel, dim3((a.n_pad + 63) / 64), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
