@@ -15,6 +15,10 @@
 
 #include <stddef.h>
 
+/* The library is built with hidden visibility: what this header declares is
+ * all it exports. */
+#pragma GCC visibility push(default)
+
 #ifdef __cplusplus
 /*
  * The three reference entry points, with C++ linkage so that code compiled
@@ -230,5 +234,7 @@ const char *knnhip_version(void);
 #ifdef __cplusplus
 }
 #endif
+
+#pragma GCC visibility pop
 
 #endif /* KNNHIP_KNNCUDA_H */

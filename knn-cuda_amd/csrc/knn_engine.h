@@ -1,0 +1,197 @@
+// Internal interface of the host engine, shared by its modules:
+//   knn_engine.hip      plan, workspace, the plain search and the drop-in API
+//   knn_index_host.hip  block index build / search / save / load
+//   knn_router_host.hip learned router training and scoring
+//   knn_batch.hip       batched search and its slice-length autotuner
+// Every C entry point takes g_mu for its whole call, so the state below is
+// touched by one host thread at a time.
+#pragma once
+
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/knncuda.h"
+#include "knn_common.h"
+
+namespace knnhip {
+
+// Owned device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    bool alloc(size_t bytes) {
+        release();
+        if (bytes == 0) bytes = 256;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return false;
+        }
+        n = bytes;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// Launchers defined in knn_scan.hip / knn_scan_bf16.hip / knn_select.hip.
+int scan_pick_ks(int dim);
+size_t scan_lds_bytes(int kp, int waves);
+int scan_blocks_per_cu(int ks, bool multi, int waves, int kp);
+hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s);
+int scan_bf3_nch(int dim);
+size_t scan_bf3_lds_bytes(int kp);
+hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s);
+hipError_t launch_norms(const NormArgs &a, hipStream_t s);
+hipError_t launch_select(const FinalArgs &a, hipStream_t s);
+hipError_t launch_exact(const ExactArgs &a, int grid, hipStream_t s);
+hipError_t launch_gen_uniform(float *out, int n, int dim, uint64_t seed, float scale,
+                              hipStream_t s);
+hipError_t launch_gen_mixture(float *out, int n, int dim, int ncomp, float sigma,
+                              uint64_t centre_seed, uint64_t point_seed, hipStream_t s);
+
+// Candidates kept beyond k (kp = k + margin): the certificate needs the
+// kp-th approximate key to clear the k-th exact distance by the FP32 error
+// bound.  Large k goes with dense neighbourhoods, so it starts wider.
+constexpr int kDefaultMargin = 16;
+constexpr int kDefaultMarginLargeK = 64;
+constexpr int kLargeK = 64;
+// Certificate failures are retried through the scan with wider margins
+// (kEscalationMargins[level - 1]) before the exact per-query scan.
+constexpr int kEscalationMargins[] = {64, 256};
+constexpr int kMaxEscalation = 2;
+inline int default_margin(int k) { return k >= kLargeK ? kDefaultMarginLargeK : kDefaultMargin; }
+constexpr int kExactGrid = 512;
+constexpr size_t kScanLdsMax = 160 * 1024;
+
+extern std::mutex g_mu;
+extern std::string g_last_error;
+extern float g_stage_ms[4];
+extern hipStream_t g_last_stream;
+extern const int *g_last_fb;  // device counter of the last search's exact-scan queries
+extern std::string g_last_plan;
+extern int g_cu_count;
+
+void set_error(const char *fmt, ...);
+
+// Grow-only device buffers, one per role.
+enum Slot {
+    kRefPad, kQryPad, kRNorm, kQNorm, kRMax, kPartVal, kPartIdx, kFbCount, kFbList,
+    kFbKey, kFbIdx, kOutDist, kOutIdx, kRefPm, kQryPm, kPacked, kTheta, kPackedBf, kNumSlots
+};
+struct Workspace {
+    void *ptr[kNumSlots] = {};
+    size_t cap[kNumSlots] = {};
+    hipEvent_t ev[5] = {};
+    bool have_events = false;
+};
+extern Workspace g_ws;
+void *ws_get(Slot s, size_t bytes);
+
+// Launch plan of one call.
+struct Plan {
+    bool use_scan = false;  // false: exact full scan for every query
+    int ks = 0;
+    bool multi = false;
+    int nchunks = 1;
+    int dpad = 0;
+    int waves = 4;
+    int kp = 0;
+    int nslices = 1;
+    int slice_len = 0;
+    int nqb = 0;
+    int xcd = 0;
+    int maxc = 0;
+    int blocks_per_cu = 0;
+    int cus = 0;
+    int r_pad = 0;            // references rounded up to whole scan steps
+    int oct = 0;              // dimension octets of the packed layout
+    long long slice_span = 0; // packed bytes per slice
+    bool lds = false;         // 128-query scan with LDS-shared reference chunks
+    bool bf3 = false;         // ... with bfloat16 three-term products
+    int nch = 0;              // its 16-dimension chunks
+};
+
+// Queries per scan block: 32 for the per-wave-tile FP32 scan, 128 for the
+// LDS-shared scan.
+inline int block_queries(const Plan &p) { return p.lds ? 128 : kQueriesPerWave; }
+
+Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o);
+std::string plan_json(const Plan &p, int R, int Q, int dim, int k);
+bool check_args(const void *ref, int R, const void *query, int Q, int dim, int k, const void *dist,
+                const void *idx);
+bool ensure_device();
+knnhip_options default_options(int variant);
+// Prints the last error on stdout as the reference's entry points do.
+bool report(bool ok);
+
+// Search on device pointers: dim rows of ldr / ldq floats; results k x Q.
+// level > 0: an escalation pass over queries that failed the certificate.
+bool run_device(const float *ref, long long ldr, int R, const float *query, long long ldq, int Q,
+                int dim, int k, float *out_dist, int *out_idx, const Plan &p,
+                const knnhip_options &o, hipStream_t s, int level = 0);
+
+// Whole-file readers and writers of the index and router file formats:
+// `ok` turns false at the first short read or write and stays false.
+struct FileOut {
+    FILE *f;
+    bool ok = true;
+    void put(const void *p, size_t n) { ok = ok && fwrite(p, 1, n, f) == n; }
+    void put_dev(const DevBuf &b, size_t n) {
+        std::vector<char> h(n);
+        ok = ok && hipMemcpy(h.data(), b.p, n, hipMemcpyDeviceToHost) == hipSuccess;
+        put(h.data(), n);
+    }
+};
+struct FileIn {
+    FILE *f;
+    bool ok = true;
+    void get(void *p, size_t n) { ok = ok && fread(p, 1, n, f) == n; }
+    void get_dev(DevBuf &b, size_t n) {
+        std::vector<char> h(n);
+        get(h.data(), n);
+        ok = ok && b.alloc(n) && hipMemcpy(b.p, h.data(), n, hipMemcpyHostToDevice) == hipSuccess;
+    }
+};
+
+#define KNN_TRY(expr, what)                                                        \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            knnhip::set_error("%s: %s", what, hipGetErrorString(_e));              \
+            return false;                                                          \
+        }                                                                          \
+    } while (0)
+
+}  // namespace knnhip
+
+// The opaque index handle of the C API.
+struct knnhip_index {
+    int R = 0, dim = 0, nb = 0, pm_ld = 0, oct = 0, dpad = 0, ks = 0, nchunks = 1;
+    bool multi = false;
+    int n_pos = 0, r_pad = 0;
+    const float *ref = nullptr;  // caller's dim-major references (exact fallback)
+    long long ldr = 0;
+    float build_ms = 0.f;
+    knnhip::DevBuf packed, pm, norms, perm, block_start, block_count, rmax;
+    // fine balls in block-sorted order: centre, squared norm of the centre,
+    // radius (< 0: empty ball), block; block b's balls are fb_start[b] ..
+    // fb_start[b+1]
+    int nf = 0;
+    knnhip::DevBuf fcent, fnorm, frad, fblock, fb_start;
+    // last search statistics
+    int last_probes = 0;
+    int last_fail = 0;
+    long long last_tasks = 0;
+    float last_ms[6] = {-1, -1, -1, -1, -1, -1};
+    float rank_ms[3] = {-1, -1, -1};  // query norms, block bounds, block choice
+};

@@ -1,0 +1,693 @@
+// Host side of the block index: build (k-means fine balls grouped into
+// balanced blocks), search (ball lower bounds, block choice, task-mode scan,
+// certificate with retry), and the index file format.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "knn_engine.h"
+#include "knn_index.h"
+
+namespace knnhip {
+namespace {
+
+constexpr int kIndexMargin = 8;  // block-index searches: fewer heap insertions per block
+
+// Lloyd iterations on the point-major rows `pm` (n points) from the `k`
+// centroids already in `cent` (point-major, ld pm_ld).  The assignment step
+// is the engine itself (centroids as the reference set, k = 1); the final
+// assignment is left in `assign`, its histogram in `counts`.
+bool kmeans(const float *pm, const float *dm, long long ldd, int n, int dim, int pm_ld, int k,
+            int iters, float *cent, int *assign, int *counts, hipStream_t s) {
+    DevBuf cent_dm, adist, start, cursor, order;
+    if (!cent_dm.alloc((size_t)k * dim * 4) || !adist.alloc((size_t)n * 4) ||
+        !start.alloc((size_t)(k + 1) * 4) || !cursor.alloc((size_t)(k + 1) * 4) ||
+        !order.alloc((size_t)n * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    knnhip_options ao;
+    memset(&ao, 0, sizeof(ao));
+    ao.margin = 8;
+    const Plan ap = make_plan(k, n, dim, 1, ao);
+    for (int it = 0; it <= iters; ++it) {
+        KNN_TRY(idx_transpose(cent, pm_ld, k, dim, cent_dm.as<float>(), s), "transpose");
+        if (!run_device(cent_dm.as<float>(), k, k, dm, ldd, n, dim, 1, adist.as<float>(), assign,
+                        ap, ao, s))
+            return false;
+        KNN_TRY(hipMemsetAsync(counts, 0, (size_t)k * 4, s), "memset");
+        KNN_TRY(idx_histogram(assign, n, counts, s), "histogram");
+        if (it == iters) break;
+        KNN_TRY(idx_prefix(counts, k, 1, start.as<int>(), nullptr, s), "prefix");
+        KNN_TRY(hipMemcpyAsync(cursor.p, start.p, (size_t)k * 4, hipMemcpyDeviceToDevice, s), "copy");
+        KNN_TRY(idx_scatter(assign, n, cursor.as<int>(), order.as<int>(), s), "scatter");
+        KNN_TRY(idx_centroids(pm, pm_ld, dim, order.as<int>(), start.as<int>(), counts, k, cent,
+                              pm_ld, s),
+                "centroids");
+    }
+    return true;
+}
+
+// Block index build.
+//  1. Fine balls: k-means over the references with kBallsPerBlock * nb
+//     centres (k-means++ seeding on an evenly strided sample of kSeedPerCentre
+//     points per centre, then `iters` Lloyd steps with the engine as the
+//     assignment step).  Each ball's radius bounds its members.
+//  2. Blocks: nb seeds drawn by k-means++ over the ball centres; each ball
+//     joins the nearest of its kGroupCandidates nearest seeds that stays
+//     within kBlockSlack times the mean block size.
+//  3. Layout: references block by block (each block starting at a multiple
+//     of 32), each ball contiguous inside its block; balls sorted by block.
+// A block's lower bound for a query is the smallest over its balls.
+constexpr int kSeedPerCentre = 16;
+constexpr int kGroupCandidates = 32;
+constexpr double kBlockSlack = 1.25;
+constexpr unsigned long long kBallSeed = 0x6B4E4E5EEDBA11Full;
+constexpr unsigned long long kBlockSeed = 0x6B4E4E5EEDB10C5ull;
+
+bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int dim, int nb,
+                 int iters, hipStream_t s) {
+    hipEvent_t e0, e1;
+    KNN_TRY(hipEventCreate(&e0), "event");
+    KNN_TRY(hipEventCreate(&e1), "event");
+    KNN_TRY(hipEventRecord(e0, s), "event");
+    ix->R = R;
+    ix->dim = dim;
+    ix->nb = nb;
+    ix->ref = ref;
+    ix->ldr = ldr;
+    ix->pm_ld = (dim + 3) & ~3;
+    if (ix->pm_ld > kIndexMaxLd) {
+        set_error("dimension too large for the block index (dim=%d, at most %d)", dim, kIndexMaxLd);
+        return false;
+    }
+    knnhip_options o;
+    memset(&o, 0, sizeof(o));
+    {
+        Plan p = make_plan(R, 32, dim, 1, o);
+        if (!p.use_scan) {
+            set_error("dimension too large for the block index");
+            return false;
+        }
+        ix->ks = p.ks;
+        ix->multi = p.multi;
+        ix->nchunks = p.nchunks;
+        ix->dpad = p.dpad;
+        ix->oct = p.oct;
+    }
+    const int pm_ld = ix->pm_ld;
+    const int nf = (int)std::min<long long>(R, (long long)kBallsPerBlock * nb);
+    ix->nf = nf;
+    const int m = (int)std::min<long long>(R, (long long)kSeedPerCentre * nf);
+    const int kc = std::min(kGroupCandidates, nb);
+    const int cap = (int)std::ceil(kBlockSlack * (double)R / (double)nb);
+    const int mmax = std::max(m, nf);
+    DevBuf pm0, norms0, fassign, fcount, fcent, bcent, fcent_dm, bcent_dm, cand_d, cand_i, fblock,
+        bfine, cursor, fstart, fsorted, mind, cidx, wsum, ticket;
+    if (!pm0.alloc((size_t)R * pm_ld * 4) || !norms0.alloc((size_t)R * 4) || !ix->rmax.alloc(4) ||
+        !fassign.alloc((size_t)R * 4) || !fcount.alloc((size_t)nf * 4) ||
+        !fcent.alloc((size_t)nf * pm_ld * 4) || !bcent.alloc((size_t)nb * pm_ld * 4) ||
+        !fcent_dm.alloc((size_t)nf * dim * 4) || !bcent_dm.alloc((size_t)nb * dim * 4) ||
+        !cand_d.alloc((size_t)kc * nf * 4) || !cand_i.alloc((size_t)kc * nf * 4) ||
+        !fblock.alloc((size_t)nf * 4) || !bfine.alloc((size_t)nb * 4) ||
+        !cursor.alloc((size_t)(nb + 1) * 4) || !fstart.alloc((size_t)nf * 4) ||
+        !fsorted.alloc((size_t)nf * 4) || !mind.alloc((size_t)mmax * 4) ||
+        !cidx.alloc((size_t)std::max(nf, nb) * 4) || !wsum.alloc((size_t)kKppGroups * 8) ||
+        !ticket.alloc(4) || !ix->block_start.alloc((size_t)(nb + 1) * 4) ||
+        !ix->block_count.alloc((size_t)nb * 4) || !ix->fcent.alloc((size_t)nf * pm_ld * 4) ||
+        !ix->frad.alloc((size_t)nf * 8) || !ix->fnorm.alloc((size_t)nf * 4) ||
+        !ix->fblock.alloc((size_t)nf * 4) || !ix->fb_start.alloc((size_t)(nb + 1) * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    // point-major copy in the caller's order, norms, max norm
+    KNN_TRY(hipMemsetAsync(ix->rmax.p, 0, 4, s), "memset");
+    NormArgs na{};
+    na.p = ref;
+    na.ld = ldr;
+    na.n = R;
+    na.n_pad = R;
+    na.dim = dim;
+    na.oct = (dim + 7) / 8;
+    na.out = norms0.as<float>();
+    na.maxbits = ix->rmax.as<unsigned>();
+    na.pm = pm0.as<float>();
+    na.pm_ld = pm_ld;
+    KNN_TRY(launch_norms(na, s), "norm kernel launch");
+    // 1. fine balls
+    KNN_TRY(idx_kpp_seed(pm0.as<float>(), pm_ld, dim, R, m, nf, kBallSeed, mind.as<float>(),
+                         cidx.as<int>(), wsum.as<double>(), ticket.as<unsigned>(),
+                         fcent.as<float>(), s),
+            "k-means++ seeding");
+    if (!kmeans(pm0.as<float>(), ref, ldr, R, dim, pm_ld, nf, iters, fcent.as<float>(),
+                fassign.as<int>(), fcount.as<int>(), s))
+        return false;
+    // 2. blocks: seeds over the ball centres, then balanced grouping
+    KNN_TRY(idx_kpp_seed(fcent.as<float>(), pm_ld, dim, nf, nf, nb, kBlockSeed, mind.as<float>(),
+                         cidx.as<int>(), wsum.as<double>(), ticket.as<unsigned>(),
+                         bcent.as<float>(), s),
+            "k-means++ seeding");
+    KNN_TRY(idx_transpose(fcent.as<float>(), pm_ld, nf, dim, fcent_dm.as<float>(), s), "transpose");
+    KNN_TRY(idx_transpose(bcent.as<float>(), pm_ld, nb, dim, bcent_dm.as<float>(), s), "transpose");
+    {
+        knnhip_options ao;
+        memset(&ao, 0, sizeof(ao));
+        const Plan ap = make_plan(nb, nf, dim, kc, ao);
+        if (!run_device(bcent_dm.as<float>(), nb, nb, fcent_dm.as<float>(), nf, nf, dim, kc,
+                        cand_d.as<float>(), cand_i.as<int>(), ap, ao, s))
+            return false;
+    }
+    KNN_TRY(idx_group_fine(cand_i.as<int>(), kc, nf, fcount.as<int>(), nb, cap, fblock.as<int>(), s),
+            "grouping");
+    // 3. layout
+    KNN_TRY(hipMemsetAsync(ix->block_count.p, 0, (size_t)nb * 4, s), "memset");
+    KNN_TRY(hipMemsetAsync(bfine.p, 0, (size_t)nb * 4, s), "memset");
+    KNN_TRY(idx_block_count(fblock.as<int>(), fcount.as<int>(), nf, ix->block_count.as<int>(),
+                            bfine.as<int>(), s),
+            "block count");
+    KNN_TRY(idx_prefix(ix->block_count.as<int>(), nb, kRefsPerTile, ix->block_start.as<int>(),
+                       nullptr, s),
+            "prefix");
+    KNN_TRY(idx_prefix(bfine.as<int>(), nb, 1, ix->fb_start.as<int>(), nullptr, s), "prefix");
+    int n_pos = 0;
+    KNN_TRY(hipMemcpyAsync(&n_pos, ix->block_start.as<int>() + nb, 4, hipMemcpyDeviceToHost, s),
+            "copy");
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    ix->n_pos = n_pos;
+    ix->r_pad = (n_pos + kRefsPerStep - 1) / kRefsPerStep * kRefsPerStep;
+    DevBuf slot_cursor;
+    if (!ix->perm.alloc((size_t)ix->r_pad * 4) ||
+        !ix->packed.alloc((size_t)ix->r_pad * ix->dpad * 4) ||
+        !ix->pm.alloc((size_t)ix->r_pad * pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4) ||
+        !slot_cursor.alloc((size_t)(nb + 1) * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(hipMemcpyAsync(cursor.p, ix->block_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+            "copy");
+    KNN_TRY(hipMemcpyAsync(slot_cursor.p, ix->fb_start.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+            "copy");
+    KNN_TRY(idx_fine_start(fblock.as<int>(), fcount.as<int>(), nf, cursor.as<int>(),
+                           fstart.as<int>(), slot_cursor.as<int>(), fsorted.as<int>(), s),
+            "ball ranges");
+    // positions: ball f's members go to fstart[f] .. fstart[f] + fcount[f]
+    KNN_TRY(hipMemcpyAsync(fcent_dm.p, fstart.p, (size_t)nf * 4, hipMemcpyDeviceToDevice, s),
+            "copy");  // fcent_dm is free now: scatter cursor
+    KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)ix->perm.p, -1, ix->r_pad, s), "memset");
+    KNN_TRY(hipMemsetAsync(ix->pm.p, 0, (size_t)ix->r_pad * pm_ld * 4, s), "memset");
+    KNN_TRY(idx_scatter(fassign.as<int>(), R, fcent_dm.as<int>(), ix->perm.as<int>(), s), "scatter");
+    KNN_TRY(idx_gather_pack(pm0.as<float>(), pm_ld, dim, ix->oct, ix->perm.as<int>(), n_pos,
+                            ix->r_pad, ix->packed.as<float>(), ix->pm.as<float>(),
+                            ix->norms.as<float>(), s),
+            "gather");
+    // balls in block order: centres, radii over their member positions, norms
+    KNN_TRY(idx_finalize_fine(ix->pm.as<float>(), pm_ld, dim, fcent.as<float>(), fsorted.as<int>(),
+                              fstart.as<int>(), fcount.as<int>(), fblock.as<int>(), nf,
+                              ix->fcent.as<float>(), ix->frad.as<double>(), ix->fblock.as<int>(), s),
+            "balls");
+    KNN_TRY(idx_row_norms(ix->fcent.as<float>(), pm_ld, nf, dim, ix->fnorm.as<float>(), s), "norms");
+    KNN_TRY(hipEventRecord(e1, s), "event");
+    KNN_TRY(hipEventSynchronize(e1), "event");
+    KNN_TRY(hipEventElapsedTime(&ix->build_ms, e0, e1), "event");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return true;
+}
+
+// On-disk format of a block index (little-endian): magic "KNNHIPIX", uint32
+// version (1), int32 R, dim, nb, nf, n_pos, r_pad; then block_start [nb+1],
+// block_count [nb], perm [r_pad] (int32), ball centres [nf][pm_ld], centre
+// norms [nf] (float32), radii [nf] (float64), ball blocks [nf] and ball
+// starts [nb+1] (int32).  The packed tiles, point-major copy and norms are
+// rebuilt from the caller's reference set when the file is loaded.
+constexpr char kIndexMagic[8] = {'K', 'N', 'N', 'H', 'I', 'P', 'I', 'X'};
+constexpr unsigned kIndexVersion = 1;
+
+bool index_save(const knnhip_index *ix, const char *path) {
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_error("cannot open %s for writing", path);
+        return false;
+    }
+    FileOut o{f};
+    const int hdr[6] = {ix->R, ix->dim, ix->nb, ix->nf, ix->n_pos, ix->r_pad};
+    o.put(kIndexMagic, 8);
+    o.put(&kIndexVersion, 4);
+    o.put(hdr, sizeof(hdr));
+    o.put_dev(ix->block_start, (size_t)(ix->nb + 1) * 4);
+    o.put_dev(ix->block_count, (size_t)ix->nb * 4);
+    o.put_dev(ix->perm, (size_t)ix->r_pad * 4);
+    o.put_dev(ix->fcent, (size_t)ix->nf * ix->pm_ld * 4);
+    o.put_dev(ix->fnorm, (size_t)ix->nf * 4);
+    o.put_dev(ix->frad, (size_t)ix->nf * 8);
+    o.put_dev(ix->fblock, (size_t)ix->nf * 4);
+    o.put_dev(ix->fb_start, (size_t)(ix->nb + 1) * 4);
+    const bool ok = o.ok && fclose(f) == 0;
+    if (!ok) set_error("cannot write %s", path);
+    return ok;
+}
+
+bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, long long ldr, int dim,
+                hipStream_t s) {
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        set_error("cannot open %s", path);
+        return false;
+    }
+    FileIn in{f};
+    char magic[8] = {0};
+    unsigned version = 0;
+    int hdr[6] = {0};
+    in.get(magic, 8);
+    in.get(&version, 4);
+    in.get(hdr, sizeof(hdr));
+    if (!in.ok || memcmp(magic, kIndexMagic, 8) != 0 || version != kIndexVersion) {
+        fclose(f);
+        set_error("%s is not a block index file of version %u", path, kIndexVersion);
+        return false;
+    }
+    if (hdr[0] != R || hdr[1] != dim || hdr[2] < 1 || hdr[3] < 1 || hdr[4] < R || hdr[5] < hdr[4]) {
+        fclose(f);
+        set_error("%s indexes %d points of dim %d, not %d of dim %d", path, hdr[0], hdr[1], R, dim);
+        return false;
+    }
+    ix->R = R;
+    ix->dim = dim;
+    ix->nb = hdr[2];
+    ix->nf = hdr[3];
+    ix->n_pos = hdr[4];
+    ix->r_pad = hdr[5];
+    ix->ref = ref;
+    ix->ldr = ldr;
+    ix->pm_ld = (dim + 3) & ~3;
+    knnhip_options o;
+    memset(&o, 0, sizeof(o));
+    const Plan p = make_plan(R, 32, dim, 1, o);
+    if (!p.use_scan || ix->pm_ld > kIndexMaxLd) {
+        fclose(f);
+        set_error("dimension too large for the block index");
+        return false;
+    }
+    ix->ks = p.ks;
+    ix->multi = p.multi;
+    ix->nchunks = p.nchunks;
+    ix->dpad = p.dpad;
+    ix->oct = p.oct;
+    in.get_dev(ix->block_start, (size_t)(ix->nb + 1) * 4);
+    in.get_dev(ix->block_count, (size_t)ix->nb * 4);
+    in.get_dev(ix->perm, (size_t)ix->r_pad * 4);
+    in.get_dev(ix->fcent, (size_t)ix->nf * ix->pm_ld * 4);
+    in.get_dev(ix->fnorm, (size_t)ix->nf * 4);
+    in.get_dev(ix->frad, (size_t)ix->nf * 8);
+    in.get_dev(ix->fblock, (size_t)ix->nf * 4);
+    in.get_dev(ix->fb_start, (size_t)(ix->nb + 1) * 4);
+    char extra;
+    const bool tail = fread(&extra, 1, 1, f) == 0;
+    fclose(f);
+    if (!in.ok || !tail) {
+        set_error("%s is truncated or has trailing data", path);
+        return false;
+    }
+    // rebuild the layouts from the references
+    DevBuf pm0, norms0;
+    if (!pm0.alloc((size_t)R * ix->pm_ld * 4) || !norms0.alloc((size_t)R * 4) || !ix->rmax.alloc(4) ||
+        !ix->packed.alloc((size_t)ix->r_pad * ix->dpad * 4) ||
+        !ix->pm.alloc((size_t)ix->r_pad * ix->pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(hipMemsetAsync(ix->rmax.p, 0, 4, s), "memset");
+    NormArgs na{};
+    na.p = ref;
+    na.ld = ldr;
+    na.n = R;
+    na.n_pad = R;
+    na.dim = dim;
+    na.oct = (dim + 7) / 8;
+    na.out = norms0.as<float>();
+    na.maxbits = ix->rmax.as<unsigned>();
+    na.pm = pm0.as<float>();
+    na.pm_ld = ix->pm_ld;
+    KNN_TRY(launch_norms(na, s), "norm kernel launch");
+    KNN_TRY(hipMemsetAsync(ix->pm.p, 0, (size_t)ix->r_pad * ix->pm_ld * 4, s), "memset");
+    KNN_TRY(idx_gather_pack(pm0.as<float>(), ix->pm_ld, dim, ix->oct, ix->perm.as<int>(), ix->n_pos,
+                            ix->r_pad, ix->packed.as<float>(), ix->pm.as<float>(), ix->norms.as<float>(),
+                            s),
+            "gather");
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    return true;
+}
+
+// Workspace of index searches (grown on demand, kept between calls).
+struct IndexWork {
+    DevBuf qnorm, qpm, lb, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks,
+        pval, pidx, theta, fb_count, fb_list, qsub, dsub, isub;
+    bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
+};
+IndexWork g_iw, g_iw_retry;
+
+// Probes of the retry that queries failing their certificate get before the
+// full scan (ranked by the lower bounds alone).
+constexpr int kRetryProbes = 32;
+
+// depth 0: a caller's search; depth 1: the retry of its failed queries.
+bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query, int Q,
+                  long long ldq, int k, int P, const float *router_scores, float *out_dist,
+                  int *out_idx, const knnhip_options &o, hipStream_t s) {
+    const int nb = ix->nb, dim = ix->dim, pm_ld = ix->pm_ld;
+    const int margin = o.margin > 0 ? o.margin : (k >= kLargeK ? kDefaultMarginLargeK : kIndexMargin);
+    const int kp = std::max(k, std::min(k + margin, ix->R));
+    if (scan_lds_bytes(kp, 4) > kScanLdsMax || (long long)P * kp > kMaxUnion) {
+        set_error("k or probe count too large for the block index (k=%d probes=%d)", k, P);
+        return false;
+    }
+    P = std::max(1, std::min(P, nb));
+    const long long npairs = (long long)Q * P;
+    // Two scan phases: each query's first-ranked block, then its other
+    // probes, which start from the thresholds the first phase left.
+    const int nphase = P > 1 ? 2 : 1;
+    const int pr[3] = {0, 1, P};
+    int max_t[2] = {0, 0};
+    for (int ph = 0; ph < nphase; ++ph)
+        max_t[ph] = (int)(((long long)Q * (pr[ph + 1] - pr[ph]) + kQueriesPerWave - 1) /
+                          kQueriesPerWave) + nb;
+    const int max_tasks = max_t[0] + max_t[1];
+    if (!w.need(w.qnorm, (size_t)Q * 4) || !w.need(w.qpm, (size_t)Q * pm_ld * 4) ||
+        !w.need(w.lb, (size_t)Q * nb * 4) ||
+        !w.need(w.chosen, (size_t)npairs * 4) || !w.need(w.lb_rest, (size_t)Q * 4) ||
+        !w.need(w.pcount, (size_t)nb * 4) || !w.need(w.qstart, (size_t)(nb + 1) * 4) ||
+        !w.need(w.cursor, (size_t)(nb + 1) * 4) ||
+        !w.need(w.qlist, (size_t)max_tasks * kQueriesPerWave * 4) ||
+        !w.need(w.rows, (size_t)npairs * 4) || !w.need(w.tasks, (size_t)max_tasks * sizeof(ScanTask)) ||
+        !w.need(w.pval, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
+        !w.need(w.pidx, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
+        !w.need(w.theta, (size_t)Q * 4) || !w.need(w.fb_count, 4) || !w.need(w.fb_list, (size_t)Q * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    hipEvent_t ev[6], evr[2];
+    const bool timing = o.timing != 0;
+    if (timing)
+        for (auto &e : evr) KNN_TRY(hipEventCreate(&e), "event");
+    if (timing)
+        for (auto &e : ev) KNN_TRY(hipEventCreate(&e), "event");
+    if (timing) KNN_TRY(hipEventRecord(ev[0], s), "event");
+    // queries: norms + point-major copy
+    NormArgs nq{};
+    nq.p = query;
+    nq.ld = ldq;
+    nq.n = Q;
+    nq.n_pad = Q;
+    nq.dim = dim;
+    nq.oct = (dim + 7) / 8;
+    nq.out = w.qnorm.as<float>();
+    nq.pm = w.qpm.as<float>();
+    nq.pm_ld = pm_ld;
+    KNN_TRY(launch_norms(nq, s), "norm kernel launch");
+    if (timing) KNN_TRY(hipEventRecord(evr[0], s), "event");
+    // block lower bounds (smallest over each block's balls), block-major;
+    // without a router the ranking is by the negated bound (nearest first)
+    KNN_TRY(idx_ball_lb(w.qpm.as<float>(), pm_ld, Q, dim, w.qnorm.as<float>(), ix->fcent.as<float>(),
+                        ix->nf, ix->fnorm.as<float>(), ix->frad.as<double>(), ix->fblock.as<int>(),
+                        ix->fb_start.as<int>(), nb, w.lb.as<float>(), s),
+            "block bounds");
+    const float *score = router_scores;
+    if (timing) KNN_TRY(hipEventRecord(evr[1], s), "event");
+    KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
+                            w.chosen.as<int>(), w.lb_rest.as<float>(), s),
+            "pick blocks");
+    if (timing) KNN_TRY(hipEventRecord(ev[1], s), "event");
+    if (timing) KNN_TRY(hipEventRecord(ev[2], s), "event");
+    KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Q, s), "memset");
+    KNN_TRY(hipMemsetAsync(w.fb_count.p, 0, 4, s), "memset");
+    int tbase = 0;
+    for (int ph = 0; ph < nphase; ++ph) {
+        // (query, block) pairs of this phase -> per-block query lists -> tasks
+        const int base = tbase * kQueriesPerWave;
+        KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nb * 4, s), "memset");
+        KNN_TRY(idx_pair_count(w.chosen.as<int>(), Q, P, pr[ph], pr[ph + 1], w.pcount.as<int>(), s),
+                "pairs");
+        KNN_TRY(idx_prefix(w.pcount.as<int>(), nb, kQueriesPerWave, w.qstart.as<int>(), nullptr, s),
+                "prefix");
+        KNN_TRY(hipMemcpyAsync(w.cursor.p, w.qstart.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+                "copy");
+        KNN_TRY(idx_pair_scatter(w.chosen.as<int>(), Q, P, pr[ph], pr[ph + 1], w.cursor.as<int>(),
+                                 w.qlist.as<int>() + base, w.rows.as<int>(), base, s),
+                "pairs");
+        KNN_TRY(idx_make_tasks(w.qstart.as<int>(), w.pcount.as<int>(), nb, ix->block_start.as<int>(),
+                               ix->block_count.as<int>(), w.tasks.as<ScanTask>() + tbase, max_t[ph],
+                               s),
+                "tasks");
+        ScanArgs sa{};
+        sa.packed = ix->packed.as<float>();
+        sa.oct = ix->oct;
+        sa.R = ix->n_pos;
+        sa.dim = dim;
+        sa.query = query;
+        sa.ldq = ldq;
+        sa.Q = Q;
+        sa.rnorm = ix->norms.as<float>();
+        sa.theta = w.theta.as<unsigned>();
+        sa.slice_len = ix->r_pad;
+        sa.slice_span = (long long)ix->r_pad * ix->dpad * 4;
+        sa.r_pad = ix->r_pad;
+        sa.nslices = 1;
+        sa.kp = kp;
+        sa.nqb = 1;
+        sa.nchunks = ix->nchunks;
+        sa.xcd_map = 0;
+        sa.part_val = w.pval.as<float>() + (size_t)base * kp;
+        sa.part_idx = w.pidx.as<int>() + (size_t)base * kp;
+        sa.tasks = w.tasks.as<ScanTask>() + tbase;
+        sa.ntasks = max_t[ph];
+        sa.qlist = w.qlist.as<int>() + base;
+        sa.query_pm = w.qpm.as<float>();
+        sa.query_pm_ld = pm_ld;
+        KNN_TRY(launch_scan(sa, ix->ks, ix->multi, 4, s), "scan kernel launch");
+        tbase += max_t[ph];
+    }
+    if (timing) KNN_TRY(hipEventRecord(ev[3], s), "event");
+    // exact re-rank + certificate (skipped blocks bounded by lb_rest)
+    FinalArgs fa{};
+    fa.ref = ix->ref;
+    fa.ldr = ix->ldr;
+    fa.R = ix->R;
+    fa.query = query;
+    fa.ldq = ldq;
+    fa.Q = Q;
+    fa.dim = dim;
+    fa.k = k;
+    fa.kp = kp;
+    fa.nrows = P;
+    fa.rows = w.rows.as<int>();
+    fa.perm = ix->perm.as<int>();
+    fa.maxc = std::min(kMaxCand, std::max(2 * kp, 256));
+    fa.part_val = w.pval.as<float>();
+    fa.part_idx = w.pidx.as<int>();
+    fa.ref_pm = ix->pm.as<float>();
+    fa.query_pm = w.qpm.as<float>();
+    fa.pm_ld = pm_ld;
+    fa.qnorm = w.qnorm.as<float>();
+    fa.rmax = ix->rmax.as<unsigned>();
+    fa.theta = w.theta.as<unsigned>();
+    fa.lb_rest = w.lb_rest.as<float>();
+    fa.force_fallback = o.force_fallback;
+    fa.out_dist = out_dist;
+    fa.out_idx = out_idx;
+    fa.fb_count = w.fb_count.as<int>();
+    fa.fb_list = w.fb_list.as<int>();
+    KNN_TRY(launch_select(fa, s), "select kernel launch");
+    if (timing) KNN_TRY(hipEventRecord(ev[4], s), "event");
+    // queries whose certificate failed: once more through the index with
+    // more probes, then (or at once, on a retry) the plain full scan
+    int nfail = 0;
+    KNN_TRY(hipMemcpyAsync(&nfail, w.fb_count.p, 4, hipMemcpyDeviceToHost, s), "copy");
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    if (nfail > 0) {
+        DevBuf &qsub = w.qsub, &dsub = w.dsub, &isub = w.isub;
+        if (!w.need(qsub, (size_t)nfail * dim * 4) || !w.need(dsub, (size_t)nfail * k * 4) ||
+            !w.need(isub, (size_t)nfail * k * 4)) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        KNN_TRY(idx_gather_columns(query, ldq, dim, w.fb_list.as<int>(), nfail, qsub.as<float>(), s),
+                "gather");
+        knnhip_options po = o;
+        po.force_fallback = 0;
+        po.timing = 0;
+        // the retry widens both the probe count and the candidate margin
+        po.margin = std::max(2 * margin, kDefaultMargin);
+        const int kp2 = std::max(k, std::min(k + po.margin, ix->R));
+        const int P2 = std::min({nb, kMaxUnion / kp2, std::max(4 * P, kRetryProbes)});
+        if (depth == 0 && P2 > P) {
+            if (!index_search(ix, g_iw_retry, 1, qsub.as<float>(), nfail, nfail, k, P2, nullptr,
+                              dsub.as<float>(), isub.as<int>(), po, s))
+                return false;
+        } else {
+            const Plan pp = make_plan(ix->R, nfail, dim, k, po);
+            if (!run_device(ix->ref, ix->ldr, ix->R, qsub.as<float>(), nfail, nfail, dim, k,
+                            dsub.as<float>(), isub.as<int>(), pp, po, s))
+                return false;
+        }
+        KNN_TRY(idx_scatter_results(dsub.as<float>(), isub.as<int>(), k, nfail, w.fb_list.as<int>(),
+                                    Q, out_dist, out_idx, s),
+                "scatter");
+        KNN_TRY(hipStreamSynchronize(s), "sync");
+    }
+    ix->last_probes = P;
+    ix->last_tasks = max_tasks;
+    g_last_stream = s;
+    for (float &m : g_stage_ms) m = -1.f;
+    if (timing) {
+        KNN_TRY(hipEventRecord(ev[5], s), "event");
+        KNN_TRY(hipEventSynchronize(ev[5]), "event");
+        ix->last_fail = nfail;
+        for (int i = 0; i < 5; ++i) KNN_TRY(hipEventElapsedTime(&ix->last_ms[i], ev[i], ev[i + 1]), "event");
+        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[0], ev[0], evr[0]), "event");
+        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[1], evr[0], evr[1]), "event");
+        KNN_TRY(hipEventElapsedTime(&ix->rank_ms[2], evr[1], ev[1]), "event");
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (auto &e : evr) (void)hipEventDestroy(e);
+        // stage slots: 0 ranking (norms, bounds, pick + tasks), 1 scan, 2 select, 3 fallback
+        g_stage_ms[0] = ix->last_ms[0] + ix->last_ms[1];
+        g_stage_ms[1] = ix->last_ms[2];
+        g_stage_ms[2] = ix->last_ms[3];
+        g_stage_ms[3] = ix->last_ms[4];
+    }
+    g_last_fb = w.fb_count.as<int>();
+    return true;
+}
+
+}  // namespace
+}  // namespace knnhip
+
+using namespace knnhip;
+
+extern "C" {
+
+int knnhip_index_build(const float *ref, int ref_nb, long long ref_ld, int dim, int nblocks,
+                       int iters, knnhip_index **out, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!out || !ref || ref_nb < 1 || dim < 1 || ref_ld < ref_nb || nblocks < 1 ||
+        nblocks > ref_nb || nblocks > knnhip::kIndexMaxBlocks || iters < 0) {
+        set_error("bad index arguments");
+        return 0;
+    }
+    *out = nullptr;
+    if (!ensure_device()) return 0;
+    knnhip_index *ix = new knnhip_index();
+    if (!index_build(ix, ref, ref_nb, ref_ld, dim, nblocks, iters, (hipStream_t)stream)) {
+        delete ix;
+        return 0;
+    }
+    *out = ix;
+    return 1;
+}
+
+int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long long query_ld,
+                        int k, int probes, const float *scores, float *knn_dist, int *knn_index,
+                        const knnhip_options *opt, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ix) {
+        set_error("null index");
+        return 0;
+    }
+    if (!check_args(ix->ref, ix->R, query, query_nb, ix->dim, k, knn_dist, knn_index)) return 0;
+    if (query_ld < query_nb || probes < 1) {
+        set_error("bad search arguments");
+        return 0;
+    }
+    if (!ensure_device()) return 0;
+    knnhip_options o = opt ? *opt : default_options(0);
+    char buf[256];
+    snprintf(buf, sizeof(buf),
+             "{\"index_blocks\": %d, \"probes\": %d, \"router\": %s, \"query_nb\": %d, \"k\": %d}",
+             ix->nb, std::min(probes, ix->nb), scores ? "true" : "false", query_nb, k);
+    g_last_plan = buf;
+    return index_search(ix, g_iw, 0, query, query_nb, query_ld, k, probes, scores, knn_dist,
+                        knn_index, o, (hipStream_t)stream)
+               ? 1
+               : 0;
+}
+
+void knnhip_index_free(knnhip_index *ix) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    delete ix;
+}
+
+int knnhip_index_save(const knnhip_index *ix, const char *path) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ix || !path) {
+        set_error("bad index save arguments");
+        return 0;
+    }
+    return index_save(ix, path) ? 1 : 0;
+}
+
+int knnhip_index_load(const char *path, const float *ref, int ref_nb, long long ref_ld, int dim,
+                      knnhip_index **out, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!path || !ref || !out || ref_nb < 1 || dim < 1 || ref_ld < ref_nb) {
+        set_error("bad index load arguments");
+        return 0;
+    }
+    *out = nullptr;
+    if (!ensure_device()) return 0;
+    knnhip_index *ix = new knnhip_index();
+    if (!index_load(ix, path, ref, ref_nb, ref_ld, dim, (hipStream_t)stream)) {
+        delete ix;
+        return 0;
+    }
+    *out = ix;
+    return 1;
+}
+
+int knnhip_index_stage_ms(const knnhip_index *ix, float *ms, int n) {
+    if (!ix || !ms) return 0;
+    const float all[7] = {ix->rank_ms[0], ix->rank_ms[1], ix->rank_ms[2], ix->last_ms[1],
+                          ix->last_ms[2], ix->last_ms[3], ix->last_ms[4]};
+    for (int i = 0; i < n && i < 7; ++i) ms[i] = all[i];
+    return 1;
+}
+
+int knnhip_index_info(const knnhip_index *ix, int *nblocks, int *n_pos, float *build_ms) {
+    if (!ix) return 0;
+    if (nblocks) *nblocks = ix->nb;
+    if (n_pos) *n_pos = ix->n_pos;
+    if (build_ms) *build_ms = ix->build_ms;
+    return 1;
+}
+
+int knnhip_index_layout(const knnhip_index *ix, int *block_start, int *block_count, int *ref_block,
+                        void *stream) {
+    if (!ix) {
+        set_error("null index");
+        return 0;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ensure_device()) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (ref_block)
+        KNN_TRY(idx_ref_block(ix->block_start.as<int>(), ix->block_count.as<int>(), ix->nb,
+                                ix->perm.as<int>(), ref_block, s),
+                  "block of each reference");
+    if (block_start)
+        KNN_TRY(hipMemcpyAsync(block_start, ix->block_start.p, (size_t)ix->nb * 4,
+                                 hipMemcpyDeviceToHost, s),
+                  "copy");
+    if (block_count)
+        KNN_TRY(hipMemcpyAsync(block_count, ix->block_count.p, (size_t)ix->nb * 4,
+                                 hipMemcpyDeviceToHost, s),
+                  "copy");
+    if (block_start || block_count) KNN_TRY(hipStreamSynchronize(s), "sync");
+    return 1;
+}
+}  // extern "C"
