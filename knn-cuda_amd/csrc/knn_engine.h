@@ -51,6 +51,10 @@ hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStre
 int scan_bf3_nch(int dim);
 size_t scan_bf3_lds_bytes(int kp);
 hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s);
+int scan_bf3_task_nch(int dim);
+hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s);
+hipError_t launch_pack_bf3_rows(const float *pm, int ld, int n_pad, int nch, unsigned short *out,
+                                hipStream_t s);
 hipError_t launch_norms(const NormArgs &a, hipStream_t s);
 hipError_t launch_select(const FinalArgs &a, hipStream_t s);
 hipError_t launch_exact(const ExactArgs &a, int grid, hipStream_t s);
@@ -131,6 +135,9 @@ bool check_args(const void *ref, int R, const void *query, int Q, int dim, int k
                 const void *idx);
 bool ensure_device();
 knnhip_options default_options(int variant);
+// Key error coefficients of the bfloat16 three-term products over nch
+// 16-dimension chunks (for FinalArgs::err_q / err_r).
+void bf3_error_bound(int dim, int nch, double *err_q, double *err_r);
 // Prints the last error on stdout as the reference's entry points do.
 bool report(bool ok);
 
@@ -188,6 +195,10 @@ struct knnhip_index {
     // fb_start[b+1]
     int nf = 0;
     knnhip::DevBuf fcent, fnorm, frad, fblock, fb_start;
+    // bfloat16 hi/lo tiles of the block-ordered references for the precision
+    // scan, built by the first search that asks for it; nch_bf chunks
+    int nch_bf = 0;
+    knnhip::DevBuf packed_bf;
     // last search statistics
     int last_probes = 0;
     int last_fail = 0;

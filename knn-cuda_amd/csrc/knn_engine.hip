@@ -220,8 +220,6 @@ bool ensure_device() {
     return true;
 }
 
-namespace {
-
 // Key error bound of the precision scan (see knn_scan_bf16.hip): with
 // u = 2^-8, n = 3 * 16 * nch + 1 FP32 additions (unit 2^-23, allowing for any
 // rounding inside the MFMA) and the FP32 norm over dim terms,
@@ -235,6 +233,8 @@ void bf3_error_bound(int dim, int nch, double *err_q, double *err_r) {
     *err_q = 1.05 * (3.01 * u2 + 1.02 * gn);
     *err_r = 1.05 * (gd + 2.03 * gn + 3.01 * u2);
 }
+
+namespace {
 
 // Buffers of one escalation level: the failed queries' ids, their columns
 // and their results.

@@ -1,5 +1,6 @@
 // Device helpers shared by the scan kernels: accumulator row map, LDS
-// max-heaps of (key, index) pairs and the per-query LDS locks that guard them.
+// max-heaps of (key, index) pairs, the per-query LDS locks that guard them,
+// and the LDS layout of a 32-query scan block.
 #pragma once
 
 #include "knn_common.h"
@@ -61,6 +62,91 @@ __device__ __forceinline__ bool try_lock(int *l) {
 __device__ __forceinline__ void unlock(int *l) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __hip_atomic_store(l, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// LDS of a 32-query scan block: the shared heaps and locks, then one
+// candidate queue (key, reference, query column) per wave.
+constexpr int kQueueCap = 768;  // >= 512 (one half tile of candidates) + slack
+constexpr size_t kQueueBytes = (size_t)kQueueCap * 3 * sizeof(float);
+__host__ __device__ constexpr size_t scan_heap_bytes(int kp) {
+    return (((size_t)kQueriesPerWave * heap_stride(kp) * 2 + kQueriesPerWave) * sizeof(float) + 15) &
+           ~(size_t)15;
+}
+
+// Inserts a wave's queued candidates into the block's shared heaps (heap c
+// at hv / hi + c * kps, guarded by lock[c]).  The wave takes 64 entries at a
+// time, one per lane; entries of one heap are serialised by its lock.  The
+// retry loop is wave-uniform (its condition is a ballot) and each lane makes
+// one attempt per iteration, releasing a lock in the iteration that took it:
+// a per-lane spin loop would let the compiler move a winner's critical
+// section past the loop while a losing lane of the same wave spins on that
+// lock.  Entries no longer below their heap's root are dropped unlocked.
+__device__ __forceinline__ void drain_queue(const float *qv, const int *qr, const int *qc,
+                                            int qcount, float *hv, int *hi, int *lock, int kps,
+                                            int kp, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (int base = 0; base < qcount; base += kWave) {
+        const int j = base + lane;
+        bool pending = j < qcount;
+        const float v = pending ? qv[j] : 0.0f;
+        const int r = pending ? qr[j] : 0;
+        const int c = pending ? qc[j] : 0;
+        float *cv = hv + c * kps;
+        int *ci = hi + c * kps;
+        while (__ballot(pending)) {
+            if (pending) {
+                if (!(v < shared_root(cv))) {
+                    pending = false;
+                } else if (try_lock(lock + c)) {
+                    if (v < cv[0]) heap_replace_top(cv, ci, kp, v, r);
+                    unlock(lock + c);
+                    pending = false;
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Step-local cut of one query column.  A block's first steps can put most of
+// a step's keys below the query's threshold (an empty heap's root is +inf),
+// and every queued key costs a pass through the heap locks.  Bisection on
+// the value of the column's keys below theta (lane halves h = 0, 1 hold the
+// column's rows 8(i/4) + 4h + i%4 of every tile) returns c with at least kp
+// of them below c: queueing only keys below c still brings kp of the step's
+// keys to the heap, so every key left out is at or above the heap's final
+// root.  Rows past the task must already read +inf.  Lanes with need ==
+// false keep theta (the loop runs wave-wide).
+constexpr int kCutIters = 8;
+template <class V, int NT>
+__device__ __forceinline__ float step_cut(const V (&acc)[NT], float theta, int kp, bool need) {
+    float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const float v = acc[t][i];
+            const bool c = v < theta;
+            lo = c ? fminf(lo, v) : lo;
+            hi = c ? fmaxf(hi, v) : hi;
+        }
+    lo = fminf(lo, __shfl_xor(lo, 32));
+    hi = nextafterf(fmaxf(hi, __shfl_xor(hi, 32)), INFINITY);  // every candidate is below hi
+#pragma unroll 1
+    for (int it = 0; it < kCutIters; ++it) {
+        const float mid = 0.5f * lo + 0.5f * hi;
+        int c = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) c += acc[t][i] < mid ? 1 : 0;
+        c += __shfl_xor(c, 32);
+        if (c >= kp)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return need ? hi : theta;
 }
 
 }  // namespace knnhip

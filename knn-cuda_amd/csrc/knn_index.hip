@@ -11,6 +11,8 @@
 // that fails is answered by the plain full scan.
 #include <float.h>
 
+#include <algorithm>
+
 #include "knn_common.h"
 #include "knn_index.h"
 
@@ -341,21 +343,27 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
 }
 
 // The same selection for few queries: one workgroup per query, its threads
-// spread over the blocks, so each round is a workgroup argmax.
+// spread over the blocks, so each round is a workgroup argmax.  The query's
+// rank scores are staged in LDS first (NaN marks an empty block; rank_score
+// never returns NaN), so the P rounds read LDS only.
 __global__ __launch_bounds__(256) void pick_blocks_single_kernel(
     const float *__restrict__ score, int Q, int nb, int P, const float *__restrict__ lb,
     const int *__restrict__ counts, int *__restrict__ chosen, float *__restrict__ lb_rest) {
+    extern __shared__ float sc[];  // nb rank scores
     __shared__ float red_s[4];
     __shared__ int red_b[4];
     const int q = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int b = tid; b < nb; b += blockDim.x)
+        sc[b] = counts[b] == 0 ? NAN : rank_score(score, lb, (long long)b * Q + q);
+    __syncthreads();
     float last_s = INFINITY;
     int last_b = -1;
     for (int p = 0; p < P; ++p) {
         float bs = -INFINITY;
         int bb = -1;
         for (int b = tid; b < nb; b += blockDim.x) {
-            if (counts[b] == 0) continue;
-            const float s = rank_score(score, lb, (long long)b * Q + q);
+            const float s = sc[b];
+            if (s != s) continue;
             const bool after = (s < last_s) || (s == last_s && b > last_b);
             if (after && (bb < 0 || s > bs)) {
                 bs = s;
@@ -393,8 +401,8 @@ __global__ __launch_bounds__(256) void pick_blocks_single_kernel(
     }
     float m = INFINITY;
     for (int b = tid; b < nb; b += blockDim.x) {
-        if (counts[b] == 0) continue;
-        const float s = rank_score(score, lb, (long long)b * Q + q);
+        const float s = sc[b];
+        if (s != s) continue;
         const bool after = (s < last_s) || (s == last_s && b > last_b);
         if (after) m = fminf(m, lb[(long long)b * Q + q]);
     }
@@ -970,14 +978,16 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
 }
 
 // The same bounds with the query operands in registers: a workgroup takes
-// 128 queries (32 per wave) and the kBallTilesPerWg consecutive ball tiles
-// [F0, F1); every wave walks those tiles (so their loads are shared through
+// 128 queries (32 per wave) and `tpw` consecutive ball tiles [F0, F1)
+// (fewer for few queries, so that the grid still covers the chip); every
+// wave walks those tiles (so their loads are shared through
 // L1) against its own 32 queries, whose half-rows (KH floats per lane,
 // KH % 4 == 0, KH <= KHM) stay in VGPRs.  The per-block fold runs on lanes
 // 0..31 (one query each) over the balls in order and carries the open block
 // from tile to tile, so a block is merged with an atomic min only when it
 // extends past [F0, F1); every other block is stored once.
-constexpr int kBallTilesPerWg = 8;
+constexpr int kBallTilesPerWg = 8;  // tiles per workgroup for many queries
+constexpr int kBallMinWgs = 1024;    // workgroups aimed for when there are few
 
 template <int KHM>
 __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restrict__ qpm, int pm_ld,
@@ -988,7 +998,7 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
                                                           const double *__restrict__ frad,
                                                           const int *__restrict__ block_s,
                                                           const int *__restrict__ fb_start,
-                                                          float *__restrict__ lb) {
+                                                          float *__restrict__ lb, int tpw) {
     __shared__ float patch[4][32][33];
     // per wave and tile: each ball's block, and that block's first ball and
     // one past its last (loaded once per tile by lanes 0..31)
@@ -996,12 +1006,12 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
     const int ntf = (nf + 31) / 32;
-    const int ngrp = (ntf + kBallTilesPerWg - 1) / kBallTilesPerWg;
+    const int ngrp = (ntf + tpw - 1) / tpw;
     const int qg = blockIdx.x / ngrp, grp = blockIdx.x - qg * ngrp;
     const int q0 = qg * 128 + wave * 32;
     if (q0 >= Q) return;  // whole wave past the last query (no workgroup barrier below)
-    const int F0 = grp * kBallTilesPerWg * 32;
-    const int F1 = min(nf, F0 + kBallTilesPerWg * 32);
+    const int F0 = grp * tpw * 32;
+    const int F1 = min(nf, F0 + tpw * 32);
     const int KH = (dim + 1) / 2;
     const int qa = min(q0 + col, Q - 1);
     // this lane's query half-row: dims h*KH .. h*KH + KH - 1
@@ -1026,7 +1036,7 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
         else
             atomicMin(reinterpret_cast<unsigned *>(dst), __float_as_uint(m));
     };
-    for (int tt = 0; tt < kBallTilesPerWg; ++tt) {
+    for (int tt = 0; tt < tpw; ++tt) {
         const int f0 = F0 + tt * 32;
         if (f0 >= F1) break;  // uniform over the wave
         const int fa = min(f0 + col, nf - 1);
@@ -1133,10 +1143,16 @@ hipError_t idx_gather_pack(const float *pm_src, int pm_ld, int dim, int oct, con
 }
 hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float *lb,
                            const int *counts, int *chosen, float *lb_rest, hipStream_t s) {
-    if (Q <= kPickSingleMaxQ)
-        hipLaunchKernelGGL(pick_blocks_single_kernel, dim3(Q), dim3(256), 0, s, score, Q, nb, P, lb,
+    if (Q <= kPickSingleMaxQ) {
+        const size_t lds = (size_t)nb * sizeof(float);
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void *)pick_blocks_single_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(pick_blocks_single_kernel, dim3(Q), dim3(256), lds, s, score, Q, nb, P, lb,
                            counts, chosen, lb_rest);
-    else if (P <= 8)
+    } else if (P <= 8)
         hipLaunchKernelGGL(pick_blocks_list_kernel<8>, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
                            score, Q, nb, P, lb, counts, chosen, lb_rest);
     else if (P <= 16)
@@ -1243,14 +1259,15 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
     if (e != hipSuccess) return e;
     const int KH = (dim + 1) / 2;
     if (KH % 4 == 0 && pm_ld % 4 == 0 && KH <= 64) {
-        const long long wgs = (long long)((Q + 127) / 128) *
-                              (((nf + 31) / 32 + kBallTilesPerWg - 1) / kBallTilesPerWg);
+        const long long qgs = (Q + 127) / 128, ntf = (nf + 31) / 32;
+        const int tpw = (int)std::max(1ll, std::min<long long>(kBallTilesPerWg, qgs * ntf / kBallMinWgs));
+        const long long wgs = qgs * ((ntf + tpw - 1) / tpw);
         if (KH <= 32)
             hipLaunchKernelGGL(ball_lb_reg_kernel<32>, dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
-                               Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb);
+                               Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb, tpw);
         else
             hipLaunchKernelGGL(ball_lb_reg_kernel<64>, dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
-                               Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb);
+                               Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb, tpw);
         return hipGetLastError();
     }
     const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);

@@ -15,6 +15,16 @@ namespace knnhip {
 namespace {
 
 constexpr int kIndexMargin = 8;  // block-index searches: fewer heap insertions per block
+// ... with bfloat16 products, whose wider error bound needs a wider gap
+// between the k-th and the kp-th candidate to certify
+constexpr int kIndexMarginBf3 = 12;
+// Waves per block-scan task.  Heap insertions of one query run one after
+// another, so where the chip holds many tasks (kOneWaveQueries or more
+// queries per block on average) one wave per task (up to eight tasks per CU,
+// each with its own heaps) keeps more insertions in flight than four waves
+// sharing one task's heaps; where tasks are few (a retry of a few queries)
+// four waves per task shorten each task.
+constexpr double kOneWaveQueries = 4.0;
 
 // Lloyd iterations on the point-major rows `pm` (n points) from the `k`
 // centroids already in `cent` (point-major, ld pm_ld).  The assignment step
@@ -358,7 +368,13 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                   long long ldq, int k, int P, const float *router_scores, float *out_dist,
                   int *out_idx, const knnhip_options &o, hipStream_t s) {
     const int nb = ix->nb, dim = ix->dim, pm_ld = ix->pm_ld;
-    const int margin = o.margin > 0 ? o.margin : (k >= kLargeK ? kDefaultMarginLargeK : kIndexMargin);
+    // precision scan: bfloat16 three-term products for dimensions up to 128
+    // (else FP32 products)
+    const int nch_bf = o.precision == 1 ? scan_bf3_task_nch(dim) : 0;
+    const int margin = o.margin > 0     ? o.margin
+                       : k >= kLargeK   ? kDefaultMarginLargeK
+                       : nch_bf         ? kIndexMarginBf3
+                                        : kIndexMargin;
     const int kp = std::max(k, std::min(k + margin, ix->R));
     if (scan_lds_bytes(kp, 4) > kScanLdsMax || (long long)P * kp > kMaxUnion) {
         set_error("k or probe count too large for the block index (k=%d probes=%d)", k, P);
@@ -366,6 +382,17 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     }
     P = std::max(1, std::min(P, nb));
     const long long npairs = (long long)Q * P;
+    if (nch_bf && ix->nch_bf != nch_bf) {
+        ix->nch_bf = 0;
+        if (!ix->packed_bf.alloc((size_t)ix->r_pad / kRefsPerTile * nch_bf * 2 * kWave * 16)) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        KNN_TRY(launch_pack_bf3_rows(ix->pm.as<float>(), pm_ld, ix->r_pad, nch_bf,
+                                     ix->packed_bf.as<unsigned short>(), s),
+                "bfloat16 tiles");
+        ix->nch_bf = nch_bf;
+    }
     // Two scan phases: each query's first-ranked block, then its other
     // probes, which start from the thresholds the first phase left.
     const int nphase = P > 1 ? 2 : 1;
@@ -466,7 +493,14 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         sa.qlist = w.qlist.as<int>() + base;
         sa.query_pm = w.qpm.as<float>();
         sa.query_pm_ld = pm_ld;
-        KNN_TRY(launch_scan(sa, ix->ks, ix->multi, 4, s), "scan kernel launch");
+        const int waves = (double)Q * (pr[ph + 1] - pr[ph]) / nb >= kOneWaveQueries ? 1 : 4;
+        if (nch_bf) {
+            sa.packed_bf = ix->packed_bf.as<unsigned short>();
+            KNN_TRY(launch_scan_bf3_task(sa, nch_bf, waves, scan_lds_bytes(kp, waves), s),
+                    "scan kernel launch");
+        } else {
+            KNN_TRY(launch_scan(sa, ix->ks, ix->multi, waves, s), "scan kernel launch");
+        }
         tbase += max_t[ph];
     }
     if (timing) KNN_TRY(hipEventRecord(ev[3], s), "event");
@@ -494,6 +528,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     fa.rmax = ix->rmax.as<unsigned>();
     fa.theta = w.theta.as<unsigned>();
     fa.lb_rest = w.lb_rest.as<float>();
+    if (nch_bf) bf3_error_bound(dim, nch_bf, &fa.err_q, &fa.err_r);
     fa.force_fallback = o.force_fallback;
     fa.out_dist = out_dist;
     fa.out_idx = out_idx;
@@ -518,8 +553,10 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         knnhip_options po = o;
         po.force_fallback = 0;
         po.timing = 0;
-        // the retry widens both the probe count and the candidate margin
+        // the retry widens both the probe count and the candidate margin, and
+        // takes FP32 products (a bfloat16 failure is mostly its wider bound)
         po.margin = std::max(2 * margin, kDefaultMargin);
+        po.precision = 0;
         const int kp2 = std::max(k, std::min(k + po.margin, ix->R));
         const int P2 = std::min({nb, kMaxUnion / kp2, std::max(4 * P, kRetryProbes)});
         if (depth == 0 && P2 > P) {
@@ -605,9 +642,12 @@ int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long
     if (!ensure_device()) return 0;
     knnhip_options o = opt ? *opt : default_options(0);
     char buf[256];
+    const bool bf3 = o.precision == 1 && scan_bf3_task_nch(ix->dim) != 0;
     snprintf(buf, sizeof(buf),
-             "{\"index_blocks\": %d, \"probes\": %d, \"router\": %s, \"query_nb\": %d, \"k\": %d}",
-             ix->nb, std::min(probes, ix->nb), scores ? "true" : "false", query_nb, k);
+             "{\"index_blocks\": %d, \"probes\": %d, \"router\": %s, \"query_nb\": %d, \"k\": %d, "
+             "\"precision\": \"%s\"}",
+             ix->nb, std::min(probes, ix->nb), scores ? "true" : "false", query_nb, k,
+             bf3 ? "bf16x3" : "fp32");
     g_last_plan = buf;
     return index_search(ix, g_iw, 0, query, query_nb, query_ld, k, probes, scores, knn_dist,
                         knn_index, o, (hipStream_t)stream)

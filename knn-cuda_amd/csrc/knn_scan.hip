@@ -36,15 +36,6 @@ namespace knnhip {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// LDS of one scan block: the shared heaps and locks, then one candidate
-// queue per wave.
-constexpr int kQueueCap = 768;  // >= 512 (one half tile of candidates) + slack
-constexpr size_t kQueueBytes = (size_t)kQueueCap * 3 * sizeof(float);
-__host__ __device__ constexpr size_t scan_heap_bytes(int kp) {
-    return (((size_t)kQueriesPerWave * heap_stride(kp) * 2 + kQueriesPerWave) * sizeof(float) + 15) &
-           ~(size_t)15;
-}
-
 // KS: k-steps per dimension chunk (B operands held in VGPRs), a multiple of 4.
 // NBUF: ring of one-octet (4 k-step) load buffers; loads run NBUF-1 octets
 // ahead of the MFMAs that consume them.
@@ -98,7 +89,10 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         task_row = (slice * a.nqb + qb) * kQueriesPerWave;
     }
     if (nq <= 0) return;  // whole block is past the last query
-    const int qsel = col < nq ? col : 0;
+    // Columns past the task's queries compute query 0's keys (their heaps are
+    // never written out) and take no candidates.
+    const bool qvalid = col < nq;
+    const int qsel = qvalid ? col : 0;
     const int q = qids ? qids[qsel] : qfirst + qsel;
 
     // Shared heaps (odd row stride: the lanes of a half walk 32 heaps in
@@ -280,18 +274,31 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
         // appended to the wave's candidate queue; the queue is drained into
         // the heaps by all lanes at once when it fills.
         theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
-        bool cand = false;
+        if (r0 + kRefsPerStep > r_end) {
+            // rows past the task take no part below
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                    if (r0 + kRefsPerTile * mt + acc_row(i, h) >= r_end) acc[mt][i] = INFINITY;
+        }
+        int ncand = 0;
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) cand |= acc[mt][i] < theta;
-        if (__ballot(cand)) {
+            for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
+        if (!qvalid) ncand = 0;
+        if (__ballot(ncand > 0)) {
+            // many candidates (a young heap): queue only a step-local cut
+            const int npair = ncand + __shfl_xor(ncand, 32);
+            float cut = theta;
+            if (__ballot(npair >= 2 * kp)) cut = step_cut(acc, theta, kp, npair >= 2 * kp);
 #pragma unroll
             for (int mt = 0; mt < kMTiles; ++mt) {
                 bool cm = false;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) cm |= acc[mt][i] < theta;
-                if (__ballot(cm) == 0) continue;
+                for (int i = 0; i < 16; ++i) cm |= acc[mt][i] < cut;
+                if (__ballot(cm && qvalid) == 0) continue;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     if ((i & 7) == 0 && qcount > kQueueCap - 8 * kWave) {
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
                         theta = fminf(theta, shared_root(myv));
                     }
                     const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
-                    const bool take = acc[mt][i] < theta && r < r_end;
+                    const bool take = qvalid && acc[mt][i] < cut && r < r_end;
                     const unsigned long long m = __ballot(take);
                     if (m == 0) continue;
                     const int n = __popcll(m);
@@ -316,7 +323,7 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
             if (qcount >= kQueueCap / 2) {
                 drain();
                 theta = fminf(theta, shared_root(myv));
-                if (h == 0 && theta < published) {
+                if (h == 0 && qvalid && theta < published) {
                     published = theta;
                     __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
@@ -326,7 +333,7 @@ __global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
     }
     drain();
     theta = fminf(theta, shared_root(myv));
-    if (h == 0 && theta < published)
+    if (h == 0 && qvalid && theta < published)
         __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
 
@@ -440,6 +447,43 @@ __device__ inline void write_bf3(unsigned short *base, int tile, int nch, int j,
     pl.w = lo[6] | ((unsigned)lo[7] << 16);
     *reinterpret_cast<uint4 *>(dst) = ph;
     *reinterpret_cast<uint4 *>(dst + kWave * 8) = pl;
+}
+
+// bfloat16 hi/lo tiles of n_pad points given as point-major rows (row stride
+// ld, a multiple of 4, pads zero; dimensions past ld read as zero): one
+// thread per (tile, chunk, lane).
+__global__ __launch_bounds__(256) void pack_bf3_rows_kernel(const float *pm, int ld, int n_pad,
+                                                            int nch, unsigned short *out) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)(n_pad / kRefsPerTile) * nch * kWave) return;
+    const int lane = (int)(g % kWave);
+    const long long tc = g / kWave;
+    const int c = (int)(tc % nch);
+    const int tile = (int)(tc / nch);
+    const int m = lane & 31, h = lane >> 5;
+    const float *row = pm + ((long long)tile * kRefsPerTile + m) * ld;
+    float v[8];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const int d4 = 16 * c + 8 * h + 4 * p;
+        const float4 x = d4 < ld ? *reinterpret_cast<const float4 *>(row + d4)
+                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        v[4 * p + 0] = x.x;
+        v[4 * p + 1] = x.y;
+        v[4 * p + 2] = x.z;
+        v[4 * p + 3] = x.w;
+    }
+    write_bf3(out, tile, nch, 2 * c + h, m, v);
+}
+
+hipError_t launch_pack_bf3_rows(const float *pm, int ld, int n_pad, int nch, unsigned short *out,
+                                hipStream_t s) {
+    if (n_pad % kRefsPerTile || ld % 4 || nch < 1) return hipErrorInvalidValue;
+    const long long threads = (long long)(n_pad / kRefsPerTile) * nch * kWave;
+    if (threads == 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_bf3_rows_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       s, pm, ld, n_pad, nch, out);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -365,4 +365,283 @@ hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s) 
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------
+// Task form of the precision scan, for task lists whose tasks hold up to 32
+// queries (block-index and batched searches): the FP32 scan's block shape
+// (four waves on one task's 32 queries, wave w taking reference steps w,
+// w+4, ..., all feeding one LDS heap per query) with the three-term bfloat16
+// products.  The hi + lo tiles hold as many bytes per reference as the FP32
+// tiles, and a 32 x 32 tile costs 3 x 32 MFMA cycles per 16 dimensions
+// instead of 8 x 64.  Keys start from ||r||^2 and add the same products in
+// the same chunk order as the 128-query form, so the same error bound holds.
+// NB: chunks in the register ring (loads run NB-1 chunks ahead).
+// ---------------------------------------------------------------------------
+template <int NCH, int NB>
+__global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    static_assert(NCH % NB == 0 && NB >= 2, "ring position must repeat every step");
+    const int waves = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 31, h = lane >> 5;
+
+    const ScanTask tk = a.tasks[blockIdx.x];
+    const int r_begin = tk.ref_begin, r_end = tk.ref_end, nq = tk.qcount;
+    if (nq <= 0) return;  // empty task: uniform over the block
+    // columns past the task's queries compute query 0's keys and take nothing
+    const bool qvalid = col < nq;
+    const int q = a.qlist[tk.qoff + (qvalid ? col : 0)];
+    const int task_row = blockIdx.x * kQueriesPerWave;
+
+    const int kp = a.kp, kps = heap_stride(kp);
+    float *hv = reinterpret_cast<float *>(smem);
+    int *hi = reinterpret_cast<int *>(smem + (size_t)kQueriesPerWave * kps * sizeof(float));
+    int *lock = hi + (size_t)kQueriesPerWave * kps;
+    for (int e = threadIdx.x; e < kQueriesPerWave * kps; e += blockDim.x) {
+        hv[e] = INFINITY;
+        hi[e] = -1;
+    }
+    if (threadIdx.x < kQueriesPerWave) lock[threadIdx.x] = 0;
+    __syncthreads();
+    float *myv = hv + col * kps;
+    float theta = INFINITY;
+    float published = INFINITY;
+    unsigned *theta_q = a.theta + q;
+    unsigned char *qbase = smem + scan_heap_bytes(kp) + (size_t)wave * kQueueBytes;
+    float *qv = reinterpret_cast<float *>(qbase);
+    int *qr = reinterpret_cast<int *>(qbase + kQueueCap * sizeof(float));
+    int *qc = reinterpret_cast<int *>(qbase + kQueueCap * 2 * sizeof(float));
+    int qcount = 0;  // wave-uniform
+    const unsigned long long lanes_below = (1ull << lane) - 1ull;
+
+    // B operands: bfloat16 hi / lo of -2q, lane (col, h) holding dimensions
+    // 16c + 8h .. +7 of chunk c
+    bf16x8_t bh[NCH], bl[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        float x[8];
+        if (a.query_pm) {
+            const float *row = a.query_pm + (long long)q * a.query_pm_ld;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int d4 = 16 * c + 8 * h + 4 * p;
+                const float4 v = d4 < a.query_pm_ld ? *reinterpret_cast<const float4 *>(row + d4)
+                                                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                x[4 * p + 0] = -2.0f * v.x;
+                x[4 * p + 1] = -2.0f * v.y;
+                x[4 * p + 2] = -2.0f * v.z;
+                x[4 * p + 3] = -2.0f * v.w;
+            }
+        } else {
+            const float *qcol = a.query + q;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int d = 16 * c + 8 * h + e;
+                const float v = qcol[(long long)min(d, a.dim - 1) * a.ldq];
+                x[e] = d < a.dim ? -2.0f * v : 0.0f;
+            }
+        }
+        bh[c] = split_hi_lo(x, bl[c]);
+    }
+
+    // A operands through one buffer resource over the task's tiles: loads
+    // past them (the prefetch beyond a wave's last step) return zeros.
+    constexpr long long kTileBytes = (long long)NCH * 2 * kWave * 16;
+    const int span_refs = (r_end - r_begin + kRefsPerStep - 1) / kRefsPerStep * kRefsPerStep;
+    const long long valid_tiles = max(0, min(span_refs, a.r_pad - r_begin)) / kRefsPerTile;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<unsigned short *>(a.packed_bf) + (long long)(r_begin / kRefsPerTile) * (kTileBytes / 2),
+        (short)0, (int)(unsigned)(valid_tiles * kTileBytes), 0x00020000);
+    auto lane_offsets = [&](int r_start, unsigned *vo) {
+        const long long t0 = (r_start - r_begin) / kRefsPerTile;
+#pragma unroll
+        for (int mt = 0; mt < kMTiles; ++mt)
+            vo[mt] = (unsigned)min((t0 + mt) * kTileBytes + lane * 16, 0xFFFFFFF0ll);
+    };
+    u32x4_t ring[NB][kMTiles][2];
+    auto issue = [&](u32x4_t(&buf)[kMTiles][2], int c, const unsigned *vo) {
+#pragma unroll
+        for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+            for (int hl = 0; hl < 2; ++hl)
+                buf[mt][hl] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)vo[mt],
+                                                                   (2 * c + hl) * kWave * 16, 0);
+    };
+
+    const int wstride = waves * kRefsPerStep;
+    const int r_first = r_begin + wave * kRefsPerStep;
+    unsigned voff[kMTiles];
+    lane_offsets(r_first, voff);
+    if (r_first < r_end) {
+#pragma unroll
+        for (int j = 0; j < NB - 1; ++j) issue(ring[j], j, voff);
+    }
+
+    for (int r0 = r_first; r0 < r_end; r0 += wstride) {
+        f32x16_t acc[kMTiles];
+        // start from ||r||^2: element i of tile mt is reference
+        // r0 + 32 mt + 8 (i / 4) + 4h + i % 4
+        if (r0 + kRefsPerStep <= a.R) {
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+                for (int g4 = 0; g4 < 4; ++g4) {
+                    const float4 nr = *reinterpret_cast<const float4 *>(
+                        a.rnorm + r0 + kRefsPerTile * mt + 8 * g4 + 4 * h);
+                    acc[mt][4 * g4 + 0] = nr.x;
+                    acc[mt][4 * g4 + 1] = nr.y;
+                    acc[mt][4 * g4 + 2] = nr.z;
+                    acc[mt][4 * g4 + 3] = nr.w;
+                }
+        } else {
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                    acc[mt][i] = r < a.R ? a.rnorm[r] : INFINITY;
+                }
+        }
+        const unsigned tq = __hip_atomic_load(theta_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned voff_next[kMTiles];
+        lane_offsets(r0 + wstride, voff_next);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int ci = c + NB - 1;
+            if (ci < NCH)
+                issue(ring[ci % NB], ci, voff);
+            else
+                issue(ring[ci % NB], ci - NCH, voff_next);
+            __builtin_amdgcn_sched_barrier(0);
+            // three products per tile; consecutive MFMAs target different
+            // accumulators
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(bf16x8_t, ring[c % NB][mt][0]), bh[c], acc[mt], 0, 0, 0);
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(bf16x8_t, ring[c % NB][mt][0]), bl[c], acc[mt], 0, 0, 0);
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(bf16x8_t, ring[c % NB][mt][1]), bh[c], acc[mt], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int mt = 0; mt < kMTiles; ++mt) voff[mt] = voff_next[mt];
+
+        // Epilogue (as the FP32 scan): keys below the freshest threshold go
+        // to the wave's queue, drained into the shared heaps when it fills.
+        theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
+        if (r0 + kRefsPerStep > r_end) {
+            // rows past the task take no part below
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                    if (r0 + kRefsPerTile * mt + acc_row(i, h) >= r_end) acc[mt][i] = INFINITY;
+        }
+        int ncand = 0;
+#pragma unroll
+        for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
+        if (!qvalid) ncand = 0;
+        if (__ballot(ncand > 0)) {
+            // many candidates (a young heap): queue only a step-local cut
+            const int npair = ncand + __shfl_xor(ncand, 32);
+            float cut = theta;
+            if (__ballot(npair >= 2 * kp)) cut = step_cut(acc, theta, kp, npair >= 2 * kp);
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt) {
+                bool cm = false;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) cm |= acc[mt][i] < cut;
+                if (__ballot(cm && qvalid) == 0) continue;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    if ((i & 7) == 0 && qcount > kQueueCap - 8 * kWave) {
+                        drain_queue(qv, qr, qc, qcount, hv, hi, lock, kps, kp, lane);
+                        qcount = 0;
+                        theta = fminf(theta, shared_root(myv));
+                    }
+                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                    const bool take = qvalid && acc[mt][i] < cut && r < r_end;
+                    const unsigned long long m = __ballot(take);
+                    if (m == 0) continue;
+                    if (take) {
+                        const int pos = qcount + __popcll(m & lanes_below);
+                        qv[pos] = acc[mt][i];
+                        qr[pos] = r;
+                        qc[pos] = col;
+                    }
+                    qcount += __popcll(m);
+                }
+            }
+            if (qcount >= kQueueCap / 2) {
+                drain_queue(qv, qr, qc, qcount, hv, hi, lock, kps, kp, lane);
+                qcount = 0;
+                theta = fminf(theta, shared_root(myv));
+                if (h == 0 && qvalid && theta < published) {
+                    published = theta;
+                    __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+    }
+    drain_queue(qv, qr, qc, qcount, hv, hi, lock, kps, kp, lane);
+    theta = fminf(theta, shared_root(myv));
+    if (h == 0 && qvalid && theta < published)
+        __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+
+    // the task's heaps leave as rows task_row .. task_row + nq - 1
+    __syncthreads();
+    const long long obase = (long long)task_row * kp;
+    for (int e = threadIdx.x; e < nq * kp; e += blockDim.x) {
+        const int row = e / kp, j = e - row * kp;
+        a.part_val[obase + e] = hv[row * kps + j];
+        a.part_idx[obase + e] = hi[row * kps + j];
+    }
+}
+
+#define KNN_BF3_TASK_INSTANCES(X) \
+    X(2, 2)                       \
+    X(4, 2)                       \
+    X(6, 3)                       \
+    X(8, 2)
+
+// Chunks of the task form for a dimension, or 0: wider dimensions keep FP32
+// products (twelve chunks of B operands would not fit in VGPRs beside the
+// four accumulators and the load ring).
+int scan_bf3_task_nch(int dim) {
+    const int n = (dim + 15) / 16;
+    for (int c : {2, 4, 6, 8})
+        if (c >= n) return c;
+    return 0;
+}
+
+// Task lists only (a.tasks, one block of `waves` waves per task), with the
+// LDS of the FP32 scan's 32-query blocks of that many waves.
+hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s) {
+    if (!a.tasks || !a.packed_bf) return hipErrorInvalidValue;
+#define X(N, B)                                                                                 \
+    if (nch == N) {                                                                             \
+        const void *fn = (const void *)scan_bf3_task_kernel<N, B>;                              \
+        if (lds > 65536) {                                                                      \
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                               (int)lds);                                       \
+            if (e != hipSuccess) return e;                                                      \
+        }                                                                                       \
+        hipLaunchKernelGGL((scan_bf3_task_kernel<N, B>), dim3(a.ntasks), dim3(waves * kWave), lds, s, \
+                           a);                                                                  \
+        return hipGetLastError();                                                               \
+    }
+    KNN_BF3_TASK_INSTANCES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 }  // namespace knnhip

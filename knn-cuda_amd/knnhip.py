@@ -414,7 +414,7 @@ class BlockIndex:
 
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
                margin: int = 0, force_fallback: bool = False, timing: bool = False,
-               stream: Optional[int] = None):
+               precision: str = "fp32", stream: Optional[int] = None):
         import torch
 
         lib = load()
@@ -433,10 +433,13 @@ class BlockIndex:
                 raise KnnError("scores must be float32 (nblocks, query_nb): one column per query")
             scores = scores.contiguous()
             sp = scores.data_ptr()
+        if precision not in ("fp32", "bf16x3"):
+            raise KnnError("precision must be 'fp32' or 'bf16x3'")
         opt = Options()
         opt.margin = margin
         opt.force_fallback = 1 if force_fallback else 0
         opt.timing = 1 if timing else 0
+        opt.precision = 1 if precision == "bf16x3" else 0
         if stream is None:
             stream = torch.cuda.current_stream(query.device).cuda_stream
         ok = lib.knnhip_index_search(self._h, query.data_ptr(), query_nb, query.stride(0), k, probes,

@@ -153,3 +153,62 @@ def test_many_queries_both_rankings(ranking):
     say(f"{ranking}: first-pass certificate failures {knnhip.last_fallback_count()} / {Q}")
     check(d, i, gd, gi, ranking)
     ix.close()
+
+
+# precision scan in the index: every task-form instance (2, 4, 6, 8 chunks of 16 dims), ragged
+# dims, one and several probes, uniform and clustered data
+BF3_CASES = [
+    (6000, 300, 24, 8, "uniform"),
+    (9000, 257, 64, 16, "mixture"),
+    (12000, 400, 96, 16, "mixture"),
+    (7000, 129, 128, 32, "uniform"),
+    (5000, 70, 100, 1, "mixture"),
+]
+
+
+@pytest.mark.parametrize("probes", [1, 3])
+@pytest.mark.parametrize("case", BF3_CASES)
+def test_precision_scan_equals_oracle(case, probes):
+    R, Q, D, k, dist = case
+    say(f"index bf16x3 {case} probes={probes}")
+    if dist == "uniform":
+        ref = knnhip.generate_uniform(R, D, seed=R + D)
+        qry = knnhip.generate_uniform(Q, D, seed=R + D + 1)
+    else:
+        ref = knnhip.generate_mixture(R, D, 48, 0.1, R + D, 1)
+        qry = knnhip.generate_mixture(Q, D, 48, 0.1, R + D, 2)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    ix = knnhip.BlockIndex(ref, nblocks=48, iters=4)
+    d, i = ix.search(qry, k, probes=probes, precision="bf16x3")
+    torch.cuda.synchronize()
+    assert knnhip.last_plan()["precision"] == "bf16x3"
+    say(f"certificate failures {knnhip.last_fallback_count()}")
+    check(d, i, gd, gi, f"{case} probes={probes}")
+    # the FP32 products give the same result
+    d2, i2 = ix.search(qry, k, probes=probes)
+    assert knnhip.last_plan()["precision"] == "fp32"
+    assert torch.equal(d, d2) and torch.equal(i, i2)
+    ix.close()
+
+
+def test_precision_scan_with_scores_and_ties():
+    # coarse lattice (many equal distances) and duplicated points, ranked by random scores
+    rng = np.random.default_rng(17)
+    base = (np.floor(rng.random((40, 2000)) * 4) / 4).astype(np.float32)
+    ref = torch.from_numpy(np.concatenate([base, base[:, :500]], axis=1)).cuda()
+    qry = torch.from_numpy((np.floor(rng.random((40, 150)) * 4) / 4).astype(np.float32)).cuda()
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), 12)
+    ix = knnhip.BlockIndex(ref, nblocks=16, iters=3)
+    scores = torch.rand((16, 150), device="cuda")
+    d, i = ix.search(qry, 12, probes=2, scores=scores, precision="bf16x3")
+    check(d, i, gd, gi, "lattice bf16x3")
+
+
+def test_precision_scan_keeps_fp32_above_128_dims():
+    ref = knnhip.generate_uniform(3000, 160, seed=3)
+    qry = knnhip.generate_uniform(40, 160, seed=4)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), 5)
+    ix = knnhip.BlockIndex(ref, nblocks=8, iters=2)
+    d, i = ix.search(qry, 5, probes=2, precision="bf16x3")
+    assert knnhip.last_plan()["precision"] == "fp32"
+    check(d, i, gd, gi, "dim 160")
