@@ -149,36 +149,49 @@ def router_form(args, knnhip, torch):
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t
     rows = []
-    for P in (2, 4, 8):
-        ms, (d, i) = timed(lambda: ix.search(qry, k, probes=P), args.router_reps)
-        equal = bool(torch.equal(i, pi) and torch.equal(d, pd))
-        ix.search(qry, k, probes=P, timing=True)
-        rows.append({"probes": P, "ms": ms, "queries_per_s": Q / ms * 1e3, "speedup_vs_plain": plain_ms / ms,
-                     "certificate_failures": knnhip.last_fallback_count(), "equal_to_plain": equal,
-                     "stages_ms": {a: round(b, 4) for a, b in ix.stage_ms().items()}})
+    for prec in ("fp32", "bf16x3"):
+        for P in (4, 6, 8):
+            ms, (d, i) = timed(lambda: ix.search(qry, k, probes=P, precision=prec), args.router_reps)
+            equal = bool(torch.equal(i, pi) and torch.equal(d, pd))
+            ix.search(qry, k, probes=P, precision=prec, timing=True)
+            rows.append({"precision": prec, "probes": P, "ms": ms, "queries_per_s": Q / ms * 1e3,
+                         "speedup_vs_plain": plain_ms / ms,
+                         "certificate_failures": knnhip.last_fallback_count(), "equal_to_plain": equal,
+                         "stages_ms": {a: round(b, 4) for a, b in ix.stage_ms().items()}})
     best = min(rows, key=lambda r: r["ms"])
-    # learned router: trained on the GPU, then used as the block ranking
+    # learned router: trained on the GPU, then used as the block ranking with either product
     trq = knnhip.generate_mixture(65536, D, 4096, 0.1, seed_c, seed_t)
     mlp = knnhip.Router(D, 256, NB, seed=0x6B4E4E07)
     t = time.perf_counter()
     loss = mlp.train(ix, trq, k, epochs=8, batch=1024, lr=1e-3)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t
-    P = best["probes"]
-    ms_m, (d, i) = timed(lambda: ix.search(qry, k, probes=P, scores=mlp.scores(qry)), args.router_reps)
-    mlp_row = {"probes": P, "ms": ms_m, "speedup_vs_plain": plain_ms / ms_m,
-               "certificate_failures": knnhip.last_fallback_count(),
-               "equal_to_plain": bool(torch.equal(i, pi) and torch.equal(d, pd)),
-               "train_s": train_s, "train_queries": 65536, "epochs": 8, "final_loss": loss,
-               "layers": [D, 256, 256, NB], "dtype": "bf16 products, fp32 accumulation"}
+    mlp_rows = []
+    for prec in ("fp32", "bf16x3"):
+        P = min((r for r in rows if r["precision"] == prec), key=lambda r: r["ms"])["probes"]
+        ms_m, (d, i) = timed(lambda: ix.search(qry, k, probes=P, scores=mlp.scores(qry), precision=prec),
+                             args.router_reps)
+        mlp_rows.append({"precision": prec, "probes": P, "ms": ms_m, "queries_per_s": Q / ms_m * 1e3,
+                         "speedup_vs_plain": plain_ms / ms_m,
+                         "certificate_failures": knnhip.last_fallback_count(),
+                         "equal_to_plain": bool(torch.equal(i, pi) and torch.equal(d, pd))})
     mlp.close()
     ix.close()
+    bf = mlp_rows[1]
     return {"workload": "4,194,304 refs x 16,384 queries, dim 96, k 16, mixture of 4,096 Gaussians "
                         "(sigma 0.1), 2,048 blocks",
             "plain_fp32_ms": plain_ms, "plain_queries_per_s": Q / plain_ms * 1e3,
             "index_build_s": build_s, "ranking": "certificate lower bounds",
-            "best": {"probes": best["probes"], "ms": best["ms"], "speedup_vs_plain": best["speedup_vs_plain"]},
-            "probes": rows, "mlp_router": mlp_row}
+            "best": {"precision": best["precision"], "probes": best["probes"], "ms": best["ms"],
+                     "speedup_vs_plain": best["speedup_vs_plain"]},
+            # the form the target names: bfloat16 three-term products, exact FP32 re-rank, learned router
+            "bf16_rerank_router": {"probes": bf["probes"], "ms": bf["ms"], "queries_per_s": bf["queries_per_s"],
+                                   "speedup_vs_plain": bf["speedup_vs_plain"],
+                                   "equal_to_plain": bf["equal_to_plain"]},
+            "probes": rows,
+            "mlp_router": {"train_s": train_s, "train_queries": 65536, "epochs": 8, "final_loss": loss,
+                           "layers": [D, 256, 256, NB], "dtype": "bf16 products, fp32 accumulation",
+                           "searches": mlp_rows}}
 
 
 def batched_form(knnhip, torch):

@@ -195,6 +195,9 @@ struct knnhip_index {
     // fb_start[b+1]
     int nf = 0;
     knnhip::DevBuf fcent, fnorm, frad, fblock, fb_start;
+    // per ball, rebuilt from fnorm / frad: sqrt of the centre norm and the
+    // radius, both rounded up to float
+    knnhip::DevBuf fsq, fradf;
     // bfloat16 hi/lo tiles of the block-ordered references for the precision
     // scan, built by the first search that asks for it; nch_bf chunks
     int nch_bf = 0;

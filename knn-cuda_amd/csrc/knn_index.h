@@ -54,9 +54,12 @@ hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcen
                              const int *fine_sorted, const int *fstart, const int *fcount,
                              const int *fblock, int nf, float *cent_s, double *rad_s, int *block_s,
                              hipStream_t s);
+hipError_t idx_ball_aux(const float *fnorm, const double *frad, int nf, float *fsq, float *fradf,
+                        hipStream_t s);
 hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
-                       const float *fcent, int nf, const float *fnorm, const double *frad,
-                       const int *block_s, const int *fb_start, int nb, float *lb, hipStream_t s);
+                       const float *fcent, int nf, const float *fnorm, const float *fsq,
+                       const float *fradf, const int *block_s, const int *fb_start, int nb, float *lb,
+                       hipStream_t s);
 hipError_t idx_batch_perm(const int *starts, const int *counts, int nsets, int n_pos, int *perm,
                           hipStream_t s);
 

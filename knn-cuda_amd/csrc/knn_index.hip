@@ -259,12 +259,25 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
         ll[j] = INFINITY;
         lbk[j] = 0x7FFFFFFF;
     }
-    if (valid)
-        for (int b = b0; b < b1; ++b) {
-            if (counts[b] == 0) continue;
+    // bounds are read kPickBatch blocks at a time (independent loads), then
+    // taken in block order
+    constexpr int kPickBatch = 16;
+    for (int bb = b0; valid && bb < b1; bb += kPickBatch) {
+        float sv[kPickBatch], lv[kPickBatch];
+        bool live[kPickBatch];
+#pragma unroll
+        for (int j = 0; j < kPickBatch; ++j) {
+            const int b = min(bb + j, b1 - 1);
+            live[j] = bb + j < b1 && counts[b] != 0;
             const long long e = (long long)b * Q + q;
-            const float s = rank_score(score, lb, e);
-            const float l = lb[e];
+            sv[j] = rank_score(score, lb, e);
+            lv[j] = lb[e];
+        }
+#pragma unroll
+        for (int j = 0; j < kPickBatch; ++j) {
+            if (!live[j]) continue;
+            const int b = bb + j;
+            const float s = sv[j], l = lv[j];
             // blocks come in increasing order, so an equal score ranks after
             // the entries already kept
             if (n < P || s > ls[P - 1]) {
@@ -296,6 +309,7 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
                 rest = fminf(rest, l);
             }
         }
+    }
     // merge rounds
     int head = 0;
     for (int p = 0; p < P; ++p) {
@@ -868,13 +882,51 @@ __global__ __launch_bounds__(256) void finalize_fine_kernel(const float *__restr
 // per-block minima: a block wholly inside those 16 balls is stored, one
 // that extends past them is merged with an atomic min on the float bits
 // (non-negative floats order like their bit patterns).
+// max(0, |q - c| - rho) rounded down, in FP32 arithmetic.  nq, nc, dot
+// carry the FP32 norm and MFMA errors, within 1.5 g S for
+// S = (sqrt(nq) + sqrt(nc))^2 (g15 >= 1.5 g).  S also bounds nq + nc, 2|dot|
+// and every partial result of d2 = nq + nc - 2 dot - A, so each of its three
+// roundings is within 1.01 u S (u = 2^-24); the allowance
+// A = S_f (g15 + 2^-21)(1 + 2^-20), with S_f = (sq + sc)^2 from square roots
+// rounded up, covers both.  The hardware square root (within 1 ulp) is
+// lowered by 2^-21 relative, the radius comes rounded up (ru < 0: empty
+// ball), and the difference is lowered by 2^-22 relative.
+__device__ __forceinline__ float ball_lower_bound(float nq, float sq, float nc, float sc, float dot,
+                                                  float ru, float g15) {
+    if (ru < 0.0f) return INFINITY;
+    const float st = sq + sc;
+    const float a = st * st * (1.0f + 0x1p-20f) * (g15 + 0x1p-21f);
+    const float d2 = fmaxf(fmaf(-2.0f, dot, nq + nc) - a, 0.0f);
+    const float v = __builtin_amdgcn_sqrtf(d2) * (1.0f - 0x1p-21f) - ru;
+    return v > 0.0f ? v * (1.0f - 0x1p-22f) : 0.0f;
+}
+
+// Per ball: the square root of the centre's squared norm and the radius,
+// both rounded up to float (radius -1 for an empty ball), for the bounds.
+__global__ void ball_aux_kernel(const float *__restrict__ fnorm, const double *__restrict__ frad,
+                                int nf, float *__restrict__ fsq, float *__restrict__ fradf) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        fsq[f] = nextafterf(sqrtf(fnorm[f]), INFINITY);
+        const double rad = frad[f];
+        float ru = (float)rad;
+        if ((double)ru < rad) ru = nextafterf(ru, INFINITY);
+        fradf[f] = rad >= 0.0 ? ru : -1.0f;
+    }
+}
+
+// 1.5 g, g = 2 (dim + 4) 2^-24, rounded up to float.
+__device__ __forceinline__ float ball_error_coeff(int dim) {
+    return (float)(3.0 * (double)(dim + 4) * 5.9604644775390625e-8) * (1.0f + 0x1p-20f);
+}
+
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ qpm, int pm_ld, int Q,
                                                       int dim, const float *__restrict__ qnorm,
                                                       const float *__restrict__ fcent, int nf,
                                                       const float *__restrict__ fnorm,
-                                                      const double *__restrict__ frad,
+                                                      const float *__restrict__ fsq,
+                                                      const float *__restrict__ fradf,
                                                       const int *__restrict__ block_s,
                                                       const int *__restrict__ fb_start,
                                                       float *__restrict__ lb) {
@@ -928,25 +980,15 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
             }
         }
         // acc[i] is ball f0 + 8(i/4) + 4h + i%4 for query q0 + col
-        const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
-        const double nq = (double)qnorm[qa];
+        const float g15 = ball_error_coeff(dim);
+        const float nq = qnorm[qa];
+        const float sq = nextafterf(sqrtf(nq), INFINITY);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int r = ((i >> 2) << 3) + (h << 2) + (i & 3);
             const int f = f0 + r;
-            float out = INFINITY;
-            const double rad = f < nf ? frad[f] : -1.0;
-            if (rad >= 0.0) {
-                const double nc = (double)fnorm[f];
-                const double dot = (double)acc[i];
-                double d2 = nq + nc - 2.0 * dot - g * (nq + nc + 2.0 * sqrt(nq * nc)) * 1.5;
-                d2 = d2 > 0.0 ? d2 : 0.0;
-                const double v = sqrt(d2) * (1.0 - 1e-12) - rad;
-                float fv = (float)v;
-                if ((double)fv > v) fv = nextafterf(fv, -INFINITY);
-                out = fv > 0.0f ? fv : 0.0f;
-            }
-            patch[wave][r][col] = out;
+            patch[wave][r][col] = f < nf ? ball_lower_bound(nq, sq, fnorm[f], fsq[f], acc[i], fradf[f], g15)
+                                         : INFINITY;
         }
     }
     __syncthreads();
@@ -980,12 +1022,15 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
 // The same bounds with the query operands in registers: a workgroup takes
 // 128 queries (32 per wave) and `tpw` consecutive ball tiles [F0, F1)
 // (fewer for few queries, so that the grid still covers the chip); every
-// wave walks those tiles (so their loads are shared through
-// L1) against its own 32 queries, whose half-rows (KH floats per lane,
-// KH % 4 == 0, KH <= KHM) stay in VGPRs.  The per-block fold runs on lanes
-// 0..31 (one query each) over the balls in order and carries the open block
-// from tile to tile, so a block is merged with an atomic min only when it
-// extends past [F0, F1); every other block is stored once.
+// wave walks those tiles (so their loads are shared through L1) against its
+// own 32 queries, whose half-rows (KH floats per lane, KH % 4 == 0,
+// KH <= KHM) stay in VGPRs.  The per-block fold runs in registers on lanes
+// 0..31 (one query each): a tile's 32 bounds of a query sit on its two lane
+// halves (rows 8(i/4) + 4h + i%4), the partner half's come over by a lane
+// swap, and each row's block and that block's ball range come from the lane
+// of the row by readlane.  The fold carries the open block from tile to
+// tile, so a block is merged with an atomic min only when it extends past
+// [F0, F1); every other block is stored once.
 constexpr int kBallTilesPerWg = 8;  // tiles per workgroup for many queries
 constexpr int kBallMinWgs = 1024;    // workgroups aimed for when there are few
 
@@ -995,14 +1040,11 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
                                                           const float *__restrict__ qnorm,
                                                           const float *__restrict__ fcent, int nf,
                                                           const float *__restrict__ fnorm,
-                                                          const double *__restrict__ frad,
+                                                          const float *__restrict__ fsq,
+                                                          const float *__restrict__ fradf,
                                                           const int *__restrict__ block_s,
                                                           const int *__restrict__ fb_start,
                                                           float *__restrict__ lb, int tpw) {
-    __shared__ float patch[4][32][33];
-    // per wave and tile: each ball's block, and that block's first ball and
-    // one past its last (loaded once per tile by lanes 0..31)
-    __shared__ int binfo[4][3][32];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
     const int ntf = (nf + 31) / 32;
@@ -1022,8 +1064,9 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
         for (int u = 0; u < KHM / 4; ++u)
             bq[u] = 4 * u < KH ? *reinterpret_cast<const float4 *>(qr + 4 * u) : make_float4(0, 0, 0, 0);
     }
-    const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
-    const double nq = (double)qnorm[qa];
+    const float g15 = ball_error_coeff(dim);
+    const float nq = qnorm[qa];
+    const float sq = nextafterf(sqrtf(nq), INFINITY);
     const int q = q0 + col;
     const bool folder = h == 0 && q < Q;
     int cur = -1, cur_s = 0, cur_e = 0;
@@ -1052,53 +1095,67 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
                 acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[u].w, acc, 0, 0, 0);
             }
         }
+        // element i is ball f0 + 8(i/4) + 4h + i%4: four consecutive balls per
+        // group, read as one float4 each of norms, their roots and radii
+        const bool full = f0 + 32 <= nf;
+        float out[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int r = ((i >> 2) << 3) + (h << 2) + (i & 3);
-            const int f = f0 + r;
-            float out = INFINITY;
-            const double rad = f < nf ? frad[f] : -1.0;
-            if (rad >= 0.0) {
-                const double nc = (double)fnorm[f];
-                const double dot = (double)acc[i];
-                double d2 = nq + nc - 2.0 * dot - g * (nq + nc + 2.0 * sqrt(nq * nc)) * 1.5;
-                d2 = d2 > 0.0 ? d2 : 0.0;
-                const double v = sqrt(d2) * (1.0 - 1e-12) - rad;
-                float fv = (float)v;
-                if ((double)fv > v) fv = nextafterf(fv, -INFINITY);
-                out = fv > 0.0f ? fv : 0.0f;
+        for (int g4 = 0; g4 < 4; ++g4) {
+            const int fb = f0 + 8 * g4 + 4 * h;
+            float4 n4, s4, r4;
+            if (full) {
+                n4 = *reinterpret_cast<const float4 *>(fnorm + fb);
+                s4 = *reinterpret_cast<const float4 *>(fsq + fb);
+                r4 = *reinterpret_cast<const float4 *>(fradf + fb);
+            } else {
+                n4.x = fb + 0 < nf ? fnorm[fb + 0] : 0.0f;
+                n4.y = fb + 1 < nf ? fnorm[fb + 1] : 0.0f;
+                n4.z = fb + 2 < nf ? fnorm[fb + 2] : 0.0f;
+                n4.w = fb + 3 < nf ? fnorm[fb + 3] : 0.0f;
+                s4.x = fb + 0 < nf ? fsq[fb + 0] : 0.0f;
+                s4.y = fb + 1 < nf ? fsq[fb + 1] : 0.0f;
+                s4.z = fb + 2 < nf ? fsq[fb + 2] : 0.0f;
+                s4.w = fb + 3 < nf ? fsq[fb + 3] : 0.0f;
+                r4.x = fb + 0 < nf ? fradf[fb + 0] : -1.0f;
+                r4.y = fb + 1 < nf ? fradf[fb + 1] : -1.0f;
+                r4.z = fb + 2 < nf ? fradf[fb + 2] : -1.0f;
+                r4.w = fb + 3 < nf ? fradf[fb + 3] : -1.0f;
             }
-            patch[wave][r][col] = out;
+            out[4 * g4 + 0] = ball_lower_bound(nq, sq, n4.x, s4.x, acc[4 * g4 + 0], r4.x, g15);
+            out[4 * g4 + 1] = ball_lower_bound(nq, sq, n4.y, s4.y, acc[4 * g4 + 1], r4.y, g15);
+            out[4 * g4 + 2] = ball_lower_bound(nq, sq, n4.z, s4.z, acc[4 * g4 + 2], r4.z, g15);
+            out[4 * g4 + 3] = ball_lower_bound(nq, sq, n4.w, s4.w, acc[4 * g4 + 3], r4.w, g15);
         }
-        if (h == 0) {
-            const int f = f0 + col;
-            const int b = f < nf ? block_s[f] : -1;
-            binfo[wave][0][col] = b;
-            binfo[wave][1][col] = b >= 0 ? fb_start[b] : 0;
-            binfo[wave][2][col] = b >= 0 ? fb_start[b + 1] : 0;
+        // the partner half's rows, and (lane r < 32) row r's block and that
+        // block's first ball and one past its last
+        float oth[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) oth[i] = __shfl_xor(out[i], 32);
+        int rb = -1, rs = 0, re = 0;
+        if (h == 0 && f0 + col < nf) {
+            rb = block_s[f0 + col];
+            rs = fb_start[rb];
+            re = fb_start[rb + 1];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int n = min(32, F1 - f0);
         if (folder) {
-            const int n = min(32, F1 - f0);
-            for (int r = 0; r < n; ++r) {
-                const int b = binfo[wave][0][r];
-                const float v = patch[wave][r][col];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                if (r >= n) continue;  // (not break: the loop must unroll)
+                const int b = __builtin_amdgcn_readlane(rb, r);
+                // on lane half 0, rows with (r / 4) even are its own
+                const float v = ((r >> 2) & 1) ? oth[4 * (r >> 3) + (r & 3)] : out[4 * (r >> 3) + (r & 3)];
                 if (b != cur) {
                     flush();
                     cur = b;
-                    cur_s = binfo[wave][1][r];
-                    cur_e = binfo[wave][2][r];
+                    cur_s = __builtin_amdgcn_readlane(rs, r);
+                    cur_e = __builtin_amdgcn_readlane(re, r);
                     m = v;
                 } else {
                     m = fminf(m, v);
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (folder) flush();
 }
@@ -1252,9 +1309,15 @@ hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcen
                        fine_sorted, fstart, fcount, fblock, cent_s, rad_s, block_s);
     return hipGetLastError();
 }
+hipError_t idx_ball_aux(const float *fnorm, const double *frad, int nf, float *fsq, float *fradf,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(ball_aux_kernel, dim3(grid_for(nf)), dim3(256), 0, s, fnorm, frad, nf, fsq, fradf);
+    return hipGetLastError();
+}
 hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
-                       const float *fcent, int nf, const float *fnorm, const double *frad,
-                       const int *block_s, const int *fb_start, int nb, float *lb, hipStream_t s) {
+                       const float *fcent, int nf, const float *fnorm, const float *fsq,
+                       const float *fradf, const int *block_s, const int *fb_start, int nb, float *lb,
+                       hipStream_t s) {
     hipError_t e = hipMemsetD32Async((hipDeviceptr_t)lb, 0x7F800000, (size_t)nb * Q, s);
     if (e != hipSuccess) return e;
     const int KH = (dim + 1) / 2;
@@ -1264,15 +1327,15 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
         const long long wgs = qgs * ((ntf + tpw - 1) / tpw);
         if (KH <= 32)
             hipLaunchKernelGGL(ball_lb_reg_kernel<32>, dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
-                               Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb, tpw);
+                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw);
         else
             hipLaunchKernelGGL(ball_lb_reg_kernel<64>, dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
-                               Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb, tpw);
+                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw);
         return hipGetLastError();
     }
     const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);
     hipLaunchKernelGGL(ball_lb_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, qpm,
-                       pm_ld, Q, dim, qnorm, fcent, nf, fnorm, frad, block_s, fb_start, lb);
+                       pm_ld, Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb);
     return hipGetLastError();
 }
 hipError_t idx_batch_perm(const int *starts, const int *counts, int nsets, int n_pos, int *perm,

@@ -219,6 +219,13 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
                               ix->fcent.as<float>(), ix->frad.as<double>(), ix->fblock.as<int>(), s),
             "balls");
     KNN_TRY(idx_row_norms(ix->fcent.as<float>(), pm_ld, nf, dim, ix->fnorm.as<float>(), s), "norms");
+    if (!ix->fsq.alloc((size_t)nf * 4) || !ix->fradf.alloc((size_t)nf * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(idx_ball_aux(ix->fnorm.as<float>(), ix->frad.as<double>(), nf, ix->fsq.as<float>(),
+                         ix->fradf.as<float>(), s),
+            "ball roots");
     KNN_TRY(hipEventRecord(e1, s), "event");
     KNN_TRY(hipEventSynchronize(e1), "event");
     KNN_TRY(hipEventElapsedTime(&ix->build_ms, e0, e1), "event");
@@ -324,11 +331,15 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
     // rebuild the layouts from the references
     DevBuf pm0, norms0;
     if (!pm0.alloc((size_t)R * ix->pm_ld * 4) || !norms0.alloc((size_t)R * 4) || !ix->rmax.alloc(4) ||
+        !ix->fsq.alloc((size_t)ix->nf * 4) || !ix->fradf.alloc((size_t)ix->nf * 4) ||
         !ix->packed.alloc((size_t)ix->r_pad * ix->dpad * 4) ||
         !ix->pm.alloc((size_t)ix->r_pad * ix->pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4)) {
         set_error("Memory allocation error");
         return false;
     }
+    KNN_TRY(idx_ball_aux(ix->fnorm.as<float>(), ix->frad.as<double>(), ix->nf, ix->fsq.as<float>(),
+                         ix->fradf.as<float>(), s),
+            "ball roots");
     KNN_TRY(hipMemsetAsync(ix->rmax.p, 0, 4, s), "memset");
     NormArgs na{};
     na.p = ref;
@@ -438,8 +449,8 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     // block lower bounds (smallest over each block's balls), block-major;
     // without a router the ranking is by the negated bound (nearest first)
     KNN_TRY(idx_ball_lb(w.qpm.as<float>(), pm_ld, Q, dim, w.qnorm.as<float>(), ix->fcent.as<float>(),
-                        ix->nf, ix->fnorm.as<float>(), ix->frad.as<double>(), ix->fblock.as<int>(),
-                        ix->fb_start.as<int>(), nb, w.lb.as<float>(), s),
+                        ix->nf, ix->fnorm.as<float>(), ix->fsq.as<float>(), ix->fradf.as<float>(),
+                        ix->fblock.as<int>(), ix->fb_start.as<int>(), nb, w.lb.as<float>(), s),
             "block bounds");
     const float *score = router_scores;
     if (timing) KNN_TRY(hipEventRecord(evr[1], s), "event");
