@@ -70,7 +70,10 @@ typedef struct knnhip_options {
                            result; dim <= 192 and k + margin <= 107, else FP32) */
     int block_queries;  /* FP32 scan shape: 0 chosen, 32 (reference tiles per wave), 128
                            (reference tiles shared through LDS; dim <= 192, k + margin <= 107) */
-    int reserved[6];
+    int score_all;      /* index searches given scores: 0 the scores choose each query's
+                           first probe (scanned first), the lower bounds rank the others;
+                           1 the scores rank every probe */
+    int reserved[5];
 } knnhip_options;
 
 /*
@@ -122,8 +125,9 @@ int knnhip_generate_mixture(float *out, int n, int dim, int ncomp, float sigma,
  * rest only where the triangle inequality (over the blocks' balls) proves
  * they hold no neighbour, so results equal knn_cuda_global's exactly.
  * `scores` (nblocks x query_nb, row-major, device: one column per query, like
- * the point layout) ranks blocks per query, e.g. router outputs; NULL ranks
- * by the blocks' distance lower bounds.  The index keeps a pointer to `ref`
+ * the point layout), e.g. router outputs, picks each query's first probe, the
+ * block scanned first (all probes with options.score_all); the distance lower
+ * bounds rank the rest, and all of them when scores is NULL.  The index keeps a pointer to `ref`
  * (the full-scan fallback reads it): the caller keeps that buffer alive and
  * unchanged.  nblocks <= min(ref_nb, 32768).
  */

@@ -27,6 +27,8 @@ hipError_t idx_gather_pack(const float *pm_src, int pm_ld, int dim, int oct, con
                            hipStream_t s);
 hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float *lb,
                            const int *counts, int *chosen, float *lb_rest, hipStream_t s);
+hipError_t idx_router_first(const float *score, int Q, int nb, const int *counts, float *lb,
+                            hipStream_t s);
 hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int *counts,
                           hipStream_t s);
 hipError_t idx_pair_scatter(const int *chosen, int Q, int P, int p0, int p1, int *cursor,

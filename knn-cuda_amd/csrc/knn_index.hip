@@ -430,6 +430,53 @@ __global__ __launch_bounds__(256) void pick_blocks_single_kernel(
     }
 }
 
+// Router-first ranking: each query's highest-scored non-empty block (ties to
+// the lower block) gets the bound -inf, so that the bound ranking takes it
+// as the first probe and ranks the others by their bounds.  A workgroup
+// takes 64 queries (one per lane) and splits the blocks over its 16 waves
+// (scores read kBatch blocks at a time); the waves' best are merged in LDS.
+__global__ __launch_bounds__(64 * kPickWaves) void router_first_kernel(
+    const float *__restrict__ score, int Q, int nb, const int *__restrict__ counts,
+    float *__restrict__ lb) {
+    constexpr int kBatch = 16;
+    __shared__ float best_s[kPickWaves][64];
+    __shared__ int best_b[kPickWaves][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = blockIdx.x * 64 + lane;
+    const int per = (nb + kPickWaves - 1) / kPickWaves;
+    const int b0 = min(nb, wave * per), b1 = min(nb, b0 + per);
+    float bs = -INFINITY;
+    int bb = -1;
+    for (int bq = b0; q < Q && bq < b1; bq += kBatch) {
+        float sv[kBatch];
+        bool live[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int b = min(bq + j, b1 - 1);
+            live[j] = bq + j < b1 && counts[b] != 0;
+            sv[j] = rank_score(score, lb, (long long)b * Q + q);
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j)
+            if (live[j] && (bb < 0 || sv[j] > bs)) {
+                bs = sv[j];
+                bb = bq + j;
+            }
+    }
+    best_s[wave][lane] = bs;
+    best_b[wave][lane] = bb;
+    __syncthreads();
+    if (wave == 0 && q < Q) {
+        // waves hold increasing block ranges: strict > keeps the lower block
+        for (int w = 1; w < kPickWaves; ++w)
+            if (best_b[w][lane] >= 0 && (bb < 0 || best_s[w][lane] > bs)) {
+                bs = best_s[w][lane];
+                bb = best_b[w][lane];
+            }
+        if (bb >= 0) lb[(long long)bb * Q + q] = -INFINITY;
+    }
+}
+
 // Invert the (query -> blocks) lists, probes [p0, p1) of each query's P, into
 // per-block query lists and scan tasks of up to 32 queries.  Step 1: count
 // pairs per block.
@@ -1218,6 +1265,12 @@ hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float
     else
         hipLaunchKernelGGL(pick_blocks_kernel, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
                            score, Q, nb, P, lb, counts, chosen, lb_rest);
+    return hipGetLastError();
+}
+hipError_t idx_router_first(const float *score, int Q, int nb, const int *counts, float *lb,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(router_first_kernel, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s, score, Q,
+                       nb, counts, lb);
     return hipGetLastError();
 }
 hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int *counts,

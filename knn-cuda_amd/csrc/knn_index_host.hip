@@ -452,7 +452,11 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                         ix->nf, ix->fnorm.as<float>(), ix->fsq.as<float>(), ix->fradf.as<float>(),
                         ix->fblock.as<int>(), ix->fb_start.as<int>(), nb, w.lb.as<float>(), s),
             "block bounds");
-    const float *score = router_scores;
+    // scores pick the first probe (or rank every probe, with score_all)
+    const float *score = o.score_all ? router_scores : nullptr;
+    if (router_scores && !o.score_all)
+        KNN_TRY(idx_router_first(router_scores, Q, nb, ix->block_count.as<int>(), w.lb.as<float>(), s),
+                "router first probe");
     if (timing) KNN_TRY(hipEventRecord(evr[1], s), "event");
     KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
                             w.chosen.as<int>(), w.lb_rest.as<float>(), s),
@@ -655,10 +659,10 @@ int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long
     char buf[256];
     const bool bf3 = o.precision == 1 && scan_bf3_task_nch(ix->dim) != 0;
     snprintf(buf, sizeof(buf),
-             "{\"index_blocks\": %d, \"probes\": %d, \"router\": %s, \"query_nb\": %d, \"k\": %d, "
-             "\"precision\": \"%s\"}",
-             ix->nb, std::min(probes, ix->nb), scores ? "true" : "false", query_nb, k,
-             bf3 ? "bf16x3" : "fp32");
+             "{\"index_blocks\": %d, \"probes\": %d, \"router\": %s, \"scores_rank\": \"%s\", "
+             "\"query_nb\": %d, \"k\": %d, \"precision\": \"%s\"}",
+             ix->nb, std::min(probes, ix->nb), scores ? "true" : "false",
+             !scores ? "none" : o.score_all ? "all" : "first", query_nb, k, bf3 ? "bf16x3" : "fp32");
     g_last_plan = buf;
     return index_search(ix, g_iw, 0, query, query_nb, query_ld, k, probes, scores, knn_dist,
                         knn_index, o, (hipStream_t)stream)

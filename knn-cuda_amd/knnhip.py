@@ -84,7 +84,8 @@ class Options(ctypes.Structure):
         ("timing", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("block_queries", ctypes.c_int),
-        ("reserved", ctypes.c_int * 6),
+        ("score_all", ctypes.c_int),
+        ("reserved", ctypes.c_int * 5),
     ]
 
 
@@ -414,7 +415,7 @@ class BlockIndex:
 
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
                margin: int = 0, force_fallback: bool = False, timing: bool = False,
-               precision: str = "fp32", stream: Optional[int] = None):
+               precision: str = "fp32", score_all: bool = False, stream: Optional[int] = None):
         import torch
 
         lib = load()
@@ -440,6 +441,7 @@ class BlockIndex:
         opt.force_fallback = 1 if force_fallback else 0
         opt.timing = 1 if timing else 0
         opt.precision = 1 if precision == "bf16x3" else 0
+        opt.score_all = 1 if score_all else 0
         if stream is None:
             stream = torch.cuda.current_stream(query.device).cuda_stream
         ok = lib.knnhip_index_search(self._h, query.data_ptr(), query_nb, query.stride(0), k, probes,

@@ -131,13 +131,12 @@ def test_router_search_is_exact_and_gains_over_centroid_ranking(trained):
     rec_c = neighbour_block_recall(cscore, nbr_blocks, 1)
     say(f"neighbour share in the top block: router {rec_r:.4f}, centroid ranking {rec_c:.4f}")
     for P in (1, 2, 4):
-        d, i = ix.search(test_q, k, probes=P, scores=scores)
-        fr = knnhip.last_fallback_count()
-        d2, i2 = ix.search(test_q, k, probes=P, scores=cscore)
-        fc = knnhip.last_fallback_count()
-        d3, i3 = ix.search(test_q, k, probes=P)
-        fb = knnhip.last_fallback_count()
-        say(f"probes {P}: certificate failures router {fr}, centroid {fc}, bounds {fb}")
-        for dd, ii in ((d, i), (d2, i2), (d3, i3)):
-            assert torch.equal(ii, pi) and torch.equal(dd, pd)
+        res = {}
+        for name, sc, all_ in (("router", scores, True), ("centroid", cscore, True),
+                               ("router first", scores, False), ("bounds", None, False)):
+            d, i = ix.search(test_q, k, probes=P, scores=sc, score_all=all_)
+            res[name] = knnhip.last_fallback_count()
+            assert knnhip.last_plan()["scores_rank"] == ("none" if sc is None else "all" if all_ else "first")
+            assert torch.equal(i, pi) and torch.equal(d, pd), f"{name}, probes {P}"
+        say(f"probes {P}: certificate failures {res}")
     assert rec_r >= rec_c
