@@ -64,13 +64,13 @@ int batch_max_slices(const BatchClass &c, int L) {
 // Untuned choice: the slice lengths whose per-query row count fits the
 // re-rank's union (rows * kp <= kMaxUnion), costed as rounds of resident
 // blocks times (steps per wave + a fixed per-task cost of ~3 steps).
-int batch_default_slice(const BatchClass &c, int kp, int resident) {
+int batch_default_slice(const BatchClass &c, int kp, int resident, int waves) {
     int best = 0;
     double best_cost = 1e300;
     for (int L : kBatchSlices) {
         if ((long long)batch_max_slices(c, L) * kp > kMaxUnion) continue;
         double cost = 0.0;
-        // per-task work: a task's four waves share its slice's 128-ref steps
+        // per-task work: a task's waves share its slice's 128-ref steps
         long long tasks = 0, steps = 0;
         for (const knnhip_problem *p : c.probs) {
             const long long ns = (p->ref_nb + L - 1) / L, ng = (p->query_nb + 31) / 32;
@@ -78,7 +78,7 @@ int batch_default_slice(const BatchClass &c, int kp, int resident) {
             const long long span = std::min<long long>(L, p->ref_nb);
             steps += ns * ng * ((span + kRefsPerStep - 1) / kRefsPerStep);
         }
-        const double avg = (double)steps / std::max<long long>(1, tasks) / 4.0 + 3.0;
+        const double avg = (double)steps / std::max<long long>(1, tasks) / waves + 3.0;
         cost = std::ceil((double)tasks / resident) * avg;
         if (cost < best_cost) {
             best_cost = cost;
@@ -243,7 +243,7 @@ bool batch_class_run(const BatchClass &c, int L, const knnhip_options &o, hipStr
     sa.qlist = w.qlist.as<int>();
     sa.query_pm = w.qpm.as<float>();
     sa.query_pm_ld = pm_ld;
-    KNN_TRY(launch_scan(sa, pl.ks, pl.multi, 4, s), "scan kernel launch");
+    KNN_TRY(launch_scan(sa, pl.ks, pl.multi, pl.waves, s), "scan kernel launch");
     if (scan_ms) KNN_TRY(hipEventRecord(e1, s), "event");
     FinalArgs fa{};
     fa.R = (int)n_pos;
@@ -361,10 +361,8 @@ bool batch_args(const knnhip_problem *probs, int n) {
     return true;
 }
 
-int batch_resident() {
-    // two resident scan blocks per CU is the common case
-    return 2 * std::max(1, g_cu_count);
-}
+// Resident scan blocks of a class's plan.
+int batch_resident(const Plan &pl) { return std::max(1, pl.cus) * std::max(1, pl.blocks_per_cu); }
 
 }  // namespace
 }  // namespace knnhip
@@ -386,7 +384,7 @@ int knnhip_search_batched(const knnhip_problem *problems, int count, const knnhi
     for (const BatchClass &c : batch_classes(problems, count)) {
         const Plan pl = make_plan(1 << 20, kQueriesPerWave, c.dim, c.k, o);
         auto it = g_batch_table.find(BatchTuneKey{c.dim, c.k});
-        int L = it != g_batch_table.end() ? it->second : batch_default_slice(c, std::max(pl.kp, c.k), batch_resident());
+        int L = it != g_batch_table.end() ? it->second : batch_default_slice(c, std::max(pl.kp, c.k), batch_resident(pl), pl.waves);
         if (L == 0) L = kBatchSlices[sizeof(kBatchSlices) / sizeof(int) - 1];
         char buf[160];
         snprintf(buf, sizeof(buf), "%s{\"dim\": %d, \"k\": %d, \"problems\": %d, \"slice\": %d, \"tuned\": %s}",

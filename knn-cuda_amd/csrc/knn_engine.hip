@@ -62,12 +62,12 @@ void *ws_get(Slot s, size_t bytes) {
 namespace {
 
 // Cost model of one slice count: the scan runs in `rounds` waves of
-// resident blocks, each block's four waves sharing ceil(steps / S) reference
-// steps, plus a warm-up of about two steps while the heaps fill.
-double slice_cost(int S, int steps, int nqb, int resident) {
+// resident blocks, each block's `waves` waves sharing ceil(steps / S)
+// reference steps, plus a warm-up of about two steps while the heaps fill.
+double slice_cost(int S, int steps, int nqb, int resident, int waves) {
     const long long units = (long long)nqb * S;
     const long long rounds = (units + resident - 1) / resident;
-    return (double)rounds * ((double)((steps + S - 1) / S) / 4.0 + 2.0);
+    return (double)rounds * ((double)((steps + S - 1) / S) / waves + 2.0);
 }
 
 }  // namespace
@@ -114,14 +114,20 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     }
     p.cus = g_cu_count > 0 ? g_cu_count : 256;
     p.blocks_per_cu = std::max(1, scan_blocks_per_cu(p.ks, p.multi, p.waves, p.kp));
+    if (p.blocks_per_cu == 1 && scan_lds_bytes(p.kp, 8) <= kScanLdsMax) {
+        // the heaps leave LDS for one block per CU: eight waves on its queries
+        // keep two waves per SIMD
+        p.waves = 8;
+        p.blocks_per_cu = std::max(1, scan_blocks_per_cu(p.ks, p.multi, p.waves, p.kp));
+    }
     const int resident = p.cus * p.blocks_per_cu;
     int s = smin;
     if (o.slices > 0) {
         s = std::max(smin, std::min(o.slices, smax));
     } else {
-        double best = slice_cost(smin, steps, p.nqb, resident);
+        double best = slice_cost(smin, steps, p.nqb, resident, p.waves);
         for (int c = smin + 1; c <= smax; ++c) {
-            const double cost = slice_cost(c, steps, p.nqb, resident);
+            const double cost = slice_cost(c, steps, p.nqb, resident, p.waves);
             if (cost < best) {
                 best = cost;
                 s = c;
@@ -156,9 +162,9 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
             if (o.slices > 0) {
                 bs = std::max(bmin, std::min(o.slices, smax));
             } else {
-                double best = slice_cost(bs, steps, p.nqb, p.cus) ;
+                double best = slice_cost(bs, steps, p.nqb, p.cus, 4);
                 for (int c = bs + 1; c <= smax; ++c) {
-                    const double cost = slice_cost(c, steps, p.nqb, p.cus);
+                    const double cost = slice_cost(c, steps, p.nqb, p.cus, 4);
                     if (cost < best) {
                         best = cost;
                         bs = c;

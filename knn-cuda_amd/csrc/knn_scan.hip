@@ -44,8 +44,10 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 // steps w, w+W, w+2W, ...  All W waves feed one heap per query in LDS,
 // guarded by a per-query LDS lock, so every query's threshold is the kp-th
 // smallest key over everything the block has scanned so far.
-template <int KS, int NBUF, bool MULTI>
-__global__ __launch_bounds__(256, 2) void scan_f32_kernel(ScanArgs a) {
+// MAXW: most waves per block (4, or 8 where the heaps of large kp leave LDS
+// for one block per CU: eight waves then keep two per SIMD).
+template <int KS, int NBUF, bool MULTI, int MAXW>
+__global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int OCT = KS / 4;  // octets per chunk
     static_assert(KS % 4 == 0, "k-steps come in octets of 4");
@@ -373,9 +375,11 @@ size_t scan_lds_bytes(int kp, int waves) {
     X(96, 2, false)           \
     X(64, 2, true)
 
-static const void *scan_fn(int ks, bool multi) {
-#define X(K, N, M) \
-    if (ks == K && multi == M) return (const void *)scan_f32_kernel<K, N, M>;
+static const void *scan_fn(int ks, bool multi, int waves) {
+#define X(K, N, M)                                                                         \
+    if (ks == K && multi == M)                                                             \
+        return waves > 4 ? (const void *)scan_f32_kernel<K, N, M, 8>                        \
+                         : (const void *)scan_f32_kernel<K, N, M, 4>;
     KNN_SCAN_INSTANCES(X)
 #undef X
     return nullptr;
@@ -383,7 +387,8 @@ static const void *scan_fn(int ks, bool multi) {
 
 // Resident blocks per CU of one scan instance (VGPR and LDS limits).
 int scan_blocks_per_cu(int ks, bool multi, int waves, int kp) {
-    const void *fn = scan_fn(ks, multi);
+    if (waves != 4 && waves != 8 && waves != 1) return 0;
+    const void *fn = scan_fn(ks, multi, waves);
     if (!fn) return 0;
     const size_t lds = scan_lds_bytes(kp, waves);
     if (lds > 65536 &&
@@ -398,7 +403,8 @@ int scan_blocks_per_cu(int ks, bool multi, int waves, int kp) {
 }
 
 hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s) {
-    const void *fn = scan_fn(ks, multi);
+    if (waves < 1 || waves > 8) return hipErrorInvalidValue;
+    const void *fn = scan_fn(ks, multi, waves);
     if (!fn) return hipErrorInvalidValue;
     const size_t lds = scan_lds_bytes(a.kp, waves);
     if (lds > 65536) {
@@ -408,8 +414,12 @@ hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStre
     const int blocks = a.tasks ? a.ntasks : a.nqb * a.nslices;
 #define X(K, N, M)                                                                             \
     if (ks == K && multi == M) {                                                               \
-        hipLaunchKernelGGL((scan_f32_kernel<K, N, M>), dim3(blocks), dim3(waves * kWave), lds, \
-                           s, a);                                                              \
+        if (waves > 4)                                                                         \
+            hipLaunchKernelGGL((scan_f32_kernel<K, N, M, 8>), dim3(blocks), dim3(waves * kWave), \
+                               lds, s, a);                                                     \
+        else                                                                                   \
+            hipLaunchKernelGGL((scan_f32_kernel<K, N, M, 4>), dim3(blocks), dim3(waves * kWave), \
+                               lds, s, a);                                                     \
         return hipGetLastError();                                                              \
     }
     KNN_SCAN_INSTANCES(X)
