@@ -243,7 +243,7 @@ bool batch_class_run(const BatchClass &c, int L, const knnhip_options &o, hipStr
     sa.qlist = w.qlist.as<int>();
     sa.query_pm = w.qpm.as<float>();
     sa.query_pm_ld = pm_ld;
-    KNN_TRY(launch_scan(sa, pl.ks, pl.multi, pl.waves, s), "scan kernel launch");
+    KNN_TRY(launch_scan(sa, pl.ks, pl.bmode, pl.waves, s), "scan kernel launch");
     if (scan_ms) KNN_TRY(hipEventRecord(e1, s), "event");
     FinalArgs fa{};
     fa.R = (int)n_pos;

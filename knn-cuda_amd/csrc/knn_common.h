@@ -16,6 +16,12 @@ constexpr int kQueriesPerWave = 32;  // one MFMA 32-column block of queries per 
 constexpr int kRefsPerTile = 32;     // one MFMA 32-row block of references
 constexpr int kMTiles = 4;           // reference tiles per scan step (128 refs)
 constexpr int kRefsPerStep = kRefsPerTile * kMTiles;
+// Where the FP32 scan keeps the -2q operands: kBRegs, the padded dimension's
+// values in VGPRs (one chunk of 2*KS dims); kBChunks, chunks of 2*KS dims
+// reloaded into VGPRs chunk by chunk (any dimension); kBLds, the block's 32
+// query rows staged in LDS once and read 16 bytes per dimension octet, every
+// octet in one pass.
+constexpr int kBRegs = 0, kBChunks = 1, kBLds = 2;
 constexpr int kMaxUnion = 2048;      // max slices*kp entries merged per query
 constexpr int kMaxCand = 1024;       // max candidates re-ranked exactly per query
 

@@ -46,8 +46,9 @@ struct DevBuf {
 // Launchers defined in knn_scan.hip / knn_scan_bf16.hip / knn_select.hip.
 int scan_pick_ks(int dim);
 size_t scan_lds_bytes(int kp, int waves);
-int scan_blocks_per_cu(int ks, bool multi, int waves, int kp);
-hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s);
+size_t scan_block_lds(int kp, int waves, int bmode, int oct);
+int scan_blocks_per_cu(int ks, int bmode, int waves, int kp, int oct);
+hipError_t launch_scan(const ScanArgs &a, int ks, int bmode, int waves, hipStream_t s);
 int scan_bf3_nch(int dim);
 size_t scan_bf3_lds_bytes(int kp);
 hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s);
@@ -105,7 +106,7 @@ void *ws_get(Slot s, size_t bytes);
 struct Plan {
     bool use_scan = false;  // false: exact full scan for every query
     int ks = 0;
-    bool multi = false;
+    int bmode = kBRegs;       // where the scan keeps the -2q operands
     int nchunks = 1;
     int dpad = 0;
     int waves = 4;
@@ -184,7 +185,7 @@ struct FileIn {
 // The opaque index handle of the C API.
 struct knnhip_index {
     int R = 0, dim = 0, nb = 0, pm_ld = 0, oct = 0, dpad = 0, ks = 0, nchunks = 1;
-    bool multi = false;
+    int bmode = knnhip::kBRegs;
     int n_pos = 0, r_pad = 0;
     const float *ref = nullptr;  // caller's dim-major references (exact fallback)
     long long ldr = 0;

@@ -84,13 +84,24 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     p.ks = scan_pick_ks(dim);
     if (p.ks) {
         p.nchunks = 1;
-        p.multi = false;
+        p.bmode = kBRegs;
+        p.dpad = 2 * p.ks;
+    } else if (scan_block_lds(p.kp, 4, kBLds, (dim + 127) / 128 * 16) <= kScanLdsMax) {
+        // wider dimensions: the block's query rows staged in LDS.  Padded to
+        // 128 dimensions like the chunked layout, so that a search with a
+        // larger kp over the same packed tiles can take the chunked instance.
+        p.ks = 16;
+        p.bmode = kBLds;
+        p.nchunks = (dim + 127) / 128;
+        p.dpad = 128 * p.nchunks;
     } else {
+        // ... or, where they do not fit beside the heaps, reloaded into VGPRs
+        // 128 dimensions at a time
         p.ks = 64;
-        p.multi = true;
+        p.bmode = kBChunks;
         p.nchunks = (dim + 2 * p.ks - 1) / (2 * p.ks);
+        p.dpad = 2 * p.ks * p.nchunks;
     }
-    p.dpad = 2 * p.ks * p.nchunks;
     p.oct = p.dpad / 8;
     p.nqb = (Q + kQueriesPerWave - 1) / kQueriesPerWave;
     // Slices: the count whose blocks fill whole rounds of resident blocks
@@ -113,12 +124,12 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
             (void)hipGetLastError();
     }
     p.cus = g_cu_count > 0 ? g_cu_count : 256;
-    p.blocks_per_cu = std::max(1, scan_blocks_per_cu(p.ks, p.multi, p.waves, p.kp));
-    if (p.blocks_per_cu == 1 && scan_lds_bytes(p.kp, 8) <= kScanLdsMax) {
-        // the heaps leave LDS for one block per CU: eight waves on its queries
+    p.blocks_per_cu = std::max(1, scan_blocks_per_cu(p.ks, p.bmode, p.waves, p.kp, p.oct));
+    if (p.blocks_per_cu == 1 && scan_block_lds(p.kp, 8, p.bmode, p.oct) <= kScanLdsMax) {
+        // LDS leaves room for one block per CU: eight waves on its queries
         // keep two waves per SIMD
         p.waves = 8;
-        p.blocks_per_cu = std::max(1, scan_blocks_per_cu(p.ks, p.multi, p.waves, p.kp));
+        p.blocks_per_cu = std::max(1, scan_blocks_per_cu(p.ks, p.bmode, p.waves, p.kp, p.oct));
     }
     const int resident = p.cus * p.blocks_per_cu;
     int s = smin;
@@ -185,10 +196,12 @@ std::string plan_json(const Plan &p, int R, int Q, int dim, int k) {
     char buf[512];
     snprintf(buf, sizeof(buf),
              "{\"ref_nb\": %d, \"query_nb\": %d, \"dim\": %d, \"k\": %d, \"scan\": %s, "
-             "\"ks\": %d, \"chunks\": %d, \"dpad\": %d, \"waves_per_block\": %d, \"kp\": %d, "
+             "\"ks\": %d, \"b_operands\": \"%s\", \"chunks\": %d, \"dpad\": %d, \"waves_per_block\": %d, \"kp\": %d, "
              "\"slices\": %d, \"slice_len\": %d, \"query_blocks\": %d, \"xcd_map\": %d, "
              "\"blocks_per_cu\": %d, \"cus\": %d, \"max_candidates\": %d, \"precision\": \"%s\", \"block_queries\": %d}",
-             R, Q, dim, k, p.use_scan ? "true" : "false", p.ks, p.nchunks, p.dpad, p.waves, p.kp,
+             R, Q, dim, k, p.use_scan ? "true" : "false", p.ks,
+             p.bmode == kBLds ? "lds" : p.bmode == kBChunks ? "chunks" : "registers", p.nchunks, p.dpad,
+             p.waves, p.kp,
              p.nslices, p.slice_len, p.nqb, p.xcd, p.blocks_per_cu, p.cus, p.maxc,
              p.bf3 ? "bf16x3" : "fp32", p.lds ? 128 : 32);
     return buf;
@@ -369,7 +382,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         if (p.lds)
             KNN_TRY(launch_scan_lds(sa, p.nch, p.bf3, s), "scan kernel launch");
         else
-            KNN_TRY(launch_scan(sa, p.ks, p.multi, p.waves, s), "scan kernel launch");
+            KNN_TRY(launch_scan(sa, p.ks, p.bmode, p.waves, s), "scan kernel launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[2], s), "event");
 
         FinalArgs fa{};

@@ -103,7 +103,7 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
             return false;
         }
         ix->ks = p.ks;
-        ix->multi = p.multi;
+        ix->bmode = p.bmode;
         ix->nchunks = p.nchunks;
         ix->dpad = p.dpad;
         ix->oct = p.oct;
@@ -309,7 +309,7 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
         return false;
     }
     ix->ks = p.ks;
-    ix->multi = p.multi;
+    ix->bmode = p.bmode;
     ix->nchunks = p.nchunks;
     ix->dpad = p.dpad;
     ix->oct = p.oct;
@@ -387,7 +387,14 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                        : nch_bf         ? kIndexMarginBf3
                                         : kIndexMargin;
     const int kp = std::max(k, std::min(k + margin, ix->R));
-    if (scan_lds_bytes(kp, 4) > kScanLdsMax || (long long)P * kp > kMaxUnion) {
+    // wide dimensions: query rows staged in LDS where they fit beside this
+    // search's heaps, else the chunked instance over the same tiles
+    int bmode = ix->bmode, ks = ix->ks;
+    if (bmode == kBLds && scan_block_lds(kp, 4, kBLds, ix->oct) > kScanLdsMax) {
+        bmode = kBChunks;
+        ks = 64;
+    }
+    if (scan_block_lds(kp, 4, bmode, ix->oct) > kScanLdsMax || (long long)P * kp > kMaxUnion) {
         set_error("k or probe count too large for the block index (k=%d probes=%d)", k, P);
         return false;
     }
@@ -514,7 +521,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
             KNN_TRY(launch_scan_bf3_task(sa, nch_bf, waves, scan_lds_bytes(kp, waves), s),
                     "scan kernel launch");
         } else {
-            KNN_TRY(launch_scan(sa, ix->ks, ix->multi, waves, s), "scan kernel launch");
+            KNN_TRY(launch_scan(sa, ks, bmode, waves, s), "scan kernel launch");
         }
         tbase += max_t[ph];
     }

@@ -44,12 +44,15 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 // steps w, w+W, w+2W, ...  All W waves feed one heap per query in LDS,
 // guarded by a per-query LDS lock, so every query's threshold is the kp-th
 // smallest key over everything the block has scanned so far.
+// BMODE: where the B operands live (kBRegs, kBChunks, kBLds).
 // MAXW: most waves per block (4, or 8 where the heaps of large kp leave LDS
 // for one block per CU: eight waves then keep two per SIMD).
-template <int KS, int NBUF, bool MULTI, int MAXW>
+template <int KS, int NBUF, int BMODE, int MAXW>
 __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int OCT = KS / 4;  // octets per chunk
+    constexpr bool MULTI = BMODE == kBChunks;
+    constexpr bool BLDS = BMODE == kBLds;
     static_assert(KS % 4 == 0, "k-steps come in octets of 4");
     static_assert(OCT % NBUF == 0, "ring position must repeat every chunk");
 
@@ -109,6 +112,33 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
         hi[e] = -1;
     }
     if (threadIdx.x < kQueriesPerWave) lock[threadIdx.x] = 0;
+    // kBLds: the block's 32 query rows (-2q, zero past dim and past the
+    // task's queries) after the heaps and queues, row pitch 8 oct + 4 floats
+    float *bl = nullptr;
+    const int bpitch = 8 * a.oct + 4;
+    if constexpr (BLDS) {
+        bl = reinterpret_cast<float *>(smem + scan_heap_bytes(kp) + (size_t)waves * kQueueBytes);
+        const int dpad = 8 * a.oct;
+        if (a.query_pm) {
+            // point-major rows: consecutive threads take consecutive dims of a row
+            for (int e = threadIdx.x; e < kQueriesPerWave * dpad; e += blockDim.x) {
+                const int qi = e / dpad, d = e - qi * dpad;
+                float v = 0.0f;
+                if (qi < nq && d < a.dim)
+                    v = -2.0f * a.query_pm[(long long)(qids ? qids[qi] : qfirst + qi) * a.query_pm_ld + d];
+                bl[qi * bpitch + d] = v;
+            }
+        } else {
+            // dim-major columns: consecutive threads take consecutive queries of a dim
+            for (int e = threadIdx.x; e < kQueriesPerWave * dpad; e += blockDim.x) {
+                const int d = e / kQueriesPerWave, qi = e - d * kQueriesPerWave;
+                float v = 0.0f;
+                if (qi < nq && d < a.dim)
+                    v = -2.0f * a.query[(long long)d * a.ldq + (qids ? qids[qi] : qfirst + qi)];
+                bl[qi * bpitch + d] = v;
+            }
+        }
+    }
     __syncthreads();
     float *myv = hv + col * kps;
     float theta = INFINITY;      // threshold in use (<= the shared heap root)
@@ -185,7 +215,7 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
             bq[kk] = d < a.dim ? -2.0f * v : 0.0f;
         }
     };
-    if (!MULTI) load_b(0);
+    if (BMODE == kBRegs) load_b(0);
     const int nchunks = MULTI ? a.nchunks : 1;
 
     const int wstride = waves * kRefsPerStep;  // a wave's distance between steps
@@ -235,7 +265,39 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
         const int r_next = r0 + wstride;
         unsigned voff_next[kMTiles];
         lane_offsets(r_next, voff_next);
-        for (int c = 0; c < nchunks; ++c) {
+        if constexpr (BLDS) {
+            // every octet of the padded dimension (a multiple of NBUF octets) in
+            // one pass; octet o sits in ring slot o % NBUF, and its B operands
+            // are read from LDS one octet ahead
+            const int octs = a.oct;
+            float4 bcur = *reinterpret_cast<const float4 *>(bl + col * bpitch + 4 * h);
+            for (int o0 = 0; o0 < octs; o0 += NBUF) {
+#pragma unroll
+                for (int j = 0; j < NBUF; ++j) {
+                    const int o = o0 + j;
+                    const int oi = o + NBUF - 1;
+                    if (oi < octs)
+                        issue(abuf[(j + NBUF - 1) % NBUF], oi, voff);
+                    else
+                        issue(abuf[(j + NBUF - 1) % NBUF], oi - octs, voff_next);
+                    const int on = o + 1 < octs ? o + 1 : 0;
+                    const float4 bnext =
+                        *reinterpret_cast<const float4 *>(bl + col * bpitch + 8 * on + 4 * h);
+                    __builtin_amdgcn_sched_barrier(0);
+                    const float bv[4] = {bcur.x, bcur.y, bcur.z, bcur.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                        for (int mt = 0; mt < kMTiles; ++mt)
+                            acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                                __uint_as_float(abuf[j][mt][u]), bv[u], acc[mt], 0, 0, 0);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                    bcur = bnext;
+                }
+            }
+        }
+        for (int c = 0; c < (BLDS ? 0 : nchunks); ++c) {
             if (MULTI) {
                 load_b(c);
 #pragma unroll
@@ -364,20 +426,28 @@ size_t scan_lds_bytes(int kp, int waves) {
     return scan_heap_bytes(kp) + (size_t)waves * kQueueBytes;
 }
 
-// Ring depth per instance: 4 octets when the chunk's octet count allows it.
-#define KNN_SCAN_INSTANCES(X) \
-    X(8, 2, false)            \
-    X(16, 4, false)           \
-    X(24, 2, false)           \
-    X(32, 4, false)           \
-    X(48, 4, false)           \
-    X(64, 4, false)           \
-    X(96, 2, false)           \
-    X(64, 2, true)
+// The same, with the staged query rows of the kBLds instance (oct octets).
+size_t scan_block_lds(int kp, int waves, int bmode, int oct) {
+    return scan_lds_bytes(kp, waves) +
+           (bmode == kBLds ? (size_t)kQueriesPerWave * (8 * oct + 4) * sizeof(float) : 0);
+}
 
-static const void *scan_fn(int ks, bool multi, int waves) {
+// Ring depth per instance: 4 octets when the chunk's octet count allows it.
+// The kBLds instance's KS only sets its ring (any multiple of 4 octets).
+#define KNN_SCAN_INSTANCES(X) \
+    X(8, 2, kBRegs)           \
+    X(16, 4, kBRegs)          \
+    X(24, 2, kBRegs)          \
+    X(32, 4, kBRegs)          \
+    X(48, 4, kBRegs)          \
+    X(64, 4, kBRegs)          \
+    X(96, 2, kBRegs)          \
+    X(64, 2, kBChunks)        \
+    X(16, 4, kBLds)
+
+static const void *scan_fn(int ks, int bmode, int waves) {
 #define X(K, N, M)                                                                         \
-    if (ks == K && multi == M)                                                             \
+    if (ks == K && bmode == M)                                                             \
         return waves > 4 ? (const void *)scan_f32_kernel<K, N, M, 8>                        \
                          : (const void *)scan_f32_kernel<K, N, M, 4>;
     KNN_SCAN_INSTANCES(X)
@@ -386,11 +456,11 @@ static const void *scan_fn(int ks, bool multi, int waves) {
 }
 
 // Resident blocks per CU of one scan instance (VGPR and LDS limits).
-int scan_blocks_per_cu(int ks, bool multi, int waves, int kp) {
+int scan_blocks_per_cu(int ks, int bmode, int waves, int kp, int oct) {
     if (waves != 4 && waves != 8 && waves != 1) return 0;
-    const void *fn = scan_fn(ks, multi, waves);
+    const void *fn = scan_fn(ks, bmode, waves);
     if (!fn) return 0;
-    const size_t lds = scan_lds_bytes(kp, waves);
+    const size_t lds = scan_block_lds(kp, waves, bmode, oct);
     if (lds > 65536 &&
         hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return 0;
@@ -402,18 +472,19 @@ int scan_blocks_per_cu(int ks, bool multi, int waves, int kp) {
     return n;
 }
 
-hipError_t launch_scan(const ScanArgs &a, int ks, bool multi, int waves, hipStream_t s) {
+hipError_t launch_scan(const ScanArgs &a, int ks, int bmode, int waves, hipStream_t s) {
     if (waves < 1 || waves > 8) return hipErrorInvalidValue;
-    const void *fn = scan_fn(ks, multi, waves);
+    if (bmode == kBLds && a.oct % 4) return hipErrorInvalidValue;  // whole rings of 4 octets
+    const void *fn = scan_fn(ks, bmode, waves);
     if (!fn) return hipErrorInvalidValue;
-    const size_t lds = scan_lds_bytes(a.kp, waves);
+    const size_t lds = scan_block_lds(a.kp, waves, bmode, a.oct);
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const int blocks = a.tasks ? a.ntasks : a.nqb * a.nslices;
 #define X(K, N, M)                                                                             \
-    if (ks == K && multi == M) {                                                               \
+    if (ks == K && bmode == M) {                                                               \
         if (waves > 4)                                                                         \
             hipLaunchKernelGGL((scan_f32_kernel<K, N, M, 8>), dim3(blocks), dim3(waves * kWave), \
                                lds, s, a);                                                     \
