@@ -21,6 +21,8 @@ plain FP32 search against the block-index search on the same data, both exact
 (the outputs are compared for equality), and the learned MLP router.
 `batched_form` times the 512-problem batched workload in one call against
 the same problems searched one call at a time (outputs compared for equality).
+`reference_form` runs the reference's own benchmark: its three host-pointer
+entry points on its test configuration, per call, beside its published times.
 """
 from __future__ import annotations
 
@@ -55,6 +57,8 @@ def parse():
     ap.add_argument("--no-router-form", action="store_true",
                     help="skip the router-form measurement (4M clustered refs, block index)")
     ap.add_argument("--router-reps", type=int, default=5)
+    ap.add_argument("--no-reference-form", action="store_true",
+                    help="skip the reference's own 16K x 4K host-pointer benchmark")
     ap.add_argument("--no-batched-form", action="store_true",
                     help="skip the batched-form measurement (512 problems, tests/golden/c4_problems.json)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -196,6 +200,72 @@ def router_form(args, knnhip, torch):
                            "searches": mlp_rows}}
 
 
+def reference_form(knnhip, np):
+    """The reference's own benchmark (code/test.cpp:295-298, 344-347; README.md:61-85): 16,384 refs,
+    4,096 queries, dim 128, k 16, data 10*U[0,1) as host arrays, each of the three drop-in entry
+    points timed per call host to host (H2D, search, D2H), next to the published per-call times
+    (NVIDIA Titan X, README.md:76-78).  All three outputs are compared for equality."""
+    import time
+
+    R, Q, D, k = 16384, 4096, 128, 16
+    rng = np.random.default_rng(0x6B4E4E01)
+    ref = (10.0 * rng.random((D, R), dtype=np.float32)).astype(np.float32)
+    qry = (10.0 * rng.random((D, Q), dtype=np.float32)).astype(np.float32)
+    published = {"knn_cuda_global": 41.66, "knn_cuda_texture": 62.96, "knn_cublas": 31.12}
+    calls = {"knn_cuda_global": knnhip.knn_cuda_global, "knn_cuda_texture": knnhip.knn_cuda_texture,
+             "knn_cublas": knnhip.knn_cublas}
+    rows, outs = {}, []
+    for name, fn in calls.items():
+        fn(ref, qry, k)
+        n = 20
+        t = time.perf_counter()
+        for _ in range(n):
+            d, i = fn(ref, qry, k)
+        ms = (time.perf_counter() - t) / n * 1e3
+        outs.append((d, i))
+        rows[name] = {"ms_per_call": ms, "queries_per_s": Q / ms * 1e3,
+                      "published_ms_per_call": published[name],
+                      "speedup_vs_published": published[name] / ms}
+    equal = all(np.array_equal(o[0].view(np.uint32), outs[0][0].view(np.uint32)) and
+                np.array_equal(o[1], outs[0][1]) for o in outs)
+    # the same entry point at the plain workload's size from host arrays: the references go up in
+    # chunks, each scanned as it lands; against one copy of them followed by the device search
+    import torch
+
+    R2, Q2, D2, k2 = 1 << 20, 16384, 128, 32
+    ref2 = rng.random((D2, R2), dtype=np.float32)
+    qry2 = rng.random((D2, Q2), dtype=np.float32)
+    knnhip.knn_cuda_global(ref2, qry2, k2)
+    t = time.perf_counter()
+    for _ in range(3):
+        d2, i2 = knnhip.knn_cuda_global(ref2, qry2, k2)
+    host_ms = (time.perf_counter() - t) / 3 * 1e3
+    chunks = knnhip.last_plan().get("upload_chunks", 1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ref_d = torch.from_numpy(ref2).cuda()
+    qry_d = torch.from_numpy(qry2).cuda()
+    torch.cuda.synchronize()
+    copy_ms = (time.perf_counter() - t) * 1e3
+    knnhip.search_device(ref_d, qry_d, k2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    dd, di = knnhip.search_device(ref_d, qry_d, k2)
+    torch.cuda.synchronize()
+    dev_ms = (time.perf_counter() - t) * 1e3
+    same = bool(np.array_equal(i2, di.cpu().numpy()) and np.array_equal(d2.view(np.uint32),
+                                                                        dd.cpu().numpy().view(np.uint32)))
+    return {"workload": "16384 refs x 4096 queries, dim 128, k 16, 10*U[0,1) host arrays (the reference's "
+                        "test.cpp configuration), per call host to host",
+            "published_hardware": "NVIDIA Titan X (README.md:76-78)", "calls": rows,
+            "outputs_equal": bool(equal),
+            "host_arrays_at_scale": {"workload": "1048576 x 16384, dim 128, k 32, U[0,1) host arrays",
+                                     "knn_cuda_global_ms": host_ms, "upload_chunks": chunks,
+                                     "h2d_copy_ms": copy_ms, "device_search_ms": dev_ms,
+                                     "copy_then_search_ms": copy_ms + dev_ms,
+                                     "equal_to_device_search": same}}
+
+
 def batched_form(knnhip, torch):
     """Batched form: the 512 problems of tests/golden/c4_problems.json (R log-uniform in [1K, 256K],
     Q in [16, 4096], dim in {32..512}, k in {1..128}, uniform data from the listed seeds), all
@@ -332,6 +402,7 @@ def main():
                                                                       dist, idx, ms_per_step, scan_avg)
     router = None if args.no_router_form else router_form(args, knnhip, torch)
     batched = None if args.no_batched_form else batched_form(knnhip, torch)
+    reference = None if args.no_reference_form else reference_form(knnhip, np)
 
     out = {
         "metric": METRIC,
@@ -361,6 +432,7 @@ def main():
         "precision_form": precision,
         "router_form": router,
         "batched_form": batched,
+        "reference_form": reference,
     }
     print(json.dumps(out), flush=True)
 

@@ -58,6 +58,8 @@ struct ScanArgs {
     const ScanTask *tasks;  // explicit tasks (nullptr: the plain slice x query-block grid)
     int ntasks;           // one block per task
     const int *qlist;     // query ids referenced by the tasks
+    int slice0;           // plain searches: first slice of this launch (a chunked
+                          // upload scans each chunk's slices as it lands)
     const unsigned short *packed_bf;  // bfloat16 hi/lo tiles (precision scan), else unused
     // Point-major query rows (row stride query_pm_ld, a multiple of 4, pads
     // zero), or nullptr.  Task-mode searches pass them: a lane then reads its

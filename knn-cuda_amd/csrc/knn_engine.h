@@ -98,6 +98,8 @@ struct Workspace {
     size_t cap[kNumSlots] = {};
     hipEvent_t ev[5] = {};
     bool have_events = false;
+    hipEvent_t up_ev[8] = {};    // one per upload chunk
+    hipStream_t copy = nullptr;  // upload stream
 };
 extern Workspace g_ws;
 void *ws_get(Slot s, size_t bytes);
@@ -142,11 +144,25 @@ void bf3_error_bound(int dim, int nch, double *err_q, double *err_r);
 // Prints the last error on stdout as the reference's entry points do.
 bool report(bool ok);
 
+// Host-pointer references uploaded in chunks of whole slices on a copy
+// stream; each chunk's norms, packing and slices are queued on the search
+// stream as soon as its copy has landed.
+struct Upload {
+    const float *ref;    // host, dim rows of ld floats
+    long long ld;
+    hipStream_t stream;  // copy stream (non-blocking)
+    int chunks;          // the plan's slices are grouped into this many chunks
+};
+constexpr int kUploadChunks = 8;
+constexpr int kUploadMinRefs = 1 << 18;  // smaller sets are uploaded in one copy
+
 // Search on device pointers: dim rows of ldr / ldq floats; results k x Q.
 // level > 0: an escalation pass over queries that failed the certificate.
+// up: the references are still on the host (ref is their device buffer,
+// filled chunk by chunk); FP32 32-query plans only.
 bool run_device(const float *ref, long long ldr, int R, const float *query, long long ldq, int Q,
                 int dim, int k, float *out_dist, int *out_idx, const Plan &p,
-                const knnhip_options &o, hipStream_t s, int level = 0);
+                const knnhip_options &o, hipStream_t s, int level = 0, const Upload *up = nullptr);
 
 // Whole-file readers and writers of the index and router file formats:
 // `ok` turns false at the first short read or write and stays false.

@@ -267,7 +267,11 @@ Escalation g_esc[kMaxEscalation];
 // Core: device pointers, dim rows of ldr / ldq floats.
 bool run_device(const float *ref, long long ldr, int R, const float *query, long long ldq, int Q,
                 int dim, int k, float *out_dist, int *out_idx, const Plan &p,
-                const knnhip_options &o, hipStream_t s, int level) {
+                const knnhip_options &o, hipStream_t s, int level, const Upload *up) {
+    if (up && (!p.use_scan || p.lds || p.bf3 || up->chunks > kUploadChunks)) {
+        set_error("chunked upload needs an FP32 32-query scan plan");
+        return false;
+    }
     const bool timing = o.timing != 0;
     if (timing && !g_ws.have_events) {
         for (auto &e : g_ws.ev) KNN_TRY(hipEventCreate(&e), "event create");
@@ -344,7 +348,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
             na.nch_bf = p.nch;
         } else
             na.packed = (float *)packed;
-        KNN_TRY(launch_norms(na, s), "norm kernel launch");
+        if (!up) KNN_TRY(launch_norms(na, s), "norm kernel launch");
         NormArgs nq{};
         nq.p = query;
         nq.ld = ldq;
@@ -379,10 +383,41 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         sa.xcd_map = p.xcd;
         sa.part_val = pval;
         sa.part_idx = pidx;
-        if (p.lds)
+        if (up) {
+            // chunk g: slices [s0, s1), references [r0, r1); its copy lands on
+            // the upload stream while the search stream works on chunk g - 1
+            const int G = std::min(up->chunks, p.nslices);
+            for (int g = 0; g < G; ++g) {
+                const int s0 = (int)((long long)p.nslices * g / G);
+                const int s1 = (int)((long long)p.nslices * (g + 1) / G);
+                const int r0 = s0 * p.slice_len, r1 = std::min(R, s1 * p.slice_len);
+                if (s1 <= s0 || r1 <= r0) continue;
+                KNN_TRY(hipMemcpy2DAsync(const_cast<float *>(ref) + r0, (size_t)ldr * sizeof(float),
+                                         up->ref + r0, (size_t)up->ld * sizeof(float),
+                                         (size_t)(r1 - r0) * sizeof(float), dim, hipMemcpyHostToDevice,
+                                         up->stream),
+                        "Unable to copy data from host to device");
+                KNN_TRY(hipEventRecord(g_ws.up_ev[g], up->stream), "event");
+                KNN_TRY(hipStreamWaitEvent(s, g_ws.up_ev[g], 0), "event wait");
+                NormArgs nc = na;
+                nc.p = ref + r0;
+                nc.n = r1 - r0;
+                nc.n_pad = (s1 == p.nslices ? p.r_pad : s1 * p.slice_len) - r0;
+                nc.out = rnorm + r0;
+                nc.pm = ref_pm + (size_t)r0 * pm_ld;
+                nc.packed = (float *)packed + (size_t)(r0 / kRefsPerTile) * p.oct * kWave * 4;
+                KNN_TRY(launch_norms(nc, s), "norm kernel launch");
+                ScanArgs sc = sa;
+                sc.slice0 = s0;
+                sc.nslices = s1 - s0;
+                sc.xcd_map = 0;
+                KNN_TRY(launch_scan(sc, p.ks, p.bmode, p.waves, s), "scan kernel launch");
+            }
+        } else if (p.lds) {
             KNN_TRY(launch_scan_lds(sa, p.nch, p.bf3, s), "scan kernel launch");
-        else
+        } else {
             KNN_TRY(launch_scan(sa, p.ks, p.bmode, p.waves, s), "scan kernel launch");
+        }
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[2], s), "event");
 
         FinalArgs fa{};
@@ -494,8 +529,27 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
         return false;
     if (!ensure_device()) return false;
     const knnhip_options o = default_options(variant);
-    const Plan p = make_plan(R, Q, dim, k, o);
+    Plan p = make_plan(R, Q, dim, k, o);
+    // Large reference sets go up in chunks of whole slices, each scanned as
+    // soon as it lands, so the copy overlaps the search.
+    bool chunked = R >= kUploadMinRefs && p.use_scan && !p.lds && !p.bf3;
+    if (chunked) {
+        knnhip_options oc = o;
+        oc.slices = std::max(p.nslices, kUploadChunks);
+        p = make_plan(R, Q, dim, k, oc);
+        chunked = p.use_scan && !p.lds && !p.bf3 && p.nslices >= 2;
+    }
+    if (chunked) {
+        if (!g_ws.copy) {
+            KNN_TRY(hipStreamCreateWithFlags(&g_ws.copy, hipStreamNonBlocking), "stream create");
+            for (auto &e : g_ws.up_ev) KNN_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
+        }
+    }
     g_last_plan = plan_json(p, R, Q, dim, k);
+    if (chunked) {
+        g_last_plan.pop_back();
+        g_last_plan += ", \"upload_chunks\": " + std::to_string(std::min(kUploadChunks, p.nslices)) + "}";
+    }
     float *ref_d = (float *)ws_get(kRefPad, (size_t)dim * R * sizeof(float));
     float *qry_d = (float *)ws_get(kQryPad, (size_t)dim * Q * sizeof(float));
     float *dist_d = (float *)ws_get(kOutDist, (size_t)k * Q * sizeof(float));
@@ -505,12 +559,17 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
         return false;
     }
     hipStream_t s = nullptr;
-    KNN_TRY(hipMemcpyAsync(ref_d, ref, (size_t)dim * R * sizeof(float), hipMemcpyHostToDevice, s),
-            "Unable to copy data from host to device");
     KNN_TRY(hipMemcpyAsync(qry_d, query, (size_t)dim * Q * sizeof(float), hipMemcpyHostToDevice,
                            s),
             "Unable to copy data from host to device");
-    if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s)) return false;
+    if (chunked) {
+        const Upload up{ref, R, g_ws.copy, kUploadChunks};
+        if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s, 0, &up)) return false;
+    } else {
+        KNN_TRY(hipMemcpyAsync(ref_d, ref, (size_t)dim * R * sizeof(float), hipMemcpyHostToDevice, s),
+                "Unable to copy data from host to device");
+        if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s)) return false;
+    }
     if (kth_only) {
         KNN_TRY(hipMemcpyAsync(knn_dist, dist_d + (size_t)(k - 1) * Q, (size_t)Q * sizeof(float),
                                hipMemcpyDeviceToHost, s),

@@ -82,9 +82,9 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
             // one slice the same XCD so the slice is read from HBM once per L2.
             const int xcd = t & 7, j = t >> 3;
             qb = j % a.nqb;
-            slice = (j / a.nqb) * 8 + xcd;
+            slice = a.slice0 + (j / a.nqb) * 8 + xcd;
         } else {
-            slice = t / a.nqb;
+            slice = a.slice0 + t / a.nqb;
             qb = t % a.nqb;
         }
         qfirst = qb * kQueriesPerWave;

@@ -209,3 +209,28 @@ def test_dense_clusters_escalate_margin_not_exact_scan():
         torch.cuda.synchronize()
         assert knnhip.last_fallback_count() == 0, f"margin {margin}: exact-scan queries"
         assert_exact(d, i, gd, gi, f"margin {margin}")
+
+
+@pytest.mark.parametrize("entry", ["knn_cuda_global", "kth_distance"])
+def test_host_entry_chunked_upload(entry):
+    # 300,001 refs (not a multiple of the 128-point step): the host entry points upload the
+    # references in chunks of whole slices, each scanned as it lands; the result equals the
+    # device-resident search bit for bit and the oracle on a sample
+    R, Q, D, k = 300001, 200, 64, 8
+    say(f"chunked upload {entry} {R}x{Q}x{D} k={k}")
+    rng = np.random.default_rng(300001)
+    ref = rng.random((D, R), dtype=np.float32)
+    qry = rng.random((D, Q), dtype=np.float32)
+    dd, di = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k)
+    dd, di = dd.cpu().numpy(), di.cpu().numpy()
+    if entry == "kth_distance":
+        kth = knnhip.kth_distance(ref, qry, k)
+        assert knnhip.last_plan().get("upload_chunks", 0) >= 2
+        assert np.array_equal(bits(kth), bits(dd[k - 1]))
+        return
+    d, i = knnhip.knn_cuda_global(ref, qry, k)
+    assert knnhip.last_plan().get("upload_chunks", 0) >= 2
+    assert np.array_equal(i, di) and np.array_equal(bits(d), bits(dd))
+    sample = [0, 99, 199]
+    gd, gi = oracle.knn_c(ref, np.ascontiguousarray(qry[:, sample]), k)
+    assert np.array_equal(i[:, sample], gi) and np.array_equal(bits(d[:, sample]), bits(gd))
