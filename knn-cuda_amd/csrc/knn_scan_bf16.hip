@@ -15,11 +15,14 @@
 // and w+4 share query set w%4 (its heaps, under the per-query LDS locks)
 // and take tiles {0,1} / {2,3} of every 128-reference step.  Each 16-dim
 // chunk of a step (4 tiles x hi/lo x 64 lanes x 16 bytes = 8 KiB) is loaded
-// once per block, one 16-byte load per thread, into a three-slot LDS ring,
+// once per block, one 16-byte load per thread, into a four-slot LDS ring,
 // so reference traffic is a quarter of the FP32 scan's (32 queries per
 // block).  Global loads run PF chunks ahead in registers; each wave reads a
 // chunk's operands from LDS one chunk ahead of its MFMAs; one barrier per
-// chunk hands slots between the loading threads and the computing waves.
+// two chunks hands slots between the loading threads and the computing
+// waves (chunk g + 3 is stored during chunk g: with a barrier after every
+// odd chunk, a chunk read one ahead was stored before the last barrier, and
+// a slot is rewritten only after every wave's last read of it passed one).
 #include "knn_common.h"
 #include "knn_heap.h"
 
@@ -34,7 +37,7 @@ constexpr int kBfQueries = 128;
 constexpr int kBfQueueCap = 384;  // >= 4 keys x 64 lanes between drain checks, plus slack
 constexpr size_t kBfQueueBytes = (size_t)kBfQueueCap * 3 * sizeof(float);
 constexpr int kBfChunkBytes = 4 * 2 * kWave * 16;  // one 16-dim chunk of a 128-reference step
-constexpr int kBfRing = 3;                         // LDS slots
+constexpr int kBfRing = 4;                         // LDS slots
 
 __host__ __device__ constexpr size_t bf_heap_bytes(int kp) {
     return (((size_t)kBfQueries * heap_stride(kp) * 2 + kBfQueries) * sizeof(float) + 15) &
@@ -73,6 +76,7 @@ template <int NCH, int PF, bool BF3>
 __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     static_assert(NCH % PF == 0, "ring position must repeat every step");
+    static_assert(NCH % 2 == 0, "one barrier per two chunks, ending every step");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
     const int qset = wave & 3, half = wave >> 2;
@@ -182,18 +186,20 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
         const long long off = st * 4 * tile_bytes + c * 2 * kWave * 16 + my_off;
         return (unsigned)min(off, 0x7FFFFFF0ll);
     };
+    // Ring invariant at the top of chunk g: chunks g .. g+2 sit in slots
+    // x % 4 (chunks g+1 and g+2 complete since the last barrier when g is
+    // even, chunk g+2 possibly still being stored when g is odd),
+    // pre[x % PF] holds chunk x for g+3 <= x < g+3+PF, and this wave's
+    // operands of chunk g are in cur[].  Prologue: chunks 0..2 into slots
+    // 0..2.
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        *reinterpret_cast<u32x4_t *>(ring + j * kBfChunkBytes + threadIdx.x * 16) =
+            __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(j), 0, 0);
     u32x4_t pre[PF];
 #pragma unroll
     for (int j = 0; j < PF; ++j)
-        pre[j] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(j), 0, 0);
-    // Ring invariant at the top of chunk g: slot g%3 and slot (g+1)%3 hold
-    // chunks g and g+1 (stored in iterations g-2 and g-1), pre[x % PF] holds
-    // chunk x for g+2 <= x < g+2+PF, and this wave's operands of chunk g are
-    // in cur[].  Prologue: chunks 0 and 1 into slots 0 and 1.
-    *reinterpret_cast<u32x4_t *>(ring + threadIdx.x * 16) = pre[0];
-    *reinterpret_cast<u32x4_t *>(ring + kBfChunkBytes + threadIdx.x * 16) = pre[1 % PF];
-    pre[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(PF), 0, 0);
-    if (PF > 1) pre[1 % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(PF + 1), 0, 0);
+        pre[(3 + j) % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(3 + j), 0, 0);
     __syncthreads();
     const int my_t0 = 2 * half;  // my tiles: my_t0, my_t0 + 1
     auto read_ops = [&](int slotidx, u32x4_t *ops) {
@@ -227,16 +233,18 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const int g = st * NCH + c;
-            // next chunk's operands (slot (g+1)%3, complete since the last barrier)
+            // next chunk's operands (slot (g+1)%4, stored in iteration g-2,
+            // complete since the last barrier)
             u32x4_t nxt[4];
             read_ops((g + 1) % kBfRing, nxt);
-            // chunk g+2 into slot (g+2)%3 (it held chunk g-1, which every wave
-            // finished reading before the last barrier); refill its register
+            // chunk g+3 into slot (g+3)%4 (it held chunk g-1, which every wave
+            // read in iteration g-2, before the last barrier); refill its
+            // register
             {
-                const int rslot = (c + 2) % PF;
-                *reinterpret_cast<u32x4_t *>(ring + ((g + 2) % kBfRing) * kBfChunkBytes + threadIdx.x * 16) =
+                const int rslot = (c + 3) % PF;
+                *reinterpret_cast<u32x4_t *>(ring + ((g + 3) % kBfRing) * kBfChunkBytes + threadIdx.x * 16) =
                     pre[rslot];
-                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 2 + PF), 0, 0);
+                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 3 + PF), 0, 0);
             }
             if constexpr (BF3) {
                 const bf16x8_t a0h = __builtin_bit_cast(bf16x8_t, cur[0]);
@@ -264,7 +272,9 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
-            lds_barrier();  // chunk g+2 stored; everyone is past chunk g's reads
+            // every second chunk (NCH is even): chunks g+2 and g+3 stored,
+            // everyone past chunk g's and g+1's reads
+            if (c & 1) lds_barrier();
         }
 
         // Epilogue (as the FP32 scan): keys below the query's threshold go
