@@ -212,3 +212,18 @@ def test_precision_scan_keeps_fp32_above_128_dims():
     d, i = ix.search(qry, 5, probes=2, precision="bf16x3")
     assert knnhip.last_plan()["precision"] == "fp32"
     check(d, i, gd, gi, "dim 160")
+
+
+@pytest.mark.parametrize("k", [8, 100])
+def test_wide_dims_staged_and_chunked_operands(k):
+    # dim 768: the index's tiles are padded to 128 dims, so a search whose heaps leave no room
+    # for the staged query rows (k = 100) takes the chunked operands over the same tiles
+    R, Q, D = 6000, 90, 768
+    say(f"index dim {D} k={k}")
+    ref = knnhip.generate_mixture(R, D, 20, 0.2, 61, 62)
+    qry = knnhip.generate_mixture(Q, D, 20, 0.2, 61, 63)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    ix = knnhip.BlockIndex(ref, nblocks=24, iters=3)
+    d, i = ix.search(qry, k, probes=3)
+    check(d, i, gd, gi, f"dim {D} k={k}")
+    ix.close()

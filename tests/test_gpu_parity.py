@@ -234,3 +234,19 @@ def test_host_entry_chunked_upload(entry):
     sample = [0, 99, 199]
     gd, gi = oracle.knn_c(ref, np.ascontiguousarray(qry[:, sample]), k)
     assert np.array_equal(i[:, sample], gi) and np.array_equal(bits(d[:, sample]), bits(gd))
+
+
+@pytest.mark.parametrize("shape", [(1200, 40, 1100, 10), (900, 33, 768, 100)])
+def test_wide_dims_take_chunked_operands(shape):
+    # dims whose staged query rows do not fit in LDS beside the heaps take the chunked-register
+    # scan (B operands reloaded 128 dimensions at a time)
+    R, Q, D, k = shape
+    say(f"wide dims {shape}")
+    rng = np.random.default_rng(sum(shape))
+    ref = rng.random((D, R), dtype=np.float32)
+    qry = rng.random((D, Q), dtype=np.float32)
+    gd, gi = oracle.knn_c(ref, qry, k)
+    d, i = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k)
+    torch.cuda.synchronize()
+    assert knnhip.last_plan()["b_operands"] == "chunks"
+    assert np.array_equal(i.cpu().numpy(), gi) and np.array_equal(bits(d.cpu().numpy()), bits(gd))
