@@ -154,7 +154,7 @@ def router_form(args, knnhip, torch):
     build_s = time.perf_counter() - t
     rows = []
     for prec in ("fp32", "bf16x3"):
-        for P in (4, 6, 8):
+        for P in (4, 5, 6, 8):
             ms, (d, i) = timed(lambda: ix.search(qry, k, probes=P, precision=prec), args.router_reps)
             equal = bool(torch.equal(i, pi) and torch.equal(d, pd))
             ix.search(qry, k, probes=P, precision=prec, timing=True)

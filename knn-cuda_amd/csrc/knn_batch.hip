@@ -28,7 +28,8 @@ long long g_batch_fail = 0;  // certificate failures of the last batched search
 // ---------------------------------------------------------------------------
 struct BatchWork {
     DevBuf packed, pm, norms, perm, rmax, starts, counts, query, qpm, qnorm, theta, tasks, qlist,
-        rows, pval, pidx, dist, idx, fb_count, fb_list, qsub, dsub, isub, flist;
+        rows, pval, pidx, dist, idx, fb_count, fb_list, qsub, dsub, isub, flist, rjobs, qjobs, rwg, qwg,
+        outs;
     bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
 BatchWork g_bw;
@@ -178,39 +179,52 @@ bool batch_class_run(const BatchClass &c, int L, const knnhip_options &o, hipStr
         KNN_TRY(hipMemsetAsync(w.packed.as<float>() + (size_t)n_pos * pl.dpad, 0,
                                (size_t)(r_pad - n_pos) * pl.dpad * 4, s),
                 "memset");
+    // references and queries: one norms launch each over all the class's
+    // sets (norms, point-major rows, and the references' packed tiles at
+    // each set's start)
+    std::vector<NormJob> rjobs(np), qjobs(np);
+    std::vector<int2> rwg, qwg;
     for (int i = 0; i < np; ++i) {
         const knnhip_problem *p = c.probs[i];
-        NormArgs na{};
-        na.p = p->ref;
-        na.ld = p->ref_ld;
-        na.n = p->ref_nb;
-        na.n_pad = (p->ref_nb + kRefsPerTile - 1) / kRefsPerTile * kRefsPerTile;
-        na.dim = dim;
-        na.oct = pl.oct;
-        na.out = w.norms.as<float>() + starts[i];
-        na.maxbits = w.rmax.as<unsigned>();
-        na.pm = w.pm.as<float>() + (size_t)starts[i] * pm_ld;
-        na.pm_ld = pm_ld;
-        na.packed = w.packed.as<float>() + (size_t)starts[i] * pl.dpad;
-        KNN_TRY(launch_norms(na, s), "norm kernel launch");
-        KNN_TRY(hipMemcpy2DAsync(w.query.as<float>() + qstart[i], (size_t)Qtot * 4, p->query,
-                                 (size_t)p->query_ld * 4, (size_t)p->query_nb * 4, dim,
-                                 hipMemcpyDeviceToDevice, s),
-                "query copy");
+        rjobs[i] = NormJob{p->ref, p->ref_ld, p->ref_nb,
+                           (p->ref_nb + kRefsPerTile - 1) / kRefsPerTile * kRefsPerTile, starts[i]};
+        qjobs[i] = NormJob{p->query, p->query_ld, p->query_nb, p->query_nb, qstart[i]};
+        for (int i0 = 0; i0 < rjobs[i].n_pad; i0 += kNormPoints) rwg.push_back(make_int2(i, i0));
+        for (int i0 = 0; i0 < p->query_nb; i0 += kNormPoints) qwg.push_back(make_int2(i, i0));
     }
+    if (!w.need(w.rjobs, (size_t)np * sizeof(NormJob)) || !w.need(w.qjobs, (size_t)np * sizeof(NormJob)) ||
+        !w.need(w.rwg, rwg.size() * sizeof(int2)) || !w.need(w.qwg, qwg.size() * sizeof(int2))) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(hipMemcpyAsync(w.rjobs.p, rjobs.data(), (size_t)np * sizeof(NormJob), hipMemcpyHostToDevice, s),
+            "copy");
+    KNN_TRY(hipMemcpyAsync(w.qjobs.p, qjobs.data(), (size_t)np * sizeof(NormJob), hipMemcpyHostToDevice, s),
+            "copy");
+    KNN_TRY(hipMemcpyAsync(w.rwg.p, rwg.data(), rwg.size() * sizeof(int2), hipMemcpyHostToDevice, s), "copy");
+    KNN_TRY(hipMemcpyAsync(w.qwg.p, qwg.data(), qwg.size() * sizeof(int2), hipMemcpyHostToDevice, s), "copy");
+    NormArgs na{};
+    na.dim = dim;
+    na.oct = pl.oct;
+    na.out = w.norms.as<float>();
+    na.maxbits = w.rmax.as<unsigned>();
+    na.pm = w.pm.as<float>();
+    na.pm_ld = pm_ld;
+    na.packed = w.packed.as<float>();
+    KNN_TRY(launch_norms_jobs(na, w.rjobs.as<NormJob>(), w.rwg.as<int2>(), (int)rwg.size(), s),
+            "norm kernel launch");
     KNN_TRY(idx_batch_perm(w.starts.as<int>(), w.counts.as<int>(), np, (int)n_pos, w.perm.as<int>(), s),
             "positions");
     NormArgs nq{};
-    nq.p = w.query.as<float>();
-    nq.ld = Qtot;
-    nq.n = (int)Qtot;
-    nq.n_pad = (int)Qtot;
     nq.dim = dim;
     nq.oct = (dim + 7) / 8;
     nq.out = w.qnorm.as<float>();
     nq.pm = w.qpm.as<float>();
     nq.pm_ld = pm_ld;
-    KNN_TRY(launch_norms(nq, s), "norm kernel launch");
+    KNN_TRY(launch_norms_jobs(nq, w.qjobs.as<NormJob>(), w.qwg.as<int2>(), (int)qwg.size(), s),
+            "norm kernel launch");
+    // the class's dim-major query block (read by the chunked scan) from the rows
+    KNN_TRY(idx_transpose(w.qpm.as<float>(), pm_ld, (int)Qtot, dim, w.query.as<float>(), s), "queries");
     KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Qtot, s), "memset");
     KNN_TRY(hipMemsetAsync(w.fb_count.p, 0, 4, s), "memset");
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -315,16 +329,18 @@ bool batch_class_run(const BatchClass &c, int L, const knnhip_options &o, hipStr
             a = b;
         }
     }
-    // each problem's k x query_nb outputs
-    for (int i = 0; i < np; ++i) {
-        const knnhip_problem *p = c.probs[i];
-        KNN_TRY(hipMemcpy2DAsync(p->knn_dist, (size_t)p->query_nb * 4, w.dist.as<float>() + qstart[i],
-                                 (size_t)Qtot * 4, (size_t)p->query_nb * 4, k, hipMemcpyDeviceToDevice, s),
-                "output copy");
-        KNN_TRY(hipMemcpy2DAsync(p->knn_index, (size_t)p->query_nb * 4, w.idx.as<int>() + qstart[i],
-                                 (size_t)Qtot * 4, (size_t)p->query_nb * 4, k, hipMemcpyDeviceToDevice, s),
-                "output copy");
+    // each problem's k x query_nb outputs, one launch for the class
+    std::vector<BatchOut> outs(np);
+    for (int i = 0; i < np; ++i)
+        outs[i] = BatchOut{c.probs[i]->knn_dist, c.probs[i]->knn_index, c.probs[i]->query_nb, qstart[i]};
+    if (!w.need(w.outs, (size_t)np * sizeof(BatchOut))) {
+        set_error("Memory allocation error");
+        return false;
     }
+    KNN_TRY(hipMemcpyAsync(w.outs.p, outs.data(), (size_t)np * sizeof(BatchOut), hipMemcpyHostToDevice, s),
+            "copy");
+    KNN_TRY(idx_batch_outputs(w.dist.as<float>(), w.idx.as<int>(), k, (int)Qtot, w.outs.as<BatchOut>(), np, s),
+            "outputs");
     return true;
 }
 

@@ -86,6 +86,16 @@ struct NormArgs {
     int nch_bf;                 // its 16-dimension chunks per tile (>= ceil(dim / 16))
 };
 
+// One point set of a multi-set norms pass: its points land at `start` (a
+// multiple of 32) in the pass's outputs (norms, point-major rows, tiles).
+struct NormJob {
+    const float *p;  // dim rows of ld floats
+    long long ld;
+    int n;
+    int n_pad;       // points covered (a multiple of 32 for tiles)
+    int start;
+};
+
 // Arguments of the select / exact re-rank / certificate stage.
 struct FinalArgs {
     const float *ref;

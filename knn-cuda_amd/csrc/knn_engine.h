@@ -57,6 +57,9 @@ hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t ld
 hipError_t launch_pack_bf3_rows(const float *pm, int ld, int n_pad, int nch, unsigned short *out,
                                 hipStream_t s);
 hipError_t launch_norms(const NormArgs &a, hipStream_t s);
+hipError_t launch_norms_jobs(const NormArgs &a, const NormJob *jobs, const int2 *wg, int nwg,
+                             hipStream_t s);
+constexpr int kNormPoints = 64;  // points per workgroup of the norms pass
 hipError_t launch_select(const FinalArgs &a, hipStream_t s);
 hipError_t launch_exact(const ExactArgs &a, int grid, hipStream_t s);
 hipError_t launch_gen_uniform(float *out, int n, int dim, uint64_t seed, float scale,

@@ -62,6 +62,16 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
                        const float *fcent, int nf, const float *fnorm, const float *fsq,
                        const float *fradf, const int *block_s, const int *fb_start, int nb, float *lb,
                        hipStream_t s);
+// Outputs of one problem of a batched class: its queries are columns
+// qstart .. qstart + qn - 1 of the class's k x Qtot results.
+struct BatchOut {
+    float *dist;
+    int *idx;
+    int qn;
+    int qstart;
+};
+hipError_t idx_batch_outputs(const float *dist, const int *idx, int k, int Qtot, const BatchOut *outs,
+                             int np, hipStream_t s);
 hipError_t idx_batch_perm(const int *starts, const int *counts, int nsets, int n_pos, int *perm,
                           hipStream_t s);
 

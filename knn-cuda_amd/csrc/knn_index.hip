@@ -1391,6 +1391,34 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
                        pm_ld, Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb);
     return hipGetLastError();
 }
+// Batched outputs: element e = j * Qtot + q of the class's k x Qtot results
+// goes to row j of the problem whose queries hold column q (outs by
+// ascending qstart), at its own column q - qstart.
+__global__ void batch_outputs_kernel(const float *__restrict__ dist, const int *__restrict__ idx, int k,
+                                     int Qtot, const BatchOut *__restrict__ outs, int np) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)k * Qtot;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(e / Qtot), q = (int)(e - (long long)j * Qtot);
+        int lo = 0, hi = np - 1;  // the last problem with qstart <= q
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (outs[mid].qstart <= q)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchOut o = outs[lo];
+        const long long d = (long long)j * o.qn + (q - o.qstart);
+        o.dist[d] = dist[e];
+        o.idx[d] = idx[e];
+    }
+}
+hipError_t idx_batch_outputs(const float *dist, const int *idx, int k, int Qtot, const BatchOut *outs,
+                             int np, hipStream_t s) {
+    hipLaunchKernelGGL(batch_outputs_kernel, dim3(grid_for((long long)k * Qtot)), dim3(256), 0, s, dist, idx, k,
+                       Qtot, outs, np);
+    return hipGetLastError();
+}
 hipError_t idx_batch_perm(const int *starts, const int *counts, int nsets, int n_pos, int *perm,
                           hipStream_t s) {
     hipLaunchKernelGGL(batch_perm_kernel, dim3(grid_for(n_pos)), dim3(256), 0, s, starts, counts,

@@ -574,14 +574,14 @@ hipError_t launch_pack_bf3_rows(const float *pm, int ld, int n_pad, int nch, uns
 // is an FP32 fused multiply-add sum in dimension order.  Replaces
 // code/knncuda.cu:217-227.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void norms_pack_kernel(NormArgs a) {
+constexpr int kNormPts = 64;  // points per workgroup of the norms pass
+__device__ __forceinline__ void norms_pack_points(const NormArgs &a, int i0) {
     // 64 points per workgroup; 32 dimensions (4 octets) at a time go through
     // LDS so that the column reads, the packed-tile writes and the
     // point-major row writes are all contiguous.
-    constexpr int kPts = 64, kDims = 32;
+    constexpr int kPts = kNormPts, kDims = 32;
     __shared__ float tile[kDims][kPts + 1];
     const int t = threadIdx.x;
-    const int i0 = blockIdx.x * kPts;
     float s = 0.0f;  // thread t < 64: squared norm of point i0 + t, summed in dimension order
     const int ngrp = (a.oct + 3) / 4;
     for (int g = 0; g < ngrp; ++g) {
@@ -666,8 +666,40 @@ __global__ __launch_bounds__(256) void norms_pack_kernel(NormArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void norms_pack_kernel(NormArgs a) {
+    norms_pack_points(a, blockIdx.x * kNormPts);
+}
+
+// Many point sets in one launch: workgroup w takes points wg[w].y .. +63 of
+// set wg[w].x, written at that set's start in the outputs of `a`.
+__global__ __launch_bounds__(256) void norms_pack_jobs_kernel(NormArgs a, const NormJob *__restrict__ jobs,
+                                                              const int2 *__restrict__ wg) {
+    const int2 w = wg[blockIdx.x];
+    const NormJob j = jobs[w.x];
+    NormArgs b = a;
+    b.p = j.p;
+    b.ld = j.ld;
+    b.n = j.n;
+    b.n_pad = j.n_pad;
+    b.out = a.out + j.start;
+    b.pm = a.pm ? a.pm + (long long)j.start * a.pm_ld : nullptr;
+    b.packed = a.packed ? a.packed + (long long)j.start * 8 * a.oct : nullptr;
+    b.packed_bf = a.packed_bf ? a.packed_bf + (long long)(j.start / kRefsPerTile) * a.nch_bf * 2 * kWave * 8
+                              : nullptr;
+    norms_pack_points(b, w.y);
+}
+
 hipError_t launch_norms(const NormArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(norms_pack_kernel, dim3((a.n_pad + 63) / 64), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(norms_pack_kernel, dim3((a.n_pad + kNormPts - 1) / kNormPts), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// jobs / wg: device arrays (wg: nwg (set, first point) pairs, the first point
+// a multiple of 64 below the set's n_pad).
+hipError_t launch_norms_jobs(const NormArgs &a, const NormJob *jobs, const int2 *wg, int nwg,
+                             hipStream_t s) {
+    if (nwg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(norms_pack_jobs_kernel, dim3(nwg), dim3(256), 0, s, a, jobs, wg);
     return hipGetLastError();
 }
 
