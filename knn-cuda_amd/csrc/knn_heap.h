@@ -65,9 +65,17 @@ __device__ __forceinline__ void unlock(int *l) {
 }
 
 // LDS of a 32-query scan block: the shared heaps and locks, then one
-// candidate queue (key, reference, query column) per wave.
-constexpr int kQueueCap = 768;  // >= 512 (one half tile of candidates) + slack
+// candidate queue (key, reference, query column) per wave.  Blocks of up to
+// four waves queue up to 768 candidates a wave, checked for a drain every 8
+// keys (one check admits at most 8 x 64); eight-wave blocks, whose LDS is
+// short, 384, checked every 4 keys.
+constexpr int kQueueCap = 768;
 constexpr size_t kQueueBytes = (size_t)kQueueCap * 3 * sizeof(float);
+__host__ __device__ constexpr int scan_queue_cap(int waves) { return waves > 4 ? 384 : kQueueCap; }
+__host__ __device__ constexpr int scan_queue_check(int waves) { return waves > 4 ? 4 : 8; }
+__host__ __device__ constexpr size_t scan_queue_bytes(int waves) {
+    return (size_t)scan_queue_cap(waves) * 3 * sizeof(float);
+}
 __host__ __device__ constexpr size_t scan_heap_bytes(int kp) {
     return (((size_t)kQueriesPerWave * heap_stride(kp) * 2 + kQueriesPerWave) * sizeof(float) + 15) &
            ~(size_t)15;

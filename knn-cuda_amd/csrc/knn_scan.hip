@@ -53,6 +53,8 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
     constexpr int OCT = KS / 4;  // octets per chunk
     constexpr bool MULTI = BMODE == kBChunks;
     constexpr bool BLDS = BMODE == kBLds;
+    constexpr int QCAP = scan_queue_cap(MAXW), QCHK = scan_queue_check(MAXW);
+    constexpr size_t QBYTES = scan_queue_bytes(MAXW);
     static_assert(KS % 4 == 0, "k-steps come in octets of 4");
     static_assert(OCT % NBUF == 0, "ring position must repeat every chunk");
 
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
     float *bl = nullptr;
     const int bpitch = 8 * a.oct + 4;
     if constexpr (BLDS) {
-        bl = reinterpret_cast<float *>(smem + scan_heap_bytes(kp) + (size_t)waves * kQueueBytes);
+        bl = reinterpret_cast<float *>(smem + scan_heap_bytes(kp) + (size_t)waves * QBYTES);
         const int dpad = 8 * a.oct;
         if (a.query_pm) {
             // point-major rows: consecutive threads take consecutive dims of a row
@@ -147,10 +149,10 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
 
     // Per-wave candidate queue (key, reference, query column) in LDS, after
     // the heaps and locks.
-    unsigned char *qbase_ptr = smem + scan_heap_bytes(kp) + (size_t)wave * kQueueBytes;
+    unsigned char *qbase_ptr = smem + scan_heap_bytes(kp) + (size_t)wave * QBYTES;
     float *qv = reinterpret_cast<float *>(qbase_ptr);
-    int *qr = reinterpret_cast<int *>(qbase_ptr + kQueueCap * sizeof(float));
-    int *qc = reinterpret_cast<int *>(qbase_ptr + kQueueCap * 2 * sizeof(float));
+    int *qr = reinterpret_cast<int *>(qbase_ptr + QCAP * sizeof(float));
+    int *qc = reinterpret_cast<int *>(qbase_ptr + QCAP * 2 * sizeof(float));
     int qcount = 0;  // wave-uniform
     const unsigned long long lanes_below = (1ull << lane) - 1ull;
     // Insert every queued candidate still below its query's heap root.  The
@@ -365,8 +367,8 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
                 if (__ballot(cm && qvalid) == 0) continue;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    if ((i & 7) == 0 && qcount > kQueueCap - 8 * kWave) {
-                        // room for a half tile of candidates (8 keys x 64 lanes)
+                    if (i % QCHK == 0 && qcount > QCAP - QCHK * kWave) {
+                        // room for QCHK keys x 64 lanes of candidates
                         drain();
                         theta = fminf(theta, shared_root(myv));
                     }
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
                     qcount += n;
                 }
             }
-            if (qcount >= kQueueCap / 2) {
+            if (qcount >= QCAP / 2) {
                 drain();
                 theta = fminf(theta, shared_root(myv));
                 if (h == 0 && qvalid && theta < published) {
@@ -423,7 +425,7 @@ int scan_pick_ks(int dim) {
 }
 
 size_t scan_lds_bytes(int kp, int waves) {
-    return scan_heap_bytes(kp) + (size_t)waves * kQueueBytes;
+    return scan_heap_bytes(kp) + (size_t)waves * scan_queue_bytes(waves);
 }
 
 // The same, with the staged query rows of the kBLds instance (oct octets).
