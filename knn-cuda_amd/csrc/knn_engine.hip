@@ -564,7 +564,11 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
             "Unable to copy data from host to device");
     if (chunked) {
         const Upload up{ref, R, g_ws.copy, kUploadChunks};
-        if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s, 0, &up)) return false;
+        if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s, 0, &up)) {
+            // copies already queued still read the caller's buffer: let them finish
+            (void)hipStreamSynchronize(g_ws.copy);
+            return false;
+        }
     } else {
         KNN_TRY(hipMemcpyAsync(ref_d, ref, (size_t)dim * R * sizeof(float), hipMemcpyHostToDevice, s),
                 "Unable to copy data from host to device");
