@@ -32,8 +32,6 @@ struct BatchWork {
         outs;
     bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
-BatchWork g_bw;
-
 constexpr int kBatchSlices[] = {2048, 8192, 32768, 131072, 1 << 20};
 
 struct BatchTuneKey {
@@ -46,6 +44,55 @@ struct BatchClass {
     int dim = 0, k = 0;
     std::vector<const knnhip_problem *> probs;
 };
+
+// Classes in flight: each on its own stream and workspace.  A class is
+// queued (layouts, scan, select, an asynchronous read-back of its failure
+// count) and finished (failures searched again, outputs scattered) only when
+// its lane comes round again, so the host prepares the next classes while
+// the GPU runs the earlier ones, and small classes share the chip.
+constexpr int kBatchLanes = 4;
+BatchWork g_bw[kBatchLanes];
+struct BatchLanes {
+    bool ready = false;
+    hipStream_t stream[kBatchLanes] = {};
+    hipEvent_t counted[kBatchLanes] = {};  // failure count copied to host
+    hipEvent_t start = nullptr;            // the caller's inputs are ready
+    int *nfail = nullptr;                  // pinned, one per lane
+};
+BatchLanes g_lanes;
+
+bool batch_lanes_ready() {
+    if (g_lanes.ready) return true;
+    for (int l = 0; l < kBatchLanes; ++l) {
+        KNN_TRY(hipStreamCreateWithFlags(&g_lanes.stream[l], hipStreamNonBlocking), "stream create");
+        KNN_TRY(hipEventCreateWithFlags(&g_lanes.counted[l], hipEventDisableTiming), "event create");
+    }
+    KNN_TRY(hipEventCreateWithFlags(&g_lanes.start, hipEventDisableTiming), "event create");
+    KNN_TRY(hipHostMalloc((void **)&g_lanes.nfail, kBatchLanes * sizeof(int), hipHostMallocDefault),
+            "pinned allocation");
+    g_lanes.ready = true;
+    return true;
+}
+
+// Lane streams wait for what the caller queued on its stream.
+bool batch_lanes_follow(hipStream_t s) {
+    KNN_TRY(hipEventRecord(g_lanes.start, s), "event");
+    for (int l = 0; l < kBatchLanes; ++l) KNN_TRY(hipStreamWaitEvent(g_lanes.stream[l], g_lanes.start, 0), "event wait");
+    return true;
+}
+
+// A class between its two phases.
+struct ClassRun {
+    const BatchClass *c = nullptr;
+    int lane = 0;
+    bool active = false;
+    bool direct = false;  // searched problem by problem, already finished
+    long long Qtot = 0;
+    std::vector<int> qstart;
+    hipEvent_t e0 = nullptr, e1 = nullptr;  // scan timing, when asked for
+    float *scan_ms = nullptr;
+};
+
 
 // Tasks of a class for slice length L: per problem ceil(R/L) slices times
 // ceil(Q/32) query groups.
@@ -89,20 +136,34 @@ int batch_default_slice(const BatchClass &c, int kp, int resident, int waves) {
     return best;
 }
 
-bool batch_class_run(const BatchClass &c, int L, const knnhip_options &o, hipStream_t s,
-                     float *scan_ms) {
-    BatchWork &w = g_bw;
+// Queue phase of class c on `lane` (its stream and workspace; the lane's
+// previous class must be finished).
+bool batch_class_queue(const BatchClass &c, int L, const knnhip_options &o, int lane, float *scan_ms,
+                       ClassRun &run) {
+    BatchWork &w = g_bw[lane];
+    const hipStream_t s = g_lanes.stream[lane];
+    // the lane's previous outputs are out before its buffers are reused
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    run = ClassRun{};
+    run.c = &c;
+    run.lane = lane;
+    run.active = true;
+    run.scan_ms = scan_ms;
     const int dim = c.dim, k = c.k;
     int maxR = 0;
     for (const knnhip_problem *p : c.probs) maxR = std::max(maxR, p->ref_nb);
     const Plan pl = make_plan(maxR, kQueriesPerWave, dim, k, o);
     if (!pl.use_scan) {
-        // k too large for the scan's heaps: each problem on its own
+        // k too large for the scan's heaps: each problem on its own, through
+        // the plain engine's shared workspace, so synchronously
         for (const knnhip_problem *p : c.probs)
             if (!run_device(p->ref, p->ref_ld, p->ref_nb, p->query, p->query_ld, p->query_nb, dim, k,
                             p->knn_dist, p->knn_index, make_plan(p->ref_nb, p->query_nb, dim, k, o), o,
                             s))
                 return false;
+        KNN_TRY(hipStreamSynchronize(s), "sync");
+        run.direct = true;
+        if (scan_ms) *scan_ms = 0.f;
         return true;
     }
     const int kp = pl.kp;
@@ -285,14 +346,36 @@ bool batch_class_run(const BatchClass &c, int L, const knnhip_options &o, hipStr
     fa.fb_count = w.fb_count.as<int>();
     fa.fb_list = w.fb_list.as<int>();
     KNN_TRY(launch_select(fa, s), "select kernel launch");
-    // certificate failures: the plain search of their own problem
-    int nfail = 0;
-    KNN_TRY(hipMemcpyAsync(&nfail, w.fb_count.p, 4, hipMemcpyDeviceToHost, s), "copy");
-    KNN_TRY(hipStreamSynchronize(s), "sync");
-    if (scan_ms) {
-        KNN_TRY(hipEventElapsedTime(scan_ms, e0, e1), "event");
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+    // the failure count comes back asynchronously
+    KNN_TRY(hipMemcpyAsync(g_lanes.nfail + lane, w.fb_count.p, 4, hipMemcpyDeviceToHost, s), "copy");
+    KNN_TRY(hipEventRecord(g_lanes.counted[lane], s), "event");
+    run.Qtot = Qtot;
+    run.qstart = qstart;
+    run.e0 = e0;
+    run.e1 = e1;
+    return true;
+}
+
+// Finish phase: certificate failures searched again (the plain search of
+// their own problem), then each problem's k x query_nb outputs.
+bool batch_class_finish(ClassRun &run, const knnhip_options &o) {
+    if (!run.active) return true;
+    run.active = false;
+    if (run.direct) return true;
+    const BatchClass &c = *run.c;
+    const int lane = run.lane;
+    BatchWork &w = g_bw[lane];
+    const hipStream_t s = g_lanes.stream[lane];
+    const int dim = c.dim, k = c.k, np = (int)c.probs.size();
+    const long long Qtot = run.Qtot;
+    const std::vector<int> &qstart = run.qstart;
+    KNN_TRY(hipEventSynchronize(g_lanes.counted[lane]), "sync");
+    const int nfail = g_lanes.nfail[lane];
+    if (run.scan_ms) {
+        KNN_TRY(hipEventSynchronize(run.e1), "sync");
+        KNN_TRY(hipEventElapsedTime(run.scan_ms, run.e0, run.e1), "event");
+        (void)hipEventDestroy(run.e0);
+        (void)hipEventDestroy(run.e1);
     }
     g_batch_fail += nfail;
     if (nfail > 0) {
@@ -394,10 +477,14 @@ int knnhip_search_batched(const knnhip_problem *problems, int count, const knnhi
     if (!batch_args(problems, count) || !ensure_device()) return 0;
     const knnhip_options o = opt ? *opt : default_options(0);
     hipStream_t s = (hipStream_t)stream;
+    if (!batch_lanes_ready() || !batch_lanes_follow(s)) return 0;
     g_batch_fail = 0;
     std::string plan = "{\"classes\": [";
     bool first = true;
-    for (const BatchClass &c : batch_classes(problems, count)) {
+    const std::vector<BatchClass> classes = batch_classes(problems, count);
+    ClassRun runs[kBatchLanes];
+    int ci = 0;
+    for (const BatchClass &c : classes) {
         const Plan pl = make_plan(1 << 20, kQueriesPerWave, c.dim, c.k, o);
         auto it = g_batch_table.find(BatchTuneKey{c.dim, c.k});
         int L = it != g_batch_table.end() ? it->second : batch_default_slice(c, std::max(pl.kp, c.k), batch_resident(pl), pl.waves);
@@ -408,7 +495,16 @@ int knnhip_search_batched(const knnhip_problem *problems, int count, const knnhi
                  it != g_batch_table.end() ? "true" : "false");
         plan += buf;
         first = false;
-        if (!batch_class_run(c, L, o, s, nullptr)) return 0;
+        const int lane = ci++ % kBatchLanes;
+        if (!batch_class_finish(runs[lane], o) || !batch_class_queue(c, L, o, lane, nullptr, runs[lane]))
+            return 0;
+    }
+    for (int l = 0; l < kBatchLanes; ++l) {
+        if (!batch_class_finish(runs[l], o)) return 0;
+        if (hipStreamSynchronize(g_lanes.stream[l]) != hipSuccess) {
+            set_error("sync: %s", hipGetErrorString(hipGetLastError()));
+            return 0;
+        }
     }
     g_last_plan = plan + "]}";
     g_last_fb = nullptr;
@@ -423,7 +519,9 @@ int knnhip_batched_autotune(const knnhip_problem *problems, int count, int reps,
     if (!batch_args(problems, count) || !ensure_device()) return 0;
     knnhip_options o = default_options(0);
     hipStream_t s = (hipStream_t)stream;
-    for (const BatchClass &c : batch_classes(problems, count)) {
+    if (!batch_lanes_ready() || !batch_lanes_follow(s)) return 0;
+    const std::vector<BatchClass> classes = batch_classes(problems, count);
+    for (const BatchClass &c : classes) {
         const Plan pl = make_plan(1 << 20, kQueriesPerWave, c.dim, c.k, o);
         const int kp = std::max(pl.kp, c.k);
         int best = 0;
@@ -434,7 +532,8 @@ int knnhip_batched_autotune(const knnhip_problem *problems, int count, int reps,
             float tot = 0.f;
             for (int r = 0; r < std::max(1, reps); ++r) {
                 float ms = 0.f;
-                if (!batch_class_run(c, L, o, s, &ms)) return 0;
+                ClassRun run;
+                if (!batch_class_queue(c, L, o, 0, &ms, run) || !batch_class_finish(run, o)) return 0;
                 tot += ms;
             }
             if (!best || tot < best_ms) {

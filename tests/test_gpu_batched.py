@@ -98,3 +98,25 @@ def test_bad_problem_is_rejected():
     qry = knnhip.generate_uniform(10, 8, seed=2)
     with pytest.raises(knnhip.KnnError):
         knnhip.search_batched([(ref, qry, 5), (ref, qry, 101)])  # k > ref_nb
+
+
+def test_certificate_failures_across_classes_in_flight():
+    # lattice data (many equal distances) with a one-candidate margin: certificates fail in
+    # several classes that are in flight together, and each failed query is searched again
+    rng = np.random.default_rng(44)
+    shapes = [(900, 40, 16, 10), (1300, 70, 16, 10), (800, 33, 24, 6), (1100, 50, 24, 6),
+              (700, 20, 40, 12), (1500, 64, 40, 12), (600, 30, 8, 4), (1000, 45, 8, 4), (950, 25, 32, 9)]
+    items = []
+    for R, Q, D, k in shapes:
+        ref = torch.from_numpy((np.floor(rng.random((D, R)) * 3) / 3).astype(np.float32)).cuda()
+        qry = torch.from_numpy((np.floor(rng.random((D, Q)) * 3) / 3).astype(np.float32)).cuda()
+        items.append((ref, qry, k))
+    outs = knnhip.search_batched(items, margin=1)
+    torch.cuda.synchronize()
+    fails = knnhip.batched_last_failures()
+    say(f"classes {len({(r.shape[0], k) for r, q, k in items})}, certificate failures {fails}")
+    assert fails > 0
+    for j, ((ref, qry, k), (d, i)) in enumerate(zip(items, outs)):
+        gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+        assert np.array_equal(i.cpu().numpy(), gi), f"problem {j}: indexes"
+        assert np.array_equal(bits(d.cpu().numpy()), bits(gd)), f"problem {j}: distances"
