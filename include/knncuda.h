@@ -232,6 +232,27 @@ int knnhip_batched_autotune(const knnhip_problem *problems, int count, int reps,
 int knnhip_batched_table(char *buf, int len);
 int knnhip_batched_set_slice(int dim, int k, int slice_len);
 
+/*
+ * Device memory budget of the host-pointer entry points (knn_cuda_global,
+ * knn_cuda_texture, knn_cublas, their knnhip_ aliases, knnhip_kth_distance).
+ * A search whose buffers would exceed it streams: the reference set goes
+ * through two chunk buffers (the next chunk's copy overlapping the current
+ * chunk's search), the queries go in groups, and the per-chunk results are
+ * merged on the device, with the same exact result.  0 (the default) means
+ * 90 % of the free device memory plus this library's caches, the share the
+ * reference's legacy memory-adaptive split uses
+ * (legacy/KNN.CUDA.with.indexes/knn_cuda_with_indexes.cu:49,340-349, which
+ * splits only the queries).  A streamed call frees the library's cached
+ * buffers first.  Returns 1, or 0 for a negative value.
+ */
+int knnhip_set_device_budget(long long bytes);
+long long knnhip_device_budget(void);
+/* Reference chunk and query group sizes a streamed search of these sizes
+ * would use within device_bytes (ref_chunk == ref_nb: references resident).
+ * Returns 0 when nothing fits (knnhip_last_error says why). */
+int knnhip_stream_chunks(int ref_nb, int query_nb, int dim, int k, long long device_bytes,
+                         int *ref_chunk, int *query_chunk);
+
 /* Library version string. */
 const char *knnhip_version(void);
 

@@ -3,6 +3,7 @@
 //   knn_index_host.hip  block index build / search / save / load
 //   knn_router_host.hip learned router training and scoring
 //   knn_batch.hip       batched search and its slice-length autotuner
+//   knn_stream.hip      searches streamed through a device memory budget
 // Every C entry point takes g_mu for its whole call, so the state below is
 // touched by one host thread at a time.
 #pragma once
@@ -166,6 +167,21 @@ constexpr int kUploadMinRefs = 1 << 18;  // smaller sets are uploaded in one cop
 bool run_device(const float *ref, long long ldr, int R, const float *query, long long ldq, int Q,
                 int dim, int k, float *out_dist, int *out_idx, const Plan &p,
                 const knnhip_options &o, hipStream_t s, int level = 0, const Upload *up = nullptr);
+
+// Searches larger than the device memory they may use (knn_stream.hip).
+// Device bytes of one host-pointer search of R references and Q queries,
+// resident or streamed (two reference chunk buffers and merged results).
+size_t search_footprint(int R, int Q, int dim, int k, const knnhip_options &o, bool streamed);
+// Reference chunk and query group sizes of a streamed search within budget.
+bool stream_chunks(int R, int Q, int dim, int k, const knnhip_options &o, size_t budget, int *rc,
+                   int *qc);
+// knnhip_set_device_budget's value, else 90 % of free plus cached memory.
+size_t device_budget(bool *explicit_budget);
+// Frees every cached workspace buffer (after the device is idle).
+void ws_release();
+bool run_streamed(const float *ref, int R, const float *query, int Q, int dim, int k,
+                  float *knn_dist, int *knn_index, const knnhip_options &o, size_t budget,
+                  bool kth_only);
 
 // Whole-file readers and writers of the index and router file formats:
 // `ok` turns false at the first short read or write and stays false.

@@ -529,6 +529,11 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
         return false;
     if (!ensure_device()) return false;
     const knnhip_options o = default_options(variant);
+    // Problems beyond the device memory budget stream through it.
+    bool explicit_budget = false;
+    const size_t budget = device_budget(&explicit_budget);
+    if (search_footprint(R, Q, dim, k, o, false) > budget)
+        return run_streamed(ref, R, query, Q, dim, k, knn_dist, knn_index, o, budget, kth_only);
     Plan p = make_plan(R, Q, dim, k, o);
     // Large reference sets go up in chunks of whole slices, each scanned as
     // soon as it lands, so the copy overlaps the search.

@@ -63,6 +63,9 @@ C_SYMBOLS = (
     "knnhip_batched_autotune",
     "knnhip_batched_table",
     "knnhip_batched_set_slice",
+    "knnhip_set_device_budget",
+    "knnhip_device_budget",
+    "knnhip_stream_chunks",
     "knnhip_version",
 )
 
@@ -190,6 +193,12 @@ def load() -> ctypes.CDLL:
     lib.knnhip_batched_table.restype = i32
     lib.knnhip_batched_set_slice.argtypes = [i32, i32, i32]
     lib.knnhip_batched_set_slice.restype = i32
+    lib.knnhip_set_device_budget.argtypes = [i64]
+    lib.knnhip_set_device_budget.restype = i32
+    lib.knnhip_device_budget.argtypes = []
+    lib.knnhip_device_budget.restype = i64
+    lib.knnhip_stream_chunks.argtypes = [i32, i32, i32, i32, i64, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.knnhip_stream_chunks.restype = i32
     _lib = lib
     return lib
 
@@ -665,3 +674,24 @@ def batched_set_slice(dim: int, k: int, slice_len: int):
 
 def batched_last_failures() -> int:
     return int(load().knnhip_batched_last_failures())
+
+
+def set_device_budget(nbytes: int):
+    """Device memory the host-pointer entry points may use (0: 90 % of free memory).
+
+    Larger searches stream the reference set through it in chunks, with the same exact result."""
+    if not load().knnhip_set_device_budget(int(nbytes)):
+        raise KnnError(last_error())
+
+
+def device_budget() -> int:
+    return int(load().knnhip_device_budget())
+
+
+def stream_chunks(ref_nb: int, query_nb: int, dim: int, k: int, device_bytes: int) -> tuple:
+    """(reference chunk, query group) sizes of a streamed search within device_bytes."""
+    rc, qc = ctypes.c_int32(0), ctypes.c_int32(0)
+    if not load().knnhip_stream_chunks(ref_nb, query_nb, dim, k, int(device_bytes), ctypes.byref(rc),
+                                       ctypes.byref(qc)):
+        raise KnnError(last_error())
+    return rc.value, qc.value
