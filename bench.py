@@ -37,6 +37,7 @@ sys.path.insert(0, os.path.join(ROOT, "knn-cuda_amd"))
 
 METRIC = "exact-kNN queries/s on 1 MI355X; BF16+re-rank+router speedup vs plain FP32"
 FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "scan_f32_1Mx16k_d128_k32_traffic.json")
 
 
@@ -418,6 +419,15 @@ def main():
                          f"single-threaded oracle restatement of knn_c, {cpu_s:.1f} s"}
 
     plan = knnhip.last_plan()
+    # selection / re-rank stage: HBM-bound gathers.  Algorithmic bytes per launch = the candidates'
+    # point-major rows (4 D bytes each, kp per query) + the slice heaps (8 bytes per entry) + the
+    # outputs (8 k bytes per query)
+    sel_ms = sum(stage_ms[2]) / len(stage_ms[2])
+    kp, slices = plan.get("kp", k), plan.get("slices", 1)
+    sel_bytes = 4.0 * Q * kp * D + 8.0 * slices * Q * kp + 8.0 * k * Q
+    select_stage = {"kernel": "select_rerank_kernel", "ms": sel_ms, "algorithmic_bytes": sel_bytes,
+                    "achieved_gb_per_s": sel_bytes / (sel_ms * 1e-3) / 1e9,
+                    "hbm_peak_frac": sel_bytes / (sel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     precision = None if args.no_precision_form else precision_form(args, knnhip, torch, ref, qry, k,
                                                                       dist, idx, ms_per_step, scan_avg)
     router = None if args.no_router_form else router_form(args, knnhip, torch)
@@ -444,6 +454,7 @@ def main():
                      "kernel": "scan_f32_kernel", "kernel_ms": scan_avg,
                      "flops_per_launch": flops},
         "cpu_baseline": cpu,
+        "select_stage": select_stage,
         "stages_ms": {name: sum(v) / len(v) for name, v in
                       zip(("norms", "scan", "select_rerank", "fallback"), stage_ms)},
         "fallback_queries": fallbacks,

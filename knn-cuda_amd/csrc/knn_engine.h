@@ -4,6 +4,7 @@
 //   knn_router_host.hip learned router training and scoring
 //   knn_batch.hip       batched search and its slice-length autotuner
 //   knn_stream.hip      searches streamed through a device memory budget
+//   knn_hostio.hip      host transfers through pinned staging
 // Every C entry point takes g_mu for its whole call, so the state below is
 // touched by one host thread at a time.
 #pragma once
@@ -103,6 +104,7 @@ struct Workspace {
     hipEvent_t ev[5] = {};
     bool have_events = false;
     hipEvent_t up_ev[8] = {};    // one per upload chunk
+    hipEvent_t qry_ev = nullptr; // query copy on the upload stream
     hipStream_t copy = nullptr;  // upload stream
 };
 extern Workspace g_ws;
@@ -160,13 +162,43 @@ struct Upload {
 constexpr int kUploadChunks = 8;
 constexpr int kUploadMinRefs = 1 << 18;  // smaller sets are uploaded in one copy
 
+// A search whose certificate-failure count is read back later: run_device
+// queues an asynchronous copy of the count and an event instead of
+// synchronising, and finish_device waits on the event and searches the
+// failed queries again (queued on the same stream).
+struct Deferred {
+    bool pending = false;
+    const float *ref = nullptr, *query = nullptr;
+    long long ldr = 0, ldq = 0;
+    int R = 0, Q = 0, dim = 0, k = 0;
+    float *out_dist = nullptr;
+    int *out_idx = nullptr;
+    const int *fb_list = nullptr, *fb_count = nullptr;
+    knnhip_options o{};
+    int margin = 0, level = 0;
+    hipStream_t s = nullptr;
+    int *nfail_host = nullptr;  // pinned
+    hipEvent_t ev = nullptr;
+};
+bool finish_device(Deferred &d);
+
 // Search on device pointers: dim rows of ldr / ldq floats; results k x Q.
 // level > 0: an escalation pass over queries that failed the certificate.
 // up: the references are still on the host (ref is their device buffer,
 // filled chunk by chunk); FP32 32-query plans only.
 bool run_device(const float *ref, long long ldr, int R, const float *query, long long ldq, int Q,
                 int dim, int k, float *out_dist, int *out_idx, const Plan &p,
-                const knnhip_options &o, hipStream_t s, int level = 0, const Upload *up = nullptr);
+                const knnhip_options &o, hipStream_t s, int level = 0, const Upload *up = nullptr,
+                Deferred *dfr = nullptr);
+
+// Copies between pageable host arrays (rows of width 4-byte words, pitch
+// sld / dld words) and the device through pinned staging, ordered on stream
+// s (knn_hostio.hip).  host_upload returns once the pieces are queued and
+// the source is no longer read; host_download returns when dst holds the data.
+bool host_upload(void *dst, long long dld, const void *src, long long sld, long long width, int rows,
+                 hipStream_t s);
+bool host_download(void *dst, long long dld, const void *src, long long sld, long long width, int rows,
+                   hipStream_t s);
 
 // Searches larger than the device memory they may use (knn_stream.hip).
 // Device bytes of one host-pointer search of R references and Q queries,

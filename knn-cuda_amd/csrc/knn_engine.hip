@@ -264,14 +264,54 @@ Escalation g_esc[kMaxEscalation];
 
 }  // namespace
 
+// The nfail queries of d's search that failed the certificate (ids in
+// d.fb_list) are searched again through the scan with margin d.margin and
+// FP32 products, and their results scattered into d's outputs.
+bool escalate(const Deferred &d, int nfail) {
+    if (nfail <= 0) return true;
+    Escalation &e = g_esc[d.level];
+    auto need = [](DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); };
+    if (!need(e.list, (size_t)nfail * 4) || !need(e.qsub, (size_t)nfail * d.dim * 4) ||
+        !need(e.dsub, (size_t)nfail * d.k * 4) || !need(e.isub, (size_t)nfail * d.k * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    const hipStream_t s = d.s;
+    KNN_TRY(hipMemcpyAsync(e.list.p, d.fb_list, (size_t)nfail * 4, hipMemcpyDeviceToDevice, s), "copy");
+    KNN_TRY(idx_gather_columns(d.query, d.ldq, d.dim, e.list.as<int>(), nfail, e.qsub.as<float>(), s),
+            "gather");
+    knnhip_options o2 = d.o;
+    o2.margin = d.margin;
+    o2.force_fallback = 0;
+    o2.timing = 0;
+    o2.precision = 0;  // the rescans use FP32 products
+    const Plan p2 = make_plan(d.R, nfail, d.dim, d.k, o2);
+    if (!run_device(d.ref, d.ldr, d.R, e.qsub.as<float>(), nfail, nfail, d.dim, d.k, e.dsub.as<float>(),
+                    e.isub.as<int>(), p2, o2, s, d.level + 1))
+        return false;
+    KNN_TRY(idx_scatter_results(e.dsub.as<float>(), e.isub.as<int>(), d.k, nfail, e.list.as<int>(), d.Q,
+                                d.out_dist, d.out_idx, s),
+            "scatter");
+    g_last_fb = d.fb_count;
+    return true;
+}
+
+bool finish_device(Deferred &d) {
+    if (!d.pending) return true;
+    d.pending = false;
+    KNN_TRY(hipEventSynchronize(d.ev), "event sync");
+    return escalate(d, *d.nfail_host);
+}
+
 // Core: device pointers, dim rows of ldr / ldq floats.
 bool run_device(const float *ref, long long ldr, int R, const float *query, long long ldq, int Q,
                 int dim, int k, float *out_dist, int *out_idx, const Plan &p,
-                const knnhip_options &o, hipStream_t s, int level, const Upload *up) {
+                const knnhip_options &o, hipStream_t s, int level, const Upload *up, Deferred *dfr) {
     if (up && (!p.use_scan || p.lds || p.bf3 || up->chunks > kUploadChunks)) {
         set_error("chunked upload needs an FP32 32-query scan plan");
         return false;
     }
+    if (dfr) dfr->pending = false;
     const bool timing = o.timing != 0;
     if (timing && !g_ws.have_events) {
         for (auto &e : g_ws.ev) KNN_TRY(hipEventCreate(&e), "event create");
@@ -392,11 +432,9 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
                 const int s1 = (int)((long long)p.nslices * (g + 1) / G);
                 const int r0 = s0 * p.slice_len, r1 = std::min(R, s1 * p.slice_len);
                 if (s1 <= s0 || r1 <= r0) continue;
-                KNN_TRY(hipMemcpy2DAsync(const_cast<float *>(ref) + r0, (size_t)ldr * sizeof(float),
-                                         up->ref + r0, (size_t)up->ld * sizeof(float),
-                                         (size_t)(r1 - r0) * sizeof(float), dim, hipMemcpyHostToDevice,
-                                         up->stream),
-                        "Unable to copy data from host to device");
+                if (!host_upload(const_cast<float *>(ref) + r0, ldr, up->ref + r0, up->ld, r1 - r0, dim,
+                                 up->stream))
+                    return false;
                 KNN_TRY(hipEventRecord(g_ws.up_ev[g], up->stream), "event");
                 KNN_TRY(hipStreamWaitEvent(s, g_ws.up_ev[g], 0), "event wait");
                 NormArgs nc = na;
@@ -458,36 +496,43 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         int next_margin = 0;
         for (int l = level; l < kMaxEscalation && !next_margin && !o.force_fallback; ++l)
             if (kEscalationMargins[l] > cur_margin && k + cur_margin < R) next_margin = kEscalationMargins[l];
-        int nfail = 0;
         if (next_margin) {
-            KNN_TRY(hipMemcpyAsync(&nfail, fb_count, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-            KNN_TRY(hipStreamSynchronize(s), "sync");
-        }
-        if (next_margin && nfail > 0) {
-            Escalation &e = g_esc[level];
-            auto need = [](DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); };
-            if (!need(e.list, (size_t)nfail * 4) || !need(e.qsub, (size_t)nfail * dim * 4) ||
-                !need(e.dsub, (size_t)nfail * k * 4) || !need(e.isub, (size_t)nfail * k * 4)) {
-                set_error("Memory allocation error");
-                return false;
+            Deferred local;
+            Deferred &d = dfr ? *dfr : local;
+            d.ref = ref;
+            d.ldr = ldr;
+            d.R = R;
+            d.query = query;
+            d.ldq = ldq;
+            d.Q = Q;
+            d.dim = dim;
+            d.k = k;
+            d.out_dist = out_dist;
+            d.out_idx = out_idx;
+            d.fb_list = fb_list;
+            d.fb_count = fb_count;
+            d.o = o;
+            d.margin = next_margin;
+            d.level = level;
+            d.s = s;
+            if (dfr) {
+                // the caller reads the count back later (finish_device)
+                if (!d.nfail_host) {
+                    KNN_TRY(hipHostMalloc((void **)&d.nfail_host, sizeof(int), hipHostMallocDefault),
+                            "host alloc");
+                    KNN_TRY(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming), "event create");
+                }
+                KNN_TRY(hipMemcpyAsync(d.nfail_host, fb_count, sizeof(int), hipMemcpyDeviceToHost, s),
+                        "copy");
+                KNN_TRY(hipEventRecord(d.ev, s), "event");
+                d.pending = true;
+            } else {
+                int nfail = 0;
+                KNN_TRY(hipMemcpyAsync(&nfail, fb_count, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+                KNN_TRY(hipStreamSynchronize(s), "sync");
+                if (!escalate(d, nfail)) return false;
             }
-            KNN_TRY(hipMemcpyAsync(e.list.p, fb_list, (size_t)nfail * 4, hipMemcpyDeviceToDevice, s),
-                    "copy");
-            KNN_TRY(idx_gather_columns(query, ldq, dim, e.list.as<int>(), nfail, e.qsub.as<float>(), s),
-                    "gather");
-            knnhip_options o2 = o;
-            o2.margin = next_margin;
-            o2.force_fallback = 0;
-            o2.timing = 0;
-            o2.precision = 0;  // the rescans use FP32 products
-            const Plan p2 = make_plan(R, nfail, dim, k, o2);
-            if (!run_device(ref, ldr, R, e.qsub.as<float>(), nfail, nfail, dim, k, e.dsub.as<float>(),
-                            e.isub.as<int>(), p2, o2, s, level + 1))
-                return false;
-            KNN_TRY(idx_scatter_results(e.dsub.as<float>(), e.isub.as<int>(), k, nfail,
-                                        e.list.as<int>(), Q, out_dist, out_idx, s),
-                    "scatter");
-        } else if (!next_margin) {
+        } else {
             ea.list = fb_list;
             ea.count = fb_count;
             KNN_TRY(launch_exact(ea, egrid, s), "exact scan launch");
@@ -548,6 +593,7 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
         if (!g_ws.copy) {
             KNN_TRY(hipStreamCreateWithFlags(&g_ws.copy, hipStreamNonBlocking), "stream create");
             for (auto &e : g_ws.up_ev) KNN_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
+            KNN_TRY(hipEventCreateWithFlags(&g_ws.qry_ev, hipEventDisableTiming), "event create");
         }
     }
     g_last_plan = plan_json(p, R, Q, dim, k);
@@ -564,9 +610,17 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
         return false;
     }
     hipStream_t s = nullptr;
-    KNN_TRY(hipMemcpyAsync(qry_d, query, (size_t)dim * Q * sizeof(float), hipMemcpyHostToDevice,
-                           s),
-            "Unable to copy data from host to device");
+    if (chunked) {
+        // queries first on the upload stream, then the reference chunks
+        if (!host_upload(qry_d, (long long)dim * Q, query, (long long)dim * Q, (long long)dim * Q, 1,
+                         g_ws.copy))
+            return false;
+        KNN_TRY(hipEventRecord(g_ws.qry_ev, g_ws.copy), "event");
+        KNN_TRY(hipStreamWaitEvent(s, g_ws.qry_ev, 0), "event wait");
+    } else {
+        KNN_TRY(hipMemcpy(qry_d, query, (size_t)dim * Q * sizeof(float), hipMemcpyHostToDevice),
+                "Unable to copy data from host to device");
+    }
     if (chunked) {
         const Upload up{ref, R, g_ws.copy, kUploadChunks};
         if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s, 0, &up)) {
@@ -575,24 +629,21 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
             return false;
         }
     } else {
-        KNN_TRY(hipMemcpyAsync(ref_d, ref, (size_t)dim * R * sizeof(float), hipMemcpyHostToDevice, s),
+        KNN_TRY(hipMemcpy(ref_d, ref, (size_t)dim * R * sizeof(float), hipMemcpyHostToDevice),
                 "Unable to copy data from host to device");
         if (!run_device(ref_d, R, R, qry_d, Q, Q, dim, k, dist_d, idx_d, p, o, s)) return false;
     }
+    // synchronous copies on the null stream, behind the search
     if (kth_only) {
-        KNN_TRY(hipMemcpyAsync(knn_dist, dist_d + (size_t)(k - 1) * Q, (size_t)Q * sizeof(float),
-                               hipMemcpyDeviceToHost, s),
+        KNN_TRY(hipMemcpy(knn_dist, dist_d + (size_t)(k - 1) * Q, (size_t)Q * sizeof(float),
+                          hipMemcpyDeviceToHost),
                 "Unable to copy data from device to host");
-        KNN_TRY(hipStreamSynchronize(s), "Unable to execute kernel");
         return true;
     }
-    KNN_TRY(hipMemcpyAsync(knn_dist, dist_d, (size_t)k * Q * sizeof(float), hipMemcpyDeviceToHost,
-                           s),
+    KNN_TRY(hipMemcpy(knn_dist, dist_d, (size_t)k * Q * sizeof(float), hipMemcpyDeviceToHost),
             "Unable to copy data from device to host");
-    KNN_TRY(hipMemcpyAsync(knn_index, idx_d, (size_t)k * Q * sizeof(int), hipMemcpyDeviceToHost,
-                           s),
+    KNN_TRY(hipMemcpy(knn_index, idx_d, (size_t)k * Q * sizeof(int), hipMemcpyDeviceToHost),
             "Unable to copy data from device to host");
-    KNN_TRY(hipStreamSynchronize(s), "Unable to execute kernel");
     return true;
 }
 

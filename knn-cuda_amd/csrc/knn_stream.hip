@@ -15,12 +15,13 @@
 //   - the running result and the chunk's are merged by (distance, global
 //     index), so after the last chunk each query holds the k smallest pairs of
 //     the whole set under the same order, which is the reference's answer;
-//   - while chunk c is searched, chunk c + 1 is copied into the other buffer
-//     on the upload stream from a helper thread (a pageable host copy holds
-//     the thread that issues it until the data is staged).
+//   - chunk c's search is queued without waiting for its certificate count,
+//     then chunk c + 1's copy into the other buffer is issued on the upload
+//     stream (a pageable host copy holds the calling thread until the data
+//     is staged, while the device searches chunk c), and only then are chunk
+//     c's failed queries, if any, searched again.
 #include <algorithm>
 #include <cmath>
-#include <future>
 #include <string>
 
 #include "knn_engine.h"
@@ -69,8 +70,9 @@ __global__ __launch_bounds__(kMergeThreads) void merge_topk_kernel(
 
 struct StreamBufs {
     DevBuf ref[2], qry, dist[3], idx[3];
-    hipEvent_t loaded[2] = {}, freed[2] = {};
+    hipEvent_t loaded[2] = {}, freed[2] = {}, qloaded = {}, qfree = {}, merged = {};
     bool have_events = false;
+    Deferred dfr;  // certificate read-back of the chunk being searched
     void release() {
         for (auto &b : ref) b.release();
         qry.release();
@@ -213,74 +215,68 @@ bool run_streamed(const float *ref, int R, const float *query, int Q, int dim, i
     if (!g_ws.copy) {
         KNN_TRY(hipStreamCreateWithFlags(&g_ws.copy, hipStreamNonBlocking), "stream create");
         for (auto &e : g_ws.up_ev) KNN_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
+        KNN_TRY(hipEventCreateWithFlags(&g_ws.qry_ev, hipEventDisableTiming), "event create");
     }
     if (!g_sb.have_events) {
         for (int b = 0; b < 2; ++b) {
             KNN_TRY(hipEventCreateWithFlags(&g_sb.loaded[b], hipEventDisableTiming), "event create");
             KNN_TRY(hipEventCreateWithFlags(&g_sb.freed[b], hipEventDisableTiming), "event create");
         }
+        for (hipEvent_t *e : {&g_sb.qloaded, &g_sb.qfree, &g_sb.merged})
+            KNN_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming), "event create");
         g_sb.have_events = true;
     }
     hipStream_t s = nullptr, cs = g_ws.copy;
-    int dev = 0;
-    KNN_TRY(hipGetDevice(&dev), "device");
     for (int b = 0; b < 2; ++b) KNN_TRY(hipEventRecord(g_sb.freed[b], s), "event");
 
     // Reference chunk c into buffer b on the upload stream, after the search
-    // that last read the buffer; issued from a helper thread.
-    auto copy_chunk = [=](int c, int b) {
-        return std::async(std::launch::async, [=]() -> hipError_t {
-            hipError_t e = hipSetDevice(dev);
-            const int r0 = (int)((long long)R * c / nr), r1 = (int)((long long)R * (c + 1) / nr);
-            if (e == hipSuccess) e = hipStreamWaitEvent(cs, g_sb.freed[b], 0);
-            if (e == hipSuccess)
-                e = hipMemcpy2DAsync(g_sb.ref[b].p, (size_t)Rc * 4, ref + r0, (size_t)R * 4,
-                                     (size_t)(r1 - r0) * 4, dim, hipMemcpyHostToDevice, cs);
-            if (e == hipSuccess) e = hipEventRecord(g_sb.loaded[b], cs);
-            return e;
-        });
+    // that last read the buffer.
+    auto copy_chunk = [&](int c, int b) -> bool {
+        const int r0 = r_at(c), r1 = r_at(c + 1);
+        KNN_TRY(hipStreamWaitEvent(cs, g_sb.freed[b], 0), "event wait");
+        // the chunk's rows packed at leading dimension r1 - r0
+        if (!host_upload(g_sb.ref[b].p, r1 - r0, ref + r0, R, r1 - r0, dim, cs)) return false;
+        KNN_TRY(hipEventRecord(g_sb.loaded[b], cs), "event");
+        return true;
     };
-    std::future<hipError_t> pending;
     // No return while a copy may still read the caller's buffer.
     struct Drain {
-        std::future<hipError_t> *f;
         hipStream_t cs;
-        ~Drain() {
-            if (f->valid()) (void)f->get();
-            (void)hipStreamSynchronize(cs);
-        }
-    } drain{&pending, cs};
+        ~Drain() { (void)hipStreamSynchronize(cs); }
+    } drain{cs};
 
-    if (nr == 1) {
-        KNN_TRY(hipMemcpy2DAsync(g_sb.ref[0].p, (size_t)Rc * 4, ref, (size_t)R * 4, (size_t)R * 4, dim,
-                                 hipMemcpyHostToDevice, s),
-                "Unable to copy data from host to device");
-    } else {
-        pending = copy_chunk(0, 0);
-    }
+    // Every copy between host and device goes on the upload stream, through
+    // pinned staging, in program order; events order the copies against the
+    // searches.
+    if (!copy_chunk(0, 0)) return false;
     const int total = nq * nr;
     int cur = 0;  // running result buffer
     for (int it = 0; it < total; ++it) {
-        const int qi = it / nr, c = it % nr, b = it % 2;
+        const int qi = it / nr, c = it % nr;
         const int q0 = q_at(qi), nqc = q_at(qi + 1) - q0;
         const int r0 = r_at(c), nrc = r_at(c + 1) - r0;
+        const int rb = nr > 1 ? it % 2 : 0;
         if (c == 0) {
-            KNN_TRY(hipMemcpy2DAsync(g_sb.qry.p, (size_t)nqc * 4, query + q0, (size_t)Q * 4,
-                                     (size_t)nqc * 4, dim, hipMemcpyHostToDevice, s),
-                    "Unable to copy data from host to device");
+            // the group's queries, once the previous group's searches are done with the buffer
+            KNN_TRY(hipEventRecord(g_sb.qfree, s), "event");
+            KNN_TRY(hipStreamWaitEvent(cs, g_sb.qfree, 0), "event wait");
+            if (!host_upload(g_sb.qry.p, nqc, query + q0, Q, nqc, dim, cs)) return false;
+            KNN_TRY(hipEventRecord(g_sb.qloaded, cs), "event");
+            KNN_TRY(hipStreamWaitEvent(s, g_sb.qloaded, 0), "event wait");
             cur = 0;
         }
-        const int rb = nr > 1 ? b : 0;
-        if (nr > 1) {
-            KNN_TRY(pending.get(), "Unable to copy data from host to device");
-            KNN_TRY(hipStreamWaitEvent(s, g_sb.loaded[rb], 0), "event wait");
-            if (it + 1 < total) pending = copy_chunk((it + 1) % nr, rb ^ 1);
-        }
+        KNN_TRY(hipStreamWaitEvent(s, g_sb.loaded[rb], 0), "event wait");
         const int out = c == 0 ? cur : 2;
         const Plan p = make_plan(nrc, nqc, dim, k, o);
-        if (!run_device(g_sb.ref[rb].as<float>(), Rc, nrc, g_sb.qry.as<float>(), nqc, nqc, dim, k,
-                        g_sb.dist[out].as<float>(), g_sb.idx[out].as<int>(), p, o, s))
+        // queued without waiting for the certificate count; the next chunk's
+        // copy is issued while this chunk is searched, then the failed
+        // queries (if any) are searched again
+        if (!run_device(g_sb.ref[rb].as<float>(), nrc, nrc, g_sb.qry.as<float>(), nqc, nqc, dim, k,
+                        g_sb.dist[out].as<float>(), g_sb.idx[out].as<int>(), p, o, s, 0, nullptr,
+                        &g_sb.dfr))
             return false;
+        if (nr > 1 && it + 1 < total && !copy_chunk((it + 1) % nr, rb ^ 1)) return false;
+        if (!finish_device(g_sb.dfr)) return false;
         if (nr > 1) KNN_TRY(hipEventRecord(g_sb.freed[rb], s), "event");
         if (c > 0) {
             hipLaunchKernelGGL(merge_topk_kernel, dim3((nqc + kMergeThreads - 1) / kMergeThreads),
@@ -291,21 +287,24 @@ bool run_streamed(const float *ref, int R, const float *query, int Q, int dim, i
             cur ^= 1;
         }
         if (c == nr - 1) {
+            // the group's results, behind its last merge
+            KNN_TRY(hipEventRecord(g_sb.merged, s), "event");
+            KNN_TRY(hipStreamWaitEvent(cs, g_sb.merged, 0), "event wait");
             const float *dd = g_sb.dist[cur].as<float>();
+            const int *ii = g_sb.idx[cur].as<int>();
             if (kth_only) {
-                KNN_TRY(hipMemcpyAsync(knn_dist + q0, dd + (size_t)(k - 1) * nqc, (size_t)nqc * 4,
-                                       hipMemcpyDeviceToHost, s),
-                        "Unable to copy data from device to host");
-            } else {
-                KNN_TRY(hipMemcpy2DAsync(knn_dist + q0, (size_t)Q * 4, dd, (size_t)nqc * 4,
-                                         (size_t)nqc * 4, k, hipMemcpyDeviceToHost, s),
-                        "Unable to copy data from device to host");
-                KNN_TRY(hipMemcpy2DAsync(knn_index + q0, (size_t)Q * 4, g_sb.idx[cur].p, (size_t)nqc * 4,
-                                         (size_t)nqc * 4, k, hipMemcpyDeviceToHost, s),
-                        "Unable to copy data from device to host");
+                if (!host_download(knn_dist + q0, Q, dd + (size_t)(k - 1) * nqc, nqc, nqc, 1, cs))
+                    return false;
+            } else if (!host_download(knn_dist + q0, Q, dd, nqc, nqc, k, cs) ||
+                       !host_download(knn_index + q0, Q, ii, nqc, nqc, k, cs)) {
+                return false;
             }
+            // the next group overwrites the result buffers
+            KNN_TRY(hipEventRecord(g_sb.qfree, cs), "event");
+            KNN_TRY(hipStreamWaitEvent(s, g_sb.qfree, 0), "event wait");
         }
     }
+    KNN_TRY(hipStreamSynchronize(cs), "Unable to copy data from device to host");
     KNN_TRY(hipStreamSynchronize(s), "Unable to execute kernel");
     return true;
 }

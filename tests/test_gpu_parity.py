@@ -236,6 +236,22 @@ def test_host_entry_chunked_upload(entry):
     assert np.array_equal(i[:, sample], gi) and np.array_equal(bits(d[:, sample]), bits(gd))
 
 
+def test_chunked_upload_back_to_back_different_data():
+    # two calls of one shape with different data: each chunk's scan must wait for its own copy,
+    # so a search that ran ahead of its upload would see the previous call's references
+    R, Q, D, k = 300001, 120, 32, 8
+    say(f"chunked upload back to back {R}x{Q}x{D} k={k}")
+    rng = np.random.default_rng(77)
+    for rep in range(3):
+        ref = rng.random((D, R), dtype=np.float32)
+        qry = rng.random((D, Q), dtype=np.float32)
+        d, i = knnhip.knn_cuda_global(ref, qry, k)
+        assert knnhip.last_plan().get("upload_chunks", 0) >= 2
+        dd, di = knnhip.search_device(torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda(), k)
+        assert np.array_equal(i, di.cpu().numpy()), f"call {rep}: indexes"
+        assert np.array_equal(bits(d), bits(dd.cpu().numpy())), f"call {rep}: distances"
+
+
 @pytest.mark.parametrize("shape", [(1200, 40, 1100, 10), (900, 33, 768, 100)])
 def test_wide_dims_take_chunked_operands(shape):
     # dims whose staged query rows do not fit in LDS beside the heaps take the chunked-register

@@ -93,6 +93,25 @@ def test_ties_across_chunk_boundaries():
     check_entry_points(ref, qry, k, nr, nq)
 
 
+def test_back_to_back_calls_with_different_data():
+    # the chunk and query buffers keep the previous call's contents: a search that ran ahead of
+    # its own copies would return the previous data's neighbours
+    R, Q, dim, k = 40000, 3000, 16, 10
+    budget, nr, nq = find_budget(R, Q, dim, k, lambda nr, nq: nr >= 3 and nq >= 2, largest=True)
+    say(f"budget {budget >> 20} MiB, {nr} reference chunks, {nq} query groups")
+    knnhip.set_device_budget(budget)
+    rng = np.random.default_rng(5)
+    for rep in range(3):
+        ref = rng.random((dim, R), dtype=np.float32)
+        qry = rng.random((dim, Q), dtype=np.float32)
+        gd, gi = oracle.knn_c(ref, qry, k)
+        d, i = knnhip.knn_cuda_global(ref, qry, k)
+        plan = knnhip.last_plan()
+        assert plan["ref_chunks"] == nr and plan["query_chunks"] == nq
+        assert np.array_equal(i, gi), f"call {rep}: {np.count_nonzero(i != gi)} index mismatches"
+        assert np.array_equal(bits(d), bits(gd)), f"call {rep}: distances"
+
+
 def test_budget_too_small_is_an_error():
     ref = np.zeros((16, 5000), dtype=np.float32)
     qry = np.zeros((16, 10), dtype=np.float32)
