@@ -16,10 +16,10 @@
 //     index), so after the last chunk each query holds the k smallest pairs of
 //     the whole set under the same order, which is the reference's answer;
 //   - chunk c's search is queued without waiting for its certificate count,
-//     then chunk c + 1's copy into the other buffer is issued on the upload
-//     stream (a pageable host copy holds the calling thread until the data
-//     is staged, while the device searches chunk c), and only then are chunk
-//     c's failed queries, if any, searched again.
+//     then the host gathers chunk c + 1 into pinned staging and queues its
+//     DMA into the other buffer on the upload stream (host_upload) while the
+//     device searches chunk c, and only then are chunk c's failed queries,
+//     if any, searched again.
 #include <algorithm>
 #include <cmath>
 #include <string>
