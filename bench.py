@@ -173,18 +173,28 @@ def router_form(args, knnhip, torch):
     train_s = time.perf_counter() - t
     # the router picks each query's first probe (scanned first) and the bounds rank the rest;
     # "all": the router ranks every probe
+    # "first (fused)": the router handle, its first probe from the fused inference kernel (the
+    # three layers and the argmax in one launch, no score matrix)
     mlp_rows = []
-    for prec, score_all in (("fp32", False), ("bf16x3", False), ("bf16x3", True)):
+    for prec, mode in (("fp32", "first"), ("bf16x3", "first"), ("bf16x3", "all"), ("fp32", "first (fused)"),
+                       ("bf16x3", "first (fused)")):
         P = min((r for r in rows if r["precision"] == prec), key=lambda r: r["ms"])["probes"]
-        ms_m, (d, i) = timed(lambda: ix.search(qry, k, probes=P, scores=mlp.scores(qry), precision=prec,
-                                               score_all=score_all), args.router_reps)
-        mlp_rows.append({"precision": prec, "scores_rank": "all" if score_all else "first", "probes": P,
+        if mode == "first (fused)":
+            fn = lambda: ix.search(qry, k, probes=P, router=mlp, precision=prec)
+        else:
+            fn = lambda: ix.search(qry, k, probes=P, scores=mlp.scores(qry), precision=prec,
+                                   score_all=mode == "all")
+        ms_m, (d, i) = timed(fn, args.router_reps)
+        mlp_rows.append({"precision": prec, "scores_rank": mode, "probes": P,
                          "ms": ms_m, "queries_per_s": Q / ms_m * 1e3, "speedup_vs_plain": plain_ms / ms_m,
                          "certificate_failures": knnhip.last_fallback_count(),
                          "equal_to_plain": bool(torch.equal(i, pi) and torch.equal(d, pd))})
+    # the router's own inference: full score matrix, against the fused first-probe kernel
+    scores_ms, _ = timed(lambda: mlp.scores(qry), args.router_reps)
+    top1_ms, _ = timed(lambda: mlp.top1(qry), args.router_reps)
     mlp.close()
     ix.close()
-    bf = mlp_rows[1]
+    bf = min((r for r in mlp_rows if r["precision"] == "bf16x3"), key=lambda r: r["ms"])
     return {"workload": "4,194,304 refs x 16,384 queries, dim 96, k 16, mixture of 4,096 Gaussians "
                         "(sigma 0.1), 2,048 blocks",
             "plain_fp32_ms": plain_ms, "plain_queries_per_s": Q / plain_ms * 1e3,
@@ -192,11 +202,13 @@ def router_form(args, knnhip, torch):
             "best": {"precision": best["precision"], "probes": best["probes"], "ms": best["ms"],
                      "speedup_vs_plain": best["speedup_vs_plain"]},
             # the form the target names: bfloat16 three-term products, exact FP32 re-rank, learned router
-            "bf16_rerank_router": {"probes": bf["probes"], "ms": bf["ms"], "queries_per_s": bf["queries_per_s"],
+            "bf16_rerank_router": {"probes": bf["probes"], "scores_rank": bf["scores_rank"], "ms": bf["ms"],
+                                   "queries_per_s": bf["queries_per_s"],
                                    "speedup_vs_plain": bf["speedup_vs_plain"],
                                    "equal_to_plain": bf["equal_to_plain"]},
             "probes": rows,
             "mlp_router": {"train_s": train_s, "train_queries": 65536, "epochs": 8, "final_loss": loss,
+                           "inference_ms": {"score_matrix": scores_ms, "fused_first_probe": top1_ms},
                            "layers": [D, 256, 256, NB], "dtype": "bf16 products, fp32 accumulation",
                            "searches": mlp_rows}}
 

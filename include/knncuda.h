@@ -198,6 +198,19 @@ int knnhip_router_load(const char *path, knnhip_router **out);
 /* Block scores of query_nb queries into scores (nblocks x query_nb). */
 int knnhip_router_scores(knnhip_router *router, const float *query, int query_nb,
                          long long query_ld, float *scores, void *stream);
+/* First probe of each query: the block with the highest score, from one fused
+ * kernel (the three layers run back to back per 64 queries, bfloat16
+ * products, and only the argmax leaves the kernel; ties keep the lower
+ * block).  top1: query_nb ints on the device.  Needs hidden 64, 128 or 256
+ * and dim <= 512.  No counterpart in the reference. */
+int knnhip_router_top1(knnhip_router *router, const float *query, int query_nb, long long query_ld,
+                       int *top1, void *stream);
+/* knnhip_index_search with the router's first probe for each query taken
+ * from the fused kernel above (options.score_all = 1: the router's scores
+ * rank every probe).  The router must match the index (dim, nblocks). */
+int knnhip_index_search_routed(knnhip_index *index, knnhip_router *router, const float *query,
+                               int query_nb, long long query_ld, int k, int probes, float *knn_dist,
+                               int *knn_index, const knnhip_options *opt, void *stream);
 
 /*
  * Batched form.  No counterpart in the reference.  Runs many independent

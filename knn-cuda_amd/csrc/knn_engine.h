@@ -215,6 +215,14 @@ bool run_streamed(const float *ref, int R, const float *query, int Q, int dim, i
                   float *knn_dist, int *knn_index, const knnhip_options &o, size_t budget,
                   bool kth_only);
 
+// Learned router inference (knn_router_host.hip): the first probe of each
+// query, into top1 (nullable) and/or lb[top1 * Q + q] = -inf (nullable).
+bool router_pick_first(knnhip_router *r, const float *q, int Q, long long ld, const int *counts,
+                       int *top1, float *lb, hipStream_t s);
+bool router_scores(knnhip_router *r, const float *q, int Q, long long ld, float *scores, hipStream_t s);
+int router_dim(const knnhip_router *r);
+int router_nblocks(const knnhip_router *r);
+
 // Whole-file readers and writers of the index and router file formats:
 // `ok` turns false at the first short read or write and stays false.
 struct FileOut {

@@ -365,7 +365,7 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
 // Workspace of index searches (grown on demand, kept between calls).
 struct IndexWork {
     DevBuf qnorm, qpm, lb, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks,
-        pval, pidx, theta, fb_count, fb_list, qsub, dsub, isub;
+        pval, pidx, theta, fb_count, fb_list, qsub, dsub, isub, scores;
     bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
 IndexWork g_iw, g_iw_retry;
@@ -377,7 +377,7 @@ constexpr int kRetryProbes = 32;
 // depth 0: a caller's search; depth 1: the retry of its failed queries.
 bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query, int Q,
                   long long ldq, int k, int P, const float *router_scores, float *out_dist,
-                  int *out_idx, const knnhip_options &o, hipStream_t s) {
+                  int *out_idx, const knnhip_options &o, hipStream_t s, knnhip_router *router = nullptr) {
     const int nb = ix->nb, dim = ix->dim, pm_ld = ix->pm_ld;
     // precision scan: bfloat16 three-term products for dimensions up to 128
     // (else FP32 products)
@@ -459,11 +459,24 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                         ix->nf, ix->fnorm.as<float>(), ix->fsq.as<float>(), ix->fradf.as<float>(),
                         ix->fblock.as<int>(), ix->fb_start.as<int>(), nb, w.lb.as<float>(), s),
             "block bounds");
-    // scores pick the first probe (or rank every probe, with score_all)
+    // scores pick the first probe (or rank every probe, with score_all); a
+    // router handle picks it with the fused inference kernel
+    if (router && o.score_all) {
+        if (w.scores.n < (size_t)nb * Q * 4 && !w.scores.alloc((size_t)nb * Q * 4)) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        if (!knnhip::router_scores(router, query, Q, ldq, w.scores.as<float>(), s)) return false;
+        router_scores = w.scores.as<float>();
+    }
     const float *score = o.score_all ? router_scores : nullptr;
-    if (router_scores && !o.score_all)
+    if (router && !o.score_all) {
+        if (!router_pick_first(router, query, Q, ldq, ix->block_count.as<int>(), nullptr, w.lb.as<float>(), s))
+            return false;
+    } else if (router_scores && !o.score_all) {
         KNN_TRY(idx_router_first(router_scores, Q, nb, ix->block_count.as<int>(), w.lb.as<float>(), s),
                 "router first probe");
+    }
     if (timing) KNN_TRY(hipEventRecord(evr[1], s), "event");
     KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
                             w.chosen.as<int>(), w.lb_rest.as<float>(), s),
@@ -673,6 +686,41 @@ int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long
     g_last_plan = buf;
     return index_search(ix, g_iw, 0, query, query_nb, query_ld, k, probes, scores, knn_dist,
                         knn_index, o, (hipStream_t)stream)
+               ? 1
+               : 0;
+}
+
+int knnhip_index_search_routed(knnhip_index *ix, knnhip_router *router, const float *query,
+                               int query_nb, long long query_ld, int k, int probes, float *knn_dist,
+                               int *knn_index, const knnhip_options *opt, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ix || !router) {
+        set_error("null index or router");
+        return 0;
+    }
+    if (!check_args(ix->ref, ix->R, query, query_nb, ix->dim, k, knn_dist, knn_index)) return 0;
+    if (query_ld < query_nb || probes < 1) {
+        set_error("bad search arguments");
+        return 0;
+    }
+    if (router_dim(router) != ix->dim || router_nblocks(router) != ix->nb) {
+        set_error("router (dim %d, %d blocks) does not match the index (dim %d, %d blocks)",
+                  router_dim(router), router_nblocks(router), ix->dim, ix->nb);
+        return 0;
+    }
+    if (!ensure_device()) return 0;
+    knnhip_options o = opt ? *opt : default_options(0);
+    char buf[256];
+    const bool bf3 = o.precision == 1 && scan_bf3_task_nch(ix->dim) != 0;
+    snprintf(buf, sizeof(buf),
+             "{\"index_blocks\": %d, \"probes\": %d, \"router\": true, \"router_kernel\": \"%s\", "
+             "\"scores_rank\": \"%s\", \"query_nb\": %d, \"k\": %d, \"precision\": \"%s\"}",
+             ix->nb, std::min(probes, ix->nb), o.score_all ? "scores" : "fused first probe",
+             o.score_all ? "all" : "first", query_nb, k, bf3 ? "bf16x3" : "fp32");
+    g_last_plan = buf;
+    return index_search(ix, g_iw, 0, query, query_nb, query_ld, k, probes, nullptr, knn_dist, knn_index,
+                        o, (hipStream_t)stream, router)
                ? 1
                : 0;
 }

@@ -34,6 +34,26 @@ struct RouterLayout {
     long long total;
 };
 
+// Fused inference (first probe per query): queries x (dim-major, ldx), the
+// bfloat16 weight copy (W1 rows padded to dpad, W2 and W3 rows of H), FP32
+// biases; counts (nullable) marks empty blocks; outputs top1[q] (nullable)
+// and lb[top1*Q + q] = -inf (nullable).
+struct RouterFirstArgs {
+    const float *x;
+    long long ldx;
+    int Q, dim, dpad;
+    const unsigned short *w1, *w2, *w3;
+    const float *b1, *b2, *b3;
+    int nb;
+    const int *counts;
+    int *top1;
+    float *lb;
+};
+bool router_first_supported(int dpad, int H);
+hipError_t launch_router_bf16_weights(const float *w, const RouterLayout &lay, int dim, int dpad, int H,
+                                      int nb, unsigned short *out, hipStream_t s);
+hipError_t launch_router_first_fused(const RouterFirstArgs &a, int H, hipStream_t s);
+
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
 hipError_t launch_softmax_ce(const float *Z, int n, int nb, const int *tgt, long long ldt, int k,
                              float scale, float *dZ, float *loss, hipStream_t s);

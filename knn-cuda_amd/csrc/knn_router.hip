@@ -305,4 +305,203 @@ hipError_t launch_sum(const float *x, int n, float *out, hipStream_t s) {
     return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// Inference: the first probe of each query is the non-empty block with the
+// highest router score.  A bfloat16 copy of the weights (rows padded to 16
+// elements, zero-filled) feeds one fused kernel per 64 queries: the three
+// layers run back to back with the activations in LDS (bfloat16, as the
+// training forward rounds them), and the last layer keeps a running argmax
+// per query instead of writing the nb x Q score matrix.  Ties and NaN follow
+// router_first_kernel: NaN scores count as -inf, equal scores keep the lower
+// block.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
+__global__ void router_bf16_weights_kernel(const float *__restrict__ w, RouterLayout lay, int dim,
+                                           int dpad, int H, int nb, unsigned short *__restrict__ out) {
+    const long long n1 = (long long)H * dpad, n2 = (long long)H * H, n3 = (long long)nb * H;
+    const long long n = n1 + n2 + n3;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        float v;
+        if (e < n1) {
+            const long long r = e / dpad, c = e - r * dpad;
+            v = c < dim ? w[lay.w_off[0] + r * dim + c] : 0.0f;
+        } else if (e < n1 + n2) {
+            v = w[lay.w_off[1] + (e - n1)];
+        } else {
+            v = w[lay.w_off[2] + (e - n1 - n2)];
+        }
+        out[e] = to_bf16(v);
+    }
+}
+
+constexpr int kRtQueries = 64;  // queries per workgroup (two 32-column tiles)
+constexpr int kRtWaves = 8;
+
+__device__ inline bool rt_better(float s, int b, float bs, int bb) {
+    return bb < 0 || s > bs || (s == bs && b < bb);
+}
+
+// out[n][m] = bf16(relu(sum_k W[m][k] in[n][k] + bias[m])) for the block's
+// 64 queries n and H units m; in / out are LDS rows of pitch ip / op.
+template <int KH>
+__device__ inline void rt_layer(const unsigned short *__restrict__ W, int wp, int ksteps,
+                                const float *__restrict__ bias, const unsigned short *in, int ip,
+                                unsigned short *out, int op, int wave, int lane) {
+    constexpr int H = 16 * KH;
+    const int h = lane >> 5, col = lane & 31;
+    for (int t = wave; t < (H / 32) * 2; t += kRtWaves) {
+        const int ut = t >> 1, qt = t & 1;
+        const unsigned short *wr = W + (long long)(ut * 32 + col) * wp + 8 * h;
+        const unsigned short *xr = in + (qt * 32 + col) * ip + 8 * h;
+        f32x16_t acc = {};
+        for (int ks = 0; ks < ksteps; ++ks) {
+            const u32x4_t av = *reinterpret_cast<const u32x4_t *>(wr + 16 * ks);
+            const u32x4_t bv = *reinterpret_cast<const u32x4_t *>(xr + 16 * ks);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, av),
+                                                          __builtin_bit_cast(bf16x8_t, bv), acc, 0, 0, 0);
+        }
+        // element i: unit ut*32 + 8(i/4) + 4h + i%4, query qt*32 + col
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int u0 = ut * 32 + 8 * g + 4 * h;
+            unsigned short v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = to_bf16(fmaxf(acc[4 * g + j] + bias[u0 + j], 0.0f));
+            uint2 pk;
+            pk.x = (unsigned)v[0] | ((unsigned)v[1] << 16);
+            pk.y = (unsigned)v[2] | ((unsigned)v[3] << 16);
+            *reinterpret_cast<uint2 *>(out + (qt * 32 + col) * op + u0) = pk;
+        }
+    }
+}
+
+template <int KH>
+__global__ __launch_bounds__(64 * kRtWaves) void router_first_fused_kernel(RouterFirstArgs a) {
+    constexpr int H = 16 * KH, hp = H + 8;
+    extern __shared__ __attribute__((aligned(16))) unsigned char rt_smem[];
+    const int xp = a.dpad + 8;
+    unsigned short *xs = reinterpret_cast<unsigned short *>(rt_smem);
+    unsigned short *h1 = xs + kRtQueries * xp;
+    unsigned short *h2 = h1 + kRtQueries * hp;
+    float *cs = reinterpret_cast<float *>(h2 + kRtQueries * hp);
+    int *cb = reinterpret_cast<int *>(cs + kRtWaves * 2 * kRtQueries);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+    const int q0 = blockIdx.x * kRtQueries;
+    // queries to LDS as bfloat16 rows (coalesced over queries)
+    for (int e = threadIdx.x; e < kRtQueries * a.dpad; e += 64 * kRtWaves) {
+        const int j = e % kRtQueries, d = e / kRtQueries;
+        const float v = (q0 + j < a.Q && d < a.dim) ? a.x[(long long)d * a.ldx + q0 + j] : 0.0f;
+        xs[j * xp + d] = to_bf16(v);
+    }
+    __syncthreads();
+    rt_layer<KH>(a.w1, a.dpad, a.dpad / 16, a.b1, xs, xp, h1, hp, wave, lane);
+    __syncthreads();
+    rt_layer<KH>(a.w2, H, KH, a.b2, h1, hp, h2, hp, wave, lane);
+    __syncthreads();
+    // last layer: the block's two query tiles as B operands, block tiles
+    // w, w + 4, ... as A; running best per (lane, query tile)
+    u32x4_t bq[2][KH];
+#pragma unroll
+    for (int qt = 0; qt < 2; ++qt)
+#pragma unroll
+        for (int ks = 0; ks < KH; ++ks)
+            bq[qt][ks] = *reinterpret_cast<const u32x4_t *>(h2 + (qt * 32 + col) * hp + 16 * ks + 8 * h);
+    float bs[2] = {-INFINITY, -INFINITY};
+    int bb[2] = {-1, -1};
+    const int ntb = (a.nb + 31) / 32;
+    for (int bt = wave; bt < ntb; bt += kRtWaves) {
+        const int row = min(bt * 32 + col, a.nb - 1);
+        const unsigned short *wr = a.w3 + (long long)row * H + 8 * h;
+        f32x16_t acc0 = {}, acc1 = {};
+#pragma unroll
+        for (int ks = 0; ks < KH; ++ks) {
+            const bf16x8_t av = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t *>(wr + 16 * ks));
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8_t, bq[0][ks]), acc0,
+                                                           0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8_t, bq[1][ks]), acc1,
+                                                           0, 0, 0);
+        }
+        // element i: block bt*32 + 8(i/4) + 4h + i%4 (increasing in i)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int b = bt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+            if (b >= a.nb || (a.counts && a.counts[b] == 0)) continue;
+            const float b3 = a.b3[b];
+            float s0 = acc0[i] + b3, s1 = acc1[i] + b3;
+            s0 = s0 == s0 ? s0 : -INFINITY;
+            s1 = s1 == s1 ? s1 : -INFINITY;
+            if (rt_better(s0, b, bs[0], bb[0])) {
+                bs[0] = s0;
+                bb[0] = b;
+            }
+            if (rt_better(s1, b, bs[1], bb[1])) {
+                bs[1] = s1;
+                bb[1] = b;
+            }
+        }
+    }
+#pragma unroll
+    for (int qt = 0; qt < 2; ++qt) {
+        cs[(wave * 2 + h) * kRtQueries + qt * 32 + col] = bs[qt];
+        cb[(wave * 2 + h) * kRtQueries + qt * 32 + col] = bb[qt];
+    }
+    __syncthreads();
+    if (threadIdx.x < kRtQueries && q0 + (int)threadIdx.x < a.Q) {
+        const int j = threadIdx.x, q = q0 + j;
+        float s = -INFINITY;
+        int b = -1;
+        for (int c = 0; c < kRtWaves * 2; ++c) {
+            const int cbb = cb[c * kRtQueries + j];
+            if (cbb >= 0 && rt_better(cs[c * kRtQueries + j], cbb, s, b)) {
+                s = cs[c * kRtQueries + j];
+                b = cbb;
+            }
+        }
+        if (a.top1) a.top1[q] = b;
+        if (a.lb && b >= 0) a.lb[(long long)b * a.Q + q] = -INFINITY;
+    }
+}
+
+size_t router_first_lds(int dpad, int H) {
+    return (size_t)kRtQueries * (dpad + 8) * 2 + 2 * (size_t)kRtQueries * (H + 8) * 2 +
+           (size_t)kRtWaves * 2 * kRtQueries * 8;
+}
+
+bool router_first_supported(int dpad, int H) {
+    return (H == 64 || H == 128 || H == 256) && dpad <= 512 && router_first_lds(dpad, H) <= 160 * 1024;
+}
+
+hipError_t launch_router_bf16_weights(const float *w, const RouterLayout &lay, int dim, int dpad, int H,
+                                      int nb, unsigned short *out, hipStream_t s) {
+    const long long n = (long long)H * dpad + (long long)H * H + (long long)nb * H;
+    hipLaunchKernelGGL(router_bf16_weights_kernel, dim3(grid_cap(n)), dim3(256), 0, s, w, lay, dim, dpad,
+                       H, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_router_first_fused(const RouterFirstArgs &a, int H, hipStream_t s) {
+    const size_t lds = router_first_lds(a.dpad, H);
+    const dim3 grid((a.Q + kRtQueries - 1) / kRtQueries);
+#define RT(KHV)                                                                                   \
+    if (H == 16 * KHV) {                                                                          \
+        if (lds > 65536) {                                                                        \
+            const hipError_t e = hipFuncSetAttribute((const void *)router_first_fused_kernel<KHV>, \
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                                     (int)lds);                                    \
+            if (e != hipSuccess) return e;                                                        \
+        }                                                                                         \
+        hipLaunchKernelGGL(router_first_fused_kernel<KHV>, grid, dim3(64 * kRtWaves), lds, s, a);  \
+        return hipGetLastError();                                                                 \
+    }
+    RT(4)
+    RT(8)
+    RT(16)
+#undef RT
+    return hipErrorInvalidValue;
+}
+
 }  // namespace knnhip

@@ -20,6 +20,11 @@ struct knnhip_router {
     int step = 0;
     // batch workspace (grown on demand)
     knnhip::DevBuf h1, h2, z3, dz3, d2, d1, loss, lsum;
+    // bfloat16 weights of the fused inference kernel, converted again
+    // whenever the parameters have changed (gen counts the changes)
+    knnhip::DevBuf wbf;
+    int gen = 0, wbf_gen = -1;
+    knnhip::DevBuf scores;  // score matrix where the fused kernel does not apply
 };
 
 namespace knnhip {
@@ -141,11 +146,14 @@ bool router_step(knnhip_router *r, const float *x, int n, long long ld, const in
                  long long ldt, int k, float lr, hipStream_t s) {
     if (!router_loss_grad(r, x, n, ld, tgt, ldt, k, s)) return false;
     r->step += 1;
+    r->gen += 1;
     KNN_TRY(launch_adam(r->params.as<float>(), r->grad.as<float>(), r->m.as<float>(),
                         r->v.as<float>(), r->lay.total, lr, kAdamB1, kAdamB2, kAdamEps, r->step, s),
             "adam");
     return true;
 }
+
+}  // namespace
 
 // Block scores of Q queries (dim-major, ld) into scores[b*Q + q].
 bool router_scores(knnhip_router *r, const float *q, int Q, long long ld, float *scores,
@@ -159,6 +167,69 @@ bool router_scores(knnhip_router *r, const float *q, int Q, long long ld, float 
     KNN_TRY(launch_gemm(g, s), "router scores");
     return true;
 }
+
+int router_dim(const knnhip_router *r) { return r->dim; }
+int router_nblocks(const knnhip_router *r) { return r->nb; }
+
+// First probe of each of Q queries (dim-major, ld): the non-empty block
+// (counts[b] != 0; counts may be null) with the highest score, into top1
+// and/or as lb[top1 * Q + q] = -inf.  One fused kernel where the router's
+// shape allows it (hidden 64, 128 or 256; dim up to 512), else the score
+// matrix and the index's first-probe kernel (lb only).
+bool router_pick_first(knnhip_router *r, const float *q, int Q, long long ld, const int *counts,
+                       int *top1, float *lb, hipStream_t s) {
+    const int dpad = (r->dim + 15) / 16 * 16;
+    if (router_first_supported(dpad, r->H)) {
+        if (r->wbf_gen != r->gen) {
+            const size_t n = (size_t)r->H * dpad + (size_t)r->H * r->H + (size_t)r->nb * r->H;
+            if (r->wbf.n < n * 2 && !r->wbf.alloc(n * 2)) {
+                set_error("Memory allocation error");
+                return false;
+            }
+            KNN_TRY(launch_router_bf16_weights(r->params.as<float>(), r->lay, r->dim, dpad, r->H, r->nb,
+                                               r->wbf.as<unsigned short>(), s),
+                    "router weights");
+            r->wbf_gen = r->gen;
+        }
+        const float *w = r->params.as<float>();
+        const unsigned short *wb = r->wbf.as<unsigned short>();
+        RouterFirstArgs a{};
+        a.x = q;
+        a.ldx = ld;
+        a.Q = Q;
+        a.dim = r->dim;
+        a.dpad = dpad;
+        a.w1 = wb;
+        a.w2 = wb + (size_t)r->H * dpad;
+        a.w3 = a.w2 + (size_t)r->H * r->H;
+        a.b1 = w + r->lay.b_off[0];
+        a.b2 = w + r->lay.b_off[1];
+        a.b3 = w + r->lay.b_off[2];
+        a.nb = r->nb;
+        a.counts = counts;
+        a.top1 = top1;
+        a.lb = lb;
+        KNN_TRY(launch_router_first_fused(a, r->H, s), "router first probe");
+        return true;
+    }
+    if (top1) {
+        set_error("router shape (dim %d, hidden %d) has no fused first-probe kernel", r->dim, r->H);
+        return false;
+    }
+    if (!counts) {
+        set_error("block counts needed");
+        return false;
+    }
+    if (r->scores.n < (size_t)r->nb * Q * 4 && !r->scores.alloc((size_t)r->nb * Q * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    if (!router_scores(r, q, Q, ld, r->scores.as<float>(), s)) return false;
+    KNN_TRY(idx_router_first(r->scores.as<float>(), Q, r->nb, counts, lb, s), "router first probe");
+    return true;
+}
+
+namespace {
 
 // Training: exact neighbours of the training queries (plain engine), their
 // blocks as targets, then `epochs` passes of shuffled mini-batches.
@@ -353,6 +424,7 @@ int knnhip_router_params(knnhip_router *r, float *host, long long n, int set) {
     }
     if (set) {
         r->step = 0;
+        r->gen += 1;
         if (hipMemset(r->m.p, 0, (size_t)n * 4) != hipSuccess ||
             hipMemset(r->v.p, 0, (size_t)n * 4) != hipSuccess) {
             set_error("router optimiser reset failed");
@@ -414,6 +486,19 @@ int knnhip_router_train(knnhip_router *r, const knnhip_index *ix, const float *t
                         final_loss, (hipStream_t)stream)
                ? 1
                : 0;
+}
+
+int knnhip_router_top1(knnhip_router *r, const float *query, int query_nb, long long query_ld,
+                       int *top1, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!r || !query || !top1 || query_nb < 1 || query_ld < query_nb) {
+        set_error("bad router top1 arguments");
+        return 0;
+    }
+    if (!ensure_device()) return 0;
+    return router_pick_first(r, query, query_nb, query_ld, nullptr, top1, nullptr, (hipStream_t)stream) ? 1
+                                                                                                          : 0;
 }
 
 int knnhip_router_scores(knnhip_router *r, const float *query, int query_nb, long long query_ld,

@@ -63,6 +63,8 @@ C_SYMBOLS = (
     "knnhip_batched_autotune",
     "knnhip_batched_table",
     "knnhip_batched_set_slice",
+    "knnhip_router_top1",
+    "knnhip_index_search_routed",
     "knnhip_set_device_budget",
     "knnhip_device_budget",
     "knnhip_stream_chunks",
@@ -173,6 +175,11 @@ def load() -> ctypes.CDLL:
     lib.knnhip_router_train.restype = i32
     lib.knnhip_router_scores.argtypes = [vp, vp, i32, i64, vp, vp]
     lib.knnhip_router_scores.restype = i32
+    lib.knnhip_router_top1.argtypes = [vp, vp, i32, i64, vp, vp]
+    lib.knnhip_router_top1.restype = i32
+    lib.knnhip_index_search_routed.argtypes = [vp, vp, vp, i32, i64, i32, i32, vp, vp,
+                                               ctypes.POINTER(Options), vp]
+    lib.knnhip_index_search_routed.restype = i32
     lib.knnhip_kth_distance.argtypes = [vp, i32, vp, i32, i32, i32, vp]
     lib.knnhip_kth_distance.restype = i32
     lib.knnhip_index_save.argtypes = [vp, ctypes.c_char_p]
@@ -424,7 +431,11 @@ class BlockIndex:
 
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
                margin: int = 0, force_fallback: bool = False, timing: bool = False,
-               precision: str = "fp32", score_all: bool = False, stream: Optional[int] = None):
+               precision: str = "fp32", score_all: bool = False, router=None,
+               stream: Optional[int] = None):
+        """Exact k-NN through the index.  ``scores`` (nblocks x query_nb) or ``router`` (a Router
+        of this index's shape, its first probe taken from the fused inference kernel) choose each
+        query's first probe; the lower bounds rank the rest."""
         import torch
 
         lib = load()
@@ -453,9 +464,16 @@ class BlockIndex:
         opt.score_all = 1 if score_all else 0
         if stream is None:
             stream = torch.cuda.current_stream(query.device).cuda_stream
-        ok = lib.knnhip_index_search(self._h, query.data_ptr(), query_nb, query.stride(0), k, probes,
-                                     sp, dist.data_ptr(), index.data_ptr(), ctypes.byref(opt),
-                                     ctypes.c_void_p(stream))
+        if router is not None:
+            if scores is not None:
+                raise KnnError("pass scores or a router, not both")
+            ok = lib.knnhip_index_search_routed(self._h, router._h, query.data_ptr(), query_nb,
+                                                query.stride(0), k, probes, dist.data_ptr(),
+                                                index.data_ptr(), ctypes.byref(opt), ctypes.c_void_p(stream))
+        else:
+            ok = lib.knnhip_index_search(self._h, query.data_ptr(), query_nb, query.stride(0), k, probes,
+                                         sp, dist.data_ptr(), index.data_ptr(), ctypes.byref(opt),
+                                         ctypes.c_void_p(stream))
         if not ok:
             raise KnnError(last_error())
         return dist, index
@@ -583,6 +601,18 @@ class Router:
                               device=query.device)
         ok = load().knnhip_router_scores(self._h, query.data_ptr(), query.shape[1],
                                          query.stride(0), out.data_ptr(), _stream_of(query, stream))
+        if not ok:
+            raise KnnError(last_error())
+        return out
+
+    def top1(self, query, out=None, stream: Optional[int] = None):
+        """Each query's highest-scoring block (int32, query_nb) from the fused inference kernel."""
+        import torch
+
+        if out is None:
+            out = torch.empty(query.shape[1], dtype=torch.int32, device=query.device)
+        ok = load().knnhip_router_top1(self._h, query.data_ptr(), query.shape[1], query.stride(0),
+                                       out.data_ptr(), _stream_of(query, stream))
         if not ok:
             raise KnnError(last_error())
         return out

@@ -102,6 +102,47 @@ def trained():
     ix.close()
 
 
+@pytest.mark.parametrize("shape", [(40, 64, 100, 300), (96, 256, 2048, 1000), (33, 128, 129, 64)])
+def test_fused_top1_is_the_score_argmax(shape):
+    # the fused first-probe kernel against the argmax of the score matrix: the two sum in a
+    # different order (and a hidden unit near a bfloat16 rounding boundary may round the other
+    # way), so the chosen block's score is within 4e-3 of the column's largest magnitude of the
+    # best score, and the block is the argmax wherever the best two differ by more than 8e-3
+    dim, H, nb, Q = shape
+    say(f"fused top1 dim={dim} H={H} nb={nb} Q={Q}")
+    rng = np.random.default_rng(dim + H)
+    r = knnhip.Router(dim, H, nb, seed=3)
+    params = r.get_params()
+    for name, (o, c) in groups(dim, H, nb).items():
+        if name.startswith("b"):
+            params[o:o + c] = rng.uniform(-0.2, 0.2, size=c).astype(np.float32)
+    r.set_params(params)
+    x = torch.from_numpy(rng.normal(size=(dim, Q)).astype(np.float32)).cuda()
+    top = r.top1(x).long()
+    sc = r.scores(x)
+    best = sc.max(dim=0)
+    picked = sc.gather(0, top.unsqueeze(0)).squeeze(0)
+    scale = sc.abs().max(dim=0).values
+    assert bool(((best.values - picked) <= 4e-3 * scale).all())
+    top2 = torch.topk(sc, 2, dim=0).values
+    clear = (top2[0] - top2[1]) > 8e-3 * scale
+    assert bool((top[clear] == best.indices[clear]).all())
+    say(f"  argmax agreement {(top == best.indices).float().mean().item():.4f}")
+    # parameters changed after the first call: the kernel's bfloat16 weights follow
+    params[groups(dim, H, nb)["b3"][0]] += 100.0
+    r.set_params(params)
+    assert bool((r.top1(x) == 0).all())
+    r.close()
+
+
+def test_fused_top1_rejects_unsupported_hidden_width():
+    r = knnhip.Router(24, 40, 70, seed=1)
+    x = torch.zeros((24, 8), device="cuda")
+    with pytest.raises(knnhip.KnnError, match="fused"):
+        r.top1(x)
+    r.close()
+
+
 def test_training_reduces_loss(trained):
     ref, test_q, ix, r, k, nb, loss0, loss = trained
     assert loss < loss0
@@ -140,3 +181,21 @@ def test_router_search_is_exact_and_gains_over_centroid_ranking(trained):
             assert torch.equal(i, pi) and torch.equal(d, pd), f"{name}, probes {P}"
         say(f"probes {P}: certificate failures {res}")
     assert rec_r >= rec_c
+
+
+def test_routed_search_is_exact(trained):
+    # the router handle: its first probe from the fused kernel (or its scores ranking every
+    # probe), the lower bounds for the rest; results equal the plain search
+    ref, test_q, ix, r, k, nb, _, _ = trained
+    pd, pi = knnhip.search_device(ref, test_q, k)
+    for P in (1, 3):
+        for prec in ("fp32", "bf16x3"):
+            for all_ in (False, True):
+                d, i = ix.search(test_q, k, probes=P, router=r, precision=prec, score_all=all_)
+                plan = knnhip.last_plan()
+                assert plan["router_kernel"] == ("scores" if all_ else "fused first probe")
+                assert torch.equal(i, pi) and torch.equal(d, pd), f"probes {P} {prec} all={all_}"
+    first = r.top1(test_q).long()
+    start, count, ref_block = ix.layout()
+    hit = (ref_block[pi[0].long()] == first).float().mean().item()
+    say(f"nearest neighbour in the router's first probe: {hit:.4f}")
