@@ -139,6 +139,13 @@ int knnhip_index_search(knnhip_index *index, const float *query, int query_nb, l
                         const knnhip_options *opt, void *stream);
 void knnhip_index_free(knnhip_index *index);
 int knnhip_index_info(const knnhip_index *index, int *nblocks, int *n_pos, float *build_ms);
+/* The lower bounds a search ranks and prunes by: lb[b * query_nb + q] (device,
+ * nblocks x query_nb) <= the distance from query q to every point of block b
+ * (+inf for an empty block).  precision = 1: bfloat16 three-term products
+ * (dim <= 128), as searches with options.precision = 1 use.  For tests and
+ * diagnostics. */
+int knnhip_index_bounds(knnhip_index *index, const float *query, int query_nb, long long query_ld,
+                        int precision, float *lb, void *stream);
 /* Versioned file of an index's build (block layout, permutation, balls; see
  * knn_engine.hip for the byte layout).  Loading rebuilds the tiles from the
  * same reference set (ref_nb and dim must match the file). */

@@ -278,6 +278,10 @@ struct knnhip_index {
     // scan, built by the first search that asks for it; nch_bf chunks
     int nch_bf = 0;
     knnhip::DevBuf packed_bf;
+    // bfloat16 hi/lo ball centres for the three-term bound products, built
+    // by the first precision search; fcent_nch chunks
+    int fcent_nch = 0;
+    knnhip::DevBuf fcent_bf;
     // last search statistics
     int last_probes = 0;
     int last_fail = 0;

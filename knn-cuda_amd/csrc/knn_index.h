@@ -58,10 +58,16 @@ hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcen
                              hipStream_t s);
 hipError_t idx_ball_aux(const float *fnorm, const double *frad, int nf, float *fsq, float *fradf,
                         hipStream_t s);
+// cbf (nullable): bfloat16 hi/lo ball centres (idx_pack_ball_bf3) for
+// three-term bound products over nch_bf 16-dim chunks (dim <= 128), with
+// error coefficient g15_bf; else FP32 products.
 hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
                        const float *fcent, int nf, const float *fnorm, const float *fsq,
                        const float *fradf, const int *block_s, const int *fb_start, int nb, float *lb,
-                       hipStream_t s);
+                       hipStream_t s, const unsigned short *cbf = nullptr, int nch_bf = 0,
+                       float g15_bf = 0.0f);
+hipError_t idx_pack_ball_bf3(const float *fcent, int pm_ld, int nf, int dim, int nch, unsigned short *out,
+                             hipStream_t s);
 // Outputs of one problem of a batched class: its queries are columns
 // qstart .. qstart + qn - 1 of the class's k x Qtot results.
 struct BatchOut {

@@ -967,6 +967,28 @@ __device__ __forceinline__ float ball_error_coeff(int dim) {
 }
 
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
+// Ball centres as bfloat16 hi/lo pairs for the three-term bound products:
+// ball f, 16-dim chunk c at out[(f*nch + c)*32]: 16 hi values (dims 16c ..
+// 16c + 15, zero past dim), then the 16 lo values, lo = bf16(x - hi).
+__global__ void ball_bf3_pack_kernel(const float *__restrict__ fcent, int pm_ld, int nf, int dim, int nch,
+                                     unsigned short *__restrict__ out) {
+    const long long n = (long long)nf * nch * 16;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(e % 16);
+        const long long fc = e / 16;
+        const int c = (int)(fc % nch), f = (int)(fc / nch);
+        const int d = 16 * c + j;
+        const float x = d < dim ? fcent[(long long)f * pm_ld + d] : 0.0f;
+        const __bf16 hi = (__bf16)x;
+        const __bf16 lo = (__bf16)(x - (float)hi);
+        out[fc * 32 + j] = __builtin_bit_cast(unsigned short, hi);
+        out[fc * 32 + 16 + j] = __builtin_bit_cast(unsigned short, lo);
+    }
+}
 
 __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ qpm, int pm_ld, int Q,
                                                       int dim, const float *__restrict__ qnorm,
@@ -1081,7 +1103,13 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
 constexpr int kBallTilesPerWg = 8;  // tiles per workgroup for many queries
 constexpr int kBallMinWgs = 1024;    // workgroups aimed for when there are few
 
-template <int KHM>
+//
+// NCH > 0: the products are bfloat16 three-term (q_hi c_hi + q_hi c_lo +
+// q_lo c_hi per 16-dim chunk, v_mfma_f32_32x32x16_bf16, FP32 accumulation)
+// on the packed centres cbf; the accumulator layout is the FP32 MFMA's, so
+// the bounds and the fold are shared.  g15_bf then replaces the FP32 error
+// coefficient (the host's ball_error_coeff_bf3).
+template <int KHM, int NCH>
 __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restrict__ qpm, int pm_ld,
                                                           int Q, int dim,
                                                           const float *__restrict__ qnorm,
@@ -1091,7 +1119,9 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
                                                           const float *__restrict__ fradf,
                                                           const int *__restrict__ block_s,
                                                           const int *__restrict__ fb_start,
-                                                          float *__restrict__ lb, int tpw) {
+                                                          float *__restrict__ lb, int tpw,
+                                                          const unsigned short *__restrict__ cbf,
+                                                          float g15_bf) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
     const int ntf = (nf + 31) / 32;
@@ -1103,15 +1133,30 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
     const int F1 = min(nf, F0 + tpw * 32);
     const int KH = (dim + 1) / 2;
     const int qa = min(q0 + col, Q - 1);
-    // this lane's query half-row: dims h*KH .. h*KH + KH - 1
-    float4 bq[KHM / 4];
-    {
+    // this lane's query half-row: dims h*KH .. h*KH + KH - 1 (FP32), or its
+    // hi / lo parts of dims 16c + 8h .. + 7 per chunk c (bfloat16)
+    float4 bq[NCH ? 1 : KHM / 4];
+    bf16x8_t bh[NCH ? NCH : 1], bl[NCH ? NCH : 1];
+    if constexpr (NCH > 0) {
+        const float *qr = qpm + (long long)qa * pm_ld;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int d = 16 * c + 8 * h + e;
+                const float x = d < dim ? qr[d] : 0.0f;
+                const __bf16 hi = (__bf16)x;
+                bh[c][e] = hi;
+                bl[c][e] = (__bf16)(x - (float)hi);
+            }
+        }
+    } else {
         const float *qr = qpm + (long long)qa * pm_ld + h * KH;
 #pragma unroll
         for (int u = 0; u < KHM / 4; ++u)
             bq[u] = 4 * u < KH ? *reinterpret_cast<const float4 *>(qr + 4 * u) : make_float4(0, 0, 0, 0);
     }
-    const float g15 = ball_error_coeff(dim);
+    const float g15 = NCH > 0 ? g15_bf : ball_error_coeff(dim);
     const float nq = qnorm[qa];
     const float sq = nextafterf(sqrtf(nq), INFINITY);
     const int q = q0 + col;
@@ -1130,16 +1175,29 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
         const int f0 = F0 + tt * 32;
         if (f0 >= F1) break;  // uniform over the wave
         const int fa = min(f0 + col, nf - 1);
-        const float *ca = fcent + (long long)fa * pm_ld + h * KH;
         f32x16_t acc = {};
+        if constexpr (NCH > 0) {
+            const unsigned short *ca = cbf + (long long)fa * NCH * 32 + 8 * h;
 #pragma unroll
-        for (int u = 0; u < KHM / 4; ++u) {
-            if (4 * u < KH) {
-                const float4 av = *reinterpret_cast<const float4 *>(ca + 4 * u);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[u].x, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[u].y, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[u].z, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[u].w, acc, 0, 0, 0);
+            for (int c = 0; c < NCH; ++c) {
+                const bf16x8_t ah = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t *>(ca + 32 * c));
+                const bf16x8_t al =
+                    __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t *>(ca + 32 * c + 16));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[c], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[c], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[c], acc, 0, 0, 0);
+            }
+        } else {
+            const float *ca = fcent + (long long)fa * pm_ld + h * KH;
+#pragma unroll
+            for (int u = 0; u < KHM / 4; ++u) {
+                if (4 * u < KH) {
+                    const float4 av = *reinterpret_cast<const float4 *>(ca + 4 * u);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[u].x, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[u].y, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[u].z, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[u].w, acc, 0, 0, 0);
+                }
             }
         }
         // element i is ball f0 + 8(i/4) + 4h + i%4: four consecutive balls per
@@ -1367,23 +1425,43 @@ hipError_t idx_ball_aux(const float *fnorm, const double *frad, int nf, float *f
     hipLaunchKernelGGL(ball_aux_kernel, dim3(grid_for(nf)), dim3(256), 0, s, fnorm, frad, nf, fsq, fradf);
     return hipGetLastError();
 }
+hipError_t idx_pack_ball_bf3(const float *fcent, int pm_ld, int nf, int dim, int nch, unsigned short *out,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(ball_bf3_pack_kernel, dim3(grid_for((long long)nf * nch * 16)), dim3(256), 0, s,
+                       fcent, pm_ld, nf, dim, nch, out);
+    return hipGetLastError();
+}
+
 hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
                        const float *fcent, int nf, const float *fnorm, const float *fsq,
                        const float *fradf, const int *block_s, const int *fb_start, int nb, float *lb,
-                       hipStream_t s) {
+                       hipStream_t s, const unsigned short *cbf, int nch_bf, float g15_bf) {
     hipError_t e = hipMemsetD32Async((hipDeviceptr_t)lb, 0x7F800000, (size_t)nb * Q, s);
     if (e != hipSuccess) return e;
     const int KH = (dim + 1) / 2;
+    const long long qgs = (Q + 127) / 128, ntf = (nf + 31) / 32;
+    const int tpw = (int)std::max(1ll, std::min<long long>(kBallTilesPerWg, qgs * ntf / kBallMinWgs));
+    const long long wgs = qgs * ((ntf + tpw - 1) / tpw);
+    if (cbf && nch_bf >= 1 && nch_bf <= 8 && 16 * nch_bf >= dim) {
+#define BF(N)                                                                                          \
+    if (nch_bf == N) {                                                                                 \
+        hipLaunchKernelGGL((ball_lb_reg_kernel<4, N>), dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld, Q, \
+                           dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw, cbf,   \
+                           g15_bf);                                                                    \
+        return hipGetLastError();                                                                      \
+    }
+        BF(1) BF(2) BF(3) BF(4) BF(5) BF(6) BF(7) BF(8)
+#undef BF
+    }
     if (KH % 4 == 0 && pm_ld % 4 == 0 && KH <= 64) {
-        const long long qgs = (Q + 127) / 128, ntf = (nf + 31) / 32;
-        const int tpw = (int)std::max(1ll, std::min<long long>(kBallTilesPerWg, qgs * ntf / kBallMinWgs));
-        const long long wgs = qgs * ((ntf + tpw - 1) / tpw);
         if (KH <= 32)
-            hipLaunchKernelGGL(ball_lb_reg_kernel<32>, dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
-                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw);
+            hipLaunchKernelGGL((ball_lb_reg_kernel<32, 0>), dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
+                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw,
+                               nullptr, 0.0f);
         else
-            hipLaunchKernelGGL(ball_lb_reg_kernel<64>, dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
-                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw);
+            hipLaunchKernelGGL((ball_lb_reg_kernel<64, 0>), dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
+                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw,
+                               nullptr, 0.0f);
         return hipGetLastError();
     }
     const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);

@@ -374,6 +374,55 @@ IndexWork g_iw, g_iw_retry;
 // full scan (ranked by the lower bounds alone).
 constexpr int kRetryProbes = 32;
 
+// Error coefficient of the block bounds with bfloat16 three-term products
+// (replaces 1.5 g of the FP32 products in ball_lower_bound).  With u = 2^-8,
+// n = 48 nch product terms accumulated in FP32 (unit 2^-23, allowing for any
+// rounding inside the MFMA) and the FP32 norms of q and c (unit 2^-23 as
+// well):
+//   |nq + nc - 2 dot - |q - c|^2| <= g_D (|q|^2 + |c|^2)
+//                                   + 2 (3.01 u^2 + 1.02 g_n) |q| |c|
+//                                 <= (g_D + (3.01 u^2 + 1.02 g_n) / 2) S,
+// S = (|q| + |c|)^2, since 2 |q| |c| <= S / 2; widened by 5 % and rounded up.
+float ball_error_coeff_bf3(int dim, int nch) {
+    const double u2 = 1.0 / 65536.0;
+    const double n = 48.0 * nch;
+    const double gn = n * 0x1p-23 / (1.0 - n * 0x1p-23);
+    const double gd = dim * 0x1p-23 / (1.0 - dim * 0x1p-23);
+    const double c = 1.05 * (gd + 0.5 * (3.01 * u2 + 1.02 * gn));
+    return (float)c * (1.0f + 0x1p-20f);
+}
+
+// The bfloat16 ball centres of the precision bounds (built once per index).
+bool ball_bf3_ready(knnhip_index *ix, int nch, hipStream_t s) {
+    if (ix->fcent_nch == nch) return true;
+    ix->fcent_nch = 0;
+    if (!ix->fcent_bf.alloc((size_t)ix->nf * nch * 32 * 2)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(idx_pack_ball_bf3(ix->fcent.as<float>(), ix->pm_ld, ix->nf, ix->dim, nch,
+                              ix->fcent_bf.as<unsigned short>(), s),
+            "bfloat16 ball centres");
+    ix->fcent_nch = nch;
+    return true;
+}
+
+// Block lower bounds of the Q queries whose norms and point-major rows are
+// in qnorm / qpm, into lb (nb x Q, block-major); bfloat16 three-term
+// products with `precision` where the dimension allows (<= 128).
+bool index_bounds(knnhip_index *ix, const float *qpm, const float *qnorm, int Q, bool precision, float *lb,
+                  hipStream_t s) {
+    const int nch = precision && ix->dim <= 128 ? (ix->dim + 15) / 16 : 0;
+    if (nch && !ball_bf3_ready(ix, nch, s)) return false;
+    KNN_TRY(idx_ball_lb(qpm, ix->pm_ld, Q, ix->dim, qnorm, ix->fcent.as<float>(), ix->nf,
+                        ix->fnorm.as<float>(), ix->fsq.as<float>(), ix->fradf.as<float>(),
+                        ix->fblock.as<int>(), ix->fb_start.as<int>(), ix->nb, lb, s,
+                        nch ? ix->fcent_bf.as<unsigned short>() : nullptr, nch,
+                        nch ? ball_error_coeff_bf3(ix->dim, nch) : 0.0f),
+            "block bounds");
+    return true;
+}
+
 // depth 0: a caller's search; depth 1: the retry of its failed queries.
 bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query, int Q,
                   long long ldq, int k, int P, const float *router_scores, float *out_dist,
@@ -455,10 +504,8 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     if (timing) KNN_TRY(hipEventRecord(evr[0], s), "event");
     // block lower bounds (smallest over each block's balls), block-major;
     // without a router the ranking is by the negated bound (nearest first)
-    KNN_TRY(idx_ball_lb(w.qpm.as<float>(), pm_ld, Q, dim, w.qnorm.as<float>(), ix->fcent.as<float>(),
-                        ix->nf, ix->fnorm.as<float>(), ix->fsq.as<float>(), ix->fradf.as<float>(),
-                        ix->fblock.as<int>(), ix->fb_start.as<int>(), nb, w.lb.as<float>(), s),
-            "block bounds");
+    if (!index_bounds(ix, w.qpm.as<float>(), w.qnorm.as<float>(), Q, o.precision == 1, w.lb.as<float>(), s))
+        return false;
     // scores pick the first probe (or rank every probe, with score_all); a
     // router handle picks it with the fused inference kernel
     if (router && o.score_all) {
@@ -723,6 +770,38 @@ int knnhip_index_search_routed(knnhip_index *ix, knnhip_router *router, const fl
                         o, (hipStream_t)stream, router)
                ? 1
                : 0;
+}
+
+int knnhip_index_bounds(knnhip_index *ix, const float *query, int query_nb, long long query_ld,
+                        int precision, float *lb, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ix || !query || !lb || query_nb < 1 || query_ld < query_nb) {
+        set_error("bad bound arguments");
+        return 0;
+    }
+    if (!ensure_device()) return 0;
+    IndexWork &w = g_iw;
+    if (!w.need(w.qnorm, (size_t)query_nb * 4) || !w.need(w.qpm, (size_t)query_nb * ix->pm_ld * 4)) {
+        set_error("Memory allocation error");
+        return 0;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    NormArgs nq{};
+    nq.p = query;
+    nq.ld = query_ld;
+    nq.n = query_nb;
+    nq.n_pad = query_nb;
+    nq.dim = ix->dim;
+    nq.oct = (ix->dim + 7) / 8;
+    nq.out = w.qnorm.as<float>();
+    nq.pm = w.qpm.as<float>();
+    nq.pm_ld = ix->pm_ld;
+    if (launch_norms(nq, s) != hipSuccess) {
+        set_error("norm kernel launch");
+        return 0;
+    }
+    return index_bounds(ix, w.qpm.as<float>(), w.qnorm.as<float>(), query_nb, precision == 1, lb, s) ? 1 : 0;
 }
 
 void knnhip_index_free(knnhip_index *ix) {

@@ -64,6 +64,7 @@ C_SYMBOLS = (
     "knnhip_batched_table",
     "knnhip_batched_set_slice",
     "knnhip_router_top1",
+    "knnhip_index_bounds",
     "knnhip_index_search_routed",
     "knnhip_set_device_budget",
     "knnhip_device_budget",
@@ -175,6 +176,8 @@ def load() -> ctypes.CDLL:
     lib.knnhip_router_train.restype = i32
     lib.knnhip_router_scores.argtypes = [vp, vp, i32, i64, vp, vp]
     lib.knnhip_router_scores.restype = i32
+    lib.knnhip_index_bounds.argtypes = [vp, vp, i32, i64, i32, vp, vp]
+    lib.knnhip_index_bounds.restype = i32
     lib.knnhip_router_top1.argtypes = [vp, vp, i32, i64, vp, vp]
     lib.knnhip_router_top1.restype = i32
     lib.knnhip_index_search_routed.argtypes = [vp, vp, vp, i32, i64, i32, i32, vp, vp,
@@ -428,6 +431,20 @@ class BlockIndex:
             raise KnnError(last_error())
         torch.cuda.synchronize(self.ref.device)
         return start, count, ref_block
+
+    def bounds(self, query, precision: str = "fp32", stream: Optional[int] = None):
+        """The block lower bounds searches rank and prune by: (nblocks, query_nb) float32."""
+        import torch
+
+        if query.dtype != torch.float32 or not query.is_cuda or query.stride(1) != 1:
+            raise KnnError("query must be a float32 device tensor with contiguous rows")
+        out = torch.empty((self.nblocks, query.shape[1]), dtype=torch.float32, device=query.device)
+        ok = load().knnhip_index_bounds(self._h, query.data_ptr(), query.shape[1], query.stride(0),
+                                        1 if precision == "bf16x3" else 0, out.data_ptr(),
+                                        _stream_of(query, stream))
+        if not ok:
+            raise KnnError(last_error())
+        return out
 
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
                margin: int = 0, force_fallback: bool = False, timing: bool = False,

@@ -227,3 +227,33 @@ def test_wide_dims_staged_and_chunked_operands(k):
     d, i = ix.search(qry, k, probes=3)
     check(d, i, gd, gi, f"dim {D} k={k}")
     ix.close()
+
+
+@pytest.mark.parametrize("dim,prec", [(24, "fp32"), (24, "bf16x3"), (96, "bf16x3"), (128, "bf16x3"),
+                                      (100, "fp32"), (7, "bf16x3")])
+def test_block_bounds_never_exceed_the_true_distance(dim, prec):
+    # what pruning rests on: every block bound is at most the float64 distance from the query to
+    # every point of the block (FP32 products, or bfloat16 three-term products with their own
+    # error allowance); far-away sigma keeps many bounds positive so the check has teeth
+    R, Q, nb = 40000, 257, 64
+    say(f"block bounds dim={dim} {prec}")
+    ref = knnhip.generate_mixture(R, dim, 128, 0.03, 41 + dim, 42)
+    qry = knnhip.generate_mixture(Q, dim, 128, 0.03, 41 + dim, 43)
+    ix = knnhip.BlockIndex(ref, nblocks=nb, iters=6)
+    lb = ix.bounds(qry, precision=prec).t().double()  # Q x nb
+    _, _, ref_block = ix.layout()
+    d64 = torch.cdist(qry.t().double(), ref.t().double(), compute_mode="donot_use_mm_for_euclid_dist")
+    true = torch.full((Q, nb), float("inf"), dtype=torch.float64, device="cuda")
+    true = true.scatter_reduce(1, ref_block.long().unsqueeze(0).expand(Q, R), d64, reduce="amin")
+    bad = lb > true
+    assert not bool(bad.any()), f"{int(bad.sum())} bounds above the true distance"
+    live = torch.isfinite(true)
+    positive = (lb[live] > 0).float().mean().item()
+    tight = (lb[live] / true[live]).median().item()
+    say(f"  positive bounds {positive:.3f}, median bound / distance {tight:.3f}")
+    assert positive > 0.5
+    if prec == "bf16x3":
+        fp = ix.bounds(qry, precision="fp32").t().double()
+        fin = torch.isfinite(fp)
+        assert bool((fp[fin] - lb[fin]).abs().max() <= 1e-2 * (1 + fp[fin].abs().max()))
+    ix.close()
