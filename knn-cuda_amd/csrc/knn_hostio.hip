@@ -63,12 +63,22 @@ template <class F> void parallel_for(int n, size_t bytes, F f) {
         for (int i = 0; i < n; ++i) f(i);
         return;
     }
+    // a thread that cannot be started leaves its share to this one (no
+    // exception crosses the C interface)
     std::vector<std::thread> th;
-    th.reserve(t);
-    for (int j = 0; j < t; ++j)
-        th.emplace_back([=]() {
+    try {
+        th.reserve(t);
+    } catch (...) {
+    }
+    for (int j = 0; j < t; ++j) {
+        try {
+            th.emplace_back([=]() {
+                for (int i = (int)((long long)n * j / t); i < (int)((long long)n * (j + 1) / t); ++i) f(i);
+            });
+        } catch (...) {
             for (int i = (int)((long long)n * j / t); i < (int)((long long)n * (j + 1) / t); ++i) f(i);
-        });
+        }
+    }
     for (auto &x : th) x.join();
 }
 
