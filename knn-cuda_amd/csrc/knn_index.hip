@@ -275,39 +275,31 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
         }
 #pragma unroll
         for (int j = 0; j < kPickBatch; ++j) {
-            if (!live[j]) continue;
-            const int b = bb + j;
-            const float s = sv[j], l = lv[j];
-            // blocks come in increasing order, so an equal score ranks after
-            // the entries already kept
-            if (n < P || s > ls[P - 1]) {
-                const int old_n = n;
-                if (n == P)
-                    rest = fminf(rest, ll[P - 1]);  // the entry that drops out
-                else
-                    ++n;
-                // bubble insert: from the new entry's place on, every entry
-                // moves one slot down
-                float cs = s, cl = l;
-                int cb = b;
-                bool moving = false;
+            if (!live[j]) continue;  // uniform: the block is the same for every lane
+            // one compare-swap pass over the P kept slots (sorted by score):
+            // the new block takes the first unfilled slot or the first slot
+            // whose score is strictly lower, and pushes the rest down (blocks
+            // come in increasing order, so an equal score ranks after the kept
+            // ones); whatever leaves the last slot, the new block itself or a
+            // kept one, is not chosen and bounds the rest
+            float cs = sv[j], cl = lv[j];
+            int cb = bb + j;
 #pragma unroll
-                for (int j = 0; j < PMAX; ++j) {
-                    if (j < n && (moving || j >= old_n || cs > ls[j])) {
-                        const float ts = ls[j], tl = ll[j];
-                        const int tb = lbk[j];
-                        ls[j] = cs;
-                        ll[j] = cl;
-                        lbk[j] = cb;
-                        cs = ts;
-                        cl = tl;
-                        cb = tb;
-                        moving = true;
-                    }
+            for (int t = 0; t < PMAX; ++t) {
+                if (t < P) {
+                    const bool sw = cs > ls[t] || lbk[t] == 0x7FFFFFFF;
+                    const float ts = ls[t], tl = ll[t];
+                    const int tb = lbk[t];
+                    ls[t] = sw ? cs : ts;
+                    ll[t] = sw ? cl : tl;
+                    lbk[t] = sw ? cb : tb;
+                    cs = sw ? ts : cs;
+                    cl = sw ? tl : cl;
+                    cb = sw ? tb : cb;
                 }
-            } else {
-                rest = fminf(rest, l);
             }
+            rest = fminf(rest, cl);  // an unfilled slot leaves +inf
+            n = min(n + 1, P);
         }
     }
     // merge rounds
