@@ -1152,16 +1152,22 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
     const float nq = qnorm[qa];
     const float sq = nextafterf(sqrtf(nq), INFINITY);
     const int q = q0 + col;
+    // every lane runs the fold, so the open block (cur, its ball range) stays
+    // wave-uniform in scalar registers and the row loop branches on scalars;
+    // only the lanes of half 0 (one query each) store
     const bool folder = h == 0 && q < Q;
     int cur = -1, cur_s = 0, cur_e = 0;
     float m = INFINITY;
     auto flush = [&]() {
         if (cur < 0) return;
-        float *dst = lb + (long long)cur * Q + q;
-        if (cur_s >= F0 && cur_e <= F1)
-            *dst = m;
-        else
-            atomicMin(reinterpret_cast<unsigned *>(dst), __float_as_uint(m));
+        const bool whole = cur_s >= F0 && cur_e <= F1;
+        if (folder) {
+            float *dst = lb + (long long)cur * Q + q;
+            if (whole)
+                *dst = m;
+            else
+                atomicMin(reinterpret_cast<unsigned *>(dst), __float_as_uint(m));
+        }
     };
     for (int tt = 0; tt < tpw; ++tt) {
         const int f0 = F0 + tt * 32;
@@ -1235,26 +1241,24 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
             re = fb_start[rb + 1];
         }
         const int n = min(32, F1 - f0);
-        if (folder) {
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                if (r >= n) continue;  // (not break: the loop must unroll)
-                const int b = __builtin_amdgcn_readlane(rb, r);
-                // on lane half 0, rows with (r / 4) even are its own
-                const float v = ((r >> 2) & 1) ? oth[4 * (r >> 3) + (r & 3)] : out[4 * (r >> 3) + (r & 3)];
-                if (b != cur) {
-                    flush();
-                    cur = b;
-                    cur_s = __builtin_amdgcn_readlane(rs, r);
-                    cur_e = __builtin_amdgcn_readlane(re, r);
-                    m = v;
-                } else {
-                    m = fminf(m, v);
-                }
+        for (int r = 0; r < 32; ++r) {
+            if (r >= n) continue;  // (not break: the loop must unroll)
+            const int b = __builtin_amdgcn_readlane(rb, r);
+            // on lane half 0, rows with (r / 4) even are its own
+            const float v = ((r >> 2) & 1) ? oth[4 * (r >> 3) + (r & 3)] : out[4 * (r >> 3) + (r & 3)];
+            if (b != cur) {
+                flush();
+                cur = b;
+                cur_s = __builtin_amdgcn_readlane(rs, r);
+                cur_e = __builtin_amdgcn_readlane(re, r);
+                m = v;
+            } else {
+                m = fminf(m, v);
             }
         }
     }
-    if (folder) flush();
+    flush();
 }
 
 // ---------------------------------------------------------------------------
