@@ -82,26 +82,29 @@ template <class F> void parallel_for(int n, size_t bytes, F f) {
     for (auto &x : th) x.join();
 }
 
-// Rows [r0, r0 + nr) x columns [c0, c0 + w) of a pitched array, 4-byte words.
+// Rows of a pitched array (4-byte words) to or from a packed piece of nr
+// rows of w words: blocks over (columns, rows), no index division.
 constexpr int kMoveThreads = 256;
 __global__ __launch_bounds__(kMoveThreads) void scatter_rows_kernel(const unsigned *__restrict__ src,
                                                                     unsigned *__restrict__ dst,
-                                                                    long long dld, long long w,
-                                                                    long long n) {
-    for (long long e = (long long)blockIdx.x * kMoveThreads + threadIdx.x; e < n;
-         e += (long long)gridDim.x * kMoveThreads)
-        dst[(e / w) * dld + e % w] = src[e];
+                                                                    long long dld, long long w, int nr) {
+    for (int r = blockIdx.y; r < nr; r += gridDim.y)
+        for (long long c = (long long)blockIdx.x * kMoveThreads + threadIdx.x; c < w;
+             c += (long long)gridDim.x * kMoveThreads)
+            dst[(long long)r * dld + c] = src[(long long)r * w + c];
 }
 __global__ __launch_bounds__(kMoveThreads) void gather_rows_kernel(const unsigned *__restrict__ src,
-                                                                   long long sld, long long w,
-                                                                   long long n,
+                                                                   long long sld, long long w, int nr,
                                                                    unsigned *__restrict__ dst) {
-    for (long long e = (long long)blockIdx.x * kMoveThreads + threadIdx.x; e < n;
-         e += (long long)gridDim.x * kMoveThreads)
-        dst[e] = src[(e / w) * sld + e % w];
+    for (int r = blockIdx.y; r < nr; r += gridDim.y)
+        for (long long c = (long long)blockIdx.x * kMoveThreads + threadIdx.x; c < w;
+             c += (long long)gridDim.x * kMoveThreads)
+            dst[(long long)r * w + c] = src[(long long)r * sld + c];
 }
-int move_grid(long long n) {
-    return (int)std::min<long long>((n + kMoveThreads - 1) / kMoveThreads, 4096);
+dim3 move_grid(long long w, int nr) {
+    const long long gx = std::min<long long>((w + kMoveThreads - 1) / kMoveThreads, 1024);
+    const long long gy = std::min<long long>(nr, std::max<long long>(1, 4096 / gx));
+    return dim3((unsigned)std::max(1ll, gx), (unsigned)std::max(1ll, gy));
 }
 
 // Pieces of at most kStageBytes: whole rows when a row fits, else segments
@@ -154,9 +157,8 @@ bool host_upload(void *dst, long long dld, const void *src, long long sld, long 
             if (!stage_dev(slot)) return false;
             KNN_TRY(hipMemcpyAsync(g_st.dev[slot].p, stage, bytes, hipMemcpyHostToDevice, s),
                     "Unable to copy data from host to device");
-            const long long n = (long long)nr * w;
-            hipLaunchKernelGGL(scatter_rows_kernel, dim3(move_grid(n)), dim3(kMoveThreads), 0, s,
-                               g_st.dev[slot].as<unsigned>(), out, dld, w, n);
+            hipLaunchKernelGGL(scatter_rows_kernel, move_grid(w, nr), dim3(kMoveThreads), 0, s,
+                               g_st.dev[slot].as<unsigned>(), out, dld, w, nr);
             KNN_TRY(hipGetLastError(), "copy kernel launch");
         }
         KNN_TRY(hipEventRecord(g_st.done[slot], s), "event");
@@ -182,9 +184,8 @@ bool host_download(void *dst, long long dld, const void *src, long long sld, lon
                     "Unable to copy data from device to host");
         } else {
             if (!stage_dev(slot)) return false;
-            const long long n = (long long)nr * w;
-            hipLaunchKernelGGL(gather_rows_kernel, dim3(move_grid(n)), dim3(kMoveThreads), 0, s, in, sld, w,
-                               n, g_st.dev[slot].as<unsigned>());
+            hipLaunchKernelGGL(gather_rows_kernel, move_grid(w, nr), dim3(kMoveThreads), 0, s, in, sld, w, nr,
+                               g_st.dev[slot].as<unsigned>());
             KNN_TRY(hipGetLastError(), "copy kernel launch");
             KNN_TRY(hipMemcpyAsync(stage, g_st.dev[slot].p, bytes, hipMemcpyDeviceToHost, s),
                     "Unable to copy data from device to host");
