@@ -586,13 +586,41 @@ __device__ __forceinline__ void norms_pack_points(const NormArgs &a, int i0) {
     const int t = threadIdx.x;
     float s = 0.0f;  // thread t < 64: squared norm of point i0 + t, summed in dimension order
     const int ngrp = (a.oct + 3) / 4;
+    // rows starting on 16 bytes (pitch a multiple of 4, aligned base) are read
+    // four points at a time; i0 is a multiple of 64
+    const bool vec = (a.ld & 3) == 0 && (reinterpret_cast<unsigned long long>(a.p) & 15) == 0;
     for (int g = 0; g < ngrp; ++g) {
+        if (vec) {
 #pragma unroll
-        for (int e = 0; e < (kDims * kPts) / 256; ++e) {
-            const int idx = t + 256 * e;
-            const int dl = idx / kPts, p = idx % kPts;
-            const int d = kDims * g + dl, i = i0 + p;
-            tile[dl][p] = (i < a.n && d < a.dim) ? a.p[(long long)d * a.ld + i] : 0.0f;
+            for (int e = 0; e < (kDims * kPts) / (4 * 256); ++e) {
+                const int idx = t + 256 * e;
+                const int dl = idx / (kPts / 4), p = 4 * (idx % (kPts / 4));
+                const int d = kDims * g + dl, i = i0 + p;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (d < a.dim) {
+                    const float *row = a.p + (long long)d * a.ld;
+                    if (i + 3 < a.n) {
+                        v = *reinterpret_cast<const float4 *>(row + i);
+                    } else {
+                        v.x = i + 0 < a.n ? row[i + 0] : 0.0f;
+                        v.y = i + 1 < a.n ? row[i + 1] : 0.0f;
+                        v.z = i + 2 < a.n ? row[i + 2] : 0.0f;
+                        v.w = i + 3 < a.n ? row[i + 3] : 0.0f;
+                    }
+                }
+                tile[dl][p + 0] = v.x;
+                tile[dl][p + 1] = v.y;
+                tile[dl][p + 2] = v.z;
+                tile[dl][p + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < (kDims * kPts) / 256; ++e) {
+                const int idx = t + 256 * e;
+                const int dl = idx / kPts, p = idx % kPts;
+                const int d = kDims * g + dl, i = i0 + p;
+                tile[dl][p] = (i < a.n && d < a.dim) ? a.p[(long long)d * a.ld + i] : 0.0f;
+            }
         }
         __syncthreads();
         if (t < kPts) {
