@@ -230,7 +230,7 @@ def test_wide_dims_staged_and_chunked_operands(k):
 
 
 @pytest.mark.parametrize("dim,prec", [(24, "fp32"), (24, "bf16x3"), (96, "bf16x3"), (128, "bf16x3"),
-                                      (100, "fp32"), (7, "bf16x3")])
+                                      (100, "fp32"), (7, "bf16x3"), (200, "bf16x3")])
 def test_block_bounds_never_exceed_the_true_distance(dim, prec):
     # what pruning rests on: every block bound is at most the float64 distance from the query to
     # every point of the block (FP32 products, or bfloat16 three-term products with their own
