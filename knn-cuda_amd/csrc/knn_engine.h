@@ -55,7 +55,11 @@ int scan_bf3_nch(int dim);
 size_t scan_bf3_lds_bytes(int kp);
 hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s);
 int scan_bf3_task_nch(int dim);
-hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s);
+// tasks_per_group > 1 (with waves = 1): that many one-wave tasks per
+// workgroup, each with its own heaps, so a block's consecutive query groups
+// share a CU's caches.
+hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
+                                int tasks_per_group = 1);
 hipError_t launch_pack_bf3_rows(const float *pm, int ld, int n_pad, int nch, unsigned short *out,
                                 hipStream_t s);
 hipError_t launch_norms(const NormArgs &a, hipStream_t s);

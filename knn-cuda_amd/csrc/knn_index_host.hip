@@ -25,6 +25,8 @@ constexpr int kIndexMarginBf3 = 12;
 // sharing one task's heaps; where tasks are few (a retry of a few queries)
 // four waves per task shorten each task.
 constexpr double kOneWaveQueries = 4.0;
+// One-wave bfloat16 tasks per workgroup (each with its own heaps).
+constexpr int kTasksPerGroup = 2;
 
 // Lloyd iterations on the point-major rows `pm` (n points) from the `k`
 // centroids already in `cent` (point-major, ld pm_ld).  The assignment step
@@ -578,7 +580,10 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         const int waves = (double)Q * (pr[ph + 1] - pr[ph]) / nb >= kOneWaveQueries ? 1 : 4;
         if (nch_bf) {
             sa.packed_bf = ix->packed_bf.as<unsigned short>();
-            KNN_TRY(launch_scan_bf3_task(sa, nch_bf, waves, scan_lds_bytes(kp, waves), s),
+            // one-wave tasks go kTasksPerGroup to a workgroup: a block's
+            // consecutive query groups then read its tiles through one CU
+            KNN_TRY(launch_scan_bf3_task(sa, nch_bf, waves, scan_lds_bytes(kp, waves), s,
+                                         waves == 1 ? kTasksPerGroup : 1),
                     "scan kernel launch");
         } else {
             KNN_TRY(launch_scan(sa, ks, bmode, waves, s), "scan kernel launch");

@@ -385,33 +385,52 @@ hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s) 
 // instead of 8 x 64.  Keys start from ||r||^2 and add the same products in
 // the same chunk order as the 128-query form, so the same error bound holds.
 // NB: chunks in the register ring (loads run NB-1 chunks ahead).
+// IND: every wave is a task of its own (tasks waves * block + wave), with its
+// own heaps and queue in its own LDS region and no workgroup barrier: the
+// tasks next to each other in the list, which hold the same block's next
+// query groups, run on one CU and read the block's tiles once from HBM.
 // ---------------------------------------------------------------------------
-template <int NCH, int NB>
+template <int NCH, int NB, bool IND>
 __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem_all[];
     static_assert(NCH % NB == 0 && NB >= 2, "ring position must repeat every step");
-    const int waves = blockDim.x >> 6;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nwaves = blockDim.x >> 6;
+    const int wave_id = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
+    // within the task: IND tasks are one wave
+    const int waves = IND ? 1 : nwaves, wave = IND ? 0 : wave_id;
+    const int tix = IND ? blockIdx.x * nwaves + wave_id : blockIdx.x;
+    if (IND && tix >= a.ntasks) return;  // (no workgroup barrier in this form)
+    const int kp = a.kp, kps = heap_stride(kp);
+    unsigned char *smem = smem_all + (IND ? (size_t)wave_id * (scan_heap_bytes(kp) + kQueueBytes) : 0);
 
-    const ScanTask tk = a.tasks[blockIdx.x];
+    const ScanTask tk = a.tasks[tix];
     const int r_begin = tk.ref_begin, r_end = tk.ref_end, nq = tk.qcount;
-    if (nq <= 0) return;  // empty task: uniform over the block
+    if (nq <= 0) return;  // empty task: uniform over the block (IND: over the wave)
     // columns past the task's queries compute query 0's keys and take nothing
     const bool qvalid = col < nq;
     const int q = a.qlist[tk.qoff + (qvalid ? col : 0)];
-    const int task_row = blockIdx.x * kQueriesPerWave;
+    const int task_row = tix * kQueriesPerWave;
+    const int tid = IND ? lane : threadIdx.x, tstride = IND ? kWave : blockDim.x;
+    auto task_sync = [&]() {
+        if constexpr (IND) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    };
 
-    const int kp = a.kp, kps = heap_stride(kp);
     float *hv = reinterpret_cast<float *>(smem);
     int *hi = reinterpret_cast<int *>(smem + (size_t)kQueriesPerWave * kps * sizeof(float));
     int *lock = hi + (size_t)kQueriesPerWave * kps;
-    for (int e = threadIdx.x; e < kQueriesPerWave * kps; e += blockDim.x) {
+    for (int e = tid; e < kQueriesPerWave * kps; e += tstride) {
         hv[e] = INFINITY;
         hi[e] = -1;
     }
-    if (threadIdx.x < kQueriesPerWave) lock[threadIdx.x] = 0;
-    __syncthreads();
+    if (tid < kQueriesPerWave) lock[tid] = 0;
+    task_sync();
     float *myv = hv + col * kps;
     float theta = INFINITY;
     float published = INFINITY;
@@ -608,9 +627,9 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
                                __HIP_MEMORY_SCOPE_AGENT);
 
     // the task's heaps leave as rows task_row .. task_row + nq - 1
-    __syncthreads();
+    task_sync();
     const long long obase = (long long)task_row * kp;
-    for (int e = threadIdx.x; e < nq * kp; e += blockDim.x) {
+    for (int e = tid; e < nq * kp; e += tstride) {
         const int row = e / kp, j = e - row * kp;
         a.part_val[obase + e] = hv[row * kps + j];
         a.part_idx[obase + e] = hi[row * kps + j];
@@ -635,17 +654,32 @@ int scan_bf3_task_nch(int dim) {
 
 // Task lists only (a.tasks, one block of `waves` waves per task), with the
 // LDS of the FP32 scan's 32-query blocks of that many waves.
-hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s) {
+hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
+                                int tasks_per_group) {
     if (!a.tasks || !a.packed_bf) return hipErrorInvalidValue;
+    if (tasks_per_group > 1 && (waves != 1 || tasks_per_group > 4)) return hipErrorInvalidValue;
+    const size_t lds_ind = (size_t)tasks_per_group * (scan_heap_bytes(a.kp) + kQueueBytes);
 #define X(N, B)                                                                                 \
     if (nch == N) {                                                                             \
-        const void *fn = (const void *)scan_bf3_task_kernel<N, B>;                              \
+        if (tasks_per_group > 1) {                                                              \
+            const void *fi = (const void *)scan_bf3_task_kernel<N, B, true>;                    \
+            if (lds_ind > 65536) {                                                              \
+                hipError_t e = hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                   (int)lds_ind);                               \
+                if (e != hipSuccess) return e;                                                  \
+            }                                                                                   \
+            hipLaunchKernelGGL((scan_bf3_task_kernel<N, B, true>),                              \
+                               dim3((a.ntasks + tasks_per_group - 1) / tasks_per_group),        \
+                               dim3(tasks_per_group * kWave), lds_ind, s, a);                   \
+            return hipGetLastError();                                                           \
+        }                                                                                       \
+        const void *fn = (const void *)scan_bf3_task_kernel<N, B, false>;                       \
         if (lds > 65536) {                                                                      \
             hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                                (int)lds);                                       \
             if (e != hipSuccess) return e;                                                      \
         }                                                                                       \
-        hipLaunchKernelGGL((scan_bf3_task_kernel<N, B>), dim3(a.ntasks), dim3(waves * kWave), lds, s, \
+        hipLaunchKernelGGL((scan_bf3_task_kernel<N, B, false>), dim3(a.ntasks), dim3(waves * kWave), lds, s, \
                            a);                                                                  \
         return hipGetLastError();                                                               \
     }
