@@ -66,13 +66,20 @@ __device__ __forceinline__ void unlock(int *l) {
 
 // LDS of a 32-query scan block: the shared heaps and locks, then one
 // candidate queue (key, reference, query column) per wave.  Blocks of up to
-// four waves queue up to 768 candidates a wave, checked for a drain every 8
-// keys (one check admits at most 8 x 64); eight-wave blocks, whose LDS is
-// short, 384, checked every 4 keys.
+// four waves queue up to 768 candidates a wave; eight-wave blocks, whose LDS
+// is short, 384.
 constexpr int kQueueCap = 768;
 constexpr size_t kQueueBytes = (size_t)kQueueCap * 3 * sizeof(float);
 __host__ __device__ constexpr int scan_queue_cap(int waves) { return waves > 4 ? 384 : kQueueCap; }
-__host__ __device__ constexpr int scan_queue_check(int waves) { return waves > 4 ? 4 : 8; }
+// A step's keys are queued four accumulator rows at a time.  A group whose
+// four keys are at or above the threshold on every lane is skipped with one
+// test; before a group that has a candidate, a queue with less than room for
+// 4 x 64 keys is drained.
+constexpr int kQueueGroup = 4;
+template <class V> __device__ __forceinline__ float group_min(const V &acc, int g) {
+    return fminf(fminf(acc[kQueueGroup * g], acc[kQueueGroup * g + 1]),
+                 fminf(acc[kQueueGroup * g + 2], acc[kQueueGroup * g + 3]));
+}
 __host__ __device__ constexpr size_t scan_queue_bytes(int waves) {
     return (size_t)scan_queue_cap(waves) * 3 * sizeof(float);
 }

@@ -53,7 +53,7 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
     constexpr int OCT = KS / 4;  // octets per chunk
     constexpr bool MULTI = BMODE == kBChunks;
     constexpr bool BLDS = BMODE == kBLds;
-    constexpr int QCAP = scan_queue_cap(MAXW), QCHK = scan_queue_check(MAXW);
+    constexpr int QCAP = scan_queue_cap(MAXW);
     constexpr size_t QBYTES = scan_queue_bytes(MAXW);
     static_assert(KS % 4 == 0, "k-steps come in octets of 4");
     static_assert(OCT % NBUF == 0, "ring position must repeat every chunk");
@@ -366,24 +366,28 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
                 for (int i = 0; i < 16; ++i) cm |= acc[mt][i] < cut;
                 if (__ballot(cm && qvalid) == 0) continue;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    if (i % QCHK == 0 && qcount > QCAP - QCHK * kWave) {
-                        // room for QCHK keys x 64 lanes of candidates
+                for (int g = 0; g < 16 / kQueueGroup; ++g) {
+                    if (__ballot(qvalid && group_min(acc[mt], g) < cut) == 0) continue;
+                    if (qcount > QCAP - kQueueGroup * kWave) {
                         drain();
                         theta = fminf(theta, shared_root(myv));
                     }
-                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
-                    const bool take = qvalid && acc[mt][i] < cut && r < r_end;
-                    const unsigned long long m = __ballot(take);
-                    if (m == 0) continue;
-                    const int n = __popcll(m);
-                    if (take) {
-                        const int pos = qcount + __popcll(m & lanes_below);
-                        qv[pos] = acc[mt][i];
-                        qr[pos] = r;
-                        qc[pos] = col;
+#pragma unroll
+                    for (int e = 0; e < kQueueGroup; ++e) {
+                        const int i = kQueueGroup * g + e;
+                        const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                        const bool take = qvalid && acc[mt][i] < cut && r < r_end;
+                        const unsigned long long m = __ballot(take);
+                        if (m == 0) continue;
+                        const int n = __popcll(m);
+                        if (take) {
+                            const int pos = qcount + __popcll(m & lanes_below);
+                            qv[pos] = acc[mt][i];
+                            qr[pos] = r;
+                            qc[pos] = col;
+                        }
+                        qcount += n;
                     }
-                    qcount += n;
                 }
             }
             if (qcount >= QCAP / 2) {

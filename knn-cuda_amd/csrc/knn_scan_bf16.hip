@@ -44,8 +44,10 @@ __host__ __device__ constexpr size_t bf_heap_bytes(int kp) {
            ~(size_t)15;
 }
 
+// heaps, queues, the chunk ring, then two slots of one step's reference norms
 size_t scan_bf3_lds_bytes(int kp) {
-    return bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes + kBfRing * (size_t)kBfChunkBytes;
+    return bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes + kBfRing * (size_t)kBfChunkBytes +
+           2 * kRefsPerStep * sizeof(float);
 }
 
 // Workgroup barrier for LDS hand-offs: waits for this wave's LDS operations
@@ -92,7 +94,8 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
     const bool qvalid = qlocal < nq;
     const int q = qfirst + (qvalid ? qlocal : 0);
 
-    // LDS: heaps + locks (128 queries), 8 candidate queues, 2 ring slots
+    // LDS: heaps + locks (128 queries), 8 candidate queues, the chunk ring,
+    // 2 norm slots
     const int kp = a.kp, kps = heap_stride(kp);
     float *hv = reinterpret_cast<float *>(smem);
     int *hi = reinterpret_cast<int *>(smem + (size_t)kBfQueries * kps * sizeof(float));
@@ -102,6 +105,7 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
     int *qr = reinterpret_cast<int *>(qbase + kBfQueueCap * sizeof(float));
     int *qc = reinterpret_cast<int *>(qbase + kBfQueueCap * 2 * sizeof(float));
     unsigned char *ring = smem + bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes;
+    float *nslot = reinterpret_cast<float *>(ring + kBfRing * kBfChunkBytes);
     for (int e = threadIdx.x; e < kBfQueries * kps; e += blockDim.x) {
         hv[e] = INFINITY;
         hi[e] = -1;
@@ -200,6 +204,14 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
 #pragma unroll
     for (int j = 0; j < PF; ++j)
         pre[(3 + j) % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(3 + j), 0, 0);
+    // ||r||^2 of step st sits in norm slot st % 2, 16 bytes per loading
+    // thread (threads 0..31).  Step st + 1's norms are loaded at step st's
+    // first chunk and stored before its last barrier; the accumulators of a
+    // step start from LDS instead of waiting for a global load.
+    const bool nloader = threadIdx.x < kRefsPerStep / 4;
+    const float *nsrc = a.rnorm + r_begin + 4 * threadIdx.x;
+    if (nloader && nsteps > 0)
+        *reinterpret_cast<float4 *>(nslot + 4 * threadIdx.x) = *reinterpret_cast<const float4 *>(nsrc);
     __syncthreads();
     const int my_t0 = 2 * half;  // my tiles: my_t0, my_t0 + 1
     auto read_ops = [&](int slotidx, u32x4_t *ops) {
@@ -218,17 +230,22 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
         const int r0 = r_begin + st * kRefsPerStep;
         f32x16_t acc[2];
         // start from ||r||^2 (rows 8(i/4) + 4h + i%4 of each tile)
+        const float *nrm = nslot + (st & 1) * kRefsPerStep + kRefsPerTile * my_t0 + 4 * h;
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
             for (int g4 = 0; g4 < 4; ++g4) {
-                const float4 nr = *reinterpret_cast<const float4 *>(
-                    a.rnorm + r0 + kRefsPerTile * (my_t0 + mt) + 8 * g4 + 4 * h);
+                const float4 nr = *reinterpret_cast<const float4 *>(nrm + kRefsPerTile * mt + 8 * g4);
                 acc[mt][4 * g4 + 0] = nr.x;
                 acc[mt][4 * g4 + 1] = nr.y;
                 acc[mt][4 * g4 + 2] = nr.z;
                 acc[mt][4 * g4 + 3] = nr.w;
             }
+        // the next step's norms (slot (st + 1) % 2 was last read at the start
+        // of step st - 1, before that step's barriers)
+        const bool nload = nloader && st + 1 < nsteps;
+        float4 nnext = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (nload) nnext = *reinterpret_cast<const float4 *>(nsrc + (st + 1) * kRefsPerStep);
         const unsigned tq = __hip_atomic_load(theta_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
@@ -272,6 +289,8 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
+            if (c == NCH - 1 && nload)
+                *reinterpret_cast<float4 *>(nslot + ((st + 1) & 1) * kRefsPerStep + 4 * threadIdx.x) = nnext;
             // every second chunk (NCH is even): chunks g+2 and g+3 stored,
             // everyone past chunk g's and g+1's reads
             if (c & 1) lds_barrier();
@@ -290,22 +309,28 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    if ((i & 3) == 0 && qcount > kBfQueueCap - 4 * kWave) {
+                for (int g = 0; g < 16 / kQueueGroup; ++g) {
+                    // theta only falls, so a skipped group holds no key below it
+                    if (__ballot(qvalid && group_min(acc[mt], g) < theta) == 0) continue;
+                    if (qcount > kBfQueueCap - kQueueGroup * kWave) {
                         drain();
                         theta = fminf(theta, shared_root(myv));
                     }
-                    const int r = r0 + kRefsPerTile * (my_t0 + mt) + acc_row(i, h);
-                    const bool take = qvalid && acc[mt][i] < theta && r < r_end;
-                    const unsigned long long m = __ballot(take);
-                    if (m == 0) continue;
-                    if (take) {
-                        const int pos = qcount + __popcll(m & lanes_below);
-                        qv[pos] = acc[mt][i];
-                        qr[pos] = r;
-                        qc[pos] = qlocal;
+#pragma unroll
+                    for (int e = 0; e < kQueueGroup; ++e) {
+                        const int i = kQueueGroup * g + e;
+                        const int r = r0 + kRefsPerTile * (my_t0 + mt) + acc_row(i, h);
+                        const bool take = qvalid && acc[mt][i] < theta && r < r_end;
+                        const unsigned long long m = __ballot(take);
+                        if (m == 0) continue;
+                        if (take) {
+                            const int pos = qcount + __popcll(m & lanes_below);
+                            qv[pos] = acc[mt][i];
+                            qr[pos] = r;
+                            qc[pos] = qlocal;
+                        }
+                        qcount += __popcll(m);
                     }
-                    qcount += __popcll(m);
                 }
             }
             if (qcount >= kBfQueueCap / 2) {
@@ -590,6 +615,8 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
                 if (__ballot(cm && qvalid) == 0) continue;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
+                    // block-index tasks see dense candidates: every row is
+                    // tested (a group test costs more than it skips there)
                     if ((i & 7) == 0 && qcount > kQueueCap - 8 * kWave) {
                         drain_queue(qv, qr, qc, qcount, hv, hi, lock, kps, kp, lane);
                         qcount = 0;
