@@ -184,11 +184,13 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
         (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
     const int ltile = threadIdx.x >> 7, lhl = (threadIdx.x >> 6) & 1, lln = threadIdx.x & 63;
     const unsigned my_off = (unsigned)(ltile * tile_bytes + lhl * kWave * 16 + lln * 16);
-    // chunk g = step * NCH + c lives at step*4 tiles + c * 2 KiB
-    auto chunk_off = [&](int g) -> unsigned {
-        const long long st = g / NCH, c = g - st * NCH;
-        const long long off = st * 4 * tile_bytes + c * 2 * kWave * 16 + my_off;
-        return (unsigned)min(off, 0x7FFFFFF0ll);
+    // chunk st * NCH + j (j >= 0 may pass NCH) lives at (st + j / NCH) * 4
+    // tiles + (j % NCH) * 2 KiB.  32-bit offsets: the host keeps a slice's
+    // tiles within 2^31 bytes and loads run at most two steps past the slice
+    // (they read zeros there: the offset is past the resource's size).
+    constexpr unsigned step_bytes = 4u * (unsigned)tile_bytes;
+    auto chunk_off = [&](int st, int j) -> unsigned {
+        return my_off + (unsigned)(st + j / NCH) * step_bytes + (unsigned)(j % NCH) * (2u * kWave * 16);
     };
     // Ring invariant at the top of chunk g: chunks g .. g+2 sit in slots
     // x % 4 (chunks g+1 and g+2 complete since the last barrier when g is
@@ -199,11 +201,11 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
 #pragma unroll
     for (int j = 0; j < 3; ++j)
         *reinterpret_cast<u32x4_t *>(ring + j * kBfChunkBytes + threadIdx.x * 16) =
-            __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(j), 0, 0);
+            __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(0, j), 0, 0);
     u32x4_t pre[PF];
 #pragma unroll
     for (int j = 0; j < PF; ++j)
-        pre[(3 + j) % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(3 + j), 0, 0);
+        pre[(3 + j) % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(0, 3 + j), 0, 0);
     // ||r||^2 of step st sits in norm slot st % 2, 16 bytes per loading
     // thread (threads 0..31).  Step st + 1's norms are loaded at step st's
     // first chunk and stored before its last barrier; the accumulators of a
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
                 const int rslot = (c + 3) % PF;
                 *reinterpret_cast<u32x4_t *>(ring + ((g + 3) % kBfRing) * kBfChunkBytes + threadIdx.x * 16) =
                     pre[rslot];
-                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 3 + PF), 0, 0);
+                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(st, c + 3 + PF), 0, 0);
             }
             if constexpr (BF3) {
                 const bf16x8_t a0h = __builtin_bit_cast(bf16x8_t, cur[0]);
