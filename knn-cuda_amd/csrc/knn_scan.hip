@@ -390,7 +390,11 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
                     }
                 }
             }
-            if (qcount >= QCAP / 2) {
+            // Drain and publish once a wave's pass of keys is queued: the
+            // blocks scanning the other slices of these queries filter with
+            // the published threshold, which a rarer drain leaves stale
+            // (64 measured best against 16, 32 and half the queue).
+            if (qcount >= kWave) {
                 drain();
                 theta = fminf(theta, shared_root(myv));
                 if (h == 0 && qvalid && theta < published) {

@@ -335,7 +335,10 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
                     }
                 }
             }
-            if (qcount >= kBfQueueCap / 2) {
+            // drain and publish early, so the other slices' blocks of these
+            // queries filter with a fresh threshold (32 keys measured best
+            // against 16, 64, 128 and half the queue)
+            if (qcount >= kWave / 2) {
                 drain();
                 theta = fminf(theta, shared_root(myv));
                 if (h == 0 && qvalid && theta < published) {
