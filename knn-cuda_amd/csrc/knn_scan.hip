@@ -390,10 +390,11 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
                     }
                 }
             }
-            // Drain and publish once a wave's pass of keys is queued: the
-            // blocks scanning the other slices of these queries filter with
-            // the published threshold, which a rarer drain leaves stale
-            // (64 measured best against 16, 32 and half the queue).
+            // Drain once a wave's pass of keys is queued: the heap root that
+            // filters the next steps' keys (and the threshold published to
+            // other slices' blocks) falls only at a drain, so a fuller queue
+            // filters with a stale root (64 measured best against 16, 32 and
+            // half the queue).
             if (qcount >= kWave) {
                 drain();
                 theta = fminf(theta, shared_root(myv));

@@ -335,9 +335,9 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
                     }
                 }
             }
-            // drain and publish early, so the other slices' blocks of these
-            // queries filter with a fresh threshold (32 keys measured best
-            // against 16, 64, 128 and half the queue)
+            // drain early: the heap root that filters the next steps' keys
+            // (and the published threshold) falls only at a drain (32 keys
+            // measured best against 16, 64, 128 and half the queue)
             if (qcount >= kWave / 2) {
                 drain();
                 theta = fminf(theta, shared_root(myv));
