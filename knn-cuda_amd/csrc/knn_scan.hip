@@ -348,13 +348,20 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
                 for (int i = 0; i < 16; ++i)
                     if (r0 + kRefsPerTile * mt + acc_row(i, h) >= r_end) acc[mt][i] = INFINITY;
         }
-        int ncand = 0;
+        // fast path: the step's smallest key against theta (a min3 chain);
+        // the candidates are counted only on steps that have one
+        float kmin = INFINITY;
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
-        if (!qvalid) ncand = 0;
-        if (__ballot(ncand > 0)) {
+            for (int i = 0; i < 16; ++i) kmin = fminf(kmin, acc[mt][i]);
+        if (__ballot(qvalid && kmin < theta)) {
+            int ncand = 0;
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
+            if (!qvalid) ncand = 0;
             // many candidates (a young heap): queue only a step-local cut
             const int npair = ncand + __shfl_xor(ncand, 32);
             float cut = theta;

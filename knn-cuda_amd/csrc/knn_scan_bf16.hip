@@ -44,10 +44,8 @@ __host__ __device__ constexpr size_t bf_heap_bytes(int kp) {
            ~(size_t)15;
 }
 
-// heaps, queues, the chunk ring, then two slots of one step's reference norms
 size_t scan_bf3_lds_bytes(int kp) {
-    return bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes + kBfRing * (size_t)kBfChunkBytes +
-           2 * kRefsPerStep * sizeof(float);
+    return bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes + kBfRing * (size_t)kBfChunkBytes;
 }
 
 // Workgroup barrier for LDS hand-offs: waits for this wave's LDS operations
@@ -94,8 +92,7 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
     const bool qvalid = qlocal < nq;
     const int q = qfirst + (qvalid ? qlocal : 0);
 
-    // LDS: heaps + locks (128 queries), 8 candidate queues, the chunk ring,
-    // 2 norm slots
+    // LDS: heaps + locks (128 queries), 8 candidate queues, 2 ring slots
     const int kp = a.kp, kps = heap_stride(kp);
     float *hv = reinterpret_cast<float *>(smem);
     int *hi = reinterpret_cast<int *>(smem + (size_t)kBfQueries * kps * sizeof(float));
@@ -105,7 +102,6 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
     int *qr = reinterpret_cast<int *>(qbase + kBfQueueCap * sizeof(float));
     int *qc = reinterpret_cast<int *>(qbase + kBfQueueCap * 2 * sizeof(float));
     unsigned char *ring = smem + bf_heap_bytes(kp) + (size_t)kBfWaves * kBfQueueBytes;
-    float *nslot = reinterpret_cast<float *>(ring + kBfRing * kBfChunkBytes);
     for (int e = threadIdx.x; e < kBfQueries * kps; e += blockDim.x) {
         hv[e] = INFINITY;
         hi[e] = -1;
@@ -184,13 +180,11 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
         (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
     const int ltile = threadIdx.x >> 7, lhl = (threadIdx.x >> 6) & 1, lln = threadIdx.x & 63;
     const unsigned my_off = (unsigned)(ltile * tile_bytes + lhl * kWave * 16 + lln * 16);
-    // chunk st * NCH + j (j >= 0 may pass NCH) lives at (st + j / NCH) * 4
-    // tiles + (j % NCH) * 2 KiB.  32-bit offsets: the host keeps a slice's
-    // tiles within 2^31 bytes and loads run at most two steps past the slice
-    // (they read zeros there: the offset is past the resource's size).
-    constexpr unsigned step_bytes = 4u * (unsigned)tile_bytes;
-    auto chunk_off = [&](int st, int j) -> unsigned {
-        return my_off + (unsigned)(st + j / NCH) * step_bytes + (unsigned)(j % NCH) * (2u * kWave * 16);
+    // chunk g = step * NCH + c lives at step*4 tiles + c * 2 KiB
+    auto chunk_off = [&](int g) -> unsigned {
+        const long long st = g / NCH, c = g - st * NCH;
+        const long long off = st * 4 * tile_bytes + c * 2 * kWave * 16 + my_off;
+        return (unsigned)min(off, 0x7FFFFFF0ll);
     };
     // Ring invariant at the top of chunk g: chunks g .. g+2 sit in slots
     // x % 4 (chunks g+1 and g+2 complete since the last barrier when g is
@@ -201,19 +195,11 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
 #pragma unroll
     for (int j = 0; j < 3; ++j)
         *reinterpret_cast<u32x4_t *>(ring + j * kBfChunkBytes + threadIdx.x * 16) =
-            __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(0, j), 0, 0);
+            __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(j), 0, 0);
     u32x4_t pre[PF];
 #pragma unroll
     for (int j = 0; j < PF; ++j)
-        pre[(3 + j) % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(0, 3 + j), 0, 0);
-    // ||r||^2 of step st sits in norm slot st % 2, 16 bytes per loading
-    // thread (threads 0..31).  Step st + 1's norms are loaded at step st's
-    // first chunk and stored before its last barrier; the accumulators of a
-    // step start from LDS instead of waiting for a global load.
-    const bool nloader = threadIdx.x < kRefsPerStep / 4;
-    const float *nsrc = a.rnorm + r_begin + 4 * threadIdx.x;
-    if (nloader && nsteps > 0)
-        *reinterpret_cast<float4 *>(nslot + 4 * threadIdx.x) = *reinterpret_cast<const float4 *>(nsrc);
+        pre[(3 + j) % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(3 + j), 0, 0);
     __syncthreads();
     const int my_t0 = 2 * half;  // my tiles: my_t0, my_t0 + 1
     auto read_ops = [&](int slotidx, u32x4_t *ops) {
@@ -232,22 +218,17 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
         const int r0 = r_begin + st * kRefsPerStep;
         f32x16_t acc[2];
         // start from ||r||^2 (rows 8(i/4) + 4h + i%4 of each tile)
-        const float *nrm = nslot + (st & 1) * kRefsPerStep + kRefsPerTile * my_t0 + 4 * h;
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
             for (int g4 = 0; g4 < 4; ++g4) {
-                const float4 nr = *reinterpret_cast<const float4 *>(nrm + kRefsPerTile * mt + 8 * g4);
+                const float4 nr = *reinterpret_cast<const float4 *>(
+                    a.rnorm + r0 + kRefsPerTile * (my_t0 + mt) + 8 * g4 + 4 * h);
                 acc[mt][4 * g4 + 0] = nr.x;
                 acc[mt][4 * g4 + 1] = nr.y;
                 acc[mt][4 * g4 + 2] = nr.z;
                 acc[mt][4 * g4 + 3] = nr.w;
             }
-        // the next step's norms (slot (st + 1) % 2 was last read at the start
-        // of step st - 1, before that step's barriers)
-        const bool nload = nloader && st + 1 < nsteps;
-        float4 nnext = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (nload) nnext = *reinterpret_cast<const float4 *>(nsrc + (st + 1) * kRefsPerStep);
         const unsigned tq = __hip_atomic_load(theta_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
@@ -263,7 +244,7 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
                 const int rslot = (c + 3) % PF;
                 *reinterpret_cast<u32x4_t *>(ring + ((g + 3) % kBfRing) * kBfChunkBytes + threadIdx.x * 16) =
                     pre[rslot];
-                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(st, c + 3 + PF), 0, 0);
+                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 3 + PF), 0, 0);
             }
             if constexpr (BF3) {
                 const bf16x8_t a0h = __builtin_bit_cast(bf16x8_t, cur[0]);
@@ -291,8 +272,6 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
-            if (c == NCH - 1 && nload)
-                *reinterpret_cast<float4 *>(nslot + ((st + 1) & 1) * kRefsPerStep + 4 * threadIdx.x) = nnext;
             // every second chunk (NCH is even): chunks g+2 and g+3 stored,
             // everyone past chunk g's and g+1's reads
             if (c & 1) lds_barrier();
@@ -311,34 +290,25 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
-                for (int g = 0; g < 16 / kQueueGroup; ++g) {
-                    // theta only falls, so a skipped group holds no key below it
-                    if (__ballot(qvalid && group_min(acc[mt], g) < theta) == 0) continue;
-                    if (qcount > kBfQueueCap - kQueueGroup * kWave) {
+                for (int i = 0; i < 16; ++i) {
+                    if ((i & 3) == 0 && qcount > kBfQueueCap - 4 * kWave) {
                         drain();
                         theta = fminf(theta, shared_root(myv));
                     }
-#pragma unroll
-                    for (int e = 0; e < kQueueGroup; ++e) {
-                        const int i = kQueueGroup * g + e;
-                        const int r = r0 + kRefsPerTile * (my_t0 + mt) + acc_row(i, h);
-                        const bool take = qvalid && acc[mt][i] < theta && r < r_end;
-                        const unsigned long long m = __ballot(take);
-                        if (m == 0) continue;
-                        if (take) {
-                            const int pos = qcount + __popcll(m & lanes_below);
-                            qv[pos] = acc[mt][i];
-                            qr[pos] = r;
-                            qc[pos] = qlocal;
-                        }
-                        qcount += __popcll(m);
+                    const int r = r0 + kRefsPerTile * (my_t0 + mt) + acc_row(i, h);
+                    const bool take = qvalid && acc[mt][i] < theta && r < r_end;
+                    const unsigned long long m = __ballot(take);
+                    if (m == 0) continue;
+                    if (take) {
+                        const int pos = qcount + __popcll(m & lanes_below);
+                        qv[pos] = acc[mt][i];
+                        qr[pos] = r;
+                        qc[pos] = qlocal;
                     }
+                    qcount += __popcll(m);
                 }
             }
-            // drain early: the heap root that filters the next steps' keys
-            // (and the published threshold) falls only at a drain (32 keys
-            // measured best against 16, 64, 128 and half the queue)
-            if (qcount >= kWave / 2) {
+            if (qcount >= kBfQueueCap / 2) {
                 drain();
                 theta = fminf(theta, shared_root(myv));
                 if (h == 0 && qvalid && theta < published) {
@@ -601,13 +571,20 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
                 for (int i = 0; i < 16; ++i)
                     if (r0 + kRefsPerTile * mt + acc_row(i, h) >= r_end) acc[mt][i] = INFINITY;
         }
-        int ncand = 0;
+        // the step's smallest key against theta first; candidates are counted
+        // only on steps that have one
+        float kmin = INFINITY;
 #pragma unroll
         for (int mt = 0; mt < kMTiles; ++mt)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
-        if (!qvalid) ncand = 0;
-        if (__ballot(ncand > 0)) {
+            for (int i = 0; i < 16; ++i) kmin = fminf(kmin, acc[mt][i]);
+        if (__ballot(qvalid && kmin < theta)) {
+            int ncand = 0;
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
+            if (!qvalid) ncand = 0;
             // many candidates (a young heap): queue only a step-local cut
             const int npair = ncand + __shfl_xor(ncand, 32);
             float cut = theta;

@@ -59,6 +59,25 @@ def test_lds_scan_equals_oracle(shape, form):
     check(d, i, gd, gi, f"{shape}")
 
 
+def test_lds_scan_repeats_stay_exact():
+    # The 128-query scan's heaps are shared by two waves under LDS locks and its slices exchange
+    # thresholds across blocks, so its schedule differs from run to run: every repeat of every
+    # shape, in both products, must still equal the oracle.  (A timing-dependent defect in this
+    # kernel once dropped one reference from a few queries' answers in about 1 of 500 searches,
+    # with the certificate passing; ten repeats of sixteen cases give it 160 chances.)
+    for R, Q, D, k in SHAPES:
+        rng = np.random.default_rng(sum((R, Q, D, k)))
+        ref = rng.random((D, R), dtype=np.float32)
+        qry = rng.random((D, Q), dtype=np.float32)
+        gd, gi = oracle.knn_c(ref, qry, k)
+        rt, qt = torch.from_numpy(ref).cuda(), torch.from_numpy(qry).cuda()
+        for rep in range(10):
+            for prec in ("bf16x3", "fp32"):
+                d, i = knnhip.search_device(rt, qt, k, precision=prec, block_queries=128)
+                torch.cuda.synchronize()
+                check(d, i, gd, gi, f"{(R, Q, D, k)} {prec} repeat {rep}")
+
+
 def test_clustered_data_rescans_in_fp32_and_stays_exact():
     # tight clusters: the bfloat16 bound is wider than the gap after the k-th neighbour, so most
     # queries fail the first certificate and are rescanned with FP32 products
@@ -92,7 +111,7 @@ def test_plan_falls_back_to_fp32_where_not_eligible():
     ref = knnhip.generate_uniform(4000, 64, seed=5)
     qry = knnhip.generate_uniform(50, 64, seed=6)
     knnhip.search_device(ref, qry, 64, precision="bf16x3")
-    assert knnhip.last_plan()["precision"] == "fp32"  # k + margin = 128 > 107
+    assert knnhip.last_plan()["precision"] == "fp32"  # k + margin = 128 > 91
 
 
 def test_precision_equals_fp32_at_1m_refs():
