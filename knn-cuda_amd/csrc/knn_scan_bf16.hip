@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
     const bool qvalid = qlocal < nq;
     const int q = qfirst + (qvalid ? qlocal : 0);
 
-    // LDS: heaps + locks (128 queries), 8 candidate queues, 2 ring slots
+    // LDS: heaps + locks (128 queries), 8 candidate queues, the chunk ring
     const int kp = a.kp, kps = heap_stride(kp);
     float *hv = reinterpret_cast<float *>(smem);
     int *hi = reinterpret_cast<int *>(smem + (size_t)kBfQueries * kps * sizeof(float));
@@ -180,11 +180,13 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
         (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
     const int ltile = threadIdx.x >> 7, lhl = (threadIdx.x >> 6) & 1, lln = threadIdx.x & 63;
     const unsigned my_off = (unsigned)(ltile * tile_bytes + lhl * kWave * 16 + lln * 16);
-    // chunk g = step * NCH + c lives at step*4 tiles + c * 2 KiB
-    auto chunk_off = [&](int g) -> unsigned {
-        const long long st = g / NCH, c = g - st * NCH;
-        const long long off = st * 4 * tile_bytes + c * 2 * kWave * 16 + my_off;
-        return (unsigned)min(off, 0x7FFFFFF0ll);
+    // chunk st * NCH + j (j >= 0 may pass NCH) lives at (st + j / NCH) * 4
+    // tiles + (j % NCH) * 2 KiB.  32-bit offsets: the host keeps a slice's
+    // tiles within 2^31 bytes and loads run at most two steps past the slice
+    // (they read zeros there: the offset is past the resource's size).
+    constexpr unsigned step_bytes = 4u * (unsigned)tile_bytes;
+    auto chunk_off = [&](int st, int j) -> unsigned {
+        return my_off + (unsigned)(st + j / NCH) * step_bytes + (unsigned)(j % NCH) * (2u * kWave * 16);
     };
     // Ring invariant at the top of chunk g: chunks g .. g+2 sit in slots
     // x % 4 (chunks g+1 and g+2 complete since the last barrier when g is
@@ -195,11 +197,11 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
 #pragma unroll
     for (int j = 0; j < 3; ++j)
         *reinterpret_cast<u32x4_t *>(ring + j * kBfChunkBytes + threadIdx.x * 16) =
-            __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(j), 0, 0);
+            __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(0, j), 0, 0);
     u32x4_t pre[PF];
 #pragma unroll
     for (int j = 0; j < PF; ++j)
-        pre[(3 + j) % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(3 + j), 0, 0);
+        pre[(3 + j) % PF] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(0, 3 + j), 0, 0);
     __syncthreads();
     const int my_t0 = 2 * half;  // my tiles: my_t0, my_t0 + 1
     auto read_ops = [&](int slotidx, u32x4_t *ops) {
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
                 const int rslot = (c + 3) % PF;
                 *reinterpret_cast<u32x4_t *>(ring + ((g + 3) % kBfRing) * kBfChunkBytes + threadIdx.x * 16) =
                     pre[rslot];
-                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(g + 3 + PF), 0, 0);
+                pre[rslot] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)chunk_off(st, c + 3 + PF), 0, 0);
             }
             if constexpr (BF3) {
                 const bf16x8_t a0h = __builtin_bit_cast(bf16x8_t, cur[0]);
@@ -290,25 +292,34 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    if ((i & 3) == 0 && qcount > kBfQueueCap - 4 * kWave) {
+                for (int g = 0; g < 16 / kQueueGroup; ++g) {
+                    // theta only falls, so a skipped group holds no key below it
+                    if (__ballot(qvalid && group_min(acc[mt], g) < theta) == 0) continue;
+                    if (qcount > kBfQueueCap - kQueueGroup * kWave) {
                         drain();
                         theta = fminf(theta, shared_root(myv));
                     }
-                    const int r = r0 + kRefsPerTile * (my_t0 + mt) + acc_row(i, h);
-                    const bool take = qvalid && acc[mt][i] < theta && r < r_end;
-                    const unsigned long long m = __ballot(take);
-                    if (m == 0) continue;
-                    if (take) {
-                        const int pos = qcount + __popcll(m & lanes_below);
-                        qv[pos] = acc[mt][i];
-                        qr[pos] = r;
-                        qc[pos] = qlocal;
+#pragma unroll
+                    for (int e = 0; e < kQueueGroup; ++e) {
+                        const int i = kQueueGroup * g + e;
+                        const int r = r0 + kRefsPerTile * (my_t0 + mt) + acc_row(i, h);
+                        const bool take = qvalid && acc[mt][i] < theta && r < r_end;
+                        const unsigned long long m = __ballot(take);
+                        if (m == 0) continue;
+                        if (take) {
+                            const int pos = qcount + __popcll(m & lanes_below);
+                            qv[pos] = acc[mt][i];
+                            qr[pos] = r;
+                            qc[pos] = qlocal;
+                        }
+                        qcount += __popcll(m);
                     }
-                    qcount += __popcll(m);
                 }
             }
-            if (qcount >= kBfQueueCap / 2) {
+            // drain early: the heap root that filters the next steps' keys
+            // (and the published threshold) falls only at a drain (32 keys
+            // measured best against 16, 64, 128 and half the queue)
+            if (qcount >= kWave / 2) {
                 drain();
                 theta = fminf(theta, shared_root(myv));
                 if (h == 0 && qvalid && theta < published) {
