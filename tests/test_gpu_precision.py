@@ -62,9 +62,10 @@ def test_lds_scan_equals_oracle(shape, form):
 def test_lds_scan_repeats_stay_exact():
     # The 128-query scan's heaps are shared by two waves under LDS locks and its slices exchange
     # thresholds across blocks, so its schedule differs from run to run: every repeat of every
-    # shape, in both products, must still equal the oracle.  (A timing-dependent defect in this
-    # kernel once dropped one reference from a few queries' answers in about 1 of 500 searches,
-    # with the certificate passing; ten repeats of sixteen cases give it 160 chances.)
+    # shape, in both products, must still equal the oracle.  (Staging each step's reference norms
+    # in LDS once made this kernel drop one reference from a few queries' answers, with the
+    # certificate passing, in up to 1 of 30 searches on some machines and none on others; ten
+    # repeats of sixteen cases give such a defect 160 chances.)
     for R, Q, D, k in SHAPES:
         rng = np.random.default_rng(sum((R, Q, D, k)))
         ref = rng.random((D, R), dtype=np.float32)
