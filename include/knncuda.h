@@ -61,7 +61,9 @@ int knnhip_kth_distance(const float *ref, int ref_nb, const float *query, int qu
 /* Options of the extended entry point.  Zero-initialise, then set fields;
  * zero means "library default" for every field. */
 typedef struct knnhip_options {
-    int variant;        /* 0 global (default), 1 texture (LDS-staged), 2 cublas form */
+    int variant;        /* a label only: 0 global (default), 1 texture, 2 cublas.  The three
+                           reference variants compute one result, so all three run the same
+                           kernels (recorded as "variant" in knnhip_last_plan()) */
     int margin;         /* extra candidates kept per query beyond k (default 16) */
     int slices;         /* reference slices scanned in parallel (default: chosen) */
     int force_fallback; /* 1: every query takes the exact full-scan path (testing) */
@@ -80,10 +82,19 @@ typedef struct knnhip_options {
  * Extended entry point: same problem as knn_cuda_global, with every pointer a
  * device pointer (ref is dim rows of ref_ld floats, query dim rows of query_ld
  * floats; outputs k x query_nb, leading dimension query_nb).  Work is queued
- * on `stream` (a hipStream_t, NULL for the null stream) and the call returns
- * without synchronising, except when the exact fallback is needed.  Returns 1
- * on success, 0 on failure (knnhip_last_error).  No counterpart in the
- * reference: the reference only takes host pointers (code/knncuda.h:14-21).
+ * on `stream` (a hipStream_t, NULL for the null stream).  While a wider
+ * candidate list is left to retry with (margin below 256, the usual case)
+ * the call synchronises the stream once, after the re-rank, to read how many
+ * queries failed their certificate; the work queued after that (the retry's
+ * scatter, the exact per-query scan) may still be running when it returns.
+ * With margin 256 or more nothing is read back and the call returns without
+ * synchronising.  Calls are ordered on the device: a call on another stream
+ * than the previous library call's waits for that call's work first (the
+ * library's device buffers are shared).  Inputs must be finite: a point whose
+ * squared norm is NaN or infinite makes every output index -1 and distance NaN,
+ * and knnhip_last_fallback_count() then returns -1.  Returns 1 on success, 0
+ * on failure (knnhip_last_error).  No counterpart in the reference: the
+ * reference only takes host pointers (code/knncuda.h:14-21).
  */
 int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const float *query,
                          int query_nb, long long query_ld, int dim, int k, float *knn_dist,

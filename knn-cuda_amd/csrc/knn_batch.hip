@@ -477,6 +477,7 @@ int knnhip_search_batched(const knnhip_problem *problems, int count, const knnhi
     if (!batch_args(problems, count) || !ensure_device()) return 0;
     const knnhip_options o = opt ? *opt : default_options(0);
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(s);
     if (!batch_lanes_ready() || !batch_lanes_follow(s)) return 0;
     g_batch_fail = 0;
     std::string plan = "{\"classes\": [";
@@ -508,6 +509,7 @@ int knnhip_search_batched(const knnhip_problem *problems, int count, const knnhi
     }
     g_last_plan = plan + "]}";
     g_last_fb = nullptr;
+    g_last_nonfinite = nullptr;
     return 1;
 }
 
@@ -519,6 +521,7 @@ int knnhip_batched_autotune(const knnhip_problem *problems, int count, int reps,
     if (!batch_args(problems, count) || !ensure_device()) return 0;
     knnhip_options o = default_options(0);
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(s);
     if (!batch_lanes_ready() || !batch_lanes_follow(s)) return 0;
     const std::vector<BatchClass> classes = batch_classes(problems, count);
     for (const BatchClass &c : classes) {

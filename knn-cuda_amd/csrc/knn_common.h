@@ -84,6 +84,7 @@ struct NormArgs {
     float *packed;       // packed tile layout, or nullptr
     unsigned short *packed_bf;  // bfloat16 hi/lo tile layout of the precision scan, or nullptr
     int nch_bf;                 // its 16-dimension chunks per tile (>= ceil(dim / 16))
+    unsigned *nonfinite;        // set to 1 when a point's squared norm is NaN or infinite, or nullptr
 };
 
 // One point set of a multi-set norms pass: its points land at `start` (a
@@ -130,6 +131,9 @@ struct FinalArgs {
     // Key error bound of the scan: |key - (||r||^2 - 2 r.q)| <= err_q * ||q||^2 +
     // err_r * max ||r||^2.  Zero: the FP32 scan's bound (g and 3.1 g).
     double err_q, err_r;
+    // Non-zero when an input point has a NaN or infinite squared norm: every
+    // output of the search is then index -1, distance NaN (nullptr: unchecked).
+    const unsigned *nonfinite;
 };
 
 // Arguments of the exact per-query full scan (fallback).
@@ -148,6 +152,7 @@ struct ExactArgs {
     int *scratch_idx;
     float *out_dist;
     int *out_idx;
+    const unsigned *nonfinite;  // as FinalArgs::nonfinite
 };
 
 // Order-preserving map of a float onto uint32 (total order, -0 < +0).

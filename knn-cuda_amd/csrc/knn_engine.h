@@ -100,7 +100,8 @@ void set_error(const char *fmt, ...);
 // Grow-only device buffers, one per role.
 enum Slot {
     kRefPad, kQryPad, kRNorm, kQNorm, kRMax, kPartVal, kPartIdx, kFbCount, kFbList,
-    kFbKey, kFbIdx, kOutDist, kOutIdx, kRefPm, kQryPm, kPacked, kTheta, kPackedBf, kNumSlots
+    kFbKey, kFbIdx, kOutDist, kOutIdx, kRefPm, kQryPm, kPacked, kTheta, kPackedBf, kNonFinite,
+    kNumSlots
 };
 struct Workspace {
     void *ptr[kNumSlots] = {};
@@ -113,6 +114,23 @@ struct Workspace {
 };
 extern Workspace g_ws;
 void *ws_get(Slot s, size_t bytes);
+
+// Non-finite input of the plain search: one device word that the norms pass
+// sets when a reference or query point has a NaN or infinite squared norm;
+// the re-rank and the exact scan then write index -1 / distance NaN instead
+// of ordering NaN keys.  run_device clears it at level 0, except while a
+// NonfiniteHold is alive (a streamed search clears it once for all of its
+// chunk searches).
+struct NonfiniteHold {
+    NonfiniteHold();
+    ~NonfiniteHold();
+};
+extern const unsigned *g_last_nonfinite;  // flag of the last plain search
+// Clears the flag on stream s (before a held sequence of searches).
+bool nonfinite_clear(hipStream_t s);
+// After the search's work on stream s: false with an error when the flag is
+// set (the host entry points' check).
+bool nonfinite_check(hipStream_t s);
 
 // Launch plan of one call.
 struct Plan {
@@ -147,6 +165,24 @@ std::string plan_json(const Plan &p, int R, int Q, int dim, int k);
 bool check_args(const void *ref, int R, const void *query, int Q, int dim, int k, const void *dist,
                 const void *idx);
 bool ensure_device();
+
+// Device-side order of consecutive calls.  Calls share the library's cached
+// device buffers (the plain search's workspace, the index search's, the
+// batched lanes') and a call may return while its kernels still run.  A
+// call on another stream than the previous call's therefore first makes its
+// stream wait for an event recorded at the end of the previous call, so no
+// call's kernels touch a buffer that the previous call's kernels still use.
+// Construct after ensure_device(), with g_mu held.
+class CallOrder {
+  public:
+    explicit CallOrder(hipStream_t s);
+    ~CallOrder();
+    CallOrder(const CallOrder &) = delete;
+    CallOrder &operator=(const CallOrder &) = delete;
+
+  private:
+    hipStream_t s_;
+};
 knnhip_options default_options(int variant);
 // Key error coefficients of the bfloat16 three-term products over nch
 // 16-dimension chunks (for FinalArgs::err_q / err_r).

@@ -59,6 +59,39 @@ void *ws_get(Slot s, size_t bytes) {
     return p;
 }
 
+const unsigned *g_last_nonfinite = nullptr;
+namespace {
+int g_nonfinite_holds = 0;
+}  // namespace
+NonfiniteHold::NonfiniteHold() { ++g_nonfinite_holds; }
+NonfiniteHold::~NonfiniteHold() { --g_nonfinite_holds; }
+
+bool nonfinite_clear(hipStream_t s) {
+    unsigned *f = (unsigned *)ws_get(kNonFinite, sizeof(unsigned));
+    if (!f) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(hipMemsetAsync(f, 0, sizeof(unsigned), s), "memset");
+    return true;
+}
+
+bool nonfinite_check(hipStream_t s) {
+    const unsigned *f = (const unsigned *)ws_get(kNonFinite, sizeof(unsigned));
+    unsigned bad = 0;
+    if (!f) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(hipMemcpyAsync(&bad, f, sizeof(unsigned), hipMemcpyDeviceToHost, s), "copy");
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    if (bad) {
+        set_error("non-finite input: a reference or query point has a NaN or infinite squared norm");
+        return false;
+    }
+    return true;
+}
+
 namespace {
 
 // Cost model of one slice count: the scan runs in `rounds` waves of
@@ -239,6 +272,35 @@ bool ensure_device() {
     return true;
 }
 
+namespace {
+hipEvent_t g_call_ev = nullptr;     // end of the last call's work
+hipStream_t g_call_stream = nullptr;
+bool g_call_recorded = false;
+}  // namespace
+
+CallOrder::CallOrder(hipStream_t s) : s_(s) {
+    if (g_call_recorded && g_call_stream != s && hipStreamWaitEvent(s, g_call_ev, 0) != hipSuccess) {
+        // cannot order on the device: wait on the host instead
+        (void)hipGetLastError();
+        (void)hipEventSynchronize(g_call_ev);
+    }
+}
+
+CallOrder::~CallOrder() {
+    if (!g_call_ev && hipEventCreateWithFlags(&g_call_ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        g_call_ev = nullptr;
+        return;
+    }
+    if (hipEventRecord(g_call_ev, s_) != hipSuccess) {
+        (void)hipGetLastError();
+        g_call_recorded = false;
+        return;
+    }
+    g_call_stream = s_;
+    g_call_recorded = true;
+}
+
 // Key error bound of the precision scan (see knn_scan_bf16.hip): with
 // u = 2^-8, n = 3 * 16 * nch + 1 FP32 additions (unit 2^-23, allowing for any
 // rounding inside the MFMA) and the FP32 norm over dim terms,
@@ -319,11 +381,15 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
     }
     for (float &m : g_stage_ms) m = -1.f;
     int *fb_count = (int *)ws_get(kFbCount, sizeof(int));
-    if (!fb_count) {
+    unsigned *nonfinite = (unsigned *)ws_get(kNonFinite, sizeof(unsigned));
+    if (!fb_count || !nonfinite) {
         set_error("Memory allocation error");
         return false;
     }
     KNN_TRY(hipMemsetAsync(fb_count, 0, sizeof(int), s), "memset");
+    if (level == 0 && g_nonfinite_holds == 0)
+        KNN_TRY(hipMemsetAsync(nonfinite, 0, sizeof(unsigned), s), "memset");
+    g_last_nonfinite = nonfinite;
 
     ExactArgs ea{};
     ea.ref = ref;
@@ -336,6 +402,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
     ea.k = k;
     ea.out_dist = out_dist;
     ea.out_idx = out_idx;
+    ea.nonfinite = nonfinite;
     const int egrid = std::min(kExactGrid, Q);
     ea.scratch_key = (float *)ws_get(kFbKey, (size_t)egrid * kWave * k * sizeof(float));
     ea.scratch_idx = (int *)ws_get(kFbIdx, (size_t)egrid * kWave * k * sizeof(int));
@@ -345,6 +412,24 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
     }
 
     if (!p.use_scan) {
+        // the norms only for the non-finite check
+        float *rn = (float *)ws_get(kRNorm, (size_t)R * sizeof(float));
+        float *qn = (float *)ws_get(kQNorm, (size_t)Q * sizeof(float));
+        if (!rn || !qn) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        for (int side = 0; side < 2; ++side) {
+            NormArgs nc{};
+            nc.p = side ? query : ref;
+            nc.ld = side ? ldq : ldr;
+            nc.n = nc.n_pad = side ? Q : R;
+            nc.dim = dim;
+            nc.oct = (dim + 7) / 8;
+            nc.out = side ? qn : rn;
+            nc.nonfinite = nonfinite;
+            KNN_TRY(launch_norms(nc, s), "norm kernel launch");
+        }
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[3], s), "event");
         KNN_TRY(launch_exact(ea, egrid, s), "exact scan launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[4], s), "event");
@@ -383,6 +468,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         na.maxbits = rmax;
         na.pm = ref_pm;
         na.pm_ld = pm_ld;
+        na.nonfinite = nonfinite;
         if (p.bf3) {
             na.packed_bf = (unsigned short *)packed;
             na.nch_bf = p.nch;
@@ -399,6 +485,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         nq.out = qnorm;
         nq.pm = qry_pm;
         nq.pm_ld = pm_ld;
+        nq.nonfinite = nonfinite;
         KNN_TRY(launch_norms(nq, s), "norm kernel launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[1], s), "event");
 
@@ -485,6 +572,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         fa.out_idx = out_idx;
         fa.fb_count = fb_count;
         fa.fb_list = fb_list;
+        fa.nonfinite = nonfinite;
         KNN_TRY(launch_select(fa, s), "select kernel launch");
         if (timing) KNN_TRY(hipEventRecord(g_ws.ev[3], s), "event");
 
@@ -573,12 +661,14 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
     if (!check_args(ref, R, query, Q, dim, k, knn_dist, kth_only ? (const void *)knn_dist : knn_index))
         return false;
     if (!ensure_device()) return false;
+    CallOrder order(nullptr);
     const knnhip_options o = default_options(variant);
     // Problems beyond the device memory budget stream through it.
     bool explicit_budget = false;
     const size_t budget = device_budget(&explicit_budget);
     if (search_footprint(R, Q, dim, k, o, false) > budget)
-        return run_streamed(ref, R, query, Q, dim, k, knn_dist, knn_index, o, budget, kth_only);
+        return run_streamed(ref, R, query, Q, dim, k, knn_dist, knn_index, o, budget, kth_only) &&
+               nonfinite_check(nullptr);
     Plan p = make_plan(R, Q, dim, k, o);
     // Large reference sets go up in chunks of whole slices, each scanned as
     // soon as it lands, so the copy overlaps the search.
@@ -597,6 +687,8 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
         }
     }
     g_last_plan = plan_json(p, R, Q, dim, k);
+    g_last_plan.pop_back();
+    g_last_plan += ", \"variant\": " + std::to_string(variant) + "}";
     if (chunked) {
         g_last_plan.pop_back();
         g_last_plan += ", \"upload_chunks\": " + std::to_string(std::min(kUploadChunks, p.nslices)) + "}";
@@ -638,13 +730,13 @@ bool run_host(const float *ref, int R, const float *query, int Q, int dim, int k
         KNN_TRY(hipMemcpy(knn_dist, dist_d + (size_t)(k - 1) * Q, (size_t)Q * sizeof(float),
                           hipMemcpyDeviceToHost),
                 "Unable to copy data from device to host");
-        return true;
+        return nonfinite_check(nullptr);
     }
     KNN_TRY(hipMemcpy(knn_dist, dist_d, (size_t)k * Q * sizeof(float), hipMemcpyDeviceToHost),
             "Unable to copy data from device to host");
     KNN_TRY(hipMemcpy(knn_index, idx_d, (size_t)k * Q * sizeof(int), hipMemcpyDeviceToHost),
             "Unable to copy data from device to host");
-    return true;
+    return nonfinite_check(nullptr);
 }
 
 }  // namespace
@@ -715,8 +807,11 @@ int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const f
     if (!ensure_device()) return 0;
     knnhip_options o = opt ? *opt : default_options(0);
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(s);
     const Plan p = make_plan(ref_nb, query_nb, dim, k, o);
     g_last_plan = plan_json(p, ref_nb, query_nb, dim, k);
+    g_last_plan.pop_back();
+    g_last_plan += ", \"variant\": " + std::to_string(o.variant) + "}";
     return run_device(ref, ref_ld, ref_nb, query, query_ld, query_nb, dim, k, knn_dist, knn_index,
                       p, o, s)
                ? 1
@@ -727,9 +822,16 @@ int knnhip_last_fallback_count(void) {
     std::lock_guard<std::mutex> lock(g_mu);
     if (!g_last_fb) return 0;
     int n = -1;
+    unsigned bad = 0;
     if (hipStreamSynchronize(g_last_stream) != hipSuccess ||
-        hipMemcpy(&n, g_last_fb, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+        hipMemcpy(&n, g_last_fb, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        (g_last_nonfinite &&
+         hipMemcpy(&bad, g_last_nonfinite, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)) {
         set_error("cannot read the fallback count");
+        return -1;
+    }
+    if (bad) {
+        set_error("non-finite input: a reference or query point has a NaN or infinite squared norm");
         return -1;
     }
     return n;

@@ -58,6 +58,12 @@ hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcen
                              hipStream_t s);
 hipError_t idx_ball_aux(const float *fnorm, const double *frad, int nf, float *fsq, float *fradf,
                         hipStream_t s);
+// Ball radii refitted to the block-ordered points pm (a loaded index):
+// every point joins the nearest ball of its block.  d2bits / member: nf
+// words of scratch each; rad: nf radii (-1: no member).
+hipError_t idx_ball_refit(const float *pm, int ld, int dim, const int *block_start, const int *block_count,
+                          const int *fb_start, int nb, const float *fcent, int nf,
+                          unsigned long long *d2bits, int *member, double *rad, hipStream_t s);
 // cbf (nullable): bfloat16 hi/lo ball centres (idx_pack_ball_bf3) for
 // three-term bound products over nch_bf 16-dim chunks (dim <= 128), with
 // error coefficient g15_bf; else FP32 products.

@@ -903,6 +903,52 @@ __global__ __launch_bounds__(256) void finalize_fine_kernel(const float *__restr
     }
 }
 
+// Radii of the balls of a loaded index, refitted to the reference set it is
+// loaded over: every point of block b joins the ball of b whose centre is
+// nearest (FP64), and a ball's radius becomes the largest such member
+// distance.  Every point then lies within the radius of a ball of its own
+// block, which is all the block lower bounds need, whatever the file held.
+// One workgroup per block; d2bits: nf zeroed words (FP64 bit patterns of
+// non-negative squared distances order like the values); member: nf zeroed
+// flags.
+__global__ __launch_bounds__(256) void ball_refit_kernel(const float *__restrict__ pm, int ld, int dim,
+                                                         const int *__restrict__ block_start,
+                                                         const int *__restrict__ block_count,
+                                                         const int *__restrict__ fb_start,
+                                                         const float *__restrict__ fcent,
+                                                         unsigned long long *d2bits, int *member) {
+    const int b = blockIdx.x;
+    const long long p0 = block_start[b];
+    const int n = block_count[b], f0 = fb_start[b], f1 = fb_start[b + 1];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const float *r = pm + (p0 + j) * ld;
+        double best = INFINITY;
+        int bf = -1;
+        for (int f = f0; f < f1; ++f) {
+            const float *c = fcent + (long long)f * ld;
+            double acc = 0.0;
+            for (int d = 0; d < dim; ++d) {
+                const double t = (double)r[d] - (double)c[d];
+                acc += t * t;
+            }
+            if (acc < best) {
+                best = acc;
+                bf = f;
+            }
+        }
+        if (bf >= 0) {
+            atomicMax(d2bits + bf, (unsigned long long)__double_as_longlong(best));
+            atomicOr(member + bf, 1);
+        }
+    }
+}
+
+__global__ void ball_refit_finish_kernel(const unsigned long long *__restrict__ d2bits,
+                                         const int *__restrict__ member, int nf, double *rad) {
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x)
+        rad[f] = member[f] ? sqrt(__longlong_as_double((long long)d2bits[f])) * (1.0 + 1e-12) : -1.0;
+}
+
 // ---------------------------------------------------------------------------
 // Search: block lower bounds
 // ---------------------------------------------------------------------------
@@ -1414,6 +1460,18 @@ hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcen
                              hipStream_t s) {
     hipLaunchKernelGGL(finalize_fine_kernel, dim3(nf), dim3(256), 0, s, pm, ld, dim, fcent,
                        fine_sorted, fstart, fcount, fblock, cent_s, rad_s, block_s);
+    return hipGetLastError();
+}
+hipError_t idx_ball_refit(const float *pm, int ld, int dim, const int *block_start, const int *block_count,
+                          const int *fb_start, int nb, const float *fcent, int nf,
+                          unsigned long long *d2bits, int *member, double *rad, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(d2bits, 0, (size_t)nf * sizeof(unsigned long long), s);
+    if (e == hipSuccess) e = hipMemsetAsync(member, 0, (size_t)nf * sizeof(int), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ball_refit_kernel, dim3(nb), dim3(256), 0, s, pm, ld, dim, block_start, block_count,
+                       fb_start, fcent, d2bits, member);
+    hipLaunchKernelGGL(ball_refit_finish_kernel, dim3(grid_for(nf)), dim3(256), 0, s, d2bits, member, nf,
+                       rad);
     return hipGetLastError();
 }
 hipError_t idx_ball_aux(const float *fnorm, const double *frad, int nf, float *fsq, float *fradf,

@@ -269,6 +269,45 @@ bool index_save(const knnhip_index *ix, const char *path) {
     return ok;
 }
 
+// Structural checks of a loaded index, or nullptr when it is consistent:
+// blocks start on whole tiles, in order, and end by n_pos; the first count
+// positions of each block hold distinct reference indexes in [0, R) and the
+// rest -1, so that every reference appears exactly once; ball ranges
+// partition [0, nf) in block order, and every non-empty block has a ball;
+// centres are finite.
+const char *index_layout_error(int R, int dim, int nb, int nf, int n_pos, int r_pad, int pm_ld, const int *bs,
+                               const int *bc, const int *perm, const int *fbs, const int *fbl, const float *fc) {
+    if (r_pad != (n_pos + kRefsPerStep - 1) / kRefsPerStep * kRefsPerStep) return "padded size";
+    if (bs[0] != 0 || bs[nb] != n_pos) return "block starts";
+    if (fbs[0] != 0 || fbs[nb] != nf) return "ball ranges";
+    std::vector<char> seen(R, 0);
+    long long found = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (bs[b] % kRefsPerTile != 0 || bc[b] < 0 || (long long)bs[b] + bc[b] > bs[b + 1])
+            return "block starts";
+        if (fbs[b + 1] < fbs[b] || (bc[b] > 0 && fbs[b + 1] == fbs[b])) return "ball ranges";
+        for (int f = fbs[b]; f < fbs[b + 1]; ++f)
+            if (fbl[f] != b) return "ball blocks";
+        for (int p = bs[b]; p < bs[b + 1]; ++p) {
+            const int v = perm[p];
+            if (p < bs[b] + bc[b]) {
+                if (v < 0 || v >= R || seen[v]) return "permutation";
+                seen[v] = 1;
+                ++found;
+            } else if (v != -1) {
+                return "permutation";
+            }
+        }
+    }
+    for (int p = n_pos; p < r_pad; ++p)
+        if (perm[p] != -1) return "permutation";
+    if (found != R) return "permutation";
+    for (int f = 0; f < nf; ++f)
+        for (int d = 0; d < dim; ++d)
+            if (!std::isfinite(fc[(size_t)f * pm_ld + d])) return "ball centres";
+    return nullptr;
+}
+
 bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, long long ldr, int dim,
                 hipStream_t s) {
     FILE *f = fopen(path, "rb");
@@ -315,14 +354,22 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
     ix->nchunks = p.nchunks;
     ix->dpad = p.dpad;
     ix->oct = p.oct;
-    in.get_dev(ix->block_start, (size_t)(ix->nb + 1) * 4);
-    in.get_dev(ix->block_count, (size_t)ix->nb * 4);
-    in.get_dev(ix->perm, (size_t)ix->r_pad * 4);
-    in.get_dev(ix->fcent, (size_t)ix->nf * ix->pm_ld * 4);
-    in.get_dev(ix->fnorm, (size_t)ix->nf * 4);
-    in.get_dev(ix->frad, (size_t)ix->nf * 8);
-    in.get_dev(ix->fblock, (size_t)ix->nf * 4);
-    in.get_dev(ix->fb_start, (size_t)(ix->nb + 1) * 4);
+    // The arrays pass through host memory and are checked before any kernel
+    // indexes with them.  The centres' norms and the radii in the file are
+    // not trusted: both are recomputed below from the centres and from the
+    // reference set the index is loaded over.
+    const int nb = ix->nb, nf = ix->nf;
+    std::vector<int> bs(nb + 1), bc(nb), perm(ix->r_pad), fbl(nf), fbs(nb + 1);
+    std::vector<float> fc((size_t)nf * ix->pm_ld), fn(nf);
+    std::vector<double> fr(nf);
+    in.get(bs.data(), bs.size() * 4);
+    in.get(bc.data(), bc.size() * 4);
+    in.get(perm.data(), perm.size() * 4);
+    in.get(fc.data(), fc.size() * 4);
+    in.get(fn.data(), fn.size() * 4);
+    in.get(fr.data(), fr.size() * 8);
+    in.get(fbl.data(), fbl.size() * 4);
+    in.get(fbs.data(), fbs.size() * 4);
     char extra;
     const bool tail = fread(&extra, 1, 1, f) == 0;
     fclose(f);
@@ -330,18 +377,31 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
         set_error("%s is truncated or has trailing data", path);
         return false;
     }
-    // rebuild the layouts from the references
-    DevBuf pm0, norms0;
-    if (!pm0.alloc((size_t)R * ix->pm_ld * 4) || !norms0.alloc((size_t)R * 4) || !ix->rmax.alloc(4) ||
-        !ix->fsq.alloc((size_t)ix->nf * 4) || !ix->fradf.alloc((size_t)ix->nf * 4) ||
-        !ix->packed.alloc((size_t)ix->r_pad * ix->dpad * 4) ||
-        !ix->pm.alloc((size_t)ix->r_pad * ix->pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4)) {
+    if (const char *why = index_layout_error(R, dim, nb, nf, ix->n_pos, ix->r_pad, ix->pm_ld, bs.data(),
+                                             bc.data(), perm.data(), fbs.data(), fbl.data(), fc.data())) {
+        set_error("%s: inconsistent index (%s)", path, why);
+        return false;
+    }
+    auto upload = [](DevBuf &b, const void *src, size_t n) {
+        return b.alloc(n) && hipMemcpy(b.p, src, n, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!upload(ix->block_start, bs.data(), bs.size() * 4) || !upload(ix->block_count, bc.data(), bc.size() * 4) ||
+        !upload(ix->perm, perm.data(), perm.size() * 4) || !upload(ix->fcent, fc.data(), fc.size() * 4) ||
+        !upload(ix->fblock, fbl.data(), fbl.size() * 4) || !upload(ix->fb_start, fbs.data(), fbs.size() * 4) ||
+        !ix->fnorm.alloc((size_t)nf * 4) || !ix->frad.alloc((size_t)nf * 8)) {
         set_error("Memory allocation error");
         return false;
     }
-    KNN_TRY(idx_ball_aux(ix->fnorm.as<float>(), ix->frad.as<double>(), ix->nf, ix->fsq.as<float>(),
-                         ix->fradf.as<float>(), s),
-            "ball roots");
+    // rebuild the layouts from the references
+    DevBuf pm0, norms0, d2bits, member;
+    if (!pm0.alloc((size_t)R * ix->pm_ld * 4) || !norms0.alloc((size_t)R * 4) || !ix->rmax.alloc(4) ||
+        !ix->fsq.alloc((size_t)ix->nf * 4) || !ix->fradf.alloc((size_t)ix->nf * 4) ||
+        !ix->packed.alloc((size_t)ix->r_pad * ix->dpad * 4) ||
+        !ix->pm.alloc((size_t)ix->r_pad * ix->pm_ld * 4) || !ix->norms.alloc((size_t)ix->r_pad * 4) ||
+        !d2bits.alloc((size_t)nf * 8) || !member.alloc((size_t)nf * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
     KNN_TRY(hipMemsetAsync(ix->rmax.p, 0, 4, s), "memset");
     NormArgs na{};
     na.p = ref;
@@ -360,6 +420,15 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
                             ix->r_pad, ix->packed.as<float>(), ix->pm.as<float>(), ix->norms.as<float>(),
                             s),
             "gather");
+    // balls: norms of the centres, radii over the points they now hold
+    KNN_TRY(idx_row_norms(ix->fcent.as<float>(), ix->pm_ld, nf, dim, ix->fnorm.as<float>(), s), "norms");
+    KNN_TRY(idx_ball_refit(ix->pm.as<float>(), ix->pm_ld, dim, ix->block_start.as<int>(),
+                           ix->block_count.as<int>(), ix->fb_start.as<int>(), nb, ix->fcent.as<float>(), nf,
+                           d2bits.as<unsigned long long>(), member.as<int>(), ix->frad.as<double>(), s),
+            "ball radii");
+    KNN_TRY(idx_ball_aux(ix->fnorm.as<float>(), ix->frad.as<double>(), ix->nf, ix->fsq.as<float>(),
+                         ix->fradf.as<float>(), s),
+            "ball roots");
     KNN_TRY(hipStreamSynchronize(s), "sync");
     return true;
 }
@@ -682,6 +751,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         g_stage_ms[3] = ix->last_ms[4];
     }
     g_last_fb = w.fb_count.as<int>();
+    g_last_nonfinite = nullptr;
     return true;
 }
 
@@ -703,6 +773,7 @@ int knnhip_index_build(const float *ref, int ref_nb, long long ref_ld, int dim, 
     }
     *out = nullptr;
     if (!ensure_device()) return 0;
+    CallOrder order((hipStream_t)stream);
     knnhip_index *ix = new knnhip_index();
     if (!index_build(ix, ref, ref_nb, ref_ld, dim, nblocks, iters, (hipStream_t)stream)) {
         delete ix;
@@ -727,6 +798,7 @@ int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long
         return 0;
     }
     if (!ensure_device()) return 0;
+    CallOrder order((hipStream_t)stream);
     knnhip_options o = opt ? *opt : default_options(0);
     char buf[256];
     const bool bf3 = o.precision == 1 && scan_bf3_task_nch(ix->dim) != 0;
@@ -762,6 +834,7 @@ int knnhip_index_search_routed(knnhip_index *ix, knnhip_router *router, const fl
         return 0;
     }
     if (!ensure_device()) return 0;
+    CallOrder order((hipStream_t)stream);
     knnhip_options o = opt ? *opt : default_options(0);
     char buf[256];
     const bool bf3 = o.precision == 1 && scan_bf3_task_nch(ix->dim) != 0;
@@ -786,6 +859,7 @@ int knnhip_index_bounds(knnhip_index *ix, const float *query, int query_nb, long
         return 0;
     }
     if (!ensure_device()) return 0;
+    CallOrder order((hipStream_t)stream);
     IndexWork &w = g_iw;
     if (!w.need(w.qnorm, (size_t)query_nb * 4) || !w.need(w.qpm, (size_t)query_nb * ix->pm_ld * 4)) {
         set_error("Memory allocation error");
@@ -834,6 +908,7 @@ int knnhip_index_load(const char *path, const float *ref, int ref_nb, long long 
     }
     *out = nullptr;
     if (!ensure_device()) return 0;
+    CallOrder order((hipStream_t)stream);
     knnhip_index *ix = new knnhip_index();
     if (!index_load(ix, path, ref, ref_nb, ref_ld, dim, (hipStream_t)stream)) {
         delete ix;
@@ -868,6 +943,7 @@ int knnhip_index_layout(const knnhip_index *ix, int *block_start, int *block_cou
     std::lock_guard<std::mutex> lock(g_mu);
     g_last_error.clear();
     if (!ensure_device()) return 0;
+    CallOrder order((hipStream_t)stream);
     hipStream_t s = (hipStream_t)stream;
     if (ref_block)
         KNN_TRY(idx_ref_block(ix->block_start.as<int>(), ix->block_count.as<int>(), ix->nb,

@@ -449,6 +449,7 @@ int knnhip_router_loss_grad(knnhip_router *r, const float *x, int n, long long l
     g_last_error.clear();
     if (!router_batch_args(r, x, n, ld, tgt, ldt, k)) return 0;
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(s);
     if (!router_loss_grad(r, x, n, ld, tgt, ldt, k, s)) return 0;
     if (grad)
         KNN_TRY(hipMemcpyAsync(grad, r->grad.p, (size_t)r->lay.total * 4, hipMemcpyDeviceToDevice, s),
@@ -468,6 +469,7 @@ int knnhip_router_step(knnhip_router *r, const float *x, int n, long long ld, co
     std::lock_guard<std::mutex> lock(g_mu);
     g_last_error.clear();
     if (!router_batch_args(r, x, n, ld, tgt, ldt, k)) return 0;
+    CallOrder order((hipStream_t)stream);
     return router_step(r, x, n, ld, tgt, ldt, k, lr, (hipStream_t)stream) ? 1 : 0;
 }
 
@@ -482,6 +484,7 @@ int knnhip_router_train(knnhip_router *r, const knnhip_index *ix, const float *t
         return 0;
     }
     if (!ensure_device()) return 0;
+    CallOrder order((hipStream_t)stream);
     return router_train(r, ix, train_query, n, ld, k, epochs, std::min(batch, n), lr, seed,
                         final_loss, (hipStream_t)stream)
                ? 1
@@ -497,6 +500,7 @@ int knnhip_router_top1(knnhip_router *r, const float *query, int query_nb, long 
         return 0;
     }
     if (!ensure_device()) return 0;
+    CallOrder order((hipStream_t)stream);
     return router_pick_first(r, query, query_nb, query_ld, nullptr, top1, nullptr, (hipStream_t)stream) ? 1
                                                                                                           : 0;
 }
@@ -509,6 +513,7 @@ int knnhip_router_scores(knnhip_router *r, const float *query, int query_nb, lon
         set_error("bad router score arguments");
         return 0;
     }
+    CallOrder order((hipStream_t)stream);
     return router_scores(r, query, query_nb, query_ld, scores, (hipStream_t)stream) ? 1 : 0;
 }
 }  // extern "C"

@@ -702,6 +702,10 @@ __device__ __forceinline__ void norms_pack_points(const NormArgs &a, int i0) {
     if (t < kPts) {
         const int i = i0 + t;
         if (i < a.n) a.out[i] = s;
+        // NaN or infinite coordinates (or squares that overflow) leave a norm
+        // that is not a finite float: the search reports the input instead
+        // of ordering NaN keys
+        if (a.nonfinite && i < a.n && !(s <= 3.402823466e38f)) atomicOr(a.nonfinite, 1u);
         if (a.maxbits) {
             unsigned mbits = i < a.n ? __float_as_uint(s) : 0u;
 #pragma unroll

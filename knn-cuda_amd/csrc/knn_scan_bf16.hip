@@ -20,9 +20,15 @@
 // block).  Global loads run PF chunks ahead in registers; each wave reads a
 // chunk's operands from LDS one chunk ahead of its MFMAs; one barrier per
 // two chunks hands slots between the loading threads and the computing
-// waves (chunk g + 3 is stored during chunk g: with a barrier after every
-// odd chunk, a chunk read one ahead was stored before the last barrier, and
-// a slot is rewritten only after every wave's last read of it passed one).
+// waves.  Chunk g + 3 is stored during chunk g into slot (g + 3) % 4, and
+// chunk g + 1 is read during chunk g.  With a barrier after every odd chunk
+// and one after the prologue's read of chunk 0:
+//   read-after-write: chunk g + 1 was stored in iteration g - 2 (or in the
+//     prologue), and a barrier lies between (after g - 1 or g - 2, whichever
+//     is odd);
+//   write-after-read: slot (g + 3) % 4 last held chunk g - 1, read in
+//     iteration g - 2 (chunk 0: in the prologue), and a barrier lies between
+//     that read and iteration g's store (for g = 1 it is the prologue's).
 #include "knn_common.h"
 #include "knn_heap.h"
 
@@ -215,6 +221,13 @@ __global__ __launch_bounds__(kBfWaves * kWave, 1) void scan_lds_kernel(ScanArgs 
     };
     u32x4_t cur[4];
     read_ops(0, cur);
+    // Slot 0 is rewritten (with chunk 4) in iteration 1, before the loop's
+    // first barrier: every wave's read of chunk 0 above must be done first.
+    // Without this barrier a wave that reached iteration 1 early could
+    // overwrite chunk 0 while a slower wave had yet to read it, and that
+    // wave would then multiply chunk 4's tiles as chunk 0's: wrong keys for
+    // its first step's references, which the certificate cannot see.
+    lds_barrier();
 
     for (int st = 0; st < nsteps; ++st) {
         const int r0 = r_begin + st * kRefsPerStep;

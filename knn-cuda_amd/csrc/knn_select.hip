@@ -43,6 +43,14 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
     const int lane = threadIdx.x & 63;
     const int q = blockIdx.x * waves + wave;
     if (q >= a.Q) return;
+    if (a.nonfinite && *a.nonfinite) {
+        // non-finite input: no answer, and no fallback either
+        for (int j = lane; j < a.k; j += kWave) {
+            a.out_dist[(long long)j * a.Q + q] = __uint_as_float(0x7FC00000u);
+            a.out_idx[(long long)j * a.Q + q] = -1;
+        }
+        return;
+    }
 
     const int maxc = a.maxc;
     int *cidx = reinterpret_cast<int *>(smem) + (size_t)wave * maxc;      // reference position
@@ -183,8 +191,17 @@ __global__ __launch_bounds__(64) void exact_scan_kernel(ExactArgs a) {
     const int k = a.k;
     float *mykey = a.scratch_key + ((long long)blockIdx.x * kWave + lane) * k;
     int *myidx = a.scratch_idx + ((long long)blockIdx.x * kWave + lane) * k;
+    const bool bad = a.nonfinite && *a.nonfinite;
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         const int q = a.list ? a.list[t] : t;
+        if (bad) {
+            // non-finite input (see FinalArgs::nonfinite)
+            for (int j = lane; j < k; j += kWave) {
+                a.out_dist[(long long)j * a.Q + q] = __uint_as_float(0x7FC00000u);
+                a.out_idx[(long long)j * a.Q + q] = -1;
+            }
+            continue;
+        }
         int filled = 0;
         for (int r = lane; r < a.R; r += kWave) {
             const float key = sqrt_rn(canonical_sq(a.ref, a.ldr, r, a.query, a.ldq, q, a.dim));

@@ -172,6 +172,7 @@ size_t device_budget(bool *explicit_budget) {
 void ws_release() {
     (void)hipDeviceSynchronize();
     g_last_fb = nullptr;
+    g_last_nonfinite = nullptr;
     for (int s = 0; s < kNumSlots; ++s) {
         if (g_ws.ptr[s]) (void)hipFree(g_ws.ptr[s]);
         g_ws.ptr[s] = nullptr;
@@ -187,6 +188,10 @@ bool run_streamed(const float *ref, int R, const float *query, int Q, int dim, i
         ws_release();
         g_sb.release();
     }
+    // one non-finite flag for every chunk search of the call (the caller
+    // checks it at the end)
+    if (!nonfinite_clear(nullptr)) return false;
+    NonfiniteHold hold;
     int Rc = 0, Qc = 0;
     if (!stream_chunks(R, Q, dim, k, o, budget, &Rc, &Qc)) return false;
     const int nr = (int)(((long long)R + Rc - 1) / Rc);

@@ -268,6 +268,45 @@ def kth_distance(ref, query, k: int):
     return out
 
 
+def _check_points(name: str, t, dim: Optional[int] = None):
+    """A dim x n float32 point set with contiguous rows (shape and dtype first, device last, so
+    that shape errors are reported the same way with or without a GPU)."""
+    import torch
+
+    if not isinstance(t, torch.Tensor) or t.ndim != 2:
+        raise KnnError(f"{name} must be a 2-D (dim x n) tensor")
+    if t.dtype != torch.float32:
+        raise KnnError(f"{name} must be float32")
+    if dim is not None and t.shape[0] != dim:
+        raise KnnError(f"{name} has dim {t.shape[0]}, expected {dim} (ref and query must share dim)")
+    if t.shape[1] < 1 or t.shape[0] < 1:
+        raise KnnError(f"{name} must hold at least one point of at least one dimension")
+    if t.stride(1) != 1 or t.stride(0) < t.shape[1]:
+        raise KnnError(f"{name} must have contiguous rows (row stride >= n)")
+
+
+def _check_out(name: str, t, shape, dtype, device):
+    """A caller-supplied output: exactly `shape`, contiguous, of `dtype`, on `device`."""
+    import torch
+
+    if not isinstance(t, torch.Tensor) or tuple(t.shape) != tuple(shape):
+        raise KnnError(f"{name} must be a tensor of shape {tuple(shape)}")
+    if t.dtype != dtype:
+        raise KnnError(f"{name} must be {dtype}")
+    if not t.is_contiguous():
+        raise KnnError(f"{name} must be contiguous")
+    if t.device != device:
+        raise KnnError(f"{name} must be on {device}")
+
+
+def _check_device(*ts):
+    for t in ts:
+        if not t.is_cuda:
+            raise KnnError("tensors must be device (GPU) tensors")
+    if any(t.device != ts[0].device for t in ts):
+        raise KnnError("tensors must be on one device")
+
+
 def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "global",
                   margin: int = 0, slices: int = 0, force_fallback: bool = False,
                   precision: str = "fp32", block_queries: int = 0,
@@ -278,15 +317,18 @@ def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "
     """
     import torch
 
-    lib = load()
-    if ref.dtype != torch.float32 or query.dtype != torch.float32:
-        raise KnnError("ref and query must be float32")
-    if not (ref.is_cuda and query.is_cuda):
-        raise KnnError("ref and query must be device tensors")
-    if ref.stride(1) != 1 or query.stride(1) != 1:
-        raise KnnError("rows must be contiguous")
+    _check_points("ref", ref)
+    _check_points("query", query, dim=ref.shape[0])
     dim, ref_nb = ref.shape
     query_nb = query.shape[1]
+    if not 1 <= k <= ref_nb:
+        raise KnnError(f"k must satisfy 1 <= k <= ref_nb (k={k} ref_nb={ref_nb})")
+    if dist is not None:
+        _check_out("dist", dist, (k, query_nb), torch.float32, query.device)
+    if index is not None:
+        _check_out("index", index, (k, query_nb), torch.int32, query.device)
+    _check_device(ref, query)
+    lib = load()
     if dist is None:
         dist = torch.empty((k, query_nb), dtype=torch.float32, device=query.device)
     if index is None:
@@ -436,8 +478,8 @@ class BlockIndex:
         """The block lower bounds searches rank and prune by: (nblocks, query_nb) float32."""
         import torch
 
-        if query.dtype != torch.float32 or not query.is_cuda or query.stride(1) != 1:
-            raise KnnError("query must be a float32 device tensor with contiguous rows")
+        _check_points("query", query, dim=self.dim)
+        _check_device(query, self.ref)
         out = torch.empty((self.nblocks, query.shape[1]), dtype=torch.float32, device=query.device)
         ok = load().knnhip_index_bounds(self._h, query.data_ptr(), query.shape[1], query.stride(0),
                                         1 if precision == "bf16x3" else 0, out.data_ptr(),
@@ -456,11 +498,15 @@ class BlockIndex:
         import torch
 
         lib = load()
-        if query.dtype != torch.float32 or not query.is_cuda or query.stride(1) != 1:
-            raise KnnError("query must be a float32 device tensor with contiguous rows")
+        _check_points("query", query, dim=self.dim)
         dim, query_nb = query.shape
-        if dim != self.dim:
-            raise KnnError("query dim differs from the index")
+        if not 1 <= k <= self.ref_nb:
+            raise KnnError(f"k must satisfy 1 <= k <= ref_nb (k={k} ref_nb={self.ref_nb})")
+        if dist is not None:
+            _check_out("dist", dist, (k, query_nb), torch.float32, query.device)
+        if index is not None:
+            _check_out("index", index, (k, query_nb), torch.int32, query.device)
+        _check_device(query, self.ref)
         if dist is None:
             dist = torch.empty((k, query_nb), dtype=torch.float32, device=query.device)
         if index is None:
@@ -568,14 +614,14 @@ class Router:
         if not load().knnhip_router_params(self._h, p.ctypes.data, p.size, 1):
             raise KnnError(last_error())
 
-    @staticmethod
-    def _check(x, tgt):
+    def _check(self, x, tgt):
         import torch
 
-        if x.dtype != torch.float32 or not x.is_cuda or x.stride(1) != 1:
-            raise KnnError("x must be a float32 device tensor with contiguous rows (dim x n)")
-        if tgt.dtype != torch.int32 or not tgt.is_cuda or tgt.stride(1) != 1:
-            raise KnnError("tgt must be an int32 device tensor with contiguous rows (k x n)")
+        _check_points("x", x, dim=self.dim)
+        if not isinstance(tgt, torch.Tensor) or tgt.ndim != 2 or tgt.dtype != torch.int32 or \
+                tgt.stride(1) != 1 or tgt.shape[1] != x.shape[1] or tgt.shape[0] < 1:
+            raise KnnError("tgt must be an int32 (k x n) tensor with contiguous rows, n as in x")
+        _check_device(x, tgt)
 
     def loss_grad(self, x, tgt, stream: Optional[int] = None):
         import torch
@@ -601,6 +647,8 @@ class Router:
 
     def train(self, index: "BlockIndex", train_query, k: int, epochs: int = 10, batch: int = 1024,
               lr: float = 1e-3, seed: int = 0x6B4E4E17, stream: Optional[int] = None) -> float:
+        _check_points("train_query", train_query, dim=self.dim)
+        _check_device(train_query)
         loss = ctypes.c_float()
         ok = load().knnhip_router_train(self._h, index._h, train_query.data_ptr(),
                                         train_query.shape[1], train_query.stride(0), k, epochs,
@@ -613,6 +661,10 @@ class Router:
     def scores(self, query, out=None, stream: Optional[int] = None):
         import torch
 
+        _check_points("query", query, dim=self.dim)
+        if out is not None:
+            _check_out("out", out, (self.nblocks, query.shape[1]), torch.float32, query.device)
+        _check_device(query)
         if out is None:
             out = torch.empty((self.nblocks, query.shape[1]), dtype=torch.float32,
                               device=query.device)
@@ -626,6 +678,10 @@ class Router:
         """Each query's highest-scoring block (int32, query_nb) from the fused inference kernel."""
         import torch
 
+        _check_points("query", query, dim=self.dim)
+        if out is not None:
+            _check_out("out", out, (query.shape[1],), torch.int32, query.device)
+        _check_device(query)
         if out is None:
             out = torch.empty(query.shape[1], dtype=torch.int32, device=query.device)
         ok = load().knnhip_router_top1(self._h, query.data_ptr(), query.shape[1], query.stride(0),
@@ -663,10 +719,19 @@ def _problems(items):
 
     arr = (Problem * len(items))()
     outs = []
+    for check_device in (False, True):
+        for j, (ref, query, k) in enumerate(items):
+            try:
+                if check_device:
+                    _check_device(ref, query)
+                    continue
+                _check_points("ref", ref)
+                _check_points("query", query, dim=ref.shape[0])
+                if not 1 <= k <= ref.shape[1]:
+                    raise KnnError(f"k must satisfy 1 <= k <= ref_nb (k={k} ref_nb={ref.shape[1]})")
+            except KnnError as e:
+                raise KnnError(f"problem {j}: {e}") from None
     for j, (ref, query, k) in enumerate(items):
-        if ref.dtype != torch.float32 or query.dtype != torch.float32 or not ref.is_cuda or \
-                not query.is_cuda or ref.stride(1) != 1 or query.stride(1) != 1:
-            raise KnnError(f"problem {j}: float32 device tensors with contiguous rows expected")
         d = torch.empty((k, query.shape[1]), dtype=torch.float32, device=query.device)
         i = torch.empty((k, query.shape[1]), dtype=torch.int32, device=query.device)
         arr[j] = Problem(ref.data_ptr(), ref.shape[1], ref.stride(0), query.data_ptr(),

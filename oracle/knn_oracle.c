@@ -46,11 +46,10 @@ float oracle_canonical_sq(const float *ref, int ref_nb, const float *query, int 
     return acc;
 }
 
-/* Ground-truth key of the reference: sqrtf of the sum above (code/test.cpp:66). */
-static float key_of(const float *ref, int ref_nb, const float *query, int query_nb, int dim,
-                    int j, int i) {
-    return sqrtf(oracle_canonical_sq(ref, ref_nb, query, query_nb, dim, j, i));
-}
+/* The ground-truth key of the reference is sqrtf of the sum above
+ * (code/test.cpp:66). */
+static void topk_insert_block(const float *keys, int n, int j0, int k, int *filled_io,
+                              float *best_key, int *best_idx);
 
 /*
  * Top-k selection with the semantics of the reference's modified insertion
@@ -61,12 +60,45 @@ static float key_of(const float *ref, int ref_nb, const float *query, int query_
  * result is therefore the k smallest (key, index) pairs in lexicographic
  * order.  Written as a bounded sorted list rather than in place over the full
  * distance array; the output is the same.
+ *
+ * The keys are formed kBlock references at a time: for each d, the partial
+ * sums of the block's references take their d-th term.  Every reference's sum
+ * still adds its terms one by one in ascending d with the same FP32
+ * roundings as oracle_canonical_sq (so each key is bit-identical to sqrtf of
+ * it), but the dim-major rows are read contiguously instead of one element
+ * per row.
  */
+enum { kBlock = 256 };
 static void topk_one_query(const float *ref, int ref_nb, const float *query, int query_nb,
                            int dim, int k, int qi, float *best_key, int *best_idx) {
     int filled = 0;
-    for (int j = 0; j < ref_nb; ++j) {
-        float key = key_of(ref, ref_nb, query, query_nb, dim, j, qi);
+    float acc[kBlock];
+    for (int j0 = 0; j0 < ref_nb; j0 += kBlock) {
+        const int nb = ref_nb - j0 < kBlock ? ref_nb - j0 : kBlock;
+        for (int jj = 0; jj < nb; ++jj)
+            acc[jj] = 0.0f;
+        for (int d = 0; d < dim; ++d) {
+            const float qd = query[(size_t)d * (size_t)query_nb + (size_t)qi];
+            const float *row = ref + (size_t)d * (size_t)ref_nb + (size_t)j0;
+            for (int jj = 0; jj < nb; ++jj) {
+                float t = row[jj] - qd;
+                float sq = t * t;
+                acc[jj] = acc[jj] + sq;
+            }
+        }
+        for (int jj = 0; jj < nb; ++jj)
+            acc[jj] = sqrtf(acc[jj]);
+        topk_insert_block(acc, nb, j0, k, &filled, best_key, best_idx);
+    }
+}
+
+/* Insertion of one block of keys (references j0 .. j0 + n - 1, in order). */
+static void topk_insert_block(const float *keys, int n, int j0, int k, int *filled_io,
+                              float *best_key, int *best_idx) {
+    int filled = *filled_io;
+    for (int jj = 0; jj < n; ++jj) {
+        const int j = j0 + jj;
+        float key = keys[jj];
         if (filled == k && !(key < best_key[k - 1]))
             continue;
         int pos = (filled < k) ? filled : k - 1;
@@ -80,6 +112,7 @@ static void topk_one_query(const float *ref, int ref_nb, const float *query, int
         if (filled < k)
             ++filled;
     }
+    *filled_io = filled;
 }
 
 /* Output layout of the reference: rank j of query i at [j*query_nb + i]

@@ -1,0 +1,205 @@
+"""GPU tests of the boundary's behaviour beyond the golden cases.
+
+* The reference's own test program (code/test.cpp, its main unchanged), built against this
+  library under oracle/_ref, runs its full benchmark: ground truth by its knn_c, then its three
+  GPU entry points through their C++ symbols, each held to its own check (code/test.cpp:255-284).
+* Non-finite input: the host entry points fail the reference's way ("ERROR:" line, false);
+  the device entry point writes index -1 / distance NaN; the next search is unaffected.
+* Calls on different streams are ordered on the device (they share the library's buffers).
+* A saved index loaded over a different reference set of the same shape stays exact, and a
+  file whose layout is inconsistent is refused.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import knnhip
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libknn_ref.so")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LIB), reason="reference test program not built")
+@pytest.mark.timeout(600)
+def test_reference_test_program_passes_against_library(capfd):
+    # The reference's main (code/test.cpp:292-356), compiled unchanged under oracle/_ref with only
+    # its name changed, runs in this process: its ground truth by its own knn_c, then its three
+    # GPU entry points through their C++ symbols, which the loader resolves to this library.
+    # (In this process rather than as the oracle/_ref/test_ref child: a GPU process starts no
+    # other program here.)
+    say("running the reference's test program (16384 x 4096, dim 128, k 16)")
+    lib = ctypes.CDLL(REF_LIB)
+    main = lib._Z23knn_reference_test_mainv
+    main.restype = ctypes.c_int
+    main.argtypes = []
+    rc = main()
+    ctypes.CDLL(None).fflush(None)
+    stdout = capfd.readouterr().out
+    with capfd.disabled():
+        say(stdout[-2000:])
+    assert rc == 0, stdout[-2000:]
+    lines = [l for l in stdout.splitlines() if l.startswith("- ")]
+    names = ["knn_c", "knn_cuda_global", "knn_cuda_texture", "knn_cublas"]
+    for name in names:
+        row = [l for l in lines if l.split(":")[0].strip() == f"- {name}"]
+        assert row and "PASSED" in row[0], f"{name}: {row}"
+    assert stdout.count("PASSED") == 4
+
+
+@pytest.mark.parametrize("entry", ["knn_cuda_global", "knn_cuda_texture", "knn_cublas", "kth_distance"])
+@pytest.mark.parametrize("case", ["nan_ref", "inf_query", "overflow"])
+def test_host_entry_rejects_non_finite_input(entry, case, capfd):
+    rng = np.random.default_rng(7)
+    ref = rng.random((16, 500), dtype=np.float32)
+    qry = rng.random((16, 40), dtype=np.float32)
+    if case == "nan_ref":
+        ref[3, 77] = np.nan
+    elif case == "inf_query":
+        qry[15, 39] = -np.inf
+    else:
+        ref[0, 5] = 3e19  # finite, but its square overflows FP32
+    fn = knnhip.kth_distance if entry == "kth_distance" else getattr(knnhip, entry)
+    with pytest.raises(knnhip.KnnError, match="non-finite"):
+        fn(ref, qry, 4)
+    assert "ERROR: non-finite input" in capfd.readouterr().out
+    # the next call on finite data is unaffected
+    ref[np.isnan(ref) | (ref > 1)] = 0.5
+    qry[~np.isfinite(qry)] = 0.5
+    gd, gi = oracle.knn_c(ref, qry, 4)
+    if entry == "kth_distance":
+        assert np.array_equal(bits(fn(ref, qry, 4)), bits(gd[3]))
+    else:
+        d, i = fn(ref, qry, 4)
+        assert np.array_equal(i, gi) and np.array_equal(bits(d), bits(gd))
+
+
+@pytest.mark.parametrize("k", [4, 300])
+def test_device_entry_marks_non_finite_input(k):
+    # k = 300 has no room in the scan's heaps: every query takes the exact scan
+    ref = knnhip.generate_uniform(3000, 24, seed=5)
+    qry = knnhip.generate_uniform(50, 24, seed=6)
+    ref[7, 1234] = float("nan")
+    d, i = knnhip.search_device(ref, qry, k)
+    torch.cuda.synchronize()
+    assert knnhip.last_fallback_count() == -1
+    assert "non-finite" in knnhip.last_error()
+    assert torch.all(i == -1) and torch.all(torch.isnan(d))
+    ref[7, 1234] = 0.25
+    d, i = knnhip.search_device(ref, qry, k)
+    torch.cuda.synchronize()
+    assert knnhip.last_fallback_count() >= 0
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    assert np.array_equal(i.cpu().numpy(), gi) and np.array_equal(bits(d.cpu().numpy()), bits(gd))
+
+
+def test_host_entry_streamed_rejects_non_finite_input():
+    rng = np.random.default_rng(8)
+    ref = rng.random((24, 30000), dtype=np.float32)
+    qry = rng.random((24, 500), dtype=np.float32)
+    ref[11, 29999] = np.inf  # in the last reference chunk
+    budget = None
+    for mib in range(8, 1024):
+        try:
+            rc, _ = knnhip.stream_chunks(30000, 500, 24, 8, mib << 20)
+        except knnhip.KnnError:
+            continue
+        if -(-30000 // rc) >= 3:
+            budget = mib << 20
+            break
+    assert budget, "no budget streams this search in three or more chunks"
+    knnhip.set_device_budget(budget)
+    try:
+        with pytest.raises(knnhip.KnnError, match="non-finite"):
+            knnhip.knn_cuda_global(ref, qry, 8)
+        assert knnhip.last_plan().get("streamed")
+    finally:
+        knnhip.set_device_budget(0)
+
+
+def test_searches_on_two_streams_are_ordered():
+    # back-to-back searches of different data on two streams: the second call's kernels use the
+    # library's buffers only after the first call's are done
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    shapes = [(200000, 4000, 64, 16), (150000, 3000, 64, 16)]
+    data = []
+    for n, (R, Q, D, k) in enumerate(shapes):
+        data.append((knnhip.generate_uniform(R, D, seed=40 + n), knnhip.generate_uniform(Q, D, seed=50 + n), k))
+    torch.cuda.synchronize()
+    outs = []
+    for rep in range(3):
+        for (ref, qry, k), st in zip(data, (s1, s2)):
+            with torch.cuda.stream(st):
+                outs.append(knnhip.search_device(ref, qry, k, margin=256, stream=st.cuda_stream))
+    torch.cuda.synchronize()
+    for n, (ref, qry, k) in enumerate(data):
+        sample = [0, 1, qry.shape[1] - 1]
+        gd, gi = oracle.knn_c(ref.cpu().numpy(), qry[:, sample].cpu().numpy(), k)
+        for rep in range(3):
+            d, i = outs[2 * rep + n]
+            assert np.array_equal(i[:, sample].cpu().numpy(), gi), (n, rep)
+            assert np.array_equal(bits(d[:, sample].cpu().numpy()), bits(gd)), (n, rep)
+
+
+def test_index_loaded_over_other_data_stays_exact(tmp_path):
+    R, Q, D, k = 60000, 300, 32, 10
+    ref = knnhip.generate_mixture(R, D, 64, 0.1, 91, 92)
+    qry = knnhip.generate_mixture(Q, D, 64, 0.1, 91, 93)
+    ix = knnhip.BlockIndex(ref, nblocks=64)
+    path = str(tmp_path / "ix.bin")
+    ix.save(path)
+    ix.close()
+    # the same shape, other points: the file's radii do not fit them; loading refits the radii
+    other = knnhip.generate_mixture(R, D, 64, 0.1, 94, 95)
+    ix2 = knnhip.BlockIndex.load(path, other)
+    gd, gi = oracle.knn_c(other.cpu().numpy(), qry.cpu().numpy(), k)
+    for probes in (1, 4):
+        d, i = ix2.search(qry, k, probes=probes)
+        torch.cuda.synchronize()
+        assert np.array_equal(i.cpu().numpy(), gi), probes
+        assert np.array_equal(bits(d.cpu().numpy()), bits(gd)), probes
+    ix2.close()
+
+
+@pytest.mark.parametrize("corruption", ["duplicate_ref", "out_of_range", "ball_block"])
+def test_inconsistent_index_file_is_refused(tmp_path, corruption):
+    R, D = 5000, 16
+    ref = knnhip.generate_uniform(R, D, seed=3)
+    ix = knnhip.BlockIndex(ref, nblocks=8)
+    path = str(tmp_path / "ix.bin")
+    ix.save(path)
+    ix.close()
+    raw = bytearray(open(path, "rb").read())
+    hdr = np.frombuffer(bytes(raw[12:36]), dtype=np.int32)
+    nb, nf, r_pad = int(hdr[2]), int(hdr[3]), int(hdr[5])
+    perm_off = 36 + 4 * (nb + 1) + 4 * nb
+    perm = np.frombuffer(bytes(raw[perm_off:perm_off + 4 * r_pad]), dtype=np.int32).copy()
+    if corruption == "duplicate_ref":
+        perm[1] = perm[0]
+    elif corruption == "out_of_range":
+        perm[2] = R + 5
+    if corruption in ("duplicate_ref", "out_of_range"):
+        raw[perm_off:perm_off + 4 * r_pad] = perm.tobytes()
+    else:
+        pm_ld = (D + 3) & ~3
+        fbl_off = perm_off + 4 * r_pad + 4 * nf * pm_ld + 4 * nf + 8 * nf
+        fbl = np.frombuffer(bytes(raw[fbl_off:fbl_off + 4 * nf]), dtype=np.int32).copy()
+        fbl[0] = (fbl[0] + 1) % nb
+        raw[fbl_off:fbl_off + 4 * nf] = fbl.tobytes()
+    open(path, "wb").write(bytes(raw))
+    with pytest.raises(knnhip.KnnError, match="inconsistent index"):
+        knnhip.BlockIndex.load(path, ref)
