@@ -69,7 +69,9 @@ typedef struct knnhip_options {
     int force_fallback; /* 1: every query takes the exact full-scan path (testing) */
     int timing;         /* 1: record per-stage device times (knnhip_last_stage_ms) */
     int precision;      /* 0: FP32 products; 1: bfloat16 three-term products (same exact
-                           result; dim <= 192 and k + margin <= 107, else FP32) */
+                           result; dim <= 192 and k + margin <= 107, else FP32); 2: index
+                           searches take single-pass bfloat16 products on block-centroid
+                           residuals (dim <= 256, else as 1), plain searches as 1 */
     int block_queries;  /* FP32 scan shape: 0 chosen, 32 (reference tiles per wave), 128
                            (reference tiles shared through LDS; dim <= 192, k + margin <= 107) */
     int score_all;      /* index searches given scores: 0 the scores choose each query's
@@ -167,6 +169,14 @@ int knnhip_index_load(const char *path, const float *ref, int ref_nb, long long 
  * norms, block bounds, block choice, task build, scan, select/re-rank,
  * full-scan fallback (n <= 7 values written). */
 int knnhip_index_stage_ms(const knnhip_index *index, float *ms, int n);
+/* Scan work of the last index search (read back from its task list, so call
+ * it before the next index search): the (reference, query) pairs its MFMA
+ * tiles executed (whole 128-reference steps times 32 query columns per task),
+ * the pairs its tasks held, the bytes of reference tiles its tasks read, and
+ * the products it used (0 FP32, 1 bfloat16 three-term, 2 bfloat16 on block
+ * residuals).  Any pointer may be NULL.  For benchmarks (roofline figures). */
+int knnhip_index_last_work(const knnhip_index *index, long long *exec_pairs, long long *pairs,
+                           long long *tile_bytes, int *precision);
 /* Block layout of the index: block_start / block_count (host, nblocks ints
  * each: first position and size of each block in the block-contiguous copy)
  * and ref_block (device, ref_nb ints: the block holding each reference point,

@@ -67,6 +67,21 @@ struct ScanArgs {
     // columns of the dim-major matrix.
     const float *query_pm;
     int query_pm_ld;
+    // Residual form (block-index tasks): bfloat16 residual tiles r - c of
+    // the block-ordered positions, their FP32 squared norms, the block
+    // centroids (rows of query_pm_ld floats) and radii, the block of each
+    // 32-position tile, and per heap row (task_row + column) four floats:
+    // the row's distance bound from the triangle inequality, eps and eta
+    // (see knn_scan_resid.hip).
+    const unsigned short *packed_res;
+    const float *res_norm;
+    const float *res_cent;
+    const float *res_rho;
+    const int *tile_block;
+    float *res_row;
+    float res_eta_coef;  // key error per (||e_q|| + ||e_r||)^2
+    float res_norm_up;   // exact norm over its FP32 sum, at most
+    float res_norm_dn;   // ... at least
 };
 
 // Arguments of the norms / layout pass.
@@ -134,6 +149,11 @@ struct FinalArgs {
     // Non-zero when an input point has a NaN or infinite squared norm: every
     // output of the search is then index -1, distance NaN (nullptr: unchecked).
     const unsigned *nonfinite;
+    // Residual-form scan (non-null): keys are approximate squared distances;
+    // per heap row four floats (L, eps, eta, -): every reference of the
+    // row's block is at distance >= L, and every one the scan rejected is
+    // also at distance >= sqrt(tau - eta) - eps.
+    const float *res_row;
 };
 
 // Arguments of the exact per-query full scan (fallback).

@@ -62,6 +62,18 @@ hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t ld
                                 int tasks_per_group = 1);
 hipError_t launch_pack_bf3_rows(const float *pm, int ld, int n_pad, int nch, unsigned short *out,
                                 hipStream_t s);
+// Residual form of the block-index scan (knn_scan_resid.hip): chunks for a
+// dimension (0: too wide), its key error coefficients, the per-block
+// centroids / residual tiles / radii of an index, and the task scan.
+int scan_res_nch(int dim);
+float res_eta_coeff(int dim, int nch);
+float res_norm_up(int dim);
+float res_norm_dn(int dim);
+hipError_t launch_resid_build(const float *pm, int ld, int dim, int nch, const int *block_start,
+                              const int *block_count, int nb, float *cent, unsigned short *tiles, float *rnorm,
+                              float *rho, int *tile_block, hipStream_t s);
+hipError_t launch_scan_res_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
+                                int tasks_per_group = 1);
 hipError_t launch_norms(const NormArgs &a, hipStream_t s);
 hipError_t launch_norms_jobs(const NormArgs &a, const NormJob *jobs, const int2 *wg, int nwg,
                              hipStream_t s);
@@ -322,10 +334,19 @@ struct knnhip_index {
     // by the first precision search; fcent_nch chunks
     int fcent_nch = 0;
     knnhip::DevBuf fcent_bf;
+    // residual form: block centroids (rows of pm_ld floats), bfloat16
+    // residual tiles over res_nch chunks, their squared norms, block radii
+    // and the block of each tile; built by the first search that asks
+    int res_nch = 0;
+    knnhip::DevBuf res_cent, res_tiles, res_norm, res_rho, tile_block;
     // last search statistics
     int last_probes = 0;
     int last_fail = 0;
     long long last_tasks = 0;
+    // the last search's tile bytes per reference and products (0 FP32, 1
+    // three-term bfloat16, 2 residual bfloat16), for knnhip_index_last_work
+    long long last_ref_bytes = 0;
+    int last_prec = 0;
     float last_ms[6] = {-1, -1, -1, -1, -1, -1};
     float rank_ms[3] = {-1, -1, -1};  // query norms, block bounds, block choice
 };

@@ -185,13 +185,13 @@ Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o) {
     p.xcd = (s % 8 == 0) ? 1 : 0;
     p.maxc = std::min(kMaxCand, std::max(2 * p.kp, 256));
     p.use_scan = true;
-    if (o.precision == 1 || o.block_queries == 128) {
+    if (o.precision >= 1 || o.block_queries == 128) {
         // LDS-shared scan: 128 queries per block, one block per CU; slices
         // fill whole rounds of CUs (the same cost model, resident = CUs).
         const int nch = scan_bf3_nch(dim);
         if (nch && scan_bf3_lds_bytes(p.kp) <= kScanLdsMax) {
             p.lds = true;
-            p.bf3 = o.precision == 1;
+            p.bf3 = o.precision >= 1;
             p.nch = nch;
             p.waves = 8;
             p.dpad = 16 * nch;

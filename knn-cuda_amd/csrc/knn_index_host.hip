@@ -18,6 +18,9 @@ constexpr int kIndexMargin = 8;  // block-index searches: fewer heap insertions 
 // ... with bfloat16 products, whose wider error bound needs a wider gap
 // between the k-th and the kp-th candidate to certify
 constexpr int kIndexMarginBf3 = 12;
+// ... and with single-pass bfloat16 products on block residuals, whose bound
+// grows with the block's radius
+constexpr int kIndexMarginRes = 24;
 // Waves per block-scan task.  Heap insertions of one query run one after
 // another, so where the chip holds many tasks (kOneWaveQueries or more
 // queries per block on average) one wave per task (up to eight tasks per CU,
@@ -436,7 +439,7 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
 // Workspace of index searches (grown on demand, kept between calls).
 struct IndexWork {
     DevBuf qnorm, qpm, lb, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks,
-        pval, pidx, theta, fb_count, fb_list, qsub, dsub, isub, scores;
+        pval, pidx, theta, fb_count, fb_list, qsub, dsub, isub, scores, resrow;
     bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
 IndexWork g_iw, g_iw_retry;
@@ -499,11 +502,15 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                   long long ldq, int k, int P, const float *router_scores, float *out_dist,
                   int *out_idx, const knnhip_options &o, hipStream_t s, knnhip_router *router = nullptr) {
     const int nb = ix->nb, dim = ix->dim, pm_ld = ix->pm_ld;
-    // precision scan: bfloat16 three-term products for dimensions up to 128
-    // (else FP32 products)
-    const int nch_bf = o.precision == 1 ? scan_bf3_task_nch(dim) : 0;
+    // precision 1: bfloat16 three-term products for dimensions up to 128;
+    // precision 2: single-pass bfloat16 products on block residuals for
+    // dimensions up to 256, else the three-term products where they apply;
+    // otherwise FP32 products
+    const int nch_res = o.precision == 2 ? scan_res_nch(dim) : 0;
+    const int nch_bf = (o.precision == 1 || (o.precision == 2 && !nch_res)) ? scan_bf3_task_nch(dim) : 0;
     const int margin = o.margin > 0     ? o.margin
                        : k >= kLargeK   ? kDefaultMarginLargeK
+                       : nch_res        ? kIndexMarginRes
                        : nch_bf         ? kIndexMarginBf3
                                         : kIndexMargin;
     const int kp = std::max(k, std::min(k + margin, ix->R));
@@ -531,6 +538,25 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                 "bfloat16 tiles");
         ix->nch_bf = nch_bf;
     }
+    if (nch_res && ix->res_nch != nch_res) {
+        ix->res_nch = 0;
+        const size_t ntile = (size_t)ix->r_pad / kRefsPerTile;
+        if (!ix->res_cent.alloc((size_t)nb * pm_ld * 4) || !ix->res_tiles.alloc(ntile * nch_res * kWave * 16) ||
+            !ix->res_norm.alloc((size_t)ix->r_pad * 4) || !ix->res_rho.alloc((size_t)nb * 4) ||
+            !ix->tile_block.alloc(ntile * 4)) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        KNN_TRY(hipMemsetAsync(ix->res_tiles.p, 0, ix->res_tiles.n, s), "memset");
+        KNN_TRY(hipMemsetAsync(ix->res_norm.p, 0, ix->res_norm.n, s), "memset");
+        KNN_TRY(hipMemsetAsync(ix->tile_block.p, 0, ix->tile_block.n, s), "memset");
+        KNN_TRY(launch_resid_build(ix->pm.as<float>(), pm_ld, dim, nch_res, ix->block_start.as<int>(),
+                                   ix->block_count.as<int>(), nb, ix->res_cent.as<float>(),
+                                   ix->res_tiles.as<unsigned short>(), ix->res_norm.as<float>(),
+                                   ix->res_rho.as<float>(), ix->tile_block.as<int>(), s),
+                "residual tiles");
+        ix->res_nch = nch_res;
+    }
     // Two scan phases: each query's first-ranked block, then its other
     // probes, which start from the thresholds the first phase left.
     const int nphase = P > 1 ? 2 : 1;
@@ -549,7 +575,8 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         !w.need(w.rows, (size_t)npairs * 4) || !w.need(w.tasks, (size_t)max_tasks * sizeof(ScanTask)) ||
         !w.need(w.pval, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
         !w.need(w.pidx, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
-        !w.need(w.theta, (size_t)Q * 4) || !w.need(w.fb_count, 4) || !w.need(w.fb_list, (size_t)Q * 4)) {
+        !w.need(w.theta, (size_t)Q * 4) || !w.need(w.fb_count, 4) || !w.need(w.fb_list, (size_t)Q * 4) ||
+        (nch_res && !w.need(w.resrow, (size_t)max_tasks * kQueriesPerWave * 16))) {
         set_error("Memory allocation error");
         return false;
     }
@@ -575,7 +602,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     if (timing) KNN_TRY(hipEventRecord(evr[0], s), "event");
     // block lower bounds (smallest over each block's balls), block-major;
     // without a router the ranking is by the negated bound (nearest first)
-    if (!index_bounds(ix, w.qpm.as<float>(), w.qnorm.as<float>(), Q, o.precision == 1, w.lb.as<float>(), s))
+    if (!index_bounds(ix, w.qpm.as<float>(), w.qnorm.as<float>(), Q, o.precision >= 1, w.lb.as<float>(), s))
         return false;
     // scores pick the first probe (or rank every probe, with score_all); a
     // router handle picks it with the fused inference kernel
@@ -647,7 +674,20 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         sa.query_pm = w.qpm.as<float>();
         sa.query_pm_ld = pm_ld;
         const int waves = (double)Q * (pr[ph + 1] - pr[ph]) / nb >= kOneWaveQueries ? 1 : 4;
-        if (nch_bf) {
+        if (nch_res) {
+            sa.packed_res = ix->res_tiles.as<unsigned short>();
+            sa.res_norm = ix->res_norm.as<float>();
+            sa.res_cent = ix->res_cent.as<float>();
+            sa.res_rho = ix->res_rho.as<float>();
+            sa.tile_block = ix->tile_block.as<int>();
+            sa.res_row = w.resrow.as<float>() + (size_t)base * 4;
+            sa.res_eta_coef = res_eta_coeff(dim, nch_res);
+            sa.res_norm_up = res_norm_up(dim);
+            sa.res_norm_dn = res_norm_dn(dim);
+            KNN_TRY(launch_scan_res_task(sa, nch_res, waves, scan_lds_bytes(kp, waves), s,
+                                         waves == 1 ? kTasksPerGroup : 1),
+                    "scan kernel launch");
+        } else if (nch_bf) {
             sa.packed_bf = ix->packed_bf.as<unsigned short>();
             // one-wave tasks go kTasksPerGroup to a workgroup: a block's
             // consecutive query groups then read its tiles through one CU
@@ -685,6 +725,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     fa.theta = w.theta.as<unsigned>();
     fa.lb_rest = w.lb_rest.as<float>();
     if (nch_bf) bf3_error_bound(dim, nch_bf, &fa.err_q, &fa.err_r);
+    if (nch_res) fa.res_row = w.resrow.as<float>();
     fa.force_fallback = o.force_fallback;
     fa.out_dist = out_dist;
     fa.out_idx = out_idx;
@@ -750,6 +791,11 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         g_stage_ms[2] = ix->last_ms[3];
         g_stage_ms[3] = ix->last_ms[4];
     }
+    // what knnhip_index_last_work reads back from this search's task list
+    if (depth == 0) {
+        ix->last_ref_bytes = nch_res ? 32ll * nch_res : nch_bf ? 64ll * nch_bf : 4ll * ix->dpad;
+        ix->last_prec = nch_res ? 2 : nch_bf ? 1 : 0;
+    }
     g_last_fb = w.fb_count.as<int>();
     g_last_nonfinite = nullptr;
     return true;
@@ -801,12 +847,14 @@ int knnhip_index_search(knnhip_index *ix, const float *query, int query_nb, long
     CallOrder order((hipStream_t)stream);
     knnhip_options o = opt ? *opt : default_options(0);
     char buf[256];
-    const bool bf3 = o.precision == 1 && scan_bf3_task_nch(ix->dim) != 0;
+    const char *prec = o.precision == 2 && scan_res_nch(ix->dim)           ? "bf16-residual"
+                       : o.precision >= 1 && scan_bf3_task_nch(ix->dim) ? "bf16x3"
+                                                                        : "fp32";
     snprintf(buf, sizeof(buf),
              "{\"index_blocks\": %d, \"probes\": %d, \"router\": %s, \"scores_rank\": \"%s\", "
              "\"query_nb\": %d, \"k\": %d, \"precision\": \"%s\"}",
              ix->nb, std::min(probes, ix->nb), scores ? "true" : "false",
-             !scores ? "none" : o.score_all ? "all" : "first", query_nb, k, bf3 ? "bf16x3" : "fp32");
+             !scores ? "none" : o.score_all ? "all" : "first", query_nb, k, prec);
     g_last_plan = buf;
     return index_search(ix, g_iw, 0, query, query_nb, query_ld, k, probes, scores, knn_dist,
                         knn_index, o, (hipStream_t)stream)
@@ -837,12 +885,14 @@ int knnhip_index_search_routed(knnhip_index *ix, knnhip_router *router, const fl
     CallOrder order((hipStream_t)stream);
     knnhip_options o = opt ? *opt : default_options(0);
     char buf[256];
-    const bool bf3 = o.precision == 1 && scan_bf3_task_nch(ix->dim) != 0;
+    const char *prec = o.precision == 2 && scan_res_nch(ix->dim)           ? "bf16-residual"
+                       : o.precision >= 1 && scan_bf3_task_nch(ix->dim) ? "bf16x3"
+                                                                        : "fp32";
     snprintf(buf, sizeof(buf),
              "{\"index_blocks\": %d, \"probes\": %d, \"router\": true, \"router_kernel\": \"%s\", "
              "\"scores_rank\": \"%s\", \"query_nb\": %d, \"k\": %d, \"precision\": \"%s\"}",
              ix->nb, std::min(probes, ix->nb), o.score_all ? "scores" : "fused first probe",
-             o.score_all ? "all" : "first", query_nb, k, bf3 ? "bf16x3" : "fp32");
+             o.score_all ? "all" : "first", query_nb, k, prec);
     g_last_plan = buf;
     return index_search(ix, g_iw, 0, query, query_nb, query_ld, k, probes, nullptr, knn_dist, knn_index,
                         o, (hipStream_t)stream, router)
@@ -923,6 +973,35 @@ int knnhip_index_stage_ms(const knnhip_index *ix, float *ms, int n) {
     const float all[7] = {ix->rank_ms[0], ix->rank_ms[1], ix->rank_ms[2], ix->last_ms[1],
                           ix->last_ms[2], ix->last_ms[3], ix->last_ms[4]};
     for (int i = 0; i < n && i < 7; ++i) ms[i] = all[i];
+    return 1;
+}
+
+int knnhip_index_last_work(const knnhip_index *ix, long long *exec_pairs, long long *pairs,
+                           long long *tile_bytes, int *precision) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!ix) {
+        set_error("null index");
+        return 0;
+    }
+    // the task list of the last search is still in the search workspace
+    long long ex = 0, pr = 0, tb = 0;
+    const long long nt = std::min<long long>(ix->last_tasks, (long long)(g_iw.tasks.n / sizeof(ScanTask)));
+    if (nt > 0) {
+        std::vector<ScanTask> tl(nt);
+        KNN_TRY(hipMemcpy(tl.data(), g_iw.tasks.p, (size_t)nt * sizeof(ScanTask), hipMemcpyDeviceToHost), "copy");
+        for (const ScanTask &t : tl) {
+            if (t.qcount <= 0) continue;
+            const long long steps = (t.ref_end - t.ref_begin + kRefsPerStep - 1) / kRefsPerStep;
+            ex += steps * kRefsPerStep * kQueriesPerWave;
+            pr += (long long)(t.ref_end - t.ref_begin) * t.qcount;
+            tb += steps * kRefsPerStep * ix->last_ref_bytes;
+        }
+    }
+    if (exec_pairs) *exec_pairs = ex;
+    if (pairs) *pairs = pr;
+    if (tile_bytes) *tile_bytes = tb;
+    if (precision) *precision = ix->last_prec;
     return 1;
 }
 

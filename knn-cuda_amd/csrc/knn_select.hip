@@ -153,11 +153,31 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
         if (ok && tau_f < INFINITY) {
             const double tau = (double)tau_f;
             const double g = 2.0 * (double)(a.dim + 4) * 5.9604644775390625e-8;
-            const double nq = (double)a.qnorm[q];
-            const double nr = (double)__uint_as_float(*a.rmax);
-            const double eq = a.err_q > 0.0 ? a.err_q : g, er = a.err_r > 0.0 ? a.err_r : 3.1 * g;
-            const double lower =
-                (tau + nq * (1.0 - g) - (eq * nq + er * nr) - (double)a.dim * 1e-36) * (1.0 - g);
+            double lower;
+            if (a.res_row) {
+                // residual keys: a rejected reference r of the block of row j
+                // has approximate squared distance >= tau, so
+                // ||q - r|| >= max(L_j, sqrt(tau - eta_j) - eps_j); the
+                // smallest such bound over the query's rows bounds every
+                // rejected reference, and its ground-truth FP32 sum is within
+                // the relative factor g of ||q - r||^2
+                double bmin = INFINITY;
+                for (int j = 0; j < a.nrows; ++j) {
+                    const long long row = a.rows ? (long long)a.rows[(long long)q * a.nrows + j]
+                                                 : j * a.row_stride + q;
+                    if (row < 0) continue;
+                    const float4 e = *reinterpret_cast<const float4 *>(a.res_row + 4 * row);
+                    const double t2 = tau - (double)e.z;
+                    const double dl = (t2 > 0.0 ? sqrt(t2) * (1.0 - 1e-15) : 0.0) - (double)e.y;
+                    bmin = fmin(bmin, fmax((double)e.x, dl));
+                }
+                lower = bmin > 0.0 ? bmin * bmin * (1.0 - g) : 0.0;
+            } else {
+                const double nq = (double)a.qnorm[q];
+                const double nr = (double)__uint_as_float(*a.rmax);
+                const double eq = a.err_q > 0.0 ? a.err_q : g, er = a.err_r > 0.0 ? a.err_r : 3.1 * g;
+                lower = (tau + nq * (1.0 - g) - (eq * nq + er * nr) - (double)a.dim * 1e-36) * (1.0 - g);
+            }
             const double kth = (double)(*kth_slot) * (1.0 + 2.384185791015625e-7);
             ok = lower > kth * kth;
         }

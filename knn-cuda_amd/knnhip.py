@@ -70,9 +70,14 @@ C_SYMBOLS = (
     "knnhip_device_budget",
     "knnhip_stream_chunks",
     "knnhip_version",
+    "knnhip_index_last_work",
 )
 
 VARIANTS = {"global": 0, "texture": 1, "cublas": 2}
+# knnhip_options.precision: FP32 products; bfloat16 three-term products; single-pass bfloat16
+# products on block-centroid residuals (block-index searches; plain searches take the three-term
+# form).  Every form returns the same exact result.
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16": 2}
 
 
 class KnnError(RuntimeError):
@@ -154,6 +159,9 @@ def load() -> ctypes.CDLL:
     lib.knnhip_index_info.restype = ctypes.c_int
     lib.knnhip_index_stage_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     lib.knnhip_index_stage_ms.restype = ctypes.c_int
+    lib.knnhip_index_last_work.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 3 + [
+        ctypes.POINTER(ctypes.c_int)]
+    lib.knnhip_index_last_work.restype = ctypes.c_int
     lib.knnhip_index_layout.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]
     lib.knnhip_index_layout.restype = ctypes.c_int
@@ -339,9 +347,9 @@ def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "
     opt.slices = slices
     opt.force_fallback = 1 if force_fallback else 0
     opt.timing = 1 if timing else 0
-    if precision not in ("fp32", "bf16x3"):
-        raise KnnError("precision must be 'fp32' or 'bf16x3'")
-    opt.precision = 1 if precision == "bf16x3" else 0
+    if precision not in PRECISIONS:
+        raise KnnError(f"precision must be one of {sorted(PRECISIONS)}")
+    opt.precision = PRECISIONS[precision]
     opt.block_queries = block_queries
     if stream is None:
         stream = torch.cuda.current_stream(query.device).cuda_stream
@@ -427,6 +435,16 @@ class BlockIndex:
         return {"nblocks": nb.value, "positions": npos.value, "build_ms": ms.value}
 
     STAGES = ("norms", "bounds", "pick", "tasks", "scan", "select", "fallback")
+
+    def last_work(self) -> dict:
+        """Scan work of the last search made with timing=True (executed and held (reference,
+        query) pairs, tile bytes read, products used)."""
+        ex, pr, tb = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        prec = ctypes.c_int()
+        load().knnhip_index_last_work(self._h, ctypes.byref(ex), ctypes.byref(pr), ctypes.byref(tb),
+                                      ctypes.byref(prec))
+        return {"exec_pairs": ex.value, "pairs": pr.value, "tile_bytes": tb.value,
+                "precision": {0: "fp32", 1: "bf16x3", 2: "bf16-residual"}.get(prec.value, "?")}
 
     def stage_ms(self) -> dict:
         buf = (ctypes.c_float * 7)()
@@ -517,13 +535,13 @@ class BlockIndex:
                 raise KnnError("scores must be float32 (nblocks, query_nb): one column per query")
             scores = scores.contiguous()
             sp = scores.data_ptr()
-        if precision not in ("fp32", "bf16x3"):
-            raise KnnError("precision must be 'fp32' or 'bf16x3'")
+        if precision not in PRECISIONS:
+            raise KnnError(f"precision must be one of {sorted(PRECISIONS)}")
         opt = Options()
         opt.margin = margin
         opt.force_fallback = 1 if force_fallback else 0
         opt.timing = 1 if timing else 0
-        opt.precision = 1 if precision == "bf16x3" else 0
+        opt.precision = PRECISIONS[precision]
         opt.score_all = 1 if score_all else 0
         if stream is None:
             stream = torch.cuda.current_stream(query.device).cuda_stream

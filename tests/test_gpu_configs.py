@@ -106,6 +106,21 @@ def test_precision_form_full_size_plain(dense_clusters):
     assert np.array_equal(i2.cpu().numpy(), i) and np.array_equal(bits(d2.cpu().numpy()), bits(d))
 
 
+def test_precision_form_full_size_block_residuals(dense_clusters):
+    # the precision form as the workload names it: bfloat16 products (single pass, on block-centroid
+    # residuals) and the k + 64 candidate list re-ranked in FP32, through a block index
+    ref, qry, k, d, i = dense_clusters
+    ix = knnhip.BlockIndex(ref, nblocks=1024, iters=8)
+    for probes in (1, 2):
+        d2, i2 = ix.search(qry, k, probes=probes, precision="bf16")
+        torch.cuda.synchronize()
+        plan = knnhip.last_plan()
+        assert plan["precision"] == "bf16-residual"
+        say(f"residual P={probes}: certificate failures {knnhip.last_fallback_count()}")
+        assert np.array_equal(i2.cpu().numpy(), i) and np.array_equal(bits(d2.cpu().numpy()), bits(d))
+    ix.close()
+
+
 def test_batched_form_all_512_problems():
     with open(os.path.join(GOLDEN, "c4_problems.json")) as f:
         P = json.load(f)["problems"]
@@ -142,7 +157,9 @@ def test_learned_router_form_full_size():
     mlp.train(ix, trq, k, epochs=4, batch=1024, lr=1e-3)
     for what, kw in (("index fp32 P=5", dict(probes=5)),
                      ("index bf16x3 P=5", dict(probes=5, precision="bf16x3")),
-                     ("index bf16x3 P=5 routed", dict(probes=5, precision="bf16x3", router=mlp))):
+                     ("index bf16x3 P=5 routed", dict(probes=5, precision="bf16x3", router=mlp)),
+                     ("index bf16 residual P=5", dict(probes=5, precision="bf16")),
+                     ("index bf16 residual P=5 routed", dict(probes=5, precision="bf16", router=mlp))):
         d2, i2 = ix.search(qry, k, **kw)
         torch.cuda.synchronize()
         say(f"{what}: certificate failures {knnhip.last_fallback_count()}")
