@@ -102,6 +102,26 @@ int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const f
                          int query_nb, long long query_ld, int dim, int k, float *knn_dist,
                          int *knn_index, const knnhip_options *opt, void *stream);
 
+/*
+ * knnhip_search_device for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, same exact
+ * result; the call makes no hipMalloc / hipFree and calls no allocating
+ * routine: every buffer besides the caller's inputs and outputs lives in one
+ * static scratch array of 64 MiB per device (held from the library's first
+ * use on that device until the process ends).  A search whose buffers do not
+ * fit (about 2 * 4 * ref_nb * dim bytes of reference layouts plus heaps and
+ * query rows) returns 0 with an error naming the size.  Queries that fail the
+ * candidate certificate go straight to the exact per-query scan (no
+ * wider-margin rescan, whose buffers would be allocated); options.timing is
+ * ignored.  Every call on a device shares that device's scratch: calls that
+ * overlap on one device race, so callers order them (the library orders its
+ * own calls on the device, as for knnhip_search_device).
+ */
+int knnhip_search_device_noalloc(const float *ref, int ref_nb, long long ref_ld, const float *query,
+                                 int query_nb, long long query_ld, int dim, int k, float *knn_dist,
+                                 int *knn_index, const knnhip_options *opt, void *stream);
+
 /* Number of queries of the last call that failed the candidate certificate
  * and were answered by the exact full scan.  Synchronises the last stream. */
 int knnhip_last_fallback_count(void);

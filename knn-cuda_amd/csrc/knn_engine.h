@@ -127,6 +127,19 @@ struct Workspace {
 extern Workspace g_ws;
 void *ws_get(Slot s, size_t bytes);
 
+// While an Arena is active, ws_get carves each slot's buffer out of a
+// caller-given region (256-byte aligned pieces) instead of the cached
+// hipMalloc buffers, and run_device skips its wider-margin rescans (whose
+// buffers it would allocate): queries that fail the certificate go straight
+// to the exact per-query scan.
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *slot[kNumSlots] = {};
+    size_t size[kNumSlots] = {};
+};
+extern Arena *g_arena;
+
 // Non-finite input of the plain search: one device word that the norms pass
 // sets when a reference or query point has a NaN or infinite squared norm;
 // the re-rank and the exact scan then write index -1 / distance NaN instead
@@ -173,6 +186,9 @@ struct Plan {
 inline int block_queries(const Plan &p) { return p.lds ? 128 : kQueriesPerWave; }
 
 Plan make_plan(int R, int Q, int dim, int k, const knnhip_options &o);
+// Bytes run_device takes from ws_get for this plan (each piece rounded up to
+// 256 bytes): what an Arena must hold.
+size_t run_device_bytes(const Plan &p, int R, int Q, int dim, int k);
 std::string plan_json(const Plan &p, int R, int Q, int dim, int k);
 bool check_args(const void *ref, int R, const void *query, int Q, int dim, int k, const void *dist,
                 const void *idx);

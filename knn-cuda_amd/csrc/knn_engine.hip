@@ -40,8 +40,21 @@ void set_error(const char *fmt, ...) {
     g_last_error = buf;
 }
 
+Arena *g_arena = nullptr;
+
 void *ws_get(Slot s, size_t bytes) {
     if (bytes == 0) bytes = 256;
+    if (g_arena) {
+        // a slot asked for again within one call gets its piece back
+        Arena &a = *g_arena;
+        if (a.slot[s] && a.size[s] >= bytes) return a.slot[s];
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (a.used + need > a.cap) return nullptr;
+        a.slot[s] = a.base + a.used;
+        a.size[s] = need;
+        a.used += need;
+        return a.slot[s];
+    }
     if (g_ws.cap[s] >= bytes) return g_ws.ptr[s];
     if (g_ws.ptr[s]) {
         (void)hipFree(g_ws.ptr[s]);
@@ -582,7 +595,7 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         // fallback goes to the exact scan directly.
         const int cur_margin = p.kp - k;
         int next_margin = 0;
-        for (int l = level; l < kMaxEscalation && !next_margin && !o.force_fallback; ++l)
+        for (int l = level; l < kMaxEscalation && !next_margin && !o.force_fallback && !g_arena; ++l)
             if (kEscalationMargins[l] > cur_margin && k + cur_margin < R) next_margin = kEscalationMargins[l];
         if (next_margin) {
             Deferred local;
@@ -640,6 +653,19 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
         }
     }
     return true;
+}
+
+size_t run_device_bytes(const Plan &p, int R, int Q, int dim, int k) {
+    auto piece = [](size_t b) { return ((b ? b : 256) + 255) & ~(size_t)255; };
+    const int egrid = std::min(kExactGrid, Q);
+    size_t n = piece(sizeof(int)) + piece(sizeof(unsigned)) + 2 * piece((size_t)egrid * kWave * k * 4);
+    if (!p.use_scan) return n + piece((size_t)R * 4) + piece((size_t)Q * 4);
+    const int pm_ld = (dim + 3) & ~3;
+    const size_t npart = (size_t)p.nslices * p.nqb * block_queries(p) * p.kp;
+    n += piece((size_t)p.r_pad * 4) + piece((size_t)Q * 4) + piece(4) + 2 * piece(npart * 4) +
+         piece((size_t)Q * 4) + piece((size_t)R * pm_ld * 4) + piece((size_t)Q * pm_ld * 4) +
+         piece((size_t)p.r_pad * p.dpad * 4) + piece((size_t)Q * 4);
+    return n;
 }
 
 knnhip_options default_options(int variant) {
@@ -816,6 +842,82 @@ int knnhip_search_device(const float *ref, int ref_nb, long long ref_ld, const f
                       p, o, s)
                ? 1
                : 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scratch of knnhip_search_device_noalloc: every buffer its search needs
+// besides the caller's inputs and outputs (norms, packed tiles, point-major
+// rows, heaps, thresholds, counters, exact-scan lists).  Each device the
+// library runs on holds one copy until the process ends.  In host code the
+// array's name is only a stand-in: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr size_t kNoAllocBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address (nullptr with an error set).
+void *noalloc_scratch() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        set_error("cannot query the current device");
+        return nullptr;
+    }
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            set_error("scratch address: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int knnhip_search_device_noalloc(const float *ref, int ref_nb, long long ref_ld, const float *query,
+                                 int query_nb, long long query_ld, int dim, int k, float *knn_dist,
+                                 int *knn_index, const knnhip_options *opt, void *stream) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_last_error.clear();
+    if (!check_args(ref, ref_nb, query, query_nb, dim, k, knn_dist, knn_index)) return 0;
+    if (ref_ld < ref_nb || query_ld < query_nb) {
+        set_error("leading dimensions must be >= point counts");
+        return 0;
+    }
+    if (!ensure_device()) return 0;
+    knnhip_options o = opt ? *opt : default_options(0);
+    o.timing = 0;  // (its events would be created on first use)
+    hipStream_t s = (hipStream_t)stream;
+    const Plan p = make_plan(ref_nb, query_nb, dim, k, o);
+    const size_t need = run_device_bytes(p, ref_nb, query_nb, dim, k);
+    if (need > kNoAllocBytes) {
+        set_error("knnhip_search_device_noalloc: this search needs %zu bytes of scratch, more than the "
+                  "%zu it has (use knnhip_search_device)",
+                  need, kNoAllocBytes);
+        return 0;
+    }
+    void *base = noalloc_scratch();
+    if (!base) return 0;
+    CallOrder order(s);
+    g_last_plan = plan_json(p, ref_nb, query_nb, dim, k);
+    Arena arena;
+    arena.base = static_cast<unsigned char *>(base);
+    arena.cap = kNoAllocBytes;
+    g_arena = &arena;
+    const bool ok = run_device(ref, ref_ld, ref_nb, query, query_ld, query_nb, dim, k, knn_dist, knn_index,
+                               p, o, s);
+    g_arena = nullptr;
+    return ok ? 1 : 0;
 }
 
 int knnhip_last_fallback_count(void) {

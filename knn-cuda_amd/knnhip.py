@@ -71,6 +71,7 @@ C_SYMBOLS = (
     "knnhip_stream_chunks",
     "knnhip_version",
     "knnhip_index_last_work",
+    "knnhip_search_device_noalloc",
 )
 
 VARIANTS = {"global": 0, "texture": 1, "cublas": 2}
@@ -123,6 +124,8 @@ def load() -> ctypes.CDLL:
         ctypes.POINTER(Options), ctypes.c_void_p,
     ]
     lib.knnhip_search_device.restype = ctypes.c_int
+    lib.knnhip_search_device_noalloc.argtypes = lib.knnhip_search_device.argtypes
+    lib.knnhip_search_device_noalloc.restype = ctypes.c_int
     lib.knnhip_last_fallback_count.restype = ctypes.c_int
     lib.knnhip_last_stage_ms.argtypes = [ctypes.c_int]
     lib.knnhip_last_stage_ms.restype = ctypes.c_float
@@ -354,6 +357,37 @@ def search_device(ref, query, k: int, dist=None, index=None, *, variant: str = "
     if stream is None:
         stream = torch.cuda.current_stream(query.device).cuda_stream
     ok = lib.knnhip_search_device(
+        ref.data_ptr(), ref_nb, ref.stride(0), query.data_ptr(), query_nb, query.stride(0), dim, k,
+        dist.data_ptr(), index.data_ptr(), ctypes.byref(opt), ctypes.c_void_p(stream),
+    )
+    if not ok:
+        raise KnnError(last_error())
+    return dist, index
+
+
+def search_device_noalloc(ref, query, k: int, dist, index, *, margin: int = 0, precision: str = "fp32",
+                          stream: Optional[int] = None):
+    """``search_device`` through the entry point that allocates no device memory (its buffers live
+    in the library's static scratch array): the caller passes the (k, query_nb) outputs."""
+    import torch
+
+    _check_points("ref", ref)
+    _check_points("query", query, dim=ref.shape[0])
+    dim, ref_nb = ref.shape
+    query_nb = query.shape[1]
+    if not 1 <= k <= ref_nb:
+        raise KnnError(f"k must satisfy 1 <= k <= ref_nb (k={k} ref_nb={ref_nb})")
+    _check_out("dist", dist, (k, query_nb), torch.float32, query.device)
+    _check_out("index", index, (k, query_nb), torch.int32, query.device)
+    _check_device(ref, query)
+    if precision not in PRECISIONS:
+        raise KnnError(f"precision must be one of {sorted(PRECISIONS)}")
+    opt = Options()
+    opt.margin = margin
+    opt.precision = PRECISIONS[precision]
+    if stream is None:
+        stream = torch.cuda.current_stream(query.device).cuda_stream
+    ok = load().knnhip_search_device_noalloc(
         ref.data_ptr(), ref_nb, ref.stride(0), query.data_ptr(), query_nb, query.stride(0), dim, k,
         dist.data_ptr(), index.data_ptr(), ctypes.byref(opt), ctypes.c_void_p(stream),
     )
