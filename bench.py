@@ -275,14 +275,6 @@ def plain_form(args, knnhip, torch):
     dist = torch.empty((k, Q), dtype=torch.float32, device=qry.device)
     idx = torch.empty((k, Q), dtype=torch.int32, device=qry.device)
     stream = torch.cuda.current_stream(qry.device)
-    if args.copy_calibration_mib > 0:
-        n = args.copy_calibration_mib * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device=qry.device)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     for _ in range(2):
         knnhip.search_device(ref, qry, k, dist, idx, stream=stream.cuda_stream)
     torch.cuda.synchronize()
@@ -555,6 +547,15 @@ def main():
     import knnhip
 
     torch.cuda.synchronize()
+    if args.copy_calibration_mib > 0:
+        # one device copy of known size, the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     head, ref, qry, dist, idx, ix, mlp = headline(args, knnhip, torch)
     R, Q, D, k = C5["R"], C5["Q"], C5["D"], C5["k"]
 

@@ -77,7 +77,10 @@ typedef struct knnhip_options {
     int score_all;      /* index searches given scores: 0 the scores choose each query's
                            first probe (scanned first), the lower bounds rank the others;
                            1 the scores rank every probe */
-    int reserved[5];
+    int whole_blocks;   /* index searches: 0 the scan visits the balls of each probed block
+                           that may hold a neighbour (indexes that know their balls' position
+                           ranges: built ones and version-2 files), 1 whole probed blocks */
+    int reserved[4];
 } knnhip_options;
 
 /*
@@ -197,6 +200,11 @@ int knnhip_index_stage_ms(const knnhip_index *index, float *ms, int n);
  * residuals).  Any pointer may be NULL.  For benchmarks (roofline figures). */
 int knnhip_index_last_work(const knnhip_index *index, long long *exec_pairs, long long *pairs,
                            long long *tile_bytes, int *precision);
+/* Scan units of the last index search: balls (1) or whole blocks (0), its
+ * heap rows per query (units visited at most), and the index's ball count
+ * when it knows the balls' position ranges (else 0).  Any pointer may be
+ * NULL.  No counterpart in the reference (it has no index). */
+int knnhip_index_last_units(const knnhip_index *index, int *balls, int *rows_per_query, int *nballs);
 /* Block layout of the index: block_start / block_count (host, nblocks ints
  * each: first position and size of each block in the block-contiguous copy)
  * and ref_block (device, ref_nb ints: the block holding each reference point,

@@ -203,6 +203,8 @@ bool batch_class_queue(const BatchClass &c, int L, const knnhip_options &o, int 
                 tk.ref_end = starts[i] + std::min(p->ref_nb, (sl + 1) * L);
                 tk.qoff = t * kQueriesPerWave;
                 tk.qcount = std::min(kQueriesPerWave, p->query_nb - g);
+                tk.ref_lo = tk.ref_begin;
+                tk.unit = i;
                 tasks.push_back(tk);
                 for (int j = 0; j < kQueriesPerWave; ++j) {
                     const int q = qstart[i] + std::min(g + j, p->query_nb - 1);

@@ -25,13 +25,17 @@ constexpr int kBRegs = 0, kBChunks = 1, kBLds = 2;
 constexpr int kMaxUnion = 2048;      // max slices*kp entries merged per query
 constexpr int kMaxCand = 1024;       // max candidates re-ranked exactly per query
 
-// One scan task: references [ref_begin, ref_end) (ref_begin a multiple of
-// 32 in the packed layout) against up to 32 queries qlist[qoff .. qoff+qcount).
+// One scan task: references [ref_lo, ref_end) against up to 32 queries
+// qlist[qoff .. qoff+qcount).  ref_begin <= ref_lo is ref_lo rounded down to
+// a multiple of 32 (the packed tiles start there); rows in [ref_begin,
+// ref_lo) take no part.  unit: the block (or ball) the range belongs to.
 struct ScanTask {
     int ref_begin;
     int ref_end;
     int qoff;
     int qcount;
+    int ref_lo;
+    int unit;
 };
 
 // Arguments of the FP32 MFMA scan (one launch covers all queries and slices).
@@ -77,7 +81,6 @@ struct ScanArgs {
     const float *res_norm;
     const float *res_cent;
     const float *res_rho;
-    const int *tile_block;
     float *res_row;
     float res_eta_coef;  // key error per (||e_q|| + ||e_r||)^2
     float res_norm_up;   // exact norm over its FP32 sum, at most

@@ -69,9 +69,9 @@ int scan_res_nch(int dim);
 float res_eta_coeff(int dim, int nch);
 float res_norm_up(int dim);
 float res_norm_dn(int dim);
-hipError_t launch_resid_build(const float *pm, int ld, int dim, int nch, const int *block_start,
-                              const int *block_count, int nb, float *cent, unsigned short *tiles, float *rnorm,
-                              float *rho, int *tile_block, hipStream_t s);
+hipError_t launch_resid_build(const float *pm, int ld, int dim, int nch, const int *unit_start,
+                              const int *unit_count, int nunits, float *cent, unsigned short *tiles, float *rnorm,
+                              float *rho, hipStream_t s);
 hipError_t launch_scan_res_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
                                 int tasks_per_group = 1);
 hipError_t launch_norms(const NormArgs &a, hipStream_t s);
@@ -350,11 +350,20 @@ struct knnhip_index {
     // by the first precision search; fcent_nch chunks
     int fcent_nch = 0;
     knnhip::DevBuf fcent_bf;
-    // residual form: block centroids (rows of pm_ld floats), bfloat16
-    // residual tiles over res_nch chunks, their squared norms, block radii
-    // and the block of each tile; built by the first search that asks
+    // position range of every ball (block-sorted order): its members occupy
+    // positions ball_pos[f] .. ball_pos[f] + ball_cnt[f]; the ranges of a
+    // block's balls partition the block.  Known for built indexes and for
+    // files that store them; without them searches probe whole blocks only.
+    bool ball_ranges = false;
+    int max_bpb = 0;  // most balls in one block
+    knnhip::DevBuf ball_pos, ball_cnt;
+    // residual form: per unit (a ball where the ranges are known, else a
+    // block) its centroid (rows of pm_ld floats) and radius, and the bfloat16
+    // residual tiles of every position over res_nch chunks with their squared
+    // norms; built by the first search that asks
     int res_nch = 0;
-    knnhip::DevBuf res_cent, res_tiles, res_norm, res_rho, tile_block;
+    bool res_balls = false;
+    knnhip::DevBuf res_cent, res_tiles, res_norm, res_rho;
     // last search statistics
     int last_probes = 0;
     int last_fail = 0;
@@ -363,6 +372,9 @@ struct knnhip_index {
     // three-term bfloat16, 2 residual bfloat16), for knnhip_index_last_work
     long long last_ref_bytes = 0;
     int last_prec = 0;
+    // the last search's scan units (balls or whole blocks) and heap rows per query
+    bool last_balls = false;
+    int last_rows = 0;
     float last_ms[6] = {-1, -1, -1, -1, -1, -1};
     float rank_ms[3] = {-1, -1, -1};  // query norms, block bounds, block choice
 };

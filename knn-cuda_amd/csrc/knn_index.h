@@ -58,6 +58,24 @@ hipError_t idx_finalize_fine(const float *pm, int ld, int dim, const float *fcen
                              hipStream_t s);
 hipError_t idx_ball_aux(const float *fnorm, const double *frad, int nf, float *fsq, float *fradf,
                         hipStream_t s);
+// Ball probes (knn_index.hip): the nearest ball of each query's first
+// block into out[q * Pb], then, from the threshold theta that scan left,
+// up to cap more balls of its P blocks into out[q * Pb + 1 ..] with the
+// others' lower bounds folded into lb_rest.  res: keys approximate squared
+// distances (else ||r||^2 - 2 r.q with error coefficients eq, er).  At most
+// idx_ball_cand_max() candidate balls per query.
+hipError_t idx_ball_first(const float *qpm, int pm_ld, int dim, int Q, const int *chosen, int P,
+                          const float *fcent, const double *frad, const int *fb_start, int *out, int Pb,
+                          hipStream_t s);
+hipError_t idx_ball_rest(const float *qpm, int pm_ld, int dim, int Q, const float *qnorm, const unsigned *theta,
+                         const unsigned *rmax, double eq, double er, int res, const int *chosen, int P,
+                         const float *fcent, const double *frad, const int *fb_start, int *out, int Pb, int cap,
+                         float *lb_rest, hipStream_t s);
+int idx_ball_cand_max();
+// Radii of balls with known position ranges (largest member distance, FP64;
+// -1 for an empty ball).
+hipError_t idx_ball_range_radius(const float *pm, int ld, int dim, const float *fcent, const int *ball_pos,
+                                 const int *ball_cnt, int nf, double *rad, hipStream_t s);
 // Ball radii refitted to the block-ordered points pm (a loaded index):
 // every point joins the nearest ball of its block.  d2bits / member: nf
 // words of scratch each; rad: nf radii (-1: no member).

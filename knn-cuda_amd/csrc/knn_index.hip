@@ -469,6 +469,169 @@ __global__ __launch_bounds__(64 * kPickWaves) void router_first_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Ball probes.  Inside the blocks a query probes, the scan can visit balls
+// (each a contiguous position range of its block) instead of whole blocks:
+// first the ball of the query's first block whose centre is nearest, then,
+// once that scan has set the query's threshold, every other ball of its
+// probed blocks whose lower bound lies below the distance the threshold
+// stands for (nearest bounds first, at most `cap`).  Each ball left out
+// lowers the query's bound on unscanned references to its own lower bound,
+// so the threshold only steers the work: exactness rests on the certificate.
+// ---------------------------------------------------------------------------
+
+// FP64 distance between two FP32 rows (16-byte aligned): each difference is
+// exact in FP64 and each fused square-add rounds once, so the result is
+// within (dim + 2) * 2^-53 relative of the true distance.
+__device__ inline double row_dist64(const float *__restrict__ a, const float *__restrict__ b, int dim) {
+    double acc = 0.0;
+    int d = 0;
+    for (; d + 4 <= dim; d += 4) {
+        const float4 x = *reinterpret_cast<const float4 *>(a + d);
+        const float4 y = *reinterpret_cast<const float4 *>(b + d);
+        double t = (double)x.x - (double)y.x;
+        acc = fma(t, t, acc);
+        t = (double)x.y - (double)y.y;
+        acc = fma(t, t, acc);
+        t = (double)x.z - (double)y.z;
+        acc = fma(t, t, acc);
+        t = (double)x.w - (double)y.w;
+        acc = fma(t, t, acc);
+    }
+    for (; d < dim; ++d) {
+        const double t = (double)a[d] - (double)b[d];
+        acc = fma(t, t, acc);
+    }
+    return sqrt(acc);
+}
+
+// Lower bound, rounded down to FP32, on the distance from a query to every
+// member of a ball: FP64 centre distance d (relative error far below 1e-12)
+// less the ball's FP64 radius.
+__device__ inline float ball_member_lb(double d, double rho) {
+    return __double2float_rd(d * (1.0 - 1e-12) - rho);
+}
+
+constexpr int kBallWaves = 4;  // queries (one wave each) per workgroup of the ball kernels
+
+// First ball: out[q * Pb] = the non-empty ball of block chosen[q * P] whose
+// centre is nearest to query q (ties: lower ball), or -1.
+__global__ __launch_bounds__(64 * kBallWaves) void ball_first_kernel(
+    const float *__restrict__ qpm, int pm_ld, int dim, int Q, const int *__restrict__ chosen, int P,
+    const float *__restrict__ fcent, const double *__restrict__ frad, const int *__restrict__ fb_start,
+    int *__restrict__ out, int Pb) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = blockIdx.x * kBallWaves + wave;
+    if (q >= Q) return;
+    const int b = chosen[(long long)q * P];
+    double best = INFINITY;
+    int bf = -1;
+    if (b >= 0) {
+        const float *qr = qpm + (long long)q * pm_ld;
+        for (int f = fb_start[b] + lane; f < fb_start[b + 1]; f += kWave) {
+            if (!(frad[f] >= 0.0)) continue;
+            const double d = row_dist64(qr, fcent + (long long)f * pm_ld, dim);
+            if (d < best || (d == best && f < bf)) {
+                best = d;
+                bf = f;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double od = __shfl_xor(best, o);
+        const int of = __shfl_xor(bf, o);
+        if (of >= 0 && (bf < 0 || od < best || (od == best && of < bf))) {
+            best = od;
+            bf = of;
+        }
+    }
+    if (lane == 0) out[(long long)q * Pb] = bf;
+}
+
+// Other balls: for query q, every non-empty ball of blocks chosen[q * P ..
+// q * P + P) except out[q * Pb] is a candidate with bound lb_f.  Those with
+// lb_f below T_q, nearest bound first (ties: lower ball), fill out[q * Pb +
+// 1 ..] up to cap entries (the rest -1); every other candidate lowers
+// lb_rest[q] to lb_f.  T_q: with keys ||r||^2 - 2 r.q (res == 0),
+// sqrt(theta + ||q||^2 (1 + g + eq) + er max||r||^2), the distance the
+// threshold theta stands for widened by the key error bound; with keys that
+// approximate squared distances (res != 0), sqrt(theta) * 1.02 + 1e-3.
+// Candidates per query: at most kBallCand (the host checks P times the
+// largest ball count of a block against it).
+constexpr int kBallCand = 512;
+__global__ __launch_bounds__(64 * kBallWaves) void ball_rest_kernel(
+    const float *__restrict__ qpm, int pm_ld, int dim, int Q, const float *__restrict__ qnorm,
+    const unsigned *__restrict__ theta, const unsigned *__restrict__ rmax, double eq, double er, int res,
+    const int *__restrict__ chosen, int P, const float *__restrict__ fcent, const double *__restrict__ frad,
+    const int *__restrict__ fb_start, int *__restrict__ out, int Pb, int cap, float *__restrict__ lb_rest) {
+    __shared__ float c_lb[kBallWaves][kBallCand];
+    __shared__ int c_f[kBallWaves][kBallCand];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = blockIdx.x * kBallWaves + wave;
+    if (q >= Q) return;
+    float *lbs = c_lb[wave];
+    int *fs = c_f[wave];
+    const long long o0 = (long long)q * Pb;
+    const int f0 = out[o0];
+    const float *qr = qpm + (long long)q * pm_ld;
+    // candidates, block by block: slot n + (f - first ball of the block)
+    int n = 0;
+    for (int j = 0; j < P; ++j) {
+        const int b = chosen[(long long)q * P + j];
+        if (b < 0) continue;
+        const int fl = fb_start[b], fh = fb_start[b + 1];
+        for (int f = fl + lane; f < fh; f += kWave) {
+            const double rho = frad[f];
+            int ff = -1;
+            float lb = INFINITY;
+            if (f != f0 && rho >= 0.0) {
+                lb = ball_member_lb(row_dist64(qr, fcent + (long long)f * pm_ld, dim), rho);
+                ff = f;
+            }
+            lbs[n + f - fl] = lb;
+            fs[n + f - fl] = ff;
+        }
+        n += fh - fl;
+    }
+    wave_sync();
+    double T;
+    const float th = ordered_to_float(theta[q]);
+    if (res) {
+        T = th > 0.0f ? sqrt((double)th) * 1.02 + 1e-3 : 1e-3;
+    } else {
+        const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
+        const double nq = (double)qnorm[q], nr = (double)__uint_as_float(*rmax);
+        const double t2 = (double)th + nq * (1.0 + g + eq) + er * nr;
+        T = t2 > 0.0 ? sqrt(t2) : 0.0;
+    }
+    // taken: rank among the passing candidates below cap
+    int npass = 0;
+    for (int i = 0; i < n; ++i) npass += (fs[i] >= 0 && (double)lbs[i] < T) ? 1 : 0;
+    float lbmin = INFINITY;
+    for (int i = lane; i < n; i += kWave) {
+        const int f = fs[i];
+        if (f < 0) continue;
+        const float lb = lbs[i];
+        bool take = false;
+        if ((double)lb < T) {
+            int rank = 0;
+            for (int t = 0; t < n; ++t) {
+                const int g2 = fs[t];
+                const float l2 = lbs[t];
+                rank += (g2 >= 0 && (double)l2 < T && (l2 < lb || (l2 == lb && g2 < f))) ? 1 : 0;
+            }
+            if (rank < cap) {
+                out[o0 + 1 + rank] = f;
+                take = true;
+            }
+        }
+        if (!take) lbmin = fminf(lbmin, lb);
+    }
+    for (int i = min(npass, cap) + lane; i < Pb - 1; i += kWave) out[o0 + 1 + i] = -1;
+    for (int o = 32; o > 0; o >>= 1) lbmin = fminf(lbmin, __shfl_xor(lbmin, o));
+    if (lane == 0 && lbmin < lb_rest[q]) lb_rest[q] = lbmin;
+}
+
 // Invert the (query -> blocks) lists, probes [p0, p1) of each query's P, into
 // per-block query lists and scan tasks of up to 32 queries.  Step 1: count
 // pairs per block.
@@ -529,10 +692,11 @@ __global__ void scatter_results_kernel(const float *__restrict__ dsub, const int
     }
 }
 
-// Step 3: tasks.  Block b's pairs occupy qlist[qstart[b] .. qstart[b] +
+// Step 3: tasks.  Unit b's pairs occupy qlist[qstart[b] .. qstart[b] +
 // count[b]) with qstart[b] a multiple of 32; task t covers list slots
 // [32t, 32t+32) (slot == output row), so tasks are enumerated from the
-// aligned slot space directly.
+// aligned slot space directly.  Units are blocks (tile-aligned starts) or
+// balls (any start: the task's tiles begin at the start rounded down to 32).
 __global__ void make_tasks_kernel(const int *__restrict__ qstart, const int *__restrict__ counts,
                                   int nb, const int *__restrict__ block_start,
                                   const int *__restrict__ block_count, ScanTask *__restrict__ tasks,
@@ -543,10 +707,12 @@ __global__ void make_tasks_kernel(const int *__restrict__ qstart, const int *__r
             const int t = (qstart[b] + j) / kQueriesPerWave;
             if (t >= max_tasks) continue;
             ScanTask tk;
-            tk.ref_begin = block_start[b];
+            tk.ref_lo = block_start[b];
+            tk.ref_begin = block_start[b] / kRefsPerTile * kRefsPerTile;
             tk.ref_end = block_start[b] + block_count[b];
             tk.qoff = qstart[b] + j;
             tk.qcount = min(kQueriesPerWave, c - j);
+            tk.unit = b;
             tasks[t] = tk;
         }
     }
@@ -560,6 +726,8 @@ __global__ void clear_tasks_kernel(ScanTask *__restrict__ tasks, int n) {
         tk.ref_end = 0;
         tk.qoff = 0;
         tk.qcount = 0;
+        tk.ref_lo = 0;
+        tk.unit = 0;
         tasks[t] = tk;
     }
 }
@@ -941,6 +1109,36 @@ __global__ __launch_bounds__(256) void ball_refit_kernel(const float *__restrict
             atomicOr(member + bf, 1);
         }
     }
+}
+
+// Radii of balls with known position ranges: one workgroup per ball, the
+// largest FP64 distance from its centre to a position of its range.
+__global__ __launch_bounds__(256) void ball_range_radius_kernel(const float *__restrict__ pm, int ld, int dim,
+                                                                const float *__restrict__ fcent,
+                                                                const int *__restrict__ ball_pos,
+                                                                const int *__restrict__ ball_cnt, double *rad) {
+    __shared__ double red[256];
+    const int f = blockIdx.x, tid = threadIdx.x;
+    const long long p0 = ball_pos[f];
+    const int n = ball_cnt[f];
+    const float *c = fcent + (long long)f * ld;
+    double best = 0.0;
+    for (int j = tid; j < n; j += blockDim.x) {
+        const float *r = pm + (p0 + j) * ld;
+        double acc = 0.0;
+        for (int d = 0; d < dim; ++d) {
+            const double t = (double)r[d] - (double)c[d];
+            acc += t * t;
+        }
+        best = fmax(best, acc);
+    }
+    red[tid] = best;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+        __syncthreads();
+    }
+    if (tid == 0) rad[f] = n > 0 ? sqrt(red[0]) * (1.0 + 1e-12) : -1.0;
 }
 
 __global__ void ball_refit_finish_kernel(const unsigned long long *__restrict__ d2bits,
@@ -1471,6 +1669,30 @@ hipError_t idx_ball_refit(const float *pm, int ld, int dim, const int *block_sta
     hipLaunchKernelGGL(ball_refit_kernel, dim3(nb), dim3(256), 0, s, pm, ld, dim, block_start, block_count,
                        fb_start, fcent, d2bits, member);
     hipLaunchKernelGGL(ball_refit_finish_kernel, dim3(grid_for(nf)), dim3(256), 0, s, d2bits, member, nf,
+                       rad);
+    return hipGetLastError();
+}
+hipError_t idx_ball_first(const float *qpm, int pm_ld, int dim, int Q, const int *chosen, int P,
+                          const float *fcent, const double *frad, const int *fb_start, int *out, int Pb,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(ball_first_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves), 0, s,
+                       qpm, pm_ld, dim, Q, chosen, P, fcent, frad, fb_start, out, Pb);
+    return hipGetLastError();
+}
+int idx_ball_cand_max() { return kBallCand; }
+hipError_t idx_ball_rest(const float *qpm, int pm_ld, int dim, int Q, const float *qnorm, const unsigned *theta,
+                         const unsigned *rmax, double eq, double er, int res, const int *chosen, int P,
+                         const float *fcent, const double *frad, const int *fb_start, int *out, int Pb, int cap,
+                         float *lb_rest, hipStream_t s) {
+    if (cap < 0 || cap > Pb - 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ball_rest_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves), 0, s,
+                       qpm, pm_ld, dim, Q, qnorm, theta, rmax, eq, er, res, chosen, P, fcent, frad, fb_start, out,
+                       Pb, cap, lb_rest);
+    return hipGetLastError();
+}
+hipError_t idx_ball_range_radius(const float *pm, int ld, int dim, const float *fcent, const int *ball_pos,
+                                 const int *ball_cnt, int nf, double *rad, hipStream_t s) {
+    hipLaunchKernelGGL(ball_range_radius_kernel, dim3(nf), dim3(256), 0, s, pm, ld, dim, fcent, ball_pos, ball_cnt,
                        rad);
     return hipGetLastError();
 }

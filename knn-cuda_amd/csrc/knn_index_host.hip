@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <vector>
 
@@ -30,6 +31,9 @@ constexpr int kIndexMarginRes = 24;
 constexpr double kOneWaveQueries = 4.0;
 // One-wave bfloat16 tasks per workgroup (each with its own heaps).
 constexpr int kTasksPerGroup = 2;
+// Ball probes: at most this many balls (heap rows) per query, the first
+// included; further balls below the threshold are left to the certificate.
+constexpr int kMaxBallRows = 24;
 
 // Lloyd iterations on the point-major rows `pm` (n points) from the `k`
 // centroids already in `cent` (point-major, ld pm_ld).  The assignment step
@@ -63,6 +67,22 @@ bool kmeans(const float *pm, const float *dm, long long ldd, int n, int dim, int
                               pm_ld, s),
                 "centroids");
     }
+    return true;
+}
+
+// Keeps the balls' position ranges (host arrays, block-sorted order) and the
+// largest ball count of a block.
+bool ball_ranges_set(knnhip_index *ix, const int *pos, const int *cnt, const int *fbs) {
+    if (!ix->ball_pos.alloc((size_t)ix->nf * 4) || !ix->ball_cnt.alloc((size_t)ix->nf * 4)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    KNN_TRY(hipMemcpy(ix->ball_pos.p, pos, (size_t)ix->nf * 4, hipMemcpyHostToDevice), "copy");
+    KNN_TRY(hipMemcpy(ix->ball_cnt.p, cnt, (size_t)ix->nf * 4, hipMemcpyHostToDevice), "copy");
+    int m = 0;
+    for (int b = 0; b < ix->nb; ++b) m = std::max(m, fbs[b + 1] - fbs[b]);
+    ix->max_bpb = m;
+    ix->ball_ranges = true;
     return true;
 }
 
@@ -231,6 +251,22 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
     KNN_TRY(idx_ball_aux(ix->fnorm.as<float>(), ix->frad.as<double>(), nf, ix->fsq.as<float>(),
                          ix->fradf.as<float>(), s),
             "ball roots");
+    // each ball's position range, in the block-sorted order of the balls
+    {
+        std::vector<int> hs(nf), hst(nf), hc(nf), hfb(nb + 1);
+        KNN_TRY(hipMemcpyAsync(hs.data(), fsorted.p, (size_t)nf * 4, hipMemcpyDeviceToHost, s), "copy");
+        KNN_TRY(hipMemcpyAsync(hst.data(), fstart.p, (size_t)nf * 4, hipMemcpyDeviceToHost, s), "copy");
+        KNN_TRY(hipMemcpyAsync(hc.data(), fcount.p, (size_t)nf * 4, hipMemcpyDeviceToHost, s), "copy");
+        KNN_TRY(hipMemcpyAsync(hfb.data(), ix->fb_start.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s),
+                "copy");
+        KNN_TRY(hipStreamSynchronize(s), "sync");
+        std::vector<int> bp(nf), bc(nf);
+        for (int j = 0; j < nf; ++j) {
+            bp[j] = hst[hs[j]];
+            bc[j] = hc[hs[j]];
+        }
+        if (!ball_ranges_set(ix, bp.data(), bc.data(), hfb.data())) return false;
+    }
     KNN_TRY(hipEventRecord(e1, s), "event");
     KNN_TRY(hipEventSynchronize(e1), "event");
     KNN_TRY(hipEventElapsedTime(&ix->build_ms, e0, e1), "event");
@@ -240,13 +276,14 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
 }
 
 // On-disk format of a block index (little-endian): magic "KNNHIPIX", uint32
-// version (1), int32 R, dim, nb, nf, n_pos, r_pad; then block_start [nb+1],
-// block_count [nb], perm [r_pad] (int32), ball centres [nf][pm_ld], centre
-// norms [nf] (float32), radii [nf] (float64), ball blocks [nf] and ball
-// starts [nb+1] (int32).  The packed tiles, point-major copy and norms are
-// rebuilt from the caller's reference set when the file is loaded.
+// version (1 or 2), int32 R, dim, nb, nf, n_pos, r_pad; then block_start
+// [nb+1], block_count [nb], perm [r_pad] (int32), ball centres [nf][pm_ld],
+// centre norms [nf] (float32), radii [nf] (float64), ball blocks [nf] and
+// ball starts [nb+1] (int32); version 2 adds each ball's first position and
+// member count [nf] each (int32).  The packed tiles, point-major copy and
+// norms are rebuilt from the caller's reference set when the file is loaded.
 constexpr char kIndexMagic[8] = {'K', 'N', 'N', 'H', 'I', 'P', 'I', 'X'};
-constexpr unsigned kIndexVersion = 1;
+constexpr unsigned kIndexVersion = 2;
 
 bool index_save(const knnhip_index *ix, const char *path) {
     FILE *f = fopen(path, "wb");
@@ -256,8 +293,9 @@ bool index_save(const knnhip_index *ix, const char *path) {
     }
     FileOut o{f};
     const int hdr[6] = {ix->R, ix->dim, ix->nb, ix->nf, ix->n_pos, ix->r_pad};
+    const unsigned version = ix->ball_ranges ? 2u : 1u;
     o.put(kIndexMagic, 8);
-    o.put(&kIndexVersion, 4);
+    o.put(&version, 4);
     o.put(hdr, sizeof(hdr));
     o.put_dev(ix->block_start, (size_t)(ix->nb + 1) * 4);
     o.put_dev(ix->block_count, (size_t)ix->nb * 4);
@@ -267,6 +305,10 @@ bool index_save(const knnhip_index *ix, const char *path) {
     o.put_dev(ix->frad, (size_t)ix->nf * 8);
     o.put_dev(ix->fblock, (size_t)ix->nf * 4);
     o.put_dev(ix->fb_start, (size_t)(ix->nb + 1) * 4);
+    if (ix->ball_ranges) {
+        o.put_dev(ix->ball_pos, (size_t)ix->nf * 4);
+        o.put_dev(ix->ball_cnt, (size_t)ix->nf * 4);
+    }
     const bool ok = o.ok && fclose(f) == 0;
     if (!ok) set_error("cannot write %s", path);
     return ok;
@@ -311,6 +353,29 @@ const char *index_layout_error(int R, int dim, int nb, int nf, int n_pos, int r_
     return nullptr;
 }
 
+// Ball position ranges of a loaded index, or nullptr when consistent: the
+// ranges of a block's balls (in any order) tile exactly the block's first
+// count positions.
+const char *ball_range_error(int nb, const int *bs, const int *bc, const int *fbs, const int *bp,
+                             const int *bn) {
+    std::vector<std::pair<long long, long long>> r;
+    for (int b = 0; b < nb; ++b) {
+        r.clear();
+        for (int f = fbs[b]; f < fbs[b + 1]; ++f) {
+            if (bn[f] < 0) return "ball position ranges";
+            r.emplace_back(bp[f], (long long)bp[f] + bn[f]);
+        }
+        std::sort(r.begin(), r.end());
+        long long at = bs[b];
+        for (const auto &x : r) {
+            if (x.first != at) return "ball position ranges";
+            at = x.second;
+        }
+        if (at != (long long)bs[b] + bc[b]) return "ball position ranges";
+    }
+    return nullptr;
+}
+
 bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, long long ldr, int dim,
                 hipStream_t s) {
     FILE *f = fopen(path, "rb");
@@ -325,9 +390,9 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
     in.get(magic, 8);
     in.get(&version, 4);
     in.get(hdr, sizeof(hdr));
-    if (!in.ok || memcmp(magic, kIndexMagic, 8) != 0 || version != kIndexVersion) {
+    if (!in.ok || memcmp(magic, kIndexMagic, 8) != 0 || version < 1 || version > kIndexVersion) {
         fclose(f);
-        set_error("%s is not a block index file of version %u", path, kIndexVersion);
+        set_error("%s is not a block index file of version 1 to %u", path, kIndexVersion);
         return false;
     }
     if (hdr[0] != R || hdr[1] != dim || hdr[2] < 1 || hdr[3] < 1 || hdr[4] < R || hdr[5] < hdr[4]) {
@@ -373,6 +438,13 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
     in.get(fr.data(), fr.size() * 8);
     in.get(fbl.data(), fbl.size() * 4);
     in.get(fbs.data(), fbs.size() * 4);
+    std::vector<int> bp, bn;
+    if (version >= 2) {
+        bp.resize(nf);
+        bn.resize(nf);
+        in.get(bp.data(), bp.size() * 4);
+        in.get(bn.data(), bn.size() * 4);
+    }
     char extra;
     const bool tail = fread(&extra, 1, 1, f) == 0;
     fclose(f);
@@ -384,6 +456,15 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
                                              bc.data(), perm.data(), fbs.data(), fbl.data(), fc.data())) {
         set_error("%s: inconsistent index (%s)", path, why);
         return false;
+    }
+    if (version >= 2) {
+        if (const char *why = ball_range_error(nb, bs.data(), bc.data(), fbs.data(), bp.data(), bn.data())) {
+            set_error("%s: inconsistent index (%s)", path, why);
+            return false;
+        }
+        if (!ball_ranges_set(ix, bp.data(), bn.data(), fbs.data())) return false;
+    } else {
+        ix->ball_ranges = false;
     }
     auto upload = [](DevBuf &b, const void *src, size_t n) {
         return b.alloc(n) && hipMemcpy(b.p, src, n, hipMemcpyHostToDevice) == hipSuccess;
@@ -425,10 +506,17 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
             "gather");
     // balls: norms of the centres, radii over the points they now hold
     KNN_TRY(idx_row_norms(ix->fcent.as<float>(), ix->pm_ld, nf, dim, ix->fnorm.as<float>(), s), "norms");
-    KNN_TRY(idx_ball_refit(ix->pm.as<float>(), ix->pm_ld, dim, ix->block_start.as<int>(),
-                           ix->block_count.as<int>(), ix->fb_start.as<int>(), nb, ix->fcent.as<float>(), nf,
-                           d2bits.as<unsigned long long>(), member.as<int>(), ix->frad.as<double>(), s),
-            "ball radii");
+    if (ix->ball_ranges) {
+        // each ball keeps its stored members: radius over its own range
+        KNN_TRY(idx_ball_range_radius(ix->pm.as<float>(), ix->pm_ld, dim, ix->fcent.as<float>(),
+                                      ix->ball_pos.as<int>(), ix->ball_cnt.as<int>(), nf, ix->frad.as<double>(), s),
+                "ball radii");
+    } else {
+        KNN_TRY(idx_ball_refit(ix->pm.as<float>(), ix->pm_ld, dim, ix->block_start.as<int>(),
+                               ix->block_count.as<int>(), ix->fb_start.as<int>(), nb, ix->fcent.as<float>(), nf,
+                               d2bits.as<unsigned long long>(), member.as<int>(), ix->frad.as<double>(), s),
+                "ball radii");
+    }
     KNN_TRY(idx_ball_aux(ix->fnorm.as<float>(), ix->frad.as<double>(), ix->nf, ix->fsq.as<float>(),
                          ix->fradf.as<float>(), s),
             "ball roots");
@@ -439,7 +527,7 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
 // Workspace of index searches (grown on demand, kept between calls).
 struct IndexWork {
     DevBuf qnorm, qpm, lb, chosen, lb_rest, pcount, qstart, cursor, qlist, rows, tasks,
-        pval, pidx, theta, fb_count, fb_list, qsub, dsub, isub, scores, resrow;
+        pval, pidx, theta, fb_count, fb_list, qsub, dsub, isub, scores, resrow, bchosen;
     bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
 IndexWork g_iw, g_iw_retry;
@@ -526,7 +614,17 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         return false;
     }
     P = std::max(1, std::min(P, nb));
-    const long long npairs = (long long)Q * P;
+    // Ball probes (where the index knows its balls' position ranges): the
+    // scan visits the nearest ball of the first block, then up to `cap`
+    // more balls of the P blocks whose bounds lie below the threshold that
+    // scan left; Pb heap rows per query.  Whole blocks otherwise: P rows.
+    const bool balls = ix->ball_ranges && !o.whole_blocks && (long long)P * ix->max_bpb <= idx_ball_cand_max();
+    const int cap = balls ? std::max(0, std::min({P * ix->max_bpb - 1, kMaxUnion / kp - 1, kMaxBallRows - 1})) : 0;
+    const int Pb = balls ? 1 + cap : P;
+    const int nu = balls ? ix->nf : nb;
+    const int *unit_start = balls ? ix->ball_pos.as<int>() : ix->block_start.as<int>();
+    const int *unit_count = balls ? ix->ball_cnt.as<int>() : ix->block_count.as<int>();
+    const long long npairs = (long long)Q * Pb;
     if (nch_bf && ix->nch_bf != nch_bf) {
         ix->nch_bf = 0;
         if (!ix->packed_bf.alloc((size_t)ix->r_pad / kRefsPerTile * nch_bf * 2 * kWave * 16)) {
@@ -538,39 +636,40 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                 "bfloat16 tiles");
         ix->nch_bf = nch_bf;
     }
-    if (nch_res && ix->res_nch != nch_res) {
+    if (nch_res && (ix->res_nch != nch_res || ix->res_balls != balls)) {
+        // residual tiles against the centre of each scan unit (ball or block)
         ix->res_nch = 0;
         const size_t ntile = (size_t)ix->r_pad / kRefsPerTile;
-        if (!ix->res_cent.alloc((size_t)nb * pm_ld * 4) || !ix->res_tiles.alloc(ntile * nch_res * kWave * 16) ||
-            !ix->res_norm.alloc((size_t)ix->r_pad * 4) || !ix->res_rho.alloc((size_t)nb * 4) ||
-            !ix->tile_block.alloc(ntile * 4)) {
+        if (!ix->res_cent.alloc((size_t)nu * pm_ld * 4) || !ix->res_tiles.alloc(ntile * nch_res * kWave * 16) ||
+            !ix->res_norm.alloc((size_t)ix->r_pad * 4) || !ix->res_rho.alloc((size_t)nu * 4)) {
             set_error("Memory allocation error");
             return false;
         }
         KNN_TRY(hipMemsetAsync(ix->res_tiles.p, 0, ix->res_tiles.n, s), "memset");
         KNN_TRY(hipMemsetAsync(ix->res_norm.p, 0, ix->res_norm.n, s), "memset");
-        KNN_TRY(hipMemsetAsync(ix->tile_block.p, 0, ix->tile_block.n, s), "memset");
-        KNN_TRY(launch_resid_build(ix->pm.as<float>(), pm_ld, dim, nch_res, ix->block_start.as<int>(),
-                                   ix->block_count.as<int>(), nb, ix->res_cent.as<float>(),
-                                   ix->res_tiles.as<unsigned short>(), ix->res_norm.as<float>(),
-                                   ix->res_rho.as<float>(), ix->tile_block.as<int>(), s),
+        KNN_TRY(launch_resid_build(ix->pm.as<float>(), pm_ld, dim, nch_res, unit_start, unit_count, nu,
+                                   ix->res_cent.as<float>(), ix->res_tiles.as<unsigned short>(),
+                                   ix->res_norm.as<float>(), ix->res_rho.as<float>(), s),
                 "residual tiles");
         ix->res_nch = nch_res;
+        ix->res_balls = balls;
     }
-    // Two scan phases: each query's first-ranked block, then its other
-    // probes, which start from the thresholds the first phase left.
-    const int nphase = P > 1 ? 2 : 1;
-    const int pr[3] = {0, 1, P};
+    // Two scan phases: each query's first unit (its first-ranked block, or
+    // that block's nearest ball), then its other units, which start from the
+    // thresholds the first phase left.
+    const int nphase = Pb > 1 ? 2 : 1;
+    const int pr[3] = {0, 1, Pb};
     int max_t[2] = {0, 0};
     for (int ph = 0; ph < nphase; ++ph)
         max_t[ph] = (int)(((long long)Q * (pr[ph + 1] - pr[ph]) + kQueriesPerWave - 1) /
-                          kQueriesPerWave) + nb;
+                          kQueriesPerWave) + nu;
     const int max_tasks = max_t[0] + max_t[1];
     if (!w.need(w.qnorm, (size_t)Q * 4) || !w.need(w.qpm, (size_t)Q * pm_ld * 4) ||
         !w.need(w.lb, (size_t)Q * nb * 4) ||
-        !w.need(w.chosen, (size_t)npairs * 4) || !w.need(w.lb_rest, (size_t)Q * 4) ||
-        !w.need(w.pcount, (size_t)nb * 4) || !w.need(w.qstart, (size_t)(nb + 1) * 4) ||
-        !w.need(w.cursor, (size_t)(nb + 1) * 4) ||
+        !w.need(w.chosen, (size_t)Q * P * 4) || !w.need(w.lb_rest, (size_t)Q * 4) ||
+        (balls && !w.need(w.bchosen, (size_t)npairs * 4)) ||
+        !w.need(w.pcount, (size_t)nu * 4) || !w.need(w.qstart, (size_t)(nu + 1) * 4) ||
+        !w.need(w.cursor, (size_t)(nu + 1) * 4) ||
         !w.need(w.qlist, (size_t)max_tasks * kQueriesPerWave * 4) ||
         !w.need(w.rows, (size_t)npairs * 4) || !w.need(w.tasks, (size_t)max_tasks * sizeof(ScanTask)) ||
         !w.need(w.pval, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
@@ -626,27 +725,46 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
                             w.chosen.as<int>(), w.lb_rest.as<float>(), s),
             "pick blocks");
+    if (balls)
+        KNN_TRY(idx_ball_first(w.qpm.as<float>(), pm_ld, dim, Q, w.chosen.as<int>(), P, ix->fcent.as<float>(),
+                               ix->frad.as<double>(), ix->fb_start.as<int>(), w.bchosen.as<int>(), Pb, s),
+                "first balls");
     if (timing) KNN_TRY(hipEventRecord(ev[1], s), "event");
     if (timing) KNN_TRY(hipEventRecord(ev[2], s), "event");
     KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Q, s), "memset");
     KNN_TRY(hipMemsetAsync(w.fb_count.p, 0, 4, s), "memset");
+    // the scan units each query visits: Pb per query
+    int *units = balls ? w.bchosen.as<int>() : w.chosen.as<int>();
     int tbase = 0;
     for (int ph = 0; ph < nphase; ++ph) {
-        // (query, block) pairs of this phase -> per-block query lists -> tasks
+        if (balls && ph == 1) {
+            // the other balls, now that phase 0 has set the thresholds
+            double eq = 0.0, er = 0.0;
+            if (nch_bf) {
+                bf3_error_bound(dim, nch_bf, &eq, &er);
+            } else {
+                eq = 2.0 * (dim + 4) * 0x1p-24;
+                er = 3.1 * eq;
+            }
+            KNN_TRY(idx_ball_rest(w.qpm.as<float>(), pm_ld, dim, Q, w.qnorm.as<float>(), w.theta.as<unsigned>(),
+                                  ix->rmax.as<unsigned>(), eq, er, nch_res ? 1 : 0, w.chosen.as<int>(), P,
+                                  ix->fcent.as<float>(), ix->frad.as<double>(), ix->fb_start.as<int>(), units, Pb,
+                                  cap, w.lb_rest.as<float>(), s),
+                    "other balls");
+        }
+        // (query, unit) pairs of this phase -> per-unit query lists -> tasks
         const int base = tbase * kQueriesPerWave;
-        KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nb * 4, s), "memset");
-        KNN_TRY(idx_pair_count(w.chosen.as<int>(), Q, P, pr[ph], pr[ph + 1], w.pcount.as<int>(), s),
-                "pairs");
-        KNN_TRY(idx_prefix(w.pcount.as<int>(), nb, kQueriesPerWave, w.qstart.as<int>(), nullptr, s),
+        KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nu * 4, s), "memset");
+        KNN_TRY(idx_pair_count(units, Q, Pb, pr[ph], pr[ph + 1], w.pcount.as<int>(), s), "pairs");
+        KNN_TRY(idx_prefix(w.pcount.as<int>(), nu, kQueriesPerWave, w.qstart.as<int>(), nullptr, s),
                 "prefix");
-        KNN_TRY(hipMemcpyAsync(w.cursor.p, w.qstart.p, (size_t)nb * 4, hipMemcpyDeviceToDevice, s),
+        KNN_TRY(hipMemcpyAsync(w.cursor.p, w.qstart.p, (size_t)nu * 4, hipMemcpyDeviceToDevice, s),
                 "copy");
-        KNN_TRY(idx_pair_scatter(w.chosen.as<int>(), Q, P, pr[ph], pr[ph + 1], w.cursor.as<int>(),
-                                 w.qlist.as<int>() + base, w.rows.as<int>(), base, s),
+        KNN_TRY(idx_pair_scatter(units, Q, Pb, pr[ph], pr[ph + 1], w.cursor.as<int>(), w.qlist.as<int>() + base,
+                                 w.rows.as<int>(), base, s),
                 "pairs");
-        KNN_TRY(idx_make_tasks(w.qstart.as<int>(), w.pcount.as<int>(), nb, ix->block_start.as<int>(),
-                               ix->block_count.as<int>(), w.tasks.as<ScanTask>() + tbase, max_t[ph],
-                               s),
+        KNN_TRY(idx_make_tasks(w.qstart.as<int>(), w.pcount.as<int>(), nu, unit_start, unit_count,
+                               w.tasks.as<ScanTask>() + tbase, max_t[ph], s),
                 "tasks");
         ScanArgs sa{};
         sa.packed = ix->packed.as<float>();
@@ -673,13 +791,12 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         sa.qlist = w.qlist.as<int>() + base;
         sa.query_pm = w.qpm.as<float>();
         sa.query_pm_ld = pm_ld;
-        const int waves = (double)Q * (pr[ph + 1] - pr[ph]) / nb >= kOneWaveQueries ? 1 : 4;
+        const int waves = (double)Q * (pr[ph + 1] - pr[ph]) / nu >= kOneWaveQueries || balls ? 1 : 4;
         if (nch_res) {
             sa.packed_res = ix->res_tiles.as<unsigned short>();
             sa.res_norm = ix->res_norm.as<float>();
             sa.res_cent = ix->res_cent.as<float>();
             sa.res_rho = ix->res_rho.as<float>();
-            sa.tile_block = ix->tile_block.as<int>();
             sa.res_row = w.resrow.as<float>() + (size_t)base * 4;
             sa.res_eta_coef = res_eta_coeff(dim, nch_res);
             sa.res_norm_up = res_norm_up(dim);
@@ -711,7 +828,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     fa.dim = dim;
     fa.k = k;
     fa.kp = kp;
-    fa.nrows = P;
+    fa.nrows = Pb;
     fa.rows = w.rows.as<int>();
     fa.perm = ix->perm.as<int>();
     fa.maxc = std::min(kMaxCand, std::max(2 * kp, 256));
@@ -795,6 +912,8 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     if (depth == 0) {
         ix->last_ref_bytes = nch_res ? 32ll * nch_res : nch_bf ? 64ll * nch_bf : 4ll * ix->dpad;
         ix->last_prec = nch_res ? 2 : nch_bf ? 1 : 0;
+        ix->last_balls = balls;
+        ix->last_rows = Pb;
     }
     g_last_fb = w.fb_count.as<int>();
     g_last_nonfinite = nullptr;
@@ -994,7 +1113,7 @@ int knnhip_index_last_work(const knnhip_index *ix, long long *exec_pairs, long l
             if (t.qcount <= 0) continue;
             const long long steps = (t.ref_end - t.ref_begin + kRefsPerStep - 1) / kRefsPerStep;
             ex += steps * kRefsPerStep * kQueriesPerWave;
-            pr += (long long)(t.ref_end - t.ref_begin) * t.qcount;
+            pr += (long long)(t.ref_end - t.ref_lo) * t.qcount;
             tb += steps * kRefsPerStep * ix->last_ref_bytes;
         }
     }
@@ -1002,6 +1121,17 @@ int knnhip_index_last_work(const knnhip_index *ix, long long *exec_pairs, long l
     if (pairs) *pairs = pr;
     if (tile_bytes) *tile_bytes = tb;
     if (precision) *precision = ix->last_prec;
+    return 1;
+}
+
+int knnhip_index_last_units(const knnhip_index *ix, int *balls, int *rows_per_query, int *nballs) {
+    if (!ix) {
+        set_error("null index");
+        return 0;
+    }
+    if (balls) *balls = ix->last_balls ? 1 : 0;
+    if (rows_per_query) *rows_per_query = ix->last_rows;
+    if (nballs) *nballs = ix->ball_ranges ? ix->nf : 0;
     return 1;
 }
 

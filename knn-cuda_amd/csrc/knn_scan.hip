@@ -68,12 +68,14 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
     // leave as output rows task_row .. task_row+31.
     const int t = blockIdx.x;
     int r_begin, r_end, nq, task_row;
+    int r_lo = 0;  // rows below r_lo (a task's tile-rounded start) take no part
     const int *qids = nullptr;  // query ids of the task (nullptr: consecutive from qfirst)
     int qfirst = 0;
     if (a.tasks) {
         const ScanTask tk = a.tasks[t];
         r_begin = tk.ref_begin;
         r_end = tk.ref_end;
+        r_lo = tk.ref_lo;
         nq = tk.qcount;
         qids = a.qlist + tk.qoff;
         task_row = t * kQueriesPerWave;
@@ -340,13 +342,15 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
         // appended to the wave's candidate queue; the queue is drained into
         // the heaps by all lanes at once when it fills.
         theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
-        if (r0 + kRefsPerStep > r_end) {
-            // rows past the task take no part below
+        if (r0 + kRefsPerStep > r_end || r0 < r_lo) {
+            // rows outside the task take no part below
 #pragma unroll
             for (int mt = 0; mt < kMTiles; ++mt)
 #pragma unroll
-                for (int i = 0; i < 16; ++i)
-                    if (r0 + kRefsPerTile * mt + acc_row(i, h) >= r_end) acc[mt][i] = INFINITY;
+                for (int i = 0; i < 16; ++i) {
+                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                    if (r >= r_end || r < r_lo) acc[mt][i] = INFINITY;
+                }
         }
         // fast path: the step's smallest key against theta (a min3 chain);
         // the candidates are counted only on steps that have one

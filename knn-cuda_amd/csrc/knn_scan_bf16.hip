@@ -587,13 +587,15 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
         // Epilogue (as the FP32 scan): keys below the freshest threshold go
         // to the wave's queue, drained into the shared heaps when it fills.
         theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
-        if (r0 + kRefsPerStep > r_end) {
-            // rows past the task take no part below
+        if (r0 + kRefsPerStep > r_end || r0 < tk.ref_lo) {
+            // rows outside the task take no part below
 #pragma unroll
             for (int mt = 0; mt < kMTiles; ++mt)
 #pragma unroll
-                for (int i = 0; i < 16; ++i)
-                    if (r0 + kRefsPerTile * mt + acc_row(i, h) >= r_end) acc[mt][i] = INFINITY;
+                for (int i = 0; i < 16; ++i) {
+                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                    if (r >= r_end || r < tk.ref_lo) acc[mt][i] = INFINITY;
+                }
         }
         // the step's smallest key against theta first; candidates are counted
         // only on steps that have one

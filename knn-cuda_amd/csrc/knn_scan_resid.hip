@@ -43,15 +43,15 @@ constexpr float kResDelta = 0x1p-8f + 0x1p-23f;
 
 __device__ __forceinline__ unsigned short bf_bits(__bf16 x) { return __builtin_bit_cast(unsigned short, x); }
 
-// One workgroup per block: its centroid (mean in FP64), the residual tiles
-// and squared norms of its positions (padding positions of its last tile get
-// zeros), its radius, and the block of each of its tiles.
+// One workgroup per unit (a block, or a ball: a contiguous range of
+// positions): its centroid (mean in FP64), the residual tiles and squared
+// norms of its positions, and its radius.  Positions outside every unit keep
+// the zeros the tiles were cleared to.
 __global__ __launch_bounds__(256) void resid_build_kernel(const float *__restrict__ pm, int ld, int dim, int nch,
                                                           const int *__restrict__ block_start,
                                                           const int *__restrict__ block_count,
                                                           float *__restrict__ cent, unsigned short *__restrict__ tiles,
-                                                          float *__restrict__ rnorm, float *__restrict__ rho,
-                                                          int *__restrict__ tile_block) {
+                                                          float *__restrict__ rnorm, float *__restrict__ rho) {
     __shared__ float c_s[kResMaxDim];
     __shared__ double red[256];
     const int b = blockIdx.x, t = threadIdx.x;
@@ -65,13 +65,12 @@ __global__ __launch_bounds__(256) void resid_build_kernel(const float *__restric
         if (d < kResMaxDim) c_s[d] = c;
     }
     __syncthreads();
-    const int npos = (n + kRefsPerTile - 1) / kRefsPerTile * kRefsPerTile;
     double emax = 0.0;
-    for (int j = t; j < npos; j += blockDim.x) {
+    for (int j = t; j < n; j += blockDim.x) {
         const long long p = p0 + j;
         const long long tile = p / kRefsPerTile;
         const int m = (int)(p % kRefsPerTile);
-        const bool member = j < n;
+        const bool member = true;
         const float *row = pm + p * ld;
         float nr = 0.0f;
         double ex = 0.0;
@@ -100,9 +99,8 @@ __global__ __launch_bounds__(256) void resid_build_kernel(const float *__restric
                 pk.w = v[6] | ((unsigned)v[7] << 16);
                 *reinterpret_cast<uint4 *>(tiles + ((tile * nch + cc) * kWave + m + 32 * h) * 8) = pk;
             }
-        rnorm[p] = member ? nr : 0.0f;
+        rnorm[p] = nr;
         emax = fmax(emax, ex);
-        if (m == 0) tile_block[tile] = b;
     }
     red[t] = emax;
     __syncthreads();
@@ -176,7 +174,7 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
     // B operands: -2 e_q, e_q = bf16(q - c) of the task's block, lane
     // (col, h) holding dimensions 16c + 8h .. +7 of chunk c; n_q over both
     // lane halves
-    const int blk = a.tile_block[r_begin / kRefsPerTile];
+    const int blk = tk.unit;  // the task's block (or ball): its centroid and radius
     const float *cen = a.res_cent + (long long)blk * a.query_pm_ld;
     const float *qrow = a.query_pm + (long long)q * a.query_pm_ld;
     bf16x8_t bq[NCH];
@@ -298,12 +296,15 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
         // Epilogue (as the three-term task scan): keys below the freshest
         // threshold go to the wave's queue, drained into the heaps.
         theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
-        if (r0 + kRefsPerStep > r_end) {
+        if (r0 + kRefsPerStep > r_end || r0 < tk.ref_lo) {
+            // rows outside the task take no part below
 #pragma unroll
             for (int mt = 0; mt < kMTiles; ++mt)
 #pragma unroll
-                for (int i = 0; i < 16; ++i)
-                    if (r0 + kRefsPerTile * mt + acc_row(i, h) >= r_end) acc[mt][i] = INFINITY;
+                for (int i = 0; i < 16; ++i) {
+                    const int r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                    if (r >= r_end || r < tk.ref_lo) acc[mt][i] = INFINITY;
+                }
         }
         float kmin = INFINITY;
 #pragma unroll
@@ -413,19 +414,18 @@ float res_norm_dn(int dim) {
     return (float)(1.0 / (1.0 + gd)) * (1.0f - 0x1p-22f);
 }
 
-hipError_t launch_resid_build(const float *pm, int ld, int dim, int nch, const int *block_start,
-                              const int *block_count, int nb, float *cent, unsigned short *tiles, float *rnorm,
-                              float *rho, int *tile_block, hipStream_t s) {
+hipError_t launch_resid_build(const float *pm, int ld, int dim, int nch, const int *unit_start,
+                              const int *unit_count, int nunits, float *cent, unsigned short *tiles, float *rnorm,
+                              float *rho, hipStream_t s) {
     if (dim > kResMaxDim || 16 * nch < dim || ld % 4) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(resid_build_kernel, dim3(nb), dim3(256), 0, s, pm, ld, dim, nch, block_start, block_count,
-                       cent, tiles, rnorm, rho, tile_block);
+    hipLaunchKernelGGL(resid_build_kernel, dim3(nunits), dim3(256), 0, s, pm, ld, dim, nch, unit_start, unit_count,
+                       cent, tiles, rnorm, rho);
     return hipGetLastError();
 }
 
 hipError_t launch_scan_res_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
                                 int tasks_per_group) {
-    if (!a.tasks || !a.packed_res || !a.res_norm || !a.res_cent || !a.res_rho || !a.tile_block || !a.res_row ||
-        !a.query_pm)
+    if (!a.tasks || !a.packed_res || !a.res_norm || !a.res_cent || !a.res_rho || !a.res_row || !a.query_pm)
         return hipErrorInvalidValue;
     if (tasks_per_group > 1 && (waves != 1 || tasks_per_group > 4)) return hipErrorInvalidValue;
     const size_t lds_ind = (size_t)tasks_per_group * (scan_heap_bytes(a.kp) + kQueueBytes);

@@ -71,6 +71,7 @@ C_SYMBOLS = (
     "knnhip_stream_chunks",
     "knnhip_version",
     "knnhip_index_last_work",
+    "knnhip_index_last_units",
     "knnhip_search_device_noalloc",
 )
 
@@ -97,7 +98,8 @@ class Options(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("block_queries", ctypes.c_int),
         ("score_all", ctypes.c_int),
-        ("reserved", ctypes.c_int * 5),
+        ("whole_blocks", ctypes.c_int),
+        ("reserved", ctypes.c_int * 4),
     ]
 
 
@@ -165,6 +167,8 @@ def load() -> ctypes.CDLL:
     lib.knnhip_index_last_work.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 3 + [
         ctypes.POINTER(ctypes.c_int)]
     lib.knnhip_index_last_work.restype = ctypes.c_int
+    lib.knnhip_index_last_units.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3
+    lib.knnhip_index_last_units.restype = ctypes.c_int
     lib.knnhip_index_layout.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]
     lib.knnhip_index_layout.restype = ctypes.c_int
@@ -480,6 +484,14 @@ class BlockIndex:
         return {"exec_pairs": ex.value, "pairs": pr.value, "tile_bytes": tb.value,
                 "precision": {0: "fp32", 1: "bf16x3", 2: "bf16-residual"}.get(prec.value, "?")}
 
+    def last_units(self) -> dict:
+        """Scan units of the last search: balls or whole blocks, heap rows per query, and the
+        index's ball count (0 when it does not know its balls' position ranges)."""
+        b, rows, nballs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        load().knnhip_index_last_units(self._h, ctypes.byref(b), ctypes.byref(rows), ctypes.byref(nballs))
+        return {"units": "balls" if b.value else "blocks", "rows_per_query": rows.value,
+                "balls": nballs.value}
+
     def stage_ms(self) -> dict:
         buf = (ctypes.c_float * 7)()
         load().knnhip_index_stage_ms(self._h, buf, 7)
@@ -543,10 +555,11 @@ class BlockIndex:
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
                margin: int = 0, force_fallback: bool = False, timing: bool = False,
                precision: str = "fp32", score_all: bool = False, router=None,
-               stream: Optional[int] = None):
+               whole_blocks: bool = False, stream: Optional[int] = None):
         """Exact k-NN through the index.  ``scores`` (nblocks x query_nb) or ``router`` (a Router
         of this index's shape, its first probe taken from the fused inference kernel) choose each
-        query's first probe; the lower bounds rank the rest."""
+        query's first probe; the lower bounds rank the rest.  ``whole_blocks`` scans every probed
+        block whole instead of only its balls that may hold a neighbour."""
         import torch
 
         lib = load()
@@ -577,6 +590,7 @@ class BlockIndex:
         opt.timing = 1 if timing else 0
         opt.precision = PRECISIONS[precision]
         opt.score_all = 1 if score_all else 0
+        opt.whole_blocks = 1 if whole_blocks else 0
         if stream is None:
             stream = torch.cuda.current_stream(query.device).cuda_stream
         if router is not None:

@@ -1,0 +1,150 @@
+"""GPU tests of ball probes in block-index searches: the scan visits the nearest ball of each
+query's first block, then only the balls of its probed blocks whose lower bound lies below the
+threshold that first scan left.  Every search equals the oracle bit for bit, and equals the
+whole-block search; index files of version 2 keep the balls' position ranges, version-1 files
+still load (whole-block searches)."""
+import numpy as np
+import pytest
+
+import knnhip
+import oracle
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def check(d, i, gd, gi, what):
+    d, i = d.cpu().numpy(), i.cpu().numpy()
+    assert np.array_equal(i, gi), f"{what}: {int((i != gi).sum())} index mismatches"
+    assert np.array_equal(bits(d), bits(gd)), f"{what}: distance mismatches"
+
+
+def say(m):
+    print(m, flush=True)
+
+
+@pytest.fixture(scope="module")
+def clustered():
+    R, Q, D, k = 40000, 700, 48, 10
+    ref = knnhip.generate_mixture(R, D, 80, 0.1, 61, 62)
+    qry = knnhip.generate_mixture(Q, D, 80, 0.1, 61, 63)
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    ix = knnhip.BlockIndex(ref, nblocks=64, iters=6)
+    yield ref, qry, k, gd, gi, ix
+    ix.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("probes", [1, 3, 8])
+def test_ball_probes_equal_oracle(clustered, precision, probes):
+    ref, qry, k, gd, gi, ix = clustered
+    say(f"balls precision={precision} probes={probes}")
+    d, i = ix.search(qry, k, probes=probes, precision=precision, timing=True)
+    torch.cuda.synchronize()
+    u = ix.last_units()
+    say(f"units {u} fallbacks {knnhip.last_fallback_count()} work {ix.last_work()}")
+    assert u["units"] == "balls" and u["balls"] > 64
+    check(d, i, gd, gi, f"balls {precision} P={probes}")
+    d2, i2 = ix.search(qry, k, probes=probes, precision=precision, whole_blocks=True)
+    torch.cuda.synchronize()
+    assert ix.last_units()["units"] == "blocks"
+    check(d2, i2, gd, gi, f"blocks {precision} P={probes}")
+
+
+def test_ball_probes_scan_fewer_pairs(clustered):
+    ref, qry, k, gd, gi, ix = clustered
+    ix.search(qry, k, probes=4, timing=True)
+    wb = ix.last_work()
+    ix.search(qry, k, probes=4, timing=True, whole_blocks=True)
+    wk = ix.last_work()
+    say(f"balls {wb} blocks {wk}")
+    assert wb["pairs"] < wk["pairs"]
+
+
+def test_ball_probes_with_router_scores_and_ties():
+    # duplicated points (exact ties across balls) and random first-probe scores
+    rng = np.random.default_rng(8)
+    base = np.round(4 * rng.random((12, 900))).astype(np.float32)
+    ref = torch.from_numpy(np.concatenate([base, base], axis=1)).cuda()
+    qry = torch.from_numpy(np.round(4 * rng.random((12, 150))).astype(np.float32)).cuda()
+    k = 9
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
+    ix = knnhip.BlockIndex(ref, nblocks=16, iters=3)
+    scores = torch.rand((16, qry.shape[1]), device="cuda")
+    for prec in ("fp32", "bf16"):
+        check(*ix.search(qry, k, probes=3, scores=scores, precision=prec), gd, gi, f"scores {prec}")
+        check(*ix.search(qry, k, probes=2, precision=prec), gd, gi, f"ties {prec}")
+    ix.close()
+
+
+def _strip_to_v1(raw):
+    """The version-1 form of a version-2 index file: version word 1, ball ranges dropped."""
+    hdr = np.frombuffer(bytes(raw[12:36]), dtype=np.int32)
+    nf = int(hdr[3])
+    assert np.frombuffer(bytes(raw[8:12]), dtype=np.uint32)[0] == 2
+    v1 = bytearray(raw[: len(raw) - 8 * nf])
+    v1[8:12] = np.array([1], dtype=np.uint32).tobytes()
+    return bytes(v1)
+
+
+def test_index_file_versions(clustered, tmp_path):
+    ref, qry, k, gd, gi, ix = clustered
+    path = tmp_path / "ix2.bin"
+    ix.save(str(path))
+    raw = path.read_bytes()
+    ix2 = knnhip.BlockIndex.load(str(path), ref)
+    check(*ix2.search(qry, k, probes=3, precision="bf16"), gd, gi, "v2 loaded")
+    assert ix2.last_units()["units"] == "balls"
+    # version 1: no ball ranges, whole-block searches, radii refitted to the nearest ball
+    p1 = tmp_path / "ix1.bin"
+    p1.write_bytes(_strip_to_v1(raw))
+    ix1 = knnhip.BlockIndex.load(str(p1), ref)
+    check(*ix1.search(qry, k, probes=3), gd, gi, "v1 loaded")
+    u = ix1.last_units()
+    assert u["units"] == "blocks" and u["balls"] == 0
+    # a v1 index saves as version 1
+    p1b = tmp_path / "ix1b.bin"
+    ix1.save(str(p1b))
+    assert p1b.read_bytes()[8:12] == np.array([1], dtype=np.uint32).tobytes()
+    ix1.close()
+    ix2.close()
+
+
+def test_inconsistent_ball_ranges_are_refused(clustered, tmp_path):
+    ref, qry, k, gd, gi, ix = clustered
+    path = tmp_path / "ix.bin"
+    ix.save(str(path))
+    raw = bytearray(path.read_bytes())
+    nf = int(np.frombuffer(bytes(raw[12:36]), dtype=np.int32)[3])
+    cnt_off = len(raw) - 4 * nf
+    cnt = np.frombuffer(bytes(raw[cnt_off:]), dtype=np.int32).copy()
+    j = int(np.argmax(cnt))
+    cnt[j] -= 1  # the block's ranges no longer cover it
+    raw[cnt_off:] = cnt.tobytes()
+    path.write_bytes(bytes(raw))
+    with pytest.raises(knnhip.KnnError, match="inconsistent index"):
+        knnhip.BlockIndex.load(str(path), ref)
+
+
+def test_ball_probes_at_scale_equal_plain():
+    # 262,144 refs (512 clusters, dim 96), 2048 queries, k 16: ball searches == plain search
+    R, Q, D, k = 1 << 18, 2048, 96, 16
+    say("balls vs plain at 262144 x 2048 x 96")
+    ref = knnhip.generate_mixture(R, D, 512, 0.1, 71, 72)
+    qry = knnhip.generate_mixture(Q, D, 512, 0.1, 71, 73)
+    pd, pi = knnhip.search_device(ref, qry, k)
+    ix = knnhip.BlockIndex(ref, nblocks=128, iters=8)
+    for prec in ("fp32", "bf16x3", "bf16"):
+        d, i = ix.search(qry, k, probes=5, precision=prec, timing=True)
+        torch.cuda.synchronize()
+        say(f"{prec}: units {ix.last_units()} fallbacks {knnhip.last_fallback_count()} "
+            f"work {ix.last_work()} stages {ix.stage_ms()}")
+        assert torch.equal(i, pi) and torch.equal(d, pd), prec
+    sample = [0, 7, 1000, 2047]
+    gd, gi = oracle.knn_c(ref.cpu().numpy(), np.ascontiguousarray(qry.cpu().numpy()[:, sample]), k)
+    check(d[:, sample], i[:, sample], gd, gi, "oracle sample")
+    ix.close()

@@ -11,6 +11,8 @@
 """
 import ctypes
 import os
+import sys
+import threading
 
 import numpy as np
 import pytest
@@ -42,16 +44,46 @@ def test_reference_test_program_passes_against_library(capfd):
     # GPU entry points through their C++ symbols, which the loader resolves to this library.
     # (In this process rather than as the oracle/_ref/test_ref child: a GPU process starts no
     # other program here.)
+    # Its lines are shown as it prints them (each of its steps can take a minute on a slow host)
+    # and kept for the checks: C stdout goes line-buffered into a pipe that a thread copies to
+    # the terminal.
     say("running the reference's test program (16384 x 4096, dim 128, k 16)")
     lib = ctypes.CDLL(REF_LIB)
     main = lib._Z23knn_reference_test_mainv
     main.restype = ctypes.c_int
     main.argtypes = []
-    rc = main()
-    ctypes.CDLL(None).fflush(None)
-    stdout = capfd.readouterr().out
+    libc = ctypes.CDLL(None)
+    libc.setvbuf.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_size_t]
+    c_stdout = ctypes.c_void_p.in_dll(libc, "stdout")
+    chunks = []
     with capfd.disabled():
-        say(stdout[-2000:])
+        libc.fflush(None)
+        sys.stdout.flush()
+        saved = os.dup(1)
+        rd, wr = os.pipe()
+        os.dup2(wr, 1)
+        os.close(wr)
+        libc.setvbuf(c_stdout, None, 1, 0)  # _IOLBF: flush at each newline
+
+        def pump():
+            while True:
+                b = os.read(rd, 4096)
+                if not b:
+                    break
+                chunks.append(b)
+                os.write(saved, b)
+
+        t = threading.Thread(target=pump, daemon=True)
+        t.start()
+        try:
+            rc = main()
+        finally:
+            libc.fflush(None)
+            os.dup2(saved, 1)  # closes the pipe's last write end: the pump reads EOF
+            t.join(timeout=30)
+            os.close(rd)
+            os.close(saved)
+    stdout = b"".join(chunks).decode(errors="replace")
     assert rc == 0, stdout[-2000:]
     lines = [l for l in stdout.splitlines() if l.startswith("- ")]
     names = ["knn_c", "knn_cuda_global", "knn_cuda_texture", "knn_cublas"]
