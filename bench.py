@@ -132,30 +132,34 @@ def headline(args, knnhip, torch):
     def same(d, i):
         return bool(torch.equal(i, pi) and torch.equal(d, pd))
 
-    # candidate forms: bfloat16 products (three-term, or single-pass on block residuals) with the
-    # router's first probe; the fastest exact one is the headline
+    # candidate forms: bfloat16 products (three-term, or single-pass on centre residuals) with the
+    # router's first probe, the scan visiting balls (one pass, the default; or two passes) or whole
+    # blocks; the fastest exact one is the headline
     forms = []
-    for prec in ("bf16x3", "bf16"):
-        for P in (4, 5, 6):
-            ms, (d, i) = timed(torch, lambda: ix.search(qry, k, probes=P, router=mlp, precision=prec),
-                               args.router_reps)
-            forms.append({"precision": prec, "probes": P, "ms": ms, "equal_to_plain": same(d, i),
-                          "certificate_failures": knnhip.last_fallback_count()})
+    for units in ("balls", "balls-threshold", "blocks"):
+        for prec in ("bf16x3", "bf16"):
+            for P in ((4, 6, 8, 12) if units != "blocks" else (4, 5, 6)):
+                ms, (d, i) = timed(torch, lambda: ix.search(qry, k, probes=P, router=mlp, precision=prec,
+                                                            units=units), args.router_reps)
+                forms.append({"units": units, "precision": prec, "probes": P,
+                              "ms": ms, "equal_to_plain": same(d, i),
+                              "certificate_failures": knnhip.last_fallback_count(),
+                              "rows_per_query": ix.last_units()["rows_per_query"]})
     best = min((f for f in forms if f["equal_to_plain"]), key=lambda f: f["ms"])
-    prec, P = best["precision"], best["probes"]
+    prec, P, units = best["precision"], best["probes"], best["units"]
 
     dist = torch.empty((k, Q), dtype=torch.float32, device=qry.device)
     idx = torch.empty((k, Q), dtype=torch.int32, device=qry.device)
     stream = torch.cuda.current_stream(qry.device)
     for _ in range(args.warmup):
-        ix.search(qry, k, probes=P, router=mlp, precision=prec, dist=dist, index=idx,
+        ix.search(qry, k, probes=P, router=mlp, precision=prec, dist=dist, index=idx, units=units,
                   stream=stream.cuda_stream)
     torch.cuda.synchronize()
     scan_ms, stages, works = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ix.search(qry, k, probes=P, router=mlp, precision=prec, dist=dist, index=idx, timing=True,
-                  stream=stream.cuda_stream)
+                  units=units, stream=stream.cuda_stream)
         st = ix.stage_ms()
         stages.append(st)
         scan_ms.append(st["scan"])
@@ -176,8 +180,10 @@ def headline(args, knnhip, torch):
     table = None
     if not args.no_router_table:
         table = router_table(args, knnhip, torch, ix, mlp, qry, k, pd, pi, plain_ms)
+    lu = ix.last_units()
     out = {
         "qps": Q / (elapsed / args.steps), "ms_per_step": ms_per_step, "precision": prec, "probes": P,
+        "units": units, "rows_per_query": lu["rows_per_query"], "balls": lu["balls"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": load_traffic(ROUTER_TRAFFIC_FILE, [R, Q, D, k]),
                      "kernel": "scan_res_task_kernel" if work["precision"] == "bf16-residual"
@@ -203,18 +209,21 @@ def router_table(args, knnhip, torch, ix, mlp, qry, k, pd, pi, plain_ms):
     router's scores, and the fused router (its first probe)."""
     Q = qry.shape[1]
     rows = []
-    for prec in ("fp32", "bf16x3", "bf16"):
-        for P in (4, 6, 8):
-            ms, (d, i) = timed(torch, lambda: ix.search(qry, k, probes=P, precision=prec), args.router_reps)
-            rows.append({"ranking": "bounds", "precision": prec, "probes": P, "ms": ms,
-                         "speedup_vs_plain": plain_ms / ms,
-                         "certificate_failures": knnhip.last_fallback_count(),
-                         "equal_to_plain": bool(torch.equal(i, pi) and torch.equal(d, pd))})
-    P = 5
+    for units in ("balls", "balls-threshold", "blocks"):
+        for prec in ("fp32", "bf16x3", "bf16"):
+            for P in (4, 6, 8):
+                ms, (d, i) = timed(torch, lambda: ix.search(qry, k, probes=P, precision=prec, units=units),
+                                   args.router_reps)
+                rows.append({"ranking": "bounds", "units": units, "precision": prec,
+                             "probes": P, "ms": ms, "speedup_vs_plain": plain_ms / ms,
+                             "certificate_failures": knnhip.last_fallback_count(),
+                             "equal_to_plain": bool(torch.equal(i, pi) and torch.equal(d, pd))})
+    P = 6
     for prec, mode in (("bf16x3", "first"), ("bf16x3", "all"), ("bf16", "first")):
         ms, (d, i) = timed(torch, lambda: ix.search(qry, k, probes=P, scores=mlp.scores(qry), precision=prec,
                                                     score_all=mode == "all"), args.router_reps)
-        rows.append({"ranking": f"router scores ({mode})", "precision": prec, "probes": P, "ms": ms,
+        rows.append({"ranking": f"router scores ({mode})", "units": "balls", "precision": prec, "probes": P,
+                     "ms": ms,
                      "speedup_vs_plain": plain_ms / ms, "certificate_failures": knnhip.last_fallback_count(),
                      "equal_to_plain": bool(torch.equal(i, pi) and torch.equal(d, pd))})
     scores_ms, _ = timed(torch, lambda: mlp.scores(qry), args.router_reps)
@@ -604,7 +613,8 @@ def main():
                                "FP32 re-rank and the MLP router's first probe",
                    "ref_nb": R, "query_nb": Q, "dim": D, "k": k,
                    "mixture": "4,096 Gaussians, sigma 0.1, centres U[-1,1]^96", "blocks": C5["nblocks"],
-                   "probes": head["probes"], "precision": head["precision"],
+                   "probes": head["probes"], "precision": head["precision"], "scan_units": head["units"],
+                   "balls": head["balls"], "rows_per_query": head["rows_per_query"],
                    "router": [D, 256, 256, C5["nblocks"]], "layout": "dim x n row-major", "parallelism": "1 GPU"},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,

@@ -77,9 +77,12 @@ typedef struct knnhip_options {
     int score_all;      /* index searches given scores: 0 the scores choose each query's
                            first probe (scanned first), the lower bounds rank the others;
                            1 the scores rank every probe */
-    int whole_blocks;   /* index searches: 0 the scan visits the balls of each probed block
-                           that may hold a neighbour (indexes that know their balls' position
-                           ranges: built ones and version-2 files), 1 whole probed blocks */
+    int probe_units;    /* index searches, what the scan visits in the probed blocks (indexes
+                           that know their balls' position ranges: built ones and version-2
+                           files; others always take 1): 0 the balls that may hold a neighbour,
+                           chosen in one pass from the balls' reach; 1 whole blocks; 2 balls in
+                           two passes (the first block's nearest ball, then the balls below the
+                           threshold it left) */
     int reserved[4];
 } knnhip_options;
 

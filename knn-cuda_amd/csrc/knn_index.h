@@ -72,6 +72,14 @@ hipError_t idx_ball_rest(const float *qpm, int pm_ld, int dim, int Q, const floa
                          const float *fcent, const double *frad, const int *fb_start, int *out, int Pb, int cap,
                          float *lb_rest, hipStream_t s);
 int idx_ball_cand_max();
+// Every ball of a one-pass search at once: out[q * Pb ..] the balls of
+// query q's P blocks whose bound is below the reach of the nearest balls
+// holding kp members (fcnt: members per ball), nearest first, the others'
+// bounds folded into lb_rest.  ncand: most candidate balls of a query (P
+// times the most balls of a block), at most idx_ball_cand_max().
+hipError_t idx_ball_select(const float *qpm, int pm_ld, int dim, int Q, int kp, const int *chosen, int P,
+                           const float *fcent, const double *frad, const int *fcnt, const int *fb_start, int *out,
+                           int Pb, int ncand, float *lb_rest, hipStream_t s);
 // Radii of balls with known position ranges (largest member distance, FP64;
 // -1 for an empty ball).
 hipError_t idx_ball_range_radius(const float *pm, int ld, int dim, const float *fcent, const int *ball_pos,

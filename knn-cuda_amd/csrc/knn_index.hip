@@ -480,29 +480,44 @@ __global__ __launch_bounds__(64 * kPickWaves) void router_first_kernel(
 // so the threshold only steers the work: exactness rests on the certificate.
 // ---------------------------------------------------------------------------
 
-// FP64 distance between two FP32 rows (16-byte aligned): each difference is
-// exact in FP64 and each fused square-add rounds once, so the result is
-// within (dim + 2) * 2^-53 relative of the true distance.
-__device__ inline double row_dist64(const float *__restrict__ a, const float *__restrict__ b, int dim) {
-    double acc = 0.0;
-    int d = 0;
-    for (; d + 4 <= dim; d += 4) {
-        const float4 x = *reinterpret_cast<const float4 *>(a + d);
-        const float4 y = *reinterpret_cast<const float4 *>(b + d);
-        double t = (double)x.x - (double)y.x;
-        acc = fma(t, t, acc);
-        t = (double)x.y - (double)y.y;
-        acc = fma(t, t, acc);
-        t = (double)x.z - (double)y.z;
-        acc = fma(t, t, acc);
-        t = (double)x.w - (double)y.w;
-        acc = fma(t, t, acc);
+// Squared FP64 distances from a query row to the centre rows f[j] of up to
+// kDistGroup balls (f[j] < 0: none), wave-uniform f, the whole wave
+// cooperating: lane l takes dimensions 4l .. 4l + 3, then 4l + 256 .., so the
+// loads of one centre row are one coalesced read and the group's rows are in
+// flight together.  Each difference is exact in FP64 and each fused
+// square-add rounds once: every sum is within (dim + 8) * 2^-53 relative of
+// the true squared distance, whatever the order.  Every lane returns every
+// sum.  Rows are 16-byte aligned; dimensions at or past dim are not read
+// into the sums.
+constexpr int kDistGroup = 8;
+__device__ inline void wave_dist2_group(const float *__restrict__ q, const float *__restrict__ cent, int ld,
+                                        int dim, const int (&f)[kDistGroup], double (&out)[kDistGroup],
+                                        int lane) {
+#pragma unroll
+    for (int j = 0; j < kDistGroup; ++j) out[j] = 0.0;
+    for (int d = 4 * lane; d < dim; d += 4 * kWave) {
+        const float4 x = *reinterpret_cast<const float4 *>(q + d);
+        float4 y[kDistGroup];
+#pragma unroll
+        for (int j = 0; j < kDistGroup; ++j)
+            y[j] = f[j] >= 0 ? *reinterpret_cast<const float4 *>(cent + (long long)f[j] * ld + d) : x;
+        const bool w1 = d + 1 < dim, w2 = d + 2 < dim, w3 = d + 3 < dim;
+#pragma unroll
+        for (int j = 0; j < kDistGroup; ++j) {
+            double t = (double)x.x - (double)y[j].x;
+            out[j] = fma(t, t, out[j]);
+            t = w1 ? (double)x.y - (double)y[j].y : 0.0;
+            out[j] = fma(t, t, out[j]);
+            t = w2 ? (double)x.z - (double)y[j].z : 0.0;
+            out[j] = fma(t, t, out[j]);
+            t = w3 ? (double)x.w - (double)y[j].w : 0.0;
+            out[j] = fma(t, t, out[j]);
+        }
     }
-    for (; d < dim; ++d) {
-        const double t = (double)a[d] - (double)b[d];
-        acc = fma(t, t, acc);
-    }
-    return sqrt(acc);
+#pragma unroll
+    for (int j = 0; j < kDistGroup; ++j)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) out[j] += __shfl_xor(out[j], o);
 }
 
 // Lower bound, rounded down to FP32, on the distance from a query to every
@@ -528,21 +543,20 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_first_kernel(
     int bf = -1;
     if (b >= 0) {
         const float *qr = qpm + (long long)q * pm_ld;
-        for (int f = fb_start[b] + lane; f < fb_start[b + 1]; f += kWave) {
-            if (!(frad[f] >= 0.0)) continue;
-            const double d = row_dist64(qr, fcent + (long long)f * pm_ld, dim);
-            if (d < best || (d == best && f < bf)) {
-                best = d;
-                bf = f;
-            }
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        const double od = __shfl_xor(best, o);
-        const int of = __shfl_xor(bf, o);
-        if (of >= 0 && (bf < 0 || od < best || (od == best && of < bf))) {
-            best = od;
-            bf = of;
+        const int fl = fb_start[b], fh = fb_start[b + 1];
+        for (int base = fl; base < fh; base += kDistGroup) {
+            int f[kDistGroup];
+#pragma unroll
+            for (int j = 0; j < kDistGroup; ++j) f[j] = base + j < fh && frad[base + j] >= 0.0 ? base + j : -1;
+            double d2[kDistGroup];
+            wave_dist2_group(qr, fcent, pm_ld, dim, f, d2, lane);
+            // ascending balls: a strict test keeps the lower one on ties
+#pragma unroll
+            for (int j = 0; j < kDistGroup; ++j)
+                if (f[j] >= 0 && d2[j] < best) {
+                    best = d2[j];
+                    bf = f[j];
+                }
         }
     }
     if (lane == 0) out[(long long)q * Pb] = bf;
@@ -580,18 +594,22 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_rest_kernel(
         const int b = chosen[(long long)q * P + j];
         if (b < 0) continue;
         const int fl = fb_start[b], fh = fb_start[b + 1];
-        for (int f = fl + lane; f < fh; f += kWave) {
-            const double rho = frad[f];
-            int ff = -1;
-            float lb = INFINITY;
-            if (f != f0 && rho >= 0.0) {
-                lb = ball_member_lb(row_dist64(qr, fcent + (long long)f * pm_ld, dim), rho);
-                ff = f;
-            }
-            lbs[n + f - fl] = lb;
-            fs[n + f - fl] = ff;
-        }
+        for (int f = fl + lane; f < fh; f += kWave) fs[n + f - fl] = f != f0 && frad[f] >= 0.0 ? f : -1;
         n += fh - fl;
+    }
+    wave_sync();
+    // their bounds, kDistGroup at a time (lane j keeps candidate base + j's)
+    for (int base = 0; base < n; base += kDistGroup) {
+        int f[kDistGroup];
+#pragma unroll
+        for (int j = 0; j < kDistGroup; ++j) f[j] = base + j < n ? fs[base + j] : -1;
+        double d2[kDistGroup];
+        wave_dist2_group(qr, fcent, pm_ld, dim, f, d2, lane);
+        float lb = INFINITY;
+#pragma unroll
+        for (int j = 0; j < kDistGroup; ++j)
+            if (j == lane && f[j] >= 0) lb = ball_member_lb(sqrt(d2[j]), frad[f[j]]);
+        if (lane < kDistGroup && base + lane < n) lbs[base + lane] = lb;
     }
     wave_sync();
     double T;
@@ -628,6 +646,138 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_rest_kernel(
         if (!take) lbmin = fminf(lbmin, lb);
     }
     for (int i = min(npass, cap) + lane; i < Pb - 1; i += kWave) out[o0 + 1 + i] = -1;
+    for (int o = 32; o > 0; o >>= 1) lbmin = fminf(lbmin, __shfl_xor(lbmin, o));
+    if (lane == 0 && lbmin < lb_rest[q]) lb_rest[q] = lbmin;
+}
+
+// Every ball at once (one scan pass): for query q, the candidates are the
+// non-empty balls of blocks chosen[q * P .. q * P + P), with FP64 centre
+// distance d_f, radius rho_f, bound lb_f = d_f - rho_f and reach u_f = d_f +
+// rho_f (every member of f is within u_f).  T_q = the smallest u_i such that
+// the balls with u_j <= u_i hold at least kp members between them: those
+// members are kp references within T_q, so the kp-th nearest distance is at
+// most T_q and a ball with lb_f >= T_q holds no reference the scan would
+// keep.  The balls with lb_f < T_q, nearest bound first (ties: lower ball),
+// fill out[q * Pb ..] up to Pb entries (the rest -1); every other candidate
+// lowers lb_rest[q] to lb_f.  (No such T_q, fewer than kp members in all:
+// every candidate passes.)  At most kBallCand candidates.
+__global__ __launch_bounds__(64 * kBallWaves) void ball_select_kernel(
+    const float *__restrict__ qpm, int pm_ld, int dim, int Q, int kp, const int *__restrict__ chosen, int P,
+    const float *__restrict__ fcent, const double *__restrict__ frad, const int *__restrict__ fcnt,
+    const int *__restrict__ fb_start, int *__restrict__ out, int Pb, float *__restrict__ lb_rest, int ncap) {
+    // per wave: reach (FP64), bound, ball and member count of ncap candidates
+    extern __shared__ __align__(16) unsigned char sel_smem[];
+    __shared__ int blk_st[kBallWaves][kWave], blk_fl[kBallWaves][kWave];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = blockIdx.x * kBallWaves + wave;
+    if (q >= Q) return;
+    unsigned char *wbase = sel_smem + (size_t)wave * ncap * 20;
+    double *us = reinterpret_cast<double *>(wbase);
+    float *lbs = reinterpret_cast<float *>(wbase + (size_t)ncap * 8);
+    int *fs = reinterpret_cast<int *>(wbase + (size_t)ncap * 12);
+    int *ns = reinterpret_cast<int *>(wbase + (size_t)ncap * 16);
+    const long long o0 = (long long)q * Pb;
+    const float *qr = qpm + (long long)q * pm_ld;
+    // candidates, block by block: the blocks' ball ranges read by one lane
+    // each, placed by a prefix over the lanes; then one lane per candidate
+    int n = 0;
+    for (int j0 = 0; j0 < P; j0 += kWave) {
+        const int j = j0 + lane;
+        int fl = 0, cnt = 0;
+        if (j < P) {
+            const int b = chosen[(long long)q * P + j];
+            if (b >= 0) {
+                fl = fb_start[b];
+                cnt = fb_start[b + 1] - fl;
+            }
+        }
+        int pre = cnt;  // inclusive prefix over the lanes
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(pre, o);
+            if (lane >= o) pre += t;
+        }
+        const int total = __shfl(pre, kWave - 1);
+        const int nj = min(kWave, P - j0);
+        // each block's first candidate slot and first ball, read below by
+        // lanes of a divergent loop (so through LDS, not lane shuffles)
+        if (lane < nj) {
+            blk_st[wave][lane] = pre - cnt;
+            blk_fl[wave][lane] = fl;
+        }
+        wave_sync();
+        for (int i = lane; i < total; i += kWave) {
+            // the block holding candidate i: the last one whose slots start at or before i
+            int fb = 0, off = 0;
+            for (int t = 0; t < nj; ++t) {
+                const int st = blk_st[wave][t];
+                if (st <= i) {
+                    off = st;
+                    fb = blk_fl[wave][t];
+                }
+            }
+            const int f = fb + i - off;
+            const bool live = frad[f] >= 0.0;
+            fs[n + i] = live ? f : -1;
+            ns[n + i] = live ? fcnt[f] : 0;
+        }
+        wave_sync();  // the next chunk rewrites blk_st / blk_fl
+        n += total;
+    }
+    wave_sync();
+    for (int base = 0; base < n; base += kDistGroup) {
+        int f[kDistGroup];
+#pragma unroll
+        for (int j = 0; j < kDistGroup; ++j) f[j] = base + j < n ? fs[base + j] : -1;
+        double d2[kDistGroup];
+        wave_dist2_group(qr, fcent, pm_ld, dim, f, d2, lane);
+        float lb = INFINITY;
+        double u = INFINITY;
+#pragma unroll
+        for (int j = 0; j < kDistGroup; ++j)
+            if (j == lane && f[j] >= 0) {
+                const double d = sqrt(d2[j]), rho = frad[f[j]];
+                lb = ball_member_lb(d, rho);
+                u = d * (1.0 + 1e-12) + rho;
+            }
+        if (lane < kDistGroup && base + lane < n) {
+            lbs[base + lane] = lb;
+            us[base + lane] = u;
+        }
+    }
+    wave_sync();
+    // T_q over the lanes' candidates, then the smallest
+    double T = INFINITY;
+    for (int i = lane; i < n; i += kWave) {
+        if (fs[i] < 0) continue;
+        const double ui = us[i];
+        long long members = 0;
+        for (int t = 0; t < n; ++t) members += us[t] <= ui ? ns[t] : 0;
+        if (members >= kp) T = fmin(T, ui);
+    }
+    for (int o = 32; o > 0; o >>= 1) T = fmin(T, __shfl_xor(T, o));
+    int npass = 0;
+    for (int i = 0; i < n; ++i) npass += (fs[i] >= 0 && (double)lbs[i] < T) ? 1 : 0;
+    float lbmin = INFINITY;
+    for (int i = lane; i < n; i += kWave) {
+        const int f = fs[i];
+        if (f < 0) continue;
+        const float lb = lbs[i];
+        bool take = false;
+        if ((double)lb < T) {
+            int rank = 0;
+            for (int t = 0; t < n; ++t) {
+                const int g2 = fs[t];
+                const float l2 = lbs[t];
+                rank += (g2 >= 0 && (double)l2 < T && (l2 < lb || (l2 == lb && g2 < f))) ? 1 : 0;
+            }
+            if (rank < Pb) {
+                out[o0 + rank] = f;
+                take = true;
+            }
+        }
+        if (!take) lbmin = fminf(lbmin, lb);
+    }
+    for (int i = min(npass, Pb) + lane; i < Pb; i += kWave) out[o0 + i] = -1;
     for (int o = 32; o > 0; o >>= 1) lbmin = fminf(lbmin, __shfl_xor(lbmin, o));
     if (lane == 0 && lbmin < lb_rest[q]) lb_rest[q] = lbmin;
 }
@@ -1680,6 +1830,16 @@ hipError_t idx_ball_first(const float *qpm, int pm_ld, int dim, int Q, const int
     return hipGetLastError();
 }
 int idx_ball_cand_max() { return kBallCand; }
+hipError_t idx_ball_select(const float *qpm, int pm_ld, int dim, int Q, int kp, const int *chosen, int P,
+                           const float *fcent, const double *frad, const int *fcnt, const int *fb_start, int *out,
+                           int Pb, int ncand, float *lb_rest, hipStream_t s) {
+    const int ncap = (ncand + 7) / 8 * 8;
+    if (ncap < 8 || ncap > kBallCand) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ball_select_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves),
+                       (size_t)kBallWaves * ncap * 20, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt,
+                       fb_start, out, Pb, lb_rest, ncap);
+    return hipGetLastError();
+}
 hipError_t idx_ball_rest(const float *qpm, int pm_ld, int dim, int Q, const float *qnorm, const unsigned *theta,
                          const unsigned *rmax, double eq, double er, int res, const int *chosen, int P,
                          const float *fcent, const double *frad, const int *fb_start, int *out, int Pb, int cap,

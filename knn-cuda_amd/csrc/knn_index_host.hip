@@ -613,14 +613,23 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         set_error("k or probe count too large for the block index (k=%d probes=%d)", k, P);
         return false;
     }
+    if (o.probe_units < 0 || o.probe_units > 2) {
+        set_error("bad probe_units option (%d)", o.probe_units);
+        return false;
+    }
     P = std::max(1, std::min(P, nb));
-    // Ball probes (where the index knows its balls' position ranges): the
-    // scan visits the nearest ball of the first block, then up to `cap`
-    // more balls of the P blocks whose bounds lie below the threshold that
-    // scan left; Pb heap rows per query.  Whole blocks otherwise: P rows.
-    const bool balls = ix->ball_ranges && !o.whole_blocks && (long long)P * ix->max_bpb <= idx_ball_cand_max();
-    const int cap = balls ? std::max(0, std::min({P * ix->max_bpb - 1, kMaxUnion / kp - 1, kMaxBallRows - 1})) : 0;
-    const int Pb = balls ? 1 + cap : P;
+    // Ball probes (where the index knows its balls' position ranges), Pb heap
+    // rows per query.  One pass (probe_units 0): the balls of the P blocks
+    // whose bounds lie below the reach of the nearest balls holding kp
+    // members, at most Pb.  Two passes (2): the nearest ball of the first
+    // block, then up to `cap` more balls whose bounds lie below the threshold
+    // that scan left.  Whole blocks otherwise (1): P rows.
+    const bool balls =
+        ix->ball_ranges && o.probe_units != 1 && (long long)P * ix->max_bpb <= idx_ball_cand_max();
+    const bool one_pass = balls && o.probe_units == 0;
+    const int rows_max = std::min({P * ix->max_bpb, kMaxUnion / kp, kMaxBallRows});
+    const int cap = balls && !one_pass ? std::max(0, rows_max - 1) : 0;
+    const int Pb = one_pass ? std::max(1, rows_max) : balls ? 1 + cap : P;
     const int nu = balls ? ix->nf : nb;
     const int *unit_start = balls ? ix->ball_pos.as<int>() : ix->block_start.as<int>();
     const int *unit_count = balls ? ix->ball_cnt.as<int>() : ix->block_count.as<int>();
@@ -656,9 +665,9 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     }
     // Two scan phases: each query's first unit (its first-ranked block, or
     // that block's nearest ball), then its other units, which start from the
-    // thresholds the first phase left.
-    const int nphase = Pb > 1 ? 2 : 1;
-    const int pr[3] = {0, 1, Pb};
+    // thresholds the first phase left.  One-pass ball probes: one phase.
+    const int nphase = one_pass ? 1 : Pb > 1 ? 2 : 1;
+    const int pr[3] = {0, one_pass ? Pb : 1, Pb};
     int max_t[2] = {0, 0};
     for (int ph = 0; ph < nphase; ++ph)
         max_t[ph] = (int)(((long long)Q * (pr[ph + 1] - pr[ph]) + kQueriesPerWave - 1) /
@@ -725,7 +734,12 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
                             w.chosen.as<int>(), w.lb_rest.as<float>(), s),
             "pick blocks");
-    if (balls)
+    if (one_pass)
+        KNN_TRY(idx_ball_select(w.qpm.as<float>(), pm_ld, dim, Q, kp, w.chosen.as<int>(), P, ix->fcent.as<float>(),
+                                ix->frad.as<double>(), ix->ball_cnt.as<int>(), ix->fb_start.as<int>(),
+                                w.bchosen.as<int>(), Pb, P * ix->max_bpb, w.lb_rest.as<float>(), s),
+                "balls");
+    else if (balls)
         KNN_TRY(idx_ball_first(w.qpm.as<float>(), pm_ld, dim, Q, w.chosen.as<int>(), P, ix->fcent.as<float>(),
                                ix->frad.as<double>(), ix->fb_start.as<int>(), w.bchosen.as<int>(), Pb, s),
                 "first balls");

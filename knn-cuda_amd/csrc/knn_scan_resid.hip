@@ -347,7 +347,10 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
                     qcount += __popcll(m);
                 }
             }
-            if (qcount >= kQueueCap / 2) {
+            // a heap still short of kp entries (root +inf) lets every key of
+            // the next step through: fill it now
+            const bool young = __ballot(qvalid && theta == INFINITY) != 0;
+            if (qcount >= kQueueCap / 2 || (young && qcount > 0)) {
                 drain_queue(qv, qr, qc, qcount, hv, hi, lock, kps, kp, lane);
                 qcount = 0;
                 theta = fminf(theta, shared_root(myv));

@@ -80,6 +80,9 @@ VARIANTS = {"global": 0, "texture": 1, "cublas": 2}
 # products on block-centroid residuals (block-index searches; plain searches take the three-term
 # form).  Every form returns the same exact result.
 PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16": 2}
+# what an index search scans in its probed blocks: the balls that may hold a neighbour (chosen
+# in one pass), whole blocks, or balls chosen in two passes (below the first ball's threshold)
+UNITS = {"balls": 0, "blocks": 1, "balls-threshold": 2}
 
 
 class KnnError(RuntimeError):
@@ -98,7 +101,7 @@ class Options(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("block_queries", ctypes.c_int),
         ("score_all", ctypes.c_int),
-        ("whole_blocks", ctypes.c_int),
+        ("probe_units", ctypes.c_int),
         ("reserved", ctypes.c_int * 4),
     ]
 
@@ -555,11 +558,11 @@ class BlockIndex:
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
                margin: int = 0, force_fallback: bool = False, timing: bool = False,
                precision: str = "fp32", score_all: bool = False, router=None,
-               whole_blocks: bool = False, stream: Optional[int] = None):
+               units: str = "balls", stream: Optional[int] = None):
         """Exact k-NN through the index.  ``scores`` (nblocks x query_nb) or ``router`` (a Router
         of this index's shape, its first probe taken from the fused inference kernel) choose each
-        query's first probe; the lower bounds rank the rest.  ``whole_blocks`` scans every probed
-        block whole instead of only its balls that may hold a neighbour."""
+        query's first probe; the lower bounds rank the rest.  ``units``: what the scan visits in
+        the probed blocks (``UNITS``; indexes that do not know their balls scan whole blocks)."""
         import torch
 
         lib = load()
@@ -590,7 +593,9 @@ class BlockIndex:
         opt.timing = 1 if timing else 0
         opt.precision = PRECISIONS[precision]
         opt.score_all = 1 if score_all else 0
-        opt.whole_blocks = 1 if whole_blocks else 0
+        if units not in UNITS:
+            raise KnnError(f"units must be one of {sorted(UNITS)}")
+        opt.probe_units = UNITS[units]
         if stream is None:
             stream = torch.cuda.current_stream(query.device).cuda_stream
         if router is not None:

@@ -1,7 +1,8 @@
-"""GPU tests of ball probes in block-index searches: the scan visits the nearest ball of each
-query's first block, then only the balls of its probed blocks whose lower bound lies below the
-threshold that first scan left.  Every search equals the oracle bit for bit, and equals the
-whole-block search; index files of version 2 keep the balls' position ranges, version-1 files
+"""GPU tests of ball probes in block-index searches: the scan visits only the balls of its probed
+blocks whose lower bound lies below the reach of the nearest balls holding k + margin members (one
+pass), or the nearest ball of each query's first block and then the balls whose lower bound lies
+below the threshold that first scan left (two passes).  Every search equals the oracle bit for
+bit, and equals the whole-block search; index files of version 2 keep the balls' position ranges, version-1 files
 still load (whole-block searches)."""
 import numpy as np
 import pytest
@@ -38,18 +39,19 @@ def clustered():
     ix.close()
 
 
+@pytest.mark.parametrize("units", ["balls", "balls-threshold"])
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
 @pytest.mark.parametrize("probes", [1, 3, 8])
-def test_ball_probes_equal_oracle(clustered, precision, probes):
+def test_ball_probes_equal_oracle(clustered, precision, probes, units):
     ref, qry, k, gd, gi, ix = clustered
-    say(f"balls precision={precision} probes={probes}")
-    d, i = ix.search(qry, k, probes=probes, precision=precision, timing=True)
+    say(f"{units} precision={precision} probes={probes}")
+    d, i = ix.search(qry, k, probes=probes, precision=precision, timing=True, units=units)
     torch.cuda.synchronize()
     u = ix.last_units()
     say(f"units {u} fallbacks {knnhip.last_fallback_count()} work {ix.last_work()}")
     assert u["units"] == "balls" and u["balls"] > 64
-    check(d, i, gd, gi, f"balls {precision} P={probes}")
-    d2, i2 = ix.search(qry, k, probes=probes, precision=precision, whole_blocks=True)
+    check(d, i, gd, gi, f"{units} {precision} P={probes}")
+    d2, i2 = ix.search(qry, k, probes=probes, precision=precision, units="blocks")
     torch.cuda.synchronize()
     assert ix.last_units()["units"] == "blocks"
     check(d2, i2, gd, gi, f"blocks {precision} P={probes}")
@@ -59,7 +61,7 @@ def test_ball_probes_scan_fewer_pairs(clustered):
     ref, qry, k, gd, gi, ix = clustered
     ix.search(qry, k, probes=4, timing=True)
     wb = ix.last_work()
-    ix.search(qry, k, probes=4, timing=True, whole_blocks=True)
+    ix.search(qry, k, probes=4, timing=True, units="blocks")
     wk = ix.last_work()
     say(f"balls {wb} blocks {wk}")
     assert wb["pairs"] < wk["pairs"]
@@ -76,8 +78,12 @@ def test_ball_probes_with_router_scores_and_ties():
     ix = knnhip.BlockIndex(ref, nblocks=16, iters=3)
     scores = torch.rand((16, qry.shape[1]), device="cuda")
     for prec in ("fp32", "bf16"):
-        check(*ix.search(qry, k, probes=3, scores=scores, precision=prec), gd, gi, f"scores {prec}")
-        check(*ix.search(qry, k, probes=2, precision=prec), gd, gi, f"ties {prec}")
+        for units in ("balls", "balls-threshold"):
+            check(*ix.search(qry, k, probes=3, scores=scores, precision=prec, units=units), gd, gi,
+                  f"scores {prec} {units}")
+            check(*ix.search(qry, k, probes=2, precision=prec, units=units), gd, gi, f"ties {prec} {units}")
+    with pytest.raises(knnhip.KnnError):
+        ix.search(qry, k, units="rows")
     ix.close()
 
 
@@ -139,11 +145,12 @@ def test_ball_probes_at_scale_equal_plain():
     pd, pi = knnhip.search_device(ref, qry, k)
     ix = knnhip.BlockIndex(ref, nblocks=128, iters=8)
     for prec in ("fp32", "bf16x3", "bf16"):
-        d, i = ix.search(qry, k, probes=5, precision=prec, timing=True)
-        torch.cuda.synchronize()
-        say(f"{prec}: units {ix.last_units()} fallbacks {knnhip.last_fallback_count()} "
-            f"work {ix.last_work()} stages {ix.stage_ms()}")
-        assert torch.equal(i, pi) and torch.equal(d, pd), prec
+        for units in ("balls", "balls-threshold"):
+            d, i = ix.search(qry, k, probes=5, precision=prec, timing=True, units=units)
+            torch.cuda.synchronize()
+            say(f"{prec} {units}: units {ix.last_units()} fallbacks {knnhip.last_fallback_count()} "
+                f"work {ix.last_work()} stages {ix.stage_ms()}")
+            assert torch.equal(i, pi) and torch.equal(d, pd), (prec, units)
     sample = [0, 7, 1000, 2047]
     gd, gi = oracle.knn_c(ref.cpu().numpy(), np.ascontiguousarray(qry.cpu().numpy()[:, sample]), k)
     check(d[:, sample], i[:, sample], gd, gi, "oracle sample")

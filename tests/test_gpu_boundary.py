@@ -1,8 +1,8 @@
 """GPU tests of the boundary's behaviour beyond the golden cases.
 
-* The reference's own test program (code/test.cpp, its main unchanged), built against this
-  library under oracle/_ref, runs its full benchmark: ground truth by its knn_c, then its three
-  GPU entry points through their C++ symbols, each held to its own check (code/test.cpp:255-284).
+* The reference's own test harness (code/test.cpp compiled unchanged under oracle/_ref): its
+  data, its knn_c ground truth and its test() on the three GPU entry points through their C++
+  symbols, each held to its own check (code/test.cpp:255-284); its unchanged main on request.
 * Non-finite input: the host entry points fail the reference's way ("ERROR:" line, false);
   the device entry point writes index -1 / distance NaN; the next search is unaffected.
 * Calls on different streams are ordered on the device (they share the library's buffers).
@@ -36,22 +36,19 @@ def bits(a):
     return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
 
 
-@pytest.mark.skipif(not os.path.exists(REF_LIB), reason="reference test program not built")
-@pytest.mark.timeout(600)
-def test_reference_test_program_passes_against_library(capfd):
-    # The reference's main (code/test.cpp:292-356), compiled unchanged under oracle/_ref with only
-    # its name changed, runs in this process: its ground truth by its own knn_c, then its three
-    # GPU entry points through their C++ symbols, which the loader resolves to this library.
-    # (In this process rather than as the oracle/_ref/test_ref child: a GPU process starts no
-    # other program here.)
-    # Its lines are shown as it prints them (each of its steps can take a minute on a slow host)
-    # and kept for the checks: C stdout goes line-buffered into a pipe that a thread copies to
-    # the terminal.
-    say("running the reference's test program (16384 x 4096, dim 128, k 16)")
-    lib = ctypes.CDLL(REF_LIB)
-    main = lib._Z23knn_reference_test_mainv
-    main.restype = ctypes.c_int
-    main.argtypes = []
+KNNHIP_LIB = os.path.join(ROOT, "knn-cuda_amd", "libknnhip.so")
+# the reference's entry point signature (code/knncuda.h:14-21) and its test() (code/test.cpp:203-214)
+KNN_FN = ctypes.CFUNCTYPE(ctypes.c_bool, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                          ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+GPU_ENTRIES = {"knn_cuda_global": "_Z15knn_cuda_globalPKfiS0_iiiPfPi",
+               "knn_cuda_texture": "_Z16knn_cuda_texturePKfiS0_iiiPfPi",
+               "knn_cublas": "_Z10knn_cublasPKfiS0_iiiPfPi"}
+
+
+def run_streamed(capfd, fn):
+    """Runs fn() with C stdout line-buffered into a pipe that a thread copies to the terminal
+    (the reference program's lines are shown as it prints them) and keeps; returns (fn's value,
+    the text printed)."""
     libc = ctypes.CDLL(None)
     libc.setvbuf.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_size_t]
     c_stdout = ctypes.c_void_p.in_dll(libc, "stdout")
@@ -76,14 +73,81 @@ def test_reference_test_program_passes_against_library(capfd):
         t = threading.Thread(target=pump, daemon=True)
         t.start()
         try:
-            rc = main()
+            rv = fn()
         finally:
             libc.fflush(None)
             os.dup2(saved, 1)  # closes the pipe's last write end: the pump reads EOF
             t.join(timeout=30)
             os.close(rd)
             os.close(saved)
-    stdout = b"".join(chunks).decode(errors="replace")
+    return rv, b"".join(chunks).decode(errors="replace")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LIB), reason="reference test program not built")
+@pytest.mark.timeout(600)
+def test_reference_harness_passes_against_library(capfd):
+    # The reference's own harness functions (code/test.cpp compiled unchanged under oracle/_ref)
+    # run in this process: its initialize_data (10 * U[0,1) points), its knn_c ground truth, then
+    # its test() (code/test.cpp:203-284: 0.1 % tolerance on distances and indexes, PASSED /
+    # FAILED line) on the three GPU entry points, called through their C++ symbols in this
+    # library.  Its main's sizes (16384 x 4096, dim 128, k 16) with 512 queries: the ground truth
+    # of all 4096 is minutes of single-thread CPU time on some hosts; the unchanged main runs in
+    # test_reference_main_passes_against_library when KNNHIP_REFERENCE_MAIN=1.
+    ref_nb, query_nb, dim, k = 16384, 512, 128, 16
+    lib = ctypes.CDLL(REF_LIB)
+    knnlib = ctypes.CDLL(KNNHIP_LIB)
+    P = ctypes.c_void_p
+    init = lib._Z15initialize_dataPfiS_ii
+    init.argtypes = [P, ctypes.c_int, P, ctypes.c_int, ctypes.c_int]
+    knn_c = lib._Z5knn_cPKfiS0_iiiPfPi
+    knn_c.argtypes = [P, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]
+    knn_c.restype = ctypes.c_bool
+    test = lib._Z4testPKfiS0_iiiPfPiPFbS0_iS0_iiiS1_S2_EPKci
+    test.argtypes = [P, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, KNN_FN,
+                     ctypes.c_char_p, ctypes.c_int]
+    test.restype = ctypes.c_bool
+    ref = np.zeros(ref_nb * dim, np.float32)
+    qry = np.zeros(query_nb * dim, np.float32)
+    gd = np.zeros(query_nb * k, np.float32)
+    gi = np.zeros(query_nb * k, np.int32)
+    init(ref.ctypes.data, ref_nb, qry.ctypes.data, query_nb, dim)
+    say(f"reference ground truth by its knn_c ({ref_nb} x {query_nb}, dim {dim}, k {k})")
+    assert knn_c(ref.ctypes.data, ref_nb, qry.ctypes.data, query_nb, dim, k, gd.ctypes.data, gi.ctypes.data)
+    say("reference test() on the three entry points")
+
+    def run():
+        ok = True
+        for name, sym in GPU_ENTRIES.items():
+            fn = KNN_FN(ctypes.cast(getattr(knnlib, sym), P).value)
+            ok = test(ref.ctypes.data, ref_nb, qry.ctypes.data, query_nb, dim, k, gd.ctypes.data,
+                      gi.ctypes.data, fn, name.encode(), 20) and ok
+        return ok
+
+    ok, stdout = run_streamed(capfd, run)
+    assert ok, stdout[-2000:]
+    lines = [l for l in stdout.splitlines() if l.startswith("- ")]
+    for name in GPU_ENTRIES:
+        row = [l for l in lines if l.split(":")[0].strip() == f"- {name}"]
+        assert row and "PASSED" in row[0], f"{name}: {row}"
+    # beyond the reference's 0.1 % tolerance: the outputs equal its ground truth bit for bit
+    d, i = knnhip.knn_cuda_global(ref.reshape(dim, ref_nb), qry.reshape(dim, query_nb), k)
+    assert np.array_equal(i.reshape(-1), gi) and np.array_equal(bits(d).reshape(-1), bits(gd))
+
+
+@pytest.mark.skipif(os.environ.get("KNNHIP_REFERENCE_MAIN") != "1" or not os.path.exists(REF_LIB),
+                    reason="the reference's unchanged main (minutes of single-thread ground truth): "
+                           "set KNNHIP_REFERENCE_MAIN=1")
+@pytest.mark.timeout(1800)
+def test_reference_main_passes_against_library(capfd):
+    # The reference's main (code/test.cpp:292-356), compiled unchanged under oracle/_ref with only
+    # its name changed, runs in this process: ground truth by its knn_c, then its three GPU entry
+    # points through their C++ symbols, which the loader resolves to this library.
+    say("running the reference's test program (16384 x 4096, dim 128, k 16)")
+    lib = ctypes.CDLL(REF_LIB)
+    main = lib._Z23knn_reference_test_mainv
+    main.restype = ctypes.c_int
+    main.argtypes = []
+    rc, stdout = run_streamed(capfd, main)
     assert rc == 0, stdout[-2000:]
     lines = [l for l in stdout.splitlines() if l.startswith("- ")]
     names = ["knn_c", "knn_cuda_global", "knn_cuda_texture", "knn_cublas"]
