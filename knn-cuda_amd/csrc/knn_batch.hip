@@ -16,15 +16,17 @@ namespace {
 long long g_batch_fail = 0;  // certificate failures of the last batched search
 
 // ---------------------------------------------------------------------------
-// Batched search: many independent problems, one scan launch per (dim, k)
-// class.  Within a class the problems' reference sets are laid out one after
-// another (each starting on a 32-point tile) in one packed copy, and the
-// task-mode scan runs tasks of (one slice of one problem's references, up
-// to 32 of that problem's queries) for every problem at once.  The rows-mode
-// re-rank maps positions back to each problem's own indexes, so every query
-// gets exactly the result of a search of its own problem.  The slice length
-// (references per task) is the class's tile shape, picked by the autotuner
-// or, untuned, by the cost model below.
+// Batched search: many independent problems, one persistent scan launch for
+// all of them.  Problems are grouped into (dim, k) classes; within a class
+// the problems' reference sets are laid out one after another (each starting
+// on a 32-point tile) in one packed copy, and its tasks are (one slice of one
+// problem's references, up to 32 of that problem's queries).  The scan's
+// work queue holds every task of every class, largest first, and each block
+// runs a task with its class's arguments and scan instance.  The rows-mode
+// re-rank (one launch per class) maps positions back to each problem's own
+// indexes, so every query gets exactly the result of a search of its own
+// problem.  The slice length (references per task) is the class's tile
+// shape, picked by the autotuner or, untuned, by the cost model below.
 // ---------------------------------------------------------------------------
 struct BatchWork {
     DevBuf packed, pm, norms, perm, rmax, starts, counts, query, qpm, qnorm, theta, tasks, qlist,
@@ -33,6 +35,10 @@ struct BatchWork {
     bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
 constexpr int kBatchSlices[] = {2048, 8192, 32768, 131072, 1 << 20};
+// Largest task of a batched launch as a share of the work per resident
+// block: untuned, and the shares the autotuner times.
+constexpr double kBatchFrac = 0.5;
+constexpr double kBatchTuneFracs[] = {0.125, 0.25, 0.5, 1.0, 2.0};
 
 struct BatchTuneKey {
     int dim, k;
@@ -45,63 +51,28 @@ struct BatchClass {
     std::vector<const knnhip_problem *> probs;
 };
 
-// Classes in flight: each on its own stream and workspace.  A class is
-// queued (layouts, scan, select, an asynchronous read-back of its failure
-// count) and finished (failures searched again, outputs scattered) only when
-// its lane comes round again, so the host prepares the next classes while
-// the GPU runs the earlier ones, and small classes share the chip.
-constexpr int kBatchLanes = 4;
-BatchWork g_bw[kBatchLanes];
-struct BatchLanes {
-    bool ready = false;
-    hipStream_t stream[kBatchLanes] = {};
-    hipEvent_t counted[kBatchLanes] = {};  // failure count copied to host
-    hipEvent_t start = nullptr;            // the caller's inputs are ready
-    int *nfail = nullptr;                  // pinned, one per lane
+// Workspaces of a batched run: one per class, and the run's work queue.
+std::vector<BatchWork> g_bwc;
+struct MultiWork {
+    DevBuf cls, variant, work, next, nfail;
+    int *nfail_host = nullptr;  // pinned, one per class
+    int nfail_cap = 0;
+    bool need(DevBuf &b, size_t bytes) { return b.n >= bytes || b.alloc(bytes); }
 };
-BatchLanes g_lanes;
+MultiWork g_mw;
 
-bool batch_lanes_ready() {
-    if (g_lanes.ready) return true;
-    for (int l = 0; l < kBatchLanes; ++l) {
-        KNN_TRY(hipStreamCreateWithFlags(&g_lanes.stream[l], hipStreamNonBlocking), "stream create");
-        KNN_TRY(hipEventCreateWithFlags(&g_lanes.counted[l], hipEventDisableTiming), "event create");
-    }
-    KNN_TRY(hipEventCreateWithFlags(&g_lanes.start, hipEventDisableTiming), "event create");
-    KNN_TRY(hipHostMalloc((void **)&g_lanes.nfail, kBatchLanes * sizeof(int), hipHostMallocDefault),
-            "pinned allocation");
-    g_lanes.ready = true;
-    return true;
-}
-
-// Lane streams wait for what the caller queued on its stream.
-bool batch_lanes_follow(hipStream_t s) {
-    KNN_TRY(hipEventRecord(g_lanes.start, s), "event");
-    for (int l = 0; l < kBatchLanes; ++l) KNN_TRY(hipStreamWaitEvent(g_lanes.stream[l], g_lanes.start, 0), "event wait");
-    return true;
-}
-
-// A class between its two phases.
+// One class of a batched run.
 struct ClassRun {
     const BatchClass *c = nullptr;
-    int lane = 0;
-    bool active = false;
     bool direct = false;  // searched problem by problem, already finished
+    Plan pl;
+    int variant = 0;
+    int L = 0, P = 0, ntasks = 0;
     long long Qtot = 0;
     std::vector<int> qstart;
-    hipEvent_t e0 = nullptr, e1 = nullptr;  // scan timing, when asked for
-    float *scan_ms = nullptr;
+    std::vector<ScanTask> tasks;
+    ScanArgs sa{};
 };
-
-
-// Tasks of a class for slice length L: per problem ceil(R/L) slices times
-// ceil(Q/32) query groups.
-long long batch_tasks(const BatchClass &c, int L) {
-    long long t = 0;
-    for (const knnhip_problem *p : c.probs)
-        t += (long long)((p->ref_nb + L - 1) / L) * ((p->query_nb + kQueriesPerWave - 1) / kQueriesPerWave);
-    return t;
-}
 
 int batch_max_slices(const BatchClass &c, int L) {
     int m = 1;
@@ -109,63 +80,104 @@ int batch_max_slices(const BatchClass &c, int L) {
     return m;
 }
 
-// Untuned choice: the slice lengths whose per-query row count fits the
-// re-rank's union (rows * kp <= kMaxUnion), costed as rounds of resident
-// blocks times (steps per wave + a fixed per-task cost of ~3 steps).
-int batch_default_slice(const BatchClass &c, int kp, int resident, int waves) {
-    int best = 0;
-    double best_cost = 1e300;
-    for (int L : kBatchSlices) {
-        if ((long long)batch_max_slices(c, L) * kp > kMaxUnion) continue;
-        double cost = 0.0;
-        // per-task work: a task's waves share its slice's 128-ref steps
-        long long tasks = 0, steps = 0;
-        for (const knnhip_problem *p : c.probs) {
-            const long long ns = (p->ref_nb + L - 1) / L, ng = (p->query_nb + 31) / 32;
-            tasks += ns * ng;
-            const long long span = std::min<long long>(L, p->ref_nb);
-            steps += ns * ng * ((span + kRefsPerStep - 1) / kRefsPerStep);
-        }
-        const double avg = (double)steps / std::max<long long>(1, tasks) / waves + 3.0;
-        cost = std::ceil((double)tasks / resident) * avg;
-        if (cost < best_cost) {
-            best_cost = cost;
-            best = L;
-        }
-    }
-    return best;
+// The batched scan's instance and padded dimension for a class's dim.
+int batch_dpad(int dim) {
+    int ks = 0, nch = 1;
+    const int v = scan_multi_variant(dim, &ks, &nch);
+    return v < 4 ? 2 * ks : 128 * nch;
 }
 
-// Queue phase of class c on `lane` (its stream and workspace; the lane's
-// previous class must be finished).
-bool batch_class_queue(const BatchClass &c, int L, const knnhip_options &o, int lane, float *scan_ms,
-                       ClassRun &run) {
-    BatchWork &w = g_bw[lane];
-    const hipStream_t s = g_lanes.stream[lane];
-    // the lane's previous outputs are out before its buffers are reused
-    KNN_TRY(hipStreamSynchronize(s), "sync");
+// Resident blocks of the batched scan for these classes (its LDS is sized
+// by their largest kp).
+int batch_multi_resident(const std::vector<const BatchClass *> &cs, const knnhip_options &o) {
+    size_t lds = scan_multi_lds(1);
+    for (const BatchClass *c : cs) {
+        const Plan pl = make_plan(1 << 20, kQueriesPerWave, c->dim, c->k, o);
+        if (pl.use_scan) lds = std::max(lds, scan_multi_lds(pl.kp));
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    return cus * std::max(1, scan_multi_blocks_per_cu(lds));
+}
+
+// Slice lengths of one batched launch over every class: per class the
+// longest slice whose largest task (128-reference steps plus ~3 steps of
+// set-up, times the padded dimension) stays within `frac` of the work each
+// resident block gets when all tasks of all classes are shared among them,
+// within the re-rank's union (rows * kp <= kMaxUnion).  All tasks queue in
+// one launch, so longer slices (fewer tasks, fewer heap rows to merge) only
+// have to leave no single task longer than a block's share.
+std::vector<int> batch_pick_slices(const std::vector<const BatchClass *> &cs, double frac, int resident,
+                                   const knnhip_options &o) {
+    const int Lmax = kBatchSlices[sizeof(kBatchSlices) / sizeof(int) - 1];
+    auto task_cost = [](int span, int dpad) {
+        return ((long long)(span + kRefsPerStep - 1) / kRefsPerStep + 3) * dpad;
+    };
+    double total = 0.0;
+    for (const BatchClass *c : cs) {
+        const int dpad = batch_dpad(c->dim);
+        for (const knnhip_problem *p : c->probs)
+            total += (double)((p->query_nb + kQueriesPerWave - 1) / kQueriesPerWave) *
+                     task_cost(std::min(p->ref_nb, Lmax), dpad);
+    }
+    const double budget = frac * total / std::max(1, resident);
+    std::vector<int> out;
+    for (const BatchClass *c : cs) {
+        const Plan pl = make_plan(1 << 20, kQueriesPerWave, c->dim, c->k, o);
+        const int kp = std::max(pl.kp, c->k), dpad = batch_dpad(c->dim);
+        int maxR = 0;
+        for (const knnhip_problem *p : c->probs) maxR = std::max(maxR, p->ref_nb);
+        int pick = 0;
+        for (int L : kBatchSlices) {
+            if ((long long)batch_max_slices(*c, L) * kp > kMaxUnion) continue;
+            if (!pick) pick = L;  // the shortest the union allows
+            if ((double)task_cost(std::min(L, maxR), dpad) <= budget) pick = L;
+        }
+        out.push_back(pick ? pick : Lmax);
+    }
+    return out;
+}
+
+// A class whose k leaves no room for the scan's heaps (searched problem by
+// problem through the plain engine).
+bool batch_class_direct(const BatchClass &c, const knnhip_options &o) {
+    int maxR = 0;
+    for (const knnhip_problem *p : c.probs) maxR = std::max(maxR, p->ref_nb);
+    return !make_plan(maxR, kQueriesPerWave, c.dim, c.k, o).use_scan;
+}
+
+// Class c's layouts on stream s in workspace w, with the scan arguments and
+// tasks for slice length L kept in run (the scan itself is the run's one
+// batched launch).  A class whose k leaves no room for the scan's heaps is
+// searched problem by problem at once (run.direct).
+bool batch_class_prepare(const BatchClass &c, int L, const knnhip_options &o, BatchWork &w, hipStream_t s,
+                         ClassRun &run) {
     run = ClassRun{};
     run.c = &c;
-    run.lane = lane;
-    run.active = true;
-    run.scan_ms = scan_ms;
     const int dim = c.dim, k = c.k;
     int maxR = 0;
     for (const knnhip_problem *p : c.probs) maxR = std::max(maxR, p->ref_nb);
-    const Plan pl = make_plan(maxR, kQueriesPerWave, dim, k, o);
+    Plan pl = make_plan(maxR, kQueriesPerWave, dim, k, o);
     if (!pl.use_scan) {
-        // k too large for the scan's heaps: each problem on its own, through
-        // the plain engine's shared workspace, so synchronously
         for (const knnhip_problem *p : c.probs)
             if (!run_device(p->ref, p->ref_ld, p->ref_nb, p->query, p->query_ld, p->query_nb, dim, k,
-                            p->knn_dist, p->knn_index, make_plan(p->ref_nb, p->query_nb, dim, k, o), o,
-                            s))
+                            p->knn_dist, p->knn_index, make_plan(p->ref_nb, p->query_nb, dim, k, o), o, s))
                 return false;
-        KNN_TRY(hipStreamSynchronize(s), "sync");
         run.direct = true;
-        if (scan_ms) *scan_ms = 0.f;
         return true;
     }
+    // the batched scan's instance for this dimension and its packed layout
+    int ks = 0, nch = 1;
+    run.variant = scan_multi_variant(dim, &ks, &nch);
+    pl.ks = ks;
+    pl.nchunks = nch;
+    pl.dpad = run.variant < 4 ? 2 * ks : 128 * nch;
+    pl.oct = pl.dpad / 8;
+    run.pl = pl;
     const int kp = pl.kp;
     const int np = (int)c.probs.size();
     const int pm_ld = (dim + 3) & ~3;
@@ -190,7 +202,7 @@ bool batch_class_queue(const BatchClass &c, int L, const knnhip_options &o, int 
         return false;
     }
     // tasks, query lists, and each query's rows (one per slice of its problem)
-    std::vector<ScanTask> tasks;
+    std::vector<ScanTask> &tasks = run.tasks;
     std::vector<int> qlist, rows((size_t)Qtot * P, -1);
     for (int i = 0; i < np; ++i) {
         const knnhip_problem *p = c.probs[i];
@@ -286,17 +298,11 @@ bool batch_class_queue(const BatchClass &c, int L, const knnhip_options &o, int 
     nq.pm_ld = pm_ld;
     KNN_TRY(launch_norms_jobs(nq, w.qjobs.as<NormJob>(), w.qwg.as<int2>(), (int)qwg.size(), s),
             "norm kernel launch");
-    // the class's dim-major query block (read by the chunked scan) from the rows
+    // the class's dim-major query block (read by the chunked instance) from the rows
     KNN_TRY(idx_transpose(w.qpm.as<float>(), pm_ld, (int)Qtot, dim, w.query.as<float>(), s), "queries");
     KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Qtot, s), "memset");
     KNN_TRY(hipMemsetAsync(w.fb_count.p, 0, 4, s), "memset");
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (scan_ms) {
-        KNN_TRY(hipEventCreate(&e0), "event");
-        KNN_TRY(hipEventCreate(&e1), "event");
-        KNN_TRY(hipEventRecord(e0, s), "event");
-    }
-    ScanArgs sa{};
+    ScanArgs &sa = run.sa;
     sa.packed = w.packed.as<float>();
     sa.oct = pl.oct;
     sa.R = (int)n_pos;
@@ -320,25 +326,35 @@ bool batch_class_queue(const BatchClass &c, int L, const knnhip_options &o, int 
     sa.qlist = w.qlist.as<int>();
     sa.query_pm = w.qpm.as<float>();
     sa.query_pm_ld = pm_ld;
-    KNN_TRY(launch_scan(sa, pl.ks, pl.bmode, pl.waves, s), "scan kernel launch");
-    if (scan_ms) KNN_TRY(hipEventRecord(e1, s), "event");
+    run.L = L;
+    run.P = P;
+    run.ntasks = ntasks;
+    run.Qtot = Qtot;
+    run.qstart = qstart;
+    return true;
+}
+
+// Class c's re-rank and certificate after the batched scan; its failure
+// count goes to *nfail_dev.
+bool batch_class_select(const ClassRun &run, const knnhip_options &o, BatchWork &w, hipStream_t s) {
+    const BatchClass &c = *run.c;
     FinalArgs fa{};
-    fa.R = (int)n_pos;
+    fa.R = run.sa.R;
     fa.query = w.query.as<float>();
-    fa.ldq = Qtot;
-    fa.Q = (int)Qtot;
-    fa.dim = dim;
-    fa.k = k;
-    fa.kp = kp;
-    fa.nrows = P;
+    fa.ldq = run.Qtot;
+    fa.Q = (int)run.Qtot;
+    fa.dim = c.dim;
+    fa.k = c.k;
+    fa.kp = run.pl.kp;
+    fa.nrows = run.P;
     fa.rows = w.rows.as<int>();
     fa.perm = w.perm.as<int>();
-    fa.maxc = std::min(kMaxCand, std::max(2 * kp, 256));
+    fa.maxc = std::min(kMaxCand, std::max(2 * run.pl.kp, 256));
     fa.part_val = w.pval.as<float>();
     fa.part_idx = w.pidx.as<int>();
     fa.ref_pm = w.pm.as<float>();
     fa.query_pm = w.qpm.as<float>();
-    fa.pm_ld = pm_ld;
+    fa.pm_ld = (c.dim + 3) & ~3;
     fa.qnorm = w.qnorm.as<float>();
     fa.rmax = w.rmax.as<unsigned>();
     fa.theta = w.theta.as<unsigned>();
@@ -348,37 +364,16 @@ bool batch_class_queue(const BatchClass &c, int L, const knnhip_options &o, int 
     fa.fb_count = w.fb_count.as<int>();
     fa.fb_list = w.fb_list.as<int>();
     KNN_TRY(launch_select(fa, s), "select kernel launch");
-    // the failure count comes back asynchronously
-    KNN_TRY(hipMemcpyAsync(g_lanes.nfail + lane, w.fb_count.p, 4, hipMemcpyDeviceToHost, s), "copy");
-    KNN_TRY(hipEventRecord(g_lanes.counted[lane], s), "event");
-    run.Qtot = Qtot;
-    run.qstart = qstart;
-    run.e0 = e0;
-    run.e1 = e1;
     return true;
 }
 
-// Finish phase: certificate failures searched again (the plain search of
-// their own problem), then each problem's k x query_nb outputs.
-bool batch_class_finish(ClassRun &run, const knnhip_options &o) {
-    if (!run.active) return true;
-    run.active = false;
-    if (run.direct) return true;
+// Finish: certificate failures (nfail of them) searched again (the plain
+// search of their own problem), then each problem's k x query_nb outputs.
+bool batch_class_finish(const ClassRun &run, int nfail, const knnhip_options &o, BatchWork &w, hipStream_t s) {
     const BatchClass &c = *run.c;
-    const int lane = run.lane;
-    BatchWork &w = g_bw[lane];
-    const hipStream_t s = g_lanes.stream[lane];
     const int dim = c.dim, k = c.k, np = (int)c.probs.size();
     const long long Qtot = run.Qtot;
     const std::vector<int> &qstart = run.qstart;
-    KNN_TRY(hipEventSynchronize(g_lanes.counted[lane]), "sync");
-    const int nfail = g_lanes.nfail[lane];
-    if (run.scan_ms) {
-        KNN_TRY(hipEventSynchronize(run.e1), "sync");
-        KNN_TRY(hipEventElapsedTime(run.scan_ms, run.e0, run.e1), "event");
-        (void)hipEventDestroy(run.e0);
-        (void)hipEventDestroy(run.e1);
-    }
     g_batch_fail += nfail;
     if (nfail > 0) {
         std::vector<int> fl(nfail);
@@ -429,6 +424,164 @@ bool batch_class_finish(ClassRun &run, const knnhip_options &o) {
     return true;
 }
 
+// Streams of a batched run: the classes' layouts, re-ranks and outputs are
+// spread over kBatchLanes streams (small launches of different classes
+// overlap); the scan is one launch on the caller's stream between them.
+constexpr int kBatchLanes = 4;
+struct BatchLanes {
+    bool ready = false;
+    hipStream_t stream[kBatchLanes] = {};
+    hipEvent_t done[kBatchLanes] = {};
+    hipEvent_t start = nullptr;
+};
+BatchLanes g_lanes;
+
+bool batch_lanes_ready() {
+    if (g_lanes.ready) return true;
+    for (int l = 0; l < kBatchLanes; ++l) {
+        KNN_TRY(hipStreamCreateWithFlags(&g_lanes.stream[l], hipStreamNonBlocking), "stream create");
+        KNN_TRY(hipEventCreateWithFlags(&g_lanes.done[l], hipEventDisableTiming), "event create");
+    }
+    KNN_TRY(hipEventCreateWithFlags(&g_lanes.start, hipEventDisableTiming), "event create");
+    g_lanes.ready = true;
+    return true;
+}
+
+// The lanes wait for what is queued on s so far.
+bool batch_lanes_follow(hipStream_t s) {
+    KNN_TRY(hipEventRecord(g_lanes.start, s), "event");
+    for (int l = 0; l < kBatchLanes; ++l) KNN_TRY(hipStreamWaitEvent(g_lanes.stream[l], g_lanes.start, 0), "event wait");
+    return true;
+}
+
+// s waits for what is queued on the lanes so far.
+bool batch_lanes_join(hipStream_t s) {
+    for (int l = 0; l < kBatchLanes; ++l) {
+        KNN_TRY(hipEventRecord(g_lanes.done[l], g_lanes.stream[l]), "event");
+        KNN_TRY(hipStreamWaitEvent(s, g_lanes.done[l], 0), "event wait");
+    }
+    return true;
+}
+
+// One batched run over classes cs with slice lengths Ls on stream s: every
+// class's layouts, one persistent scan launch over all their tasks (timed
+// into *scan_ms when asked), per-class re-rank, then failures and outputs.
+// Classes searched problem by problem (k too large for the heaps) go first,
+// on s alone: they use the plain engine's shared workspace, as do the
+// re-searches of failed queries, which run one class at a time.
+bool batch_run(const std::vector<const BatchClass *> &cs, const std::vector<int> &Ls, const knnhip_options &o,
+               hipStream_t s, float *scan_ms) {
+    const int n = (int)cs.size();
+    if ((int)g_bwc.size() < n) g_bwc.resize(n);
+    if (!batch_lanes_ready()) return false;
+    std::vector<ClassRun> runs(n);
+    std::vector<int> lane_of(n, 0);
+    int nl = 0;
+    for (int i = 0; i < n; ++i)
+        if (batch_class_direct(*cs[i], o) && !batch_class_prepare(*cs[i], Ls[i], o, g_bwc[i], s, runs[i]))
+            return false;
+    if (!batch_lanes_follow(s)) return false;
+    for (int i = 0; i < n; ++i) {
+        if (runs[i].direct) continue;
+        lane_of[i] = nl++ % kBatchLanes;
+        if (!batch_class_prepare(*cs[i], Ls[i], o, g_bwc[i], g_lanes.stream[lane_of[i]], runs[i])) return false;
+    }
+    if (!batch_lanes_join(s)) return false;
+    // the work queue: every task of every scanned class, larger first
+    std::vector<ScanArgs> args;
+    std::vector<int> variants, slot(n, -1);
+    std::vector<std::pair<long long, int2>> items;
+    size_t lds = 0;
+    for (int i = 0; i < n; ++i) {
+        const ClassRun &r = runs[i];
+        if (r.direct || r.ntasks == 0) continue;
+        slot[i] = (int)args.size();
+        args.push_back(r.sa);
+        variants.push_back(r.variant);
+        lds = std::max(lds, scan_multi_lds(r.pl.kp));
+        for (int t = 0; t < r.ntasks; ++t) {
+            const ScanTask &tk = r.tasks[t];
+            const long long steps = (tk.ref_end - tk.ref_begin + kRefsPerStep - 1) / kRefsPerStep;
+            items.push_back({(steps + 3) * r.pl.dpad, make_int2(slot[i], t)});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<long long, int2> &x, const std::pair<long long, int2> &y) {
+                         return x.first > y.first;
+                     });
+    const int nwork = (int)items.size();
+    const int ncls = (int)args.size();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (nwork > 0) {
+        std::vector<int2> work(nwork);
+        for (int i = 0; i < nwork; ++i) work[i] = items[i].second;
+        if (!g_mw.need(g_mw.cls, (size_t)ncls * sizeof(ScanArgs)) || !g_mw.need(g_mw.variant, (size_t)ncls * 4) ||
+            !g_mw.need(g_mw.work, (size_t)nwork * sizeof(int2)) || !g_mw.need(g_mw.next, 4)) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        KNN_TRY(hipMemcpyAsync(g_mw.cls.p, args.data(), (size_t)ncls * sizeof(ScanArgs), hipMemcpyHostToDevice, s),
+                "copy");
+        KNN_TRY(hipMemcpyAsync(g_mw.variant.p, variants.data(), (size_t)ncls * 4, hipMemcpyHostToDevice, s), "copy");
+        KNN_TRY(hipMemcpyAsync(g_mw.work.p, work.data(), (size_t)nwork * sizeof(int2), hipMemcpyHostToDevice, s),
+                "copy");
+        KNN_TRY(hipMemsetAsync(g_mw.next.p, 0, 4, s), "memset");
+        const int per_cu = scan_multi_blocks_per_cu(lds);
+        if (per_cu < 1) {
+            set_error("batched scan does not fit a CU (LDS %zu bytes)", lds);
+            return false;
+        }
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+            (void)hipGetLastError();
+            cus = 256;
+        }
+        const int grid = std::min(nwork, cus * per_cu);
+        if (scan_ms) {
+            KNN_TRY(hipEventCreate(&e0), "event");
+            KNN_TRY(hipEventCreate(&e1), "event");
+            KNN_TRY(hipEventRecord(e0, s), "event");
+        }
+        KNN_TRY(launch_scan_multi(g_mw.cls.as<ScanArgs>(), g_mw.variant.as<int>(), g_mw.work.as<int2>(), nwork,
+                                  g_mw.next.as<unsigned>(), lds, grid, s),
+                "batched scan launch");
+        if (scan_ms) KNN_TRY(hipEventRecord(e1, s), "event");
+    }
+    // re-rank per class on the lanes; the failure counts come back together
+    if (g_mw.nfail_cap < n) {
+        if (g_mw.nfail_host) (void)hipHostFree(g_mw.nfail_host);
+        g_mw.nfail_host = nullptr;
+        g_mw.nfail_cap = 0;
+        KNN_TRY(hipHostMalloc((void **)&g_mw.nfail_host, (size_t)n * sizeof(int), hipHostMallocDefault),
+                "pinned allocation");
+        g_mw.nfail_cap = n;
+    }
+    if (!batch_lanes_follow(s)) return false;
+    for (int i = 0; i < n; ++i) {
+        g_mw.nfail_host[i] = 0;
+        if (runs[i].direct || runs[i].ntasks == 0) continue;
+        const hipStream_t ls = g_lanes.stream[lane_of[i]];
+        if (!batch_class_select(runs[i], o, g_bwc[i], ls)) return false;
+        KNN_TRY(hipMemcpyAsync(g_mw.nfail_host + i, g_bwc[i].fb_count.p, 4, hipMemcpyDeviceToHost, ls), "copy");
+    }
+    for (int l = 0; l < kBatchLanes; ++l) KNN_TRY(hipStreamSynchronize(g_lanes.stream[l]), "sync");
+    if (scan_ms) {
+        *scan_ms = 0.f;
+        if (e0) {
+            KNN_TRY(hipEventSynchronize(e1), "sync");
+            KNN_TRY(hipEventElapsedTime(scan_ms, e0, e1), "event");
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        if (runs[i].direct || runs[i].ntasks == 0) continue;
+        if (!batch_class_finish(runs[i], g_mw.nfail_host[i], o, g_bwc[i], g_lanes.stream[lane_of[i]])) return false;
+    }
+    return batch_lanes_join(s);
+}
+
 std::vector<BatchClass> batch_classes(const knnhip_problem *probs, int n) {
     std::map<BatchTuneKey, int> at;
     std::vector<BatchClass> out;
@@ -462,8 +615,7 @@ bool batch_args(const knnhip_problem *probs, int n) {
     return true;
 }
 
-// Resident scan blocks of a class's plan.
-int batch_resident(const Plan &pl) { return std::max(1, pl.cus) * std::max(1, pl.blocks_per_cu); }
+
 
 }  // namespace
 }  // namespace knnhip
@@ -480,36 +632,32 @@ int knnhip_search_batched(const knnhip_problem *problems, int count, const knnhi
     const knnhip_options o = opt ? *opt : default_options(0);
     hipStream_t s = (hipStream_t)stream;
     CallOrder order(s);
-    if (!batch_lanes_ready() || !batch_lanes_follow(s)) return 0;
     g_batch_fail = 0;
     std::string plan = "{\"classes\": [";
     bool first = true;
     const std::vector<BatchClass> classes = batch_classes(problems, count);
-    ClassRun runs[kBatchLanes];
-    int ci = 0;
-    for (const BatchClass &c : classes) {
-        const Plan pl = make_plan(1 << 20, kQueriesPerWave, c.dim, c.k, o);
+    std::vector<const BatchClass *> cs;
+    std::vector<int> Ls;
+    for (const BatchClass &c : classes) cs.push_back(&c);
+    const std::vector<int> picked = batch_pick_slices(cs, kBatchFrac, batch_multi_resident(cs, o), o);
+    for (size_t ci = 0; ci < classes.size(); ++ci) {
+        const BatchClass &c = classes[ci];
         auto it = g_batch_table.find(BatchTuneKey{c.dim, c.k});
-        int L = it != g_batch_table.end() ? it->second : batch_default_slice(c, std::max(pl.kp, c.k), batch_resident(pl), pl.waves);
-        if (L == 0) L = kBatchSlices[sizeof(kBatchSlices) / sizeof(int) - 1];
-        char buf[160];
-        snprintf(buf, sizeof(buf), "%s{\"dim\": %d, \"k\": %d, \"problems\": %d, \"slice\": %d, \"tuned\": %s}",
-                 first ? "" : ", ", c.dim, c.k, (int)c.probs.size(), L,
-                 it != g_batch_table.end() ? "true" : "false");
+        const int L = it != g_batch_table.end() ? it->second : picked[ci];
+        int ks = 0, nch = 1;
+        const int v = scan_multi_variant(c.dim, &ks, &nch);
+        char buf[192];
+        snprintf(buf, sizeof(buf),
+                 "%s{\"dim\": %d, \"k\": %d, \"problems\": %d, \"slice\": %d, \"tuned\": %s, \"instance\": %d}",
+                 first ? "" : ", ", c.dim, c.k, (int)c.probs.size(), L, it != g_batch_table.end() ? "true" : "false",
+                 v);
         plan += buf;
         first = false;
-        const int lane = ci++ % kBatchLanes;
-        if (!batch_class_finish(runs[lane], o) || !batch_class_queue(c, L, o, lane, nullptr, runs[lane]))
-            return 0;
+        Ls.push_back(L);
     }
-    for (int l = 0; l < kBatchLanes; ++l) {
-        if (!batch_class_finish(runs[l], o)) return 0;
-        if (hipStreamSynchronize(g_lanes.stream[l]) != hipSuccess) {
-            set_error("sync: %s", hipGetErrorString(hipGetLastError()));
-            return 0;
-        }
-    }
-    g_last_plan = plan + "]}";
+    if (!batch_run(cs, Ls, o, s, nullptr)) return 0;
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    g_last_plan = plan + "], \"scan_launches\": 1}";
     g_last_fb = nullptr;
     g_last_nonfinite = nullptr;
     return 1;
@@ -524,30 +672,30 @@ int knnhip_batched_autotune(const knnhip_problem *problems, int count, int reps,
     knnhip_options o = default_options(0);
     hipStream_t s = (hipStream_t)stream;
     CallOrder order(s);
-    if (!batch_lanes_ready() || !batch_lanes_follow(s)) return 0;
+    // the whole batch in its one launch, for each share of a block's work a
+    // task may take; the fastest sets every class's slice
     const std::vector<BatchClass> classes = batch_classes(problems, count);
-    for (const BatchClass &c : classes) {
-        const Plan pl = make_plan(1 << 20, kQueriesPerWave, c.dim, c.k, o);
-        const int kp = std::max(pl.kp, c.k);
-        int best = 0;
-        float best_ms = 0.f;
-        for (int L : kBatchSlices) {
-            if ((long long)batch_max_slices(c, L) * kp > kMaxUnion) continue;
-            if (batch_tasks(c, L) > (1ll << 24)) continue;
-            float tot = 0.f;
-            for (int r = 0; r < std::max(1, reps); ++r) {
-                float ms = 0.f;
-                ClassRun run;
-                if (!batch_class_queue(c, L, o, 0, &ms, run) || !batch_class_finish(run, o)) return 0;
-                tot += ms;
-            }
-            if (!best || tot < best_ms) {
-                best = L;
-                best_ms = tot;
-            }
+    std::vector<const BatchClass *> cs;
+    for (const BatchClass &c : classes) cs.push_back(&c);
+    const int resident = batch_multi_resident(cs, o);
+    std::vector<int> best;
+    float best_ms = 0.f;
+    for (double frac : kBatchTuneFracs) {
+        const std::vector<int> Ls = batch_pick_slices(cs, frac, resident, o);
+        if (!best.empty() && Ls == best) continue;
+        float tot = 0.f;
+        for (int r = 0; r < std::max(1, reps); ++r) {
+            float ms = 0.f;
+            if (!batch_run(cs, Ls, o, s, &ms)) return 0;
+            tot += ms;
         }
-        if (best) g_batch_table[BatchTuneKey{c.dim, c.k}] = best;
+        if (best.empty() || tot < best_ms) {
+            best = Ls;
+            best_ms = tot;
+        }
     }
+    for (size_t i = 0; i < classes.size(); ++i) g_batch_table[BatchTuneKey{classes[i].dim, classes[i].k}] = best[i];
+    KNN_TRY(hipStreamSynchronize(s), "sync");
     return 1;
 }
 

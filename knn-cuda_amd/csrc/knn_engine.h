@@ -25,6 +25,24 @@ namespace knnhip {
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    // one owner per allocation: moves hand it over, copies do not exist
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     bool alloc(size_t bytes) {
         release();
         if (bytes == 0) bytes = 256;
@@ -51,6 +69,16 @@ size_t scan_lds_bytes(int kp, int waves);
 size_t scan_block_lds(int kp, int waves, int bmode, int oct);
 int scan_blocks_per_cu(int ks, int bmode, int waves, int kp, int oct);
 hipError_t launch_scan(const ScanArgs &a, int ks, int bmode, int waves, hipStream_t s);
+// Batched scan (knn_scan.hip): one persistent launch over work items (class,
+// task) of many classes, each with its own ScanArgs (device array cls) and
+// instance variant (scan_multi_variant: packed layout of 2*ks dimensions, or
+// nchunks chunks of 128).  next: a zeroed work counter.  lds: the largest
+// class's scan_multi_lds(kp).
+int scan_multi_variant(int dim, int *ks, int *nchunks);
+size_t scan_multi_lds(int kp);
+int scan_multi_blocks_per_cu(size_t lds);
+hipError_t launch_scan_multi(const ScanArgs *cls, const int *variant, const int2 *work, int nwork, unsigned *next,
+                             size_t lds, int grid, hipStream_t s);
 int scan_bf3_nch(int dim);
 size_t scan_bf3_lds_bytes(int kp);
 hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s);

@@ -661,21 +661,24 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_rest_kernel(
 // fill out[q * Pb ..] up to Pb entries (the rest -1); every other candidate
 // lowers lb_rest[q] to lb_f.  (No such T_q, fewer than kp members in all:
 // every candidate passes.)  At most kBallCand candidates.
+constexpr int kSelBytes = 28;  // LDS bytes per candidate of ball_select_kernel
 __global__ __launch_bounds__(64 * kBallWaves) void ball_select_kernel(
     const float *__restrict__ qpm, int pm_ld, int dim, int Q, int kp, const int *__restrict__ chosen, int P,
     const float *__restrict__ fcent, const double *__restrict__ frad, const int *__restrict__ fcnt,
     const int *__restrict__ fb_start, int *__restrict__ out, int Pb, float *__restrict__ lb_rest, int ncap) {
-    // per wave: reach (FP64), bound, ball and member count of ncap candidates
+    // per wave: reach and radius (FP64), bound, ball and member count of
+    // ncap candidates
     extern __shared__ __align__(16) unsigned char sel_smem[];
     __shared__ int blk_st[kBallWaves][kWave], blk_fl[kBallWaves][kWave];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = blockIdx.x * kBallWaves + wave;
     if (q >= Q) return;
-    unsigned char *wbase = sel_smem + (size_t)wave * ncap * 20;
+    unsigned char *wbase = sel_smem + (size_t)wave * ncap * kSelBytes;
     double *us = reinterpret_cast<double *>(wbase);
-    float *lbs = reinterpret_cast<float *>(wbase + (size_t)ncap * 8);
-    int *fs = reinterpret_cast<int *>(wbase + (size_t)ncap * 12);
-    int *ns = reinterpret_cast<int *>(wbase + (size_t)ncap * 16);
+    double *rh = reinterpret_cast<double *>(wbase + (size_t)ncap * 8);
+    float *lbs = reinterpret_cast<float *>(wbase + (size_t)ncap * 16);
+    int *fs = reinterpret_cast<int *>(wbase + (size_t)ncap * 20);
+    int *ns = reinterpret_cast<int *>(wbase + (size_t)ncap * 24);
     const long long o0 = (long long)q * Pb;
     const float *qr = qpm + (long long)q * pm_ld;
     // candidates, block by block: the blocks' ball ranges read by one lane
@@ -716,9 +719,11 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_select_kernel(
                 }
             }
             const int f = fb + i - off;
-            const bool live = frad[f] >= 0.0;
+            const double rho = frad[f];
+            const bool live = rho >= 0.0;
             fs[n + i] = live ? f : -1;
             ns[n + i] = live ? fcnt[f] : 0;
+            rh[n + i] = rho;
         }
         wave_sync();  // the next chunk rewrites blk_st / blk_fl
         n += total;
@@ -735,7 +740,7 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_select_kernel(
 #pragma unroll
         for (int j = 0; j < kDistGroup; ++j)
             if (j == lane && f[j] >= 0) {
-                const double d = sqrt(d2[j]), rho = frad[f[j]];
+                const double d = sqrt(d2[j]), rho = rh[base + j];
                 lb = ball_member_lb(d, rho);
                 u = d * (1.0 + 1e-12) + rho;
             }
@@ -1836,7 +1841,7 @@ hipError_t idx_ball_select(const float *qpm, int pm_ld, int dim, int Q, int kp, 
     const int ncap = (ncand + 7) / 8 * 8;
     if (ncap < 8 || ncap > kBallCand) return hipErrorInvalidValue;
     hipLaunchKernelGGL(ball_select_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves),
-                       (size_t)kBallWaves * ncap * 20, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt,
+                       (size_t)kBallWaves * ncap * kSelBytes, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt,
                        fb_start, out, Pb, lb_rest, ncap);
     return hipGetLastError();
 }

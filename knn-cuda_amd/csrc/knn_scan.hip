@@ -47,9 +47,11 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 // BMODE: where the B operands live (kBRegs, kBChunks, kBLds).
 // MAXW: most waves per block (4, or 8 where the heaps of large kp leave LDS
 // for one block per CU: eight waves then keep two per SIMD).
+// scan_f32_block: one block's task t (of a.tasks, or of the slice x query
+// block grid); scan_f32_kernel runs task blockIdx.x, the batched kernel
+// below the tasks it takes from a work queue.  smem: the block's dynamic LDS.
 template <int KS, int NBUF, int BMODE, int MAXW>
-__global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void scan_f32_block(const ScanArgs &a, const int t, unsigned char *smem) {
     constexpr int OCT = KS / 4;  // octets per chunk
     constexpr bool MULTI = BMODE == kBChunks;
     constexpr bool BLDS = BMODE == kBLds;
@@ -66,7 +68,6 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
 
     // One task per block: a reference range and up to 32 queries; its heaps
     // leave as output rows task_row .. task_row+31.
-    const int t = blockIdx.x;
     int r_begin, r_end, nq, task_row;
     int r_lo = 0;  // rows below r_lo (a task's tile-rounded start) take no part
     const int *qids = nullptr;  // query ids of the task (nullptr: consecutive from qfirst)
@@ -433,6 +434,86 @@ __global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs 
         a.part_val[obase + e] = hv[row * kps + j];
         a.part_idx[obase + e] = hi[row * kps + j];
     }
+}
+
+template <int KS, int NBUF, int BMODE, int MAXW>
+__global__ __launch_bounds__(MAXW * 64, 8 / MAXW) void scan_f32_kernel(ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    scan_f32_block<KS, NBUF, BMODE, MAXW>(a, blockIdx.x, smem);
+}
+
+// Batched scan: one persistent launch over the tasks of many problem
+// classes.  Class c's launch arguments are cls[c] (its own packed tiles,
+// queries, thresholds, task list and heap rows, its kp) and its instance
+// variant[c] (kMultiVariants: the padded dimension's k-steps in VGPRs for
+// 16, 32, 64 or 128 dimensions, else 128-dimension chunks).  A block takes
+// work items (class, task of that class) from the queue `next` until it is
+// empty; every block runs kMultiWaves waves whatever the class, with the
+// short candidate queues (scan_queue_cap(8)) so that the heaps of a large
+// kp leave two blocks per CU.  Larger items are queued first.
+constexpr int kMultiWaves = 4;
+__global__ __launch_bounds__(kMultiWaves * 64, 2) void scan_f32_multi_kernel(const ScanArgs *__restrict__ cls,
+                                                                             const int *__restrict__ variant,
+                                                                             const int2 *__restrict__ work,
+                                                                             int nwork, unsigned *__restrict__ next) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int item;
+    while (true) {
+        __syncthreads();  // the previous item's LDS (heaps, queues, item) is free
+        if (threadIdx.x == 0) item = (int)atomicAdd(next, 1u);
+        __syncthreads();
+        const int it = item;
+        if (it >= nwork) break;  // uniform over the block
+        const int2 w = work[it];
+        const ScanArgs a = cls[w.x];
+        switch (variant[w.x]) {
+            case 0: scan_f32_block<8, 2, kBRegs, 8>(a, w.y, smem); break;
+            case 1: scan_f32_block<16, 4, kBRegs, 8>(a, w.y, smem); break;
+            case 2: scan_f32_block<32, 4, kBRegs, 8>(a, w.y, smem); break;
+            case 3: scan_f32_block<64, 4, kBRegs, 8>(a, w.y, smem); break;
+            default: scan_f32_block<64, 2, kBChunks, 8>(a, w.y, smem); break;
+        }
+    }
+}
+
+int scan_multi_variant(int dim, int *ks, int *nchunks) {
+    static const int kRegsKS[] = {8, 16, 32, 64};
+    for (int v = 0; v < 4; ++v)
+        if (2 * kRegsKS[v] >= dim) {
+            *ks = kRegsKS[v];
+            *nchunks = 1;
+            return v;
+        }
+    *ks = 64;
+    *nchunks = (dim + 127) / 128;
+    return 4;
+}
+
+size_t scan_multi_lds(int kp) { return scan_heap_bytes(kp) + (size_t)kMultiWaves * scan_queue_bytes(8); }
+
+int scan_multi_blocks_per_cu(size_t lds) {
+    const void *fn = (const void *)scan_f32_multi_kernel;
+    if (lds > 65536 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return 0;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kMultiWaves * kWave, lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+hipError_t launch_scan_multi(const ScanArgs *cls, const int *variant, const int2 *work, int nwork, unsigned *next,
+                             size_t lds, int grid, hipStream_t s) {
+    if (grid < 1 || nwork < 1) return hipErrorInvalidValue;
+    const void *fn = (const void *)scan_f32_multi_kernel;
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(scan_f32_multi_kernel, dim3(grid), dim3(kMultiWaves * kWave), lds, s, cls, variant, work,
+                       nwork, next);
+    return hipGetLastError();
 }
 
 // Dimension k-steps with an instantiated single-chunk scan (padded dim = 2*KS).

@@ -1,6 +1,6 @@
-"""GPU tests of the batched form: every problem of a batch gets exactly the oracle's result (and
-exactly what a search of that problem alone returns), whatever the class mix, tile shape or
-tuning."""
+"""GPU tests of the batched form (one persistent scan launch over every class's tasks): every
+problem of a batch gets exactly the oracle's result (and exactly what a search of that problem
+alone returns), whatever the class mix, tile shape or tuning."""
 import json
 import os
 
@@ -48,7 +48,9 @@ def test_mixed_batch_equals_oracle():
     items = make_items(SHAPES, 900)
     outs = knnhip.search_batched(items)
     torch.cuda.synchronize()
-    say(f"plan {knnhip.last_plan()}; certificate failures {knnhip.batched_last_failures()}")
+    plan = knnhip.last_plan()
+    say(f"plan {plan}; certificate failures {knnhip.batched_last_failures()}")
+    assert plan["scan_launches"] == 1  # every class's tasks in one persistent scan launch
     for j, ((ref, qry, k), (d, i)) in enumerate(zip(items, outs)):
         gd, gi = oracle.knn_c(ref.cpu().numpy(), qry.cpu().numpy(), k)
         assert np.array_equal(i.cpu().numpy(), gi), f"problem {j} {SHAPES[j]}: indexes"
