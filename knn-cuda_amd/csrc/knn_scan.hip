@@ -195,10 +195,12 @@ __device__ __forceinline__ void scan_f32_block(const ScanArgs &a, const int t, u
     const float *__restrict__ qcol = a.query + q;
     float bq[KS];
     auto load_b = [&](int c) {
-        if (!MULTI && a.query_pm) {  // (the chunked instance has no VGPRs to spare)
+        if (a.query_pm) {
             // k-steps 4j .. 4j+3 read dims 8j + 4h .. +3 of the query: one
             // 16-byte piece of its row (whole pieces: the row stride is a
-            // multiple of 4 and the pads are zero)
+            // multiple of 4 and the pads are zero).  The chunked instance
+            // reloads these every chunk: KS / 4 row pieces instead of KS
+            // strided column reads.
             const float *row = a.query_pm + (long long)q * a.query_pm_ld;
 #pragma unroll
             for (int j = 0; j < KS / 4; ++j) {
