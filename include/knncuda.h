@@ -70,8 +70,9 @@ typedef struct knnhip_options {
     int timing;         /* 1: record per-stage device times (knnhip_last_stage_ms) */
     int precision;      /* 0: FP32 products; 1: bfloat16 three-term products (same exact
                            result; dim <= 192 and k + margin <= 107, else FP32); 2: index
-                           searches take single-pass bfloat16 products on block-centroid
-                           residuals (dim <= 256, else as 1), plain searches as 1 */
+                           searches take single-pass bfloat16 products on residuals against
+                           the centre of each scanned block or ball (dim <= 256, else as 1),
+                           plain searches as 1 */
     int block_queries;  /* FP32 scan shape: 0 chosen, 32 (reference tiles per wave), 128
                            (reference tiles shared through LDS; dim <= 192, k + margin <= 107) */
     int score_all;      /* index searches given scores: 0 the scores choose each query's
