@@ -199,11 +199,13 @@ int knnhip_index_stage_ms(const knnhip_index *index, float *ms, int n);
 /* Scan work of the last index search (read back from its task list, so call
  * it before the next index search): the (reference, query) pairs its MFMA
  * tiles executed (whole 128-reference steps times 32 query columns per task),
- * the pairs its tasks held, the bytes of reference tiles its tasks read, and
- * the products it used (0 FP32, 1 bfloat16 three-term, 2 bfloat16 on block
- * residuals).  Any pointer may be NULL.  For benchmarks (roofline figures). */
+ * the pairs its tasks held, the bytes of reference tiles its tasks read, the
+ * bytes of the distinct tiles they cover (each scan unit's 32-reference tiles
+ * once: what a scan that read every unit once would read), and the products
+ * it used (0 FP32, 1 bfloat16 three-term, 2 bfloat16 on centre residuals).
+ * Any pointer may be NULL.  For benchmarks (roofline figures). */
 int knnhip_index_last_work(const knnhip_index *index, long long *exec_pairs, long long *pairs,
-                           long long *tile_bytes, int *precision);
+                           long long *tile_bytes, long long *unit_bytes, int *precision);
 /* Scan units of the last index search: balls (1) or whole blocks (0), its
  * heap rows per query (units visited at most), and the index's ball count
  * when it knows the balls' position ranges (else 0).  Any pointer may be

@@ -53,6 +53,54 @@ struct BatchClass {
 
 // Workspaces of a batched run: one per class, and the run's work queue.
 std::vector<BatchWork> g_bwc;
+
+// Pinned host memory for the run's uploads (task lists, query lists, rows,
+// job tables): each array is copied into it and sent from there, a DMA
+// transfer, where a copy from pageable memory goes through a staging kernel.
+// Blocks are kept across runs; a run starts after the previous one's copies
+// are done.
+struct PinnedArena {
+    std::vector<std::pair<unsigned char *, size_t>> blocks;
+    size_t bi = 0, used = 0;
+    void reset() {
+        bi = 0;
+        used = 0;
+    }
+    unsigned char *take(size_t n) {
+        n = (n + 255) & ~(size_t)255;
+        while (bi < blocks.size() && used + n > blocks[bi].second) {
+            ++bi;
+            used = 0;
+        }
+        if (bi == blocks.size()) {
+            const size_t sz = std::max(n, (size_t)16 << 20);
+            void *p = nullptr;
+            if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+            blocks.emplace_back((unsigned char *)p, sz);
+            used = 0;
+        }
+        unsigned char *p = blocks[bi].first + used;
+        used += n;
+        return p;
+    }
+};
+PinnedArena g_pinned;
+
+// dst (device) <- bytes of src (host), through the pinned arena, on stream s.
+bool stage_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return true;
+    unsigned char *h = g_pinned.take(bytes);
+    if (!h) {
+        set_error("pinned allocation failed");
+        return false;
+    }
+    memcpy(h, src, bytes);
+    KNN_TRY(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s), "copy");
+    return true;
+}
 struct MultiWork {
     DevBuf cls, variant, work, next, nfail;
     int *nfail_host = nullptr;  // pinned, one per class
@@ -240,13 +288,11 @@ bool batch_class_prepare(const BatchClass &c, int L, const knnhip_options &o, Ba
         set_error("Memory allocation error");
         return false;
     }
-    KNN_TRY(hipMemcpyAsync(w.starts.p, starts.data(), (size_t)np * 4, hipMemcpyHostToDevice, s), "copy");
-    KNN_TRY(hipMemcpyAsync(w.counts.p, counts.data(), (size_t)np * 4, hipMemcpyHostToDevice, s), "copy");
-    KNN_TRY(hipMemcpyAsync(w.tasks.p, tasks.data(), (size_t)ntasks * sizeof(ScanTask),
-                           hipMemcpyHostToDevice, s),
-            "copy");
-    KNN_TRY(hipMemcpyAsync(w.qlist.p, qlist.data(), nrow * 4, hipMemcpyHostToDevice, s), "copy");
-    KNN_TRY(hipMemcpyAsync(w.rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, s), "copy");
+    if (!stage_copy(w.starts.p, starts.data(), (size_t)np * 4, s) ||
+        !stage_copy(w.counts.p, counts.data(), (size_t)np * 4, s) ||
+        !stage_copy(w.tasks.p, tasks.data(), (size_t)ntasks * sizeof(ScanTask), s) ||
+        !stage_copy(w.qlist.p, qlist.data(), nrow * 4, s) || !stage_copy(w.rows.p, rows.data(), rows.size() * 4, s))
+        return false;
     // references: per problem norms + packed tiles + point-major rows at its start
     KNN_TRY(hipMemsetAsync(w.rmax.p, 0, 4, s), "memset");
     KNN_TRY(hipMemsetAsync(w.norms.p, 0, (size_t)r_pad * 4, s), "memset");
@@ -272,12 +318,11 @@ bool batch_class_prepare(const BatchClass &c, int L, const knnhip_options &o, Ba
         set_error("Memory allocation error");
         return false;
     }
-    KNN_TRY(hipMemcpyAsync(w.rjobs.p, rjobs.data(), (size_t)np * sizeof(NormJob), hipMemcpyHostToDevice, s),
-            "copy");
-    KNN_TRY(hipMemcpyAsync(w.qjobs.p, qjobs.data(), (size_t)np * sizeof(NormJob), hipMemcpyHostToDevice, s),
-            "copy");
-    KNN_TRY(hipMemcpyAsync(w.rwg.p, rwg.data(), rwg.size() * sizeof(int2), hipMemcpyHostToDevice, s), "copy");
-    KNN_TRY(hipMemcpyAsync(w.qwg.p, qwg.data(), qwg.size() * sizeof(int2), hipMemcpyHostToDevice, s), "copy");
+    if (!stage_copy(w.rjobs.p, rjobs.data(), (size_t)np * sizeof(NormJob), s) ||
+        !stage_copy(w.qjobs.p, qjobs.data(), (size_t)np * sizeof(NormJob), s) ||
+        !stage_copy(w.rwg.p, rwg.data(), rwg.size() * sizeof(int2), s) ||
+        !stage_copy(w.qwg.p, qwg.data(), qwg.size() * sizeof(int2), s))
+        return false;
     NormArgs na{};
     na.dim = dim;
     na.oct = pl.oct;
@@ -417,8 +462,7 @@ bool batch_class_finish(const ClassRun &run, int nfail, const knnhip_options &o,
         set_error("Memory allocation error");
         return false;
     }
-    KNN_TRY(hipMemcpyAsync(w.outs.p, outs.data(), (size_t)np * sizeof(BatchOut), hipMemcpyHostToDevice, s),
-            "copy");
+    if (!stage_copy(w.outs.p, outs.data(), (size_t)np * sizeof(BatchOut), s)) return false;
     KNN_TRY(idx_batch_outputs(w.dist.as<float>(), w.idx.as<int>(), k, (int)Qtot, w.outs.as<BatchOut>(), np, s),
             "outputs");
     return true;
@@ -474,6 +518,10 @@ bool batch_run(const std::vector<const BatchClass *> &cs, const std::vector<int>
     const int n = (int)cs.size();
     if ((int)g_bwc.size() < n) g_bwc.resize(n);
     if (!batch_lanes_ready()) return false;
+    // the previous run's uploads have left the pinned arena
+    for (int l = 0; l < kBatchLanes; ++l) KNN_TRY(hipStreamSynchronize(g_lanes.stream[l]), "sync");
+    KNN_TRY(hipStreamSynchronize(s), "sync");
+    g_pinned.reset();
     std::vector<ClassRun> runs(n);
     std::vector<int> lane_of(n, 0);
     int nl = 0;
@@ -520,11 +568,10 @@ bool batch_run(const std::vector<const BatchClass *> &cs, const std::vector<int>
             set_error("Memory allocation error");
             return false;
         }
-        KNN_TRY(hipMemcpyAsync(g_mw.cls.p, args.data(), (size_t)ncls * sizeof(ScanArgs), hipMemcpyHostToDevice, s),
-                "copy");
-        KNN_TRY(hipMemcpyAsync(g_mw.variant.p, variants.data(), (size_t)ncls * 4, hipMemcpyHostToDevice, s), "copy");
-        KNN_TRY(hipMemcpyAsync(g_mw.work.p, work.data(), (size_t)nwork * sizeof(int2), hipMemcpyHostToDevice, s),
-                "copy");
+        if (!stage_copy(g_mw.cls.p, args.data(), (size_t)ncls * sizeof(ScanArgs), s) ||
+            !stage_copy(g_mw.variant.p, variants.data(), (size_t)ncls * 4, s) ||
+            !stage_copy(g_mw.work.p, work.data(), (size_t)nwork * sizeof(int2), s))
+            return false;
         KNN_TRY(hipMemsetAsync(g_mw.next.p, 0, 4, s), "memset");
         const int per_cu = scan_multi_blocks_per_cu(lds);
         if (per_cu < 1) {

@@ -688,10 +688,13 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         set_error("Memory allocation error");
         return false;
     }
-    hipEvent_t ev[6], evr[2];
+    hipEvent_t ev[6], evr[2], evk[2][2];
     const bool timing = o.timing != 0;
     if (timing)
         for (auto &e : evr) KNN_TRY(hipEventCreate(&e), "event");
+    if (timing)
+        for (auto &p : evk)
+            for (auto &e : p) KNN_TRY(hipEventCreate(&e), "event");
     if (timing)
         for (auto &e : ev) KNN_TRY(hipEventCreate(&e), "event");
     if (timing) KNN_TRY(hipEventRecord(ev[0], s), "event");
@@ -806,6 +809,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         sa.query_pm = w.qpm.as<float>();
         sa.query_pm_ld = pm_ld;
         const int waves = (double)Q * (pr[ph + 1] - pr[ph]) / nu >= kOneWaveQueries || balls ? 1 : 4;
+        if (timing) KNN_TRY(hipEventRecord(evk[ph][0], s), "event");
         if (nch_res) {
             sa.packed_res = ix->res_tiles.as<unsigned short>();
             sa.res_norm = ix->res_norm.as<float>();
@@ -828,6 +832,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         } else {
             KNN_TRY(launch_scan(sa, ks, bmode, waves, s), "scan kernel launch");
         }
+        if (timing) KNN_TRY(hipEventRecord(evk[ph][1], s), "event");
         tbase += max_t[ph];
     }
     if (timing) KNN_TRY(hipEventRecord(ev[3], s), "event");
@@ -911,6 +916,18 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         KNN_TRY(hipEventSynchronize(ev[5]), "event");
         ix->last_fail = nfail;
         for (int i = 0; i < 5; ++i) KNN_TRY(hipEventElapsedTime(&ix->last_ms[i], ev[i], ev[i + 1]), "event");
+        // the scan stage is its kernel launches alone; the task lists, threshold set-up and (two
+        // passes) the second ball choice between them count as the tasks stage
+        float scan_k = 0.f;
+        for (int ph = 0; ph < nphase; ++ph) {
+            float t = 0.f;
+            KNN_TRY(hipEventElapsedTime(&t, evk[ph][0], evk[ph][1]), "event");
+            scan_k += t;
+        }
+        ix->last_ms[1] += ix->last_ms[2] - scan_k;
+        ix->last_ms[2] = scan_k;
+        for (auto &p : evk)
+            for (auto &e : p) (void)hipEventDestroy(e);
         KNN_TRY(hipEventElapsedTime(&ix->rank_ms[0], ev[0], evr[0]), "event");
         KNN_TRY(hipEventElapsedTime(&ix->rank_ms[1], evr[0], evr[1]), "event");
         KNN_TRY(hipEventElapsedTime(&ix->rank_ms[2], evr[1], ev[1]), "event");
@@ -1110,7 +1127,7 @@ int knnhip_index_stage_ms(const knnhip_index *ix, float *ms, int n) {
 }
 
 int knnhip_index_last_work(const knnhip_index *ix, long long *exec_pairs, long long *pairs,
-                           long long *tile_bytes, int *precision) {
+                           long long *tile_bytes, long long *unit_bytes, int *precision) {
     std::lock_guard<std::mutex> lock(g_mu);
     g_last_error.clear();
     if (!ix) {
@@ -1118,22 +1135,35 @@ int knnhip_index_last_work(const knnhip_index *ix, long long *exec_pairs, long l
         return 0;
     }
     // the task list of the last search is still in the search workspace
-    long long ex = 0, pr = 0, tb = 0;
+    long long ex = 0, pr = 0, tb = 0, ub = 0;
     const long long nt = std::min<long long>(ix->last_tasks, (long long)(g_iw.tasks.n / sizeof(ScanTask)));
     if (nt > 0) {
         std::vector<ScanTask> tl(nt);
         KNN_TRY(hipMemcpy(tl.data(), g_iw.tasks.p, (size_t)nt * sizeof(ScanTask), hipMemcpyDeviceToHost), "copy");
+        // distinct tiles: every 32-reference tile of a unit the search scanned, once
+        std::vector<std::pair<int, int>> tiles;
         for (const ScanTask &t : tl) {
             if (t.qcount <= 0) continue;
             const long long steps = (t.ref_end - t.ref_begin + kRefsPerStep - 1) / kRefsPerStep;
             ex += steps * kRefsPerStep * kQueriesPerWave;
             pr += (long long)(t.ref_end - t.ref_lo) * t.qcount;
             tb += steps * kRefsPerStep * ix->last_ref_bytes;
+            tiles.emplace_back(t.ref_lo / kRefsPerTile, (t.ref_end + kRefsPerTile - 1) / kRefsPerTile);
         }
+        std::sort(tiles.begin(), tiles.end());
+        long long covered = 0;
+        int end = -1;
+        for (const auto &r : tiles) {
+            const int lo = std::max(r.first, end);
+            if (r.second > lo) covered += r.second - lo;
+            end = std::max(end, r.second);
+        }
+        ub = covered * kRefsPerTile * ix->last_ref_bytes;
     }
     if (exec_pairs) *exec_pairs = ex;
     if (pairs) *pairs = pr;
     if (tile_bytes) *tile_bytes = tb;
+    if (unit_bytes) *unit_bytes = ub;
     if (precision) *precision = ix->last_prec;
     return 1;
 }

@@ -167,7 +167,7 @@ def load() -> ctypes.CDLL:
     lib.knnhip_index_info.restype = ctypes.c_int
     lib.knnhip_index_stage_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     lib.knnhip_index_stage_ms.restype = ctypes.c_int
-    lib.knnhip_index_last_work.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 3 + [
+    lib.knnhip_index_last_work.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 4 + [
         ctypes.POINTER(ctypes.c_int)]
     lib.knnhip_index_last_work.restype = ctypes.c_int
     lib.knnhip_index_last_units.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3
@@ -478,13 +478,13 @@ class BlockIndex:
     STAGES = ("norms", "bounds", "pick", "tasks", "scan", "select", "fallback")
 
     def last_work(self) -> dict:
-        """Scan work of the last search made with timing=True (executed and held (reference,
-        query) pairs, tile bytes read, products used)."""
-        ex, pr, tb = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        """Scan work of the last search (executed and held (reference, query) pairs, tile bytes
+        its tasks read, bytes of the distinct tiles they cover, products used)."""
+        ex, pr, tb, ub = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
         prec = ctypes.c_int()
         load().knnhip_index_last_work(self._h, ctypes.byref(ex), ctypes.byref(pr), ctypes.byref(tb),
-                                      ctypes.byref(prec))
-        return {"exec_pairs": ex.value, "pairs": pr.value, "tile_bytes": tb.value,
+                                      ctypes.byref(ub), ctypes.byref(prec))
+        return {"exec_pairs": ex.value, "pairs": pr.value, "tile_bytes": tb.value, "unit_bytes": ub.value,
                 "precision": {0: "fp32", 1: "bf16x3", 2: "bf16-residual"}.get(prec.value, "?")}
 
     def last_units(self) -> dict:
