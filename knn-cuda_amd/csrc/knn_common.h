@@ -13,6 +13,14 @@ namespace knnhip {
 
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kQueriesPerWave = 32;  // one MFMA 32-column block of queries per wave
+// One-wave ball tasks with merged candidate lists: queries a task, pending
+// candidates a lane, candidate slot columns (both lane halves), and the
+// largest kp (a merge's union, kp entries plus both halves' candidates,
+// spans two passes of 64 lanes at most).
+constexpr int kMergeQueries = 16;
+constexpr int kMergeCand = 32;
+constexpr int kMergeSlots = 2 * kMergeQueries;
+constexpr int kMergeMaxKp = 2 * kWave - 2 * kMergeCand;
 constexpr int kRefsPerTile = 32;     // one MFMA 32-row block of references
 constexpr int kMTiles = 4;           // reference tiles per scan step (128 refs)
 constexpr int kRefsPerStep = kRefsPerTile * kMTiles;

@@ -85,9 +85,11 @@ hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s);
 int scan_bf3_task_nch(int dim);
 // tasks_per_group > 1 (with waves = 1): that many one-wave tasks per
 // workgroup, each with its own heaps, so a block's consecutive query groups
-// share a CU's caches.
+// share a CU's caches.  merge (waves = 1): tasks of up to kMergeQueries
+// queries keep sorted merged lists instead of heaps (kp <= kMergeMaxKp), and
+// write their rows from their list offset.
 hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
-                                int tasks_per_group = 1);
+                                int tasks_per_group = 1, bool merge = false);
 hipError_t launch_pack_bf3_rows(const float *pm, int ld, int n_pad, int nch, unsigned short *out,
                                 hipStream_t s);
 // Residual form of the block-index scan (knn_scan_resid.hip): chunks for a

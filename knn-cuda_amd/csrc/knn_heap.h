@@ -1,6 +1,7 @@
 // Device helpers shared by the scan kernels: accumulator row map, LDS
 // max-heaps of (key, index) pairs, the per-query LDS locks that guard them,
-// and the LDS layout of a 32-query scan block.
+// the LDS layout of a 32-query scan block, and the merged sorted lists of
+// one-wave ball tasks.
 #pragma once
 
 #include "knn_common.h"
@@ -121,6 +122,100 @@ __device__ __forceinline__ void drain_queue(const float *qv, const int *qr, cons
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// Sorted candidate lists of one-wave ball tasks.  A ball task holds a few
+// queries (2-6 on clustered data) and a step can bring tens of candidates per
+// query: a heap takes them one sift-down at a time, behind its lock, with
+// most of the wave's lanes idle.  Here each query keeps its kp best entries
+// as a sorted list, and a whole batch of candidates joins it at once: every
+// list entry and every candidate finds its rank in the union by counting
+// (one lane an element), and the kp smallest are written back in rank order.
+// A task holds up to kMergeQueries queries; between merges lane (col, h) of
+// a column col < kMergeQueries holds up to kMergeCand candidates of its
+// column in slot column h * kMergeQueries + col.
+// ---------------------------------------------------------------------------
+struct alignas(8) KeyRef {
+    float k;
+    int r;
+};
+
+__host__ __device__ constexpr size_t merge_lds_bytes(int kp) {
+    return ((size_t)kMergeQueries * kp + (size_t)kMergeCand * kMergeSlots) * sizeof(KeyRef);
+}
+
+// Merges every column's pending candidates into its list (list q at
+// lists + q * kp, nfill real entries, the rest (+inf, -1)).  cnt, nfill and
+// thr are per lane: lane (col, h) holds its own candidate count and its
+// column's fill and kp-th key (+inf while the list is short).  Wave-uniform
+// control flow; the whole wave ranks one query's union at a time, element e
+// of the union (list entries by position, then half 0's candidates, then
+// half 1's, each in append order) on lane e % 64 of pass e / 64.  Its rank
+// under (key, union position) counts the elements before it, every element's
+// key read across the wave with readlane (no LDS round trip per element);
+// the ranks are a permutation, the kp smallest land in order and the list
+// stays sorted.
+__device__ __forceinline__ float lane_key(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ void merge_lists(KeyRef *lists, const KeyRef *cand, int kp, int nq, int lane,
+                                            int col, int &cnt, int &nfill, float &thr) {
+    wave_sync();
+    for (int q = 0; q < nq; ++q) {
+        const int c0 = __builtin_amdgcn_readlane(cnt, q);
+        const int c1 = __builtin_amdgcn_readlane(cnt, q + kWave / 2);
+        if (c0 + c1 == 0) continue;
+        const int n = __builtin_amdgcn_readlane(nfill, q);
+        const int m = n + c0 + c1;
+        const bool two = m > kWave;
+        KeyRef *list = lists + (size_t)q * kp;
+        KeyRef x[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int e = p * kWave + lane, ce = e - n;
+            x[p].k = INFINITY;
+            x[p].r = -1;
+            if (p == 1 && !two) continue;
+            if (e < n)
+                x[p] = list[e];
+            else if (ce < c0)
+                x[p] = cand[ce * kMergeSlots + q];
+            else if (e < m)
+                x[p] = cand[(ce - c0) * kMergeSlots + kMergeQueries + q];
+        }
+        const float k0 = x[0].k, k1 = x[1].k;
+        int r0 = 0, r1 = 0;
+        if (!two) {
+            for (int j = 0; j < m; ++j) {
+                const float y = lane_key(k0, j);
+                r0 += (y < k0 || (y == k0 && j < lane)) ? 1 : 0;
+            }
+        } else {
+            // pass 1's elements follow every pass 0 element in union order
+            for (int j = 0; j < kWave; ++j) {
+                const float y = lane_key(k0, j);
+                r0 += (y < k0 || (y == k0 && j < lane)) ? 1 : 0;
+                r1 += y <= k1 ? 1 : 0;
+            }
+            for (int j = 0; j < m - kWave; ++j) {
+                const float y = lane_key(k1, j);
+                r0 += y < k0 ? 1 : 0;
+                r1 += (y < k1 || (y == k1 && j < lane)) ? 1 : 0;
+            }
+        }
+        wave_sync();  // every read of the list precedes the writes
+        if (lane < m && r0 < kp) list[r0] = x[0];
+        if (two && kWave + lane < m && r1 < kp) list[r1] = x[1];
+        wave_sync();
+        const int nn = m < kp ? m : kp;
+        if (col == q) {
+            nfill = nn;
+            if (nn == kp) thr = list[kp - 1].k;
+        }
+    }
+    cnt = 0;
+    wave_sync();
 }
 
 // Step-local cut of one query column.  A block's first steps can put most of

@@ -37,8 +37,9 @@ hipError_t idx_gather_columns(const float *src, long long ld, int dim, const int
                               float *dst, hipStream_t s);
 hipError_t idx_scatter_results(const float *dsub, const int *isub, int k, int n, const int *list,
                                int Q, float *dist, int *idx, hipStream_t s);
+// qpt: queries a task (the list starts are multiples of it)
 hipError_t idx_make_tasks(const int *qstart, const int *counts, int nb, const int *block_start,
-                          const int *block_count, ScanTask *tasks, int max_tasks, hipStream_t s);
+                          const int *block_count, ScanTask *tasks, int max_tasks, int qpt, hipStream_t s);
 hipError_t idx_ref_block(const int *block_start, const int *block_count, int nb, const int *perm,
                          int *out, hipStream_t s);
 hipError_t idx_row_norms(const float *pm, int ld, int n, int dim, float *out, hipStream_t s);

@@ -848,25 +848,26 @@ __global__ void scatter_results_kernel(const float *__restrict__ dsub, const int
 }
 
 // Step 3: tasks.  Unit b's pairs occupy qlist[qstart[b] .. qstart[b] +
-// count[b]) with qstart[b] a multiple of 32; task t covers list slots
-// [32t, 32t+32) (slot == output row), so tasks are enumerated from the
-// aligned slot space directly.  Units are blocks (tile-aligned starts) or
-// balls (any start: the task's tiles begin at the start rounded down to 32).
+// count[b]) with qstart[b] a multiple of qpt (queries a task: 32, or 16 for
+// merged-list tasks); task t covers list slots [qpt t, qpt t + qpt) (slot ==
+// output row), so tasks are enumerated from the aligned slot space directly.
+// Units are blocks (tile-aligned starts) or balls (any start: the task's
+// tiles begin at the start rounded down to 32).
 __global__ void make_tasks_kernel(const int *__restrict__ qstart, const int *__restrict__ counts,
                                   int nb, const int *__restrict__ block_start,
                                   const int *__restrict__ block_count, ScanTask *__restrict__ tasks,
-                                  int max_tasks) {
+                                  int max_tasks, int qpt) {
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
         const int c = counts[b];
-        for (int j = 0; j < c; j += kQueriesPerWave) {
-            const int t = (qstart[b] + j) / kQueriesPerWave;
+        for (int j = 0; j < c; j += qpt) {
+            const int t = (qstart[b] + j) / qpt;
             if (t >= max_tasks) continue;
             ScanTask tk;
             tk.ref_lo = block_start[b];
             tk.ref_begin = block_start[b] / kRefsPerTile * kRefsPerTile;
             tk.ref_end = block_start[b] + block_count[b];
             tk.qoff = qstart[b] + j;
-            tk.qcount = min(kQueriesPerWave, c - j);
+            tk.qcount = min(qpt, c - j);
             tk.unit = b;
             tasks[t] = tk;
         }
@@ -1751,11 +1752,12 @@ hipError_t idx_scatter_results(const float *dsub, const int *isub, int k, int n,
     return hipGetLastError();
 }
 hipError_t idx_make_tasks(const int *qstart, const int *counts, int nb, const int *block_start,
-                          const int *block_count, ScanTask *tasks, int max_tasks, hipStream_t s) {
+                          const int *block_count, ScanTask *tasks, int max_tasks, int qpt, hipStream_t s) {
+    if (qpt < 1 || qpt > kQueriesPerWave) return hipErrorInvalidValue;
     hipLaunchKernelGGL(clear_tasks_kernel, dim3(grid_for(max_tasks)), dim3(256), 0, s, tasks,
                        max_tasks);
     hipLaunchKernelGGL(make_tasks_kernel, dim3(grid_for(nb)), dim3(256), 0, s, qstart, counts, nb,
-                       block_start, block_count, tasks, max_tasks);
+                       block_start, block_count, tasks, max_tasks, qpt);
     return hipGetLastError();
 }
 hipError_t idx_ref_block(const int *block_start, const int *block_count, int nb, const int *perm,

@@ -668,10 +668,13 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     // thresholds the first phase left.  One-pass ball probes: one phase.
     const int nphase = one_pass ? 1 : Pb > 1 ? 2 : 1;
     const int pr[3] = {0, one_pass ? Pb : 1, Pb};
+    // one-pass ball tasks of the three-term form keep merged candidate lists
+    // (tasks of up to kMergeQueries queries); other tasks, heaps (32 queries)
+    const bool merge = one_pass && nch_bf && !nch_res && kp <= kMergeMaxKp;
+    const int qpt = merge ? kMergeQueries : kQueriesPerWave;
     int max_t[2] = {0, 0};
     for (int ph = 0; ph < nphase; ++ph)
-        max_t[ph] = (int)(((long long)Q * (pr[ph + 1] - pr[ph]) + kQueriesPerWave - 1) /
-                          kQueriesPerWave) + nu;
+        max_t[ph] = (int)(((long long)Q * (pr[ph + 1] - pr[ph]) + qpt - 1) / qpt) + nu;
     const int max_tasks = max_t[0] + max_t[1];
     if (!w.need(w.qnorm, (size_t)Q * 4) || !w.need(w.qpm, (size_t)Q * pm_ld * 4) ||
         !w.need(w.lb, (size_t)Q * nb * 4) ||
@@ -679,12 +682,12 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         (balls && !w.need(w.bchosen, (size_t)npairs * 4)) ||
         !w.need(w.pcount, (size_t)nu * 4) || !w.need(w.qstart, (size_t)(nu + 1) * 4) ||
         !w.need(w.cursor, (size_t)(nu + 1) * 4) ||
-        !w.need(w.qlist, (size_t)max_tasks * kQueriesPerWave * 4) ||
+        !w.need(w.qlist, (size_t)max_tasks * qpt * 4) ||
         !w.need(w.rows, (size_t)npairs * 4) || !w.need(w.tasks, (size_t)max_tasks * sizeof(ScanTask)) ||
-        !w.need(w.pval, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
-        !w.need(w.pidx, (size_t)max_tasks * kQueriesPerWave * kp * 4) ||
+        !w.need(w.pval, (size_t)max_tasks * qpt * kp * 4) ||
+        !w.need(w.pidx, (size_t)max_tasks * qpt * kp * 4) ||
         !w.need(w.theta, (size_t)Q * 4) || !w.need(w.fb_count, 4) || !w.need(w.fb_list, (size_t)Q * 4) ||
-        (nch_res && !w.need(w.resrow, (size_t)max_tasks * kQueriesPerWave * 16))) {
+        (nch_res && !w.need(w.resrow, (size_t)max_tasks * qpt * 16))) {
         set_error("Memory allocation error");
         return false;
     }
@@ -770,10 +773,10 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                     "other balls");
         }
         // (query, unit) pairs of this phase -> per-unit query lists -> tasks
-        const int base = tbase * kQueriesPerWave;
+        const int base = tbase * qpt;
         KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nu * 4, s), "memset");
         KNN_TRY(idx_pair_count(units, Q, Pb, pr[ph], pr[ph + 1], w.pcount.as<int>(), s), "pairs");
-        KNN_TRY(idx_prefix(w.pcount.as<int>(), nu, kQueriesPerWave, w.qstart.as<int>(), nullptr, s),
+        KNN_TRY(idx_prefix(w.pcount.as<int>(), nu, qpt, w.qstart.as<int>(), nullptr, s),
                 "prefix");
         KNN_TRY(hipMemcpyAsync(w.cursor.p, w.qstart.p, (size_t)nu * 4, hipMemcpyDeviceToDevice, s),
                 "copy");
@@ -781,7 +784,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                                  w.rows.as<int>(), base, s),
                 "pairs");
         KNN_TRY(idx_make_tasks(w.qstart.as<int>(), w.pcount.as<int>(), nu, unit_start, unit_count,
-                               w.tasks.as<ScanTask>() + tbase, max_t[ph], s),
+                               w.tasks.as<ScanTask>() + tbase, max_t[ph], qpt, s),
                 "tasks");
         ScanArgs sa{};
         sa.packed = ix->packed.as<float>();
@@ -827,7 +830,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
             // one-wave tasks go kTasksPerGroup to a workgroup: a block's
             // consecutive query groups then read its tiles through one CU
             KNN_TRY(launch_scan_bf3_task(sa, nch_bf, waves, scan_lds_bytes(kp, waves), s,
-                                         waves == 1 ? kTasksPerGroup : 1),
+                                         waves == 1 ? kTasksPerGroup : 1, merge),
                     "scan kernel launch");
         } else {
             KNN_TRY(launch_scan(sa, ks, bmode, waves, s), "scan kernel launch");

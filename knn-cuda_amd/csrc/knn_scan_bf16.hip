@@ -413,11 +413,16 @@ hipError_t launch_scan_lds(const ScanArgs &a, int nch, bool bf3, hipStream_t s) 
 // own heaps and queue in its own LDS region and no workgroup barrier: the
 // tasks next to each other in the list, which hold the same block's next
 // query groups, run on one CU and read the block's tiles once from HBM.
+// MERGE (one-wave tasks of up to kMergeQueries queries, kp <= kMergeMaxKp):
+// sorted per-query lists in place of the heaps and the queue; a step's
+// candidates join the lists in one merge (ball tasks, whose few queries each
+// see tens of candidates a step).  The task's rows start at its list offset.
 // ---------------------------------------------------------------------------
-template <int NCH, int NB, bool IND>
+template <int NCH, int NB, bool IND, bool MERGE>
 __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem_all[];
     static_assert(NCH % NB == 0 && NB >= 2, "ring position must repeat every step");
+    static_assert(IND || !MERGE, "merged lists are per wave");
     const int nwaves = blockDim.x >> 6;
     const int wave_id = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
@@ -426,15 +431,18 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
     const int tix = IND ? blockIdx.x * nwaves + wave_id : blockIdx.x;
     if (IND && tix >= a.ntasks) return;  // (no workgroup barrier in this form)
     const int kp = a.kp, kps = heap_stride(kp);
-    unsigned char *smem = smem_all + (IND ? (size_t)wave_id * (scan_heap_bytes(kp) + kQueueBytes) : 0);
+    const size_t wave_lds = MERGE ? merge_lds_bytes(kp) : scan_heap_bytes(kp) + kQueueBytes;
+    unsigned char *smem = smem_all + (IND ? (size_t)wave_id * wave_lds : 0);
 
     const ScanTask tk = a.tasks[tix];
-    const int r_begin = tk.ref_begin, r_end = tk.ref_end, nq = tk.qcount;
+    const int r_begin = tk.ref_begin, r_end = tk.ref_end;
+    // (merged lists: the task list holds at most kMergeQueries queries a task)
+    const int nq = MERGE ? min(tk.qcount, kMergeQueries) : tk.qcount;
     if (nq <= 0) return;  // empty task: uniform over the block (IND: over the wave)
     // columns past the task's queries compute query 0's keys and take nothing
     const bool qvalid = col < nq;
     const int q = a.qlist[tk.qoff + (qvalid ? col : 0)];
-    const int task_row = tix * kQueriesPerWave;
+    const int task_row = MERGE ? tk.qoff : tix * kQueriesPerWave;
     const int tid = IND ? lane : threadIdx.x, tstride = IND ? kWave : blockDim.x;
     auto task_sync = [&]() {
         if constexpr (IND) {
@@ -449,11 +457,23 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
     float *hv = reinterpret_cast<float *>(smem);
     int *hi = reinterpret_cast<int *>(smem + (size_t)kQueriesPerWave * kps * sizeof(float));
     int *lock = hi + (size_t)kQueriesPerWave * kps;
-    for (int e = tid; e < kQueriesPerWave * kps; e += tstride) {
-        hv[e] = INFINITY;
-        hi[e] = -1;
+    KeyRef *lists = reinterpret_cast<KeyRef *>(smem);
+    KeyRef *cand = lists + (size_t)kMergeQueries * kp;
+    int mcount = 0, mfill = 0;  // (merged lists) this lane's pending candidates, its column's fill
+    float mthr = INFINITY;      // (merged lists) its column's kp-th key
+    const int mslot = h * kMergeQueries + col;
+    if constexpr (MERGE) {
+        for (int e = lane; e < nq * kp; e += kWave) {
+            lists[e].k = INFINITY;
+            lists[e].r = -1;
+        }
+    } else {
+        for (int e = tid; e < kQueriesPerWave * kps; e += tstride) {
+            hv[e] = INFINITY;
+            hi[e] = -1;
+        }
+        if (tid < kQueriesPerWave) lock[tid] = 0;
     }
-    if (tid < kQueriesPerWave) lock[tid] = 0;
     task_sync();
     float *myv = hv + col * kps;
     float theta = INFINITY;
@@ -585,8 +605,9 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
         for (int mt = 0; mt < kMTiles; ++mt) voff[mt] = voff_next[mt];
 
         // Epilogue (as the FP32 scan): keys below the freshest threshold go
-        // to the wave's queue, drained into the shared heaps when it fills.
-        theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
+        // to the wave's queue, drained into the shared heaps when it fills
+        // (merged lists: to the lane's candidate slots, merged every step).
+        theta = fminf(fminf(theta, ordered_to_float(tq)), MERGE ? mthr : shared_root(myv));
         if (r0 + kRefsPerStep > r_end || r0 < tk.ref_lo) {
             // rows outside the task take no part below
 #pragma unroll
@@ -604,7 +625,49 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
         for (int mt = 0; mt < kMTiles; ++mt)
 #pragma unroll
             for (int i = 0; i < 16; ++i) kmin = fminf(kmin, acc[mt][i]);
-        if (__ballot(qvalid && kmin < theta)) {
+        if (MERGE && __ballot(qvalid && kmin < theta)) {
+            int ncand = 0;
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
+            if (!qvalid) ncand = 0;
+            // a young list: only a step-local cut (kp of the column's keys below it)
+            const int npair = ncand + __shfl_xor(ncand, 32);
+            float cut = theta;
+            if (__ballot(npair >= 2 * kp)) cut = step_cut(acc, theta, kp, npair >= 2 * kp);
+#pragma unroll
+            for (int mt = 0; mt < kMTiles; ++mt) {
+                int ct = 0;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) ct += acc[mt][i] < cut ? 1 : 0;
+                if (!qvalid) ct = 0;
+                if (__ballot(ct > 0) == 0) continue;
+                if (__ballot(mcount + ct > kMergeCand)) {
+                    merge_lists(lists, cand, kp, nq, lane, col, mcount, mfill, mthr);
+                    cut = fminf(cut, mthr);
+                }
+                if (qvalid) {
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        if (acc[mt][i] < cut) {
+                            KeyRef e;
+                            e.k = acc[mt][i];
+                            e.r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                            cand[mcount * kMergeSlots + mslot] = e;
+                            ++mcount;
+                        }
+                    }
+                }
+            }
+            merge_lists(lists, cand, kp, nq, lane, col, mcount, mfill, mthr);
+            theta = fminf(theta, mthr);
+            if (h == 0 && qvalid && theta < published) {
+                published = theta;
+                __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (!MERGE && __ballot(qvalid && kmin < theta)) {
             int ncand = 0;
 #pragma unroll
             for (int mt = 0; mt < kMTiles; ++mt)
@@ -658,6 +721,17 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
             }
         }
     }
+    if constexpr (MERGE) {
+        // (every step with candidates merged them; the lists are final)
+        task_sync();
+        const long long obase = (long long)task_row * kp;
+        for (int e = lane; e < nq * kp; e += kWave) {
+            const KeyRef v = lists[e];
+            a.part_val[obase + e] = v.k;
+            a.part_idx[obase + e] = v.r;
+        }
+        return;
+    }
     drain_queue(qv, qr, qc, qcount, hv, hi, lock, kps, kp, lane);
     theta = fminf(theta, shared_root(myv));
     if (h == 0 && qvalid && theta < published)
@@ -691,38 +765,43 @@ int scan_bf3_task_nch(int dim) {
 }
 
 // Task lists only (a.tasks, one block of `waves` waves per task), with the
-// LDS of the FP32 scan's 32-query blocks of that many waves.
+// LDS of the FP32 scan's 32-query blocks of that many waves.  merge: one-wave
+// tasks of up to kMergeQueries queries with merged lists (kp <= kMergeMaxKp).
 hipError_t launch_scan_bf3_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
-                                int tasks_per_group) {
+                                int tasks_per_group, bool merge) {
     if (!a.tasks || !a.packed_bf) return hipErrorInvalidValue;
     if (tasks_per_group > 1 && (waves != 1 || tasks_per_group > 4)) return hipErrorInvalidValue;
-    const size_t lds_ind = (size_t)tasks_per_group * (scan_heap_bytes(a.kp) + kQueueBytes);
+    if (merge && (waves != 1 || tasks_per_group < 1 || a.kp > kMergeMaxKp)) return hipErrorInvalidValue;
+    const size_t lds_ind = (size_t)tasks_per_group * (merge ? merge_lds_bytes(a.kp) : scan_heap_bytes(a.kp) + kQueueBytes);
+    const dim3 grid_ind((a.ntasks + tasks_per_group - 1) / tasks_per_group), block_ind(tasks_per_group * kWave);
+#define LAUNCH_IND(N, B, M)                                                                     \
+    {                                                                                           \
+        const void *fi = (const void *)scan_bf3_task_kernel<N, B, true, M>;                     \
+        if (lds_ind > 65536) {                                                                  \
+            hipError_t e = hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                               (int)lds_ind);                                   \
+            if (e != hipSuccess) return e;                                                      \
+        }                                                                                       \
+        hipLaunchKernelGGL((scan_bf3_task_kernel<N, B, true, M>), grid_ind, block_ind, lds_ind, s, a); \
+        return hipGetLastError();                                                               \
+    }
 #define X(N, B)                                                                                 \
     if (nch == N) {                                                                             \
-        if (tasks_per_group > 1) {                                                              \
-            const void *fi = (const void *)scan_bf3_task_kernel<N, B, true>;                    \
-            if (lds_ind > 65536) {                                                              \
-                hipError_t e = hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                                   (int)lds_ind);                               \
-                if (e != hipSuccess) return e;                                                  \
-            }                                                                                   \
-            hipLaunchKernelGGL((scan_bf3_task_kernel<N, B, true>),                              \
-                               dim3((a.ntasks + tasks_per_group - 1) / tasks_per_group),        \
-                               dim3(tasks_per_group * kWave), lds_ind, s, a);                   \
-            return hipGetLastError();                                                           \
-        }                                                                                       \
-        const void *fn = (const void *)scan_bf3_task_kernel<N, B, false>;                       \
+        if (merge) LAUNCH_IND(N, B, true)                                                       \
+        if (tasks_per_group > 1) LAUNCH_IND(N, B, false)                                        \
+        const void *fn = (const void *)scan_bf3_task_kernel<N, B, false, false>;                \
         if (lds > 65536) {                                                                      \
             hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                                (int)lds);                                       \
             if (e != hipSuccess) return e;                                                      \
         }                                                                                       \
-        hipLaunchKernelGGL((scan_bf3_task_kernel<N, B, false>), dim3(a.ntasks), dim3(waves * kWave), lds, s, \
-                           a);                                                                  \
+        hipLaunchKernelGGL((scan_bf3_task_kernel<N, B, false, false>), dim3(a.ntasks), dim3(waves * kWave), \
+                           lds, s, a);                                                          \
         return hipGetLastError();                                                               \
     }
     KNN_BF3_TASK_INSTANCES(X)
 #undef X
+#undef LAUNCH_IND
     return hipErrorInvalidValue;
 }
 
