@@ -103,7 +103,7 @@ hipError_t launch_resid_build(const float *pm, int ld, int dim, int nch, const i
                               const int *unit_count, int nunits, float *cent, unsigned short *tiles, float *rnorm,
                               float *rho, hipStream_t s);
 hipError_t launch_scan_res_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
-                                int tasks_per_group = 1);
+                                int tasks_per_group = 1, bool merge = false);
 hipError_t launch_norms(const NormArgs &a, hipStream_t s);
 hipError_t launch_norms_jobs(const NormArgs &a, const NormJob *jobs, const int2 *wg, int nwg,
                              hipStream_t s);

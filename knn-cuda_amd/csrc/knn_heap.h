@@ -259,4 +259,52 @@ __device__ __forceinline__ float step_cut(const V (&acc)[NT], float theta, int k
     return need ? hi : theta;
 }
 
+// One step's candidates of a merged-list task (acc: the step's kMTiles
+// 32 x 32 key tiles, rows outside the task already +inf): keys below theta
+// (columns with many, as with a young list: below the step-local cut) go to
+// the lane's candidate slots, merged into the lists when a tile could
+// overflow the slots and at the step's end; theta then takes the column's
+// kp-th key.
+template <class V>
+__device__ __forceinline__ void merge_step(const V (&acc)[kMTiles], float &theta, int kp, bool qvalid, int r0,
+                                           int h, int col, int lane, int nq, KeyRef *lists, KeyRef *cand,
+                                           int &mcount, int &mfill, float &mthr) {
+    int ncand = 0;
+#pragma unroll
+    for (int mt = 0; mt < kMTiles; ++mt)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
+    if (!qvalid) ncand = 0;
+    const int npair = ncand + __shfl_xor(ncand, 32);
+    float cut = theta;
+    if (__ballot(npair >= 2 * kp)) cut = step_cut(acc, theta, kp, npair >= 2 * kp);
+    const int mslot = h * kMergeQueries + col;
+#pragma unroll
+    for (int mt = 0; mt < kMTiles; ++mt) {
+        int ct = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ct += acc[mt][i] < cut ? 1 : 0;
+        if (!qvalid) ct = 0;
+        if (__ballot(ct > 0) == 0) continue;
+        if (__ballot(mcount + ct > kMergeCand)) {
+            merge_lists(lists, cand, kp, nq, lane, col, mcount, mfill, mthr);
+            cut = fminf(cut, mthr);
+        }
+        if (qvalid) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                if (acc[mt][i] < cut) {
+                    KeyRef e;
+                    e.k = acc[mt][i];
+                    e.r = r0 + kRefsPerTile * mt + acc_row(i, h);
+                    cand[mcount * kMergeSlots + mslot] = e;
+                    ++mcount;
+                }
+            }
+        }
+    }
+    merge_lists(lists, cand, kp, nq, lane, col, mcount, mfill, mthr);
+    theta = fminf(theta, mthr);
+}
+
 }  // namespace knnhip

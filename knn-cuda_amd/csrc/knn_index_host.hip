@@ -668,9 +668,9 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     // thresholds the first phase left.  One-pass ball probes: one phase.
     const int nphase = one_pass ? 1 : Pb > 1 ? 2 : 1;
     const int pr[3] = {0, one_pass ? Pb : 1, Pb};
-    // one-pass ball tasks of the three-term form keep merged candidate lists
+    // one-pass ball tasks of the bfloat16 forms keep merged candidate lists
     // (tasks of up to kMergeQueries queries); other tasks, heaps (32 queries)
-    const bool merge = one_pass && nch_bf && !nch_res && kp <= kMergeMaxKp;
+    const bool merge = one_pass && (nch_bf || nch_res) && kp <= kMergeMaxKp;
     const int qpt = merge ? kMergeQueries : kQueriesPerWave;
     int max_t[2] = {0, 0};
     for (int ph = 0; ph < nphase; ++ph)
@@ -823,7 +823,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
             sa.res_norm_up = res_norm_up(dim);
             sa.res_norm_dn = res_norm_dn(dim);
             KNN_TRY(launch_scan_res_task(sa, nch_res, waves, scan_lds_bytes(kp, waves), s,
-                                         waves == 1 ? kTasksPerGroup : 1),
+                                         waves == 1 ? kTasksPerGroup : 1, merge),
                     "scan kernel launch");
         } else if (nch_bf) {
             sa.packed_bf = ix->packed_bf.as<unsigned short>();

@@ -461,7 +461,6 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
     KeyRef *cand = lists + (size_t)kMergeQueries * kp;
     int mcount = 0, mfill = 0;  // (merged lists) this lane's pending candidates, its column's fill
     float mthr = INFINITY;      // (merged lists) its column's kp-th key
-    const int mslot = h * kMergeQueries + col;
     if constexpr (MERGE) {
         for (int e = lane; e < nq * kp; e += kWave) {
             lists[e].k = INFINITY;
@@ -626,42 +625,7 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
 #pragma unroll
             for (int i = 0; i < 16; ++i) kmin = fminf(kmin, acc[mt][i]);
         if (MERGE && __ballot(qvalid && kmin < theta)) {
-            int ncand = 0;
-#pragma unroll
-            for (int mt = 0; mt < kMTiles; ++mt)
-#pragma unroll
-                for (int i = 0; i < 16; ++i) ncand += acc[mt][i] < theta ? 1 : 0;
-            if (!qvalid) ncand = 0;
-            // a young list: only a step-local cut (kp of the column's keys below it)
-            const int npair = ncand + __shfl_xor(ncand, 32);
-            float cut = theta;
-            if (__ballot(npair >= 2 * kp)) cut = step_cut(acc, theta, kp, npair >= 2 * kp);
-#pragma unroll
-            for (int mt = 0; mt < kMTiles; ++mt) {
-                int ct = 0;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) ct += acc[mt][i] < cut ? 1 : 0;
-                if (!qvalid) ct = 0;
-                if (__ballot(ct > 0) == 0) continue;
-                if (__ballot(mcount + ct > kMergeCand)) {
-                    merge_lists(lists, cand, kp, nq, lane, col, mcount, mfill, mthr);
-                    cut = fminf(cut, mthr);
-                }
-                if (qvalid) {
-#pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        if (acc[mt][i] < cut) {
-                            KeyRef e;
-                            e.k = acc[mt][i];
-                            e.r = r0 + kRefsPerTile * mt + acc_row(i, h);
-                            cand[mcount * kMergeSlots + mslot] = e;
-                            ++mcount;
-                        }
-                    }
-                }
-            }
-            merge_lists(lists, cand, kp, nq, lane, col, mcount, mfill, mthr);
-            theta = fminf(theta, mthr);
+            merge_step(acc, theta, kp, qvalid, r0, h, col, lane, nq, lists, cand, mcount, mfill, mthr);
             if (h == 0 && qvalid && theta < published) {
                 published = theta;
                 __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,

@@ -120,11 +120,15 @@ __global__ __launch_bounds__(256) void resid_build_kernel(const float *__restric
 // structure of the bfloat16 three-term task scan with one product per 16
 // dimensions on the residual tiles.  NB: chunks in the register ring (loads
 // run NB - 1 chunks ahead).  IND: every wave is a task of its own, with its
-// own heaps and queue and no workgroup barrier.
-template <int NCH, int NB, bool IND>
+// own heaps and queue and no workgroup barrier.  MERGE (one-wave tasks of up
+// to kMergeQueries queries, kp <= kMergeMaxKp): sorted merged candidate
+// lists in place of the heaps and queue; rows start at the task's list
+// offset.
+template <int NCH, int NB, bool IND, bool MERGE>
 __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem_all[];
     static_assert(NCH % NB == 0 && NB >= 2, "ring position must repeat every step");
+    static_assert(IND || !MERGE, "merged lists are per wave");
     const int nwaves = blockDim.x >> 6;
     const int wave_id = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
@@ -132,14 +136,17 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
     const int tix = IND ? blockIdx.x * nwaves + wave_id : blockIdx.x;
     if (IND && tix >= a.ntasks) return;  // (no workgroup barrier in this form)
     const int kp = a.kp, kps = heap_stride(kp);
-    unsigned char *smem = smem_all + (IND ? (size_t)wave_id * (scan_heap_bytes(kp) + kQueueBytes) : 0);
+    const size_t wave_lds = MERGE ? merge_lds_bytes(kp) : scan_heap_bytes(kp) + kQueueBytes;
+    unsigned char *smem = smem_all + (IND ? (size_t)wave_id * wave_lds : 0);
 
     const ScanTask tk = a.tasks[tix];
-    const int r_begin = tk.ref_begin, r_end = tk.ref_end, nq = tk.qcount;
+    const int r_begin = tk.ref_begin, r_end = tk.ref_end;
+    // (merged lists: the task list holds at most kMergeQueries queries a task)
+    const int nq = MERGE ? min(tk.qcount, kMergeQueries) : tk.qcount;
     if (nq <= 0) return;  // empty task: uniform over the block (IND: over the wave)
     const bool qvalid = col < nq;
     const int q = a.qlist[tk.qoff + (qvalid ? col : 0)];
-    const int task_row = tix * kQueriesPerWave;
+    const int task_row = MERGE ? tk.qoff : tix * kQueriesPerWave;
     const int tid = IND ? lane : threadIdx.x, tstride = IND ? kWave : blockDim.x;
     auto task_sync = [&]() {
         if constexpr (IND) {
@@ -154,11 +161,22 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
     float *hv = reinterpret_cast<float *>(smem);
     int *hi = reinterpret_cast<int *>(smem + (size_t)kQueriesPerWave * kps * sizeof(float));
     int *lock = hi + (size_t)kQueriesPerWave * kps;
-    for (int e = tid; e < kQueriesPerWave * kps; e += tstride) {
-        hv[e] = INFINITY;
-        hi[e] = -1;
+    KeyRef *lists = reinterpret_cast<KeyRef *>(smem);
+    KeyRef *cand = lists + (size_t)kMergeQueries * kp;
+    int mcount = 0, mfill = 0;  // (merged lists) this lane's pending candidates, its column's fill
+    float mthr = INFINITY;      // (merged lists) its column's kp-th key
+    if constexpr (MERGE) {
+        for (int e = lane; e < nq * kp; e += kWave) {
+            lists[e].k = INFINITY;
+            lists[e].r = -1;
+        }
+    } else {
+        for (int e = tid; e < kQueriesPerWave * kps; e += tstride) {
+            hv[e] = INFINITY;
+            hi[e] = -1;
+        }
+        if (tid < kQueriesPerWave) lock[tid] = 0;
     }
-    if (tid < kQueriesPerWave) lock[tid] = 0;
     task_sync();
     float *myv = hv + col * kps;
     float theta = INFINITY;
@@ -294,8 +312,9 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
         for (int mt = 0; mt < kMTiles; ++mt) voff[mt] = voff_next[mt];
 
         // Epilogue (as the three-term task scan): keys below the freshest
-        // threshold go to the wave's queue, drained into the heaps.
-        theta = fminf(fminf(theta, ordered_to_float(tq)), shared_root(myv));
+        // threshold go to the wave's queue, drained into the heaps (merged
+        // lists: to the lane's candidate slots, merged every step).
+        theta = fminf(fminf(theta, ordered_to_float(tq)), MERGE ? mthr : shared_root(myv));
         if (r0 + kRefsPerStep > r_end || r0 < tk.ref_lo) {
             // rows outside the task take no part below
 #pragma unroll
@@ -311,7 +330,14 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
         for (int mt = 0; mt < kMTiles; ++mt)
 #pragma unroll
             for (int i = 0; i < 16; ++i) kmin = fminf(kmin, acc[mt][i]);
-        if (__ballot(qvalid && kmin < theta)) {
+        if (MERGE && __ballot(qvalid && kmin < theta)) {
+            merge_step(acc, theta, kp, qvalid, r0, h, col, lane, nq, lists, cand, mcount, mfill, mthr);
+            if (h == 0 && qvalid && theta < published) {
+                published = theta;
+                __hip_atomic_fetch_min(theta_q, float_to_ordered(theta), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (!MERGE && __ballot(qvalid && kmin < theta)) {
             int ncand = 0;
 #pragma unroll
             for (int mt = 0; mt < kMTiles; ++mt)
@@ -361,6 +387,17 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
                 }
             }
         }
+    }
+    if constexpr (MERGE) {
+        // (every step with candidates merged them; the lists are final)
+        task_sync();
+        const long long obase = (long long)task_row * kp;
+        for (int e = lane; e < nq * kp; e += kWave) {
+            const KeyRef v = lists[e];
+            a.part_val[obase + e] = v.k;
+            a.part_idx[obase + e] = v.r;
+        }
+        return;
     }
     drain_queue(qv, qr, qc, qcount, hv, hi, lock, kps, kp, lane);
     theta = fminf(theta, shared_root(myv));
@@ -427,37 +464,41 @@ hipError_t launch_resid_build(const float *pm, int ld, int dim, int nch, const i
 }
 
 hipError_t launch_scan_res_task(const ScanArgs &a, int nch, int waves, size_t lds, hipStream_t s,
-                                int tasks_per_group) {
+                                int tasks_per_group, bool merge) {
     if (!a.tasks || !a.packed_res || !a.res_norm || !a.res_cent || !a.res_rho || !a.res_row || !a.query_pm)
         return hipErrorInvalidValue;
     if (tasks_per_group > 1 && (waves != 1 || tasks_per_group > 4)) return hipErrorInvalidValue;
-    const size_t lds_ind = (size_t)tasks_per_group * (scan_heap_bytes(a.kp) + kQueueBytes);
+    if (merge && (waves != 1 || tasks_per_group < 1 || a.kp > kMergeMaxKp)) return hipErrorInvalidValue;
+    const size_t lds_ind = (size_t)tasks_per_group * (merge ? merge_lds_bytes(a.kp) : scan_heap_bytes(a.kp) + kQueueBytes);
+    const dim3 grid_ind((a.ntasks + tasks_per_group - 1) / tasks_per_group), block_ind(tasks_per_group * kWave);
+#define LAUNCH_IND(N, B, M)                                                                        \
+    {                                                                                              \
+        const void *fi = (const void *)scan_res_task_kernel<N, B, true, M>;                        \
+        if (lds_ind > 65536) {                                                                     \
+            hipError_t e = hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                                               (int)lds_ind);                                      \
+            if (e != hipSuccess) return e;                                                         \
+        }                                                                                          \
+        hipLaunchKernelGGL((scan_res_task_kernel<N, B, true, M>), grid_ind, block_ind, lds_ind, s, a); \
+        return hipGetLastError();                                                                  \
+    }
 #define X(N, B)                                                                                    \
     if (nch == N) {                                                                                \
-        if (tasks_per_group > 1) {                                                                 \
-            const void *fi = (const void *)scan_res_task_kernel<N, B, true>;                       \
-            if (lds_ind > 65536) {                                                                 \
-                hipError_t e = hipFuncSetAttribute(fi, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                                   (int)lds_ind);                                  \
-                if (e != hipSuccess) return e;                                                     \
-            }                                                                                      \
-            hipLaunchKernelGGL((scan_res_task_kernel<N, B, true>),                                 \
-                               dim3((a.ntasks + tasks_per_group - 1) / tasks_per_group),           \
-                               dim3(tasks_per_group * kWave), lds_ind, s, a);                      \
-            return hipGetLastError();                                                              \
-        }                                                                                          \
-        const void *fn = (const void *)scan_res_task_kernel<N, B, false>;                          \
+        if (merge) LAUNCH_IND(N, B, true)                                                          \
+        if (tasks_per_group > 1) LAUNCH_IND(N, B, false)                                           \
+        const void *fn = (const void *)scan_res_task_kernel<N, B, false, false>;                   \
         if (lds > 65536) {                                                                         \
             hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                                (int)lds);                                          \
             if (e != hipSuccess) return e;                                                         \
         }                                                                                          \
-        hipLaunchKernelGGL((scan_res_task_kernel<N, B, false>), dim3(a.ntasks), dim3(waves * kWave), lds, s, \
-                           a);                                                                     \
+        hipLaunchKernelGGL((scan_res_task_kernel<N, B, false, false>), dim3(a.ntasks), dim3(waves * kWave), lds, \
+                           s, a);                                                                  \
         return hipGetLastError();                                                                  \
     }
     KNN_RES_TASK_INSTANCES(X)
 #undef X
+#undef LAUNCH_IND
     return hipErrorInvalidValue;
 }
 
