@@ -1485,50 +1485,54 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
 // (fewer for few queries, so that the grid still covers the chip); every
 // wave walks those tiles (so their loads are shared through L1) against its
 // own 32 queries, whose half-rows (KH floats per lane, KH % 4 == 0,
-// KH <= KHM) stay in VGPRs.  The per-block fold runs in registers on lanes
-// 0..31 (one query each): a tile's 32 bounds of a query sit on its two lane
-// halves (rows 8(i/4) + 4h + i%4), the partner half's come over by a lane
-// swap, and each row's block and that block's ball range come from the lane
-// of the row by readlane.  The fold carries the open block from tile to
-// tile, so a block is merged with an atomic min only when it extends past
-// [F0, F1); every other block is stored once.
+// KH <= KHM) stay in VGPRs.  The workgroup first stages its tiles' per-ball
+// data in LDS (squared norm, its root and radius rounded up, the block and
+// its ball range).  The per-block fold runs in registers on lanes 0..31 (one
+// query each): a tile's 32 bounds of a query sit on its two lane halves
+// (rows 8(i/4) + 4h + i%4), the partner half's come over by a lane swap,
+// and each row's block and that block's ball range come from the lane of the
+// row by readlane.  The fold carries the open block from tile to tile, so a
+// block is merged with an atomic min only when it extends past [F0, F1);
+// every other block is stored once.
 constexpr int kBallTilesPerWg = 8;  // tiles per workgroup for many queries
 constexpr int kBallMinWgs = 1024;    // workgroups aimed for when there are few
-
 //
 // NCH > 0: the products are bfloat16 three-term (q_hi c_hi + q_hi c_lo +
 // q_lo c_hi per 16-dim chunk, v_mfma_f32_32x32x16_bf16, FP32 accumulation)
 // on the packed centres cbf; the accumulator layout is the FP32 MFMA's, so
 // the bounds and the fold are shared.  g15_bf then replaces the FP32 error
 // coefficient (the host's ball_error_coeff_bf3).
+
+// Per-ball data of a workgroup's tiles, staged in LDS (rows past the last
+// ball: radius -1, block -1).
+struct BallStage {
+    float nrm[kBallTilesPerWg * 32], sq[kBallTilesPerWg * 32], rad[kBallTilesPerWg * 32];
+    int blk[kBallTilesPerWg * 32], bs[kBallTilesPerWg * 32], be[kBallTilesPerWg * 32];
+};
+__device__ __forceinline__ void ball_stage(BallStage &st, int F0, int F1, int tpw, const float *__restrict__ fnorm,
+                                           const float *__restrict__ fsq, const float *__restrict__ fradf,
+                                           const int *__restrict__ block_s, const int *__restrict__ fb_start) {
+    for (int j = threadIdx.x; j < tpw * 32; j += blockDim.x) {
+        const int f = F0 + j;
+        const bool in = f < F1;
+        const int b = in ? block_s[f] : -1;
+        st.nrm[j] = in ? fnorm[f] : 0.0f;
+        st.sq[j] = in ? fsq[f] : 0.0f;
+        st.rad[j] = in ? fradf[f] : -1.0f;
+        st.blk[j] = b;
+        st.bs[j] = in ? fb_start[b] : 0;
+        st.be[j] = in ? fb_start[b + 1] : 0;
+    }
+}
+
+// A wave's query operands: lane (col, h) holds query qa's half-row (dims
+// h*KH .. h*KH + KH - 1, FP32), or its hi / lo parts of dims 16c + 8h .. + 7
+// per chunk c (bfloat16).
 template <int KHM, int NCH>
-__global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restrict__ qpm, int pm_ld,
-                                                          int Q, int dim,
-                                                          const float *__restrict__ qnorm,
-                                                          const float *__restrict__ fcent, int nf,
-                                                          const float *__restrict__ fnorm,
-                                                          const float *__restrict__ fsq,
-                                                          const float *__restrict__ fradf,
-                                                          const int *__restrict__ block_s,
-                                                          const int *__restrict__ fb_start,
-                                                          float *__restrict__ lb, int tpw,
-                                                          const unsigned short *__restrict__ cbf,
-                                                          float g15_bf) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int col = lane & 31, h = lane >> 5;
-    const int ntf = (nf + 31) / 32;
-    const int ngrp = (ntf + tpw - 1) / tpw;
-    const int qg = blockIdx.x / ngrp, grp = blockIdx.x - qg * ngrp;
-    const int q0 = qg * 128 + wave * 32;
-    if (q0 >= Q) return;  // whole wave past the last query (no workgroup barrier below)
-    const int F0 = grp * tpw * 32;
-    const int F1 = min(nf, F0 + tpw * 32);
+__device__ __forceinline__ void ball_query_ops(const float *__restrict__ qpm, int pm_ld, int qa, int dim, int h,
+                                               float4 (&bq)[NCH ? 1 : KHM / 4], bf16x8_t (&bh)[NCH ? NCH : 1],
+                                               bf16x8_t (&bl)[NCH ? NCH : 1]) {
     const int KH = (dim + 1) / 2;
-    const int qa = min(q0 + col, Q - 1);
-    // this lane's query half-row: dims h*KH .. h*KH + KH - 1 (FP32), or its
-    // hi / lo parts of dims 16c + 8h .. + 7 per chunk c (bfloat16)
-    float4 bq[NCH ? 1 : KHM / 4];
-    bf16x8_t bh[NCH ? NCH : 1], bl[NCH ? NCH : 1];
     if constexpr (NCH > 0) {
         const float *qr = qpm + (long long)qa * pm_ld;
 #pragma unroll
@@ -1548,6 +1552,101 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
         for (int u = 0; u < KHM / 4; ++u)
             bq[u] = 4 * u < KH ? *reinterpret_cast<const float4 *>(qr + 4 * u) : make_float4(0, 0, 0, 0);
     }
+}
+
+// Bounds of one 32-ball tile (balls f0 .. f0 + 31, staged from F0) against a
+// wave's 32 queries: out[i] is ball f0 + 8(i/4) + 4h + i%4 for query col
+// (+inf past the last ball).
+template <int KHM, int NCH>
+__device__ __forceinline__ void ball_tile_bounds(const BallStage &st, int F0, int f0, int nf, int pm_ld, int dim,
+                                                 int col, int h, const float *__restrict__ fcent,
+                                                 const unsigned short *__restrict__ cbf,
+                                                 const float4 (&bq)[NCH ? 1 : KHM / 4],
+                                                 const bf16x8_t (&bh)[NCH ? NCH : 1],
+                                                 const bf16x8_t (&bl)[NCH ? NCH : 1], float nq, float sq, float g15,
+                                                 float (&out)[16]) {
+    const int KH = (dim + 1) / 2;
+    const int fa = min(f0 + col, nf - 1);
+    f32x16_t acc = {};
+    if constexpr (NCH > 0) {
+        const unsigned short *ca = cbf + (long long)fa * NCH * 32 + 8 * h;
+        u32x4_t ca_hi[NCH], ca_lo[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            ca_hi[c] = *reinterpret_cast<const u32x4_t *>(ca + 32 * c);
+            ca_lo[c] = *reinterpret_cast<const u32x4_t *>(ca + 32 * c + 16);
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const bf16x8_t ah = __builtin_bit_cast(bf16x8_t, ca_hi[c]);
+            const bf16x8_t al = __builtin_bit_cast(bf16x8_t, ca_lo[c]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[c], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[c], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[c], acc, 0, 0, 0);
+        }
+    } else {
+        const float *ca = fcent + (long long)fa * pm_ld + h * KH;
+#pragma unroll
+        for (int u = 0; u < KHM / 4; ++u) {
+            if (4 * u < KH) {
+                const float4 av = *reinterpret_cast<const float4 *>(ca + 4 * u);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[u].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[u].y, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[u].z, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[u].w, acc, 0, 0, 0);
+            }
+        }
+    }
+    // four consecutive balls per group, read as one float4 each of norms,
+    // their roots and radii
+#pragma unroll
+    for (int g4 = 0; g4 < 4; ++g4) {
+        const int j = f0 - F0 + 8 * g4 + 4 * h;
+        const float4 n4 = *reinterpret_cast<const float4 *>(st.nrm + j);
+        const float4 s4 = *reinterpret_cast<const float4 *>(st.sq + j);
+        const float4 r4 = *reinterpret_cast<const float4 *>(st.rad + j);
+        out[4 * g4 + 0] = ball_lower_bound(nq, sq, n4.x, s4.x, acc[4 * g4 + 0], r4.x, g15);
+        out[4 * g4 + 1] = ball_lower_bound(nq, sq, n4.y, s4.y, acc[4 * g4 + 1], r4.y, g15);
+        out[4 * g4 + 2] = ball_lower_bound(nq, sq, n4.z, s4.z, acc[4 * g4 + 2], r4.z, g15);
+        out[4 * g4 + 3] = ball_lower_bound(nq, sq, n4.w, s4.w, acc[4 * g4 + 3], r4.w, g15);
+    }
+}
+
+// The ball-range split of the bounds grid: tiles per workgroup and groups.
+__host__ __device__ inline int ball_ngrp(int nf, int tpw) { return ((nf + 31) / 32 + tpw - 1) / tpw; }
+static int ball_tpw(int Q, int nf) {
+    const long long qgs = (Q + 127) / 128, ntf = (nf + 31) / 32;
+    return (int)std::max(1ll, std::min<long long>(kBallTilesPerWg, qgs * ntf / kBallMinWgs));
+}
+
+template <int KHM, int NCH>
+__global__ __launch_bounds__(256, 4) void ball_lb_reg_kernel(const float *__restrict__ qpm, int pm_ld,
+                                                             int Q, int dim,
+                                                             const float *__restrict__ qnorm,
+                                                             const float *__restrict__ fcent, int nf,
+                                                             const float *__restrict__ fnorm,
+                                                             const float *__restrict__ fsq,
+                                                             const float *__restrict__ fradf,
+                                                             const int *__restrict__ block_s,
+                                                             const int *__restrict__ fb_start,
+                                                             float *__restrict__ lb, int tpw,
+                                                             const unsigned short *__restrict__ cbf,
+                                                             float g15_bf) {
+    __shared__ __align__(16) BallStage st;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 31, h = lane >> 5;
+    const int ngrp = ball_ngrp(nf, tpw);
+    const int qg = blockIdx.x / ngrp, grp = blockIdx.x - qg * ngrp;
+    const int F0 = grp * tpw * 32;
+    const int F1 = min(nf, F0 + tpw * 32);
+    ball_stage(st, F0, F1, tpw, fnorm, fsq, fradf, block_s, fb_start);
+    __syncthreads();
+    const int q0 = qg * 128 + wave * 32;
+    if (q0 >= Q) return;  // whole wave past the last query (no workgroup barrier below)
+    const int qa = min(q0 + col, Q - 1);
+    float4 bq[NCH ? 1 : KHM / 4];
+    bf16x8_t bh[NCH ? NCH : 1], bl[NCH ? NCH : 1];
+    ball_query_ops<KHM, NCH>(qpm, pm_ld, qa, dim, h, bq, bh, bl);
     const float g15 = NCH > 0 ? g15_bf : ball_error_coeff(dim);
     const float nq = qnorm[qa];
     const float sq = nextafterf(sqrtf(nq), INFINITY);
@@ -1572,75 +1671,21 @@ __global__ __launch_bounds__(256) void ball_lb_reg_kernel(const float *__restric
     for (int tt = 0; tt < tpw; ++tt) {
         const int f0 = F0 + tt * 32;
         if (f0 >= F1) break;  // uniform over the wave
-        const int fa = min(f0 + col, nf - 1);
-        f32x16_t acc = {};
-        if constexpr (NCH > 0) {
-            const unsigned short *ca = cbf + (long long)fa * NCH * 32 + 8 * h;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                const bf16x8_t ah = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t *>(ca + 32 * c));
-                const bf16x8_t al =
-                    __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t *>(ca + 32 * c + 16));
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[c], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[c], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[c], acc, 0, 0, 0);
-            }
-        } else {
-            const float *ca = fcent + (long long)fa * pm_ld + h * KH;
-#pragma unroll
-            for (int u = 0; u < KHM / 4; ++u) {
-                if (4 * u < KH) {
-                    const float4 av = *reinterpret_cast<const float4 *>(ca + 4 * u);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[u].x, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[u].y, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[u].z, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[u].w, acc, 0, 0, 0);
-                }
-            }
-        }
-        // element i is ball f0 + 8(i/4) + 4h + i%4: four consecutive balls per
-        // group, read as one float4 each of norms, their roots and radii
-        const bool full = f0 + 32 <= nf;
         float out[16];
-#pragma unroll
-        for (int g4 = 0; g4 < 4; ++g4) {
-            const int fb = f0 + 8 * g4 + 4 * h;
-            float4 n4, s4, r4;
-            if (full) {
-                n4 = *reinterpret_cast<const float4 *>(fnorm + fb);
-                s4 = *reinterpret_cast<const float4 *>(fsq + fb);
-                r4 = *reinterpret_cast<const float4 *>(fradf + fb);
-            } else {
-                n4.x = fb + 0 < nf ? fnorm[fb + 0] : 0.0f;
-                n4.y = fb + 1 < nf ? fnorm[fb + 1] : 0.0f;
-                n4.z = fb + 2 < nf ? fnorm[fb + 2] : 0.0f;
-                n4.w = fb + 3 < nf ? fnorm[fb + 3] : 0.0f;
-                s4.x = fb + 0 < nf ? fsq[fb + 0] : 0.0f;
-                s4.y = fb + 1 < nf ? fsq[fb + 1] : 0.0f;
-                s4.z = fb + 2 < nf ? fsq[fb + 2] : 0.0f;
-                s4.w = fb + 3 < nf ? fsq[fb + 3] : 0.0f;
-                r4.x = fb + 0 < nf ? fradf[fb + 0] : -1.0f;
-                r4.y = fb + 1 < nf ? fradf[fb + 1] : -1.0f;
-                r4.z = fb + 2 < nf ? fradf[fb + 2] : -1.0f;
-                r4.w = fb + 3 < nf ? fradf[fb + 3] : -1.0f;
-            }
-            out[4 * g4 + 0] = ball_lower_bound(nq, sq, n4.x, s4.x, acc[4 * g4 + 0], r4.x, g15);
-            out[4 * g4 + 1] = ball_lower_bound(nq, sq, n4.y, s4.y, acc[4 * g4 + 1], r4.y, g15);
-            out[4 * g4 + 2] = ball_lower_bound(nq, sq, n4.z, s4.z, acc[4 * g4 + 2], r4.z, g15);
-            out[4 * g4 + 3] = ball_lower_bound(nq, sq, n4.w, s4.w, acc[4 * g4 + 3], r4.w, g15);
-        }
-        // the partner half's rows, and (lane r < 32) row r's block and that
-        // block's first ball and one past its last
+        ball_tile_bounds<KHM, NCH>(st, F0, f0, nf, pm_ld, dim, col, h, fcent, cbf, bq, bh, bl, nq, sq, g15, out);
+        // the partner half's rows (a lane swap across the halves), and
+        // (lane r < 32) row r's block and that block's first ball and one
+        // past its last
         float oth[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) oth[i] = __shfl_xor(out[i], 32);
-        int rb = -1, rs = 0, re = 0;
-        if (h == 0 && f0 + col < nf) {
-            rb = block_s[f0 + col];
-            rs = fb_start[rb];
-            re = fb_start[rb + 1];
+        for (int i = 0; i < 16; ++i) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(out[i]), __float_as_uint(out[i]),
+                                                             false, false);
+            oth[i] = __uint_as_float(h ? sw[0] : sw[1]);
         }
         const int n = min(32, F1 - f0);
+        const int rj = f0 - F0 + col;
+        const int rb = st.blk[rj], rs = st.bs[rj], re = st.be[rj];
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             if (r >= n) continue;  // (not break: the loop must unroll)
@@ -1882,9 +1927,9 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
     hipError_t e = hipMemsetD32Async((hipDeviceptr_t)lb, 0x7F800000, (size_t)nb * Q, s);
     if (e != hipSuccess) return e;
     const int KH = (dim + 1) / 2;
-    const long long qgs = (Q + 127) / 128, ntf = (nf + 31) / 32;
-    const int tpw = (int)std::max(1ll, std::min<long long>(kBallTilesPerWg, qgs * ntf / kBallMinWgs));
-    const long long wgs = qgs * ((ntf + tpw - 1) / tpw);
+    const long long qgs = (Q + 127) / 128;
+    const int tpw = ball_tpw(Q, nf);
+    const long long wgs = qgs * ball_ngrp(nf, tpw);
     if (cbf && nch_bf >= 1 && nch_bf <= 8 && 16 * nch_bf >= dim) {
 #define BF(N)                                                                                          \
     if (nch_bf == N) {                                                                                 \
