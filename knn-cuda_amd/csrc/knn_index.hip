@@ -28,42 +28,54 @@ __global__ void histogram_kernel(const int *__restrict__ assign, int n, int *__r
         atomicAdd(&counts[assign[i]], 1);
 }
 
-// Exclusive prefix sums of counts (one block; nb <= 65536): start[b] over
-// counts rounded up to `align`, and plain[b] over the raw counts.
+// Exclusive prefix sums of counts (one block of up to 1024 threads):
+// start[b] over counts rounded up to `align`, and plain[b] over the raw
+// counts, with the totals at [nb].  The block takes blockDim.x counts at a
+// time: a scan within each wave (lane shifts), the earlier waves' totals
+// from LDS, and a carry across the chunks.
 __global__ void prefix_kernel(const int *__restrict__ counts, int nb, int align,
                               int *__restrict__ start, int *__restrict__ plain) {
-    __shared__ int part[1024];
-    __shared__ int partp[1024];
-    const int per = (nb + blockDim.x - 1) / blockDim.x;
-    const int b0 = threadIdx.x * per;
-    int s = 0, sp = 0;
-    for (int b = b0; b < min(nb, b0 + per); ++b) {
-        s += (counts[b] + align - 1) / align * align;
-        sp += counts[b];
-    }
-    part[threadIdx.x] = s;
-    partp[threadIdx.x] = sp;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0, accp = 0;
-        for (int t = 0; t < (int)blockDim.x; ++t) {
-            const int v = part[t], vp = partp[t];
-            part[t] = acc;
-            partp[t] = accp;
-            acc += v;
-            accp += vp;
+    __shared__ int wsum[16], wsump[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int base = 0, basep = 0;
+    for (int c0 = 0; c0 < nb; c0 += blockDim.x) {
+        const int b = c0 + threadIdx.x;
+        const int cnt = b < nb ? counts[b] : 0;
+        const int own = (cnt + align - 1) / align * align;
+        int v = own, vp = cnt;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int x = __shfl_up(v, off), xp = __shfl_up(vp, off);
+            if (lane >= off) {
+                v += x;
+                vp += xp;
+            }
         }
-        start[nb] = acc;
-        if (plain) plain[nb] = accp;
+        if (lane == kWave - 1) {
+            wsum[wave] = v;
+            wsump[wave] = vp;
+        }
+        __syncthreads();
+        int before = 0, beforep = 0, tot = 0, totp = 0;
+        for (int w = 0; w < nw; ++w) {
+            if (w < wave) {
+                before += wsum[w];
+                beforep += wsump[w];
+            }
+            tot += wsum[w];
+            totp += wsump[w];
+        }
+        if (b < nb) {
+            start[b] = base + before + v - own;
+            if (plain) plain[b] = basep + beforep + vp - cnt;
+        }
+        base += tot;
+        basep += totp;
+        __syncthreads();  // wsum is written again by the next chunk
     }
-    __syncthreads();
-    s = part[threadIdx.x];
-    sp = partp[threadIdx.x];
-    for (int b = b0; b < min(nb, b0 + per); ++b) {
-        start[b] = s;
-        if (plain) plain[b] = sp;
-        s += (counts[b] + align - 1) / align * align;
-        sp += counts[b];
+    if (threadIdx.x == 0) {
+        start[nb] = base;
+        if (plain) plain[nb] = basep;
     }
 }
 

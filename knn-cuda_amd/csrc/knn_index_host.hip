@@ -874,9 +874,12 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     if (timing) KNN_TRY(hipEventRecord(ev[4], s), "event");
     // queries whose certificate failed: once more through the index with
     // more probes, then (or at once, on a retry) the plain full scan
-    int nfail = 0;
-    KNN_TRY(hipMemcpyAsync(&nfail, w.fb_count.p, 4, hipMemcpyDeviceToHost, s), "copy");
+    // (the count lands in pinned host memory: one DMA, no staging copy)
+    static int *nfail_host = nullptr;
+    if (!nfail_host) KNN_TRY(hipHostMalloc((void **)&nfail_host, sizeof(int), hipHostMallocPortable), "host alloc");
+    KNN_TRY(hipMemcpyAsync(nfail_host, w.fb_count.p, 4, hipMemcpyDeviceToHost, s), "copy");
     KNN_TRY(hipStreamSynchronize(s), "sync");
+    const int nfail = *nfail_host;
     if (nfail > 0) {
         DevBuf &qsub = w.qsub, &dsub = w.dsub, &isub = w.isub;
         if (!w.need(qsub, (size_t)nfail * dim * 4) || !w.need(dsub, (size_t)nfail * k * 4) ||
