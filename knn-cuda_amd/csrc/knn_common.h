@@ -186,6 +186,14 @@ struct ExactArgs {
     const unsigned *nonfinite;  // as FinalArgs::nonfinite
 };
 
+// The block-index searches' bound matrix: bound (b, q) of block b for query
+// q, in slabs of 64 queries (slab q / 64 holds nb x 64 bounds, block-major),
+// so the 64 queries of a pick workgroup read one contiguous slab.
+__host__ __device__ inline long long lb_at(int b, int q, int nb) {
+    return ((long long)(q >> 6) * nb + b) * 64 + (q & 63);
+}
+__host__ __device__ inline long long lb_elems(int Q, int nb) { return (long long)((Q + 63) / 64) * 64 * nb; }
+
 // Order-preserving map of a float onto uint32 (total order, -0 < +0).
 __device__ __forceinline__ unsigned float_to_ordered(float x) {
     unsigned b = __float_as_uint(x);

@@ -314,7 +314,8 @@ bool run_streamed(const float *ref, int R, const float *query, int Q, int dim, i
                   bool kth_only);
 
 // Learned router inference (knn_router_host.hip): the first probe of each
-// query, into top1 (nullable) and/or lb[top1 * Q + q] = -inf (nullable).
+// query, into top1 (nullable) and/or bound (top1, q) = -inf of the tiled bound
+// matrix lb (lb_at; nullable).
 bool router_pick_first(knnhip_router *r, const float *q, int Q, long long ld, const int *counts,
                        int *top1, float *lb, hipStream_t s);
 bool router_scores(knnhip_router *r, const float *q, int Q, long long ld, float *scores, hipStream_t s);

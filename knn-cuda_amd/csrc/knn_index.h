@@ -94,6 +94,8 @@ hipError_t idx_ball_refit(const float *pm, int ld, int dim, const int *block_sta
 // cbf (nullable): bfloat16 hi/lo ball centres (idx_pack_ball_bf3) for
 // three-term bound products over nch_bf 16-dim chunks (dim <= 128), with
 // error coefficient g15_bf; else FP32 products.
+// The tiled bound matrix (lb_at) as nb rows of Q.
+hipError_t idx_lb_untile(const float *tiled, int Q, int nb, float *out, hipStream_t s);
 hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
                        const float *fcent, int nf, const float *fnorm, const float *fsq,
                        const float *fradf, const int *block_s, const int *fb_start, int nb, float *lb,

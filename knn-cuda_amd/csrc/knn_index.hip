@@ -37,10 +37,22 @@ __global__ void prefix_kernel(const int *__restrict__ counts, int nb, int align,
                               int *__restrict__ start, int *__restrict__ plain) {
     __shared__ int wsum[16], wsump[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // the first chunks' counts are all read before the scans start
+    constexpr int kAhead = 8;
+    int cnts[kAhead];
+#pragma unroll
+    for (int i = 0; i < kAhead; ++i) {
+        const int b = i * blockDim.x + threadIdx.x;
+        cnts[i] = b < nb ? counts[b] : 0;
+    }
     int base = 0, basep = 0;
-    for (int c0 = 0; c0 < nb; c0 += blockDim.x) {
+    for (int c0 = 0, ci = 0; c0 < nb; c0 += blockDim.x, ++ci) {
         const int b = c0 + threadIdx.x;
-        const int cnt = b < nb ? counts[b] : 0;
+        int cnt = 0;
+#pragma unroll
+        for (int i = 0; i < kAhead; ++i)
+            if (i == ci) cnt = cnts[i];
+        if (ci >= kAhead) cnt = b < nb ? counts[b] : 0;
         const int own = (cnt + align - 1) / align * align;
         int v = own, vp = cnt;
 #pragma unroll
@@ -169,9 +181,9 @@ constexpr int kPickWaves = 16;
 // At most this many queries, one workgroup per query instead.
 constexpr int kPickSingleMaxQ = 1024;
 
-__device__ inline float rank_score(const float *__restrict__ score, const float *__restrict__ lb,
-                                   long long e) {
-    const float s = score ? score[e] : -lb[e];
+__device__ inline float rank_score(const float *__restrict__ score, const float *__restrict__ lb, int b,
+                                   int q, int Q, int nb) {
+    const float s = score ? score[(long long)b * Q + q] : -lb[lb_at(b, q, nb)];
     return s == s ? s : -INFINITY;
 }
 
@@ -197,7 +209,7 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_kernel(
         if (valid)
             for (int b = b0; b < b1; ++b) {
                 if (counts[b] == 0) continue;
-                const float s = rank_score(score, lb, (long long)b * Q + q);
+                const float s = rank_score(score, lb, b, q, Q, nb);
                 const bool after = (s < last_s) || (s == last_s && b > last_b);
                 if (after && (bb < 0 || s > bs)) {
                     bs = s;
@@ -232,9 +244,9 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_kernel(
     if (valid)
         for (int b = b0; b < b1; ++b) {
             if (counts[b] == 0) continue;
-            const float s = rank_score(score, lb, (long long)b * Q + q);
+            const float s = rank_score(score, lb, b, q, Q, nb);
             const bool after = (s < last_s) || (s == last_s && b > last_b);
-            if (after) m = fminf(m, lb[(long long)b * Q + q]);
+            if (after) m = fminf(m, lb[lb_at(b, q, nb)]);
         }
     offer_s[wave][lane] = m;
     __syncthreads();
@@ -249,7 +261,7 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_kernel(
 // share; P merge rounds then take the best head among the 16 waves' lists.
 // lb_rest = min(bounds a thread did not keep, kept entries not chosen).
 // Same result as pick_blocks_kernel, reading the bounds once.
-template <int PMAX>
+template <int PMAX, bool SC>
 __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
     const float *__restrict__ score, int Q, int nb, int P, const float *__restrict__ lb,
     const int *__restrict__ counts, int *__restrict__ chosen, float *__restrict__ lb_rest) {
@@ -261,58 +273,90 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
     const bool valid = q < Q;
     const int per = (nb + kPickWaves - 1) / kPickWaves;
     const int b0 = min(nb, wave * per), b1 = min(nb, b0 + per);
-    float ls[PMAX], ll[PMAX];
-    int lbk[PMAX];
-    int n = 0;
+    // the kept blocks as 64-bit keys, sorted: the rank score order-mapped
+    // (-0 taken as +0, NaN as -inf), then the complemented block index, so a
+    // larger key ranks first and equal scores go by the smaller block; 0
+    // marks an unfilled slot.  SC (scores given): kl, their bounds; else a
+    // bound is the negated score.
+    unsigned long long key[PMAX];
+    float kl[SC ? PMAX : 1];
     float rest = INFINITY;
 #pragma unroll
-    for (int j = 0; j < PMAX; ++j) {
-        ls[j] = -INFINITY;
-        ll[j] = INFINITY;
-        lbk[j] = 0x7FFFFFFF;
-    }
-    // bounds are read kPickBatch blocks at a time (independent loads), then
-    // taken in block order
-    constexpr int kPickBatch = 16;
-    for (int bb = b0; valid && bb < b1; bb += kPickBatch) {
-        float sv[kPickBatch], lv[kPickBatch];
-        bool live[kPickBatch];
+    for (int j = 0; j < PMAX; ++j) key[j] = 0;
+#pragma unroll
+    for (int j = 0; j < (SC ? PMAX : 1); ++j) kl[j] = INFINITY;
+    // bounds are read kPickBatch blocks at a time (independent loads, the
+    // next batch in flight while one is taken, PF), then taken in block order
+    constexpr bool PF = !(SC && PMAX > 8);  // (sixteen slots with scores: no room for two batches)
+    constexpr int kPickBatch = PF ? 8 : 4;
+    // (the loop is uniform over the wave, so the blocks' point counts are
+    // scalar loads; lanes past the last query read the last one's bounds)
+    const int qa = valid ? q : Q - 1;
+    float lv[kPickBatch], sv[SC ? kPickBatch : 1];
+    auto load = [&](int bb) {
 #pragma unroll
         for (int j = 0; j < kPickBatch; ++j) {
             const int b = min(bb + j, b1 - 1);
-            live[j] = bb + j < b1 && counts[b] != 0;
-            const long long e = (long long)b * Q + q;
-            sv[j] = rank_score(score, lb, e);
-            lv[j] = lb[e];
+            lv[j] = lb[lb_at(b, qa, nb)];
+            if constexpr (SC) sv[j] = score[(long long)b * Q + qa];
         }
+    };
+    if (PF && b0 < b1) load(b0);
+    for (int bb = b0; bb < b1; bb += kPickBatch) {
+        if (!PF) load(bb);
+        float cs_[kPickBatch], cl_[SC ? kPickBatch : 1];
+        int cc_[kPickBatch];  // the blocks' point counts (0: empty, skipped)
 #pragma unroll
         for (int j = 0; j < kPickBatch; ++j) {
-            if (!live[j]) continue;  // uniform: the block is the same for every lane
-            // one compare-swap pass over the P kept slots (sorted by score):
-            // the new block takes the first unfilled slot or the first slot
-            // whose score is strictly lower, and pushes the rest down (blocks
-            // come in increasing order, so an equal score ranks after the kept
-            // ones); whatever leaves the last slot, the new block itself or a
-            // kept one, is not chosen and bounds the rest
-            float cs = sv[j], cl = lv[j];
-            int cb = bb + j;
+            cc_[j] = counts[min(bb + j, b1 - 1)];
+            if constexpr (SC) {
+                cs_[j] = sv[j];
+                cl_[j] = lv[j];
+            } else {
+                cs_[j] = -lv[j];
+            }
+        }
+        if (PF && bb + kPickBatch < b1) load(bb + kPickBatch);
+#pragma unroll
+        for (int j = 0; j < kPickBatch; ++j) {
+            const int b = bb + j;
+            if (b >= b1 || cc_[j] == 0) continue;  // uniform: the block is the same for every lane
+            // one compare-swap pass over the P kept slots: the new block takes
+            // the first unfilled slot or the first slot of a smaller key and
+            // pushes the rest down; whatever leaves the last slot, the new
+            // block itself or a kept one, is not chosen and bounds the rest
+            const float sc = cs_[j] == cs_[j] ? cs_[j] + 0.0f : -INFINITY;
+            unsigned long long c = ((unsigned long long)float_to_ordered(sc) << 32) | (unsigned)(~b);
+            float cl = SC ? cl_[SC ? j : 0] : 0.0f;
 #pragma unroll
             for (int t = 0; t < PMAX; ++t) {
                 if (t < P) {
-                    const bool sw = cs > ls[t] || lbk[t] == 0x7FFFFFFF;
-                    const float ts = ls[t], tl = ll[t];
-                    const int tb = lbk[t];
-                    ls[t] = sw ? cs : ts;
-                    ll[t] = sw ? cl : tl;
-                    lbk[t] = sw ? cb : tb;
-                    cs = sw ? ts : cs;
-                    cl = sw ? tl : cl;
-                    cb = sw ? tb : cb;
+                    const bool sw = c > key[t];
+                    const unsigned long long tk = key[t];
+                    key[t] = sw ? c : tk;
+                    c = sw ? tk : c;
+                    if constexpr (SC) {
+                        const float tl = kl[t];
+                        kl[t] = sw ? cl : tl;
+                        cl = sw ? tl : cl;
+                    }
                 }
             }
-            rest = fminf(rest, cl);  // an unfilled slot leaves +inf
-            n = min(n + 1, P);
+            // (an unfilled slot leaves key 0: nothing)
+            const float out_lb = SC ? cl : c ? -ordered_to_float((unsigned)(c >> 32)) : INFINITY;
+            rest = fminf(rest, out_lb);
         }
+    }
+    float ls[PMAX], ll[PMAX];
+    int lbk[PMAX];
+    int n = 0;
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        const bool real = key[j] != 0;
+        ls[j] = real ? ordered_to_float((unsigned)(key[j] >> 32)) : -INFINITY;
+        lbk[j] = real ? (int)~(unsigned)key[j] : 0x7FFFFFFF;
+        ll[j] = SC ? kl[SC ? j : 0] : real ? -ls[j] : INFINITY;
+        n += real ? 1 : 0;
     }
     // merge rounds
     int head = 0;
@@ -372,7 +416,7 @@ __global__ __launch_bounds__(256) void pick_blocks_single_kernel(
     __shared__ int red_b[4];
     const int q = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (int b = tid; b < nb; b += blockDim.x)
-        sc[b] = counts[b] == 0 ? NAN : rank_score(score, lb, (long long)b * Q + q);
+        sc[b] = counts[b] == 0 ? NAN : rank_score(score, lb, b, q, Q, nb);
     __syncthreads();
     float last_s = INFINITY;
     int last_b = -1;
@@ -422,7 +466,7 @@ __global__ __launch_bounds__(256) void pick_blocks_single_kernel(
         const float s = sc[b];
         if (s != s) continue;
         const bool after = (s < last_s) || (s == last_s && b > last_b);
-        if (after) m = fminf(m, lb[(long long)b * Q + q]);
+        if (after) m = fminf(m, lb[lb_at(b, q, nb)]);
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) m = fminf(m, __shfl_xor(m, off));
@@ -458,7 +502,7 @@ __global__ __launch_bounds__(64 * kPickWaves) void router_first_kernel(
         for (int j = 0; j < kBatch; ++j) {
             const int b = min(bq + j, b1 - 1);
             live[j] = bq + j < b1 && counts[b] != 0;
-            sv[j] = rank_score(score, lb, (long long)b * Q + q);
+            sv[j] = rank_score(score, lb, b, q, Q, nb);
         }
 #pragma unroll
         for (int j = 0; j < kBatch; ++j)
@@ -477,7 +521,7 @@ __global__ __launch_bounds__(64 * kPickWaves) void router_first_kernel(
                 bs = best_s[w][lane];
                 bb = best_b[w][lane];
             }
-        if (bb >= 0) lb[(long long)bb * Q + q] = -INFINITY;
+        if (bb >= 0) lb[lb_at(bb, q, nb)] = -INFINITY;
     }
 }
 
@@ -1402,7 +1446,7 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
                                                       const float *__restrict__ fradf,
                                                       const int *__restrict__ block_s,
                                                       const int *__restrict__ fb_start,
-                                                      float *__restrict__ lb) {
+                                                      float *__restrict__ lb, int nb) {
     __shared__ float patch[4][32][33];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
@@ -1472,7 +1516,7 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
     float m = INFINITY;
     auto flush = [&](int b, float v) {
         if (b < 0) return;
-        float *dst = lb + (long long)b * Q + q;
+        float *dst = lb + lb_at(b, q, nb);
         if (fb_start[b] >= lo && fb_start[b + 1] <= hi)
             *dst = v;
         else
@@ -1641,7 +1685,7 @@ __global__ __launch_bounds__(256, 4) void ball_lb_reg_kernel(const float *__rest
                                                              const float *__restrict__ fradf,
                                                              const int *__restrict__ block_s,
                                                              const int *__restrict__ fb_start,
-                                                             float *__restrict__ lb, int tpw,
+                                                             float *__restrict__ lb, int nb, int tpw,
                                                              const unsigned short *__restrict__ cbf,
                                                              float g15_bf) {
     __shared__ __align__(16) BallStage st;
@@ -1673,7 +1717,7 @@ __global__ __launch_bounds__(256, 4) void ball_lb_reg_kernel(const float *__rest
         if (cur < 0) return;
         const bool whole = cur_s >= F0 && cur_e <= F1;
         if (folder) {
-            float *dst = lb + (long long)cur * Q + q;
+            float *dst = lb + lb_at(cur, q, nb);
             if (whole)
                 *dst = m;
             else
@@ -1767,11 +1811,17 @@ hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float
         }
         hipLaunchKernelGGL(pick_blocks_single_kernel, dim3(Q), dim3(256), lds, s, score, Q, nb, P, lb,
                            counts, chosen, lb_rest);
-    } else if (P <= 8)
-        hipLaunchKernelGGL(pick_blocks_list_kernel<8>, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
+    } else if (P <= 8 && score)
+        hipLaunchKernelGGL((pick_blocks_list_kernel<8, true>), dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
                            score, Q, nb, P, lb, counts, chosen, lb_rest);
+    else if (P <= 8)
+        hipLaunchKernelGGL((pick_blocks_list_kernel<8, false>), dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
+                           score, Q, nb, P, lb, counts, chosen, lb_rest);
+    else if (P <= 16 && score)
+        hipLaunchKernelGGL((pick_blocks_list_kernel<16, true>), dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0,
+                           s, score, Q, nb, P, lb, counts, chosen, lb_rest);
     else if (P <= 16)
-        hipLaunchKernelGGL(pick_blocks_list_kernel<16>, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0,
+        hipLaunchKernelGGL((pick_blocks_list_kernel<16, false>), dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0,
                            s, score, Q, nb, P, lb, counts, chosen, lb_rest);
     else
         hipLaunchKernelGGL(pick_blocks_kernel, dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
@@ -1932,11 +1982,23 @@ hipError_t idx_pack_ball_bf3(const float *fcent, int pm_ld, int nf, int dim, int
     return hipGetLastError();
 }
 
+// The tiled bound matrix (lb_at) as nb rows of Q: out[b * Q + q].
+__global__ void lb_untile_kernel(const float *__restrict__ tiled, int Q, int nb, float *__restrict__ out) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)Q * nb;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(e / Q), q = (int)(e - (long long)b * Q);
+        out[e] = tiled[lb_at(b, q, nb)];
+    }
+}
+hipError_t idx_lb_untile(const float *tiled, int Q, int nb, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(lb_untile_kernel, dim3(grid_for((long long)Q * nb)), dim3(256), 0, s, tiled, Q, nb, out);
+    return hipGetLastError();
+}
 hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm,
                        const float *fcent, int nf, const float *fnorm, const float *fsq,
                        const float *fradf, const int *block_s, const int *fb_start, int nb, float *lb,
                        hipStream_t s, const unsigned short *cbf, int nch_bf, float g15_bf) {
-    hipError_t e = hipMemsetD32Async((hipDeviceptr_t)lb, 0x7F800000, (size_t)nb * Q, s);
+    hipError_t e = hipMemsetD32Async((hipDeviceptr_t)lb, 0x7F800000, (size_t)lb_elems(Q, nb), s);
     if (e != hipSuccess) return e;
     const int KH = (dim + 1) / 2;
     const long long qgs = (Q + 127) / 128;
@@ -1946,7 +2008,7 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
 #define BF(N)                                                                                          \
     if (nch_bf == N) {                                                                                 \
         hipLaunchKernelGGL((ball_lb_reg_kernel<4, N>), dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld, Q, \
-                           dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw, cbf,   \
+                           dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, nb, tpw, cbf, \
                            g15_bf);                                                                    \
         return hipGetLastError();                                                                      \
     }
@@ -1956,17 +2018,17 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
     if (KH % 4 == 0 && pm_ld % 4 == 0 && KH <= 64) {
         if (KH <= 32)
             hipLaunchKernelGGL((ball_lb_reg_kernel<32, 0>), dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
-                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw,
+                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, nb, tpw,
                                nullptr, 0.0f);
         else
             hipLaunchKernelGGL((ball_lb_reg_kernel<64, 0>), dim3((unsigned)wgs), dim3(256), 0, s, qpm, pm_ld,
-                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, tpw,
+                               Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, nb, tpw,
                                nullptr, 0.0f);
         return hipGetLastError();
     }
     const long long tiles = (long long)((Q + 31) / 32) * ((nf + 31) / 32);
     hipLaunchKernelGGL(ball_lb_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, qpm,
-                       pm_ld, Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb);
+                       pm_ld, Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, nb);
     return hipGetLastError();
 }
 // Batched outputs: element e = j * Qtot + q of the class's k x Qtot results

@@ -677,7 +677,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         max_t[ph] = (int)(((long long)Q * (pr[ph + 1] - pr[ph]) + qpt - 1) / qpt) + nu;
     const int max_tasks = max_t[0] + max_t[1];
     if (!w.need(w.qnorm, (size_t)Q * 4) || !w.need(w.qpm, (size_t)Q * pm_ld * 4) ||
-        !w.need(w.lb, (size_t)Q * nb * 4) ||
+        !w.need(w.lb, (size_t)lb_elems(Q, nb) * 4) ||
         !w.need(w.chosen, (size_t)Q * P * 4) || !w.need(w.lb_rest, (size_t)Q * 4) ||
         (balls && !w.need(w.bchosen, (size_t)npairs * 4)) ||
         !w.need(w.pcount, (size_t)nu * 4) || !w.need(w.qstart, (size_t)(nu + 1) * 4) ||
@@ -1067,7 +1067,8 @@ int knnhip_index_bounds(knnhip_index *ix, const float *query, int query_nb, long
     if (!ensure_device()) return 0;
     CallOrder order((hipStream_t)stream);
     IndexWork &w = g_iw;
-    if (!w.need(w.qnorm, (size_t)query_nb * 4) || !w.need(w.qpm, (size_t)query_nb * ix->pm_ld * 4)) {
+    if (!w.need(w.qnorm, (size_t)query_nb * 4) || !w.need(w.qpm, (size_t)query_nb * ix->pm_ld * 4) ||
+        !w.need(w.lb, (size_t)lb_elems(query_nb, ix->nb) * 4)) {
         set_error("Memory allocation error");
         return 0;
     }
@@ -1086,7 +1087,14 @@ int knnhip_index_bounds(knnhip_index *ix, const float *query, int query_nb, long
         set_error("norm kernel launch");
         return 0;
     }
-    return index_bounds(ix, w.qpm.as<float>(), w.qnorm.as<float>(), query_nb, precision == 1, lb, s) ? 1 : 0;
+    // (the searches' tiled layout, then rows of query_nb)
+    if (!index_bounds(ix, w.qpm.as<float>(), w.qnorm.as<float>(), query_nb, precision == 1, w.lb.as<float>(), s))
+        return 0;
+    if (idx_lb_untile(w.lb.as<float>(), query_nb, ix->nb, lb, s) != hipSuccess) {
+        set_error("bound layout kernel launch");
+        return 0;
+    }
+    return 1;
 }
 
 void knnhip_index_free(knnhip_index *ix) {

@@ -37,7 +37,7 @@ struct RouterLayout {
 // Fused inference (first probe per query): queries x (dim-major, ldx), the
 // bfloat16 weight copy (W1 rows padded to dpad, W2 and W3 rows of H), FP32
 // biases; counts (nullable) marks empty blocks; outputs top1[q] (nullable)
-// and lb[top1*Q + q] = -inf (nullable).
+// and bound (top1, q) = -inf of the tiled bound matrix lb (lb_at; nullable).
 struct RouterFirstArgs {
     const float *x;
     long long ldx;

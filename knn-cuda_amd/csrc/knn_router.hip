@@ -462,7 +462,7 @@ __global__ __launch_bounds__(64 * kRtWaves) void router_first_fused_kernel(Route
             }
         }
         if (a.top1) a.top1[q] = b;
-        if (a.lb && b >= 0) a.lb[(long long)b * a.Q + q] = -INFINITY;
+        if (a.lb && b >= 0) a.lb[lb_at(b, q, a.nb)] = -INFINITY;
     }
 }
 

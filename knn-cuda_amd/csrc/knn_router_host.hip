@@ -173,9 +173,10 @@ int router_nblocks(const knnhip_router *r) { return r->nb; }
 
 // First probe of each of Q queries (dim-major, ld): the non-empty block
 // (counts[b] != 0; counts may be null) with the highest score, into top1
-// and/or as lb[top1 * Q + q] = -inf.  One fused kernel where the router's
-// shape allows it (hidden 64, 128 or 256; dim up to 512), else the score
-// matrix and the index's first-probe kernel (lb only).
+// and/or as bound (top1, q) = -inf of the tiled bound matrix lb (lb_at).
+// One fused kernel where the router's shape allows it (hidden 64, 128 or
+// 256; dim up to 512), else the score matrix and the index's first-probe
+// kernel (lb only).
 bool router_pick_first(knnhip_router *r, const float *q, int Q, long long ld, const int *counts,
                        int *top1, float *lb, hipStream_t s) {
     const int dpad = (r->dim + 15) / 16 * 16;
