@@ -181,9 +181,10 @@ void knnhip_index_free(knnhip_index *index);
 int knnhip_index_info(const knnhip_index *index, int *nblocks, int *n_pos, float *build_ms);
 /* The lower bounds a search ranks and prunes by: lb[b * query_nb + q] (device,
  * nblocks x query_nb) <= the distance from query q to every point of block b
- * (+inf for an empty block).  precision = 1: bfloat16 three-term products
- * (dim <= 128), as searches with options.precision = 1 use.  For tests and
- * diagnostics. */
+ * (+inf for an empty block).  precision bit 0 (1): bfloat16 three-term
+ * products (dim <= 128), as searches with options.precision = 1 use; bit 1
+ * (2): the per-ball fold form of the same bounds instead of the slot form
+ * that searches use (they agree bit for bit).  For tests and diagnostics. */
 int knnhip_index_bounds(knnhip_index *index, const float *query, int query_nb, long long query_ld,
                         int precision, float *lb, void *stream);
 /* Versioned file of an index's build (block layout, permutation, balls; see

@@ -381,6 +381,15 @@ struct knnhip_index {
     // by the first precision search; fcent_nch chunks
     int fcent_nch = 0;
     knnhip::DevBuf fcent_bf;
+    // slot layout of the bounds (idx_ball_lb_slots): nsl slots (0: a block
+    // holds more than 32 balls, the per-ball fold is used), the ball of every
+    // slot, per tile its mode word, per group of four slots its block; the
+    // per-slot centres, norms, roots and radii are gathered by the first
+    // search (sl_ready), the bfloat16 centres for sl_nch chunks
+    int nsl = 0;
+    bool sl_ready = false;
+    int sl_nch = 0;
+    knnhip::DevBuf sl_ball, sl_mode, sl_blk, sl_cent, sl_nrm, sl_sq, sl_rad, sl_bf;
     // position range of every ball (block-sorted order): its members occupy
     // positions ball_pos[f] .. ball_pos[f] + ball_cnt[f]; the ranges of a
     // block's balls partition the block.  Known for built indexes and for

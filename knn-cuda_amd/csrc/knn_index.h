@@ -101,6 +101,19 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
                        const float *fradf, const int *block_s, const int *fb_start, int nb, float *lb,
                        hipStream_t s, const unsigned short *cbf = nullptr, int nch_bf = 0,
                        float g15_bf = 0.0f);
+// Slot form of the bounds (knn_index.hip, ball_lb_slot_kernel): each block
+// in the smallest of 4, 8, 16 or 32 aligned slots that holds its balls, per
+// slot the ball (sball, -1 empty), per tile a mode word and per group of 4
+// slots the block it stores (built by the host from the blocks' ball counts).
+// The same bounds as idx_ball_lb, every entry of the matrix written.
+hipError_t idx_ball_slot_gather(const int *sball, int nsl, const float *fcent, int pm_ld, const float *fnorm,
+                                const float *fsq, const float *fradf, float *scent, float *snrm, float *ssq,
+                                float *srad, hipStream_t s);
+bool idx_ball_lb_slots_fit(int dim, int pm_ld, int nch_bf);
+hipError_t idx_ball_lb_slots(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm, const float *scent,
+                             const unsigned short *sbf, int nch_bf, float g15_bf, int nsl, const float *snrm,
+                             const float *ssq, const float *srad, const int *sblk, const int *smode, int nb,
+                             float *lb, hipStream_t s);
 hipError_t idx_pack_ball_bf3(const float *fcent, int pm_ld, int nf, int dim, int nch, unsigned short *out,
                              hipStream_t s);
 // Outputs of one problem of a batched class: its queries are columns

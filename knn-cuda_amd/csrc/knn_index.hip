@@ -1386,14 +1386,18 @@ __global__ void ball_refit_finish_kernel(const unsigned long long *__restrict__ 
 // rounded up, covers both.  The hardware square root (within 1 ulp) is
 // lowered by 2^-21 relative, the radius comes rounded up (ru < 0: empty
 // ball), and the difference is lowered by 2^-22 relative.
-__device__ __forceinline__ float ball_lower_bound(float nq, float sq, float nc, float sc, float dot,
-                                                  float ru, float g15) {
-    if (ru < 0.0f) return INFINITY;
+__device__ __forceinline__ float ball_lower_bound_core(float nq, float sq, float nc, float sc, float dot, float ru,
+                                                       float g15) {
     const float st = sq + sc;
     const float a = st * st * (1.0f + 0x1p-20f) * (g15 + 0x1p-21f);
     const float d2 = fmaxf(fmaf(-2.0f, dot, nq + nc) - a, 0.0f);
     const float v = __builtin_amdgcn_sqrtf(d2) * (1.0f - 0x1p-21f) - ru;
-    return v > 0.0f ? v * (1.0f - 0x1p-22f) : 0.0f;
+    return fmaxf(v, 0.0f) * (1.0f - 0x1p-22f);  // (ru = -inf: +inf)
+}
+__device__ __forceinline__ float ball_lower_bound(float nq, float sq, float nc, float sc, float dot,
+                                                  float ru, float g15) {
+    if (ru < 0.0f) return INFINITY;
+    return ball_lower_bound_core(nq, sq, nc, sc, dot, ru, g15);
 }
 
 // Per ball: the square root of the centre's squared norm and the radius,
@@ -1593,13 +1597,23 @@ __device__ __forceinline__ void ball_query_ops(const float *__restrict__ qpm, in
         const float *qr = qpm + (long long)qa * pm_ld;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
+            const int d0 = 16 * c + 8 * h;
+            float x[8];
+            if (d0 + 8 <= dim && pm_ld % 4 == 0) {
+                // eight dimensions inside the row: two 16-byte loads
+                const float4 u = *reinterpret_cast<const float4 *>(qr + d0);
+                const float4 w = *reinterpret_cast<const float4 *>(qr + d0 + 4);
+                x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w;
+                x[4] = w.x, x[5] = w.y, x[6] = w.z, x[7] = w.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) x[e] = d0 + e < dim ? qr[d0 + e] : 0.0f;
+            }
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int d = 16 * c + 8 * h + e;
-                const float x = d < dim ? qr[d] : 0.0f;
-                const __bf16 hi = (__bf16)x;
+                const __bf16 hi = (__bf16)x[e];
                 bh[c][e] = hi;
-                bl[c][e] = (__bf16)(x - (float)hi);
+                bl[c][e] = (__bf16)(x[e] - (float)hi);
             }
         }
     } else {
@@ -1610,11 +1624,13 @@ __device__ __forceinline__ void ball_query_ops(const float *__restrict__ qpm, in
     }
 }
 
-// Bounds of one 32-ball tile (balls f0 .. f0 + 31, staged from F0) against a
-// wave's 32 queries: out[i] is ball f0 + 8(i/4) + 4h + i%4 for query col
-// (+inf past the last ball).
+// Bounds of one 32-ball tile (balls f0 .. f0 + 31, whose squared norms, their
+// roots and radii sit in LDS from tnrm / tsq / trad on) against a wave's 32
+// queries: out[i] is ball f0 + 8(i/4) + 4h + i%4 for query col (+inf past
+// the last ball).
 template <int KHM, int NCH>
-__device__ __forceinline__ void ball_tile_bounds(const BallStage &st, int F0, int f0, int nf, int pm_ld, int dim,
+__device__ __forceinline__ void ball_tile_bounds(const float *tnrm, const float *tsq, const float *trad, int f0,
+                                                 int nf, int pm_ld, int dim,
                                                  int col, int h, const float *__restrict__ fcent,
                                                  const unsigned short *__restrict__ cbf,
                                                  const float4 (&bq)[NCH ? 1 : KHM / 4],
@@ -1657,10 +1673,10 @@ __device__ __forceinline__ void ball_tile_bounds(const BallStage &st, int F0, in
     // their roots and radii
 #pragma unroll
     for (int g4 = 0; g4 < 4; ++g4) {
-        const int j = f0 - F0 + 8 * g4 + 4 * h;
-        const float4 n4 = *reinterpret_cast<const float4 *>(st.nrm + j);
-        const float4 s4 = *reinterpret_cast<const float4 *>(st.sq + j);
-        const float4 r4 = *reinterpret_cast<const float4 *>(st.rad + j);
+        const int j = 8 * g4 + 4 * h;
+        const float4 n4 = *reinterpret_cast<const float4 *>(tnrm + j);
+        const float4 s4 = *reinterpret_cast<const float4 *>(tsq + j);
+        const float4 r4 = *reinterpret_cast<const float4 *>(trad + j);
         out[4 * g4 + 0] = ball_lower_bound(nq, sq, n4.x, s4.x, acc[4 * g4 + 0], r4.x, g15);
         out[4 * g4 + 1] = ball_lower_bound(nq, sq, n4.y, s4.y, acc[4 * g4 + 1], r4.y, g15);
         out[4 * g4 + 2] = ball_lower_bound(nq, sq, n4.z, s4.z, acc[4 * g4 + 2], r4.z, g15);
@@ -1728,7 +1744,8 @@ __global__ __launch_bounds__(256, 4) void ball_lb_reg_kernel(const float *__rest
         const int f0 = F0 + tt * 32;
         if (f0 >= F1) break;  // uniform over the wave
         float out[16];
-        ball_tile_bounds<KHM, NCH>(st, F0, f0, nf, pm_ld, dim, col, h, fcent, cbf, bq, bh, bl, nq, sq, g15, out);
+        ball_tile_bounds<KHM, NCH>(st.nrm + (f0 - F0), st.sq + (f0 - F0), st.rad + (f0 - F0), f0, nf, pm_ld, dim,
+                                   col, h, fcent, cbf, bq, bh, bl, nq, sq, g15, out);
         // the partner half's rows (a lane swap across the halves), and
         // (lane r < 32) row r's block and that block's first ball and one
         // past its last
@@ -1760,6 +1777,216 @@ __global__ __launch_bounds__(256, 4) void ball_lb_reg_kernel(const float *__rest
         }
     }
     flush();
+}
+
+// Slot form of the same bounds.  Every block takes the smallest of 4, 8, 16
+// or 32 consecutive slots that holds its balls (slots past its balls are
+// empty balls: radius -1, bound +inf), blocks by descending slot count, so
+// every block starts at a multiple of its size and never crosses a 32-slot
+// tile.  In the accumulator of a tile, slots 8j + 4h .. 8j + 4h + 3 (group
+// (j, h)) are the rows 4j .. 4j + 3 of lane half h, so one lane holds a whole
+// 4-slot block: a block's bound is a min over one lane's four rows, over the
+// two halves (8 slots: one lane swap), over two groups of both halves (16)
+// or over the tile (32).  A tile's mode word holds 2 bits per j (0: the
+// groups (j, 0) and (j, 1) are two 4-slot blocks, 1: one 8-slot block, 2: a
+// 16-slot block over j and j ^ 1, 3: one 32-slot block) and blk[8t + 2j + h]
+// the block that group (j, h) stores (-1: none, or not its first group).
+// Each block is stored once per query, so the matrix needs no +inf fill
+// and no atomics; the bounds equal the per-ball fold's bit for bit (the same
+// products and the same per-ball formula, then an exact min).
+constexpr int kSlotTilesPerWg = 16;  // slot tiles per workgroup for many queries
+constexpr int kSlotWaves = 8;        // waves (of 32 queries) per workgroup
+struct SlotStage {
+    float nrm[kSlotTilesPerWg * 32], sq[kSlotTilesPerWg * 32], rad[kSlotTilesPerWg * 32];
+    int blk[kSlotTilesPerWg * 8];
+    int mode[kSlotTilesPerWg];
+};
+
+// Per-slot copies of the per-ball arrays: centres (rows of pm_ld floats),
+// squared norms, their roots and the radii rounded up; an empty slot or
+// empty ball has a zero centre and radius -inf, which the bound formula
+// turns into +inf without a branch.
+__global__ void ball_slot_gather_kernel(const int *__restrict__ sball, int nsl, const float *__restrict__ fcent,
+                                        int pm_ld, const float *__restrict__ fnorm, const float *__restrict__ fsq,
+                                        const float *__restrict__ fradf, float *__restrict__ scent,
+                                        float *__restrict__ snrm, float *__restrict__ ssq, float *__restrict__ srad) {
+    const long long n = (long long)nsl * pm_ld;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int sl = (int)(e / pm_ld), d = (int)(e - (long long)sl * pm_ld);
+        const int f = sball[sl];
+        scent[e] = f >= 0 ? fcent[(long long)f * pm_ld + d] : 0.0f;
+        if (d == 0) {
+            snrm[sl] = f >= 0 ? fnorm[f] : 0.0f;
+            ssq[sl] = f >= 0 ? fsq[f] : 0.0f;
+            srad[sl] = f >= 0 && fradf[f] >= 0.0f ? fradf[f] : -INFINITY;
+        }
+    }
+}
+
+// Workgroup barrier for LDS hand-offs that waits for this wave's LDS
+// operations only (global loads in flight keep running).
+__device__ __forceinline__ void slot_lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// A workgroup: kSlotWaves waves of 32 queries x `tpw` consecutive slot
+// tiles, as ball_lb_reg_kernel; the tiles' per-slot data and store map are staged
+// in LDS.  Query columns past Q (up to Q rounded up to 64, the matrix's row
+// groups) store +inf, so every entry of the matrix is written.
+// NCH > 0: the workgroup also stages each tile's bfloat16 centres (32 slots
+// x NCH x 64 bytes) in LDS, loaded once for all its waves: two buffers, one
+// barrier per tile.  At the top of iteration tt the barrier makes tile tt
+// (stored into buffer tt & 1 during iteration tt - 1, or before the loop)
+// visible, and it follows every wave's reads of buffer (tt + 1) & 1 (made in
+// iteration tt - 1), which iteration tt then overwrites with tile tt + 1.
+// Every wave runs every iteration (waves past the last query compute the
+// last query's bounds and store nothing), so every barrier is reached by all.
+template <int KHM, int NCH>
+__global__ __launch_bounds__(64 * kSlotWaves) void ball_lb_slot_kernel(const float *__restrict__ qpm, int pm_ld, int Q, int dim,
+                                                           const float *__restrict__ qnorm,
+                                                           const float *__restrict__ scent,
+                                                           const unsigned short *__restrict__ sbf, int nsl,
+                                                           const float *__restrict__ snrm,
+                                                           const float *__restrict__ ssq,
+                                                           const float *__restrict__ srad,
+                                                           const int *__restrict__ sblk,
+                                                           const int *__restrict__ smode, float *__restrict__ lb,
+                                                           int nb, int tpw, float g15_bf) {
+    constexpr int SLOT_B = NCH * 64 + 16;     // slot stride of the staged centres (bytes, padded)
+    constexpr int NPIECE = NCH * 128;         // 16-byte pieces of one tile's centres
+    constexpr int NT = 64 * kSlotWaves;       // threads
+    constexpr int PPT = (NPIECE + NT - 1) / NT; // pieces per thread
+    __shared__ __align__(16) SlotStage st;
+    __shared__ __align__(16) unsigned char abuf[NCH ? 2 : 1][NCH ? 32 * SLOT_B : 16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 31, h = lane >> 5;
+    const int ntile = nsl / 32;
+    const int ngrp = (ntile + tpw - 1) / tpw;
+    const int qg = blockIdx.x / ngrp, grp = blockIdx.x - qg * ngrp;
+    const int T0 = grp * tpw, nt = min(ntile, T0 + tpw) - T0;
+    for (int j = threadIdx.x; j < nt * 32; j += blockDim.x) {
+        st.nrm[j] = snrm[T0 * 32 + j];
+        st.sq[j] = ssq[T0 * 32 + j];
+        st.rad[j] = srad[T0 * 32 + j];
+    }
+    for (int j = threadIdx.x; j < nt * 8; j += blockDim.x) st.blk[j] = sblk[T0 * 8 + j];
+    if (threadIdx.x < nt) st.mode[threadIdx.x] = smode[T0 + threadIdx.x];
+    const int q0 = qg * (32 * kSlotWaves) + wave * 32;
+    const bool live = q0 < ((Q + 63) & ~63);  // wave-uniform: the wave's columns are rows of the matrix
+    const int q = q0 + col;
+    const int qa = min(q, Q - 1);
+    float4 bq[NCH ? 1 : KHM / 4];
+    bf16x8_t bh[NCH ? NCH : 1], bl[NCH ? NCH : 1];
+    ball_query_ops<KHM, NCH>(qpm, pm_ld, qa, dim, h, bq, bh, bl);
+    const float g15 = NCH > 0 ? g15_bf : ball_error_coeff(dim);
+    const float nq = qnorm[qa];
+    const float sq = nextafterf(sqrtf(nq), INFINITY);
+    const bool real = q < Q;
+    constexpr int NA = NCH ? 2 * NCH : KHM / 4;
+    const int KH = (dim + 1) / 2;
+    u32x4_t av[NA];
+    // NCH > 0: tile pieces in flight (registers) and their LDS stores
+    u32x4_t pf[NCH ? PPT : 1];
+    const unsigned char *sbytes = reinterpret_cast<const unsigned char *>(sbf);
+    auto fetch = [&](int tile) {
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int pc = threadIdx.x + NT * i;
+            if (pc < NPIECE)
+                pf[i] = *reinterpret_cast<const u32x4_t *>(sbytes + ((long long)tile * NPIECE + pc) * 16);
+        }
+    };
+    auto put = [&](int b) {
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            const int pc = threadIdx.x + NT * i;
+            if (pc < NPIECE) {
+                const int sl = pc / (NCH * 4), w = pc - sl * (NCH * 4);
+                *reinterpret_cast<u32x4_t *>(&abuf[b][sl * SLOT_B + w * 16]) = pf[i];
+            }
+        }
+    };
+    // FP32 products: A operands from global memory, one tile ahead
+    auto load_a = [&](int f0) {
+        const float *ca = scent + (long long)(f0 + col) * pm_ld + h * KH;
+#pragma unroll
+        for (int u = 0; u < NA; ++u)
+            av[u] = 4 * u < KH ? *reinterpret_cast<const u32x4_t *>(ca + 4 * u) : u32x4_t{0, 0, 0, 0};
+    };
+    if constexpr (NCH > 0) {
+        if (nt > 0) {
+            fetch(T0);
+            put(0);
+        }
+    } else {
+        if (nt > 0) load_a(T0 * 32);
+    }
+    __syncthreads();  // the staged slot data (and tile 0's centres)
+    for (int tt = 0; tt < nt; ++tt) {
+        f32x16_t acc = {};
+        if constexpr (NCH > 0) {
+            if (tt > 0) slot_lds_barrier();
+            if (tt + 1 < nt) fetch(T0 + tt + 1);
+            const unsigned char *ab = &abuf[tt & 1][col * SLOT_B + 16 * h];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                av[2 * c] = *reinterpret_cast<const u32x4_t *>(ab + 64 * c);
+                av[2 * c + 1] = *reinterpret_cast<const u32x4_t *>(ab + 64 * c + 32);
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const bf16x8_t ah = __builtin_bit_cast(bf16x8_t, av[2 * c]);
+                const bf16x8_t al = __builtin_bit_cast(bf16x8_t, av[2 * c + 1]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[c], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[c], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[c], acc, 0, 0, 0);
+            }
+            if (tt + 1 < nt) put((tt + 1) & 1);
+        } else {
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                if (4 * u < KH) {
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av[u][0]), bq[u].x, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av[u][1]), bq[u].y, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av[u][2]), bq[u].z, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av[u][3]), bq[u].w, acc, 0, 0, 0);
+                }
+            }
+            if (tt + 1 < nt) load_a((T0 + tt + 1) * 32);
+        }
+        // the bounds of the tile's 32 slots (rows 8(i/4) + 4h + i%4); empty
+        // slots (radius -inf) come out +inf
+        float out[16];
+#pragma unroll
+        for (int g4 = 0; g4 < 4; ++g4) {
+            const int j = 32 * tt + 8 * g4 + 4 * h;
+            const float4 n4 = *reinterpret_cast<const float4 *>(st.nrm + j);
+            const float4 s4 = *reinterpret_cast<const float4 *>(st.sq + j);
+            const float4 r4 = *reinterpret_cast<const float4 *>(st.rad + j);
+            out[4 * g4 + 0] = ball_lower_bound_core(nq, sq, n4.x, s4.x, acc[4 * g4 + 0], r4.x, g15);
+            out[4 * g4 + 1] = ball_lower_bound_core(nq, sq, n4.y, s4.y, acc[4 * g4 + 1], r4.y, g15);
+            out[4 * g4 + 2] = ball_lower_bound_core(nq, sq, n4.z, s4.z, acc[4 * g4 + 2], r4.z, g15);
+            out[4 * g4 + 3] = ball_lower_bound_core(nq, sq, n4.w, s4.w, acc[4 * g4 + 3], r4.w, g15);
+        }
+        // group minima (one lane's four rows), the partner half's, and the
+        // minima over 8, 16 and 32 slots
+        float gm[4], m8[4];
+#pragma unroll
+        for (int g4 = 0; g4 < 4; ++g4) {
+            gm[g4] = fminf(fminf(out[4 * g4], out[4 * g4 + 1]), fminf(out[4 * g4 + 2], out[4 * g4 + 3]));
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(gm[g4]), __float_as_uint(gm[g4]),
+                                                             false, false);
+            m8[g4] = fminf(gm[g4], __uint_as_float(h ? sw[0] : sw[1]));
+        }
+        const float m32 = fminf(fminf(m8[0], m8[1]), fminf(m8[2], m8[3]));
+        const int mode = __builtin_amdgcn_readfirstlane(st.mode[tt]);
+#pragma unroll
+        for (int g4 = 0; g4 < 4; ++g4) {
+            const int md = (mode >> (2 * g4)) & 3;
+            const float v = md == 0 ? gm[g4] : md == 1 ? m8[g4] : md == 2 ? fminf(m8[g4], m8[g4 ^ 1]) : m32;
+            const int b = st.blk[8 * tt + 2 * g4 + h];
+            if (live && b >= 0) lb[lb_at(b, q, nb)] = real ? v : INFINITY;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2031,6 +2258,49 @@ hipError_t idx_ball_lb(const float *qpm, int pm_ld, int Q, int dim, const float 
                        pm_ld, Q, dim, qnorm, fcent, nf, fnorm, fsq, fradf, block_s, fb_start, lb, nb);
     return hipGetLastError();
 }
+hipError_t idx_ball_slot_gather(const int *sball, int nsl, const float *fcent, int pm_ld, const float *fnorm,
+                                const float *fsq, const float *fradf, float *scent, float *snrm, float *ssq,
+                                float *srad, hipStream_t s) {
+    hipLaunchKernelGGL(ball_slot_gather_kernel, dim3(grid_for((long long)nsl * pm_ld)), dim3(256), 0, s, sball, nsl,
+                       fcent, pm_ld, fnorm, fsq, fradf, scent, snrm, ssq, srad);
+    return hipGetLastError();
+}
+bool idx_ball_lb_slots_fit(int dim, int pm_ld, int nch_bf) {
+    if (nch_bf) return nch_bf >= 1 && nch_bf <= 8 && 16 * nch_bf >= dim;
+    const int KH = (dim + 1) / 2;
+    return KH % 4 == 0 && pm_ld % 4 == 0 && KH <= 64;
+}
+hipError_t idx_ball_lb_slots(const float *qpm, int pm_ld, int Q, int dim, const float *qnorm, const float *scent,
+                             const unsigned short *sbf, int nch_bf, float g15_bf, int nsl, const float *snrm,
+                             const float *ssq, const float *srad, const int *sblk, const int *smode, int nb,
+                             float *lb, hipStream_t s) {
+    if (nsl < 32 || nsl % 32 != 0 || !idx_ball_lb_slots_fit(dim, pm_ld, nch_bf)) return hipErrorInvalidValue;
+    const long long qgs = (Q + 32 * kSlotWaves - 1) / (32 * kSlotWaves), ntile = nsl / 32;
+    const int tpw = (int)std::max(1ll, std::min<long long>(kSlotTilesPerWg, qgs * ntile / kBallMinWgs));
+    const long long wgs = qgs * ((ntile + tpw - 1) / tpw);
+#define SL(KHM, N)                                                                                               \
+    hipLaunchKernelGGL((ball_lb_slot_kernel<KHM, N>), dim3((unsigned)wgs), dim3(64 * kSlotWaves), 0, s, qpm, pm_ld, Q, dim,   \
+                       qnorm, scent, sbf, nsl, snrm, ssq, srad, sblk, smode, lb, nb, tpw, g15_bf);               \
+    return hipGetLastError();
+    if (nch_bf) {
+        switch (nch_bf) {
+        case 1: SL(4, 1)
+        case 2: SL(4, 2)
+        case 3: SL(4, 3)
+        case 4: SL(4, 4)
+        case 5: SL(4, 5)
+        case 6: SL(4, 6)
+        case 7: SL(4, 7)
+        default: SL(4, 8)
+        }
+    }
+    if ((dim + 1) / 2 <= 32) {
+        SL(32, 0)
+    }
+    SL(64, 0)
+#undef SL
+}
+
 // Batched outputs: element e = j * Qtot + q of the class's k x Qtot results
 // goes to row j of the problem whose queries hold column q (outs by
 // ascending qstart), at its own column q - qstart.

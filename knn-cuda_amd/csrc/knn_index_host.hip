@@ -86,6 +86,52 @@ bool ball_ranges_set(knnhip_index *ix, const int *pos, const int *cnt, const int
     return true;
 }
 
+// Slot layout of the bounds from the blocks' ball ranges fbs (host, nb + 1):
+// block b takes the smallest of 4, 8, 16, 32 slots that holds its balls,
+// blocks by descending slot count (then by index), so each block starts at a
+// multiple of its slot count; see ball_lb_slot_kernel.  A block of more than
+// 32 balls leaves the index without a slot layout (nsl = 0).
+bool ball_slots_set(knnhip_index *ix, const int *fbs) {
+    ix->nsl = 0;
+    ix->sl_ready = false;
+    ix->sl_nch = 0;
+    const int nb = ix->nb;
+    std::vector<int> size(nb), order(nb);
+    for (int b = 0; b < nb; ++b) {
+        const int n = fbs[b + 1] - fbs[b];
+        int sz = 4;
+        while (sz < n) sz *= 2;
+        if (sz > 32) return true;
+        size[b] = sz;
+        order[b] = b;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return size[x] > size[y]; });
+    long long total = 0;
+    for (int b = 0; b < nb; ++b) total += size[b];
+    const int nsl = (int)((total + 31) / 32 * 32);
+    const int ntile = nsl / 32;
+    std::vector<int> ball(nsl, -1), blk((size_t)ntile * 8, -1), mode(ntile, 0);
+    int pos = 0;
+    for (int b : order) {
+        const int sz = size[b];
+        for (int f = fbs[b]; f < fbs[b + 1]; ++f) ball[pos + (f - fbs[b])] = f;
+        const int t = pos / 32, o = pos % 32, j = o / 8, h = (o % 8) / 4;
+        blk[(size_t)t * 8 + 2 * j + h] = b;
+        const int code = sz == 4 ? 0 : sz == 8 ? 1 : sz == 16 ? 2 : 3;
+        for (int g = j; g < j + std::max(1, sz / 8); ++g) mode[t] |= code << (2 * g);
+        pos += sz;
+    }
+    auto upload = [](DevBuf &d, const std::vector<int> &v) {
+        return d.alloc(v.size() * 4) && hipMemcpy(d.p, v.data(), v.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!upload(ix->sl_ball, ball) || !upload(ix->sl_mode, mode) || !upload(ix->sl_blk, blk)) {
+        set_error("Memory allocation error");
+        return false;
+    }
+    ix->nsl = nsl;
+    return true;
+}
+
 // Block index build.
 //  1. Fine balls: k-means over the references with kBallsPerBlock * nb
 //     centres (k-means++ seeding on an evenly strided sample of kSeedPerCentre
@@ -266,6 +312,7 @@ bool index_build(knnhip_index *ix, const float *ref, int R, long long ldr, int d
             bc[j] = hc[hs[j]];
         }
         if (!ball_ranges_set(ix, bp.data(), bc.data(), hfb.data())) return false;
+        if (!ball_slots_set(ix, hfb.data())) return false;
     }
     KNN_TRY(hipEventRecord(e1, s), "event");
     KNN_TRY(hipEventSynchronize(e1), "event");
@@ -466,6 +513,7 @@ bool index_load(knnhip_index *ix, const char *path, const float *ref, int R, lon
     } else {
         ix->ball_ranges = false;
     }
+    if (!ball_slots_set(ix, fbs.data())) return false;
     auto upload = [](DevBuf &b, const void *src, size_t n) {
         return b.alloc(n) && hipMemcpy(b.p, src, n, hipMemcpyHostToDevice) == hipSuccess;
     };
@@ -569,12 +617,55 @@ bool ball_bf3_ready(knnhip_index *ix, int nch, hipStream_t s) {
     return true;
 }
 
+// The per-slot arrays of the slot-form bounds (once per index), and the
+// bfloat16 slot centres for nch chunks.
+bool ball_slots_ready(knnhip_index *ix, int nch, hipStream_t s) {
+    const int nsl = ix->nsl;
+    if (!ix->sl_ready) {
+        if (!ix->sl_cent.alloc((size_t)nsl * ix->pm_ld * 4) || !ix->sl_nrm.alloc((size_t)nsl * 4) ||
+            !ix->sl_sq.alloc((size_t)nsl * 4) || !ix->sl_rad.alloc((size_t)nsl * 4)) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        KNN_TRY(idx_ball_slot_gather(ix->sl_ball.as<int>(), nsl, ix->fcent.as<float>(), ix->pm_ld,
+                                     ix->fnorm.as<float>(), ix->fsq.as<float>(), ix->fradf.as<float>(),
+                                     ix->sl_cent.as<float>(), ix->sl_nrm.as<float>(), ix->sl_sq.as<float>(),
+                                     ix->sl_rad.as<float>(), s),
+                "bound slots");
+        ix->sl_ready = true;
+    }
+    if (nch && ix->sl_nch != nch) {
+        ix->sl_nch = 0;
+        if (!ix->sl_bf.alloc((size_t)nsl * nch * 32 * 2)) {
+            set_error("Memory allocation error");
+            return false;
+        }
+        KNN_TRY(idx_pack_ball_bf3(ix->sl_cent.as<float>(), ix->pm_ld, nsl, ix->dim, nch,
+                                  ix->sl_bf.as<unsigned short>(), s),
+                "bfloat16 slot centres");
+        ix->sl_nch = nch;
+    }
+    return true;
+}
+
 // Block lower bounds of the Q queries whose norms and point-major rows are
 // in qnorm / qpm, into lb (nb x Q, block-major); bfloat16 three-term
-// products with `precision` where the dimension allows (<= 128).
+// products with `precision` where the dimension allows (<= 128).  The slot
+// form where the index has one and the dimension fits it, unless per_ball
+// asks for the per-ball fold (the same bounds).
 bool index_bounds(knnhip_index *ix, const float *qpm, const float *qnorm, int Q, bool precision, float *lb,
-                  hipStream_t s) {
+                  hipStream_t s, bool per_ball = false) {
     const int nch = precision && ix->dim <= 128 ? (ix->dim + 15) / 16 : 0;
+    if (!per_ball && ix->nsl > 0 && idx_ball_lb_slots_fit(ix->dim, ix->pm_ld, nch)) {
+        if (!ball_slots_ready(ix, nch, s)) return false;
+        KNN_TRY(idx_ball_lb_slots(qpm, ix->pm_ld, Q, ix->dim, qnorm, ix->sl_cent.as<float>(),
+                                  nch ? ix->sl_bf.as<unsigned short>() : nullptr, nch,
+                                  nch ? ball_error_coeff_bf3(ix->dim, nch) : 0.0f, ix->nsl, ix->sl_nrm.as<float>(),
+                                  ix->sl_sq.as<float>(), ix->sl_rad.as<float>(), ix->sl_blk.as<int>(),
+                                  ix->sl_mode.as<int>(), ix->nb, lb, s),
+                "block bounds");
+        return true;
+    }
     if (nch && !ball_bf3_ready(ix, nch, s)) return false;
     KNN_TRY(idx_ball_lb(qpm, ix->pm_ld, Q, ix->dim, qnorm, ix->fcent.as<float>(), ix->nf,
                         ix->fnorm.as<float>(), ix->fsq.as<float>(), ix->fradf.as<float>(),
@@ -1060,7 +1151,7 @@ int knnhip_index_bounds(knnhip_index *ix, const float *query, int query_nb, long
                         int precision, float *lb, void *stream) {
     std::lock_guard<std::mutex> lock(g_mu);
     g_last_error.clear();
-    if (!ix || !query || !lb || query_nb < 1 || query_ld < query_nb) {
+    if (!ix || !query || !lb || query_nb < 1 || query_ld < query_nb || precision < 0 || precision > 3) {
         set_error("bad bound arguments");
         return 0;
     }
@@ -1088,7 +1179,8 @@ int knnhip_index_bounds(knnhip_index *ix, const float *query, int query_nb, long
         return 0;
     }
     // (the searches' tiled layout, then rows of query_nb)
-    if (!index_bounds(ix, w.qpm.as<float>(), w.qnorm.as<float>(), query_nb, precision == 1, w.lb.as<float>(), s))
+    if (!index_bounds(ix, w.qpm.as<float>(), w.qnorm.as<float>(), query_nb, (precision & 1) != 0,
+                      w.lb.as<float>(), s, (precision & 2) != 0))
         return 0;
     if (idx_lb_untile(w.lb.as<float>(), query_nb, ix->nb, lb, s) != hipSuccess) {
         set_error("bound layout kernel launch");

@@ -541,16 +541,17 @@ class BlockIndex:
         torch.cuda.synchronize(self.ref.device)
         return start, count, ref_block
 
-    def bounds(self, query, precision: str = "fp32", stream: Optional[int] = None):
-        """The block lower bounds searches rank and prune by: (nblocks, query_nb) float32."""
+    def bounds(self, query, precision: str = "fp32", per_ball: bool = False, stream: Optional[int] = None):
+        """The block lower bounds searches rank and prune by: (nblocks, query_nb) float32.
+        ``per_ball``: the per-ball fold form of the same bounds (searches use the slot form)."""
         import torch
 
         _check_points("query", query, dim=self.dim)
         _check_device(query, self.ref)
         out = torch.empty((self.nblocks, query.shape[1]), dtype=torch.float32, device=query.device)
+        flags = (1 if precision == "bf16x3" else 0) | (2 if per_ball else 0)
         ok = load().knnhip_index_bounds(self._h, query.data_ptr(), query.shape[1], query.stride(0),
-                                        1 if precision == "bf16x3" else 0, out.data_ptr(),
-                                        _stream_of(query, stream))
+                                        flags, out.data_ptr(), _stream_of(query, stream))
         if not ok:
             raise KnnError(last_error())
         return out

@@ -257,3 +257,25 @@ def test_block_bounds_never_exceed_the_true_distance(dim, prec):
         fin = torch.isfinite(fp)
         assert bool((fp[fin] - lb[fin]).abs().max() <= 1e-2 * (1 + fp[fin].abs().max()))
     ix.close()
+
+
+@pytest.mark.parametrize("dim,prec,nb,Q", [(96, "bf16x3", 2048, 16384), (96, "fp32", 256, 1000),
+                                           (24, "bf16x3", 8, 77), (24, "fp32", 5, 33), (128, "bf16x3", 512, 300),
+                                           (7, "bf16x3", 64, 257), (200, "bf16x3", 64, 129)])
+def test_slot_bounds_equal_the_per_ball_fold(dim, prec, nb, Q):
+    # the searches' bounds kernel (blocks in 4/8/16/32-slot groups, minima over lanes) against the
+    # per-ball fold (bounds ball by ball, folded block by block): the same per-ball bounds, so the
+    # same minima, bit for bit; few blocks give blocks of many balls (8- to 32-slot groups), and
+    # query counts that are not multiples of 32 or 64 test the padded columns
+    R = 200000 if nb == 2048 else 30000
+    say(f"slot bounds dim={dim} {prec} nb={nb} Q={Q}")
+    ref = knnhip.generate_mixture(R, dim, 256, 0.05, 51 + dim, 52)
+    qry = knnhip.generate_mixture(Q, dim, 256, 0.05, 51 + dim, 53)
+    ix = knnhip.BlockIndex(ref, nblocks=nb, iters=4)
+    a = ix.bounds(qry, precision=prec)
+    b = ix.bounds(qry, precision=prec, per_ball=True)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32)), \
+        f"{int((a != b).sum())} bounds differ between the slot form and the per-ball fold"
+    assert bool(torch.isfinite(a).any())
+    ix.close()
