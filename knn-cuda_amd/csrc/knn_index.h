@@ -29,9 +29,10 @@ hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float
                            const int *counts, int *chosen, float *lb_rest, hipStream_t s);
 hipError_t idx_router_first(const float *score, int Q, int nb, const int *counts, float *lb,
                             hipStream_t s);
-hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int *counts,
+// Units outside [0, nu) (and -1) are no unit.
+hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int nu, int *counts,
                           hipStream_t s);
-hipError_t idx_pair_scatter(const int *chosen, int Q, int P, int p0, int p1, int *cursor,
+hipError_t idx_pair_scatter(const int *chosen, int Q, int P, int p0, int p1, int nu, int *cursor,
                             int *qlist, int *rows, int row_base, hipStream_t s);
 hipError_t idx_gather_columns(const float *src, long long ld, int dim, const int *list, int n,
                               float *dst, hipStream_t s);

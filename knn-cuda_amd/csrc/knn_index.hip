@@ -843,17 +843,163 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_select_kernel(
     if (lane == 0 && lbmin < lb_rest[q]) lb_rest[q] = lbmin;
 }
 
+// The same selection with one candidate per lane (at most 128 candidates,
+// two per lane, and at most 64 probed blocks): lane i computes candidate i's
+// FP64 centre distance over the dimensions in order, and the reach test,
+// the pass count and the ranks run as wave-uniform loops over the
+// candidates, reading each one's values from its lane (readlane), so no
+// LDS round trips sit on the critical path.  Same candidates, bounds, T_q
+// and ranks as ball_select_kernel (the FP64 sums differ only in their
+// summation order, far inside the 1e-12 allowance).
+constexpr int kSelLaneCand = 2;  // candidates per lane
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    return __hiloint2double(hi, lo);
+}
+__global__ __launch_bounds__(64 * kBallWaves) void ball_select_lanes_kernel(
+    const float *__restrict__ qpm, int pm_ld, int dim, int Q, int kp, const int *__restrict__ chosen, int P,
+    const float *__restrict__ fcent, const double *__restrict__ frad, const int *__restrict__ fcnt,
+    const int *__restrict__ fb_start, int *__restrict__ out, int Pb, float *__restrict__ lb_rest) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * kBallWaves + wave);
+    if (q >= Q) return;  // (no workgroup barrier in this kernel)
+    const long long o0 = (long long)q * Pb;
+    // lane j < P: block j's first ball, its ball count and the candidates
+    // before it (exclusive prefix)
+    int fl = 0, cnt = 0;
+    if (lane < P) {
+        const int b = chosen[(long long)q * P + lane];
+        if (b >= 0) {
+            fl = fb_start[b];
+            cnt = fb_start[b + 1] - fl;
+        }
+    }
+    int pre = cnt;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(pre, o);
+        if (lane >= o) pre += t;
+    }
+    const int n = __builtin_amdgcn_readlane(pre, kWave - 1);
+    const int excl = pre - cnt;
+    // candidate i = lane + 64 k: its block is the last j with excl_j <= i
+    int cf[kSelLaneCand], cm[kSelLaneCand];
+    double crho[kSelLaneCand];
+#pragma unroll
+    for (int k = 0; k < kSelLaneCand; ++k) {
+        const int i = lane + kWave * k;
+        int first = 0, off = 0;
+        for (int j = 0; j < P; ++j) {
+            const int ej = __builtin_amdgcn_readlane(excl, j);
+            const int fj = __builtin_amdgcn_readlane(fl, j);
+            if (ej <= i) {
+                off = ej;
+                first = fj;
+            }
+        }
+        cf[k] = -1;
+        cm[k] = 0;
+        crho[k] = -1.0;
+        if (i < n) {
+            const int f = first + i - off;
+            const double rho = frad[f];
+            if (rho >= 0.0) {
+                cf[k] = f;
+                cm[k] = fcnt[f];
+                crho[k] = rho;
+            }
+        }
+    }
+    // FP64 centre distances, one candidate per lane
+    const float *qr = qpm + (long long)q * pm_ld;
+    double d2[kSelLaneCand];
+#pragma unroll
+    for (int k = 0; k < kSelLaneCand; ++k) d2[k] = 0.0;
+    const bool two = n > kWave;  // wave-uniform
+    for (int d = 0; d < dim; d += 4) {
+        const float4 x = *reinterpret_cast<const float4 *>(qr + d);
+        const double x0 = x.x, x1 = d + 1 < dim ? (double)x.y : 0.0, x2 = d + 2 < dim ? (double)x.z : 0.0,
+                     x3 = d + 3 < dim ? (double)x.w : 0.0;
+#pragma unroll
+        for (int k = 0; k < kSelLaneCand; ++k) {
+            if (k > 0 && !two) break;
+            const int f = cf[k] >= 0 ? cf[k] : 0;
+            const float4 c = *reinterpret_cast<const float4 *>(fcent + (long long)f * pm_ld + d);
+            double t = x0 - (double)c.x;
+            d2[k] = fma(t, t, d2[k]);
+            t = d + 1 < dim ? x1 - (double)c.y : 0.0;
+            d2[k] = fma(t, t, d2[k]);
+            t = d + 2 < dim ? x2 - (double)c.z : 0.0;
+            d2[k] = fma(t, t, d2[k]);
+            t = d + 3 < dim ? x3 - (double)c.w : 0.0;
+            d2[k] = fma(t, t, d2[k]);
+        }
+    }
+    float clb[kSelLaneCand];
+    double cu[kSelLaneCand];
+#pragma unroll
+    for (int k = 0; k < kSelLaneCand; ++k) {
+        const double dd = sqrt(d2[k]);
+        clb[k] = cf[k] >= 0 ? ball_member_lb(dd, crho[k]) : INFINITY;
+        cu[k] = cf[k] >= 0 ? dd * (1.0 + 1e-12) + crho[k] : INFINITY;
+    }
+    // T_q: the smallest reach u_i whose balls u_t <= u_i hold >= kp members
+    long long mem[kSelLaneCand] = {};
+    for (int t = 0; t < n; ++t) {
+        const int kt = t / kWave, lt = t - kt * kWave;  // wave-uniform
+        const double ut = readlane_f64(kt ? cu[kSelLaneCand - 1] : cu[0], lt);
+        const int mt = __builtin_amdgcn_readlane(kt ? cm[kSelLaneCand - 1] : cm[0], lt);
+#pragma unroll
+        for (int k = 0; k < kSelLaneCand; ++k) mem[k] += ut <= cu[k] ? mt : 0;
+    }
+    double T = INFINITY;
+#pragma unroll
+    for (int k = 0; k < kSelLaneCand; ++k)
+        if (cf[k] >= 0 && mem[k] >= kp) T = fmin(T, cu[k]);
+    for (int o = 32; o > 0; o >>= 1) T = fmin(T, __shfl_xor(T, o));
+    // passing candidates (bound below T_q), ranked by (bound, ball)
+    bool pass[kSelLaneCand];
+    int npass = 0;
+#pragma unroll
+    for (int k = 0; k < kSelLaneCand; ++k) {
+        pass[k] = cf[k] >= 0 && (double)clb[k] < T;
+        npass += __popcll(__ballot(pass[k]));
+    }
+    int rank[kSelLaneCand] = {};
+    for (int t = 0; t < n; ++t) {
+        const int kt = t / kWave, lt = t - kt * kWave;
+        const int lbits = __builtin_amdgcn_readlane(__float_as_int(kt ? clb[kSelLaneCand - 1] : clb[0]), lt);
+        const int ft = __builtin_amdgcn_readlane(kt ? cf[kSelLaneCand - 1] : cf[0], lt);
+        const float l2 = __int_as_float(lbits);
+        if (ft < 0 || !((double)l2 < T)) continue;  // wave-uniform
+#pragma unroll
+        for (int k = 0; k < kSelLaneCand; ++k) rank[k] += (l2 < clb[k] || (l2 == clb[k] && ft < cf[k])) ? 1 : 0;
+    }
+    float lbmin = INFINITY;
+#pragma unroll
+    for (int k = 0; k < kSelLaneCand; ++k) {
+        if (cf[k] < 0) continue;
+        if (pass[k] && rank[k] < Pb)
+            out[o0 + rank[k]] = cf[k];
+        else
+            lbmin = fminf(lbmin, clb[k]);
+    }
+    for (int i = min(npass, Pb) + lane; i < Pb; i += kWave) out[o0 + i] = -1;
+    for (int o = 32; o > 0; o >>= 1) lbmin = fminf(lbmin, __shfl_xor(lbmin, o));
+    if (lane == 0 && lbmin < lb_rest[q]) lb_rest[q] = lbmin;
+}
+
 // Invert the (query -> blocks) lists, probes [p0, p1) of each query's P, into
 // per-block query lists and scan tasks of up to 32 queries.  Step 1: count
 // pairs per block.
-__global__ void pair_count_kernel(const int *__restrict__ chosen, int Q, int P, int p0, int p1,
+__global__ void pair_count_kernel(const int *__restrict__ chosen, int Q, int P, int p0, int p1, int nu,
                                   int *__restrict__ counts) {
     const int w = p1 - p0;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)Q * w;
          e += (long long)gridDim.x * blockDim.x) {
         const long long q = e / w;
         const int b = chosen[q * P + p0 + (e - q * w)];
-        if (b >= 0) atomicAdd(&counts[b], 1);
+        if (b >= 0 && b < nu) atomicAdd(&counts[b], 1);
     }
 }
 
@@ -861,7 +1007,7 @@ __global__ void pair_count_kernel(const int *__restrict__ chosen, int Q, int P, 
 // each block's group starting at a multiple of 32) and record the heap row
 // it will produce: the pair's slot in the list, plus row_base, is its
 // output row.
-__global__ void pair_scatter_kernel(const int *__restrict__ chosen, int Q, int P, int p0, int p1,
+__global__ void pair_scatter_kernel(const int *__restrict__ chosen, int Q, int P, int p0, int p1, int nu,
                                     int *__restrict__ cursor, int *__restrict__ qlist,
                                     int *__restrict__ rows, int row_base) {
     const int w = p1 - p0;
@@ -870,7 +1016,7 @@ __global__ void pair_scatter_kernel(const int *__restrict__ chosen, int Q, int P
         const long long q = e / w;
         const long long pi = q * P + p0 + (e - q * w);
         const int b = chosen[pi];
-        if (b < 0) {
+        if (b < 0 || b >= nu) {  // (no unit: units outside [0, nu) are never read)
             rows[pi] = -1;
             continue;
         }
@@ -2061,16 +2207,16 @@ hipError_t idx_router_first(const float *score, int Q, int nb, const int *counts
                        nb, counts, lb);
     return hipGetLastError();
 }
-hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int *counts,
+hipError_t idx_pair_count(const int *chosen, int Q, int P, int p0, int p1, int nu, int *counts,
                           hipStream_t s) {
     hipLaunchKernelGGL(pair_count_kernel, dim3(grid_for((long long)Q * (p1 - p0))), dim3(256), 0, s,
-                       chosen, Q, P, p0, p1, counts);
+                       chosen, Q, P, p0, p1, nu, counts);
     return hipGetLastError();
 }
-hipError_t idx_pair_scatter(const int *chosen, int Q, int P, int p0, int p1, int *cursor,
+hipError_t idx_pair_scatter(const int *chosen, int Q, int P, int p0, int p1, int nu, int *cursor,
                             int *qlist, int *rows, int row_base, hipStream_t s) {
     hipLaunchKernelGGL(pair_scatter_kernel, dim3(grid_for((long long)Q * (p1 - p0))), dim3(256), 0,
-                       s, chosen, Q, P, p0, p1, cursor, qlist, rows, row_base);
+                       s, chosen, Q, P, p0, p1, nu, cursor, qlist, rows, row_base);
     return hipGetLastError();
 }
 hipError_t idx_gather_columns(const float *src, long long ld, int dim, const int *list, int n,
@@ -2176,6 +2322,11 @@ hipError_t idx_ball_select(const float *qpm, int pm_ld, int dim, int Q, int kp, 
                            int Pb, int ncand, float *lb_rest, hipStream_t s) {
     const int ncap = (ncand + 7) / 8 * 8;
     if (ncap < 8 || ncap > kBallCand) return hipErrorInvalidValue;
+    if (ncand <= kSelLaneCand * kWave && P <= kWave) {
+        hipLaunchKernelGGL(ball_select_lanes_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves),
+                           0, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt, fb_start, out, Pb, lb_rest);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(ball_select_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves),
                        (size_t)kBallWaves * ncap * kSelBytes, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt,
                        fb_start, out, Pb, lb_rest, ncap);
