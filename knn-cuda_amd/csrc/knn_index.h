@@ -79,9 +79,14 @@ int idx_ball_cand_max();
 // holding kp members (fcnt: members per ball), nearest first, the others'
 // bounds folded into lb_rest.  ncand: most candidate balls of a query (P
 // times the most balls of a block), at most idx_ball_cand_max().
+// theta (nullable, only with at most 128 candidates): the scan thresholds
+// (order-mapped keys ||r||^2 - 2 r.q with error coefficients eq, er; rmax:
+// the largest ||r||^2) are seeded from T_q.
 hipError_t idx_ball_select(const float *qpm, int pm_ld, int dim, int Q, int kp, const int *chosen, int P,
                            const float *fcent, const double *frad, const int *fcnt, const int *fb_start, int *out,
-                           int Pb, int ncand, float *lb_rest, hipStream_t s);
+                           int Pb, int ncand, float *lb_rest, hipStream_t s, const float *qnorm = nullptr,
+                           const unsigned *rmax = nullptr, double eq = 0.0, double er = 0.0,
+                           unsigned *theta = nullptr);
 // Radii of balls with known position ranges (largest member distance, FP64;
 // -1 for an empty ball).
 hipError_t idx_ball_range_radius(const float *pm, int ld, int dim, const float *fcent, const int *ball_pos,

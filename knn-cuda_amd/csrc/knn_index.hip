@@ -851,6 +851,14 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_select_kernel(
 // LDS round trips sit on the critical path.  Same candidates, bounds, T_q
 // and ranks as ball_select_kernel (the FP64 sums differ only in their
 // summation order, far inside the 1e-12 allowance).
+// theta (nullable): each query's scan threshold is seeded from T_q.  The
+// kp-th nearest distance is at most T_q, and the scan's key of a reference
+// within T_q (||r||^2 - 2 r.q with error at most eq ||q||^2 + er max||r||^2)
+// is below T_q^2 - ||q||^2 (1 - g) + eq ||q||^2 + er max||r||^2, widened and
+// rounded up: the scan then starts from that threshold instead of +inf.
+// Any starting threshold keeps the certificate sound (the final threshold
+// is no larger than any the scan rejected against); a tight one saves the
+// candidates of young lists.
 constexpr int kSelLaneCand = 2;  // candidates per lane
 __device__ __forceinline__ double readlane_f64(double x, int l) {
     const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
@@ -860,7 +868,9 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {
 __global__ __launch_bounds__(64 * kBallWaves) void ball_select_lanes_kernel(
     const float *__restrict__ qpm, int pm_ld, int dim, int Q, int kp, const int *__restrict__ chosen, int P,
     const float *__restrict__ fcent, const double *__restrict__ frad, const int *__restrict__ fcnt,
-    const int *__restrict__ fb_start, int *__restrict__ out, int Pb, float *__restrict__ lb_rest) {
+    const int *__restrict__ fb_start, int *__restrict__ out, int Pb, float *__restrict__ lb_rest,
+    const float *__restrict__ qnorm, const unsigned *__restrict__ rmax, double eq, double er,
+    unsigned *__restrict__ theta) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * kBallWaves + wave);
     if (q >= Q) return;  // (no workgroup barrier in this kernel)
@@ -957,6 +967,14 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_select_lanes_kernel(
     for (int k = 0; k < kSelLaneCand; ++k)
         if (cf[k] >= 0 && mem[k] >= kp) T = fmin(T, cu[k]);
     for (int o = 32; o > 0; o >>= 1) T = fmin(T, __shfl_xor(T, o));
+    if (theta && lane == 0 && T < INFINITY) {
+        const double nq = (double)qnorm[q], nr = (double)__uint_as_float(*rmax);
+        const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
+        const double t = T * T * (1.0 + 1e-6) - nq * (1.0 - g - 1e-6) + (eq * nq + er * nr) * (1.0 + 1e-6) + 1e-30;
+        float tf = (float)t;
+        if ((double)tf <= t) tf = nextafterf(tf, INFINITY);
+        theta[q] = float_to_ordered(tf);
+    }
     // passing candidates (bound below T_q), ranked by (bound, ball)
     bool pass[kSelLaneCand];
     int npass = 0;
@@ -2319,12 +2337,14 @@ hipError_t idx_ball_first(const float *qpm, int pm_ld, int dim, int Q, const int
 int idx_ball_cand_max() { return kBallCand; }
 hipError_t idx_ball_select(const float *qpm, int pm_ld, int dim, int Q, int kp, const int *chosen, int P,
                            const float *fcent, const double *frad, const int *fcnt, const int *fb_start, int *out,
-                           int Pb, int ncand, float *lb_rest, hipStream_t s) {
+                           int Pb, int ncand, float *lb_rest, hipStream_t s, const float *qnorm,
+                           const unsigned *rmax, double eq, double er, unsigned *theta) {
     const int ncap = (ncand + 7) / 8 * 8;
     if (ncap < 8 || ncap > kBallCand) return hipErrorInvalidValue;
     if (ncand <= kSelLaneCand * kWave && P <= kWave) {
         hipLaunchKernelGGL(ball_select_lanes_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves),
-                           0, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt, fb_start, out, Pb, lb_rest);
+                           0, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt, fb_start, out, Pb, lb_rest,
+                           qnorm, rmax, eq, er, theta);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(ball_select_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves),

@@ -831,18 +831,29 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
                             w.chosen.as<int>(), w.lb_rest.as<float>(), s),
             "pick blocks");
-    if (one_pass)
+    KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Q, s), "memset");
+    if (one_pass) {
+        // keys ||r||^2 - 2 r.q (not residual keys): thresholds seeded from each query's reach
+        double eq = 0.0, er = 0.0;
+        if (nch_bf) {
+            bf3_error_bound(dim, nch_bf, &eq, &er);
+        } else {
+            eq = 2.0 * (dim + 4) * 0x1p-24;
+            er = 3.1 * eq;
+        }
         KNN_TRY(idx_ball_select(w.qpm.as<float>(), pm_ld, dim, Q, kp, w.chosen.as<int>(), P, ix->fcent.as<float>(),
                                 ix->frad.as<double>(), ix->ball_cnt.as<int>(), ix->fb_start.as<int>(),
-                                w.bchosen.as<int>(), Pb, P * ix->max_bpb, w.lb_rest.as<float>(), s),
+                                w.bchosen.as<int>(), Pb, P * ix->max_bpb, w.lb_rest.as<float>(), s,
+                                w.qnorm.as<float>(), ix->rmax.as<unsigned>(), eq, er,
+                                nch_res ? nullptr : w.theta.as<unsigned>()),
                 "balls");
+    }
     else if (balls)
         KNN_TRY(idx_ball_first(w.qpm.as<float>(), pm_ld, dim, Q, w.chosen.as<int>(), P, ix->fcent.as<float>(),
                                ix->frad.as<double>(), ix->fb_start.as<int>(), w.bchosen.as<int>(), Pb, s),
                 "first balls");
     if (timing) KNN_TRY(hipEventRecord(ev[1], s), "event");
     if (timing) KNN_TRY(hipEventRecord(ev[2], s), "event");
-    KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Q, s), "memset");
     KNN_TRY(hipMemsetAsync(w.fb_count.p, 0, 4, s), "memset");
     // the scan units each query visits: Pb per query
     int *units = balls ? w.bchosen.as<int>() : w.chosen.as<int>();
