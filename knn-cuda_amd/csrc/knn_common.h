@@ -204,6 +204,20 @@ __device__ __forceinline__ float ordered_to_float(unsigned u) {
     return __uint_as_float(b);
 }
 
+// Raw buffer resource over [base, base + nbytes) from values that are
+// uniform over the wave.  Buffer loads take their resource in scalar
+// registers; values the compiler holds in vector registers (after a branch
+// merge, or read from LDS) would make it wrap every load in a loop over the
+// distinct resources of the lanes, so the base and size are read from the
+// first lane here.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_buffer_rsrc(const void *base, unsigned nbytes) {
+    const unsigned long long p = (unsigned long long)base;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)p);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(p >> 32));
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo), (short)0,
+                                             (int)__builtin_amdgcn_readfirstlane(nbytes), 0x00020000);
+}
+
 // Compiler-level ordering point for LDS traffic shared between lanes of one
 // wave (no hardware barrier: waves of a block never wait for each other).
 __device__ __forceinline__ void wave_sync() {

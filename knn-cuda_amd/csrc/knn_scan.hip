@@ -235,9 +235,9 @@ __device__ __forceinline__ void scan_f32_block(const ScanArgs &a, const int t, u
     const long long tile_bytes = (long long)a.oct * kWave * 16;
     const int span_refs = (r_end - r_begin + kRefsPerStep - 1) / kRefsPerStep * kRefsPerStep;
     const long long valid_tiles = max(0, min(span_refs, a.r_pad - r_begin)) / kRefsPerTile;
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float *>(a.packed) + (long long)(r_begin / kRefsPerTile) * a.oct * kWave * 4,
-        (short)0, (int)(unsigned)(valid_tiles * tile_bytes), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs =
+        uniform_buffer_rsrc(a.packed + (long long)(r_begin / kRefsPerTile) * a.oct * kWave * 4,
+                            (unsigned)(valid_tiles * tile_bytes));
     auto lane_offsets = [&](int r_start, unsigned *vo) {
         const long long t0 = (r_start - r_begin) / kRefsPerTile;
 #pragma unroll
@@ -464,10 +464,12 @@ __global__ __launch_bounds__(kMultiWaves * 64, 2) void scan_f32_multi_kernel(con
         __syncthreads();  // the previous item's LDS (heaps, queues, item) is free
         if (threadIdx.x == 0) item = (int)atomicAdd(next, 1u);
         __syncthreads();
-        const int it = item;
+        // (uniform, read as such: the item's arguments load as scalars and its
+        // tiles' buffer resource sits in scalar registers, as buffer loads need)
+        const int it = __builtin_amdgcn_readfirstlane(item);
         if (it >= nwork) break;  // uniform over the block
         const int2 w = work[it];
-        const ScanArgs a = cls[w.x];
+        const ScanArgs &a = cls[w.x];  // (read in place: uniform loads, no private copy)
         switch (variant[w.x]) {
             case 0: scan_f32_block<8, 2, kBRegs, 8>(a, w.y, smem); break;
             case 1: scan_f32_block<16, 4, kBRegs, 8>(a, w.y, smem); break;

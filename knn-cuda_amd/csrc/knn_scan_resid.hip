@@ -133,7 +133,9 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
     const int wave_id = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 31, h = lane >> 5;
     const int waves = IND ? 1 : nwaves, wave = IND ? 0 : wave_id;
-    const int tix = IND ? blockIdx.x * nwaves + wave_id : blockIdx.x;
+    // (wave-uniform, and read as such: the task's fields then load as scalars and
+    // the tiles' buffer resource sits in scalar registers, as buffer loads need)
+    const int tix = __builtin_amdgcn_readfirstlane(IND ? blockIdx.x * nwaves + wave_id : blockIdx.x);
     if (IND && tix >= a.ntasks) return;  // (no workgroup barrier in this form)
     const int kp = a.kp, kps = heap_stride(kp);
     const size_t wave_lds = MERGE ? merge_lds_bytes(kp) : scan_heap_bytes(kp) + kQueueBytes;
@@ -241,9 +243,9 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
     constexpr long long kTileBytes = (long long)NCH * kWave * 16;
     const int span_refs = (r_end - r_begin + kRefsPerStep - 1) / kRefsPerStep * kRefsPerStep;
     const long long valid_tiles = max(0, min(span_refs, a.r_pad - r_begin)) / kRefsPerTile;
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<unsigned short *>(a.packed_res) + (long long)(r_begin / kRefsPerTile) * (kTileBytes / 2),
-        (short)0, (int)(unsigned)(valid_tiles * kTileBytes), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs =
+        uniform_buffer_rsrc(a.packed_res + (long long)(r_begin / kRefsPerTile) * (kTileBytes / 2),
+                            (unsigned)(valid_tiles * kTileBytes));
     auto lane_offsets = [&](int r_start, unsigned *vo) {
         const long long t0 = (r_start - r_begin) / kRefsPerTile;
 #pragma unroll
