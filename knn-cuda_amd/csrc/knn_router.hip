@@ -416,6 +416,16 @@ __global__ __launch_bounds__(64 * kRtWaves) void router_first_fused_kernel(Route
     for (int bt = wave; bt < ntb; bt += kRtWaves) {
         const int row = min(bt * 32 + col, a.nb - 1);
         const unsigned short *wr = a.w3 + (long long)row * H + 8 * h;
+        // the rows' biases and point counts, loaded with the tile's weights
+        // (read after the products: their latency overlaps the MFMAs)
+        float b3r[16];
+        bool emp[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int b = min(bt * 32 + 8 * (i >> 2) + 4 * h + (i & 3), a.nb - 1);
+            b3r[i] = a.b3[b];
+            emp[i] = a.counts && a.counts[b] == 0;
+        }
         f32x16_t acc0 = {}, acc1 = {};
 #pragma unroll
         for (int ks = 0; ks < KH; ++ks) {
@@ -429,8 +439,8 @@ __global__ __launch_bounds__(64 * kRtWaves) void router_first_fused_kernel(Route
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int b = bt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-            if (b >= a.nb || (a.counts && a.counts[b] == 0)) continue;
-            const float b3 = a.b3[b];
+            if (b >= a.nb || emp[i]) continue;
+            const float b3 = b3r[i];
             float s0 = acc0[i] + b3, s1 = acc1[i] + b3;
             s0 = s0 == s0 ? s0 : -INFINITY;
             s1 = s1 == s1 ? s1 : -INFINITY;
