@@ -149,65 +149,59 @@ __host__ __device__ constexpr size_t merge_lds_bytes(int kp) {
 // lists + q * kp, nfill real entries, the rest (+inf, -1)).  cnt, nfill and
 // thr are per lane: lane (col, h) holds its own candidate count and its
 // column's fill and kp-th key (+inf while the list is short).  Wave-uniform
-// control flow; the whole wave ranks one query's union at a time, element e
-// of the union (list entries by position, then half 0's candidates, then
-// half 1's, each in append order) on lane e % 64 of pass e / 64.  Its rank
-// under (key, union position) counts the elements before it, every element's
-// key read across the wave with readlane (no LDS round trip per element);
-// the ranks are a permutation, the kp smallest land in order and the list
-// stays sorted.
+// control flow; the whole wave takes one query at a time: lane e holds list
+// entry e (e < n) and lane j candidate j (half 0's, then half 1's, each in
+// append order; at most 2 kMergeCand of them).  The rank of an element is
+// its position in the union ordered by (key, union position), union
+// position meaning the list's entries first, then the candidates:
+//   list entry e      e + #{candidates t : key_t < key_e}   (the list is
+//                     sorted, equal keys in union order, from the last merge)
+//   candidate j       #{entries e : key_e <= key_j} (binary search of the
+//                     sorted list) + #{candidates t : key_t < key_j, or
+//                     key_t == key_j and t < j}
+// so the loop runs over the candidates only, reading each one's key across
+// the wave with readlane.  The ranks are a permutation of the union; the kp
+// smallest land in order and the list stays sorted.
 __device__ __forceinline__ float lane_key(float v, int j) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 __device__ __forceinline__ void merge_lists(KeyRef *lists, const KeyRef *cand, int kp, int nq, int lane,
                                             int col, int &cnt, int &nfill, float &thr) {
+    static_assert(2 * kMergeCand <= kWave && kMergeMaxKp <= kWave, "one list entry and one candidate a lane");
     wave_sync();
     for (int q = 0; q < nq; ++q) {
         const int c0 = __builtin_amdgcn_readlane(cnt, q);
         const int c1 = __builtin_amdgcn_readlane(cnt, q + kWave / 2);
-        if (c0 + c1 == 0) continue;
+        const int c = c0 + c1;
+        if (c == 0) continue;
         const int n = __builtin_amdgcn_readlane(nfill, q);
-        const int m = n + c0 + c1;
-        const bool two = m > kWave;
         KeyRef *list = lists + (size_t)q * kp;
-        KeyRef x[2];
+        KeyRef le, cj;
+        le.k = INFINITY;
+        le.r = -1;
+        cj.k = INFINITY;
+        cj.r = -1;
+        if (lane < n) le = list[lane];
+        if (lane < c0)
+            cj = cand[lane * kMergeSlots + q];
+        else if (lane < c)
+            cj = cand[(lane - c0) * kMergeSlots + kMergeQueries + q];
+        // entries at or below the candidate: binary search of the sorted list
+        int below = 0;
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const int e = p * kWave + lane, ce = e - n;
-            x[p].k = INFINITY;
-            x[p].r = -1;
-            if (p == 1 && !two) continue;
-            if (e < n)
-                x[p] = list[e];
-            else if (ce < c0)
-                x[p] = cand[ce * kMergeSlots + q];
-            else if (e < m)
-                x[p] = cand[(ce - c0) * kMergeSlots + kMergeQueries + q];
-        }
-        const float k0 = x[0].k, k1 = x[1].k;
-        int r0 = 0, r1 = 0;
-        if (!two) {
-            for (int j = 0; j < m; ++j) {
-                const float y = lane_key(k0, j);
-                r0 += (y < k0 || (y == k0 && j < lane)) ? 1 : 0;
-            }
-        } else {
-            // pass 1's elements follow every pass 0 element in union order
-            for (int j = 0; j < kWave; ++j) {
-                const float y = lane_key(k0, j);
-                r0 += (y < k0 || (y == k0 && j < lane)) ? 1 : 0;
-                r1 += y <= k1 ? 1 : 0;
-            }
-            for (int j = 0; j < m - kWave; ++j) {
-                const float y = lane_key(k1, j);
-                r0 += y < k0 ? 1 : 0;
-                r1 += (y < k1 || (y == k1 && j < lane)) ? 1 : 0;
-            }
+        for (int step = kWave; step > 0; step >>= 1)
+            if (below + step <= n && list[below + step - 1].k <= cj.k) below += step;
+        int rl = lane, rc = below;
+        for (int t = 0; t < c; ++t) {
+            const float y = lane_key(cj.k, t);
+            rl += y < le.k ? 1 : 0;
+            rc += (y < cj.k || (y == cj.k && t < lane)) ? 1 : 0;
         }
         wave_sync();  // every read of the list precedes the writes
-        if (lane < m && r0 < kp) list[r0] = x[0];
-        if (two && kWave + lane < m && r1 < kp) list[r1] = x[1];
+        if (lane < n && rl < kp) list[rl] = le;
+        if (lane < c && rc < kp) list[rc] = cj;
         wave_sync();
+        const int m = n + c;
         const int nn = m < kp ? m : kp;
         if (col == q) {
             nfill = nn;
