@@ -34,6 +34,15 @@ __device__ __forceinline__ int wave_sum(int v) {
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
+// The same sum for lane values in [0, 2^B): one ballot per bit and scalar
+// bit counts, no cross-lane shuffles (a bisection step's latency).
+template <int B>
+__device__ __forceinline__ int wave_sum_bits(int v) {
+    int s = 0;
+#pragma unroll
+    for (int b = 0; b < B; ++b) s += __popcll(__ballot((v >> b) & 1)) << b;
+    return s;
+}
 
 template <int NPL>
 __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
@@ -97,7 +106,8 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
             int c = 0;
 #pragma unroll
             for (int j = 0; j < NPL; ++j) c += code[j] <= mid;
-            if (wave_sum(c) >= kp)
+            // (a lane's count is at most NPL <= 32: six bits)
+            if (wave_sum_bits<NPL < 32 ? 5 : 6>(c) >= kp)
                 hi = mid;
             else
                 lo = mid + 1;
@@ -120,7 +130,32 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
 
     // 4. Ground-truth FP32 distance of every candidate, then rank by the
     //    reference order (sqrtf key, index) and write the k best.
-    if (!overflow) {
+    if (!overflow && nc <= kWave) {
+        // one candidate a lane: its distance in registers, its rank by reading
+        // the others' (key, index) across the wave
+        float kc = INFINITY;
+        int ic = INT_MAX;
+        if (lane < nc) {
+            const int r = cidx[lane];
+            kc = sqrt_rn(canonical_sq_rows(a.ref_pm + (long long)r * a.pm_ld, a.query_pm + (long long)q * a.pm_ld,
+                                           a.dim));
+            ic = a.perm ? a.perm[r] : r;
+        }
+        int rank = 0;
+        for (int j = 0; j < nc; ++j) {
+            const float y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kc), j));
+            const int yi = __builtin_amdgcn_readlane(ic, j);
+            rank += lex_less(y, yi, kc, ic);
+        }
+        if (lane < nc) {
+            if (rank < a.k) {
+                a.out_dist[(long long)rank * a.Q + q] = kc;
+                a.out_idx[(long long)rank * a.Q + q] = ic;
+            }
+            if (rank == a.k - 1) *kth_slot = kc;
+        }
+        wave_sync();
+    } else if (!overflow) {
         for (int c = lane; c < nc; c += kWave) {
             const int r = cidx[c];
             ckey[c] = sqrt_rn(canonical_sq_rows(a.ref_pm + (long long)r * a.pm_ld,
