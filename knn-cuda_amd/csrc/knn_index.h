@@ -38,9 +38,11 @@ hipError_t idx_gather_columns(const float *src, long long ld, int dim, const int
                               float *dst, hipStream_t s);
 hipError_t idx_scatter_results(const float *dsub, const int *isub, int k, int n, const int *list,
                                int Q, float *dist, int *idx, hipStream_t s);
-// qpt: queries a task (the list starts are multiples of it)
-hipError_t idx_make_tasks(const int *qstart, const int *counts, int nb, const int *block_start,
-                          const int *block_count, ScanTask *tasks, int max_tasks, int qpt, hipStream_t s);
+// Per-unit list starts (qstart, nu + 1 entries; multiples of qpt, queries
+// a task), the scatter cursors (= qstart) and the scan tasks (slots past the
+// last task written empty) from the units' pair counts.
+hipError_t idx_unit_tasks(const int *counts, int nu, int qpt, const int *unit_start, const int *unit_count,
+                          int *qstart, int *cursor, ScanTask *tasks, int max_tasks, hipStream_t s);
 hipError_t idx_ref_block(const int *block_start, const int *block_count, int nb, const int *perm,
                          int *out, hipStream_t s);
 hipError_t idx_row_norms(const float *pm, int ld, int n, int dim, float *out, hipStream_t s);

@@ -178,6 +178,8 @@ __global__ void gather_pack_kernel(const float *__restrict__ pm_src, int pm_ld, 
 // block strictly after the previous pick and wave 0 keeps the best offer.
 // Blocks without members are never taken (they hold no references).
 constexpr int kPickWaves = 16;
+// Block field of the pick keys (blocks below 2^30).
+constexpr int kPickKeyB = 0x3FFFFFFF;
 // At most this many queries, one workgroup per query instead.
 constexpr int kPickSingleMaxQ = 1024;
 
@@ -278,11 +280,15 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
     // larger key ranks first and equal scores go by the smaller block; 0
     // marks an unfilled slot.  SC (scores given): kl, their bounds; else a
     // bound is the negated score.
-    unsigned long long key[PMAX];
+    unsigned long long key[SC ? PMAX : 1];
+    double dkey[SC ? 1 : PMAX];
+    double rkey = 0.0;  // (no score form) the largest key that left the list
     float kl[SC ? PMAX : 1];
     float rest = INFINITY;
 #pragma unroll
-    for (int j = 0; j < PMAX; ++j) key[j] = 0;
+    for (int j = 0; j < (SC ? PMAX : 1); ++j) key[j] = 0;
+#pragma unroll
+    for (int j = 0; j < (SC ? 1 : PMAX); ++j) dkey[j] = j < P ? 0.0 : (double)INFINITY;  // (+inf: a slot past P, passed by)
 #pragma unroll
     for (int j = 0; j < (SC ? PMAX : 1); ++j) kl[j] = INFINITY;
     // bounds are read kPickBatch blocks at a time (independent loads, the
@@ -320,41 +326,78 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
 #pragma unroll
         for (int j = 0; j < kPickBatch; ++j) {
             const int b = bb + j;
-            if (b >= b1 || cc_[j] == 0) continue;  // uniform: the block is the same for every lane
-            // one compare-swap pass over the P kept slots: the new block takes
-            // the first unfilled slot or the first slot of a smaller key and
-            // pushes the rest down; whatever leaves the last slot, the new
-            // block itself or a kept one, is not chosen and bounds the rest
-            const float sc = cs_[j] == cs_[j] ? cs_[j] + 0.0f : -INFINITY;
-            unsigned long long c = ((unsigned long long)float_to_ordered(sc) << 32) | (unsigned)(~b);
-            float cl = SC ? cl_[SC ? j : 0] : 0.0f;
+            // (uniform: the block is the same for every lane)
+            const bool live = b < b1 && cc_[j] != 0;
+            if constexpr (!SC) {
+                // the key as a positive double: bits 30..61 the order-mapped
+                // score, bits 0..29 kPickKeyB - b; every such key is a normal
+                // double (the order-mapped -inf is 0x007FFFFF, so the exponent
+                // field is at least 1) and 0 marks an unfilled slot, so double
+                // max/min order them as the integers: the new block takes its
+                // place by a max/min pass and whatever leaves the last slot
+                // (the smallest of P + 1) is folded into the rest key.  A block
+                // past the share or without members enters as key 0, which
+                // passes every slot and leaves the rest key as it is (no
+                // branch, so the max/min results stay known canonical)
+                const float sc = cs_[j] == cs_[j] ? cs_[j] + 0.0f : -INFINITY;
+                const unsigned long long ck = ((unsigned long long)float_to_ordered(sc) << 30) | (unsigned)(kPickKeyB - b);
+                double c = __longlong_as_double((long long)(live ? ck : 0ull));
 #pragma unroll
-            for (int t = 0; t < PMAX; ++t) {
-                if (t < P) {
-                    const bool sw = c > key[t];
-                    const unsigned long long tk = key[t];
-                    key[t] = sw ? c : tk;
-                    c = sw ? tk : c;
-                    if constexpr (SC) {
-                        const float tl = kl[t];
-                        kl[t] = sw ? cl : tl;
-                        cl = sw ? tl : cl;
+                for (int t = 0; t < PMAX; ++t) {
+                    const double kt = dkey[t];
+                    dkey[t] = __builtin_fmax(kt, c);
+                    c = __builtin_fmin(kt, c);
+                }
+                rkey = __builtin_fmax(rkey, c);
+            } else {
+                if (!live) continue;
+                // one compare-swap pass over the P kept slots: the new block takes
+                // the first unfilled slot or the first slot of a smaller key and
+                // pushes the rest down; whatever leaves the last slot, the new
+                // block itself or a kept one, is not chosen and bounds the rest
+                const float sc = cs_[j] == cs_[j] ? cs_[j] + 0.0f : -INFINITY;
+                unsigned long long c = ((unsigned long long)float_to_ordered(sc) << 32) | (unsigned)(~b);
+                float cl = SC ? cl_[SC ? j : 0] : 0.0f;
+#pragma unroll
+                for (int t = 0; t < PMAX; ++t) {
+                    if (t < P) {
+                        const bool sw = c > key[t];
+                        const unsigned long long tk = key[t];
+                        key[t] = sw ? c : tk;
+                        c = sw ? tk : c;
+                        if constexpr (SC) {
+                            const float tl = kl[t];
+                            kl[t] = sw ? cl : tl;
+                            cl = sw ? tl : cl;
+                        }
                     }
                 }
+                // (an unfilled slot leaves key 0: nothing)
+                const float out_lb = SC ? cl : c ? -ordered_to_float((unsigned)(c >> 32)) : INFINITY;
+                rest = fminf(rest, out_lb);
             }
-            // (an unfilled slot leaves key 0: nothing)
-            const float out_lb = SC ? cl : c ? -ordered_to_float((unsigned)(c >> 32)) : INFINITY;
-            rest = fminf(rest, out_lb);
         }
+    }
+    if constexpr (!SC) {
+        const unsigned long long rk = (unsigned long long)__double_as_longlong(rkey);
+        if (rk) rest = -ordered_to_float((unsigned)(rk >> 30));
     }
     float ls[PMAX], ll[PMAX];
     int lbk[PMAX];
     int n = 0;
 #pragma unroll
     for (int j = 0; j < PMAX; ++j) {
-        const bool real = key[j] != 0;
-        ls[j] = real ? ordered_to_float((unsigned)(key[j] >> 32)) : -INFINITY;
-        lbk[j] = real ? (int)~(unsigned)key[j] : 0x7FFFFFFF;
+        bool real;
+        if constexpr (SC) {
+            real = key[SC ? j : 0] != 0;
+            ls[j] = real ? ordered_to_float((unsigned)(key[SC ? j : 0] >> 32)) : -INFINITY;
+            lbk[j] = real ? (int)~(unsigned)key[SC ? j : 0] : 0x7FFFFFFF;
+        } else {
+            const unsigned long long dk = (unsigned long long)__double_as_longlong(dkey[SC ? 0 : j]);
+            real = j < P && dk != 0;
+            ls[j] = real ? ordered_to_float((unsigned)(dk >> 30)) : -INFINITY;
+            lbk[j] = real ? kPickKeyB - (int)(dk & kPickKeyB) : 0x7FFFFFFF;
+        }
         ll[j] = SC ? kl[SC ? j : 0] : real ? -ls[j] : INFINITY;
         n += real ? 1 : 0;
     }
@@ -1067,36 +1110,79 @@ __global__ void scatter_results_kernel(const float *__restrict__ dsub, const int
     }
 }
 
-// Step 3: tasks.  Unit b's pairs occupy qlist[qstart[b] .. qstart[b] +
-// count[b]) with qstart[b] a multiple of qpt (queries a task: 32, or 16 for
-// merged-list tasks); task t covers list slots [qpt t, qpt t + qpt) (slot ==
-// output row), so tasks are enumerated from the aligned slot space directly.
-// Units are blocks (tile-aligned starts) or balls (any start: the task's
-// tiles begin at the start rounded down to 32).
-__global__ void make_tasks_kernel(const int *__restrict__ qstart, const int *__restrict__ counts,
-                                  int nb, const int *__restrict__ block_start,
-                                  const int *__restrict__ block_count, ScanTask *__restrict__ tasks,
-                                  int max_tasks, int qpt) {
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
-        const int c = counts[b];
-        for (int j = 0; j < c; j += qpt) {
-            const int t = (qstart[b] + j) / qpt;
-            if (t >= max_tasks) continue;
-            ScanTask tk;
-            tk.ref_lo = block_start[b];
-            tk.ref_begin = block_start[b] / kRefsPerTile * kRefsPerTile;
-            tk.ref_end = block_start[b] + block_count[b];
-            tk.qoff = qstart[b] + j;
-            tk.qcount = min(qpt, c - j);
-            tk.unit = b;
-            tasks[t] = tk;
+// Steps 1b and 3 in one launch: the units' list starts, the cursors the
+// scatter advances, and the scan tasks.  Unit b's pairs take
+// ceil(count[b] / qpt) tasks (qpt queries a task: 32, or 16 for merged-list
+// tasks), so its list starts at qpt x the exclusive prefix of the task
+// counts (slot == output row) and the tasks of all units fill [0, T) with
+// no gaps; slots [T, max_tasks) are written empty (they scan nothing).
+// Units are blocks (tile-aligned starts) or balls (any start: a task's tiles
+// begin at the start rounded down to 32).  Every workgroup scans all nu
+// counts (a few KB, from L2) and writes the units and tail slots of its own
+// share.
+constexpr int kUnitTaskThreads = 1024, kUnitTaskIpt = 8, kUnitTaskWgs = 16;
+__global__ __launch_bounds__(kUnitTaskThreads) void unit_tasks_kernel(
+    const int *__restrict__ counts, int nu, int qpt, const int *__restrict__ unit_start,
+    const int *__restrict__ unit_count, int *__restrict__ qstart, int *__restrict__ cursor,
+    ScanTask *__restrict__ tasks, int max_tasks) {
+    constexpr int nw = kUnitTaskThreads / kWave;
+    __shared__ int wsum[nw];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int share = (nu + gridDim.x - 1) / gridDim.x;
+    const int own0 = min(nu, (int)blockIdx.x * share), own1 = min(nu, own0 + share);
+    int carry = 0;  // tasks of the units before the chunk
+    for (int c0 = 0; c0 < nu; c0 += kUnitTaskThreads * kUnitTaskIpt) {
+        const int i0 = c0 + tid * kUnitTaskIpt;
+        int cnt[kUnitTaskIpt], tot = 0;
+#pragma unroll
+        for (int i = 0; i < kUnitTaskIpt; ++i) {
+            cnt[i] = i0 + i < nu ? counts[i0 + i] : 0;
+            tot += (cnt[i] + qpt - 1) / qpt;
         }
+        // exclusive prefix of the threads' task counts: within the wave,
+        // then the earlier waves' totals
+        int v = tot;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int x = __shfl_up(v, off);
+            if (lane >= off) v += x;
+        }
+        if (lane == kWave - 1) wsum[wave] = v;
+        __syncthreads();
+        int before = 0, chunk = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int x = wsum[w];
+            before += w < wave ? x : 0;
+            chunk += x;
+        }
+        __syncthreads();  // wsum is written again by the next chunk
+        int t = carry + before + v - tot;  // the first task of the thread's units
+#pragma unroll
+        for (int i = 0; i < kUnitTaskIpt; ++i) {
+            const int b = i0 + i;
+            const int nt = (cnt[i] + qpt - 1) / qpt;
+            if (b >= own0 && b < own1) {
+                qstart[b] = t * qpt;
+                cursor[b] = t * qpt;
+                const int lo = unit_start[b], hi = lo + unit_count[b];
+                for (int j = 0; j < nt && t + j < max_tasks; ++j) {
+                    ScanTask tk;
+                    tk.ref_lo = lo;
+                    tk.ref_begin = lo / kRefsPerTile * kRefsPerTile;
+                    tk.ref_end = hi;
+                    tk.qoff = (t + j) * qpt;
+                    tk.qcount = min(qpt, cnt[i] - j * qpt);
+                    tk.unit = b;
+                    tasks[t + j] = tk;
+                }
+            }
+            t += nt;
+        }
+        carry += chunk;
     }
-}
-
-// Empty task slots (alignment gaps between blocks) scan nothing.
-__global__ void clear_tasks_kernel(ScanTask *__restrict__ tasks, int n) {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    if (blockIdx.x == 0 && tid == 0) qstart[nu] = carry * qpt;
+    for (int t = carry + (int)blockIdx.x * kUnitTaskThreads + tid; t < max_tasks;
+         t += (int)gridDim.x * kUnitTaskThreads) {
         ScanTask tk;
         tk.ref_begin = 0;
         tk.ref_end = 0;
@@ -2249,13 +2335,11 @@ hipError_t idx_scatter_results(const float *dsub, const int *isub, int k, int n,
                        dsub, isub, k, n, list, Q, dist, idx);
     return hipGetLastError();
 }
-hipError_t idx_make_tasks(const int *qstart, const int *counts, int nb, const int *block_start,
-                          const int *block_count, ScanTask *tasks, int max_tasks, int qpt, hipStream_t s) {
+hipError_t idx_unit_tasks(const int *counts, int nu, int qpt, const int *unit_start, const int *unit_count,
+                          int *qstart, int *cursor, ScanTask *tasks, int max_tasks, hipStream_t s) {
     if (qpt < 1 || qpt > kQueriesPerWave) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(clear_tasks_kernel, dim3(grid_for(max_tasks)), dim3(256), 0, s, tasks,
-                       max_tasks);
-    hipLaunchKernelGGL(make_tasks_kernel, dim3(grid_for(nb)), dim3(256), 0, s, qstart, counts, nb,
-                       block_start, block_count, tasks, max_tasks, qpt);
+    hipLaunchKernelGGL(unit_tasks_kernel, dim3(kUnitTaskWgs), dim3(kUnitTaskThreads), 0, s, counts, nu, qpt,
+                       unit_start, unit_count, qstart, cursor, tasks, max_tasks);
     return hipGetLastError();
 }
 hipError_t idx_ref_block(const int *block_start, const int *block_count, int nb, const int *perm,

@@ -878,16 +878,12 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         const int base = tbase * qpt;
         KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nu * 4, s), "memset");
         KNN_TRY(idx_pair_count(units, Q, Pb, pr[ph], pr[ph + 1], nu, w.pcount.as<int>(), s), "pairs");
-        KNN_TRY(idx_prefix(w.pcount.as<int>(), nu, qpt, w.qstart.as<int>(), nullptr, s),
-                "prefix");
-        KNN_TRY(hipMemcpyAsync(w.cursor.p, w.qstart.p, (size_t)nu * 4, hipMemcpyDeviceToDevice, s),
-                "copy");
+        KNN_TRY(idx_unit_tasks(w.pcount.as<int>(), nu, qpt, unit_start, unit_count, w.qstart.as<int>(),
+                               w.cursor.as<int>(), w.tasks.as<ScanTask>() + tbase, max_t[ph], s),
+                "tasks");
         KNN_TRY(idx_pair_scatter(units, Q, Pb, pr[ph], pr[ph + 1], nu, w.cursor.as<int>(), w.qlist.as<int>() + base,
                                  w.rows.as<int>(), base, s),
                 "pairs");
-        KNN_TRY(idx_make_tasks(w.qstart.as<int>(), w.pcount.as<int>(), nu, unit_start, unit_count,
-                               w.tasks.as<ScanTask>() + tbase, max_t[ph], qpt, s),
-                "tasks");
         ScanArgs sa{};
         sa.packed = ix->packed.as<float>();
         sa.oct = ix->oct;
