@@ -67,7 +67,8 @@ typedef struct knnhip_options {
     int margin;         /* extra candidates kept per query beyond k (default 16) */
     int slices;         /* reference slices scanned in parallel (default: chosen) */
     int force_fallback; /* 1: every query takes the exact full-scan path (testing) */
-    int timing;         /* 1: record per-stage device times (knnhip_last_stage_ms) */
+    int timing;         /* 1: record per-stage device times (knnhip_last_stage_ms); 2 (index
+                           searches): the scan stage's alone, the others read -1 */
     int precision;      /* 0: FP32 products; 1: bfloat16 three-term products (same exact
                            result; dim <= 192 and k + margin <= 107, else FP32); 2: index
                            searches take single-pass bfloat16 products on residuals against

@@ -389,7 +389,8 @@ bool run_device(const float *ref, long long ldr, int R, const float *query, long
     if (dfr) dfr->pending = false;
     const bool timing = o.timing != 0;
     if (timing && !g_ws.have_events) {
-        for (auto &e : g_ws.ev) KNN_TRY(hipEventCreate(&e), "event create");
+        // (stage timestamps only: no system-scope fence)
+        for (auto &e : g_ws.ev) KNN_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "event create");
         g_ws.have_events = true;
     }
     for (float &m : g_stage_ms) m = -1.f;

@@ -40,9 +40,11 @@ hipError_t idx_scatter_results(const float *dsub, const int *isub, int k, int n,
                                int Q, float *dist, int *idx, hipStream_t s);
 // Per-unit list starts (qstart, nu + 1 entries; multiples of qpt, queries
 // a task), the scatter cursors (= qstart) and the scan tasks (slots past the
-// last task written empty) from the units' pair counts.
+// last task written empty) from the units' pair counts.  clear (nullable):
+// a counter set to 0 in stream order (the failure count of the select that
+// follows), saving a memset node.
 hipError_t idx_unit_tasks(const int *counts, int nu, int qpt, const int *unit_start, const int *unit_count,
-                          int *qstart, int *cursor, ScanTask *tasks, int max_tasks, hipStream_t s);
+                          int *qstart, int *cursor, ScanTask *tasks, int max_tasks, int *clear, hipStream_t s);
 hipError_t idx_ref_block(const int *block_start, const int *block_count, int nb, const int *perm,
                          int *out, hipStream_t s);
 hipError_t idx_row_norms(const float *pm, int ld, int n, int dim, float *out, hipStream_t s);
@@ -81,9 +83,9 @@ int idx_ball_cand_max();
 // holding kp members (fcnt: members per ball), nearest first, the others'
 // bounds folded into lb_rest.  ncand: most candidate balls of a query (P
 // times the most balls of a block), at most idx_ball_cand_max().
-// theta (nullable, only with at most 128 candidates): the scan thresholds
-// (order-mapped keys ||r||^2 - 2 r.q with error coefficients eq, er; rmax:
-// the largest ||r||^2) are seeded from T_q.
+// theta (nullable): every query's scan threshold is written: seeded from
+// T_q with at most 128 candidates (order-mapped keys ||r||^2 - 2 r.q with
+// error coefficients eq, er; rmax: the largest ||r||^2), else +inf.
 hipError_t idx_ball_select(const float *qpm, int pm_ld, int dim, int Q, int kp, const int *chosen, int P,
                            const float *fcent, const double *frad, const int *fcnt, const int *fb_start, int *out,
                            int Pb, int ncand, float *lb_rest, hipStream_t s, const float *qnorm = nullptr,

@@ -1010,13 +1010,18 @@ __global__ __launch_bounds__(64 * kBallWaves) void ball_select_lanes_kernel(
     for (int k = 0; k < kSelLaneCand; ++k)
         if (cf[k] >= 0 && mem[k] >= kp) T = fmin(T, cu[k]);
     for (int o = 32; o > 0; o >>= 1) T = fmin(T, __shfl_xor(T, o));
-    if (theta && lane == 0 && T < INFINITY) {
-        const double nq = (double)qnorm[q], nr = (double)__uint_as_float(*rmax);
-        const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
-        const double t = T * T * (1.0 + 1e-6) - nq * (1.0 - g - 1e-6) + (eq * nq + er * nr) * (1.0 + 1e-6) + 1e-30;
-        float tf = (float)t;
-        if ((double)tf <= t) tf = nextafterf(tf, INFINITY);
-        theta[q] = float_to_ordered(tf);
+    if (theta && lane == 0) {  // (every query's threshold is written: +inf without a reach)
+        unsigned tv = 0xFF800000u;
+        if (T < INFINITY) {
+            const double nq = (double)qnorm[q], nr = (double)__uint_as_float(*rmax);
+            const double g = 2.0 * (double)(dim + 4) * 5.9604644775390625e-8;
+            const double t =
+                T * T * (1.0 + 1e-6) - nq * (1.0 - g - 1e-6) + (eq * nq + er * nr) * (1.0 + 1e-6) + 1e-30;
+            float tf = (float)t;
+            if ((double)tf <= t) tf = nextafterf(tf, INFINITY);
+            tv = float_to_ordered(tf);
+        }
+        theta[q] = tv;
     }
     // passing candidates (bound below T_q), ranked by (bound, ball)
     bool pass[kSelLaneCand];
@@ -1124,7 +1129,7 @@ constexpr int kUnitTaskThreads = 1024, kUnitTaskIpt = 8, kUnitTaskWgs = 16;
 __global__ __launch_bounds__(kUnitTaskThreads) void unit_tasks_kernel(
     const int *__restrict__ counts, int nu, int qpt, const int *__restrict__ unit_start,
     const int *__restrict__ unit_count, int *__restrict__ qstart, int *__restrict__ cursor,
-    ScanTask *__restrict__ tasks, int max_tasks) {
+    ScanTask *__restrict__ tasks, int max_tasks, int *__restrict__ clear) {
     constexpr int nw = kUnitTaskThreads / kWave;
     __shared__ int wsum[nw];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -1180,7 +1185,10 @@ __global__ __launch_bounds__(kUnitTaskThreads) void unit_tasks_kernel(
         }
         carry += chunk;
     }
-    if (blockIdx.x == 0 && tid == 0) qstart[nu] = carry * qpt;
+    if (blockIdx.x == 0 && tid == 0) {
+        qstart[nu] = carry * qpt;
+        if (clear) *clear = 0;
+    }
     for (int t = carry + (int)blockIdx.x * kUnitTaskThreads + tid; t < max_tasks;
          t += (int)gridDim.x * kUnitTaskThreads) {
         ScanTask tk;
@@ -2336,10 +2344,10 @@ hipError_t idx_scatter_results(const float *dsub, const int *isub, int k, int n,
     return hipGetLastError();
 }
 hipError_t idx_unit_tasks(const int *counts, int nu, int qpt, const int *unit_start, const int *unit_count,
-                          int *qstart, int *cursor, ScanTask *tasks, int max_tasks, hipStream_t s) {
+                          int *qstart, int *cursor, ScanTask *tasks, int max_tasks, int *clear, hipStream_t s) {
     if (qpt < 1 || qpt > kQueriesPerWave) return hipErrorInvalidValue;
     hipLaunchKernelGGL(unit_tasks_kernel, dim3(kUnitTaskWgs), dim3(kUnitTaskThreads), 0, s, counts, nu, qpt,
-                       unit_start, unit_count, qstart, cursor, tasks, max_tasks);
+                       unit_start, unit_count, qstart, cursor, tasks, max_tasks, clear);
     return hipGetLastError();
 }
 hipError_t idx_ref_block(const int *block_start, const int *block_count, int nb, const int *perm,
@@ -2430,6 +2438,10 @@ hipError_t idx_ball_select(const float *qpm, int pm_ld, int dim, int Q, int kp, 
                            0, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt, fb_start, out, Pb, lb_rest,
                            qnorm, rmax, eq, er, theta);
         return hipGetLastError();
+    }
+    if (theta) {  // (this form seeds nothing: every threshold starts at +inf)
+        const hipError_t e = hipMemsetD32Async((hipDeviceptr_t)theta, 0xFF800000u, Q, s);
+        if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(ball_select_kernel, dim3((Q + kBallWaves - 1) / kBallWaves), dim3(64 * kBallWaves),
                        (size_t)kBallWaves * ncap * kSelBytes, s, qpm, pm_ld, dim, Q, kp, chosen, P, fcent, frad, fcnt,

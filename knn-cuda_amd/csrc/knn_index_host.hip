@@ -782,15 +782,18 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         set_error("Memory allocation error");
         return false;
     }
+    // stage timing: device-side timestamps only (no system-scope fence, so
+    // recording one does not stall the stream)
+    // (timing 2: the scan stage's two events only)
     hipEvent_t ev[6], evr[2], evk[2][2];
-    const bool timing = o.timing != 0;
+    const bool timing = o.timing == 1, scan_timing = o.timing != 0;
     if (timing)
-        for (auto &e : evr) KNN_TRY(hipEventCreate(&e), "event");
-    if (timing)
+        for (auto &e : evr) KNN_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "event");
+    if (scan_timing)
         for (auto &p : evk)
-            for (auto &e : p) KNN_TRY(hipEventCreate(&e), "event");
+            for (auto &e : p) KNN_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "event");
     if (timing)
-        for (auto &e : ev) KNN_TRY(hipEventCreate(&e), "event");
+        for (auto &e : ev) KNN_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "event");
     if (timing) KNN_TRY(hipEventRecord(ev[0], s), "event");
     // queries: norms + point-major copy
     NormArgs nq{};
@@ -831,7 +834,9 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
     KNN_TRY(idx_pick_blocks(score, Q, nb, P, w.lb.as<float>(), ix->block_count.as<int>(),
                             w.chosen.as<int>(), w.lb_rest.as<float>(), s),
             "pick blocks");
-    KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Q, s), "memset");
+    // every scan threshold starts at +inf, or (one-pass ball probes with
+    // key-form scans) is seeded by the ball selection, which writes them all
+    if (!one_pass || nch_res) KNN_TRY(hipMemsetD32Async((hipDeviceptr_t)w.theta.p, 0xFF800000u, Q, s), "memset");
     if (one_pass) {
         // keys ||r||^2 - 2 r.q (not residual keys): thresholds seeded from each query's reach
         double eq = 0.0, er = 0.0;
@@ -854,7 +859,6 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
                 "first balls");
     if (timing) KNN_TRY(hipEventRecord(ev[1], s), "event");
     if (timing) KNN_TRY(hipEventRecord(ev[2], s), "event");
-    KNN_TRY(hipMemsetAsync(w.fb_count.p, 0, 4, s), "memset");
     // the scan units each query visits: Pb per query
     int *units = balls ? w.bchosen.as<int>() : w.chosen.as<int>();
     int tbase = 0;
@@ -879,7 +883,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         KNN_TRY(hipMemsetAsync(w.pcount.p, 0, (size_t)nu * 4, s), "memset");
         KNN_TRY(idx_pair_count(units, Q, Pb, pr[ph], pr[ph + 1], nu, w.pcount.as<int>(), s), "pairs");
         KNN_TRY(idx_unit_tasks(w.pcount.as<int>(), nu, qpt, unit_start, unit_count, w.qstart.as<int>(),
-                               w.cursor.as<int>(), w.tasks.as<ScanTask>() + tbase, max_t[ph], s),
+                               w.cursor.as<int>(), w.tasks.as<ScanTask>() + tbase, max_t[ph], w.fb_count.as<int>(), s),
                 "tasks");
         KNN_TRY(idx_pair_scatter(units, Q, Pb, pr[ph], pr[ph + 1], nu, w.cursor.as<int>(), w.qlist.as<int>() + base,
                                  w.rows.as<int>(), base, s),
@@ -910,7 +914,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         sa.query_pm = w.qpm.as<float>();
         sa.query_pm_ld = pm_ld;
         const int waves = (double)Q * (pr[ph + 1] - pr[ph]) / nu >= kOneWaveQueries || balls ? 1 : 4;
-        if (timing) KNN_TRY(hipEventRecord(evk[ph][0], s), "event");
+        if (scan_timing) KNN_TRY(hipEventRecord(evk[ph][0], s), "event");
         if (nch_res) {
             sa.packed_res = ix->res_tiles.as<unsigned short>();
             sa.res_norm = ix->res_norm.as<float>();
@@ -933,7 +937,7 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         } else {
             KNN_TRY(launch_scan(sa, ks, bmode, waves, s), "scan kernel launch");
         }
-        if (timing) KNN_TRY(hipEventRecord(evk[ph][1], s), "event");
+        if (scan_timing) KNN_TRY(hipEventRecord(evk[ph][1], s), "event");
         tbase += max_t[ph];
     }
     if (timing) KNN_TRY(hipEventRecord(ev[3], s), "event");
@@ -1042,6 +1046,21 @@ bool index_search(knnhip_index *ix, IndexWork &w, int depth, const float *query,
         g_stage_ms[1] = ix->last_ms[2];
         g_stage_ms[2] = ix->last_ms[3];
         g_stage_ms[3] = ix->last_ms[4];
+    } else if (scan_timing) {
+        // (the stream was synchronised above, after the select)
+        float scan_k = 0.f;
+        for (int ph = 0; ph < nphase; ++ph) {
+            float t = 0.f;
+            KNN_TRY(hipEventElapsedTime(&t, evk[ph][0], evk[ph][1]), "event");
+            scan_k += t;
+        }
+        for (auto &p : evk)
+            for (auto &e : p) (void)hipEventDestroy(e);
+        ix->last_fail = nfail;
+        for (float &m : ix->last_ms) m = -1.f;
+        for (float &m : ix->rank_ms) m = -1.f;
+        ix->last_ms[2] = scan_k;
+        g_stage_ms[1] = scan_k;
     }
     // what knnhip_index_last_work reads back from this search's task list
     if (depth == 0) {

@@ -557,13 +557,15 @@ class BlockIndex:
         return out
 
     def search(self, query, k: int, probes: int = 4, scores=None, dist=None, index=None, *,
-               margin: int = 0, force_fallback: bool = False, timing: bool = False,
+               margin: int = 0, force_fallback: bool = False, timing=False,
                precision: str = "fp32", score_all: bool = False, router=None,
                units: str = "balls", stream: Optional[int] = None):
         """Exact k-NN through the index.  ``scores`` (nblocks x query_nb) or ``router`` (a Router
         of this index's shape, its first probe taken from the fused inference kernel) choose each
         query's first probe; the lower bounds rank the rest.  ``units``: what the scan visits in
-        the probed blocks (``UNITS``; indexes that do not know their balls scan whole blocks)."""
+        the probed blocks (``UNITS``; indexes that do not know their balls scan whole blocks).
+        ``timing``: True records every stage's device time (``stage_ms``); "scan" the scan
+        stage's alone (two events, the other stages read -1)."""
         import torch
 
         lib = load()
@@ -591,7 +593,7 @@ class BlockIndex:
         opt = Options()
         opt.margin = margin
         opt.force_fallback = 1 if force_fallback else 0
-        opt.timing = 1 if timing else 0
+        opt.timing = 2 if timing == "scan" else 1 if timing else 0
         opt.precision = PRECISIONS[precision]
         opt.score_all = 1 if score_all else 0
         if units not in UNITS:

@@ -155,3 +155,25 @@ def test_ball_probes_at_scale_equal_plain():
     gd, gi = oracle.knn_c(ref.cpu().numpy(), np.ascontiguousarray(qry.cpu().numpy()[:, sample]), k)
     check(d[:, sample], i[:, sample], gd, gi, "oracle sample")
     ix.close()
+
+
+def test_scan_only_timing_reads_the_scan_stage_alone():
+    # timing="scan": the scan stage's time (positive), every other stage -1; the output is the
+    # same as with full timing or none
+    R, Q, D, k = 1 << 16, 512, 96, 16
+    ref = knnhip.generate_mixture(R, D, 128, 0.1, 81, 82)
+    qry = knnhip.generate_mixture(Q, D, 128, 0.1, 81, 83)
+    ix = knnhip.BlockIndex(ref, nblocks=64, iters=8)
+    d0, i0 = ix.search(qry, k, probes=5, precision="bf16x3")
+    for mode in ("scan", True):
+        d, i = ix.search(qry, k, probes=5, precision="bf16x3", timing=mode)
+        torch.cuda.synchronize()
+        st = ix.stage_ms()
+        assert st["scan"] > 0, st
+        others = [v for s, v in st.items() if s != "scan"]
+        if mode == "scan":
+            assert all(v == -1 for v in others), st
+        else:
+            assert all(v >= 0 for v in others), st
+        assert torch.equal(i, i0) and torch.equal(d, d0), mode
+    ix.close()
