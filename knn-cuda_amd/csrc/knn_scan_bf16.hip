@@ -489,33 +489,34 @@ __global__ __launch_bounds__(256, 2) void scan_bf3_task_kernel(ScanArgs a) {
 
     // B operands: bfloat16 hi / lo of -2q, lane (col, h) holding dimensions
     // 16c + 8h .. +7 of chunk c
+    // (every chunk's pieces are loaded before any is used: one round trip)
     bf16x8_t bh[NCH], bl[NCH];
+    float x[NCH][8];
+    if (a.query_pm) {
+        const float *row = a.query_pm + (long long)q * a.query_pm_ld;
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        float x[8];
-        if (a.query_pm) {
-            const float *row = a.query_pm + (long long)q * a.query_pm_ld;
+        for (int c = 0; c < NCH; ++c)
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-                const int d4 = 16 * c + 8 * h + 4 * p;
-                const float4 v = d4 < a.query_pm_ld ? *reinterpret_cast<const float4 *>(row + d4)
-                                                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                x[4 * p + 0] = -2.0f * v.x;
-                x[4 * p + 1] = -2.0f * v.y;
-                x[4 * p + 2] = -2.0f * v.z;
-                x[4 * p + 3] = -2.0f * v.w;
+                const float4 v = row_piece(row, 16 * c + 8 * h + 4 * p, a.query_pm_ld);
+                x[c][4 * p + 0] = -2.0f * v.x;
+                x[c][4 * p + 1] = -2.0f * v.y;
+                x[c][4 * p + 2] = -2.0f * v.z;
+                x[c][4 * p + 3] = -2.0f * v.w;
             }
-        } else {
-            const float *qcol = a.query + q;
+    } else {
+        const float *qcol = a.query + q;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int d = 16 * c + 8 * h + e;
                 const float v = qcol[(long long)min(d, a.dim - 1) * a.ldq];
-                x[e] = d < a.dim ? -2.0f * v : 0.0f;
+                x[c][e] = d < a.dim ? -2.0f * v : 0.0f;
             }
-        }
-        bh[c] = split_hi_lo(x, bl[c]);
     }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) bh[c] = split_hi_lo(x[c], bl[c]);
 
     // A operands through one buffer resource over the task's tiles: loads
     // past them (the prefetch beyond a wave's last step) return zeros.

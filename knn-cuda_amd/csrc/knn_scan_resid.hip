@@ -204,11 +204,7 @@ __global__ __launch_bounds__(256, 2) void scan_res_task_kernel(ScanArgs a) {
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             const int d4 = 16 * c + 8 * h + 4 * p;
-            float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f), m = x;
-            if (d4 < a.query_pm_ld) {
-                x = *reinterpret_cast<const float4 *>(qrow + d4);
-                m = *reinterpret_cast<const float4 *>(cen + d4);
-            }
+            const float4 x = row_piece(qrow, d4, a.query_pm_ld), m = row_piece(cen, d4, a.query_pm_ld);
             const float y[4] = {x.x - m.x, x.y - m.y, x.z - m.z, x.w - m.w};
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
