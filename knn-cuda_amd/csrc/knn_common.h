@@ -194,7 +194,6 @@ __host__ __device__ inline long long lb_at(int b, int q, int nb) {
 }
 __host__ __device__ inline long long lb_elems(int Q, int nb) { return (long long)((Q + 63) / 64) * 64 * nb; }
 
-// Order-preserving map of a float onto uint32 (total order, -0 < +0).
 // Four floats of a point-major row at d4 (a multiple of 4), zeros at and
 // past ld (the row stride: a multiple of 4, at least 4).  The load itself is
 // unconditional (clamped to the row's last piece), so a run of these issues
@@ -203,6 +202,7 @@ __device__ __forceinline__ float4 row_piece(const float *row, int d4, int ld) {
     const float4 v = *reinterpret_cast<const float4 *>(row + min(d4, ld - 4));
     return d4 < ld ? v : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
+// Order-preserving map of a float onto uint32 (total order, -0 < +0).
 __device__ __forceinline__ unsigned float_to_ordered(float x) {
     unsigned b = __float_as_uint(x);
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);

@@ -1637,25 +1637,35 @@ __global__ void ball_refit_finish_kernel(const unsigned long long *__restrict__ 
 // (non-negative floats order like their bit patterns).
 // max(0, |q - c| - rho) rounded down, in FP32 arithmetic.  nq, nc, dot
 // carry the FP32 norm and MFMA errors, within 1.5 g S for
-// S = (sqrt(nq) + sqrt(nc))^2 (g15 >= 1.5 g).  S also bounds nq + nc, 2|dot|
-// and every partial result of d2 = nq + nc - 2 dot - A, so each of its three
-// roundings is within 1.01 u S (u = 2^-24); the allowance
-// A = S_f (g15 + 2^-21)(1 + 2^-20), with S_f = (sq + sc)^2 from square roots
-// rounded up, covers both.  The hardware square root (within 1 ulp) is
-// lowered by 2^-21 relative, the radius comes rounded up (ru < 0: empty
-// ball), and the difference is lowered by 2^-22 relative.
-__device__ __forceinline__ float ball_lower_bound_core(float nq, float sq, float nc, float sc, float dot, float ru,
-                                                       float g15) {
-    const float st = sq + sc;
-    const float a = st * st * (1.0f + 0x1p-20f) * (g15 + 0x1p-21f);
-    const float d2 = fmaxf(fmaf(-2.0f, dot, nq + nc) - a, 0.0f);
+// S = (sqrt(nq) + sqrt(nc))^2 (g15 >= 1.5 g; the errors themselves are
+// within g S, which leaves 0.5 g S spare).  The allowance for them is
+// c S_f with c = (g15 + 2^-21)(1 + 2^-20) and S_f = (sq + sc)^2 from square
+// roots rounded up; S_f <= 2 sq^2 + 2 sc^2, so c (2 sq^2 + 2 sc^2) covers it,
+// split between the two ends: each end's share is taken off its own squared
+// norm once (ball_norm_share: pq = nq - 2 c sq^2 for the query, pc for the
+// ball, rounded down), and a pair's d2 = pq + pc - 2 dot costs two roundings,
+// each within 1.01 u S (u = 2^-24; S bounds nq + nc, 2|dot| and the partial
+// results), covered by the 2^-21 S in the allowance and the spare.  The
+// hardware square root (within 1 ulp) is lowered by 2^-21 relative, the
+// radius comes rounded up (ru < 0: empty ball), and the difference is
+// lowered by 2^-22 relative.
+__device__ __forceinline__ float ball_norm_share(float n, float s, float g15) {
+    const double c = ((double)g15 + 0x1p-21) * (1.0 + 0x1p-20);
+    const double v = (double)n - 2.0 * c * (double)s * (double)s;
+    const double w = v - fabs(v) * 0x1p-50;  // (below the exact value despite the FP64 roundings)
+    float f = (float)w;
+    if ((double)f > w) f = nextafterf(f, -INFINITY);
+    return f;
+}
+__device__ __forceinline__ float ball_lower_bound_core(float pq, float pc, float dot, float ru) {
+    const float d2 = fmaxf(fmaf(-2.0f, dot, pq + pc), 0.0f);
     const float v = __builtin_amdgcn_sqrtf(d2) * (1.0f - 0x1p-21f) - ru;
     return fmaxf(v, 0.0f) * (1.0f - 0x1p-22f);  // (ru = -inf: +inf)
 }
-__device__ __forceinline__ float ball_lower_bound(float nq, float sq, float nc, float sc, float dot,
-                                                  float ru, float g15) {
+// (the per-ball forms: the ball's share computed on the spot)
+__device__ __forceinline__ float ball_lower_bound(float pq, float nc, float sc, float dot, float ru, float g15) {
     if (ru < 0.0f) return INFINITY;
-    return ball_lower_bound_core(nq, sq, nc, sc, dot, ru, g15);
+    return ball_lower_bound_core(pq, ball_norm_share(nc, sc, g15), dot, ru);
 }
 
 // Per ball: the square root of the centre's squared norm and the radius,
@@ -1761,12 +1771,12 @@ __global__ __launch_bounds__(256) void ball_lb_kernel(const float *__restrict__ 
         // acc[i] is ball f0 + 8(i/4) + 4h + i%4 for query q0 + col
         const float g15 = ball_error_coeff(dim);
         const float nq = qnorm[qa];
-        const float sq = nextafterf(sqrtf(nq), INFINITY);
+        const float pq = ball_norm_share(nq, nextafterf(sqrtf(nq), INFINITY), g15);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int r = ((i >> 2) << 3) + (h << 2) + (i & 3);
             const int f = f0 + r;
-            patch[wave][r][col] = f < nf ? ball_lower_bound(nq, sq, fnorm[f], fsq[f], acc[i], fradf[f], g15)
+            patch[wave][r][col] = f < nf ? ball_lower_bound(pq, fnorm[f], fsq[f], acc[i], fradf[f], g15)
                                          : INFINITY;
         }
     }
@@ -1893,7 +1903,7 @@ __device__ __forceinline__ void ball_tile_bounds(const float *tnrm, const float 
                                                  const unsigned short *__restrict__ cbf,
                                                  const float4 (&bq)[NCH ? 1 : KHM / 4],
                                                  const bf16x8_t (&bh)[NCH ? NCH : 1],
-                                                 const bf16x8_t (&bl)[NCH ? NCH : 1], float nq, float sq, float g15,
+                                                 const bf16x8_t (&bl)[NCH ? NCH : 1], float pq, float g15,
                                                  float (&out)[16]) {
     const int KH = (dim + 1) / 2;
     const int fa = min(f0 + col, nf - 1);
@@ -1935,10 +1945,10 @@ __device__ __forceinline__ void ball_tile_bounds(const float *tnrm, const float 
         const float4 n4 = *reinterpret_cast<const float4 *>(tnrm + j);
         const float4 s4 = *reinterpret_cast<const float4 *>(tsq + j);
         const float4 r4 = *reinterpret_cast<const float4 *>(trad + j);
-        out[4 * g4 + 0] = ball_lower_bound(nq, sq, n4.x, s4.x, acc[4 * g4 + 0], r4.x, g15);
-        out[4 * g4 + 1] = ball_lower_bound(nq, sq, n4.y, s4.y, acc[4 * g4 + 1], r4.y, g15);
-        out[4 * g4 + 2] = ball_lower_bound(nq, sq, n4.z, s4.z, acc[4 * g4 + 2], r4.z, g15);
-        out[4 * g4 + 3] = ball_lower_bound(nq, sq, n4.w, s4.w, acc[4 * g4 + 3], r4.w, g15);
+        out[4 * g4 + 0] = ball_lower_bound(pq, n4.x, s4.x, acc[4 * g4 + 0], r4.x, g15);
+        out[4 * g4 + 1] = ball_lower_bound(pq, n4.y, s4.y, acc[4 * g4 + 1], r4.y, g15);
+        out[4 * g4 + 2] = ball_lower_bound(pq, n4.z, s4.z, acc[4 * g4 + 2], r4.z, g15);
+        out[4 * g4 + 3] = ball_lower_bound(pq, n4.w, s4.w, acc[4 * g4 + 3], r4.w, g15);
     }
 }
 
@@ -1979,7 +1989,7 @@ __global__ __launch_bounds__(256, 4) void ball_lb_reg_kernel(const float *__rest
     ball_query_ops<KHM, NCH>(qpm, pm_ld, qa, dim, h, bq, bh, bl);
     const float g15 = NCH > 0 ? g15_bf : ball_error_coeff(dim);
     const float nq = qnorm[qa];
-    const float sq = nextafterf(sqrtf(nq), INFINITY);
+    const float pq = ball_norm_share(nq, nextafterf(sqrtf(nq), INFINITY), g15);
     const int q = q0 + col;
     // every lane runs the fold, so the open block (cur, its ball range) stays
     // wave-uniform in scalar registers and the row loop branches on scalars;
@@ -2003,7 +2013,7 @@ __global__ __launch_bounds__(256, 4) void ball_lb_reg_kernel(const float *__rest
         if (f0 >= F1) break;  // uniform over the wave
         float out[16];
         ball_tile_bounds<KHM, NCH>(st.nrm + (f0 - F0), st.sq + (f0 - F0), st.rad + (f0 - F0), f0, nf, pm_ld, dim,
-                                   col, h, fcent, cbf, bq, bh, bl, nq, sq, g15, out);
+                                   col, h, fcent, cbf, bq, bh, bl, pq, g15, out);
         // the partner half's rows (a lane swap across the halves), and
         // (lane r < 32) row r's block and that block's first ball and one
         // past its last
@@ -2055,7 +2065,7 @@ __global__ __launch_bounds__(256, 4) void ball_lb_reg_kernel(const float *__rest
 constexpr int kSlotTilesPerWg = 16;  // slot tiles per workgroup for many queries
 constexpr int kSlotWaves = 8;        // waves (of 32 queries) per workgroup
 struct SlotStage {
-    float nrm[kSlotTilesPerWg * 32], sq[kSlotTilesPerWg * 32], rad[kSlotTilesPerWg * 32];
+    float pc[kSlotTilesPerWg * 32], rad[kSlotTilesPerWg * 32];  // (the slots' norm shares, radii)
     int blk[kSlotTilesPerWg * 8];
     int mode[kSlotTilesPerWg];
 };
@@ -2121,9 +2131,9 @@ __global__ __launch_bounds__(64 * kSlotWaves) void ball_lb_slot_kernel(const flo
     const int ngrp = (ntile + tpw - 1) / tpw;
     const int qg = blockIdx.x / ngrp, grp = blockIdx.x - qg * ngrp;
     const int T0 = grp * tpw, nt = min(ntile, T0 + tpw) - T0;
+    const float g15 = NCH > 0 ? g15_bf : ball_error_coeff(dim);
     for (int j = threadIdx.x; j < nt * 32; j += blockDim.x) {
-        st.nrm[j] = snrm[T0 * 32 + j];
-        st.sq[j] = ssq[T0 * 32 + j];
+        st.pc[j] = ball_norm_share(snrm[T0 * 32 + j], ssq[T0 * 32 + j], g15);
         st.rad[j] = srad[T0 * 32 + j];
     }
     for (int j = threadIdx.x; j < nt * 8; j += blockDim.x) st.blk[j] = sblk[T0 * 8 + j];
@@ -2135,9 +2145,8 @@ __global__ __launch_bounds__(64 * kSlotWaves) void ball_lb_slot_kernel(const flo
     float4 bq[NCH ? 1 : KHM / 4];
     bf16x8_t bh[NCH ? NCH : 1], bl[NCH ? NCH : 1];
     ball_query_ops<KHM, NCH>(qpm, pm_ld, qa, dim, h, bq, bh, bl);
-    const float g15 = NCH > 0 ? g15_bf : ball_error_coeff(dim);
     const float nq = qnorm[qa];
-    const float sq = nextafterf(sqrtf(nq), INFINITY);
+    const float pq = ball_norm_share(nq, nextafterf(sqrtf(nq), INFINITY), g15);
     const bool real = q < Q;
     constexpr int NA = NCH ? 2 * NCH : KHM / 4;
     const int KH = (dim + 1) / 2;
@@ -2217,13 +2226,12 @@ __global__ __launch_bounds__(64 * kSlotWaves) void ball_lb_slot_kernel(const flo
 #pragma unroll
         for (int g4 = 0; g4 < 4; ++g4) {
             const int j = 32 * tt + 8 * g4 + 4 * h;
-            const float4 n4 = *reinterpret_cast<const float4 *>(st.nrm + j);
-            const float4 s4 = *reinterpret_cast<const float4 *>(st.sq + j);
+            const float4 c4 = *reinterpret_cast<const float4 *>(st.pc + j);
             const float4 r4 = *reinterpret_cast<const float4 *>(st.rad + j);
-            out[4 * g4 + 0] = ball_lower_bound_core(nq, sq, n4.x, s4.x, acc[4 * g4 + 0], r4.x, g15);
-            out[4 * g4 + 1] = ball_lower_bound_core(nq, sq, n4.y, s4.y, acc[4 * g4 + 1], r4.y, g15);
-            out[4 * g4 + 2] = ball_lower_bound_core(nq, sq, n4.z, s4.z, acc[4 * g4 + 2], r4.z, g15);
-            out[4 * g4 + 3] = ball_lower_bound_core(nq, sq, n4.w, s4.w, acc[4 * g4 + 3], r4.w, g15);
+            out[4 * g4 + 0] = ball_lower_bound_core(pq, c4.x, acc[4 * g4 + 0], r4.x);
+            out[4 * g4 + 1] = ball_lower_bound_core(pq, c4.y, acc[4 * g4 + 1], r4.y);
+            out[4 * g4 + 2] = ball_lower_bound_core(pq, c4.z, acc[4 * g4 + 2], r4.z);
+            out[4 * g4 + 3] = ball_lower_bound_core(pq, c4.w, acc[4 * g4 + 3], r4.w);
         }
         // group minima (one lane's four rows), the partner half's, and the
         // minima over 8, 16 and 32 slots
