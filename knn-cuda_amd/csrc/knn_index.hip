@@ -1657,10 +1657,17 @@ __device__ __forceinline__ float ball_norm_share(float n, float s, float g15) {
     if ((double)f > w) f = nextafterf(f, -INFINITY);
     return f;
 }
-__device__ __forceinline__ float ball_lower_bound_core(float pq, float pc, float dot, float ru) {
+// (the bound is final(raw): final is monotone, so the minimum over balls of
+// the bounds is final of the minimum of the raw values, bit for bit)
+__device__ __forceinline__ float ball_bound_raw(float pq, float pc, float dot, float ru) {
     const float d2 = fmaxf(fmaf(-2.0f, dot, pq + pc), 0.0f);
-    const float v = __builtin_amdgcn_sqrtf(d2) * (1.0f - 0x1p-21f) - ru;
+    return __builtin_amdgcn_sqrtf(d2) * (1.0f - 0x1p-21f) - ru;
+}
+__device__ __forceinline__ float ball_bound_final(float v) {
     return fmaxf(v, 0.0f) * (1.0f - 0x1p-22f);  // (ru = -inf: +inf)
+}
+__device__ __forceinline__ float ball_lower_bound_core(float pq, float pc, float dot, float ru) {
+    return ball_bound_final(ball_bound_raw(pq, pc, dot, ru));
 }
 // (the per-ball forms: the ball's share computed on the spot)
 __device__ __forceinline__ float ball_lower_bound(float pq, float nc, float sc, float dot, float ru, float g15) {
@@ -2220,18 +2227,18 @@ __global__ __launch_bounds__(64 * kSlotWaves) void ball_lb_slot_kernel(const flo
             }
             if (tt + 1 < nt) load_a((T0 + tt + 1) * 32);
         }
-        // the bounds of the tile's 32 slots (rows 8(i/4) + 4h + i%4); empty
-        // slots (radius -inf) come out +inf
+        // the raw bounds of the tile's 32 slots (rows 8(i/4) + 4h + i%4), made
+        // final after the fold; empty slots (radius -inf) come out +inf
         float out[16];
 #pragma unroll
         for (int g4 = 0; g4 < 4; ++g4) {
             const int j = 32 * tt + 8 * g4 + 4 * h;
             const float4 c4 = *reinterpret_cast<const float4 *>(st.pc + j);
             const float4 r4 = *reinterpret_cast<const float4 *>(st.rad + j);
-            out[4 * g4 + 0] = ball_lower_bound_core(pq, c4.x, acc[4 * g4 + 0], r4.x);
-            out[4 * g4 + 1] = ball_lower_bound_core(pq, c4.y, acc[4 * g4 + 1], r4.y);
-            out[4 * g4 + 2] = ball_lower_bound_core(pq, c4.z, acc[4 * g4 + 2], r4.z);
-            out[4 * g4 + 3] = ball_lower_bound_core(pq, c4.w, acc[4 * g4 + 3], r4.w);
+            out[4 * g4 + 0] = ball_bound_raw(pq, c4.x, acc[4 * g4 + 0], r4.x);
+            out[4 * g4 + 1] = ball_bound_raw(pq, c4.y, acc[4 * g4 + 1], r4.y);
+            out[4 * g4 + 2] = ball_bound_raw(pq, c4.z, acc[4 * g4 + 2], r4.z);
+            out[4 * g4 + 3] = ball_bound_raw(pq, c4.w, acc[4 * g4 + 3], r4.w);
         }
         // group minima (one lane's four rows), the partner half's, and the
         // minima over 8, 16 and 32 slots
@@ -2250,7 +2257,7 @@ __global__ __launch_bounds__(64 * kSlotWaves) void ball_lb_slot_kernel(const flo
             const int md = (mode >> (2 * g4)) & 3;
             const float v = md == 0 ? gm[g4] : md == 1 ? m8[g4] : md == 2 ? fminf(m8[g4], m8[g4 ^ 1]) : m32;
             const int b = st.blk[8 * tt + 2 * g4 + h];
-            if (live && b >= 0) lb[lb_at(b, q, nb)] = real ? v : INFINITY;
+            if (live && b >= 0) lb[lb_at(b, q, nb)] = real ? ball_bound_final(v) : INFINITY;
         }
     }
 }

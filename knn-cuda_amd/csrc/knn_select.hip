@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
     unsigned code[NPL];
     int ids[NPL];
     int nreal = 0;
+    unsigned cmin = 0xFFFFFFFFu, cmax = 0u;  // the lane's smallest and largest real codes
 #pragma unroll
     for (int j = 0; j < NPL; ++j) {
         const int e = j * kWave + lane;
@@ -89,6 +90,8 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
             if (id >= 0) {
                 ids[j] = id;
                 code[j] = float_to_ordered(a.part_val[off]);
+                cmin = min(cmin, code[j]);
+                cmax = max(cmax, code[j]);
                 ++nreal;
             }
         }
@@ -100,7 +103,16 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(FinalArgs a) {
     unsigned cut = 0xFFFFFFFEu;
     const bool all_in = N < kp;
     if (!all_in) {
-        unsigned lo = 0, hi = 0xFFFFFFFEu;
+        // the cut lies between the smallest and the largest real code (at
+        // least kp real codes lie at or below the largest), so the bisection
+        // starts from that range: about 20 steps instead of 32 for keys that
+        // share an exponent
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            cmin = min(cmin, (unsigned)__shfl_xor((int)cmin, off));
+            cmax = max(cmax, (unsigned)__shfl_xor((int)cmax, off));
+        }
+        unsigned lo = __builtin_amdgcn_readfirstlane(cmin), hi = __builtin_amdgcn_readfirstlane(cmax);
         while (lo < hi) {
             const unsigned mid = lo + ((hi - lo) >> 1);
             int c = 0;
