@@ -177,3 +177,20 @@ def test_scan_only_timing_reads_the_scan_stage_alone():
             assert all(v >= 0 for v in others), st
         assert torch.equal(i, i0) and torch.equal(d, d0), mode
     ix.close()
+
+
+@pytest.mark.parametrize("D", [1, 5, 13, 130])
+def test_ball_probes_odd_dims_equal_plain(D):
+    # the ball choice's FP64 centre distances and the bounds over odd dimension counts: ball
+    # searches == plain search
+    R, Q, k = 20000, 300, 8
+    ref = knnhip.generate_mixture(R, D, 40, 0.1, 91 + D, 92)
+    qry = knnhip.generate_mixture(Q, D, 40, 0.1, 91 + D, 93)
+    pd, pi = knnhip.search_device(ref, qry, k)
+    ix = knnhip.BlockIndex(ref, nblocks=32, iters=6)
+    for prec in ("fp32", "bf16x3"):
+        d, i = ix.search(qry, k, probes=4, precision=prec, units="balls")
+        torch.cuda.synchronize()
+        say(f"dim {D} {prec}: units {ix.last_units()} fallbacks {knnhip.last_fallback_count()}")
+        assert torch.equal(i, pi) and torch.equal(d, pd), (D, prec)
+    ix.close()
