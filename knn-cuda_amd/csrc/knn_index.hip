@@ -2314,13 +2314,13 @@ hipError_t idx_pick_blocks(const float *score, int Q, int nb, int P, const float
     } else if (P <= 8 && score)
         hipLaunchKernelGGL((pick_blocks_list_kernel<8, true>), dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
                            score, Q, nb, P, lb, counts, chosen, lb_rest);
-    else if (P <= 8)
+    else if (P <= 8 && nb <= kPickKeyB)  // (the no-score keys hold block numbers below 2^30)
         hipLaunchKernelGGL((pick_blocks_list_kernel<8, false>), dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0, s,
                            score, Q, nb, P, lb, counts, chosen, lb_rest);
     else if (P <= 16 && score)
         hipLaunchKernelGGL((pick_blocks_list_kernel<16, true>), dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0,
                            s, score, Q, nb, P, lb, counts, chosen, lb_rest);
-    else if (P <= 16)
+    else if (P <= 16 && nb <= kPickKeyB)
         hipLaunchKernelGGL((pick_blocks_list_kernel<16, false>), dim3((Q + 63) / 64), dim3(64 * kPickWaves), 0,
                            s, score, Q, nb, P, lb, counts, chosen, lb_rest);
     else
