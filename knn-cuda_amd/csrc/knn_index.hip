@@ -270,7 +270,8 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
     __shared__ float offer_s[kPickWaves][64];
     __shared__ int offer_b[kPickWaves][64];
     __shared__ int winner[64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // (the wave number read as uniform: the block loop's bounds and addresses stay scalar)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int q = blockIdx.x * 64 + lane;
     const bool valid = q < Q;
     const int per = (nb + kPickWaves - 1) / kPickWaves;
@@ -298,12 +299,14 @@ __global__ __launch_bounds__(64 * kPickWaves) void pick_blocks_list_kernel(
     // (the loop is uniform over the wave, so the blocks' point counts are
     // scalar loads; lanes past the last query read the last one's bounds)
     const int qa = valid ? q : Q - 1;
+    // (the query's column of its slab: block b's bound at lbq[64 b])
+    const float *lbq = lb + lb_at(0, qa, nb);
     float lv[kPickBatch], sv[SC ? kPickBatch : 1];
     auto load = [&](int bb) {
 #pragma unroll
         for (int j = 0; j < kPickBatch; ++j) {
             const int b = min(bb + j, b1 - 1);
-            lv[j] = lb[lb_at(b, qa, nb)];
+            lv[j] = lbq[64 * b];
             if constexpr (SC) sv[j] = score[(long long)b * Q + qa];
         }
     };
